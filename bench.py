@@ -1,0 +1,159 @@
+"""Benchmark: Mrays/s of the wavefront path tracer on the 1.51 M-triangle 1080p workload.
+
+Workload (config 1 of the synthetic generators): icosphere subdivided 8x
+(1,310,720 triangles, radius 3, value-noise displacement 0.1) + 200,000 random
+small triangles in [-5,5]^3, one quad light, 512x256 lognormal env map,
+1920x1080, 1 spp, 8 bounces, binned-SAH BVH4 built on the host (not timed).
+
+A step is one frame: raygen + 8 x (extend, shade, light shadow, env shadow) +
+accumulate, all stream-ordered on the library's HIP stream, inputs resident in
+HBM.  value = (primary + extension + shadow rays) / wall time of K frames.
+
+roofline: the closest-hit (extend) kernel, HBM-bound by design (no dense
+contraction).  Algorithmic bytes per launch = node visits x 128 B (BVH4 node)
++ triangle tests x 48 B (TriRec) + sphere tests x 20 B + rays x 40 B (32 B ray
+record in, 8 B hit out), counted by an instrumented re-render of the same
+frame; divided by the HIP-event duration of the same launches in the timed
+region.
+
+cpu_baseline: the CPU oracle (reference semantics, fp32) on a row subsample
+of the same frame, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "demont-engine_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows-step", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import dmt
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    t0 = time.time()
+    sd = dmt.generate(1, seed=args.seed)
+    log(f"[bench] scene: {sd.indices.shape[0]} triangles, {sd.frame.width}x{sd.frame.height}, "
+        f"{sd.frame.max_bounces} bounces ({time.time() - t0:.1f}s)")
+    dev = dmt.Device(0)
+    t0 = time.time()
+    scene = dmt.Scene(dev, sd)
+    log(f"[bench] BVH build + upload {time.time() - t0:.1f}s")
+    frame = sd.frame.copy()
+
+    def run(k0, n):
+        for k in range(n):
+            f = frame.copy()
+            f.frame_index = k0 + k
+            f.reset_accum = 1 if k0 + k == 0 else 0
+            scene.render(f)
+
+    run(0, args.warmup)
+    dev.synchronize()
+    torch.cuda.synchronize()
+    scene.reset_stats()
+    scene.set_profiling(True)
+    dev.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.warmup, args.steps)
+    dev.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = scene.stats()
+    scene.set_profiling(False)
+    rays = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+    mrays = rays / dt / 1e6
+    log(f"[bench] {args.steps} frames in {dt * 1e3:.1f} ms: {mrays:.1f} Mrays/s "
+        f"(primary {st['primary_rays']}, extension {st['extension_rays']}, shadow {st['shadow_rays']})")
+
+    # algorithmic traffic of the closest-hit kernel, from an instrumented re-render
+    f = frame.copy()
+    f.frame_index = args.warmup
+    tr = scene.measure_traffic(f)
+    ext_rays = tr["primary_rays"] + tr["extension_rays"]
+    ext_bytes = (tr["ext_node_visits"] * 128 + tr["ext_tri_tests"] * 48 + tr["ext_sphere_tests"] * 20
+                 + ext_rays * 40)
+    launches_per_frame = frame.spp * frame.max_bounces
+    bytes_per_launch = ext_bytes / launches_per_frame
+    avg_launch_ms = st["extend_ms"] / max(st["extend_launches"], 1)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    log(f"[bench] extend: {tr['ext_node_visits'] / max(ext_rays, 1):.1f} nodes/ray, "
+        f"{tr['ext_tri_tests'] / max(ext_rays, 1):.1f} tri tests/ray, {bytes_per_launch / 1e6:.1f} MB/launch, "
+        f"avg launch {avg_launch_ms:.3f} ms -> {achieved:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ctypes as orc
+        ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, ncpu - 4) if ncpu > 4 else ncpu   # SoftwareTracer.cpp:867-868 policy
+        oscene = orc.OracleScene.from_scene_data(sd)
+        step = args.cpu_rows_step
+        f = frame.copy()
+        f.frame_index = 0
+        f.reset_accum = 1
+        t0 = time.perf_counter()
+        _, cnt = oscene.render(f, 32, threads, rows=(0, f.height, step))
+        cdt = time.perf_counter() - t0
+        crays = cnt["primary"] + cnt["extension"] + cnt["shadow"]
+        cpu = {"value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"every {step}th row of one 1920x1080 frame ({crays} rays, {cdt:.1f} s), "
+                         f"fp32 CPU oracle with the reference semantics, {threads} threads of {ncpu}"}
+        log(f"[bench] cpu baseline {cpu}")
+
+    line = {
+        "metric": "Mrays/s @1080p 1M tris 1spp",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, config 1)",
+        "config": {"workload": "icosphere-8 displaced + 200k random tris (1,510,720 tris), 512x256 lognormal "
+                               "env, 1 quad light, 1920x1080, 1 spp, max depth 8, binned-SAH BVH4",
+                   "triangles": int(sd.indices.shape[0]), "width": frame.width, "height": frame.height,
+                   "spp": frame.spp, "max_bounces": frame.max_bounces, "rays_per_frame": rays // args.steps},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_extend (closest hit)", "bytes_per_launch": int(bytes_per_launch),
+                     "avg_launch_ms": round(avg_launch_ms, 5)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    scene.close()
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

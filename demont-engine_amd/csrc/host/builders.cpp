@@ -1,0 +1,418 @@
+// Host builders for the HIP path tracer.
+//
+// * build_bvh4: binned SAH (Wald 2007, "On fast construction of SAH-based
+//   bounding volume hierarchies") over item bounds, then a top-down collapse
+//   of the binary tree into 4-wide nodes (each node pulls up the children of
+//   its largest-area internal child until it has four).  This replaces the
+//   reference's centroid-median binary trees (src/renderer/TriangleBvh.cpp:33-169,
+//   src/renderer/AnalyticBvh.cpp:369-486); the traversal result is the same
+//   closest hit, the node layout is sized to a 128 B cache line.
+// * build_light_tree: the reference light tree (src/renderer/LightTree.cpp:63-455),
+//   same leaf summaries, cone union and median split, so the kernels draw
+//   the same pick probabilities.
+// * build_env_cdf: the reference env-map CDF (src/renderer/HdrImage.cpp:169-219).
+#include "prep.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <thread>
+
+namespace dmt {
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float* l, const float* h) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], l[a]);
+            hi[a] = std::max(hi[a], h[a]);
+        }
+    }
+    void grow_pt(const float* p) { grow(p, p); }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0.0f)) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct BinNode {
+    Box box;
+    int32_t left = -1, right = -1;
+    uint32_t first = 0, count = 0;   // leaf range in the index array
+    bool leaf() const { return left < 0; }
+};
+
+struct SahBuilder {
+    const std::vector<BoundsItem>& items;
+    std::vector<uint32_t> idx;
+    std::vector<BinNode> nodes;
+    uint32_t bins, max_leaf;
+
+    SahBuilder(const std::vector<BoundsItem>& it, uint32_t b, uint32_t ml) : items(it), bins(b), max_leaf(ml) {
+        idx.resize(items.size());
+        std::iota(idx.begin(), idx.end(), 0u);
+    }
+
+    int32_t build(uint32_t first, uint32_t count) {
+        int32_t me = (int32_t)nodes.size();
+        nodes.emplace_back();
+        Box box, cbox;
+        for (uint32_t i = first; i < first + count; ++i) {
+            const BoundsItem& it = items[idx[i]];
+            box.grow(it.lo, it.hi);
+            cbox.grow_pt(it.c);
+        }
+        nodes[me].box = box;
+        if (count <= 1) {
+            nodes[me].first = first;
+            nodes[me].count = count;
+            return me;
+        }
+        // Evaluate binned SAH on all three axes.
+        float best_cost = INFINITY;
+        int best_axis = -1;
+        uint32_t best_split = 0;
+        const uint32_t B = bins;
+        std::vector<Box> bb(B);
+        std::vector<uint32_t> bc(B);
+        std::vector<float> right_area(B);
+        std::vector<uint32_t> right_count(B);
+        for (int a = 0; a < 3; ++a) {
+            float ext = cbox.hi[a] - cbox.lo[a];
+            if (!(ext > 0.0f)) continue;
+            float scale = float(B) / ext;
+            std::fill(bb.begin(), bb.end(), Box{});
+            std::fill(bc.begin(), bc.end(), 0u);
+            for (uint32_t i = first; i < first + count; ++i) {
+                const BoundsItem& it = items[idx[i]];
+                uint32_t k = std::min(B - 1, (uint32_t)((it.c[a] - cbox.lo[a]) * scale));
+                bb[k].grow(it.lo, it.hi);
+                bc[k]++;
+            }
+            Box acc;
+            uint32_t n = 0;
+            for (uint32_t k = B - 1; k > 0; --k) {
+                acc.grow(bb[k].lo, bb[k].hi);
+                n += bc[k];
+                right_area[k] = acc.area();
+                right_count[k] = n;
+            }
+            Box lacc;
+            uint32_t ln = 0;
+            for (uint32_t k = 0; k + 1 < B; ++k) {
+                lacc.grow(bb[k].lo, bb[k].hi);
+                ln += bc[k];
+                if (ln == 0 || right_count[k + 1] == 0) continue;
+                float cost = lacc.area() * float(ln) + right_area[k + 1] * float(right_count[k + 1]);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = a;
+                    best_split = k + 1;
+                }
+            }
+        }
+        float parent_area = box.area();
+        float leaf_cost = float(count);
+        float split_cost = parent_area > 0.0f ? 1.0f + best_cost / parent_area : INFINITY;
+        if (count <= max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) {
+            nodes[me].first = first;
+            nodes[me].count = count;
+            return me;
+        }
+        uint32_t mid;
+        if (best_axis >= 0) {
+            float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
+            float scale = float(B) / ext;
+            float lo = cbox.lo[best_axis];
+            auto it = std::partition(idx.begin() + first, idx.begin() + first + count, [&](uint32_t i) {
+                uint32_t k = std::min(B - 1, (uint32_t)((items[i].c[best_axis] - lo) * scale));
+                return k < best_split;
+            });
+            mid = (uint32_t)(it - idx.begin());
+            if (mid == first || mid == first + count) mid = first + count / 2;
+        } else {
+            mid = first + count / 2;   // coincident centroids: split by position
+        }
+        int32_t l = build(first, mid - first);
+        int32_t r = build(mid, first + count - mid);
+        nodes[me].left = l;
+        nodes[me].right = r;
+        return me;
+    }
+};
+
+struct Collapser {
+    const std::vector<BinNode>& bn;
+    std::vector<Node4>& out;
+    uint32_t max_depth = 0;
+
+    uint32_t child_ref(int32_t c, uint32_t depth) {
+        const BinNode& n = bn[(size_t)c];
+        if (n.leaf()) return leaf_ref(n.first, n.count);
+        return collapse(c, depth + 1);
+    }
+
+    uint32_t collapse(int32_t root, uint32_t depth) {
+        max_depth = std::max(max_depth, depth);
+        int32_t kids[4];
+        int nk = 0;
+        const BinNode& r = bn[(size_t)root];
+        if (r.leaf()) {
+            kids[nk++] = root;
+        } else {
+            kids[nk++] = r.left;
+            kids[nk++] = r.right;
+            while (nk < 4) {
+                int best = -1;
+                float best_area = -1.0f;
+                for (int i = 0; i < nk; ++i) {
+                    const BinNode& k = bn[(size_t)kids[i]];
+                    if (k.leaf()) continue;
+                    float a = k.box.area();
+                    if (a > best_area) { best_area = a; best = i; }
+                }
+                if (best < 0) break;
+                int32_t e = kids[best];
+                kids[best] = bn[(size_t)e].left;
+                kids[nk++] = bn[(size_t)e].right;
+            }
+        }
+        uint32_t me = (uint32_t)out.size();
+        out.emplace_back();
+        Node4 node{};
+        for (int i = 0; i < 4; ++i) {
+            node.lo_x[i] = node.lo_y[i] = node.lo_z[i] = 0.0f;
+            node.hi_x[i] = node.hi_y[i] = node.hi_z[i] = 0.0f;
+            node.child[i] = kEmptyChild;
+            node.pad[i] = 0u;
+        }
+        for (int i = 0; i < nk; ++i) {
+            const Box& b = bn[(size_t)kids[i]].box;
+            node.lo_x[i] = b.lo[0]; node.hi_x[i] = b.hi[0];
+            node.lo_y[i] = b.lo[1]; node.hi_y[i] = b.hi[1];
+            node.lo_z[i] = b.lo[2]; node.hi_z[i] = b.hi[2];
+        }
+        for (int i = 0; i < nk; ++i) node.child[i] = child_ref(kids[i], depth);
+        out[me] = node;
+        return me;
+    }
+};
+
+}  // namespace
+
+Bvh4 build_bvh4(const std::vector<BoundsItem>& items, uint32_t bins, uint32_t max_leaf) {
+    Bvh4 out;
+    if (items.empty()) return out;
+    if (bins < 2) bins = 16;
+    if (max_leaf < 1) max_leaf = 4;
+    SahBuilder b(items, bins, max_leaf);
+    b.nodes.reserve(items.size() * 2);
+    b.build(0, (uint32_t)items.size());
+    Collapser c{b.nodes, out.nodes};
+    out.nodes.reserve(b.nodes.size() / 2 + 1);
+    c.collapse(0, 1);
+    out.depth = c.max_depth;
+    out.order = std::move(b.idx);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Light tree (LightTree.cpp).  Node floats: [0..2] aabb_min, [3] left_first
+// bits, [4..6] aabb_max, [7] count bits, [8..10] cone axis, [11] cos_half,
+// [12] intensity, [13..15] 0.
+
+namespace {
+
+struct LtLeaf {
+    float lo[3], hi[3], axis[3];
+    float cos_half, power;
+    uint32_t light;
+};
+
+// The engine rotates the spot axis / quad normal and the quad u vector by the
+// light's orientation before summarising (Engine.cpp:5479-5508).
+void orient_rotate(const float* q, const float* v, float* out) {
+    const float cx = q[1] * v[2] - q[2] * v[1] + q[3] * v[0];
+    const float cy = q[2] * v[0] - q[0] * v[2] + q[3] * v[1];
+    const float cz = q[0] * v[1] - q[1] * v[0] + q[3] * v[2];
+    out[0] = v[0] + 2.0f * (q[1] * cz - q[2] * cy);
+    out[1] = v[1] + 2.0f * (q[2] * cx - q[0] * cz);
+    out[2] = v[2] + 2.0f * (q[0] * cy - q[1] * cx);
+}
+
+LtLeaf lt_leaf(const float* Lp, uint32_t id) {
+    float L[20];
+    std::memcpy(L, Lp, sizeof(L));
+    orient_rotate(&Lp[16], &Lp[8], &L[8]);
+    if ((uint32_t)(Lp[7] + 0.5f) != 1u) orient_rotate(&Lp[16], &Lp[12], &L[12]);
+    LtLeaf s{};
+    s.light = id;
+    const uint32_t type = (uint32_t)(L[7] + 0.5f);
+    const bool point_like = type == 0u || type == 1u;
+    if (point_like) {
+        for (int a = 0; a < 3; ++a) { s.lo[a] = L[a] - 1e-3f; s.hi[a] = L[a] + 1e-3f; }
+    } else if (type == 2u) {
+        const float r = std::max(L[3], 1e-3f);
+        for (int a = 0; a < 3; ++a) { s.lo[a] = L[a] - r; s.hi[a] = L[a] + r; }
+    } else {
+        const float vh = std::fabs(L[15]);
+        for (int a = 0; a < 3; ++a) {
+            const float e = std::fabs(L[12 + a]) + vh;
+            s.lo[a] = L[a] - e;
+            s.hi[a] = L[a] + e;
+        }
+    }
+    if (type == 0u || type == 2u) {
+        s.axis[0] = 0.0f; s.axis[1] = 1.0f; s.axis[2] = 0.0f;
+        s.cos_half = -1.0f;
+    } else {
+        s.axis[0] = L[8]; s.axis[1] = L[9]; s.axis[2] = L[10];
+        s.cos_half = (type == 1u) ? L[11] : 0.0f;
+    }
+    float p = 0.2126f * L[4] + 0.7152f * L[5] + 0.0722f * L[6];
+    if (type == 2u) {
+        const float r = std::max(L[3], 1e-3f);
+        p *= 4.0f * 3.14159265358979f * r * r;
+    } else if (type == 3u) {
+        const float ul = std::sqrt(L[12] * L[12] + L[13] * L[13] + L[14] * L[14]);
+        p *= 4.0f * ul * std::fabs(L[15]);
+    }
+    s.power = p;
+    return s;
+}
+
+// Bounding cone of two cones (LightTree.cpp:171-273).
+void cone_union(const float* a, float ca, const float* b, float cb, float* axis, float* cos_half) {
+    auto copy_a = [&](float c) {
+        axis[0] = a[0]; axis[1] = a[1]; axis[2] = a[2];
+        *cos_half = c;
+    };
+    if (ca <= -0.999f || cb <= -0.999f) { copy_a(-1.0f); return; }
+    const float d = std::clamp(a[0] * b[0] + a[1] * b[1] + a[2] * b[2], -1.0f, 1.0f);
+    const float th_d = std::acos(d);
+    const float th_a = std::acos(std::clamp(ca, -1.0f, 1.0f));
+    const float th_b = std::acos(std::clamp(cb, -1.0f, 1.0f));
+    if (th_d + th_b <= th_a) { copy_a(ca); return; }
+    if (th_d + th_a <= th_b) {
+        axis[0] = b[0]; axis[1] = b[1]; axis[2] = b[2];
+        *cos_half = cb;
+        return;
+    }
+    const float th_c = 0.5f * (th_a + th_b + th_d);
+    if (th_c >= 3.14159265358979f) { copy_a(-1.0f); return; }
+    *cos_half = std::cos(th_c);
+    const float rot = th_c - th_a;
+    const float cr = std::cos(rot), sr = std::sin(rot);
+    float perp[3] = {b[0] - d * a[0], b[1] - d * a[1], b[2] - d * a[2]};
+    const float pl = std::sqrt(perp[0] * perp[0] + perp[1] * perp[1] + perp[2] * perp[2]);
+    if (!(pl > 1e-6f)) {
+        axis[0] = a[0]; axis[1] = a[1]; axis[2] = a[2];
+        if (d < 0.0f) *cos_half = -1.0f;
+        return;
+    }
+    const float ip = 1.0f / pl;
+    for (float& v : perp) v *= ip;
+    for (int k = 0; k < 3; ++k) axis[k] = cr * a[k] + sr * perp[k];
+    const float al = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    if (al > 1e-6f) {
+        const float ia = 1.0f / al;
+        for (int k = 0; k < 3; ++k) axis[k] *= ia;
+    }
+}
+
+void lt_build(std::vector<LtLeaf>& lv, std::vector<float>& nodes, uint32_t b, uint32_t e, uint32_t dst) {
+    float* me = &nodes[(size_t)dst * 16];
+    if (e - b == 1) {
+        const LtLeaf& s = lv[b];
+        for (int a = 0; a < 3; ++a) { me[a] = s.lo[a]; me[4 + a] = s.hi[a]; me[8 + a] = s.axis[a]; }
+        uint32_t one = 1u;
+        std::memcpy(&me[3], &s.light, 4);
+        std::memcpy(&me[7], &one, 4);
+        me[11] = s.cos_half;
+        me[12] = s.power;
+        return;
+    }
+    float cl[3] = {INFINITY, INFINITY, INFINITY}, ch[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = b; i < e; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const float c = 0.5f * (lv[i].lo[a] + lv[i].hi[a]);
+            cl[a] = std::min(cl[a], c);
+            ch[a] = std::max(ch[a], c);
+        }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (ch[a] - cl[a] > ch[axis] - cl[axis]) axis = a;
+    const uint32_t mid = b + (e - b) / 2;
+    if (!(ch[axis] - cl[axis] < 1e-6f)) {
+        std::nth_element(lv.begin() + b, lv.begin() + mid, lv.begin() + e, [axis](const LtLeaf& x, const LtLeaf& y) {
+            return 0.5f * (x.lo[axis] + x.hi[axis]) < 0.5f * (y.lo[axis] + y.hi[axis]);
+        });
+    }
+    const uint32_t left = (uint32_t)(nodes.size() / 16);
+    nodes.resize(nodes.size() + 32, 0.0f);
+    lt_build(lv, nodes, b, mid, left);
+    lt_build(lv, nodes, mid, e, left + 1);
+    me = &nodes[(size_t)dst * 16];
+    const float* L = &nodes[(size_t)left * 16];
+    const float* R = &nodes[(size_t)(left + 1) * 16];
+    for (int a = 0; a < 3; ++a) {
+        me[a] = std::min(L[a], R[a]);
+        me[4 + a] = std::max(L[4 + a], R[4 + a]);
+    }
+    me[12] = L[12] + R[12];
+    cone_union(&L[8], L[11], &R[8], R[11], &me[8], &me[11]);
+    uint32_t zero = 0u;
+    std::memcpy(&me[3], &left, 4);
+    std::memcpy(&me[7], &zero, 4);
+}
+
+}  // namespace
+
+std::vector<float> build_light_tree(const float* lights, uint32_t n) {
+    std::vector<float> nodes;
+    if (n == 0) return nodes;
+    std::vector<LtLeaf> lv;
+    lv.reserve(n);
+    for (uint32_t i = 0; i < n; ++i) lv.push_back(lt_leaf(lights + (size_t)20 * i, i));
+    nodes.reserve((size_t)32 * n);
+    nodes.resize(16, 0.0f);
+    lt_build(lv, nodes, 0, n, 0);
+    return nodes;
+}
+
+double build_env_cdf(const float* rgb, uint32_t W, uint32_t H, std::vector<float>& marginal,
+                     std::vector<float>& conditional) {
+    marginal.assign(H, 0.0f);
+    conditional.assign((size_t)W * H, 0.0f);
+    double total = 0.0;
+    for (uint32_t v = 0; v < H; ++v) {
+        const double s = std::sin(3.14159265358979323846 * (double(v) + 0.5) / double(H));
+        double run = 0.0;
+        float* row = &conditional[(size_t)v * W];
+        const float* px = rgb + (size_t)v * W * 3;
+        for (uint32_t u = 0; u < W; ++u) {
+            const float lum = 0.2126f * px[3 * u] + 0.7152f * px[3 * u + 1] + 0.0722f * px[3 * u + 2];
+            run += double(lum) * s;
+            row[u] = float(run);
+        }
+        const double inv = run > 0.0 ? 1.0 / run : 0.0;
+        for (uint32_t u = 0; u < W; ++u) row[u] = float(double(row[u]) * inv);
+        marginal[v] = float(run);
+        total += run;
+    }
+    const double inv = total > 0.0 ? 1.0 / total : 0.0;
+    double run = 0.0;
+    for (uint32_t v = 0; v < H; ++v) {
+        run += marginal[v];
+        marginal[v] = float(run * inv);
+    }
+    if (H > 0 && total > 0.0) marginal[H - 1] = 1.0f;
+    return total;
+}
+
+}  // namespace dmt

@@ -1,0 +1,407 @@
+// Device-side building blocks of the gfx950 path tracer.
+//
+// Every expression that feeds a result the path tracer reports (hit distance,
+// normal, radiance) is written with an explicit operation order and the whole
+// translation unit is compiled with -ffp-contract=off, so each a*b+c rounds
+// twice exactly as the reference's shader expressions are written
+// (shaders/PathTrace.slang, shaders/PathTraceMath.slang).  Division and sqrt
+// are the correctly rounded HIP defaults.  sin/cos/acos/atan2 are evaluated
+// with the published single-precision Cephes algorithms (S. L. Moshier,
+// Cephes Math Library: sinf.c, asinf.c, atanf.c) so that the results do not
+// depend on the device math library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dmt {
+
+// ---------------------------------------------------------------------------
+// vectors
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) {
+    float inv = 1.0f / sqrtf(dot(a, a));
+    return a * inv;
+}
+__device__ __forceinline__ f3 reflect(f3 i, f3 n) {
+    float d = dot(n, i);
+    return i - n * (2.0f * d);
+}
+__device__ __forceinline__ f3 refract(f3 i, f3 n, float eta) {
+    float ni = dot(n, i);
+    float k = 1.0f - (eta * eta) * (1.0f - ni * ni);
+    if (k < 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    return i * eta - n * (eta * ni + sqrtf(k));
+}
+__device__ __forceinline__ float maxc(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ bool any_pos(f3 a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ f3 lerp3(f3 a, f3 b, float t) { return a + (b - a) * t; }
+// quatRotate (PathTrace.slang:2028-2031).
+__device__ __forceinline__ f3 quat_rotate(float4 q, f3 v) {
+    f3 u{q.x, q.y, q.z};
+    f3 c = cross(u, cross(u, v) + v * q.w);
+    return v + c * 2.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Cephes single-precision transcendental functions
+
+__device__ __forceinline__ void cephes_sincos(float xin, float* s_out, float* c_out) {
+    const float DP1 = 0.78515625f;
+    const float DP2 = 2.4187564849853515625e-4f;
+    const float DP3 = 3.77489497744594108e-8f;
+    const float FOPI = 1.27323954473516f;
+    float x = xin;
+    bool neg = false;
+    if (x < 0.0f) { x = -x; neg = true; }
+    unsigned j = (unsigned)(x * FOPI);
+    float y = (float)j;
+    if (j & 1u) { j += 1u; y += 1.0f; }
+    j &= 7u;
+    bool flip_s = neg, flip_c = false;
+    if (j > 3u) { j -= 4u; flip_s = !flip_s; flip_c = !flip_c; }
+    if (j > 1u) flip_c = !flip_c;
+    float r = ((x - y * DP1) - y * DP2) - y * DP3;
+    float z = r * r;
+    float ps = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * r + r;
+    float pc = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z -
+               0.5f * z + 1.0f;
+    bool swap = (j == 1u || j == 2u);
+    float s = swap ? pc : ps;
+    float c = swap ? ps : pc;
+    *s_out = flip_s ? -s : s;
+    *c_out = flip_c ? -c : c;
+}
+__device__ __forceinline__ float c_sin(float x) { float s, c; cephes_sincos(x, &s, &c); return s; }
+__device__ __forceinline__ float c_cos(float x) { float s, c; cephes_sincos(x, &s, &c); return c; }
+__device__ __forceinline__ float asin_kernel(float z, float x) {
+    return ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
+            1.6666752422e-1f) * z * x + x;
+}
+__device__ __forceinline__ float c_asin(float xin) {
+    float a = fabsf(xin);
+    float r;
+    if (a > 0.5f) {
+        float z = 0.5f * (1.0f - a);
+        r = asin_kernel(z, sqrtf(z));
+        r = 1.5707963267948966f - (r + r);
+    } else {
+        r = asin_kernel(a * a, a);
+    }
+    return xin < 0.0f ? -r : r;
+}
+__device__ __forceinline__ float c_acos(float x) {
+    if (x < -0.5f) return 3.14159265358979f - 2.0f * c_asin(sqrtf(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * c_asin(sqrtf(0.5f * (1.0f - x)));
+    return 1.5707963267948966f - c_asin(x);
+}
+__device__ __forceinline__ float c_atan(float xin) {
+    float x = fabsf(xin);
+    float y;
+    if (x > 2.414213562373095f) { y = 1.5707963267948966f; x = -1.0f / x; }
+    else if (x > 0.4142135623730950f) { y = 0.7853981633974483f; x = (x - 1.0f) / (x + 1.0f); }
+    else { y = 0.0f; }
+    float z = x * x;
+    y = y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z *
+                 x + x);
+    return xin < 0.0f ? -y : y;
+}
+__device__ __forceinline__ float c_atan2(float y, float x) {
+    int code = (x < 0.0f) ? 2 : 0;
+    if (y < 0.0f) code |= 1;
+    if (x == 0.0f) {
+        if (code & 1) return -1.5707963267948966f;
+        if (y == 0.0f) return 0.0f;
+        return 1.5707963267948966f;
+    }
+    if (y == 0.0f) return (code & 2) ? 3.14159265358979f : 0.0f;
+    float w = 0.0f;
+    if (code == 2) w = 3.14159265358979f;
+    else if (code == 3) w = -3.14159265358979f;
+    return w + c_atan(y / x);
+}
+
+// ---------------------------------------------------------------------------
+// RNG (PathTraceMath.slang:16-24)
+
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t x) {
+    x = x * 747796405u + 2891336453u;
+    x = ((x >> ((x >> 28u) + 4u)) ^ x) * 277803737u;
+    return (x >> 22u) ^ x;
+}
+__device__ __forceinline__ float randf(uint32_t& seed) {
+    seed = pcg_hash(seed);
+    return (float)(seed & 0x00FFFFFFu) / 16777216.0f;
+}
+
+// ---------------------------------------------------------------------------
+// scene view
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr uint32_t kMeshBit = 0x80000000u;
+
+struct SceneView {
+    const float4* prims;          // 7 float4 per prim, input order
+    uint32_t prim_count;
+    uint32_t plane_count;
+    const float4* planes;         // (n.xyz, d)
+    const uint32_t* plane_ids;
+    const float4* sph_nodes;      // Node4 = 8 float4
+    const float4* spheres;        // (c.xyz, r), leaf order
+    const uint32_t* sphere_ids;
+    const float4* tri_nodes;
+    const float4* tris;           // 3 float4 per triangle, leaf order
+    const float4* lights;         // 5 float4 per light
+    uint32_t light_count;
+    uint32_t ltree_count;
+    const float4* ltree;          // 4 float4 per node
+    const float4* env;            // texels
+    const float* env_marg;
+    const float* env_cond;
+    uint32_t env_w, env_h;
+    float env_total;
+    float env_intensity;
+    uint32_t env_present;
+    uint32_t pad;
+};
+
+// Traversal stack: `depth_lds` entries in LDS (thread-interleaved), the rest
+// in a per-thread global spill area.
+struct Stack {
+    uint32_t* lds;        // this thread's column base
+    uint32_t lds_stride;  // = block size
+    uint32_t lds_depth;
+    uint32_t* spill;      // this thread's spill area
+    uint32_t spill_depth;
+    __device__ __forceinline__ void put(uint32_t i, uint32_t v) {
+        if (i < lds_depth) lds[i * lds_stride] = v;
+        else spill[i - lds_depth] = v;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t i) const {
+        return (i < lds_depth) ? lds[i * lds_stride] : spill[i - lds_depth];
+    }
+};
+
+struct TraceCounters {
+    uint32_t nodes;
+    uint32_t tris;
+    uint32_t spheres;
+};
+
+struct HitRec {
+    float t;
+    uint32_t code;   // kMiss, analytic input index, or kMeshBit | leaf position
+};
+
+// PBRT's robust slab test bound: 1 + 2*gamma(3), gamma(n) = n*eps / (1 - n*eps).
+__device__ __forceinline__ constexpr float robust_hi() { return 1.0f + 2.0f * (3.0f * 5.9604645e-8f / (1.0f - 3.0f * 5.9604645e-8f)); }
+
+// Tests the four children of a node; returns the hit mask and entry distances.
+__device__ __forceinline__ uint32_t node4_test(const float4* np, f3 o, f3 inv, float tbest, float tn[4],
+                                               uint32_t ch[4]) {
+    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    const uint4 c = *reinterpret_cast<const uint4*>(np + 6);
+    ch[0] = c.x; ch[1] = c.y; ch[2] = c.z; ch[3] = c.w;
+    const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+    const float g = robust_hi();
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float t0x = (lxs[i] - o.x) * inv.x, t1x = (hxs[i] - o.x) * inv.x;
+        float t0y = (lys[i] - o.y) * inv.y, t1y = (hys[i] - o.y) * inv.y;
+        float t0z = (lzs[i] - o.z) * inv.z, t1z = (hzs[i] - o.z) * inv.z;
+        float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+        float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+        tmax = tmax * g;
+        tmin = tmin / g;
+        bool hit = (ch[i] != kEmpty) && tmax >= fmaxf(tmin, 0.0f) && tmin <= tbest;
+        tn[i] = hit ? tmin : __builtin_huge_valf();
+        mask |= hit ? (1u << i) : 0u;
+    }
+    return mask;
+}
+
+__device__ __forceinline__ void cswap(float& ta, uint32_t& ca, float& tb, uint32_t& cb) {
+    if (tb < ta) {
+        float t = ta; ta = tb; tb = t;
+        uint32_t c = ca; ca = cb; cb = c;
+    }
+}
+
+// Generic BVH4 walk: visits children near-to-far, pushes the farther ones.
+// `leaf(first, count)` tests a leaf range and may lower `tbest`.
+template <bool COUNT, class LeafFn>
+__device__ __forceinline__ void bvh4_walk(const float4* nodes, f3 o, f3 inv, float& tbest, Stack& st, LeafFn&& leaf,
+                                          TraceCounters& tc) {
+    uint32_t sp = 0;
+    uint32_t cur = 0u;   // root node (internal reference 0)
+    for (;;) {
+        if (cur & kLeaf) {
+            leaf(cur & ((1u << 27) - 1u), ((cur >> 27) & 15u) + 1u);
+        } else {
+            if (COUNT) tc.nodes++;
+            float tn[4];
+            uint32_t ch[4];
+            uint32_t m = node4_test(nodes + (size_t)cur * 8u, o, inv, tbest, tn, ch);
+            if (m) {
+                // sort (t, child) ascending; misses carry +inf and sink.
+                cswap(tn[0], ch[0], tn[1], ch[1]);
+                cswap(tn[2], ch[2], tn[3], ch[3]);
+                cswap(tn[0], ch[0], tn[2], ch[2]);
+                cswap(tn[1], ch[1], tn[3], ch[3]);
+                cswap(tn[1], ch[1], tn[2], ch[2]);
+                uint32_t n = __popc(m);
+                for (int i = (int)n - 1; i >= 1; --i) st.put(sp++, ch[i]);
+                cur = ch[0];
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        cur = st.get(--sp);
+    }
+}
+
+// traceScene (PathTrace.slang:2331-2693): planes, then the sphere BVH, then
+// the triangle BVH limited to the analytic hit distance.  Ties resolve to
+// the lower primitive index; analytic hits win ties against triangles.
+template <bool COUNT>
+__device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, Stack& st, TraceCounters& tc) {
+    float best = 1e30f;
+    uint32_t best_id = 0xFFFFFFFFu;   // analytic input index of the best hit
+    uint32_t code = kMiss;
+    for (uint32_t i = 0; i < S.plane_count; ++i) {
+        const float4 p = S.planes[i];
+        const uint32_t id = S.plane_ids[i];
+        f3 n = xyz(p);
+        float denom = dot(d, n);
+        if (fabsf(denom) < 1e-4f) continue;
+        float t = -(dot(o, n) + p.w) / denom;
+        if (t > 1e-3f && (t < best || (t == best && id < best_id))) {
+            best = t; best_id = id; code = id;
+        }
+    }
+    f3 rs{fabsf(d.x) > 1e-30f ? d.x : 1e-30f, fabsf(d.y) > 1e-30f ? d.y : 1e-30f, fabsf(d.z) > 1e-30f ? d.z : 1e-30f};
+    f3 inv{1.0f / rs.x, 1.0f / rs.y, 1.0f / rs.z};
+    if (S.sph_nodes) {
+        bvh4_walk<COUNT>(S.sph_nodes, o, inv, best, st, [&](uint32_t first, uint32_t count) {
+            for (uint32_t k = 0; k < count; ++k) {
+                const float4 s = S.spheres[first + k];
+                const uint32_t id = S.sphere_ids[first + k];
+                if (COUNT) tc.spheres++;
+                f3 oc = o - xyz(s);
+                float b = dot(oc, d);
+                float kk = dot(oc, oc) - s.w * s.w;
+                float h = b * b - kk;
+                if (h < 0.0f) continue;
+                h = sqrtf(h);
+                float t0 = -b - h, t1 = -b + h;
+                float t = (t0 > 1e-3f) ? t0 : t1;
+                if (t > 1e-3f && (t < best || (t == best && id < best_id))) {
+                    best = t; best_id = id; code = id;
+                }
+            }
+        }, tc);
+    }
+    if (S.tri_nodes) {
+        float tm = best;
+        uint32_t tri_id = 0u;   // a triangle at exactly the analytic distance never wins
+        uint32_t tri_pos = kMiss;
+        bvh4_walk<COUNT>(S.tri_nodes, o, inv, tm, st, [&](uint32_t first, uint32_t count) {
+            for (uint32_t k = 0; k < count; ++k) {
+                const float4* tr = S.tris + (size_t)(first + k) * 3u;
+                const float4 a = tr[0], e1v = tr[1], e2v = tr[2];
+                if (COUNT) tc.tris++;
+                const f3 v0 = xyz(a), e1 = xyz(e1v), e2 = xyz(e2v);
+                const f3 pv = cross(d, e2);
+                const float det = dot(e1, pv);
+                if (fabsf(det) < 1e-8f) continue;
+                const float inv_det = 1.0f / det;
+                const f3 tv = o - v0;
+                const float u = dot(tv, pv) * inv_det;
+                if (u < 0.0f || u > 1.0f) continue;
+                const f3 qv = cross(tv, e1);
+                const float v = dot(d, qv) * inv_det;
+                if (v < 0.0f || u + v > 1.0f) continue;
+                const float t = dot(e2, qv) * inv_det;
+                const uint32_t id = __float_as_uint(a.w);
+                if (t > 1e-3f && (t < tm || (t == tm && id < tri_id))) {
+                    tm = t; tri_id = id; tri_pos = first + k;
+                }
+            }
+        }, tc);
+        if (tri_pos != kMiss) {
+            best = tm;
+            code = kMeshBit | tri_pos;
+        }
+    }
+    return HitRec{best, code};
+}
+
+// Surface attributes at a hit (the HitInfo fill of testAnalyticPrim,
+// PathTrace.slang:2085-2150, and of the mesh path, PathTrace.slang:2640-2668).
+struct Surface {
+    f3 n;
+    f3 albedo;
+    f3 emission;
+    float rough, ior;
+    uint32_t mat;
+    uint32_t user_id;
+    uint32_t prim;    // analytic input index or triangle id
+    uint32_t kind;    // 1 analytic, 2 mesh
+};
+
+__device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, HitRec h) {
+    Surface s;
+    if (h.code & kMeshBit) {
+        const float4* tr = S.tris + (size_t)(h.code & ~kMeshBit) * 3u;
+        const float4 a = tr[0];
+        s.n = normalize(cross(xyz(tr[1]), xyz(tr[2])));
+        s.albedo = f3{1.0f, 0.85f, 0.45f};   // MESH_ALBEDO (PathTrace.slang:1428)
+        s.emission = f3{0.0f, 0.0f, 0.0f};
+        s.rough = 0.05f;
+        s.ior = 1.0f;
+        s.mat = 1u;                           // MAT_METAL (PathTrace.slang:2655)
+        s.user_id = 0u;
+        s.prim = __float_as_uint(a.w);
+        s.kind = 2u;
+    } else {
+        const float4* p = S.prims + (size_t)h.code * 7u;
+        const float4 v0 = p[0], v1 = p[1], v2 = p[2], v4 = p[4];
+        if (v2.x == 0.0f) {
+            s.n = normalize((o + d * h.t) - xyz(v0));
+        } else {
+            s.n = quat_rotate(p[5], xyz(v0));
+        }
+        s.albedo = xyz(v1);
+        s.rough = v1.w;
+        s.mat = (uint32_t)v2.y;
+        s.ior = v2.z;
+        s.emission = xyz(v4);
+        s.user_id = __float_as_uint(v2.w);
+        s.prim = h.code;
+        s.kind = 1u;
+    }
+    return s;
+}
+
+}  // namespace dmt

@@ -1,0 +1,367 @@
+"""ctypes binding of libdmt_hip.so (the C ABI declared in include/dmt.h).
+
+Thin plumbing for the test harness and the benchmark: every call goes through
+the C ABI of the in-tree library.  Loading fails loudly when the library has
+not been built -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libdmt_hip.so"))
+
+DMT_OK = 0
+DMT_ERR_INVALID_ARG = -1
+DMT_ERR_OUT_OF_MEMORY = -2
+DMT_ERR_NO_DEVICE = -3
+DMT_ERR_DEVICE = -4
+DMT_ERR_UNSUPPORTED = -5
+DMT_ERR_DEVICE_LOST = -6
+DMT_ERR_LIMIT = -7
+
+PRIM_FLOATS = 28
+LIGHT_FLOATS = 20
+
+
+class FrameParams(C.Structure):
+    _fields_ = [
+        ("pos_fovtan", C.c_float * 4),
+        ("fwd_aspect", C.c_float * 4),
+        ("right_xyz", C.c_float * 4),
+        ("up_xyz", C.c_float * 4),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("frame_index", C.c_uint32),
+        ("spp", C.c_uint32),
+        ("max_bounces", C.c_uint32),
+        ("reset_accum", C.c_uint32),
+        ("mis_enabled", C.c_uint32),
+        ("light_tree_enabled", C.c_uint32),
+        ("refract_bounces", C.c_uint32),
+        ("firefly_clamp", C.c_float),
+        ("ema_alpha", C.c_float),
+        ("reserved", C.c_uint32),
+    ]
+
+    def copy(self) -> "FrameParams":
+        f = FrameParams()
+        C.memmove(C.byref(f), C.byref(self), C.sizeof(FrameParams))
+        return f
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("prims", C.POINTER(C.c_float)), ("prim_count", C.c_uint32),
+        ("lights", C.POINTER(C.c_float)), ("light_count", C.c_uint32),
+        ("positions", C.POINTER(C.c_float)), ("vertex_count", C.c_uint32),
+        ("indices", C.POINTER(C.c_uint32)), ("index_count", C.c_uint32),
+        ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
+        ("env_intensity", C.c_float),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("sah_bins", C.c_uint32), ("max_leaf", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("normal", C.c_float * 3), ("hit", C.c_uint32), ("prim", C.c_uint32),
+                ("mat", C.c_uint32), ("user_id", C.c_uint32)]
+
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("normal", "<f4", (3,)), ("hit", "<u4"), ("prim", "<u4"),
+                      ("mat", "<u4"), ("user_id", "<u4")])
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("primary_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+        ("frames", C.c_uint64), ("extend_launches", C.c_uint64), ("extend_ms", C.c_double),
+        ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("sphere_tests", C.c_uint64),
+        ("ext_node_visits", C.c_uint64), ("ext_tri_tests", C.c_uint64), ("ext_sphere_tests", C.c_uint64),
+        ("scene_bytes", C.c_uint64), ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GenOptions(C.Structure):
+    _fields_ = [("subdivisions", C.c_int32), ("random_tris", C.c_int32), ("spheres", C.c_int32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("env_width", C.c_uint32),
+                ("env_height", C.c_uint32)]
+
+
+class GeneratedScene(C.Structure):
+    _fields_ = [
+        ("prims", C.POINTER(C.c_float)), ("prim_count", C.c_uint32),
+        ("lights", C.POINTER(C.c_float)), ("light_count", C.c_uint32),
+        ("positions", C.POINTER(C.c_float)), ("vertex_count", C.c_uint32),
+        ("indices", C.POINTER(C.c_uint32)), ("index_count", C.c_uint32),
+        ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
+        ("env_intensity", C.c_float),
+        ("frame", FrameParams),
+    ]
+
+
+class DmtError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"dmt status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, i32, st = C.c_void_p, C.c_uint32, C.c_int32, C.c_int32
+    fp = C.POINTER(C.c_float)
+    sigs = {
+        "dmt_get_abi_version": (st, [C.POINTER(u32)]),
+        "dmt_device_create": (st, [i32, C.POINTER(vp)]),
+        "dmt_device_destroy": (None, [vp]),
+        "dmt_device_stream": (st, [vp, C.POINTER(vp)]),
+        "dmt_device_synchronize": (st, [vp]),
+        "dmt_device_is_lost": (i32, [vp]),
+        "dmt_scene_create": (st, [vp, C.POINTER(SceneDesc), C.POINTER(BuildParams), C.POINTER(vp)]),
+        "dmt_scene_destroy": (None, [vp]),
+        "dmt_render": (st, [vp, C.POINTER(FrameParams)]),
+        "dmt_readback_accum": (st, [vp, fp, C.c_size_t]),
+        "dmt_accum_device_ptr": (st, [vp, C.POINTER(vp)]),
+        "dmt_trace_closest": (st, [vp, fp, u32, vp]),
+        "dmt_scene_stats": (st, [vp, C.POINTER(Stats)]),
+        "dmt_scene_reset_stats": (st, [vp]),
+        "dmt_scene_measure_traffic": (st, [vp, C.POINTER(FrameParams), C.POINTER(Stats)]),
+        "dmt_scene_set_profiling": (st, [vp, i32]),
+        "dmt_last_error": (C.c_char_p, []),
+        "dmt_scene_generate": (st, [u32, C.c_uint64, C.POINTER(GenOptions), C.POINTER(C.POINTER(GeneratedScene))]),
+        "dmt_scene_generated_free": (None, [C.POINTER(GeneratedScene)]),
+        "dmt_camera_look_at": (st, [fp, fp, C.c_float, u32, u32, C.POINTER(FrameParams)]),
+        "dmt_build_bvh4": (st, [fp, u32, C.POINTER(u32), u32, u32, u32, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
+        "dmt_build_light_tree": (u32, [fp, u32, fp]),
+        "dmt_build_env_cdf": (C.c_double, [fp, u32, u32, fp, fp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(status: int):
+    if status != DMT_OK:
+        raise DmtError(status, lib().dmt_last_error().decode())
+
+
+def _fptr(a: np.ndarray | None):
+    if a is None or a.size == 0:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _uptr(a: np.ndarray | None):
+    if a is None or a.size == 0:
+        return None
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+@dataclass
+class SceneData:
+    """Host copy of a scene in the reference wire formats."""
+    prims: np.ndarray = field(default_factory=lambda: np.zeros((0, PRIM_FLOATS), np.float32))
+    lights: np.ndarray = field(default_factory=lambda: np.zeros((0, LIGHT_FLOATS), np.float32))
+    positions: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    indices: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.uint32))
+    env: np.ndarray | None = None          # (H, W, 3) float32
+    env_intensity: float = 1.0
+    frame: FrameParams = field(default_factory=FrameParams)
+
+    def desc(self) -> SceneDesc:
+        for name in ("prims", "lights", "positions"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name), np.float32))
+        self.indices = np.ascontiguousarray(self.indices, np.uint32)
+        d = SceneDesc()
+        d.prims, d.prim_count = _fptr(self.prims), self.prims.shape[0]
+        d.lights, d.light_count = _fptr(self.lights), self.lights.shape[0]
+        d.positions, d.vertex_count = _fptr(self.positions), self.positions.shape[0]
+        d.indices, d.index_count = _uptr(self.indices), self.indices.size
+        if self.env is not None:
+            self.env = np.ascontiguousarray(self.env, np.float32)
+            d.env_rgb, d.env_height, d.env_width = _fptr(self.env), self.env.shape[0], self.env.shape[1]
+        d.env_intensity = self.env_intensity
+        return d
+
+
+def generate(config: int, seed: int = 1, subdivisions: int = -1, random_tris: int = -1, spheres: int = -1,
+             width: int = 0, height: int = 0, env_width: int = 0, env_height: int = 0) -> SceneData:
+    """Seeded synthetic workload (dmt_scene_generate), copied to numpy."""
+    opt = GenOptions(subdivisions, random_tris, spheres, width, height, env_width, env_height)
+    out = C.POINTER(GeneratedScene)()
+    _check(lib().dmt_scene_generate(config, seed, C.byref(opt), C.byref(out)))
+    g = out.contents
+    try:
+        def arr(ptr, n, dt, shape):
+            if n == 0 or not ptr:
+                return np.zeros(shape if shape[0] == 0 else (0,) + shape[1:], dt)
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).reshape(shape).copy()
+        sd = SceneData(
+            prims=arr(g.prims, g.prim_count * PRIM_FLOATS, np.float32, (g.prim_count, PRIM_FLOATS)),
+            lights=arr(g.lights, g.light_count * LIGHT_FLOATS, np.float32, (g.light_count, LIGHT_FLOATS)),
+            positions=arr(g.positions, g.vertex_count * 3, np.float32, (g.vertex_count, 3)),
+            indices=arr(g.indices, g.index_count, np.uint32, (g.index_count // 3, 3)),
+            env=arr(g.env_rgb, g.env_width * g.env_height * 3, np.float32, (g.env_height, g.env_width, 3)),
+            env_intensity=g.env_intensity,
+            frame=g.frame.copy() if hasattr(g.frame, "copy") else g.frame,
+        )
+        f = FrameParams()
+        C.memmove(C.byref(f), C.byref(g.frame), C.sizeof(FrameParams))
+        sd.frame = f
+        return sd
+    finally:
+        lib().dmt_scene_generated_free(out)
+
+
+def look_at(eye, target, fov_deg: float, width: int, height: int, base: FrameParams | None = None) -> FrameParams:
+    f = base.copy() if base is not None else FrameParams()
+    e = (C.c_float * 3)(*eye)
+    t = (C.c_float * 3)(*target)
+    _check(lib().dmt_camera_look_at(e, t, fov_deg, width, height, C.byref(f)))
+    return f
+
+
+class Device:
+    def __init__(self, index: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().dmt_device_create(index, C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stream_ptr(self) -> int:
+        s = C.c_void_p()
+        _check(lib().dmt_device_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    def synchronize(self):
+        _check(lib().dmt_device_synchronize(self._h))
+
+    def is_lost(self) -> bool:
+        return bool(lib().dmt_device_is_lost(self._h))
+
+    def close(self):
+        if self._h:
+            lib().dmt_device_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    def __init__(self, device: Device, data: SceneData, sah_bins: int = 0, max_leaf: int = 0):
+        self.device = device
+        self.data = data
+        self._h = C.c_void_p()
+        d = data.desc()
+        bp = BuildParams(sah_bins, max_leaf)
+        _check(lib().dmt_scene_create(device.handle, C.byref(d), C.byref(bp), C.byref(self._h)))
+
+    def render(self, frame: FrameParams):
+        _check(lib().dmt_render(self._h, C.byref(frame)))
+
+    def readback(self, width: int, height: int) -> np.ndarray:
+        out = np.empty((height, width, 4), np.float32)
+        _check(lib().dmt_readback_accum(self._h, _fptr(out), out.size))
+        return out
+
+    def accum_ptr(self) -> int:
+        p = C.c_void_p()
+        _check(lib().dmt_accum_device_ptr(self._h, C.byref(p)))
+        return p.value or 0
+
+    def trace(self, origins: np.ndarray, dirs: np.ndarray) -> np.ndarray:
+        n = origins.shape[0]
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = origins
+        rays[:, 4:7] = dirs
+        out = np.zeros(n, HIT_DTYPE)
+        _check(lib().dmt_trace_closest(self._h, _fptr(rays), n, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().dmt_scene_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        _check(lib().dmt_scene_reset_stats(self._h))
+
+    def set_profiling(self, on: bool):
+        _check(lib().dmt_scene_set_profiling(self._h, 1 if on else 0))
+
+    def measure_traffic(self, frame: FrameParams) -> dict:
+        s = Stats()
+        _check(lib().dmt_scene_measure_traffic(self._h, C.byref(frame), C.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().dmt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_bvh4(positions: np.ndarray, indices: np.ndarray, bins: int = 0, max_leaf: int = 0):
+    """Host SAH BVH4 build (dmt_build_bvh4): returns (nodes float32 (N,32), order uint32, depth)."""
+    pos = np.ascontiguousarray(positions, np.float32)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    count, depth = C.c_uint32(), C.c_uint32()
+    _check(lib().dmt_build_bvh4(_fptr(pos), pos.shape[0], _uptr(idx), idx.size, bins, max_leaf, None, None,
+                                C.byref(count), C.byref(depth)))
+    nodes = np.zeros((count.value, 32), np.float32)
+    order = np.zeros(idx.size // 3, np.uint32)
+    _check(lib().dmt_build_bvh4(_fptr(pos), pos.shape[0], _uptr(idx), idx.size, bins, max_leaf,
+                                nodes.ctypes.data_as(C.c_void_p), order.ctypes.data_as(C.c_void_p),
+                                C.byref(count), C.byref(depth)))
+    return nodes, order, depth.value
+
+
+def build_light_tree(lights: np.ndarray) -> np.ndarray:
+    L = np.ascontiguousarray(lights, np.float32)
+    n = L.shape[0]
+    out = np.zeros((max(2 * n - 1, 1), 16), np.float32)
+    cnt = lib().dmt_build_light_tree(_fptr(L), n, _fptr(out))
+    return out[:cnt]
+
+
+def build_env_cdf(env: np.ndarray):
+    e = np.ascontiguousarray(env, np.float32)
+    H, W = e.shape[0], e.shape[1]
+    marg = np.zeros(H, np.float32)
+    cond = np.zeros(H * W, np.float32)
+    total = lib().dmt_build_env_cdf(_fptr(e), W, H, _fptr(marg), _fptr(cond))
+    return marg, cond.reshape(H, W), total

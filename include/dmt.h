@@ -1,0 +1,184 @@
+/*
+ * dmt.h -- C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * This is the drop-in boundary for the reference's software path-trace
+ * backend.  In the reference the engine reaches the tracer through the RHI:
+ *   Device::Create(BackendType, ...)                  src/rhi/Device.cpp:20-48
+ *   CreateBLAS / CreateTLAS (Embree scene build)      src/rhi_software/SoftwareDevice.cpp:337-421
+ *   WriteBuffer (prims, lights, light tree, env CDF)  src/rhi_software/SoftwareDevice.cpp:539-558,
+ *                                                     src/engine/Engine.cpp:4743-5002, 5235-5600
+ *   PushConstants(PtPush) + Dispatch("pathtrace")     src/engine/Engine.cpp:9577-9582,
+ *                                                     src/rhi_software/SoftwareDevice.cpp:105-136
+ *   RunPathTraceKernel(...)                           src/rhi_software/SoftwareTracer.h:40-41
+ *   ReadbackTexture(accum)                            src/rhi_software/SoftwareDevice.cpp:458-483
+ * The entry points below replace those calls one for one (each declaration
+ * names the reference interface it replaces).  Plain pointers and sizes only;
+ * every host buffer is copied on entry; no callbacks.  One thread per device.
+ * Work is ordered on the device's HIP stream; functions that return host data
+ * synchronise that stream first.
+ *
+ * Wire formats are the reference's:
+ *   analytic primitive  7 float4 = 112 B (Engine.cpp:4924-4994):
+ *     v0 = centre.xyz, radius  (plane: normal.xyz, d with n.p + d = 0)
+ *     v1 = albedo.rgb, roughness
+ *     v2 = type (0 sphere, 1 plane), material (0 Lambert, 1 metal, 2 dielectric),
+ *          ior, user id (uint32 bits)
+ *     v3 = previous centre (motion blur; must equal v0 -- no motion blur)
+ *     v4 = emission.rgb, 0
+ *     v5 = orientation quaternion xyzw (planes: normal = quatRotate(q, v0.xyz))
+ *     v6 = texture tiles (ignored: no PBR atlas)
+ *   analytic light      5 float4 = 80 B (Engine.cpp:5235-5300):
+ *     v0 = pos.xyz, radius; v1 = intensity.rgb, type (0 point, 1 spot, 2 sphere, 3 quad)
+ *     v2 = dir.xyz (spot axis / quad normal), cos_outer
+ *     v3 = quad u half-extent vector.xyz (spot: x = cos_inner), v_half
+ *     v4 = orientation quaternion xyzw
+ *   mesh                float[3V] positions, uint32[3T] indices
+ *   env map             float[3*W*H] linear RGB, row 0 = +Y pole (lat-long)
+ *   accumulation        float4 per pixel: running-mean radiance rgb, sample count
+ */
+#ifndef DMT_H
+#define DMT_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMT_ABI_VERSION 1u
+
+typedef int32_t dmt_status;
+#define DMT_OK                  0
+#define DMT_ERR_INVALID_ARG    -1   /* bad pointer, size or enum value       */
+#define DMT_ERR_OUT_OF_MEMORY  -2   /* device or host allocation failed       */
+#define DMT_ERR_NO_DEVICE      -3   /* no HIP device / device index invalid   */
+#define DMT_ERR_DEVICE         -4   /* a HIP call failed                      */
+#define DMT_ERR_UNSUPPORTED    -5   /* feature outside this backend's scope   */
+#define DMT_ERR_DEVICE_LOST    -6   /* device reported a fault; terminal      */
+#define DMT_ERR_LIMIT          -7   /* size beyond a documented maximum       */
+
+#define DMT_PRIM_FLOATS   28u
+#define DMT_LIGHT_FLOATS  20u
+#define DMT_LTREE_FLOATS  16u
+#define DMT_MAX_SPP       32u       /* r_spp clamp (Engine.cpp:7223-7228) */
+#define DMT_MAX_BOUNCES   64u
+
+typedef struct dmt_device dmt_device;
+typedef struct dmt_scene  dmt_scene;
+
+typedef struct dmt_scene_desc {
+    const float*    prims;      uint32_t prim_count;
+    const float*    lights;     uint32_t light_count;
+    const float*    positions;  uint32_t vertex_count;
+    const uint32_t* indices;    uint32_t index_count;
+    const float*    env_rgb;    uint32_t env_width;  uint32_t env_height;
+    float           env_intensity;
+    uint32_t        reserved;
+} dmt_scene_desc;
+
+/* Per-frame parameters: the PtPush fields the path reads
+ * (Engine.cpp:7243, 7821-8676; camera basis Engine.cpp:7821-7828). */
+typedef struct dmt_frame_params {
+    float    pos_fovtan[4];     /* camera position, tan(fov_y / 2)           */
+    float    fwd_aspect[4];     /* forward, aspect = width / height          */
+    float    right_xyz[4];
+    float    up_xyz[4];
+    uint32_t width, height;
+    uint32_t frame_index;       /* seeds pcgHash(x*1973 + y*9277 + f*26699) */
+    uint32_t spp;               /* 1..32                                     */
+    uint32_t max_bounces;       /* r_max_bounces                             */
+    uint32_t reset_accum;
+    uint32_t mis_enabled;       /* r_mis                                     */
+    uint32_t light_tree_enabled;/* r_light_tree                              */
+    uint32_t refract_bounces;   /* r_refract_bounces (transmittance budget)  */
+    float    firefly_clamp;     /* r_firefly_clamp, <= 0 disables            */
+    float    ema_alpha;         /* r_accum_ema_alpha                         */
+    uint32_t reserved;
+} dmt_frame_params;
+
+typedef struct dmt_build_params {
+    uint32_t sah_bins;          /* binned-SAH bin count (0 = default 16)     */
+    uint32_t max_leaf;          /* triangles per leaf (0 = default 4, <= 8)  */
+    uint32_t reserved[2];
+} dmt_build_params;
+
+typedef struct dmt_hit {
+    float    t;
+    float    normal[3];
+    uint32_t hit;               /* 0 miss, 1 analytic, 2 mesh                */
+    uint32_t prim;              /* analytic: input index; mesh: triangle id  */
+    uint32_t mat;
+    uint32_t user_id;           /* analytic v2.w bits; 0 for mesh            */
+} dmt_hit;
+
+typedef struct dmt_stats {
+    uint64_t primary_rays;      /* camera rays traced                        */
+    uint64_t extension_rays;    /* bounce rays traced (b >= 1)               */
+    uint64_t shadow_rays;       /* traceScene steps inside transmittance()   */
+    uint64_t frames;
+    uint64_t extend_launches;   /* closest-hit kernel launches               */
+    double   extend_ms;         /* HIP-event time summed over those launches */
+    uint64_t node_visits;       /* filled by dmt_scene_measure_traffic:      */
+    uint64_t tri_tests;         /*   all traversals (extend + shadow)        */
+    uint64_t sphere_tests;
+    uint64_t ext_node_visits;   /*   closest-hit (extend) kernel only        */
+    uint64_t ext_tri_tests;
+    uint64_t ext_sphere_tests;
+    uint64_t scene_bytes;       /* device bytes of the acceleration data     */
+    uint32_t bvh_nodes;
+    uint32_t bvh_depth;
+} dmt_stats;
+
+/* ABI version check. */
+dmt_status dmt_get_abi_version(uint32_t* out_version);
+
+/* Replaces Device::Create(BackendType::Software, ...) / CreateSoftwareDevice
+ * (src/rhi/Device.cpp:20-48, src/rhi_software/SoftwareDevice.cpp:1165). */
+dmt_status dmt_device_create(int32_t device_index, dmt_device** out_device);
+void       dmt_device_destroy(dmt_device* dev);
+/* hipStream_t the device orders its work on (for caller-side events). */
+dmt_status dmt_device_stream(dmt_device* dev, void** out_stream);
+/* Replaces Device::WaitIdle (src/rhi_software/SoftwareDevice.h:181). */
+dmt_status dmt_device_synchronize(dmt_device* dev);
+/* Replaces Device::IsDeviceLost (src/rhi/Device.h:157-164). */
+int32_t    dmt_device_is_lost(const dmt_device* dev);
+
+/* Replaces CreateBLAS/CreateTLAS + the engine's prim/light/light-tree/env
+ * uploads (SoftwareDevice.cpp:337-421, Engine.cpp:3752-3900, 4743-5002,
+ * 5235-5600, 4571-4605): copies the scene, builds the binned-SAH BVH4, the
+ * light tree and the env CDF on the host, and uploads them to HBM. */
+dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc,
+                            const dmt_build_params* build, dmt_scene** out_scene);
+void       dmt_scene_destroy(dmt_scene* scene);
+
+/* Replaces PushConstants(PtPush) + Dispatch("pathtrace") ->
+ * RunPathTraceKernel (SoftwareTracer.cpp:684-1015, PathTrace.slang:5450-7781):
+ * renders one frame into the scene's accumulation buffer (device resident),
+ * stream-ordered, no host synchronisation. */
+dmt_status dmt_render(dmt_scene* scene, const dmt_frame_params* frame);
+
+/* Replaces ReadbackTexture(accum_hdr) (SoftwareDevice.cpp:458-483):
+ * full-or-nothing copy of width*height float4 into rgba. */
+dmt_status dmt_readback_accum(dmt_scene* scene, float* rgba, size_t float_count);
+/* Device address of the accumulation buffer (width*height float4). */
+dmt_status dmt_accum_device_ptr(dmt_scene* scene, void** out_ptr);
+
+/* traceScene() closest-hit query (PathTrace.slang:2331-2693) for n rays,
+ * 8 floats per ray (ox, oy, oz, -, dx, dy, dz, -); host in, host out. */
+dmt_status dmt_trace_closest(dmt_scene* scene, const float* rays, uint32_t n, dmt_hit* out);
+
+/* Counters of the frames rendered since the last reset. */
+dmt_status dmt_scene_stats(dmt_scene* scene, dmt_stats* out);
+dmt_status dmt_scene_reset_stats(dmt_scene* scene);
+/* Re-renders one frame with the counting traversal (outside any timed region)
+ * and fills node_visits / tri_tests / sphere_tests. */
+dmt_status dmt_scene_measure_traffic(dmt_scene* scene, const dmt_frame_params* frame, dmt_stats* out);
+/* Per-event timing of the closest-hit kernel launches on/off (default off). */
+dmt_status dmt_scene_set_profiling(dmt_scene* scene, int32_t enabled);
+
+/* Last error message of the calling thread ("" if none). */
+const char* dmt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMT_H */

@@ -1,0 +1,76 @@
+/*
+ * dmt_scenegen.h -- seeded synthetic workloads for the path tracer.
+ *
+ * No counterpart in the reference (its scenes come from console commands and
+ * glTF); these generators produce the benchmark and parity workloads in the
+ * reference wire formats of dmt.h.  Host only, no device needed.
+ *
+ *   config 0  64 spheres (centres U[-4,4]^3, radii U[0.2,1]) above the plane
+ *             y = -5; materials by seeded shuffle of i mod 3 (Lambert albedo
+ *             U[0.1,0.9]^3 / metal roughness U[0,0.5] / dielectric IOR 1.5);
+ *             one quad light (emission 10); all-zero env (black sky);
+ *             256x256, 1 spp, 4 bounces.
+ *   config 1  icosphere (radius 3) subdivided 8x, displaced along the normal by
+ *             seeded value noise (amplitude 0.1), plus 200k random triangles
+ *             (edges <= 0.05) uniform in [-5,5]^3; one quad light; 512x256
+ *             env with lognormal texels exp(N(0,1)); 1920x1080, 1 spp, 8 bounces.
+ *   config 4  config 0 with 256 spheres, 256 distinct materials; 512x512,
+ *             16 spp, 4 bounces.
+ * Camera for all: at (0, 2, 12) looking at the origin, vertical fov 50 deg.
+ */
+#ifndef DMT_SCENEGEN_H
+#define DMT_SCENEGEN_H
+#include "dmt.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dmt_gen_options {
+    int32_t  subdivisions;   /* icosphere subdivisions (< 0: config default)   */
+    int32_t  random_tris;    /* random small triangles (< 0: config default)   */
+    int32_t  spheres;        /* analytic spheres (< 0: config default)         */
+    uint32_t width, height;  /* 0: config default                              */
+    uint32_t env_width, env_height; /* 0: config default                       */
+} dmt_gen_options;
+
+typedef struct dmt_generated_scene {
+    float*           prims;      uint32_t prim_count;
+    float*           lights;     uint32_t light_count;
+    float*           positions;  uint32_t vertex_count;
+    uint32_t*        indices;    uint32_t index_count;
+    float*           env_rgb;    uint32_t env_width;  uint32_t env_height;
+    float            env_intensity;
+    dmt_frame_params frame;      /* the config's camera and render settings  */
+} dmt_generated_scene;
+
+dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_options* options,
+                              dmt_generated_scene** out);
+void       dmt_scene_generated_free(dmt_generated_scene* scene);
+
+/* Camera basis of the reference Camera (src/renderer/Camera.h) looking from
+ * eye to target; fills pos_fovtan / fwd_aspect / right_xyz / up_xyz. */
+dmt_status dmt_camera_look_at(const float eye[3], const float target[3], float fov_y_deg,
+                              uint32_t width, uint32_t height, dmt_frame_params* frame);
+
+/* Host builders behind dmt_scene_create, exposed so their output can be
+ * checked without a device.
+ *
+ * Binned-SAH BVH over triangles, collapsed to 4-wide 128 B nodes (replaces
+ * TriangleBvh::Build, src/renderer/TriangleBvh.cpp:33-91).  Two-call pattern:
+ * pass null out_nodes/out_order to get the counts.  out_nodes receives
+ * node_count * 128 B, out_order the leaf-order triangle ids. */
+dmt_status dmt_build_bvh4(const float* positions, uint32_t vertex_count, const uint32_t* indices,
+                          uint32_t index_count, uint32_t sah_bins, uint32_t max_leaf, void* out_nodes,
+                          void* out_order, uint32_t* out_node_count, uint32_t* out_depth);
+/* BuildLightTree (src/renderer/LightTree.h:122-123): 16 floats per node,
+ * at most 2 * light_count - 1 nodes; returns the node count. */
+uint32_t dmt_build_light_tree(const float* lights, uint32_t light_count, float* out_nodes);
+/* BuildEnvCdf (src/renderer/HdrImage.h:54-58): marginal[H], conditional[W*H];
+ * returns the sin-weighted luminance total. */
+double dmt_build_env_cdf(const float* rgb, uint32_t width, uint32_t height, float* out_marginal,
+                         float* out_conditional);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMT_SCENEGEN_H */

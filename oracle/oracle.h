@@ -1,0 +1,105 @@
+/* CPU oracle -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A CPU restatement of the reference path tracer (shaders/PathTrace.slang
+ * main() with the off-path features disabled, plus the host builders in
+ * src/renderer/) used as the correctness checker of the HIP path.  Nothing in
+ * the product links, loads or calls this library; only tests/, the smoke
+ * entry and bench.py's CPU-baseline leg do.
+ *
+ * Data layouts are the reference wire formats:
+ *   prims   7 float4 (112 B) per analytic primitive (src/engine/Engine.cpp:4924-4994)
+ *   lights  5 float4 (80 B) per analytic light      (src/engine/Engine.cpp:5235-5300)
+ *   mesh    float[3V] positions, uint32[3T] indices (shaders/PathTrace.slang:44-45)
+ *   env     float[3*W*H] linear RGB, row-major, v = 0 at the top (theta = 0)
+ */
+#ifndef DMT_ORACLE_H
+#define DMT_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_scene_desc {
+    const float*    prims;      uint32_t prim_count;
+    const float*    lights;     uint32_t light_count;
+    const float*    positions;  uint32_t vertex_count;
+    const uint32_t* indices;    uint32_t index_count;
+    const float*    env_rgb;    uint32_t env_width;  uint32_t env_height;
+    float           env_intensity;
+    uint32_t        reserved;
+} orc_scene_desc;
+
+typedef struct orc_frame {
+    float    pos_fovtan[4];
+    float    fwd_aspect[4];
+    float    right_xyz[4];
+    float    up_xyz[4];
+    uint32_t width, height;
+    uint32_t frame_index;
+    uint32_t spp;
+    uint32_t max_bounces;
+    uint32_t reset_accum;
+    uint32_t mis_enabled;
+    uint32_t light_tree_enabled;
+    uint32_t refract_bounces;
+    float    firefly_clamp;
+    float    ema_alpha;
+    uint32_t reserved;
+} orc_frame;
+
+typedef struct orc_hit {
+    float    t;
+    float    normal[3];
+    uint32_t hit;        /* 0 miss, 1 analytic, 2 mesh */
+    uint32_t prim;       /* analytic: index in input order; mesh: triangle index */
+    uint32_t mat;
+    uint32_t user_id;    /* prims[v2.w] for analytic, 0 for mesh (PathTrace.slang:2659) */
+} orc_hit;
+
+typedef struct orc_counts {
+    uint64_t primary;    /* b == 0 extension queries */
+    uint64_t extension;  /* b >= 1 extension queries */
+    uint64_t shadow;     /* every traceScene step inside transmittance() */
+    uint64_t node_visits;
+    uint64_t prim_tests;
+} orc_counts;
+
+typedef struct orc_scene orc_scene;
+
+/* Build the oracle's own acceleration data (median BVHs, light tree, env CDF). */
+int  orc_scene_create(const orc_scene_desc* desc, orc_scene** out);
+void orc_scene_destroy(orc_scene* s);
+
+/* Render one frame: per pixel in the selected rows/columns, run the spp loop,
+ * average, sanitize and fold into accum (float4 per pixel, running mean).
+ * precision: 32 = float arithmetic (parity oracle), 64 = double (convergence).
+ * Rows y = row_begin, row_begin + row_step, ... < row_end; columns x = 0,
+ * col_step, ...  threads <= 0 means hardware_concurrency. */
+int orc_render(const orc_scene* s, const orc_frame* f, int precision, int threads,
+               uint32_t row_begin, uint32_t row_end, uint32_t row_step, uint32_t col_step,
+               float* accum_rgba, orc_counts* counts);
+
+/* Closest-hit query (traceScene) for n rays: rays are 8 floats each
+ * (ox, oy, oz, unused, dx, dy, dz, unused). */
+int orc_trace(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
+/* Same through an exhaustive linear scan (no BVH). */
+int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
+
+/* Host builders, exposed for tests. */
+double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H,
+                         float* out_marginal, float* out_conditional);
+/* nodes: 16 floats (64 B) per node, at most 2*light_count - 1 nodes. */
+uint32_t orc_build_light_tree(const float* lights, uint32_t light_count, float* out_nodes);
+/* Median BVH over triangles (TriangleBvh): nodes 8 x u32 each, perm[T].  Returns node count. */
+uint32_t orc_build_tri_bvh(const float* positions, uint32_t vertex_count,
+                           const uint32_t* indices, uint32_t index_count,
+                           uint32_t* out_nodes, uint32_t* out_perm);
+
+/* Math kernels exposed for the pinning tests (fp32 Cephes restatement). */
+void orc_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,172 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the
+checker / baseline.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class OrcSceneDesc(C.Structure):
+    _fields_ = [
+        ("prims", C.POINTER(C.c_float)), ("prim_count", C.c_uint32),
+        ("lights", C.POINTER(C.c_float)), ("light_count", C.c_uint32),
+        ("positions", C.POINTER(C.c_float)), ("vertex_count", C.c_uint32),
+        ("indices", C.POINTER(C.c_uint32)), ("index_count", C.c_uint32),
+        ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
+        ("env_intensity", C.c_float),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class OrcCounts(C.Structure):
+    _fields_ = [("primary", C.c_uint64), ("extension", C.c_uint64), ("shadow", C.c_uint64),
+                ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("normal", "<f4", (3,)), ("hit", "<u4"), ("prim", "<u4"),
+                      ("mat", "<u4"), ("user_id", "<u4")])
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, fp = C.c_void_p, C.c_uint32, C.POINTER(C.c_float)
+        L.orc_scene_create.argtypes = [C.POINTER(OrcSceneDesc), C.POINTER(vp)]
+        L.orc_scene_create.restype = C.c_int
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_destroy.restype = None
+        L.orc_render.argtypes = [vp, vp, C.c_int, C.c_int, u32, u32, u32, u32, fp, C.POINTER(OrcCounts)]
+        L.orc_render.restype = C.c_int
+        L.orc_trace.argtypes = [vp, fp, u32, vp]
+        L.orc_trace.restype = C.c_int
+        L.orc_trace_bruteforce.argtypes = [vp, fp, u32, vp]
+        L.orc_trace_bruteforce.restype = C.c_int
+        L.orc_build_env_cdf.argtypes = [fp, u32, u32, fp, fp]
+        L.orc_build_env_cdf.restype = C.c_double
+        L.orc_build_light_tree.argtypes = [fp, u32, fp]
+        L.orc_build_light_tree.restype = u32
+        L.orc_build_tri_bvh.argtypes = [fp, u32, C.POINTER(u32), u32, C.POINTER(u32), C.POINTER(u32)]
+        L.orc_build_tri_bvh.restype = u32
+        L.orc_math_f32.argtypes = [C.c_int, fp, fp, fp, u32]
+        L.orc_math_f32.restype = None
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    if a is None or a.size == 0:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class OracleScene:
+    """Oracle copy of a scene given in the reference wire formats."""
+
+    def __init__(self, prims, lights, positions, indices, env=None, env_intensity=1.0):
+        self._keep = [np.ascontiguousarray(prims, np.float32), np.ascontiguousarray(lights, np.float32),
+                      np.ascontiguousarray(positions, np.float32), np.ascontiguousarray(indices, np.uint32)]
+        d = OrcSceneDesc()
+        p, l, v, i = self._keep
+        d.prims, d.prim_count = _fp(p), p.shape[0] if p.ndim > 1 else p.size // 28
+        d.lights, d.light_count = _fp(l), l.shape[0] if l.ndim > 1 else l.size // 20
+        d.positions, d.vertex_count = _fp(v), v.size // 3
+        d.indices = i.ctypes.data_as(C.POINTER(C.c_uint32)) if i.size else None
+        d.index_count = i.size
+        if env is not None:
+            e = np.ascontiguousarray(env, np.float32)
+            self._keep.append(e)
+            d.env_rgb, d.env_height, d.env_width = _fp(e), e.shape[0], e.shape[1]
+        d.env_intensity = env_intensity
+        self._h = C.c_void_p()
+        rc = lib().orc_scene_create(C.byref(d), C.byref(self._h))
+        if rc != 0:
+            raise ValueError("oracle rejected the scene")
+
+    @classmethod
+    def from_scene_data(cls, sd):
+        return cls(sd.prims, sd.lights, sd.positions, sd.indices, sd.env, sd.env_intensity)
+
+    def render(self, frame, precision=32, threads=0, rows=None, col_step=1, accum=None):
+        """Render one frame; `frame` is any ctypes struct with the dmt_frame_params layout.
+        rows = (begin, end, step).  Returns (accum (H, W, 4), counts dict)."""
+        W, H = frame.width, frame.height
+        if threads <= 0:
+            threads = min(16, os.cpu_count() or 1)
+        if accum is None:
+            accum = np.zeros((H, W, 4), np.float32)
+        r0, r1, rs = rows if rows is not None else (0, H, 1)
+        cnt = OrcCounts()
+        rc = lib().orc_render(self._h, C.cast(C.byref(frame), C.c_void_p), precision, threads, r0, r1, rs, col_step,
+                              _fp(accum), C.byref(cnt))
+        if rc != 0:
+            raise ValueError("oracle render failed")
+        return accum, cnt.as_dict()
+
+    def trace(self, origins, dirs, brute=False):
+        n = origins.shape[0]
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = origins
+        rays[:, 4:7] = dirs
+        out = np.zeros(n, HIT_DTYPE)
+        fn = lib().orc_trace_bruteforce if brute else lib().orc_trace
+        fn(self._h, _fp(rays), n, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().orc_scene_destroy(self._h)
+        except Exception:
+            pass
+
+
+def env_cdf(env):
+    e = np.ascontiguousarray(env, np.float32)
+    H, W = e.shape[:2]
+    m = np.zeros(H, np.float32)
+    c = np.zeros(H * W, np.float32)
+    t = lib().orc_build_env_cdf(_fp(e), W, H, _fp(m), _fp(c))
+    return m, c.reshape(H, W), t
+
+
+def light_tree(lights):
+    L = np.ascontiguousarray(lights, np.float32)
+    n = L.shape[0]
+    out = np.zeros((max(2 * n - 1, 1), 16), np.float32)
+    k = lib().orc_build_light_tree(_fp(L), n, _fp(out))
+    return out[:k]
+
+
+def tri_bvh(positions, indices):
+    p = np.ascontiguousarray(positions, np.float32)
+    i = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    t = i.size // 3
+    nodes = np.zeros((max(2 * t, 1), 8), np.uint32)
+    perm = np.zeros(max(t, 1), np.uint32)
+    k = lib().orc_build_tri_bvh(_fp(p), p.size // 3, i.ctypes.data_as(C.POINTER(C.c_uint32)), i.size,
+                                nodes.ctypes.data_as(C.POINTER(C.c_uint32)), perm.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return nodes[:k], perm
+
+
+def math_f32(fn, a, b=None):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b if b is not None else np.zeros_like(a), np.float32)
+    out = np.zeros_like(a)
+    lib().orc_math_f32({"sin": 0, "cos": 1, "acos": 2, "atan2": 3}[fn], _fp(a), _fp(b), _fp(out), a.size)
+    return out

@@ -1,0 +1,179 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY.
+//
+// Scalar/vector arithmetic of the oracle.  Everything here exists so that the
+// oracle evaluates the reference shader's expressions in a fixed, documented
+// order (no FMA contraction: this file is compiled with -ffp-contract=off) and
+// with transcendental functions whose float results do not depend on the
+// platform libm.
+//
+// Float precision ("fp32 oracle"): sin/cos/acos/atan2 follow the published
+// single-precision Cephes algorithms (S. L. Moshier, Cephes Math Library,
+// sinf.c / asinf.c / atanf.c: Cody-Waite reduction by pi/4 with a three-part
+// constant, then the minimax polynomials of that library).  They are accurate
+// to ~1-2 ulp and are pinned against libm in tests/test_oracle_math.py.
+//
+// Double precision ("fp64 convergence oracle"): std:: libm.
+//
+// Vector ops follow HLSL/Slang semantics; dot() sums left to right
+// ((x*x + y*y) + z*z); lerp(a, b, t) = a + t*(b - a).
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+
+namespace orc {
+
+template <class R> struct V3 {
+    R x, y, z;
+};
+template <class R> inline V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline V3<R> operator/(V3<R> a, V3<R> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+// normalize(v) = v * (1 / sqrt(dot(v, v))): one correctly rounded sqrt, one
+// correctly rounded division and three multiplies.
+template <class R> inline V3<R> normalize(V3<R> a) {
+    R inv = R(1) / std::sqrt(dot(a, a));
+    return a * inv;
+}
+template <class R> inline V3<R> reflect(V3<R> i, V3<R> n) {
+    R d = dot(n, i);
+    return i - n * (R(2) * d);
+}
+// HLSL refract(): zero vector on total internal reflection.
+template <class R> inline V3<R> refract(V3<R> i, V3<R> n, R eta) {
+    R ni = dot(n, i);
+    R k = R(1) - (eta * eta) * (R(1) - ni * ni);
+    if (k < R(0)) return {R(0), R(0), R(0)};
+    return i * eta - n * (eta * ni + std::sqrt(k));
+}
+template <class R> inline R maxc(V3<R> a) { return std::max(a.x, std::max(a.y, a.z)); }
+template <class R> inline bool any_pos(V3<R> a) { return a.x > R(0) || a.y > R(0) || a.z > R(0); }
+template <class R> inline R clampv(R x, R lo, R hi) { return std::min(std::max(x, lo), hi); }
+template <class R> inline V3<R> lerp3(V3<R> a, V3<R> b, R t) { return a + (b - a) * t; }
+// quatRotate (PathTrace.slang:2028-2031): v + 2 * cross(u, cross(u, v) + w * v).
+template <class R> inline V3<R> quat_rotate(const R q[4], V3<R> v) {
+    V3<R> u{q[0], q[1], q[2]};
+    V3<R> c = cross(u, cross(u, v) + v * q[3]);
+    return v + c * R(2);
+}
+
+// ---- transcendental functions -------------------------------------------
+
+struct MathF {
+    // Cephes sinf/cosf: reduce |x| by multiples of pi/4 (DP1+DP2+DP3), select
+    // octant, evaluate the sine or cosine minimax polynomial.
+    static void sincos(float xin, float* s_out, float* c_out) {
+        const float DP1 = 0.78515625f;
+        const float DP2 = 2.4187564849853515625e-4f;
+        const float DP3 = 3.77489497744594108e-8f;
+        const float FOPI = 1.27323954473516f;
+        float x = xin;
+        int sign_s = 1;
+        if (x < 0.0f) { x = -x; sign_s = -1; }
+        int sign_c = 1;
+        unsigned j = (unsigned)(x * FOPI);
+        float y = (float)j;
+        if (j & 1u) { j += 1u; y += 1.0f; }
+        j &= 7u;
+        if (j > 3u) { j -= 4u; sign_s = -sign_s; sign_c = -sign_c; }
+        if (j > 1u) sign_c = -sign_c;
+        float r = ((x - y * DP1) - y * DP2) - y * DP3;
+        float z = r * r;
+        float ps = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * r + r;
+        float pc = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z
+                   - 0.5f * z + 1.0f;
+        float s, c;
+        if (j == 1u || j == 2u) { s = pc; c = ps; } else { s = ps; c = pc; }
+        *s_out = sign_s < 0 ? -s : s;
+        *c_out = sign_c < 0 ? -c : c;
+    }
+    static float sin(float x) { float s, c; sincos(x, &s, &c); return s; }
+    static float cos(float x) { float s, c; sincos(x, &s, &c); return c; }
+    // Cephes asinf kernel on [0, 0.5] after the half-angle identity.
+    static float asin_kernel(float z, float x) {
+        return ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z
+                + 1.6666752422e-1f) * z * x + x;
+    }
+    static float asin(float xin) {
+        float a = xin < 0.0f ? -xin : xin;
+        float r;
+        if (a > 0.5f) {
+            float z = 0.5f * (1.0f - a);
+            float x = std::sqrt(z);
+            r = asin_kernel(z, x);
+            r = 1.5707963267948966f - (r + r);
+        } else {
+            r = asin_kernel(a * a, a);
+        }
+        return xin < 0.0f ? -r : r;
+    }
+    static float acos(float x) {
+        if (x < -0.5f) return 3.14159265358979f - 2.0f * asin(std::sqrt(0.5f * (1.0f + x)));
+        if (x > 0.5f) return 2.0f * asin(std::sqrt(0.5f * (1.0f - x)));
+        return 1.5707963267948966f - asin(x);
+    }
+    static float atan(float xin) {
+        float x = xin < 0.0f ? -xin : xin;
+        float y;
+        if (x > 2.414213562373095f) { y = 1.5707963267948966f; x = -1.0f / x; }
+        else if (x > 0.4142135623730950f) { y = 0.7853981633974483f; x = (x - 1.0f) / (x + 1.0f); }
+        else { y = 0.0f; }
+        float z = x * x;
+        y = y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x + x);
+        return xin < 0.0f ? -y : y;
+    }
+    static float atan2(float y, float x) {
+        int code = 0;
+        if (x < 0.0f) code = 2;
+        if (y < 0.0f) code |= 1;
+        if (x == 0.0f) {
+            if (code & 1) return -1.5707963267948966f;
+            if (y == 0.0f) return 0.0f;
+            return 1.5707963267948966f;
+        }
+        if (y == 0.0f) return (code & 2) ? 3.14159265358979f : 0.0f;
+        float w = 0.0f;
+        if (code == 2) w = 3.14159265358979f;
+        else if (code == 3) w = -3.14159265358979f;
+        return w + atan(y / x);
+    }
+};
+
+struct MathD {
+    static double sin(double x) { return std::sin(x); }
+    static double cos(double x) { return std::cos(x); }
+    static double acos(double x) { return std::acos(x); }
+    static double atan2(double y, double x) { return std::atan2(y, x); }
+};
+
+template <class R> struct MathOf;
+template <> struct MathOf<float> { using M = MathF; };
+template <> struct MathOf<double> { using M = MathD; };
+
+// PCG hash + 24-bit uniform (PathTraceMath.slang:16-24).
+inline uint32_t pcg_hash(uint32_t x) {
+    x = x * 747796405u + 2891336453u;
+    x = ((x >> ((x >> 28u) + 4u)) ^ x) * 277803737u;
+    return (x >> 22u) ^ x;
+}
+template <class R> inline R randf(uint32_t& seed) {
+    seed = pcg_hash(seed);
+    return R(seed & 0x00FFFFFFu) / R(16777216.0);
+}
+
+inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+}  // namespace orc

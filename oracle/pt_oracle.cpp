@@ -1,0 +1,1150 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// Restates, on the CPU, the subset of the reference path tracer that the HIP
+// path implements:
+//   * RNG                 shaders/PathTraceMath.slang:16-24, seed PathTrace.slang:5457
+//   * camera ray          shaders/PathTrace.slang:5729-5778 (jitter 5733)
+//   * sphere / plane      shaders/PathTraceMath.slang:194-214, testAnalyticPrim PathTrace.slang:2066-2161
+//   * slab test           shaders/PathTrace.slang:2321-2329 (made conservative, see box_hit)
+//   * traceScene          shaders/PathTrace.slang:2331-2693 (planes, analytic BVH, tri BVH, MT)
+//   * Lambert branch      shaders/PathTrace.slang:6320-6682 (light NEE, env NEE + MIS, cosine bounce)
+//   * metal / dielectric  shaders/PathTrace.slang:6683-6767
+//   * light tree pick     shaders/PathTrace.slang:3929-4017, light sampling 4035-4179
+//   * env map             shaders/PathTrace.slang:3401-3431, 3619-3735
+//   * transmittance       shaders/PathTrace.slang:5089-5196
+//   * Russian roulette    shaders/PathTrace.slang:6908-6913
+//   * accumulation        shaders/PathTrace.slang:7707-7781
+//   * median BVH build    src/renderer/TriangleBvh.cpp:33-169, src/renderer/AnalyticBvh.cpp:369-486
+//   * light tree build    src/renderer/LightTree.cpp:63-455
+//   * env CDF build       src/renderer/HdrImage.cpp:169-219
+//
+// Deliberate, documented choices where the reference leaves freedom:
+//   * closest-hit ties resolve to (t, then lower primitive index), analytic
+//     primitives beat triangles at equal t (the reference's mesh pass uses a
+//     strict t < h.t, PathTrace.slang:2631);
+//   * the slab test is widened by 2*gamma(3) (PBRT's robust bound) so the BVH
+//     result equals the exhaustive scan; orc_trace_bruteforce pins that;
+//   * multi-draw constructors float2(randf, randf) draw left to right;
+//   * lerp(a, b, t) = a + t*(b - a).
+#include "oracle.h"
+#include "orc_math.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <limits>
+#include <thread>
+#include <vector>
+
+using namespace orc;
+
+namespace {
+
+constexpr uint32_t PRIM_SPHERE = 0u, PRIM_PLANE = 1u;
+constexpr uint32_t MAT_LAMBERT = 0u, MAT_METAL = 1u, MAT_DIELECTRIC = 2u;
+constexpr uint32_t LIGHT_POINT = 0u, LIGHT_SPOT = 1u, LIGHT_SPHERE = 2u, LIGHT_QUAD = 3u;
+constexpr uint32_t kAnalyticBvhThreshold = 16u;   // r_analytic_bvh_threshold (Engine.cpp:690)
+constexpr uint32_t kMaxLeaf = 4u;                 // TriangleBvh/AnalyticBvh kMaxLeafSize
+
+struct BNode {
+    float mn[3];
+    uint32_t left_first;
+    float mx[3];
+    uint32_t count;
+};
+
+struct WorkItem {
+    float mn[3], mx[3], c[3];
+    uint32_t id;
+};
+
+// Median split over centroids along the widest centroid axis, leaves <= 4
+// (TriangleBvh.cpp:93-169 / AnalyticBvh.cpp:400-486).
+struct MedianBvh {
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> perm;
+
+    void build(std::vector<WorkItem>& w) {
+        nodes.clear();
+        perm.clear();
+        if (w.empty()) return;
+        nodes.reserve(2 * w.size());
+        nodes.push_back({});
+        rec(w, 0, 0, (uint32_t)w.size());
+        for (auto& it : w) perm.push_back(it.id);
+    }
+    void rec(std::vector<WorkItem>& w, uint32_t ni, uint32_t first, uint32_t count) {
+        float bn[3] = {INFINITY, INFINITY, INFINITY}, bx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = 0; i < count; ++i)
+            for (int a = 0; a < 3; ++a) {
+                bn[a] = std::min(bn[a], w[first + i].mn[a]);
+                bx[a] = std::max(bx[a], w[first + i].mx[a]);
+            }
+        for (int a = 0; a < 3; ++a) { nodes[ni].mn[a] = bn[a]; nodes[ni].mx[a] = bx[a]; }
+        if (count <= kMaxLeaf) { nodes[ni].left_first = first; nodes[ni].count = count; return; }
+        float cn[3] = {INFINITY, INFINITY, INFINITY}, cx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = 0; i < count; ++i)
+            for (int a = 0; a < 3; ++a) {
+                float c = w[first + i].c[a];
+                if (c < cn[a]) cn[a] = c;
+                if (c > cx[a]) cx[a] = c;
+            }
+        int axis = 0;
+        float widest = cx[0] - cn[0];
+        for (int a = 1; a < 3; ++a)
+            if (cx[a] - cn[a] > widest) { widest = cx[a] - cn[a]; axis = a; }
+        if (widest > 0.0f) {
+            std::nth_element(w.begin() + first, w.begin() + first + count / 2, w.begin() + first + count,
+                             [axis](const WorkItem& a, const WorkItem& b) { return a.c[axis] < b.c[axis]; });
+        }
+        uint32_t lc = count / 2;
+        uint32_t li = (uint32_t)nodes.size();
+        nodes[ni].left_first = li;
+        nodes[ni].count = 0;
+        nodes.push_back({});
+        nodes.push_back({});
+        rec(w, li, first, lc);
+        rec(w, li + 1, first + lc, count - lc);
+    }
+};
+
+struct LightTreeNode {
+    float mn[3];
+    uint32_t left_first;
+    float mx[3];
+    uint32_t count;
+    float axis[3];
+    float cos_half;
+    float intensity;
+    float pad[3];
+};
+static_assert(sizeof(LightTreeNode) == 64, "64 B light tree node");
+
+struct LeafSummary {
+    float mn[3], mx[3], axis[3];
+    float cos_half, intensity;
+    uint32_t id;
+};
+
+// MakeLeafSummary (LightTree.cpp:63-147).
+// Engine::EnsureLightTreeUploaded rotates dir and u_vec by the light's
+// orientation quaternion before the build (Engine.cpp:5479-5508).
+void engine_rotate(const float* q, const float* v, float* o) {
+    const float qx = q[0], qy = q[1], qz = q[2], qw = q[3];
+    const float cx = qy * v[2] - qz * v[1] + qw * v[0];
+    const float cy = qz * v[0] - qx * v[2] + qw * v[1];
+    const float cz = qx * v[1] - qy * v[0] + qw * v[2];
+    o[0] = v[0] + 2.0f * (qy * cz - qz * cy);
+    o[1] = v[1] + 2.0f * (qz * cx - qx * cz);
+    o[2] = v[2] + 2.0f * (qx * cy - qy * cx);
+}
+
+LeafSummary make_leaf(const float* Lin, uint32_t id) {
+    float L[20];
+    std::memcpy(L, Lin, sizeof(L));
+    engine_rotate(Lin + 16, Lin + 8, L + 8);
+    if ((uint32_t)(Lin[7] + 0.5f) != LIGHT_SPOT) engine_rotate(Lin + 16, Lin + 12, L + 12);
+    LeafSummary s{};
+    s.id = id;
+    uint32_t type = (uint32_t)(L[7] + 0.5f);
+    const float* pos = L;
+    float radius = L[3];
+    const float* dir = L + 8;
+    const float* u = L + 12;
+    float v_half = L[15];
+    float cos_outer = L[11];
+    if (type == LIGHT_POINT || type == LIGHT_SPOT) {
+        for (int i = 0; i < 3; ++i) { s.mn[i] = pos[i] - 1e-3f; s.mx[i] = pos[i] + 1e-3f; }
+    } else if (type == LIGHT_SPHERE) {
+        float r = std::max(radius, 1e-3f);
+        for (int i = 0; i < 3; ++i) { s.mn[i] = pos[i] - r; s.mx[i] = pos[i] + r; }
+    } else {
+        float vh = std::fabs(v_half);
+        for (int i = 0; i < 3; ++i) {
+            float e = std::fabs(u[i]) + vh;
+            s.mn[i] = pos[i] - e;
+            s.mx[i] = pos[i] + e;
+        }
+    }
+    if (type == LIGHT_POINT || type == LIGHT_SPHERE) {
+        s.axis[0] = 0; s.axis[1] = 1; s.axis[2] = 0; s.cos_half = -1.0f;
+    } else if (type == LIGHT_SPOT) {
+        s.axis[0] = dir[0]; s.axis[1] = dir[1]; s.axis[2] = dir[2]; s.cos_half = cos_outer;
+    } else {
+        s.axis[0] = dir[0]; s.axis[1] = dir[1]; s.axis[2] = dir[2]; s.cos_half = 0.0f;
+    }
+    float lum = 0.2126f * L[4] + 0.7152f * L[5] + 0.0722f * L[6];
+    if (type == LIGHT_SPHERE) {
+        float r = std::max(radius, 1e-3f);
+        lum *= 4.0f * 3.14159265358979f * r * r;
+    } else if (type == LIGHT_QUAD) {
+        float ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        lum *= 4.0f * ul * std::fabs(v_half);
+    }
+    s.intensity = lum;
+    return s;
+}
+
+// CombineCones (LightTree.cpp:171-273).
+void combine_cones(const float* a, float ca, const float* b, float cb, float* out, float& oc) {
+    if (ca <= -0.999f || cb <= -0.999f) {
+        out[0] = a[0]; out[1] = a[1]; out[2] = a[2]; oc = -1.0f;
+        return;
+    }
+    float d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    d = std::min(1.0f, std::max(-1.0f, d));
+    float td = std::acos(d);
+    float ta = std::acos(std::clamp(ca, -1.0f, 1.0f));
+    float tb = std::acos(std::clamp(cb, -1.0f, 1.0f));
+    if (td + tb <= ta) { out[0] = a[0]; out[1] = a[1]; out[2] = a[2]; oc = ca; return; }
+    if (td + ta <= tb) { out[0] = b[0]; out[1] = b[1]; out[2] = b[2]; oc = cb; return; }
+    float tc = 0.5f * (ta + tb + td);
+    if (tc >= 3.14159265358979f) { out[0] = a[0]; out[1] = a[1]; out[2] = a[2]; oc = -1.0f; return; }
+    oc = std::cos(tc);
+    float rot = tc - ta;
+    float cr = std::cos(rot), sr = std::sin(rot);
+    float p[3] = {b[0] - d * a[0], b[1] - d * a[1], b[2] - d * a[2]};
+    float pl = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (pl > 1e-6f) {
+        float inv = 1.0f / pl;
+        p[0] *= inv; p[1] *= inv; p[2] *= inv;
+    } else {
+        out[0] = a[0]; out[1] = a[1]; out[2] = a[2];
+        if (d < 0.0f) oc = -1.0f;
+        return;
+    }
+    for (int i = 0; i < 3; ++i) out[i] = cr * a[i] + sr * p[i];
+    float al = std::sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+    if (al > 1e-6f) {
+        float inv = 1.0f / al;
+        out[0] *= inv; out[1] *= inv; out[2] *= inv;
+    }
+}
+
+// BuildRecursive (LightTree.cpp:326-420).
+void lt_rec(std::vector<LeafSummary>& leaves, std::vector<LightTreeNode>& out, uint32_t b, uint32_t e,
+            uint32_t dst) {
+    uint32_t n = e - b;
+    if (n == 1) {
+        LightTreeNode& nd = out[dst];
+        const LeafSummary& s = leaves[b];
+        for (int i = 0; i < 3; ++i) { nd.mn[i] = s.mn[i]; nd.mx[i] = s.mx[i]; nd.axis[i] = s.axis[i]; }
+        nd.cos_half = s.cos_half;
+        nd.intensity = s.intensity;
+        nd.left_first = s.id;
+        nd.count = 1u;
+        return;
+    }
+    float cn[3] = {INFINITY, INFINITY, INFINITY}, cx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = b; i < e; ++i)
+        for (int k = 0; k < 3; ++k) {
+            float c = 0.5f * (leaves[i].mn[k] + leaves[i].mx[k]);
+            cn[k] = std::min(cn[k], c);
+            cx[k] = std::max(cx[k], c);
+        }
+    int axis = 0;
+    float best = cx[0] - cn[0];
+    for (int k = 1; k < 3; ++k)
+        if (cx[k] - cn[k] > best) { best = cx[k] - cn[k]; axis = k; }
+    uint32_t mid = b + n / 2;
+    if (!(best < 1e-6f)) {
+        std::nth_element(leaves.begin() + b, leaves.begin() + mid, leaves.begin() + e,
+                         [axis](const LeafSummary& x, const LeafSummary& y) {
+                             float ca = 0.5f * (x.mn[axis] + x.mx[axis]);
+                             float cb = 0.5f * (y.mn[axis] + y.mx[axis]);
+                             return ca < cb;
+                         });
+    }
+    uint32_t li = (uint32_t)out.size();
+    out.emplace_back();
+    out.emplace_back();
+    lt_rec(leaves, out, b, mid, li);
+    lt_rec(leaves, out, mid, e, li + 1);
+    LightTreeNode& self = out[dst];
+    const LightTreeNode& l = out[li];
+    const LightTreeNode& r = out[li + 1];
+    for (int i = 0; i < 3; ++i) {
+        self.mn[i] = std::min(l.mn[i], r.mn[i]);
+        self.mx[i] = std::max(l.mx[i], r.mx[i]);
+    }
+    self.intensity = l.intensity + r.intensity;
+    combine_cones(l.axis, l.cos_half, r.axis, r.cos_half, self.axis, self.cos_half);
+    self.left_first = li;
+    self.count = 0u;
+}
+
+std::vector<LightTreeNode> build_light_tree(const float* lights, uint32_t n) {
+    std::vector<LightTreeNode> out;
+    if (n == 0) return out;
+    std::vector<LeafSummary> leaves;
+    for (uint32_t i = 0; i < n; ++i) leaves.push_back(make_leaf(lights + 20 * i, i));
+    out.reserve(2 * n);
+    out.emplace_back();
+    lt_rec(leaves, out, 0, n, 0);
+    return out;
+}
+
+// BuildEnvCdf (HdrImage.cpp:169-219).
+double build_env_cdf(const float* rgb, uint32_t W, uint32_t H, std::vector<float>& marg,
+                     std::vector<float>& cond) {
+    marg.assign(H, 0.0f);
+    cond.assign((size_t)W * H, 0.0f);
+    double total = 0.0;
+    for (uint32_t v = 0; v < H; ++v) {
+        const double st = std::sin(3.14159265358979323846 * (double(v) + 0.5) / double(H));
+        double row = 0.0;
+        for (uint32_t u = 0; u < W; ++u) {
+            size_t p = (size_t)v * W + u;
+            float lum = 0.2126f * rgb[3 * p] + 0.7152f * rgb[3 * p + 1] + 0.0722f * rgb[3 * p + 2];
+            row += double(lum) * st;
+            cond[p] = float(row);
+        }
+        double nrm = row > 0.0 ? 1.0 / row : 0.0;
+        for (uint32_t u = 0; u < W; ++u) {
+            size_t p = (size_t)v * W + u;
+            cond[p] = float(double(cond[p]) * nrm);
+        }
+        marg[v] = float(row);
+        total += row;
+    }
+    double nrm = total > 0.0 ? 1.0 / total : 0.0;
+    double pre = 0.0;
+    for (uint32_t v = 0; v < H; ++v) {
+        pre += marg[v];
+        marg[v] = float(pre * nrm);
+    }
+    if (H > 0 && total > 0.0) marg[H - 1] = 1.0f;
+    return total;
+}
+
+}  // namespace
+
+struct orc_scene {
+    std::vector<float> prims;
+    uint32_t prim_count = 0;
+    std::vector<uint32_t> planes;     // infinite prims, input order
+    std::vector<uint32_t> finite;     // finite prims, input order
+    MedianBvh abvh;                   // over finite prims (perm holds input indices)
+    bool use_abvh = false;
+    std::vector<float> pos;
+    std::vector<uint32_t> idx;
+    uint32_t tri_count = 0;
+    MedianBvh tbvh;
+    std::vector<float> lights;
+    uint32_t light_count = 0;
+    std::vector<LightTreeNode> ltree;
+    std::vector<float> env;          // RGB
+    uint32_t W = 0, H = 0;
+    bool env_present = false;
+    float env_intensity = 1.0f;
+    std::vector<float> marg, cond;
+    float env_total = 0.0f;
+};
+
+namespace {
+
+template <class R> struct Hit {
+    bool hit = false;
+    R t = R(1e30);
+    V3<R> n{R(0), R(1), R(0)};
+    uint32_t mat = MAT_LAMBERT;
+    V3<R> albedo{R(0.5), R(0.5), R(0.5)};
+    R rough = R(0), ior = R(1);
+    V3<R> emission{R(0), R(0), R(0)};
+    uint32_t user_id = 0;
+    uint32_t kind = 0;   // 0 miss 1 analytic 2 mesh
+    uint32_t prim = 0;
+};
+
+struct Ctr {
+    uint64_t primary = 0, extension = 0, shadow = 0, nodes = 0, tests = 0;
+};
+
+template <class R> inline V3<R> ld3(const float* p) { return V3<R>{R(p[0]), R(p[1]), R(p[2])}; }
+
+// Slab test (PathTrace.slang:2321-2329) widened by PBRT's 1 + 2*gamma(3) so
+// that no box containing the closest primitive hit is rejected by rounding.
+template <class R> inline bool box_hit(V3<R> ro, V3<R> inv, const float* mn, const float* mx, R tbest) {
+    R t0x = (R(mn[0]) - ro.x) * inv.x, t1x = (R(mx[0]) - ro.x) * inv.x;
+    R t0y = (R(mn[1]) - ro.y) * inv.y, t1y = (R(mx[1]) - ro.y) * inv.y;
+    R t0z = (R(mn[2]) - ro.z) * inv.z, t1z = (R(mx[2]) - ro.z) * inv.z;
+    R tmin = std::max(std::max(std::min(t0x, t1x), std::min(t0y, t1y)), std::min(t0z, t1z));
+    R tmax = std::min(std::min(std::max(t0x, t1x), std::max(t0y, t1y)), std::max(t0z, t1z));
+    const R g = R(1) + R(2) * R(3.0 * 5.9604644775390625e-8 / (1.0 - 3.0 * 5.9604644775390625e-8));
+    tmax = tmax * g;
+    tmin = tmin / g;
+    return tmax >= std::max(tmin, R(0)) && tmin <= tbest;
+}
+
+template <class R> inline bool better(R t, uint32_t id, const Hit<R>& h, uint32_t kind) {
+    if (t < h.t) return true;
+    return t == h.t && h.kind == kind && id < h.prim;
+}
+
+// testAnalyticPrim (PathTrace.slang:2066-2161) with shutter_t01 = 1 and
+// prev_pos == pos (no motion blur): the sphere centre is v0.xyz.
+template <class R> void test_prim(const orc_scene& S, uint32_t i, V3<R> ro, V3<R> rd, Hit<R>& h, Ctr& c) {
+    const float* p = &S.prims[(size_t)i * 28];
+    c.tests++;
+    uint32_t type = (uint32_t)p[8];
+    R t;
+    if (type == PRIM_SPHERE) {
+        V3<R> ce = ld3<R>(p);
+        R rad = R(p[3]);
+        V3<R> oc = ro - ce;
+        R b = dot(oc, rd);
+        R k = dot(oc, oc) - rad * rad;
+        R hh = b * b - k;
+        if (hh < R(0)) return;
+        hh = std::sqrt(hh);
+        R t0 = -b - hh, t1 = -b + hh;
+        t = (t0 > R(1e-3)) ? t0 : t1;
+        if (!(t > R(1e-3))) return;
+        if (!better(t, i, h, 1u)) return;
+        h.hit = true; h.t = t;
+        h.n = normalize((ro + rd * t) - ce);
+    } else if (type == PRIM_PLANE) {
+        R q[4] = {R(p[20]), R(p[21]), R(p[22]), R(p[23])};
+        V3<R> pn = quat_rotate(q, ld3<R>(p));
+        R denom = dot(rd, pn);
+        if (std::fabs(denom) < R(1e-4)) return;
+        t = -(dot(ro, pn) + R(p[3])) / denom;
+        if (!(t > R(1e-3))) return;
+        if (!better(t, i, h, 1u)) return;
+        h.hit = true; h.t = t;
+        h.n = pn;
+    } else {
+        return;
+    }
+    h.mat = (uint32_t)p[9];
+    h.albedo = ld3<R>(p + 4);
+    h.rough = R(p[7]);
+    h.ior = R(p[10]);
+    h.emission = ld3<R>(p + 16);
+    h.user_id = f2u(p[11]);
+    h.kind = 1u;
+    h.prim = i;
+}
+
+template <class R> struct TriT {
+    bool ok;
+    R t;
+};
+
+// Moller-Trumbore as in PathTrace.slang:2600-2625.
+template <class R> inline TriT<R> tri_test(const orc_scene& S, uint32_t tri, V3<R> ro, V3<R> rd) {
+    const uint32_t* ix = &S.idx[(size_t)tri * 3];
+    V3<R> v0 = ld3<R>(&S.pos[(size_t)ix[0] * 3]);
+    V3<R> v1 = ld3<R>(&S.pos[(size_t)ix[1] * 3]);
+    V3<R> v2 = ld3<R>(&S.pos[(size_t)ix[2] * 3]);
+    V3<R> e1 = v1 - v0, e2 = v2 - v0;
+    V3<R> pv = cross(rd, e2);
+    R det = dot(e1, pv);
+    if (std::fabs(det) < R(1e-8)) return {false, R(0)};
+    R inv_det = R(1) / det;
+    V3<R> tv = ro - v0;
+    R u = dot(tv, pv) * inv_det;
+    if (u < R(0) || u > R(1)) return {false, R(0)};
+    V3<R> qv = cross(tv, e1);
+    R v = dot(rd, qv) * inv_det;
+    if (v < R(0) || u + v > R(1)) return {false, R(0)};
+    R t = dot(e2, qv) * inv_det;
+    return {t > R(1e-3), t};
+}
+
+template <class R> void mesh_fill(const orc_scene& S, uint32_t tri, R t, Hit<R>& h) {
+    const uint32_t* ix = &S.idx[(size_t)tri * 3];
+    V3<R> v0 = ld3<R>(&S.pos[(size_t)ix[0] * 3]);
+    V3<R> v1 = ld3<R>(&S.pos[(size_t)ix[1] * 3]);
+    V3<R> v2 = ld3<R>(&S.pos[(size_t)ix[2] * 3]);
+    h.hit = true;
+    h.t = t;
+    h.n = normalize(cross(v1 - v0, v2 - v0));
+    h.mat = MAT_METAL;                               // PathTrace.slang:2655-2658
+    h.albedo = V3<R>{R(1.0f), R(0.85f), R(0.45f)};   // MESH_ALBEDO (PathTrace.slang:1428)
+    h.rough = R(0.05f);
+    h.ior = R(1);
+    h.emission = V3<R>{R(0), R(0), R(0)};
+    h.user_id = 0u;
+    h.kind = 2u;
+    h.prim = tri;
+}
+
+template <class R> inline V3<R> safe_inv(V3<R> rd) {
+    auto f = [](R x) { return std::fabs(x) > R(1e-30) ? x : R(1e-30); };
+    return V3<R>{R(1) / f(rd.x), R(1) / f(rd.y), R(1) / f(rd.z)};
+}
+
+template <class R> Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ctr& c, bool brute = false) {
+    Hit<R> h;
+    for (uint32_t i : S.planes) test_prim(S, i, ro, rd, h, c);
+    V3<R> inv = safe_inv(rd);
+    if (!S.use_abvh || brute) {
+        for (uint32_t i : S.finite) test_prim(S, i, ro, rd, h, c);
+    } else {
+        uint32_t stack[64];
+        int sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0) {
+            const BNode& nd = S.abvh.nodes[stack[--sp]];
+            c.nodes++;
+            if (!box_hit(ro, inv, nd.mn, nd.mx, h.t)) continue;
+            if (nd.count > 0) {
+                for (uint32_t k = 0; k < nd.count; ++k) test_prim(S, S.abvh.perm[nd.left_first + k], ro, rd, h, c);
+            } else {
+                stack[sp++] = nd.left_first;
+                stack[sp++] = nd.left_first + 1;
+            }
+        }
+    }
+    if (S.tri_count > 0) {
+        R t_mesh = h.t;
+        uint32_t best = 0;
+        bool found = false;
+        if (brute) {
+            for (uint32_t tri = 0; tri < S.tri_count; ++tri) {
+                c.tests++;
+                TriT<R> r = tri_test(S, tri, ro, rd);
+                if (r.ok && (r.t < t_mesh || (found && r.t == t_mesh && tri < best))) {
+                    found = true; t_mesh = r.t; best = tri;
+                }
+            }
+        } else {
+            uint32_t stack[128];
+            int sp = 0;
+            stack[sp++] = 0;
+            while (sp > 0) {
+                const BNode& nd = S.tbvh.nodes[stack[--sp]];
+                c.nodes++;
+                if (!box_hit(ro, inv, nd.mn, nd.mx, t_mesh)) continue;
+                if (nd.count > 0) {
+                    for (uint32_t k = 0; k < nd.count; ++k) {
+                        uint32_t tri = S.tbvh.perm[nd.left_first + k];
+                        c.tests++;
+                        TriT<R> r = tri_test(S, tri, ro, rd);
+                        if (r.ok && (r.t < t_mesh || (found && r.t == t_mesh && tri < best))) {
+                            found = true; t_mesh = r.t; best = tri;
+                        }
+                    }
+                } else {
+                    stack[sp++] = nd.left_first;
+                    stack[sp++] = nd.left_first + 1;
+                }
+            }
+        }
+        if (found) mesh_fill(S, best, t_mesh, h);
+    }
+    return h;
+}
+
+template <class R> struct PT {
+    using M = typename MathOf<R>::M;
+    const orc_scene& S;
+    const orc_frame& F;
+    Ctr& c;
+
+    static constexpr R PI = R(3.14159265358979);
+
+    V3<R> clamp_indirect(V3<R> v) const {
+        R lim = R(F.firefly_clamp);
+        if (lim <= R(0)) return v;
+        R m = maxc(v);
+        return (m > lim) ? v * (lim / m) : v;
+    }
+
+    V3<R> cos_sample(V3<R> n, uint32_t& seed) const {
+        R u1 = randf<R>(seed);
+        R u2 = randf<R>(seed);
+        R r = std::sqrt(u1);
+        R phi = R(6.28318530718) * u2;
+        R x = r * M::cos(phi);
+        R y = r * M::sin(phi);
+        R z = std::sqrt(std::max(R(0), R(1) - u1));
+        V3<R> ax = std::fabs(n.x) > R(0.9) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
+        V3<R> t = normalize(cross(n, ax));
+        V3<R> b = cross(n, t);
+        return normalize((t * x + b * y) + n * z);
+    }
+
+    V3<R> env_texel(uint32_t x, uint32_t y) const {
+        const float* p = &S.env[((size_t)y * S.W + x) * 3];
+        return V3<R>{R(p[0]), R(p[1]), R(p[2])};
+    }
+
+    // skyColor (PathTrace.slang:3373-3431).  Without an env map: the base
+    // gradient, with the sun and moon below the horizon (y < -0.1), where
+    // sunDisc/moonDisc return 0 (PathTrace.slang:3113, 3213).
+    V3<R> sky(V3<R> rd) const {
+        if (!S.env_present) {
+            R t = clampv(R(0.5) * (rd.y + R(1)), R(0), R(1));
+            return lerp3(V3<R>{R(0.5), R(0.55), R(0.62)}, V3<R>{R(1), R(1), R(1)}, t);
+        }
+        int W = (int)S.W, H = (int)S.H;
+        R u = M::atan2(rd.z, rd.x) * (R(1) / R(6.28318530718)) + R(0.5);
+        R v = M::acos(clampv(rd.y, R(-1), R(1))) * (R(1) / R(3.14159265359));
+        R fx = u * R(W) - R(0.5), fy = v * R(H) - R(0.5);
+        R flx = std::floor(fx), fly = std::floor(fy);
+        int ix = (int)flx, iy = (int)fly;
+        R ax = fx - R(ix), ay = fy - R(iy);
+        int x0 = (ix % W + W) % W;
+        int x1 = (x0 + 1) % W;
+        int y0 = std::min(std::max(iy, 0), H - 1);
+        int y1 = std::min(std::max(iy + 1, 0), H - 1);
+        V3<R> c00 = env_texel(x0, y0), c10 = env_texel(x1, y0);
+        V3<R> c01 = env_texel(x0, y1), c11 = env_texel(x1, y1);
+        V3<R> s = lerp3(lerp3(c00, c10, ax), lerp3(c01, c11, ax), ay);
+        return s * R(S.env_intensity);
+    }
+
+    struct EnvS {
+        V3<R> dir, color;
+        R pdf;
+    };
+    uint32_t cdf_search(const float* cdf, uint32_t count, R u) const {
+        uint32_t lo = 0, hi = count;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (R(cdf[mid]) < u) lo = mid + 1; else hi = mid;
+        }
+        return std::min(lo, count - 1);
+    }
+    // sampleEnvMap (PathTrace.slang:3638-3670).
+    EnvS sample_env(R xi0, R xi1) const {
+        EnvS s{{R(0), R(1), R(0)}, {R(0), R(0), R(0)}, R(0)};
+        if (!S.env_present || !(S.env_total > 0.0f)) return s;
+        uint32_t W = S.W, H = S.H;
+        uint32_t vi = cdf_search(S.marg.data(), H, xi1);
+        uint32_t ui = cdf_search(S.cond.data() + (size_t)vi * W, W, xi0);
+        R u = (R(ui) + R(0.5)) / R(W);
+        R v = (R(vi) + R(0.5)) / R(H);
+        R theta = v * PI;
+        R phi = u * R(6.28318530717958) - PI;
+        R st = M::sin(theta);
+        s.dir = V3<R>{st * M::cos(phi), M::cos(theta), st * M::sin(phi)};
+        V3<R> raw = env_texel(ui, vi);
+        s.color = raw * R(S.env_intensity);
+        R lum = (R(0.2126) * raw.x + R(0.7152) * raw.y) + R(0.0722) * raw.z;
+        if (st > R(1e-6)) s.pdf = (lum * R(W * H)) / (((R(2) * PI) * PI) * R(S.env_total));
+        return s;
+    }
+    // pdfEnvMapDir (PathTrace.slang:3710-3727).
+    R pdf_env(V3<R> d) const {
+        if (!S.env_present || !(S.env_total > 0.0f)) return R(0);
+        R ct = clampv(d.y, R(-1), R(1));
+        R th = M::acos(ct);
+        R st = M::sin(th);
+        if (st < R(1e-6)) return R(0);
+        R ph = M::atan2(d.z, d.x);
+        uint32_t W = S.W, H = S.H;
+        R u = (ph + PI) * (R(1) / R(6.28318530717958));
+        R v = th * (R(1) / PI);
+        uint32_t ui = std::min((uint32_t)(u * R(W)), W - 1u);
+        uint32_t vi = std::min((uint32_t)(v * R(H)), H - 1u);
+        V3<R> raw = env_texel(ui, vi);
+        R lum = (R(0.2126) * raw.x + R(0.7152) * raw.y) + R(0.0722) * raw.z;
+        return (lum * R(W * H)) / (((R(2) * PI) * PI) * R(S.env_total));
+    }
+    static R mis(R a, R b) { return a / std::max(a + b, R(1e-12)); }
+
+    // lightTreeImportance (PathTrace.slang:3929-3966).
+    R importance(const LightTreeNode& n, V3<R> p, V3<R> nf) const {
+        V3<R> mn = ld3<R>(n.mn), mx = ld3<R>(n.mx);
+        V3<R> ce = (mn + mx) * R(0.5);
+        V3<R> to = ce - p;
+        R d2 = std::max(dot(to, to), R(1e-12));
+        R d = std::sqrt(d2);
+        V3<R> dir = to / d;
+        V3<R> ext = (mx - mn) * R(0.5);
+        R rad2 = std::max(dot(ext, ext), R(1e-12));
+        R fall = R(1) / std::max(d2, rad2);
+        R orient = R(1);
+        if (R(n.cos_half) > R(-0.999)) {
+            R ce2 = -dot(ld3<R>(n.axis), dir);
+            R t = (ce2 - (R(n.cos_half) - R(0.2))) / R(0.2);
+            orient = clampv(t, R(0), R(1));
+        }
+        R recv = std::max(R(0), dot(nf, dir));
+        return std::max(((R(n.intensity) * fall) * orient) * recv, R(1e-8));
+    }
+
+    struct LS {
+        V3<R> dir, Le;
+        R pdf, dist;
+    };
+    // sampleAnalyticLight (PathTrace.slang:4035-4179).
+    LS sample_light(V3<R> p, V3<R> nf, uint32_t& seed) const {
+        LS s{{R(0), R(1), R(0)}, {R(0), R(0), R(0)}, R(0), R(1e30)};
+        uint32_t n = S.light_count;
+        if (n == 0) return s;
+        uint32_t pick;
+        R pick_pdf;
+        if (F.light_tree_enabled != 0u && !S.ltree.empty()) {
+            uint32_t idx = 0, lid = 0;
+            pick_pdf = R(1);
+            for (uint32_t d = 0; d < 32u; ++d) {
+                const LightTreeNode& nd = S.ltree[idx];
+                if (nd.count > 0u) { lid = nd.left_first; break; }
+                uint32_t l = nd.left_first, r = l + 1;
+                R il = importance(S.ltree[l], p, nf);
+                R ir = importance(S.ltree[r], p, nf);
+                R tot = il + ir;
+                R pl = (tot > R(1e-30)) ? (il / tot) : R(0.5);
+                R u = randf<R>(seed);
+                if (u < pl) { idx = l; pick_pdf *= pl; } else { idx = r; pick_pdf *= (R(1) - pl); }
+            }
+            pick = std::min(lid, n - 1);
+        } else {
+            R u = randf<R>(seed);
+            pick = std::min((uint32_t)(u * R(n)), n - 1);
+            pick_pdf = R(1) / R(n);
+        }
+        const float* L = &S.lights[(size_t)pick * 20];
+        uint32_t type = (uint32_t)(L[7] + 0.5f);
+        V3<R> pos = ld3<R>(L);
+        R radius = R(L[3]);
+        V3<R> I = ld3<R>(L + 4);
+        R q[4] = {R(L[16]), R(L[17]), R(L[18]), R(L[19])};
+        if (type == LIGHT_POINT) {
+            V3<R> tl = pos - p;
+            R d2 = std::max(dot(tl, tl), R(1e-12));
+            R d = std::sqrt(d2);
+            s.dir = tl / d; s.Le = I / d2; s.pdf = pick_pdf; s.dist = d;
+            return s;
+        }
+        if (type == LIGHT_SPOT) {
+            V3<R> tl = pos - p;
+            R d2 = std::max(dot(tl, tl), R(1e-12));
+            R d = std::sqrt(d2);
+            V3<R> tld = tl / d;
+            V3<R> axis = normalize(quat_rotate(q, ld3<R>(L + 8)));
+            R cd = dot(-tld, axis);
+            R ci = R(L[12]), co = R(L[11]);
+            R fo = R(0);
+            if (cd >= ci) fo = R(1);
+            else if (cd > co) fo = (cd - co) / std::max(ci - co, R(1e-6));
+            s.dir = tld; s.Le = I * (fo / d2); s.pdf = pick_pdf; s.dist = d;
+            return s;
+        }
+        if (type == LIGHT_SPHERE) {
+            R x0 = randf<R>(seed);
+            R x1 = randf<R>(seed);
+            R z = R(1) - R(2) * x0;
+            R r = std::sqrt(std::max(R(0), R(1) - z * z));
+            R phi = R(6.28318530717958) * x1;
+            V3<R> nl{r * M::cos(phi), r * M::sin(phi), z};
+            V3<R> lp = pos + nl * radius;
+            V3<R> tl = lp - p;
+            R d2 = std::max(dot(tl, tl), R(1e-12));
+            R d = std::sqrt(d2);
+            V3<R> wi = tl / d;
+            R cl = std::max(R(0), dot(nl, -wi));
+            if (cl <= R(0)) return s;
+            R area = ((R(4) * PI) * radius) * radius;
+            R pdf_sa = d2 / std::max(area * cl, R(1e-12));
+            s.dir = wi; s.Le = I; s.pdf = pick_pdf * pdf_sa; s.dist = d;
+            return s;
+        }
+        if (type == LIGHT_QUAD) {
+            R x0 = randf<R>(seed) * R(2) - R(1);
+            R x1 = randf<R>(seed) * R(2) - R(1);
+            V3<R> uv = quat_rotate(q, ld3<R>(L + 12));
+            R ul = length(uv);
+            if (ul < R(1e-6)) return s;
+            V3<R> nq = normalize(quat_rotate(q, ld3<R>(L + 8)));
+            V3<R> va = normalize(cross(nq, uv));
+            V3<R> lp = (pos + uv * x0) + va * (R(L[15]) * x1);
+            V3<R> tl = lp - p;
+            R d2 = std::max(dot(tl, tl), R(1e-12));
+            R d = std::sqrt(d2);
+            V3<R> wi = tl / d;
+            R cl = dot(nq, -wi);
+            if (cl <= R(0)) return s;
+            R area = (R(4) * ul) * R(L[15]);
+            R pdf_sa = d2 / std::max(area * cl, R(1e-12));
+            s.dir = wi; s.Le = I; s.pdf = pick_pdf * pdf_sa; s.dist = d;
+            return s;
+        }
+        return s;
+    }
+
+    // transmittance (PathTrace.slang:5089-5196), no water / clouds.
+    V3<R> transmittance(V3<R> o, V3<R> d, R t_max) const {
+        int kmax = (int)F.refract_bounces;
+        V3<R> t{R(1), R(1), R(1)};
+        for (int i = 0; i <= kmax; ++i) {
+            c.shadow++;
+            Hit<R> h = trace_scene(S, o, d, c);
+            if (!h.hit || h.t >= t_max) return t;
+            if (h.mat != MAT_DIELECTRIC) return V3<R>{R(0), R(0), R(0)};
+            if (i == kmax) return V3<R>{R(0), R(0), R(0)};
+            R ndi = dot(h.n, d);
+            V3<R> nf = (ndi < R(0)) ? h.n : -h.n;
+            R ia = (ndi < R(0)) ? R(1) : h.ior;
+            R ib = (ndi < R(0)) ? h.ior : R(1);
+            R eta = ia / ib;
+            R ci = std::fabs(dot(nf, -d));
+            R q = (ia - ib) / (ia + ib);
+            R r0 = q * q;
+            R m = R(1) - ci;
+            R fr = r0 + ((((R(1) - r0) * m) * m * m) * m) * m;
+            V3<R> rf = refract(d, nf, eta);
+            if (length(rf) < R(1e-3)) return V3<R>{R(0), R(0), R(0)};
+            t = t * (h.albedo * (R(1) - fr));
+            R adv = h.t + R(1e-3);
+            o = o + d * adv;
+            d = normalize(rf);
+            t_max = std::max(t_max - adv, R(0));
+        }
+        return V3<R>{R(0), R(0), R(0)};
+    }
+
+    // One path sample: PathTrace.slang:5803-6914 with the off-path features
+    // (water, clouds, DOF, spectral, PBR, clearcoat, SSS, ReSTIR) disabled.
+    V3<R> path(V3<R> ro, V3<R> rd, uint32_t& seed) const {
+        V3<R> T{R(1), R(1), R(1)}, L{R(0), R(0), R(0)};
+        bool skip_sky = false, prev_lambert = false;
+        R prev_pdf = R(0);
+        const bool mis_on = F.mis_enabled != 0u;
+        for (uint32_t b = 0; b < F.max_bounces; ++b) {
+            if (b == 0) c.primary++; else c.extension++;
+            Hit<R> h = trace_scene(S, ro, rd, c);
+            if (!h.hit) {
+                if (mis_on && prev_lambert) {
+                    // transmittance(ro, rd, 1e30) re-traces this same missed
+                    // ray and returns exactly 1: not re-traced, not counted.
+                    R pe = pdf_env(rd);
+                    R w = mis(prev_pdf, pe);
+                    L = L + clamp_indirect((T * sky(rd)) * w);
+                } else if (!skip_sky) {
+                    if (b == 0) L = L + T * sky(rd);
+                    else L = L + clamp_indirect(T * sky(rd));
+                }
+                break;
+            }
+            V3<R> hp = ro + rd * h.t;
+            R ndi = dot(h.n, rd);
+            V3<R> nf = (ndi < R(0)) ? h.n : -h.n;
+            if (any_pos(h.emission)) {
+                if (b == 0) L = L + T * h.emission;
+                else L = L + clamp_indirect(T * h.emission);
+            }
+            if (h.mat == MAT_LAMBERT) {
+                if (S.light_count > 0) {
+                    LS ls = sample_light(hp, nf, seed);
+                    R ndl = dot(nf, ls.dir);
+                    if (ls.pdf > R(0) && ndl > R(0)) {
+                        R tm = std::max(ls.dist - R(1e-3), R(0));
+                        V3<R> tr = transmittance(hp + nf * R(1e-3), ls.dir, tm);
+                        if (any_pos(tr)) {
+                            V3<R> brdf = h.albedo / PI;
+                            V3<R> cc = ((((T * ls.Le) * tr) * brdf) * ndl) / ls.pdf;
+                            L = L + ((b == 0) ? cc : clamp_indirect(cc));
+                        }
+                    }
+                }
+                if (S.env_present) {
+                    R x0 = randf<R>(seed);
+                    R x1 = randf<R>(seed);
+                    EnvS e = sample_env(x0, x1);
+                    R ndl = dot(nf, e.dir);
+                    if (e.pdf > R(0) && ndl > R(0)) {
+                        V3<R> tr = transmittance(hp + nf * R(1e-3), e.dir, R(1e30));
+                        if (any_pos(tr)) {
+                            V3<R> brdf = h.albedo / PI;
+                            R pb = ndl * (R(1) / PI);
+                            bool fire = mis_on && (b + 1u) < F.max_bounces;
+                            R w = fire ? mis(e.pdf, pb) : R(1);
+                            V3<R> cc = (((((T * e.color) * tr) * brdf) * ndl) * w) / e.pdf;
+                            L = L + clamp_indirect(cc);
+                        }
+                    }
+                    skip_sky = !mis_on;
+                } else {
+                    // No env map: sun and moon below the horizon, base sky gradient.
+                    R t = clampv(R(0.5) * (nf.y + R(1)), R(0), R(1));
+                    V3<R> a{R(0.5), R(0.55), R(0.62)}, one{R(1), R(1), R(1)};
+                    L = L + (T * h.albedo) * lerp3(a, one, t);
+                    skip_sky = true;
+                }
+                T = T * h.albedo;
+                ro = hp + nf * R(1e-3);
+                rd = cos_sample(nf, seed);
+                if (mis_on && S.env_present) {
+                    prev_pdf = std::max(R(0), dot(nf, rd)) * (R(1) / PI);
+                    prev_lambert = true;
+                } else {
+                    prev_lambert = false;
+                }
+            } else if (h.mat == MAT_METAL) {
+                V3<R> r = reflect(rd, nf);
+                V3<R> cs = cos_sample(nf, seed);
+                r = normalize(r + (cs - nf) * h.rough);
+                T = T * h.albedo;
+                ro = hp + nf * R(1e-3);
+                rd = r;
+                skip_sky = false;
+                prev_lambert = false;
+            } else if (h.mat == MAT_DIELECTRIC) {
+                R ia = (ndi < R(0)) ? R(1) : h.ior;
+                R ib = (ndi < R(0)) ? h.ior : R(1);
+                R eta = ia / ib;
+                R ci = std::fabs(dot(nf, -rd));
+                R r0 = (ia - ib) / (ia + ib);
+                r0 = r0 * r0;
+                R m = R(1) - ci;
+                R fr = r0 + ((((R(1) - r0) * m) * m * m) * m) * m;
+                V3<R> refl = reflect(rd, nf);
+                V3<R> refr = refract(rd, nf, eta);
+                bool tir = length(refr) < R(1e-3);
+                bool take_refl = tir;
+                if (!tir) take_refl = randf<R>(seed) < fr;
+                if (take_refl) {
+                    rd = refl;
+                    ro = hp + nf * R(1e-3);
+                    skip_sky = false;
+                    prev_lambert = false;
+                } else {
+                    rd = normalize(refr);
+                    ro = hp - nf * R(1e-3);
+                }
+            }
+            if (b >= 2u) {
+                R p = clampv(maxc(T), R(0.05), R(1));
+                if (randf<R>(seed) > p) break;
+                T = T / p;
+            }
+        }
+        return L;
+    }
+
+    void pixel(uint32_t x, uint32_t y, float* accum) const {
+        uint32_t seed = pcg_hash(x * 1973u + y * 9277u + F.frame_index * 26699u);
+        V3<R> fr{R(0), R(0), R(0)};
+        const R w = R(F.width), hgt = R(F.height);
+        V3<R> pos{R(F.pos_fovtan[0]), R(F.pos_fovtan[1]), R(F.pos_fovtan[2])};
+        R fovtan = R(F.pos_fovtan[3]), aspect = R(F.fwd_aspect[3]);
+        V3<R> fwd{R(F.fwd_aspect[0]), R(F.fwd_aspect[1]), R(F.fwd_aspect[2])};
+        V3<R> rt{R(F.right_xyz[0]), R(F.right_xyz[1]), R(F.right_xyz[2])};
+        V3<R> up{R(F.up_xyz[0]), R(F.up_xyz[1]), R(F.up_xyz[2])};
+        uint32_t spp = std::max(1u, F.spp);
+        for (uint32_t s = 0; s < spp; ++s) {
+            R jx = randf<R>(seed);
+            R jy = randf<R>(seed);
+            jx = jx - R(0.5);
+            jy = jy - R(0.5);
+            R ux = (((R(x) + R(0.5)) + jx) / w) * R(2) - R(1);
+            R uy = (((R(y) + R(0.5)) + jy) / hgt) * R(2) - R(1);
+            uy = -uy;
+            V3<R> rd = normalize((fwd + rt * ((ux * aspect) * fovtan)) + up * (uy * fovtan));
+            fr = fr + path(pos, rd, seed);
+        }
+        fr = fr / R(spp);
+        float o[3] = {float(fr.x), float(fr.y), float(fr.z)};
+        for (int k = 0; k < 3; ++k) {
+            if (std::isnan(o[k]) || std::isinf(o[k])) o[k] = 0.0f;
+            else o[k] = std::max(o[k], 0.0f);
+        }
+        float* a = accum + ((size_t)y * F.width + x) * 4;
+        float prev[4] = {0, 0, 0, 0};
+        if (F.reset_accum == 0u) {
+            for (int k = 0; k < 4; ++k) prev[k] = a[k];
+            bool bad = false;
+            for (int k = 0; k < 4; ++k) bad = bad || std::isnan(prev[k]) || std::isinf(prev[k]);
+            if (bad) prev[0] = prev[1] = prev[2] = prev[3] = 0.0f;
+        }
+        float ema = std::min(std::max(F.ema_alpha, 0.0f), 0.999f);
+        if (prev[3] < 0.5f || ema <= 0.0f) {
+            float nn = prev[3] + 1.0f;
+            for (int k = 0; k < 3; ++k) a[k] = (prev[k] * prev[3] + o[k]) / nn;
+            a[3] = nn;
+        } else {
+            for (int k = 0; k < 3; ++k) a[k] = prev[k] * ema + o[k] * (1.0f - ema);
+            a[3] = std::min(prev[3] + 1.0f, 1.0f / std::max(1.0f - ema, 1e-3f));
+        }
+    }
+};
+
+template <class R>
+void render_rows(const orc_scene& S, const orc_frame& F, int threads, uint32_t r0, uint32_t r1, uint32_t rs,
+                 uint32_t cs, float* accum, Ctr& total) {
+    std::atomic<uint32_t> next{0};
+    uint32_t nrows = (r1 > r0) ? (r1 - r0 + rs - 1) / rs : 0;
+    std::vector<Ctr> ctrs((size_t)threads);
+    auto work = [&](int tid) {
+        Ctr& c = ctrs[(size_t)tid];
+        PT<R> pt{S, F, c};
+        for (;;) {
+            uint32_t k = next.fetch_add(1);
+            if (k >= nrows) break;
+            uint32_t y = r0 + k * rs;
+            for (uint32_t x = 0; x < F.width; x += cs) pt.pixel(x, y, accum);
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 1; t < threads; ++t) ts.emplace_back(work, t);
+    work(0);
+    for (auto& t : ts) t.join();
+    for (auto& c : ctrs) {
+        total.primary += c.primary; total.extension += c.extension; total.shadow += c.shadow;
+        total.nodes += c.nodes; total.tests += c.tests;
+    }
+}
+
+void fill_hit(orc_hit& o, const Hit<float>& h) {
+    o.t = h.hit ? h.t : 0.0f;
+    o.normal[0] = h.n.x; o.normal[1] = h.n.y; o.normal[2] = h.n.z;
+    o.hit = h.hit ? h.kind : 0u;
+    o.prim = h.hit ? h.prim : 0u;
+    o.mat = h.hit ? h.mat : 0u;
+    o.user_id = h.hit ? h.user_id : 0u;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_scene_create(const orc_scene_desc* d, orc_scene** out) {
+    if (!d || !out) return -1;
+    auto* S = new orc_scene();
+    S->prim_count = d->prim_count;
+    if (d->prim_count) S->prims.assign(d->prims, d->prims + (size_t)d->prim_count * 28);
+    std::vector<WorkItem> fw;
+    for (uint32_t i = 0; i < d->prim_count; ++i) {
+        const float* p = &S->prims[(size_t)i * 28];
+        uint32_t type = (uint32_t)p[8];
+        if (type == PRIM_PLANE) { S->planes.push_back(i); continue; }
+        S->finite.push_back(i);
+        WorkItem w{};
+        for (int a = 0; a < 3; ++a) {
+            w.c[a] = p[a];
+            w.mn[a] = p[a] - p[3];
+            w.mx[a] = p[a] + p[3];
+        }
+        w.id = i;
+        fw.push_back(w);
+    }
+    S->use_abvh = S->finite.size() >= kAnalyticBvhThreshold;
+    if (S->use_abvh) S->abvh.build(fw);
+    if (d->index_count >= 3 && d->positions && d->indices) {
+        S->pos.assign(d->positions, d->positions + (size_t)d->vertex_count * 3);
+        S->idx.assign(d->indices, d->indices + (d->index_count / 3) * 3);
+        S->tri_count = d->index_count / 3;
+        for (uint32_t i = 0; i < S->idx.size(); ++i)
+            if (S->idx[i] >= d->vertex_count) { delete S; return -1; }
+        std::vector<WorkItem> tw(S->tri_count);
+        for (uint32_t t = 0; t < S->tri_count; ++t) {
+            const float* v0 = &S->pos[(size_t)S->idx[3 * t] * 3];
+            const float* v1 = &S->pos[(size_t)S->idx[3 * t + 1] * 3];
+            const float* v2 = &S->pos[(size_t)S->idx[3 * t + 2] * 3];
+            WorkItem& w = tw[t];
+            for (int a = 0; a < 3; ++a) {
+                w.mn[a] = std::min(v0[a], std::min(v1[a], v2[a]));
+                w.mx[a] = std::max(v0[a], std::max(v1[a], v2[a]));
+                w.c[a] = (v0[a] + v1[a] + v2[a]) * (1.0f / 3.0f);
+            }
+            w.id = t;
+        }
+        S->tbvh.build(tw);
+    }
+    S->light_count = d->light_count;
+    if (d->light_count) {
+        S->lights.assign(d->lights, d->lights + (size_t)d->light_count * 20);
+        S->ltree = build_light_tree(S->lights.data(), d->light_count);
+    }
+    if (d->env_rgb && d->env_width && d->env_height) {
+        S->env_present = true;
+        S->W = d->env_width;
+        S->H = d->env_height;
+        S->env.assign(d->env_rgb, d->env_rgb + (size_t)S->W * S->H * 3);
+        S->env_intensity = d->env_intensity;
+        S->env_total = float(build_env_cdf(S->env.data(), S->W, S->H, S->marg, S->cond));
+    }
+    *out = S;
+    return 0;
+}
+
+void orc_scene_destroy(orc_scene* s) { delete s; }
+
+int orc_render(const orc_scene* s, const orc_frame* f, int precision, int threads, uint32_t row_begin,
+               uint32_t row_end, uint32_t row_step, uint32_t col_step, float* accum, orc_counts* counts) {
+    if (!s || !f || !accum || row_step == 0 || col_step == 0) return -1;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    row_end = std::min(row_end, f->height);
+    Ctr c;
+    if (precision == 64) render_rows<double>(*s, *f, threads, row_begin, row_end, row_step, col_step, accum, c);
+    else render_rows<float>(*s, *f, threads, row_begin, row_end, row_step, col_step, accum, c);
+    if (counts) {
+        counts->primary = c.primary; counts->extension = c.extension; counts->shadow = c.shadow;
+        counts->node_visits = c.nodes; counts->prim_tests = c.tests;
+    }
+    return 0;
+}
+
+static int trace_impl(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out, bool brute) {
+    if (!s || (!rays && n) || (!out && n)) return -1;
+    Ctr c;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        Hit<float> h = trace_scene<float>(*s, V3<float>{r[0], r[1], r[2]}, V3<float>{r[4], r[5], r[6]}, c, brute);
+        fill_hit(out[i], h);
+    }
+    return 0;
+}
+int orc_trace(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out) {
+    return trace_impl(s, rays, n, out, false);
+}
+int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out) {
+    return trace_impl(s, rays, n, out, true);
+}
+
+double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H, float* om, float* oc) {
+    std::vector<float> m, c;
+    double t = build_env_cdf(rgb, W, H, m, c);
+    std::copy(m.begin(), m.end(), om);
+    std::copy(c.begin(), c.end(), oc);
+    return t;
+}
+
+uint32_t orc_build_light_tree(const float* lights, uint32_t n, float* out_nodes) {
+    auto t = build_light_tree(lights, n);
+    if (!t.empty()) std::memcpy(out_nodes, t.data(), t.size() * sizeof(LightTreeNode));
+    return (uint32_t)t.size();
+}
+
+uint32_t orc_build_tri_bvh(const float* pos, uint32_t vc, const uint32_t* idx, uint32_t ic, uint32_t* on,
+                           uint32_t* op) {
+    std::vector<WorkItem> tw;
+    uint32_t tc = ic / 3;
+    for (uint32_t t = 0; t < tc; ++t) {
+        uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
+        if (i0 >= vc || i1 >= vc || i2 >= vc) continue;   // TriangleBvh.cpp:49
+        const float *v0 = pos + 3 * (size_t)i0, *v1 = pos + 3 * (size_t)i1, *v2 = pos + 3 * (size_t)i2;
+        WorkItem w{};
+        for (int a = 0; a < 3; ++a) {
+            w.mn[a] = std::min(v0[a], std::min(v1[a], v2[a]));
+            w.mx[a] = std::max(v0[a], std::max(v1[a], v2[a]));
+            w.c[a] = (v0[a] + v1[a] + v2[a]) * (1.0f / 3.0f);
+        }
+        w.id = t;
+        tw.push_back(w);
+    }
+    MedianBvh b;
+    b.build(tw);
+    if (!b.nodes.empty()) std::memcpy(on, b.nodes.data(), b.nodes.size() * sizeof(BNode));
+    if (!b.perm.empty()) std::memcpy(op, b.perm.data(), b.perm.size() * 4);
+    return (uint32_t)b.nodes.size();
+}
+
+void orc_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = MathF::sin(a[i]); break;
+            case 1: out[i] = MathF::cos(a[i]); break;
+            case 2: out[i] = MathF::acos(a[i]); break;
+            case 3: out[i] = MathF::atan2(a[i], b[i]); break;
+            default: out[i] = 0.0f;
+        }
+    }
+}
+
+}  // extern "C"
