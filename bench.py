@@ -42,7 +42,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows-step", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -114,17 +114,31 @@ def main():
         ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         threads = max(1, ncpu - 4) if ncpu > 4 else ncpu   # SoftwareTracer.cpp:867-868 policy
         oscene = orc.OracleScene.from_scene_data(sd)
-        step = args.cpu_rows_step
-        f = frame.copy()
-        f.frame_index = 0
-        f.reset_accum = 1
-        t0 = time.perf_counter()
-        _, cnt = oscene.render(f, 32, threads, rows=(0, f.height, step))
-        cdt = time.perf_counter() - t0
-        crays = cnt["primary"] + cnt["extension"] + cnt["shadow"]
+
+        def cpu_run(nthreads):
+            # whole frames (frame_index 0, 1, ...) until at least cpu_seconds of work
+            crays, t0 = 0, time.perf_counter()
+            k = 0
+            while True:
+                f = frame.copy()
+                f.frame_index = k
+                f.reset_accum = 1
+                _, cnt = oscene.render(f, 32, nthreads)
+                crays += cnt["primary"] + cnt["extension"] + cnt["shadow"]
+                k += 1
+                el = time.perf_counter() - t0
+                if el >= args.cpu_seconds or k >= 200:
+                    return crays, el, k
+
+        crays, cdt, nfr = cpu_run(threads)
         cpu = {"value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"every {step}th row of one 1920x1080 frame ({crays} rays, {cdt:.1f} s), "
-                         f"fp32 CPU oracle with the reference semantics, {threads} threads of {ncpu}"}
+               "sample": f"{nfr} whole 1920x1080 frames ({crays} rays, {cdt:.1f} s) of the fp32 CPU oracle "
+                         f"(reference semantics, same scene, median BVH), {threads} threads of {ncpu} "
+                         f"(hw - 4, SoftwareTracer.cpp:867-868)"}
+        if threads != ncpu:
+            crays2, cdt2, nfr2 = cpu_run(ncpu)
+            cpu["all_cores"] = {"value": round(crays2 / cdt2 / 1e6, 3), "cores": ncpu,
+                                "sample": f"{nfr2} frames, {cdt2:.1f} s"}
         log(f"[bench] cpu baseline {cpu}")
 
     line = {

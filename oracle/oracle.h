@@ -96,6 +96,13 @@ uint32_t orc_build_tri_bvh(const float* positions, uint32_t vertex_count,
                            const uint32_t* indices, uint32_t index_count,
                            uint32_t* out_nodes, uint32_t* out_perm);
 
+/* Probes of the shading helpers (fp32), for the pinning tests.
+ * env sample: xi 2 floats each -> 8 floats (dir.xyz, colour.rgb, pdf, 0). */
+int orc_env_sample(const orc_scene* s, const float* xi, uint32_t n, float* out);
+int orc_env_pdf(const orc_scene* s, const float* dirs, uint32_t n, float* out);
+/* Exhaustive light-tree pick PMF at point p with facing normal nf. */
+int orc_light_pmf(const orc_scene* s, const float* p, const float* nf, float* pmf);
+
 /* Math kernels exposed for the pinning tests (fp32 Cephes restatement). */
 void orc_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n);
 

@@ -63,6 +63,12 @@ def lib():
         L.orc_build_light_tree.restype = u32
         L.orc_build_tri_bvh.argtypes = [fp, u32, C.POINTER(u32), u32, C.POINTER(u32), C.POINTER(u32)]
         L.orc_build_tri_bvh.restype = u32
+        L.orc_env_sample.argtypes = [vp, fp, u32, fp]
+        L.orc_env_sample.restype = C.c_int
+        L.orc_env_pdf.argtypes = [vp, fp, u32, fp]
+        L.orc_env_pdf.restype = C.c_int
+        L.orc_light_pmf.argtypes = [vp, fp, fp, fp]
+        L.orc_light_pmf.restype = C.c_int
         L.orc_math_f32.argtypes = [C.c_int, fp, fp, fp, u32]
         L.orc_math_f32.restype = None
         _lib = L
@@ -126,6 +132,27 @@ class OracleScene:
         out = np.zeros(n, HIT_DTYPE)
         fn = lib().orc_trace_bruteforce if brute else lib().orc_trace
         fn(self._h, _fp(rays), n, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def env_sample(self, xi):
+        xi = np.ascontiguousarray(xi, np.float32)
+        n = xi.shape[0]
+        out = np.zeros((n, 8), np.float32)
+        lib().orc_env_sample(self._h, _fp(xi), n, _fp(out))
+        return out[:, 0:3], out[:, 3:6], out[:, 6]
+
+    def env_pdf(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros(d.shape[0], np.float32)
+        lib().orc_env_pdf(self._h, _fp(d), d.shape[0], _fp(out))
+        return out
+
+    def light_pmf(self, p, nf, n_lights):
+        pv = np.ascontiguousarray(p, np.float32)
+        nv = np.ascontiguousarray(nf, np.float32)
+        out = np.zeros(n_lights, np.float32)
+        if lib().orc_light_pmf(self._h, _fp(pv), _fp(nv), _fp(out)) != 0:
+            raise ValueError("no light tree")
         return out
 
     def __del__(self):

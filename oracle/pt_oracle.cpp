@@ -1135,6 +1135,55 @@ uint32_t orc_build_tri_bvh(const float* pos, uint32_t vc, const uint32_t* idx, u
     return (uint32_t)b.nodes.size();
 }
 
+int orc_env_sample(const orc_scene* s, const float* xi, uint32_t n, float* out) {
+    if (!s) return -1;
+    orc_frame f{};
+    Ctr c;
+    PT<float> pt{*s, f, c};
+    for (uint32_t i = 0; i < n; ++i) {
+        auto e = pt.sample_env(xi[2 * i], xi[2 * i + 1]);
+        float* o = out + 8 * (size_t)i;
+        o[0] = e.dir.x; o[1] = e.dir.y; o[2] = e.dir.z;
+        o[3] = e.color.x; o[4] = e.color.y; o[5] = e.color.z;
+        o[6] = e.pdf; o[7] = 0.0f;
+    }
+    return 0;
+}
+
+int orc_env_pdf(const orc_scene* s, const float* dirs, uint32_t n, float* out) {
+    if (!s) return -1;
+    orc_frame f{};
+    Ctr c;
+    PT<float> pt{*s, f, c};
+    for (uint32_t i = 0; i < n; ++i) out[i] = pt.pdf_env(V3<float>{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]});
+    return 0;
+}
+
+// Exhaustive pick PMF of the light-tree descent at (p, nf): the product of
+// the branch probabilities of pickLightFromTree along each root-to-leaf path.
+int orc_light_pmf(const orc_scene* s, const float* p, const float* nf, float* pmf) {
+    if (!s || s->ltree.empty()) return -1;
+    orc_frame f{};
+    Ctr c;
+    PT<float> pt{*s, f, c};
+    V3<float> P{p[0], p[1], p[2]}, N{nf[0], nf[1], nf[2]};
+    for (uint32_t i = 0; i < s->light_count; ++i) pmf[i] = 0.0f;
+    std::vector<std::pair<uint32_t, double>> st{{0u, 1.0}};
+    while (!st.empty()) {
+        auto [idx, pr] = st.back();
+        st.pop_back();
+        const LightTreeNode& nd = s->ltree[idx];
+        if (nd.count > 0u) { pmf[nd.left_first] += float(pr); continue; }
+        float il = pt.importance(s->ltree[nd.left_first], P, N);
+        float ir = pt.importance(s->ltree[nd.left_first + 1], P, N);
+        float tot = il + ir;
+        float pl = (tot > 1e-30f) ? (il / tot) : 0.5f;
+        st.push_back({nd.left_first, pr * pl});
+        st.push_back({nd.left_first + 1, pr * (1.0f - pl)});
+    }
+    return 0;
+}
+
 void orc_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) {
         switch (fn) {

@@ -1,0 +1,103 @@
+"""Helpers that build small scenes in the reference wire formats for tests."""
+import math
+
+import numpy as np
+
+import dmt
+
+MAT_LAMBERT, MAT_METAL, MAT_DIELECTRIC = 0, 1, 2
+
+
+def prim(kind, v0, radius_or_d, albedo=(0.5, 0.5, 0.5), rough=0.0, mat=MAT_LAMBERT, ior=1.0, user_id=0,
+         emission=(0, 0, 0)):
+    p = np.zeros(28, np.float32)
+    p[0:3] = v0
+    p[3] = radius_or_d
+    p[4:7] = albedo
+    p[7] = rough
+    p[8] = kind
+    p[9] = mat
+    p[10] = ior
+    p[11] = np.array([user_id], np.uint32).view(np.float32)[0]
+    p[12:15] = v0
+    p[16:19] = emission
+    p[23] = 1.0
+    p[24:28] = np.array([0xFFFFFFFF] * 4, np.uint32).view(np.float32)
+    return p
+
+
+def sphere(c, r, **kw):
+    return prim(0, c, r, **kw)
+
+
+def plane(n, d, **kw):
+    return prim(1, n, d, **kw)
+
+
+def light(type_, pos, intensity, radius=0.0, dir=(0, -1, 0), cos_outer=0.0, u=(1, 0, 0), v_half=0.0,
+          cos_inner=None):
+    L = np.zeros(20, np.float32)
+    L[0:3] = pos
+    L[3] = radius
+    L[4:7] = intensity
+    L[7] = type_
+    L[8:11] = dir
+    L[11] = cos_outer
+    L[12:15] = u
+    if cos_inner is not None:
+        L[12] = cos_inner
+    L[15] = v_half
+    L[19] = 1.0
+    return L
+
+
+def point_light(pos, I):
+    return light(0, pos, I)
+
+
+def quad_light(pos, n, u, v_half, I):
+    return light(3, pos, I, dir=n, u=u, v_half=v_half)
+
+
+def sphere_light(pos, r, I):
+    return light(2, pos, I, radius=r)
+
+
+def frame(eye, target, fov, w, h, spp=1, bounces=4, frame_index=0):
+    f = dmt.look_at(eye, target, fov, w, h)
+    f.spp = spp
+    f.max_bounces = bounces
+    f.frame_index = frame_index
+    f.reset_accum = 1
+    f.mis_enabled = 1
+    f.light_tree_enabled = 1
+    f.refract_bounces = 4
+    f.firefly_clamp = 10.0
+    f.ema_alpha = 0.0
+    return f
+
+
+def scene(prims=(), lights=(), positions=None, indices=None, env=None, env_intensity=1.0):
+    return dmt.SceneData(
+        prims=np.array(prims, np.float32).reshape(-1, 28),
+        lights=np.array(lights, np.float32).reshape(-1, 20),
+        positions=np.zeros((0, 3), np.float32) if positions is None else np.asarray(positions, np.float32),
+        indices=np.zeros((0, 3), np.uint32) if indices is None else np.asarray(indices, np.uint32),
+        env=env, env_intensity=env_intensity)
+
+
+def black_env(w=8, h=4):
+    return np.zeros((h, w, 3), np.float32)
+
+
+def polygon_irradiance(x, n, verts):
+    """Irradiance at x (normal n) from a unit-radiance Lambertian polygon
+    (Lambert's polygon formula): E = 1/2 sum_i angle_i * n . unit(r_i x r_{i+1})."""
+    r = [np.asarray(v, np.float64) - np.asarray(x, np.float64) for v in verts]
+    r = [v / np.linalg.norm(v) for v in r]
+    e = 0.0
+    for i in range(len(r)):
+        a, b = r[i], r[(i + 1) % len(r)]
+        c = np.cross(a, b)
+        e += math.acos(max(-1.0, min(1.0, float(np.dot(a, b))))) * float(np.dot(n, c / np.linalg.norm(c)))
+    return abs(0.5 * e)
