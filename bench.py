@@ -16,8 +16,9 @@ record in, 8 B hit out), counted by an instrumented re-render of the same
 frame; divided by the HIP-event duration of the same launches in the timed
 region.
 
-cpu_baseline: the CPU oracle (reference semantics, fp32) on a row subsample
-of the same frame, on this host's cores.
+cpu_baseline: the CPU oracle (reference semantics, fp32) rendering whole frames
+of the same workload for about 10 s, on this host's cores (hw - 4 threads as
+the reference's software tracer uses, and all cores).
 """
 import argparse
 import json
@@ -43,6 +44,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -72,6 +75,16 @@ def main():
             f.frame_index = k0 + k
             f.reset_accum = 1 if k0 + k == 0 else 0
             scene.render(f)
+
+    if args.calibration_copy_mb:
+        n = args.calibration_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {args.calibration_copy_mb} MiB read + {args.calibration_copy_mb} MiB written")
+        del src, dst
 
     run(0, args.warmup)
     dev.synchronize()

@@ -43,7 +43,8 @@ hipError_t dalloc(T** p, size_t count) {
 
 struct dmt_device {
     int id = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // exported stream; even frames, uploads, queries
+    hipStream_t aux = nullptr;      // odd frames (two frames in flight)
     bool lost = false;
 };
 
@@ -54,11 +55,18 @@ struct dmt_scene {
     uint32_t bvh_depth = 0;
     uint32_t bvh_nodes = 0;
     uint64_t scene_bytes = 0;
-    Wavefront wf{};
+    // Two wavefront sets: frame k runs on set k % 2 and its stream, so the
+    // short late-bounce launches of one frame overlap the next frame's
+    // bounce 0.  Only the final accumulation is ordered frame after frame.
+    Wavefront wf[2]{};
     std::vector<void*> wf_buffers;
     float4* accum = nullptr;
     uint32_t acc_w = 0, acc_h = 0;
+    DeviceStats* stats = nullptr;
     DeviceStats* count_stats = nullptr;
+    hipEvent_t acc_done[2] = {nullptr, nullptr};
+    int next_set = 0;
+    int last_set = -1;
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -98,39 +106,57 @@ dmt_status upload(dmt_scene* s, const T* src, size_t count, const T** out) {
     return DMT_OK;
 }
 
+hipStream_t set_stream(const dmt_scene* s, int set) { return set == 0 ? s->dev->stream : s->dev->aux; }
+
+hipError_t sync_all(dmt_device* d) {
+    hipError_t e = hipStreamSynchronize(d->stream);
+    hipError_t e2 = hipStreamSynchronize(d->aux);
+    return e != hipSuccess ? e : e2;
+}
+
 void free_wavefront(dmt_scene* s) {
     for (void* p : s->wf_buffers) (void)hipFree(p);
     s->wf_buffers.clear();
-    s->wf = Wavefront{};
+    s->wf[0] = Wavefront{};
+    s->wf[1] = Wavefront{};
     s->accum = nullptr;
     s->acc_w = s->acc_h = 0;
+    s->last_set = -1;
+    s->next_set = 0;
 }
 
 dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized) {
     *resized = false;
     if (s->accum && s->acc_w == w && s->acc_h == h) return DMT_OK;
+    hipError_t se = sync_all(s->dev);
+    if (se != hipSuccess) return mark_lost_if_fatal(s->dev, se, "synchronize before resize");
     free_wavefront(s);
     *resized = true;
     const size_t n = (size_t)w * h;
-    Wavefront& W = s->wf;
-    W.capacity = (uint32_t)n;
-    const uint32_t need = 3u * s->bvh_depth + 4u;
-    W.spill_depth = need > kLdsDepth ? need - kLdsDepth : 0u;
-    W.spill_threads = kMaxGridBlocks * kBlock;
-    std::vector<std::pair<void**, size_t>> req = {
-        {(void**)&W.ps.T, n * 16}, {(void**)&W.ps.L, n * 16}, {(void**)&W.ps.F, n * 16},
-        {(void**)&W.ps.rng, n * 4}, {(void**)&W.rq[0].o, n * 16}, {(void**)&W.rq[0].d, n * 16},
-        {(void**)&W.rq[1].o, n * 16}, {(void**)&W.rq[1].d, n * 16}, {(void**)&W.hits, n * 8},
-        {(void**)&W.counters, kCounterSlots * 4}, {(void**)&s->accum, n * 16},
-    };
-    for (int k = 0; k < 2; ++k) {
-        req.push_back({(void**)&W.sq[k].o, n * 16});
-        req.push_back({(void**)&W.sq[k].d, n * 16});
-        req.push_back({(void**)&W.sq[k].a, n * 16});
-        req.push_back({(void**)&W.sq[k].b, n * 16});
-        req.push_back({(void**)&W.sq[k].w, n * 4});
+    std::vector<std::pair<void**, size_t>> req = {{(void**)&s->accum, n * 16}};
+    for (int set = 0; set < 2; ++set) {
+        Wavefront& W = s->wf[set];
+        W.capacity = (uint32_t)n;
+        const uint32_t need = 3u * s->bvh_depth + 4u;
+        W.spill_depth = need > kLdsDepth ? need - kLdsDepth : 0u;
+        W.spill_threads = kMaxGridBlocks * kBlock;
+        W.stats = s->stats;
+        const std::pair<void**, size_t> per_set[] = {
+            {(void**)&W.ps.T, n * 16}, {(void**)&W.ps.L, n * 16}, {(void**)&W.ps.F, n * 16},
+            {(void**)&W.ps.rng, n * 4}, {(void**)&W.rq[0].o, n * 16}, {(void**)&W.rq[0].d, n * 16},
+            {(void**)&W.rq[1].o, n * 16}, {(void**)&W.rq[1].d, n * 16}, {(void**)&W.hits, n * 8},
+            {(void**)&W.counters, kCounterSlots * 4},
+        };
+        for (const auto& r : per_set) req.push_back(r);
+        for (int k = 0; k < 2; ++k) {
+            req.push_back({(void**)&W.sq[k].o, n * 16});
+            req.push_back({(void**)&W.sq[k].d, n * 16});
+            req.push_back({(void**)&W.sq[k].a, n * 16});
+            req.push_back({(void**)&W.sq[k].b, n * 16});
+            req.push_back({(void**)&W.sq[k].w, n * 4});
+        }
+        if (W.spill_depth) req.push_back({(void**)&W.spill, (size_t)W.spill_threads * W.spill_depth * 4});
     }
-    if (W.spill_depth) req.push_back({(void**)&W.spill, (size_t)W.spill_threads * W.spill_depth * 4});
     for (auto& r : req) {
         hipError_t e = hipMalloc(r.first, r.second);
         if (e != hipSuccess) {
@@ -140,12 +166,8 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
         }
         s->wf_buffers.push_back(*r.first);
     }
-    W.stats = nullptr;
-    hipError_t e = hipMalloc((void**)&W.stats, sizeof(DeviceStats));
-    if (e != hipSuccess) { free_wavefront(s); return hip_fail(e, "hipMalloc(stats)"); }
-    s->wf_buffers.push_back(W.stats);
-    e = hipMemsetAsync(W.stats, 0, sizeof(DeviceStats), s->dev->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(s->accum, 0, n * 16, s->dev->stream);
+    hipError_t e = hipMemsetAsync(s->accum, 0, n * 16, s->dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
     s->acc_w = w;
     s->acc_h = h;
@@ -170,8 +192,8 @@ FrameArgs frame_args(const dmt_frame_params* f) {
 
 dmt_status harvest_events(dmt_scene* s) {
     if (s->ev_used == 0) return DMT_OK;
-    hipError_t e = hipEventSynchronize(s->ev_pool[s->ev_used - 1]);
-    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "hipEventSynchronize");
+    hipError_t e = sync_all(s->dev);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize");
     for (size_t k = 0; k + 1 < s->ev_used; k += 2) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, s->ev_pool[k], s->ev_pool[k + 1]) == hipSuccess) s->extend_ms += ms;
@@ -205,7 +227,9 @@ dmt_status dmt_device_create(int32_t index, dmt_device** out) {
     if (!d) return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation");
     d->id = index;
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        if (d->stream) (void)hipStreamDestroy(d->stream);
         delete d;
         return hip_fail(e, "hipStreamCreate");
     }
@@ -216,9 +240,10 @@ dmt_status dmt_device_create(int32_t index, dmt_device** out) {
 void dmt_device_destroy(dmt_device* d) {
     if (!d) return;
     (void)hipSetDevice(d->id);
-    if (d->stream) {
-        (void)hipStreamSynchronize(d->stream);
-        (void)hipStreamDestroy(d->stream);
+    for (hipStream_t st : {d->stream, d->aux}) {
+        if (!st) continue;
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
     }
     delete d;
 }
@@ -232,7 +257,7 @@ dmt_status dmt_device_stream(dmt_device* d, void** out) {
 dmt_status dmt_device_synchronize(dmt_device* d) {
     dmt_status st = check_device(d);
     if (st) return st;
-    hipError_t e = hipStreamSynchronize(d->stream);
+    hipError_t e = sync_all(d);
     if (e != hipSuccess) return mark_lost_if_fatal(d, e, "hipStreamSynchronize");
     return DMT_OK;
 }
@@ -298,6 +323,12 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     hipError_t e = hipMalloc((void**)&s->count_stats, sizeof(DeviceStats));
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(stats)"));
     s->buffers.push_back(s->count_stats);
+    e = hipMalloc((void**)&s->stats, sizeof(DeviceStats));
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(stats)"));
+    s->buffers.push_back(s->stats);
+    e = hipMemset(s->stats, 0, sizeof(DeviceStats));
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&s->acc_done[k], hipEventDisableTiming);
+    if (e != hipSuccess) return bail(hip_fail(e, "scene events"));
     *out = s;
     return DMT_OK;
 }
@@ -306,9 +337,11 @@ void dmt_scene_destroy(dmt_scene* s) {
     if (!s) return;
     if (s->dev) {
         (void)hipSetDevice(s->dev->id);
-        (void)hipStreamSynchronize(s->dev->stream);
+        (void)sync_all(s->dev);
     }
     free_wavefront(s);
+    for (hipEvent_t e : s->acc_done)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : s->buffers) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     delete s;
@@ -336,8 +369,14 @@ dmt_status dmt_render(dmt_scene* s, const dmt_frame_params* f) {
         ev = s->ev_pool.data() + s->ev_used;
         s->ev_used = need;
     }
-    hipError_t e = launch_frame(s->view, F, s->wf, s->accum, s->dev->stream, ev);
+    const int set = s->next_set;
+    hipStream_t stream = set_stream(s, set);
+    hipEvent_t after = s->last_set >= 0 ? s->acc_done[s->last_set] : nullptr;
+    hipError_t e = launch_frame(s->view, F, s->wf[set], s->accum, stream, ev, after);
+    if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "launch_frame");
+    s->last_set = set;
+    s->next_set = set ^ 1;
     s->frames++;
     return DMT_OK;
 }
@@ -348,7 +387,8 @@ dmt_status dmt_readback_accum(dmt_scene* s, float* rgba, size_t count) {
     if (st) return st;
     if (!s->accum) return fail(DMT_ERR_INVALID_ARG, "nothing rendered yet");
     if (count != (size_t)s->acc_w * s->acc_h * 4u) return fail(DMT_ERR_INVALID_ARG, "size mismatch (full readback only)");
-    hipError_t e = hipMemcpyAsync(rgba, s->accum, count * 4u, hipMemcpyDeviceToHost, s->dev->stream);
+    hipError_t e = sync_all(s->dev);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, s->accum, count * 4u, hipMemcpyDeviceToHost, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "readback");
     return DMT_OK;
@@ -373,7 +413,7 @@ dmt_status dmt_trace_closest(dmt_scene* s, const float* rays, uint32_t n, dmt_hi
     hipError_t e = hipMalloc((void**)&d_rays, (size_t)n * 32u);
     if (e == hipSuccess) e = hipMalloc((void**)&d_out, (size_t)n * sizeof(HitOut));
     if (e == hipSuccess) e = hipMemcpyAsync(d_rays, rays, (size_t)n * 32u, hipMemcpyHostToDevice, s->dev->stream);
-    if (e == hipSuccess) e = launch_trace_query(s->view, d_rays, n, d_out, s->wf, s->dev->stream);
+    if (e == hipSuccess) e = launch_trace_query(s->view, d_rays, n, d_out, s->wf[0], s->dev->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(HitOut), hipMemcpyDeviceToHost, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
@@ -389,9 +429,9 @@ dmt_status dmt_scene_stats(dmt_scene* s, dmt_stats* out) {
     if (st) return st;
     std::memset(out, 0, sizeof(*out));
     if ((st = harvest_events(s))) return st;
-    if (s->wf.stats) {
+    {
         DeviceStats h{};
-        hipError_t e = hipMemcpyAsync(&h, s->wf.stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
+        hipError_t e = hipMemcpyAsync(&h, s->stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
         if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "stats");
         out->primary_rays = h.primary;
@@ -415,8 +455,9 @@ dmt_status dmt_scene_reset_stats(dmt_scene* s) {
     s->extend_ms = 0.0;
     s->extend_launches = 0;
     s->frames = 0;
-    if (s->wf.stats) {
-        hipError_t e = hipMemsetAsync(s->wf.stats, 0, sizeof(DeviceStats), s->dev->stream);
+    {
+        hipError_t e = hipMemsetAsync(s->stats, 0, sizeof(DeviceStats), s->dev->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
         if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
     }
     return DMT_OK;
@@ -440,13 +481,14 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     float4* scratch = nullptr;
     hipError_t e = hipMalloc((void**)&scratch, (size_t)f->width * f->height * 16u);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(scratch)");
-    DeviceStats* keep = s->wf.stats;
-    s->wf.stats = s->count_stats;
+    e = sync_all(s->dev);
+    if (e != hipSuccess) { (void)hipFree(scratch); return mark_lost_if_fatal(s->dev, e, "synchronize"); }
+    Wavefront W = s->wf[0];
+    W.stats = s->count_stats;
     e = hipMemsetAsync(s->count_stats, 0, sizeof(DeviceStats), s->dev->stream);
     FrameArgs F = frame_args(f);
     F.reset_accum = 1u;
-    if (e == hipSuccess) e = launch_frame_counting(s->view, F, s->wf, scratch, s->dev->stream);
-    s->wf.stats = keep;
+    if (e == hipSuccess) e = launch_frame_counting(s->view, F, W, scratch, s->dev->stream);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);

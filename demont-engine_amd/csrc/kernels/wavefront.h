@@ -16,7 +16,7 @@ namespace dmt {
 constexpr uint32_t kBlock = 256;          // threads per workgroup (4 waves)
 constexpr uint32_t kLdsDepth = 24;        // LDS stack entries per thread
 constexpr uint32_t kMaxGridBlocks = 4096; // grid-stride cap: 1M threads
-constexpr uint32_t kCounterSlots = 4 * 72;
+constexpr uint32_t kCounterSlots = 4 * 72;    // last slot: raygen slot allocator
 
 // Path state, indexed by pixel (SoA, 16 B rows).
 struct PathState {
@@ -79,9 +79,11 @@ FrameArgs make_frame_args(const float pos_fovtan[4], const float fwd_aspect[4], 
 uint32_t grid_for(uint32_t n);
 
 // Launch the kernels of one frame on `stream`.  extend_events (may be null)
-// holds 2*max_bounces*spp events recorded around each extend launch.
+// holds 2*max_bounces*spp events recorded around each extend launch.  When
+// accum_after is set, the final accumulation waits for it (the previous
+// frame's accumulation on another stream), keeping per-pixel frame order.
 hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
-                        hipEvent_t* extend_events);
+                        hipEvent_t* extend_events, hipEvent_t accum_after);
 
 // Closest-hit queries: rays (8 floats each) in device memory.
 struct HitOut {

@@ -45,17 +45,31 @@ __device__ __forceinline__ Stack make_stack(uint32_t* lds, uint32_t* spill, uint
 
 // ---------------------------------------------------------------------------
 
+// Zeroes the queue counters of a sample; the bounce-0 queue holds every pixel.
+__global__ void k_reset_counters(uint32_t* ctr, uint32_t npix) {
+    for (uint32_t k = threadIdx.x; k < kCounterSlots; k += blockDim.x) ctr[k] = (k == ctr_rays(0)) ? npix : 0u;
+}
+
 __global__ void __launch_bounds__(kBlock) k_raygen(FrameArgs F, PathState ps, RayQueue q, uint32_t s,
                                                    uint32_t* ctr) {
     const uint32_t npix = F.width * F.height;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    if (gid == 0) {
-        for (uint32_t k = 0; k < kCounterSlots; ++k) ctr[k] = 0u;
-        ctr[ctr_rays(0)] = npix;
-    }
-    for (uint32_t p = gid; p < npix; p += stride) {
-        const uint32_t x = p % F.width, y = p / F.width;
+    (void)npix;
+    // Slot q walks the image in 8x8 tiles so a wave64 traces one tile of
+    // neighbouring camera rays; the path state stays indexed by pixel.
+    const uint32_t tiles_x = (F.width + 7u) / 8u;
+    const uint32_t tiles_y = (F.height + 7u) / 8u;
+    const uint32_t nslots = tiles_x * tiles_y * 64u;
+    for (uint32_t q0 = gid - threadIdx.x % 64u; q0 < nslots; q0 += stride) {
+        const uint32_t qs = q0 + threadIdx.x % 64u;
+        const uint32_t tile = qs >> 6, lane = qs & 63u;
+        const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u);
+        const uint32_t y = (tile / tiles_x) * 8u + (lane >> 3);
+        const bool valid = qs < nslots && x < F.width && y < F.height;
+        const uint32_t slot = wave_append(&ctr[kCounterSlots - 1u], valid);
+        if (!valid) continue;
+        const uint32_t p = y * F.width + x;
         uint32_t seed = (s == 0u) ? pcg_hash(x * 1973u + y * 9277u + F.frame_index * 26699u) : ps.rng[p];
         float jx = randf(seed);
         float jy = randf(seed);
@@ -65,8 +79,8 @@ __global__ void __launch_bounds__(kBlock) k_raygen(FrameArgs F, PathState ps, Ra
         float uy = ((((float)y + 0.5f) + jy) / (float)F.height) * 2.0f - 1.0f;
         uy = -uy;
         f3 rd = normalize((F.fwd + F.right * ((ux * F.aspect) * F.fovtan)) + F.up * (uy * F.fovtan));
-        q.o[p] = make_float4(F.pos.x, F.pos.y, F.pos.z, 0.0f);
-        q.d[p] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(p));
+        q.o[slot] = make_float4(F.pos.x, F.pos.y, F.pos.z, 0.0f);
+        q.d[slot] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(p));
         ps.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
         ps.rng[p] = seed;
@@ -421,11 +435,12 @@ uint32_t grid_for(uint32_t n) {
 
 template <bool COUNT>
 static hipError_t frame_impl(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
-                             hipEvent_t* ev) {
+                             hipEvent_t* ev, hipEvent_t accum_after) {
     const uint32_t npix = F.width * F.height;
     const uint32_t g = grid_for(npix);
     uint32_t k = 0;
     for (uint32_t s = 0; s < F.spp; ++s) {
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(kBlock), 0, stream, W.counters, npix);
         hipLaunchKernelGGL(k_raygen, dim3(g), dim3(kBlock), 0, stream, F, W.ps, W.rq[0], s, W.counters);
         for (uint32_t b = 0; b < F.max_bounces; ++b) {
             if (ev) (void)hipEventRecord(ev[2 * k], stream);
@@ -440,19 +455,20 @@ static hipError_t frame_impl(const SceneView& S, const FrameArgs& F, Wavefront& 
             hipLaunchKernelGGL((k_shadow<1, COUNT>), dim3(g), dim3(kBlock), 0, stream, S, F, b, W.sq[1],
                                W.counters, W.ps, W.spill, W.spill_depth, W.stats);
         }
+        if (s + 1u == F.spp && accum_after) (void)hipStreamWaitEvent(stream, accum_after, 0);
         hipLaunchKernelGGL(k_sample_end, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum, s);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
-                        hipEvent_t* ev) {
-    return frame_impl<false>(S, F, W, accum, stream, ev);
+                        hipEvent_t* ev, hipEvent_t accum_after) {
+    return frame_impl<false>(S, F, W, accum, stream, ev, accum_after);
 }
 
 hipError_t launch_frame_counting(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum,
                                  hipStream_t stream) {
-    return frame_impl<true>(S, F, W, accum, stream, nullptr);
+    return frame_impl<true>(S, F, W, accum, stream, nullptr, nullptr);
 }
 
 hipError_t launch_trace_query(const SceneView& S, const float4* rays, uint32_t n, HitOut* out, Wavefront& W,
