@@ -65,6 +65,8 @@ struct dmt_scene {
     DeviceStats* stats = nullptr;
     DeviceStats* count_stats = nullptr;
     hipEvent_t acc_done[2] = {nullptr, nullptr};
+    hipEvent_t poll[2][kPollSlots] = {};
+    uint32_t* host_count = nullptr;   // pinned, 2 * kPollSlots
     int next_set = 0;
     int last_set = -1;
     // profiling
@@ -141,11 +143,13 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
         W.spill_depth = need > kLdsDepth ? need - kLdsDepth : 0u;
         W.spill_threads = kMaxGridBlocks * kBlock;
         W.stats = s->stats;
+        W.host_count = s->host_count + set * kPollSlots;
         const std::pair<void**, size_t> per_set[] = {
             {(void**)&W.ps.T, n * 16}, {(void**)&W.ps.L, n * 16}, {(void**)&W.ps.F, n * 16},
             {(void**)&W.ps.rng, n * 4}, {(void**)&W.rq[0].o, n * 16}, {(void**)&W.rq[0].d, n * 16},
             {(void**)&W.rq[1].o, n * 16}, {(void**)&W.rq[1].d, n * 16}, {(void**)&W.hits, n * 8},
-            {(void**)&W.counters, kCounterSlots * 4},
+            {(void**)&W.counters, kCounterSlots * 4}, {(void**)&W.ps.meta, n * 4},
+            {(void**)&W.finished, n * 4},
         };
         for (const auto& r : per_set) req.push_back(r);
         for (int k = 0; k < 2; ++k) {
@@ -187,7 +191,7 @@ dmt_status check_frame(const dmt_frame_params* f) {
 FrameArgs frame_args(const dmt_frame_params* f) {
     return make_frame_args(f->pos_fovtan, f->fwd_aspect, f->right_xyz, f->up_xyz, f->width, f->height,
                            f->frame_index, f->spp, f->max_bounces, f->reset_accum, f->mis_enabled,
-                           f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha);
+                           f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha, 1u);
 }
 
 dmt_status harvest_events(dmt_scene* s) {
@@ -327,7 +331,12 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(stats)"));
     s->buffers.push_back(s->stats);
     e = hipMemset(s->stats, 0, sizeof(DeviceStats));
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&s->acc_done[k], hipEventDisableTiming);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipEventCreateWithFlags(&s->acc_done[k], hipEventDisableTiming);
+        for (uint32_t j = 0; j < kPollSlots && e == hipSuccess; ++j)
+            e = hipEventCreateWithFlags(&s->poll[k][j], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->host_count, 2 * kPollSlots * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) return bail(hip_fail(e, "scene events"));
     *out = s;
     return DMT_OK;
@@ -342,43 +351,72 @@ void dmt_scene_destroy(dmt_scene* s) {
     free_wavefront(s);
     for (hipEvent_t e : s->acc_done)
         if (e) (void)hipEventDestroy(e);
+    for (auto& row : s->poll)
+        for (hipEvent_t e : row)
+            if (e) (void)hipEventDestroy(e);
+    if (s->host_count) (void)hipHostFree(s->host_count);
     for (void* p : s->buffers) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     delete s;
 }
 
-dmt_status dmt_render(dmt_scene* s, const dmt_frame_params* f) {
+namespace {
+
+// Renders frame_count frames on one wavefront set.  One frame alternates sets
+// and streams (two frames in flight, accumulation ordered by events); a
+// batch of frames runs on set 0 after everything before it.
+dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_count) {
     if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
     dmt_status st = check_device(s->dev);
     if (st) return st;
     if ((st = check_frame(f))) return st;
+    if (frame_count == 0) return fail(DMT_ERR_INVALID_ARG, "frame_count must be >= 1");
+    if (frame_count > 65535u) return fail(DMT_ERR_LIMIT, "frame_count must be <= 65535");
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
     FrameArgs F = frame_args(f);
+    F.frame_count = frame_count;
     if (resized) F.reset_accum = 1u;
     if (f->max_bounces == 0) return DMT_OK;
-    hipEvent_t* ev = nullptr;
+    const bool single = frame_count == 1u;
+    const int set = single ? s->next_set : 0;
+    hipStream_t stream = set_stream(s, set);
+    hipEvent_t after = s->last_set >= 0 ? s->acc_done[s->last_set] : nullptr;
+    LaunchOptions o;
+    o.poll_events = s->poll[set];
+    if (single) {
+        o.accum_after = after;
+    } else if (after) {
+        hipError_t e = hipStreamWaitEvent(stream, after, 0);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+    }
     if (s->profiling) {
-        const size_t need = s->ev_used + 2u * f->spp * f->max_bounces;
-        while (s->ev_pool.size() < need) {
+        const size_t bound = 2u * (size_t)frame_count * f->spp * f->max_bounces;
+        while (s->ev_pool.size() < s->ev_used + bound) {
             hipEvent_t e;
             hipError_t he = hipEventCreate(&e);
             if (he != hipSuccess) return hip_fail(he, "hipEventCreate");
             s->ev_pool.push_back(e);
         }
-        ev = s->ev_pool.data() + s->ev_used;
-        s->ev_used = need;
+        o.extend_events = s->ev_pool.data() + s->ev_used;
+        o.max_events = (uint32_t)bound;
     }
-    const int set = s->next_set;
-    hipStream_t stream = set_stream(s, set);
-    hipEvent_t after = s->last_set >= 0 ? s->acc_done[s->last_set] : nullptr;
-    hipError_t e = launch_frame(s->view, F, s->wf[set], s->accum, stream, ev, after);
+    hipError_t e = launch_frames(s->view, F, s->wf[set], s->accum, stream, o);
+    s->ev_used += o.events_used;
     if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
-    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "launch_frame");
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "launch_frames");
     s->last_set = set;
     s->next_set = set ^ 1;
-    s->frames++;
+    s->frames += frame_count;
     return DMT_OK;
+}
+
+}  // namespace
+
+dmt_status dmt_render(dmt_scene* s, const dmt_frame_params* f) { return render_impl(s, f, 1u); }
+
+dmt_status dmt_render_frames(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_count) {
+    return render_impl(s, f, frame_count);
 }
 
 dmt_status dmt_readback_accum(dmt_scene* s, float* rgba, size_t count) {
@@ -488,7 +526,9 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     e = hipMemsetAsync(s->count_stats, 0, sizeof(DeviceStats), s->dev->stream);
     FrameArgs F = frame_args(f);
     F.reset_accum = 1u;
-    if (e == hipSuccess) e = launch_frame_counting(s->view, F, W, scratch, s->dev->stream);
+    LaunchOptions o;
+    o.counting = true;
+    if (e == hipSuccess) e = launch_frames(s->view, F, W, scratch, s->dev->stream, o);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);

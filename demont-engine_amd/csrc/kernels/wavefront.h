@@ -1,10 +1,15 @@
-// Wavefront state and kernel launchers of the path tracer.
+// Streaming wavefront path tracer: state, queues and launchers.
 //
-// One frame = for each sample s: raygen -> (extend -> shade -> shadow(light)
-// -> shadow(env)) x max_bounces -> sample_end.  Samples of a pixel run in
-// order, so each pixel's PCG state flows from sample s to s + 1 exactly as
-// the reference's per-pixel spp loop (PathTrace.slang:5722) draws it.
-// All queue lengths live in device memory: a frame issues no host sync.
+// Every live path carries its own (bounce, sample, frame).  One iteration is
+//   k_extend -> k_shade -> k_shadow<light> -> k_shadow<env> -> k_finish
+// and k_finish turns each path that ended into (a) the next sample of the same
+// pixel, continuing its PCG state as the reference's per-pixel spp loop does
+// (PathTrace.slang:5722), or (b) the frame's final value and, when more frames
+// were asked for, the first sample of the pixel's next frame.  A pixel's
+// samples and frames therefore run strictly in order and the image equals the
+// frame-by-frame result, while the queues stay full until the batch drains.
+// Queue lengths live in device memory; the host never waits inside a batch
+// except to stop launching once the live queue is empty.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -16,15 +21,25 @@ namespace dmt {
 constexpr uint32_t kBlock = 256;          // threads per workgroup (4 waves)
 constexpr uint32_t kLdsDepth = 24;        // LDS stack entries per thread
 constexpr uint32_t kMaxGridBlocks = 4096; // grid-stride cap: 1M threads
-constexpr uint32_t kCounterSlots = 4 * 72;    // last slot: raygen slot allocator
+
+// counter slots
+constexpr uint32_t kCtrRays = 0;          // + parity: ray queue lengths
+constexpr uint32_t kCtrFinished = 2;
+constexpr uint32_t kCtrShadow = 3;        // + kind: shadow queue lengths
+constexpr uint32_t kCounterSlots = 8;
+
+constexpr uint32_t kClampBit = 0x80000000u;   // in shadow records' pixel word
 
 // Path state, indexed by pixel (SoA, 16 B rows).
 struct PathState {
     float4* T;        // throughput.xyz, prev_lambert_brdf_pdf
     float4* L;        // radiance.xyz, flags (bit0 skip_sky_on_miss, bit1 prev_was_lambert_nee)
-    float4* F;        // frame radiance sum over samples (spp > 1)
+    float4* F;        // sample sum of the frame; the frame value in deferred mode
     uint32_t* rng;
+    uint32_t* meta;   // bounce | sample << 8 | frame << 16
 };
+
+__host__ __device__ inline uint32_t meta_pack(uint32_t b, uint32_t s, uint32_t k) { return b | (s << 8) | (k << 16); }
 
 // Ray queue, indexed by slot.
 struct RayQueue {
@@ -35,9 +50,9 @@ struct RayQueue {
 // Shadow queue: one transmittance query + the factors of its contribution.
 struct ShadowQueue {
     float4* o;        // origin.xyz, t_max
-    float4* d;        // direction.xyz, pixel index bits
+    float4* d;        // direction.xyz, pixel | kClampBit
     float4* a;        // throughput*Le (or *env colour).xyz, n_dot_l
-    float4* b;        // brdf.xyz, pdf
+    float4* b;        // brdf.xyz, pdf (< 0: no-trace ambient record)
     float* w;         // MIS weight (env queue)
 };
 
@@ -59,31 +74,41 @@ struct Wavefront {
     RayQueue rq[2]{};
     uint2* hits = nullptr;    // (t bits, code) per slot
     ShadowQueue sq[2]{};      // 0: light NEE, 1: env NEE
+    uint32_t* finished = nullptr;
     uint32_t* counters = nullptr;
     uint32_t* spill = nullptr;
     uint32_t spill_depth = 0;
     uint32_t spill_threads = 0;
     DeviceStats* stats = nullptr;
+    uint32_t* host_count = nullptr;   // pinned: live-queue snapshots for the stop test
 };
-
-// Counter slots: ray queue length at bounce b, light-shadow and env-shadow
-// queue lengths emitted at bounce b.
-__host__ __device__ inline uint32_t ctr_rays(uint32_t b) { return b; }
-__host__ __device__ inline uint32_t ctr_shadow(uint32_t kind, uint32_t b) { return 72u + kind * 72u + b; }
 
 FrameArgs make_frame_args(const float pos_fovtan[4], const float fwd_aspect[4], const float right[4],
                           const float up[4], uint32_t width, uint32_t height, uint32_t frame_index, uint32_t spp,
                           uint32_t max_bounces, uint32_t reset_accum, uint32_t mis, uint32_t ltree,
-                          uint32_t refract, float clamp, float ema);
+                          uint32_t refract, float clamp, float ema, uint32_t frame_count);
 
 uint32_t grid_for(uint32_t n);
 
-// Launch the kernels of one frame on `stream`.  extend_events (may be null)
-// holds 2*max_bounces*spp events recorded around each extend launch.  When
-// accum_after is set, the final accumulation waits for it (the previous
-// frame's accumulation on another stream), keeping per-pixel frame order.
-hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
-                        hipEvent_t* extend_events, hipEvent_t accum_after);
+constexpr uint32_t kPollSlots = 4;
+constexpr uint32_t kPollChunk = 4;        // iterations between live-queue snapshots
+
+struct LaunchOptions {
+    hipEvent_t* extend_events = nullptr;  // 2 per iteration, or null
+    uint32_t max_events = 0;
+    uint32_t events_used = 0;             // out
+    hipEvent_t accum_after = nullptr;     // deferred accumulation waits for it
+    bool counting = false;                // instrumented traversal
+    hipEvent_t* poll_events = nullptr;    // kPollSlots events for the stop test
+    uint32_t iterations = 0;              // out: iterations launched
+};
+
+// Render F.frame_count frames starting at F.frame_index into accum.  With one
+// frame, the frame's values are folded into accum by a final pass that waits
+// for opts.accum_after (the previous frame on another stream); with several,
+// each pixel folds its frames in order inside the batch.
+hipError_t launch_frames(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
+                         LaunchOptions& opts);
 
 // Closest-hit queries: rays (8 floats each) in device memory.
 struct HitOut {
@@ -93,9 +118,5 @@ struct HitOut {
 };
 hipError_t launch_trace_query(const SceneView& S, const float4* rays, uint32_t n, HitOut* out, Wavefront& W,
                               hipStream_t stream);
-
-// Counting re-render: same frame with the instrumented traversal.
-hipError_t launch_frame_counting(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum,
-                                 hipStream_t stream);
 
 }  // namespace dmt

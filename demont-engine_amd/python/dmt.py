@@ -138,6 +138,7 @@ def lib() -> C.CDLL:
         "dmt_scene_create": (st, [vp, C.POINTER(SceneDesc), C.POINTER(BuildParams), C.POINTER(vp)]),
         "dmt_scene_destroy": (None, [vp]),
         "dmt_render": (st, [vp, C.POINTER(FrameParams)]),
+        "dmt_render_frames": (st, [vp, C.POINTER(FrameParams), u32]),
         "dmt_readback_accum": (st, [vp, fp, C.c_size_t]),
         "dmt_accum_device_ptr": (st, [vp, C.POINTER(vp)]),
         "dmt_trace_closest": (st, [vp, fp, u32, vp]),
@@ -287,6 +288,9 @@ class Scene:
 
     def render(self, frame: FrameParams):
         _check(lib().dmt_render(self._h, C.byref(frame)))
+
+    def render_frames(self, frame: FrameParams, count: int):
+        _check(lib().dmt_render_frames(self._h, C.byref(frame), count))
 
     def readback(self, width: int, height: int) -> np.ndarray:
         out = np.empty((height, width, 4), np.float32)

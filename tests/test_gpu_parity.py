@@ -143,3 +143,73 @@ def test_render_c1_full_size_rows(device):
     print(f"C1 full: rel-MSE on {len(rows)} rows {err:.3e}, bit-exact {exact:.6f}, rays {st}")
     assert err <= RADIANCE_REL_MSE
     assert st["primary_rays"] == sd.frame.width * sd.frame.height
+
+
+@pytest.mark.parametrize("cfg,kw,spp", [(0, dict(seed=6, width=128, height=96), 1),
+                                        (4, dict(seed=7, spheres=40, width=64, height=48), 4),
+                                        (1, dict(seed=8, subdivisions=4, random_tris=5000, width=160, height=90), 1)])
+def test_render_frames_equals_sequential_frames(device, cfg, kw, spp):
+    """dmt_render_frames(K) regenerates paths across frames inside one call;
+    its image must be bit-identical to K dmt_render calls and to the oracle."""
+    sd = dmt.generate(cfg, **kw)
+    f = sd.frame.copy()
+    f.spp = spp
+    f.frame_index = 3
+    K = 5
+    a = dmt.Scene(device, sd)
+    a.render_frames(f, K)
+    img_a = a.readback(f.width, f.height)
+    st_a = a.stats()
+    b = dmt.Scene(device, sd)
+    ref = None
+    oscene = orc.OracleScene.from_scene_data(sd)
+    cnt_total = {"primary": 0, "extension": 0, "shadow": 0}
+    for k in range(K):
+        fk = f.copy()
+        fk.frame_index = f.frame_index + k
+        fk.reset_accum = 1 if k == 0 else 0
+        b.render(fk)
+        ref, cnt = oscene.render(fk, 32, 0, accum=ref)
+        for key in cnt_total:
+            cnt_total[key] += cnt[key]
+    img_b = b.readback(f.width, f.height)
+    print(f"config {cfg}: streaming vs per-frame equal {np.array_equal(img_a, img_b)}, "
+          f"vs oracle rel-MSE {rel_mse(img_a[..., :3], ref[..., :3]):.3e}")
+    assert np.array_equal(img_a, img_b)
+    assert np.array_equal(img_a, ref)
+    assert (st_a["primary_rays"], st_a["extension_rays"], st_a["shadow_rays"]) == \
+        (cnt_total["primary"], cnt_total["extension"], cnt_total["shadow"])
+    assert np.all(img_a[..., 3] == K)
+
+
+def test_error_paths_through_the_abi(device):
+    sd = dmt.generate(0, seed=1, width=16, height=16)
+    sc = dmt.Scene(device, sd)
+    f = sd.frame.copy()
+    f.spp = 0
+    with pytest.raises(dmt.DmtError) as e:
+        sc.render(f)
+    assert e.value.status == dmt.DMT_ERR_INVALID_ARG
+    f = sd.frame.copy()
+    f.max_bounces = 65
+    with pytest.raises(dmt.DmtError) as e:
+        sc.render(f)
+    assert e.value.status == dmt.DMT_ERR_LIMIT
+    with pytest.raises(dmt.DmtError):
+        sc.render_frames(sd.frame, 0)
+    bad = dmt.generate(0, seed=1)
+    bad.prims[3, 9] = 7.0   # material id outside 0..2
+    with pytest.raises(dmt.DmtError) as e:
+        dmt.Scene(device, bad)
+    assert e.value.status == dmt.DMT_ERR_INVALID_ARG
+    bad = dmt.generate(1, seed=1, subdivisions=1, random_tris=0)
+    bad.indices[2, 1] = 10_000
+    with pytest.raises(dmt.DmtError):
+        dmt.Scene(device, bad)
+    # readback size must match exactly (full-or-nothing, SoftwareDevice.cpp:458-483)
+    sc.render(sd.frame)
+    out = np.zeros(10, np.float32)
+    import ctypes as C
+    rc = dmt.lib().dmt_readback_accum(sc._h, out.ctypes.data_as(C.POINTER(C.c_float)), out.size)
+    assert rc == dmt.DMT_ERR_INVALID_ARG
+    assert not device.is_lost()
