@@ -46,7 +46,23 @@ struct dmt_device {
     hipStream_t stream = nullptr;   // exported stream; even frames, uploads, queries
     hipStream_t aux = nullptr;      // odd frames (two frames in flight)
     bool lost = false;
+    // Scenes hold the device: dmt_device_destroy with live scenes defers the
+    // teardown to the last dmt_scene_destroy.
+    uint32_t scenes = 0;
+    bool destroy_requested = false;
 };
+
+namespace {
+void device_free(dmt_device* d) {
+    (void)hipSetDevice(d->id);
+    for (hipStream_t st : {d->stream, d->aux}) {
+        if (!st) continue;
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    delete d;
+}
+}  // namespace
 
 struct dmt_scene {
     dmt_device* dev = nullptr;
@@ -243,13 +259,11 @@ dmt_status dmt_device_create(int32_t index, dmt_device** out) {
 
 void dmt_device_destroy(dmt_device* d) {
     if (!d) return;
-    (void)hipSetDevice(d->id);
-    for (hipStream_t st : {d->stream, d->aux}) {
-        if (!st) continue;
-        (void)hipStreamSynchronize(st);
-        (void)hipStreamDestroy(st);
+    if (d->scenes > 0) {
+        d->destroy_requested = true;
+        return;
     }
-    delete d;
+    device_free(d);
 }
 
 dmt_status dmt_device_stream(dmt_device* d, void** out) {
@@ -289,6 +303,7 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     auto* s = new (std::nothrow) dmt_scene();
     if (!s) return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation");
     s->dev = dev;
+    dev->scenes++;
     SceneView& V = s->view;
     std::memset(&V, 0, sizeof(V));
     auto bail = [&](dmt_status code) {
@@ -357,7 +372,9 @@ void dmt_scene_destroy(dmt_scene* s) {
     if (s->host_count) (void)hipHostFree(s->host_count);
     for (void* p : s->buffers) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    dmt_device* d = s->dev;
     delete s;
+    if (d && --d->scenes == 0 && d->destroy_requested) device_free(d);
 }
 
 namespace {

@@ -134,6 +134,8 @@ dmt_status dmt_get_abi_version(uint32_t* out_version);
 /* Replaces Device::Create(BackendType::Software, ...) / CreateSoftwareDevice
  * (src/rhi/Device.cpp:20-48, src/rhi_software/SoftwareDevice.cpp:1165). */
 dmt_status dmt_device_create(int32_t device_index, dmt_device** out_device);
+/* Destroying a device that still has scenes defers the teardown until the
+ * last of its scenes is destroyed. */
 void       dmt_device_destroy(dmt_device* dev);
 /* hipStream_t the device orders its work on (for caller-side events). */
 dmt_status dmt_device_stream(dmt_device* dev, void** out_stream);

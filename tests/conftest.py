@@ -19,7 +19,7 @@ def device():
     import dmt
     dev = dmt.Device(0)
     yield dev
-    dev.close()
+    dev.close()   # deferred by the library while scenes are alive
 
 
 def rel_mse(a: np.ndarray, b: np.ndarray) -> float:
