@@ -18,6 +18,9 @@
 
 using namespace dmt;
 
+// Frames of per-frame dmt_render calls kept in flight (one stream each).
+constexpr uint32_t kFrameSets = 3;
+
 namespace {
 
 thread_local std::string g_err;
@@ -43,8 +46,8 @@ hipError_t dalloc(T** p, size_t count) {
 
 struct dmt_device {
     int id = 0;
-    hipStream_t stream = nullptr;   // exported stream; even frames, uploads, queries
-    hipStream_t aux = nullptr;      // odd frames (two frames in flight)
+    hipStream_t stream = nullptr;   // exported stream; frame set 0, uploads, queries
+    hipStream_t aux[kFrameSets - 1] = {};   // frame sets 1.. (frames in flight)
     bool lost = false;
     // Scenes hold the device: dmt_device_destroy with live scenes defers the
     // teardown to the last dmt_scene_destroy.
@@ -55,7 +58,9 @@ struct dmt_device {
 namespace {
 void device_free(dmt_device* d) {
     (void)hipSetDevice(d->id);
-    for (hipStream_t st : {d->stream, d->aux}) {
+    hipStream_t all[kFrameSets] = {d->stream};
+    for (uint32_t k = 1; k < kFrameSets; ++k) all[k] = d->aux[k - 1];
+    for (hipStream_t st : all) {
         if (!st) continue;
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
@@ -71,18 +76,19 @@ struct dmt_scene {
     uint32_t bvh_depth = 0;
     uint32_t bvh_nodes = 0;
     uint64_t scene_bytes = 0;
-    // Two wavefront sets: frame k runs on set k % 2 and its stream, so the
-    // short late-bounce launches of one frame overlap the next frame's
-    // bounce 0.  Only the final accumulation is ordered frame after frame.
-    Wavefront wf[2]{};
+    // kFrameSets wavefront sets: frame k of per-frame calls runs on set
+    // k % kFrameSets and its stream, so the short late-bounce launches of one
+    // frame overlap the next frames' heavy bounces.  Only the final
+    // accumulation is ordered frame after frame.
+    Wavefront wf[kFrameSets]{};
     std::vector<void*> wf_buffers;
     float4* accum = nullptr;
     uint32_t acc_w = 0, acc_h = 0;
     DeviceStats* stats = nullptr;
     DeviceStats* count_stats = nullptr;
-    hipEvent_t acc_done[2] = {nullptr, nullptr};
-    hipEvent_t poll[2][kPollSlots] = {};
-    uint32_t* host_count = nullptr;   // pinned, 2 * kPollSlots
+    hipEvent_t acc_done[kFrameSets] = {};
+    hipEvent_t poll[kFrameSets][kPollSlots] = {};
+    uint32_t* host_count = nullptr;   // pinned, kFrameSets * kPollSlots
     int next_set = 0;
     int last_set = -1;
     // profiling
@@ -124,19 +130,21 @@ dmt_status upload(dmt_scene* s, const T* src, size_t count, const T** out) {
     return DMT_OK;
 }
 
-hipStream_t set_stream(const dmt_scene* s, int set) { return set == 0 ? s->dev->stream : s->dev->aux; }
+hipStream_t set_stream(const dmt_scene* s, int set) { return set == 0 ? s->dev->stream : s->dev->aux[set - 1]; }
 
 hipError_t sync_all(dmt_device* d) {
     hipError_t e = hipStreamSynchronize(d->stream);
-    hipError_t e2 = hipStreamSynchronize(d->aux);
-    return e != hipSuccess ? e : e2;
+    for (uint32_t k = 0; k + 1 < kFrameSets; ++k) {
+        hipError_t e2 = hipStreamSynchronize(d->aux[k]);
+        if (e == hipSuccess) e = e2;
+    }
+    return e;
 }
 
 void free_wavefront(dmt_scene* s) {
     for (void* p : s->wf_buffers) (void)hipFree(p);
     s->wf_buffers.clear();
-    s->wf[0] = Wavefront{};
-    s->wf[1] = Wavefront{};
+    for (auto& w : s->wf) w = Wavefront{};
     s->accum = nullptr;
     s->acc_w = s->acc_h = 0;
     s->last_set = -1;
@@ -152,7 +160,7 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
     *resized = true;
     const size_t n = (size_t)w * h;
     std::vector<std::pair<void**, size_t>> req = {{(void**)&s->accum, n * 16}};
-    for (int set = 0; set < 2; ++set) {
+    for (int set = 0; set < (int)kFrameSets; ++set) {
         Wavefront& W = s->wf[set];
         W.capacity = (uint32_t)n;
         const uint32_t need = 3u * s->bvh_depth + 4u;
@@ -165,7 +173,7 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
             {(void**)&W.ps.rng, n * 4}, {(void**)&W.rq[0].o, n * 16}, {(void**)&W.rq[0].d, n * 16},
             {(void**)&W.rq[1].o, n * 16}, {(void**)&W.rq[1].d, n * 16}, {(void**)&W.hits, n * 8},
             {(void**)&W.counters, kCounterSlots * 4}, {(void**)&W.ps.meta, n * 4},
-            {(void**)&W.finished, n * 4},
+            {(void**)&W.finished, n * 4}, {(void**)&W.regen, n},
         };
         for (const auto& r : per_set) req.push_back(r);
         for (int k = 0; k < 2; ++k) {
@@ -187,6 +195,8 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
         s->wf_buffers.push_back(*r.first);
     }
     hipError_t e = hipMemsetAsync(s->accum, 0, n * 16, s->dev->stream);
+    for (int set = 0; set < (int)kFrameSets && e == hipSuccess; ++set)
+        e = hipMemsetAsync(s->wf[set].regen, 0, n, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
     s->acc_w = w;
@@ -247,10 +257,10 @@ dmt_status dmt_device_create(int32_t index, dmt_device** out) {
     if (!d) return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation");
     d->id = index;
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking);
+    for (uint32_t k = 0; k + 1 < kFrameSets && e == hipSuccess; ++k)
+        e = hipStreamCreateWithFlags(&d->aux[k], hipStreamNonBlocking);
     if (e != hipSuccess) {
-        if (d->stream) (void)hipStreamDestroy(d->stream);
-        delete d;
+        device_free(d);
         return hip_fail(e, "hipStreamCreate");
     }
     *out = d;
@@ -346,12 +356,13 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(stats)"));
     s->buffers.push_back(s->stats);
     e = hipMemset(s->stats, 0, sizeof(DeviceStats));
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    for (uint32_t k = 0; k < kFrameSets && e == hipSuccess; ++k) {
         e = hipEventCreateWithFlags(&s->acc_done[k], hipEventDisableTiming);
         for (uint32_t j = 0; j < kPollSlots && e == hipSuccess; ++j)
             e = hipEventCreateWithFlags(&s->poll[k][j], hipEventDisableTiming);
     }
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->host_count, 2 * kPollSlots * sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&s->host_count, kFrameSets * kPollSlots * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) return bail(hip_fail(e, "scene events"));
     *out = s;
     return DMT_OK;
@@ -423,7 +434,7 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
     if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "launch_frames");
     s->last_set = set;
-    s->next_set = set ^ 1;
+    s->next_set = (set + 1) % (int)kFrameSets;
     s->frames += frame_count;
     return DMT_OK;
 }

@@ -53,16 +53,16 @@ struct PathVertexState {
 struct VertexResult {
     bool alive;                  // the path scatters (before the depth limit)
     f3 no, nd;                   // next ray
-    bool emit_l, emit_e;
-    ShadowRec lrec, erec;
 };
 
 // Shades the vertex at bounce b for the ray (o, d) with closest hit h.
+// Shadow queries go to sink.emit(kind, record) as soon as they are formed
+// (light NEE before env NEE, the order their contributions are summed).
+template <class Sink>
 __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs& F, uint32_t b, f3 o, f3 d,
-                                             HitRec h, PathVertexState& P, VertexResult& R) {
+                                             HitRec h, PathVertexState& P, VertexResult& R, Sink& sink) {
     const bool mis_on = F.mis_enabled != 0u;
     R.alive = false;
-    R.emit_l = R.emit_e = false;
     R.no = f3{0.0f, 0.0f, 0.0f};
     R.nd = f3{0.0f, 1.0f, 0.0f};
     if (h.code == kMiss) {
@@ -91,16 +91,17 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
             LightSample ls = sample_light(S, F, hp, nf, P.seed);
             float ndl = dot(nf, ls.dir);
             if (ls.pdf > 0.0f && ndl > 0.0f) {
-                R.emit_l = true;
-                R.lrec.o = hp + nf * 1e-3f;
-                R.lrec.d = ls.dir;
-                R.lrec.t_max = fmaxf(ls.dist - 1e-3f, 0.0f);
-                R.lrec.pre = P.T * ls.Le;
-                R.lrec.ndl = ndl;
-                R.lrec.brdf = sf.albedo / kPi;
-                R.lrec.pdf = ls.pdf;
-                R.lrec.w = 1.0f;
-                R.lrec.clamp = b > 0u;
+                ShadowRec r;
+                r.o = hp + nf * 1e-3f;
+                r.d = ls.dir;
+                r.t_max = fmaxf(ls.dist - 1e-3f, 0.0f);
+                r.pre = P.T * ls.Le;
+                r.ndl = ndl;
+                r.brdf = sf.albedo / kPi;
+                r.pdf = ls.pdf;
+                r.w = 1.0f;
+                r.clamp = b > 0u;
+                sink.emit(0, r, P);
             }
         }
         if (S.env_present) {
@@ -109,34 +110,36 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
             EnvSample e = sample_env(S, x0, x1);
             float ndl = dot(nf, e.dir);
             if (e.pdf > 0.0f && ndl > 0.0f) {
-                R.emit_e = true;
                 float pb = ndl * (1.0f / kPi);
                 bool fire = mis_on && (b + 1u) < F.max_bounces;
-                R.erec.o = hp + nf * 1e-3f;
-                R.erec.d = e.dir;
-                R.erec.t_max = 1e30f;
-                R.erec.pre = P.T * e.color;
-                R.erec.ndl = ndl;
-                R.erec.brdf = sf.albedo / kPi;
-                R.erec.pdf = e.pdf;
-                R.erec.w = fire ? mis_balance(e.pdf, pb) : 1.0f;
-                R.erec.clamp = true;
+                ShadowRec r;
+                r.o = hp + nf * 1e-3f;
+                r.d = e.dir;
+                r.t_max = 1e30f;
+                r.pre = P.T * e.color;
+                r.ndl = ndl;
+                r.brdf = sf.albedo / kPi;
+                r.pdf = e.pdf;
+                r.w = fire ? mis_balance(e.pdf, pb) : 1.0f;
+                r.clamp = true;
+                sink.emit(1, r, P);
             }
             P.skip_sky = !mis_on;
         } else {
             // No env map: ambient albedo * skyColorBase(nf) (PathTrace.slang:6667),
             // added after the light NEE term.
             float tt = clampf(0.5f * (nf.y + 1.0f), 0.0f, 1.0f);
-            R.emit_e = true;
-            R.erec.o = hp;
-            R.erec.d = f3{0.0f, 1.0f, 0.0f};
-            R.erec.t_max = 0.0f;
-            R.erec.pre = (P.T * sf.albedo) * lerp3(f3{0.5f, 0.55f, 0.62f}, f3{1.0f, 1.0f, 1.0f}, tt);
-            R.erec.ndl = 0.0f;
-            R.erec.brdf = f3{0.0f, 0.0f, 0.0f};
-            R.erec.pdf = -1.0f;
-            R.erec.w = 0.0f;
-            R.erec.clamp = false;
+            ShadowRec r;
+            r.o = hp;
+            r.d = f3{0.0f, 1.0f, 0.0f};
+            r.t_max = 0.0f;
+            r.pre = (P.T * sf.albedo) * lerp3(f3{0.5f, 0.55f, 0.62f}, f3{1.0f, 1.0f, 1.0f}, tt);
+            r.ndl = 0.0f;
+            r.brdf = f3{0.0f, 0.0f, 0.0f};
+            r.pdf = -1.0f;
+            r.w = 0.0f;
+            r.clamp = false;
+            sink.emit(1, r, P);
             P.skip_sky = true;
         }
         P.T = P.T * sf.albedo;

@@ -25,10 +25,12 @@ constexpr uint32_t kMaxGridBlocks = 4096; // grid-stride cap: 1M threads
 // counter slots
 constexpr uint32_t kCtrRays = 0;          // + parity: ray queue lengths
 constexpr uint32_t kCtrFinished = 2;
-constexpr uint32_t kCtrShadow = 3;        // + kind: shadow queue lengths
 constexpr uint32_t kCounterSlots = 8;
 
-constexpr uint32_t kClampBit = 0x80000000u;   // in shadow records' pixel word
+// Shadow records live at the slot of the path that formed them (no
+// compaction); the pixel word carries these flags.
+constexpr uint32_t kClampBit = 0x80000000u;
+constexpr uint32_t kValidBit = 0x40000000u;
 
 // Path state, indexed by pixel (SoA, 16 B rows).
 struct PathState {
@@ -47,10 +49,10 @@ struct RayQueue {
     float4* d;        // direction.xyz, pixel index bits
 };
 
-// Shadow queue: one transmittance query + the factors of its contribution.
+// Shadow queries of one iteration, indexed by the slot of the emitting path.
 struct ShadowQueue {
     float4* o;        // origin.xyz, t_max
-    float4* d;        // direction.xyz, pixel | kClampBit
+    float4* d;        // direction.xyz, pixel | kValidBit | kClampBit
     float4* a;        // throughput*Le (or *env colour).xyz, n_dot_l
     float4* b;        // brdf.xyz, pdf (< 0: no-trace ambient record)
     float* w;         // MIS weight (env queue)
@@ -75,6 +77,7 @@ struct Wavefront {
     uint2* hits = nullptr;    // (t bits, code) per slot
     ShadowQueue sq[2]{};      // 0: light NEE, 1: env NEE
     uint32_t* finished = nullptr;
+    uint8_t* regen = nullptr;         // per pixel: next sample/frame pending
     uint32_t* counters = nullptr;
     uint32_t* spill = nullptr;
     uint32_t spill_depth = 0;

@@ -97,8 +97,6 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint
         // counters written by this iteration's shade / finish
         ctr[kCtrRays + (in_par ^ 1u)] = 0u;
         ctr[kCtrFinished] = 0u;
-        ctr[kCtrShadow + 0] = 0u;
-        ctr[kCtrShadow + 1] = 0u;
     }
     Stack stk = make_stack(lds, spill, spill_depth);
     TraceCounters tc{0u, 0u, 0u};
@@ -114,14 +112,22 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint
     }
 }
 
-__device__ __forceinline__ void put_shadow(ShadowQueue& q, uint32_t slot, uint32_t pix, const ShadowRec& r) {
-    const uint32_t tag = pix | (r.clamp ? kClampBit : 0u);
-    q.o[slot] = make_float4(r.o.x, r.o.y, r.o.z, r.t_max);
-    q.d[slot] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(tag));
-    q.a[slot] = make_float4(r.pre.x, r.pre.y, r.pre.z, r.ndl);
-    q.b[slot] = make_float4(r.brdf.x, r.brdf.y, r.brdf.z, r.pdf);
-    q.w[slot] = r.w;
-}
+// Writes a path's shadow queries at its own slot of the iteration.
+struct SlotSink {
+    ShadowQueue* q;
+    uint32_t slot, pix;
+    bool emitted[2];
+    __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState&) {
+        ShadowQueue& s = q[kind];
+        const uint32_t tag = pix | kValidBit | (r.clamp ? kClampBit : 0u);
+        s.o[slot] = make_float4(r.o.x, r.o.y, r.o.z, r.t_max);
+        s.d[slot] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(tag));
+        s.a[slot] = make_float4(r.pre.x, r.pre.y, r.pre.z, r.ndl);
+        s.b[slot] = make_float4(r.brdf.x, r.brdf.y, r.brdf.z, r.pdf);
+        if (kind == 1) s.w[slot] = r.w;
+        emitted[kind] = true;
+    }
+};
 
 __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint32_t in_par, RayQueue qin,
                                                   RayQueue qout, uint32_t* ctr, const uint2* hits, PathState ps,
@@ -131,6 +137,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t n_primary = 0, n_ext = 0;
+    ShadowQueue sq[2] = {sl, se};
     for (uint32_t i = gid; i < n; i += stride) {
         const float4 o4 = qin.o[i], d4 = qin.d[i];
         const uint32_t pix = __float_as_uint(d4.w);
@@ -141,7 +148,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint
         const uint32_t flags = __float_as_uint(Lv.w);
         PathVertexState P{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[pix]};
         VertexResult R;
-        shade_vertex(S, F, b, xyz(o4), xyz(d4), HitRec{__uint_as_float(hr.x), hr.y}, P, R);
+        SlotSink sink{sq, i, pix, {false, false}};
+        shade_vertex(S, F, b, xyz(o4), xyz(d4), HitRec{__uint_as_float(hr.x), hr.y}, P, R, sink);
+        for (int k = 0; k < 2; ++k)
+            if (!sink.emitted[k]) sq[k].d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
         if (b == 0u) n_primary++;
         else n_ext++;
         const uint32_t nflags = (P.skip_sky ? 1u : 0u) | (P.prev_lambert ? 2u : 0u);
@@ -157,30 +167,28 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint
         }
         slot = wave_append(&ctr[kCtrFinished], !cont);
         if (!cont) finished[slot] = pix;
-        slot = wave_append(&ctr[kCtrShadow + 0], R.emit_l);
-        if (R.emit_l) put_shadow(sl, slot, pix, R.lrec);
-        slot = wave_append(&ctr[kCtrShadow + 1], R.emit_e);
-        if (R.emit_e) put_shadow(se, slot, pix, R.erec);
     }
     add_stat(&st->primary, n_primary);
     add_stat(&st->extension, n_ext);
 }
 
 template <int KIND, bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, ShadowQueue q, const uint32_t* ctr,
-                                                   PathState ps, uint32_t* spill, uint32_t spill_depth,
-                                                   DeviceStats* st) {
+__global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uint32_t in_par, ShadowQueue q,
+                                                   const uint32_t* ctr, PathState ps, uint32_t* spill,
+                                                   uint32_t spill_depth, DeviceStats* st) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
-    const uint32_t n = ctr[kCtrShadow + KIND];
+    const uint32_t n = ctr[kCtrRays + in_par];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     Stack stk = make_stack(lds, spill, spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     uint32_t steps = 0;
     for (uint32_t i = gid; i < n; i += stride) {
-        const float4 o4 = q.o[i], d4 = q.d[i], a4 = q.a[i], b4 = q.b[i];
+        const float4 d4 = q.d[i];
         const uint32_t tag = __float_as_uint(d4.w);
-        const uint32_t pix = tag & ~kClampBit;
+        if (!(tag & kValidBit)) continue;
+        const float4 o4 = q.o[i], a4 = q.a[i], b4 = q.b[i];
+        const uint32_t pix = tag & ~(kClampBit | kValidBit);
         const float4 Lv = ps.L[pix];
         f3 L = xyz(Lv);
         if (KIND == 1 && b4.w < 0.0f) {
@@ -201,50 +209,65 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, Sha
     }
 }
 
+// Sample / frame bookkeeping of the paths that ended this iteration.  The next
+// sample or frame of a pixel is only flagged here; k_regen issues the camera
+// rays in 8x8 tile order so the new waves stay coherent.
 // DEFERRED: one frame; its value goes to ps.F for k_accumulate.
 template <bool DEFERRED>
-__global__ void __launch_bounds__(kBlock) k_finish(FrameArgs F, uint32_t out_par, PathState ps, float4* accum,
-                                                   uint32_t* ctr, const uint32_t* finished, RayQueue qout) {
+__global__ void __launch_bounds__(kBlock) k_finish(FrameArgs F, PathState ps, float4* accum, const uint32_t* ctr,
+                                                   const uint32_t* finished, uint8_t* regen) {
     const uint32_t n = ctr[kCtrFinished];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = gid - (threadIdx.x & 63u); base < n; base += stride) {
-        const uint32_t i = base + (threadIdx.x & 63u);
-        const bool active = i < n;
-        bool regen = false;
-        uint32_t p = 0u, x = 0u, y = 0u, seed = 0u, nmeta = 0u;
-        if (active) {
-            p = finished[i];
-            const uint32_t meta = ps.meta[p];
-            const uint32_t s = (meta >> 8) & 0xFFu, k = meta >> 16;
-            const f3 L = xyz(ps.L[p]);
-            const f3 sum = (s == 0u) ? L : xyz(ps.F[p]) + L;
-            x = p % F.width;
-            y = p / F.width;
-            if (s + 1u < F.spp) {
-                ps.F[p] = make_float4(sum.x, sum.y, sum.z, 0.0f);
-                seed = ps.rng[p];   // the next sample continues the pixel's stream
-                nmeta = meta_pack(0u, s + 1u, k);
-                regen = true;
-            } else {
-                const f3 v = frame_value(F, sum);
-                if (DEFERRED) {
-                    ps.F[p] = make_float4(v.x, v.y, v.z, 0.0f);
-                } else {
-                    const bool reset = (k == 0u) && F.reset_accum != 0u;
-                    accum[p] = accumulate(F, reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : accum[p], reset, v);
-                    if (k + 1u < F.frame_count) {
-                        seed = pixel_seed(F, x, y, F.frame_index + k + 1u);
-                        nmeta = meta_pack(0u, 0u, k + 1u);
-                        regen = true;
-                    }
-                }
-            }
+    for (uint32_t i = gid; i < n; i += stride) {
+        const uint32_t p = finished[i];
+        const uint32_t meta = ps.meta[p];
+        const uint32_t s = (meta >> 8) & 0xFFu, k = meta >> 16;
+        const f3 L = xyz(ps.L[p]);
+        const f3 sum = (s == 0u) ? L : xyz(ps.F[p]) + L;
+        if (s + 1u < F.spp) {
+            ps.F[p] = make_float4(sum.x, sum.y, sum.z, 0.0f);
+            ps.meta[p] = meta_pack(0u, s + 1u, k);   // the next sample continues the pixel's stream
+            regen[p] = 1u;
+            continue;
         }
-        const uint32_t slot = wave_append(&ctr[kCtrRays + out_par], regen);
-        if (regen) {
+        const f3 v = frame_value(F, sum);
+        if (DEFERRED) {
+            ps.F[p] = make_float4(v.x, v.y, v.z, 0.0f);
+            continue;
+        }
+        const bool reset = (k == 0u) && F.reset_accum != 0u;
+        accum[p] = accumulate(F, reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : accum[p], reset, v);
+        if (k + 1u < F.frame_count) {
+            const uint32_t x = p % F.width, y = p / F.width;
+            ps.rng[p] = pixel_seed(F, x, y, F.frame_index + k + 1u);
+            ps.meta[p] = meta_pack(0u, 0u, k + 1u);
+            regen[p] = 1u;
+        }
+    }
+}
+
+// Camera rays of the flagged pixels, appended in 8x8 tile order.
+__global__ void __launch_bounds__(kBlock) k_regen(FrameArgs F, uint32_t out_par, PathState ps, uint32_t* ctr,
+                                                  uint8_t* regen, RayQueue qout) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t tiles_x = (F.width + 7u) / 8u;
+    const uint32_t tiles_y = (F.height + 7u) / 8u;
+    const uint32_t nslots = tiles_x * tiles_y * 64u;
+    // wave-uniform trip count: stride and nslots are multiples of 64
+    for (uint32_t qs = gid; qs < nslots; qs += stride) {
+        const uint32_t tile = qs >> 6, lane = qs & 63u;
+        const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
+        const bool inside = x < F.width && y < F.height;
+        const uint32_t p = y * F.width + x;
+        const bool go = inside && regen[p] != 0u;
+        const uint32_t slot = wave_append(&ctr[kCtrRays + out_par], go);
+        if (go) {
+            regen[p] = 0u;
+            uint32_t seed = ps.rng[p];
             f3 rd = camera_ray(F, x, y, seed);
-            start_path(ps, qout, slot, p, F.pos, rd, seed, nmeta);
+            start_path(ps, qout, slot, p, F.pos, rd, seed, ps.meta[p]);
         }
     }
 }
@@ -338,16 +361,19 @@ static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W
     }
     hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, stream, S, F, par, W.rq[par], W.rq[par ^ 1u], W.counters,
                        W.hits, W.ps, W.sq[0], W.sq[1], W.finished, W.stats);
-    hipLaunchKernelGGL((k_shadow<0, COUNT>), dim3(g), dim3(kBlock), 0, stream, S, F, W.sq[0], W.counters, W.ps,
-                       W.spill, W.spill_depth, W.stats);
-    hipLaunchKernelGGL((k_shadow<1, COUNT>), dim3(g), dim3(kBlock), 0, stream, S, F, W.sq[1], W.counters, W.ps,
-                       W.spill, W.spill_depth, W.stats);
+    hipLaunchKernelGGL((k_shadow<0, COUNT>), dim3(g), dim3(kBlock), 0, stream, S, F, par, W.sq[0], W.counters,
+                       W.ps, W.spill, W.spill_depth, W.stats);
+    hipLaunchKernelGGL((k_shadow<1, COUNT>), dim3(g), dim3(kBlock), 0, stream, S, F, par, W.sq[1], W.counters,
+                       W.ps, W.spill, W.spill_depth, W.stats);
     if (deferred)
-        hipLaunchKernelGGL(k_finish<true>, dim3(g), dim3(kBlock), 0, stream, F, par ^ 1u, W.ps, accum, W.counters,
-                           W.finished, W.rq[par ^ 1u]);
+        hipLaunchKernelGGL(k_finish<true>, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum, W.counters, W.finished,
+                           W.regen);
     else
-        hipLaunchKernelGGL(k_finish<false>, dim3(g), dim3(kBlock), 0, stream, F, par ^ 1u, W.ps, accum, W.counters,
-                           W.finished, W.rq[par ^ 1u]);
+        hipLaunchKernelGGL(k_finish<false>, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum, W.counters,
+                           W.finished, W.regen);
+    if (F.spp > 1u || F.frame_count > 1u)
+        hipLaunchKernelGGL(k_regen, dim3(g), dim3(kBlock), 0, stream, F, par ^ 1u, W.ps, W.counters, W.regen,
+                           W.rq[par ^ 1u]);
     return hipGetLastError();
 }
 
