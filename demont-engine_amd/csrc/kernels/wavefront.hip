@@ -28,6 +28,28 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool pred) {
     return base + below;
 }
 
+// Workgroup-aggregated append: every thread of the block calls it (block-
+// uniform loops); one global atomic per workgroup and counter.
+__device__ __forceinline__ uint32_t block_append(uint32_t* ctr, bool pred, uint32_t* s_cnt) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kBlock / 64u; ++w) {
+            const uint32_t c = s_cnt[w];
+            s_cnt[w] = total;
+            total += c;
+        }
+        s_cnt[kBlock / 64u] = total ? atomicAdd(ctr, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_cnt[kBlock / 64u] + s_cnt[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();   // s_cnt is reused by the next call
+    return slot;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -93,11 +115,7 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint
     const uint32_t n = ctr[kCtrRays + in_par];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    if (gid == 0) {
-        // counters written by this iteration's shade / finish
-        ctr[kCtrRays + (in_par ^ 1u)] = 0u;
-        ctr[kCtrFinished] = 0u;
-    }
+    if (gid == 0) ctr[kCtrRays + (in_par ^ 1u)] = 0u;   // the queue this iteration fills
     Stack stk = make_stack(lds, spill, spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     for (uint32_t i = gid; i < n; i += stride) {
@@ -131,42 +149,46 @@ struct SlotSink {
 
 __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint32_t in_par, RayQueue qin,
                                                   RayQueue qout, uint32_t* ctr, const uint2* hits, PathState ps,
-                                                  ShadowQueue sl, ShadowQueue se, uint32_t* finished,
+                                                  ShadowQueue sl, ShadowQueue se, uint32_t* ended,
                                                   DeviceStats* st) {
+    __shared__ uint32_t s_cnt[kBlock / 64u + 1u];
     const uint32_t n = ctr[kCtrRays + in_par];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t n_primary = 0, n_ext = 0;
     ShadowQueue sq[2] = {sl, se};
-    for (uint32_t i = gid; i < n; i += stride) {
-        const float4 o4 = qin.o[i], d4 = qin.d[i];
-        const uint32_t pix = __float_as_uint(d4.w);
-        const uint2 hr = hits[i];
-        const float4 Tv = ps.T[pix], Lv = ps.L[pix];
-        const uint32_t meta = ps.meta[pix];
-        const uint32_t b = meta & 0xFFu;
-        const uint32_t flags = __float_as_uint(Lv.w);
-        PathVertexState P{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[pix]};
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {   // block-uniform
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false;
+        uint32_t pix = 0u;
         VertexResult R;
-        SlotSink sink{sq, i, pix, {false, false}};
-        shade_vertex(S, F, b, xyz(o4), xyz(d4), HitRec{__uint_as_float(hr.x), hr.y}, P, R, sink);
-        for (int k = 0; k < 2; ++k)
-            if (!sink.emitted[k]) sq[k].d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-        if (b == 0u) n_primary++;
-        else n_ext++;
-        const uint32_t nflags = (P.skip_sky ? 1u : 0u) | (P.prev_lambert ? 2u : 0u);
-        ps.T[pix] = make_float4(P.T.x, P.T.y, P.T.z, P.prev_pdf);
-        ps.L[pix] = make_float4(P.L.x, P.L.y, P.L.z, __uint_as_float(nflags));
-        ps.rng[pix] = P.seed;
-        const bool cont = R.alive && (b + 1u) < F.max_bounces;
-        if (cont) ps.meta[pix] = meta + 1u;
-        uint32_t slot = wave_append(&ctr[kCtrRays + (in_par ^ 1u)], cont);
+        if (i < n) {
+            const float4 o4 = qin.o[i], d4 = qin.d[i];
+            pix = __float_as_uint(d4.w);
+            const uint2 hr = hits[i];
+            const float4 Tv = ps.T[pix], Lv = ps.L[pix];
+            const uint32_t meta = ps.meta[pix];
+            const uint32_t b = meta & 0xFFu;
+            const uint32_t flags = __float_as_uint(Lv.w);
+            PathVertexState P{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[pix]};
+            SlotSink sink{sq, i, pix, {false, false}};
+            shade_vertex(S, F, b, xyz(o4), xyz(d4), HitRec{__uint_as_float(hr.x), hr.y}, P, R, sink);
+            for (int k = 0; k < 2; ++k)
+                if (!sink.emitted[k]) sq[k].d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+            if (b == 0u) n_primary++;
+            else n_ext++;
+            const uint32_t nflags = (P.skip_sky ? 1u : 0u) | (P.prev_lambert ? 2u : 0u);
+            ps.T[pix] = make_float4(P.T.x, P.T.y, P.T.z, P.prev_pdf);
+            ps.L[pix] = make_float4(P.L.x, P.L.y, P.L.z, __uint_as_float(nflags));
+            ps.rng[pix] = P.seed;
+            cont = R.alive && (b + 1u) < F.max_bounces;
+            if (cont) ps.meta[pix] = meta + 1u;
+            ended[i] = cont ? kMiss : pix;
+        }
+        const uint32_t slot = block_append(&ctr[kCtrRays + (in_par ^ 1u)], cont, s_cnt);
         if (cont) {
             qout.o[slot] = make_float4(R.no.x, R.no.y, R.no.z, 0.0f);
             qout.d[slot] = make_float4(R.nd.x, R.nd.y, R.nd.z, __uint_as_float(pix));
         }
-        slot = wave_append(&ctr[kCtrFinished], !cont);
-        if (!cont) finished[slot] = pix;
     }
     add_stat(&st->primary, n_primary);
     add_stat(&st->extension, n_ext);
@@ -214,13 +236,14 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
 // rays in 8x8 tile order so the new waves stay coherent.
 // DEFERRED: one frame; its value goes to ps.F for k_accumulate.
 template <bool DEFERRED>
-__global__ void __launch_bounds__(kBlock) k_finish(FrameArgs F, PathState ps, float4* accum, const uint32_t* ctr,
-                                                   const uint32_t* finished, uint8_t* regen) {
-    const uint32_t n = ctr[kCtrFinished];
+__global__ void __launch_bounds__(kBlock) k_finish(FrameArgs F, uint32_t in_par, PathState ps, float4* accum,
+                                                   const uint32_t* ctr, const uint32_t* ended, uint8_t* regen) {
+    const uint32_t n = ctr[kCtrRays + in_par];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = gid; i < n; i += stride) {
-        const uint32_t p = finished[i];
+        const uint32_t p = ended[i];
+        if (p == kMiss) continue;
         const uint32_t meta = ps.meta[p];
         const uint32_t s = (meta >> 8) & 0xFFu, k = meta >> 16;
         const f3 L = xyz(ps.L[p]);
@@ -250,19 +273,20 @@ __global__ void __launch_bounds__(kBlock) k_finish(FrameArgs F, PathState ps, fl
 // Camera rays of the flagged pixels, appended in 8x8 tile order.
 __global__ void __launch_bounds__(kBlock) k_regen(FrameArgs F, uint32_t out_par, PathState ps, uint32_t* ctr,
                                                   uint8_t* regen, RayQueue qout) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t tiles_x = (F.width + 7u) / 8u;
     const uint32_t tiles_y = (F.height + 7u) / 8u;
     const uint32_t nslots = tiles_x * tiles_y * 64u;
-    // wave-uniform trip count: stride and nslots are multiples of 64
-    for (uint32_t qs = gid; qs < nslots; qs += stride) {
+    __shared__ uint32_t s_cnt[kBlock / 64u + 1u];
+    // block-uniform trip count
+    for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < nslots; q0 += stride) {
+        const uint32_t qs = q0 + threadIdx.x;
         const uint32_t tile = qs >> 6, lane = qs & 63u;
         const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
-        const bool inside = x < F.width && y < F.height;
+        const bool inside = qs < nslots && x < F.width && y < F.height;
         const uint32_t p = y * F.width + x;
         const bool go = inside && regen[p] != 0u;
-        const uint32_t slot = wave_append(&ctr[kCtrRays + out_par], go);
+        const uint32_t slot = block_append(&ctr[kCtrRays + out_par], go, s_cnt);
         if (go) {
             regen[p] = 0u;
             uint32_t seed = ps.rng[p];
@@ -366,10 +390,10 @@ static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W
     hipLaunchKernelGGL((k_shadow<1, COUNT>), dim3(g), dim3(kBlock), 0, stream, S, F, par, W.sq[1], W.counters,
                        W.ps, W.spill, W.spill_depth, W.stats);
     if (deferred)
-        hipLaunchKernelGGL(k_finish<true>, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum, W.counters, W.finished,
-                           W.regen);
+        hipLaunchKernelGGL(k_finish<true>, dim3(g), dim3(kBlock), 0, stream, F, par, W.ps, accum, W.counters,
+                           W.finished, W.regen);
     else
-        hipLaunchKernelGGL(k_finish<false>, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum, W.counters,
+        hipLaunchKernelGGL(k_finish<false>, dim3(g), dim3(kBlock), 0, stream, F, par, W.ps, accum, W.counters,
                            W.finished, W.regen);
     if (F.spp > 1u || F.frame_count > 1u)
         hipLaunchKernelGGL(k_regen, dim3(g), dim3(kBlock), 0, stream, F, par ^ 1u, W.ps, W.counters, W.regen,
