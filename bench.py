@@ -6,12 +6,10 @@ small triangles in [-5,5]^3, one quad light, 512x256 lognormal env map,
 1920x1080, 1 spp, 8 bounces, binned-SAH BVH4 built on the host (not timed).
 
 A step is one frame (1920x1080, 1 spp, up to 8 bounces, accumulated into the
-running mean).  The K timed frames are rendered by one dmt_render_frames call:
-the streaming wavefront (extend, shade, light shadow, env shadow, finish per
-iteration) regenerates each pixel's next frame as soon as its path ends, and
-its image is bit-identical to K dmt_render calls (tested).  value = (primary +
+running mean), issued as one dmt_render call; the library keeps three frames in
+flight on three HIP streams (the accumulation passes stay ordered frame after
+frame, so the image equals the sequential result).  value = (primary +
 extension + shadow rays) / wall time of the K frames, inputs resident in HBM.
-The per-frame-call rate is logged beside it.
 
 roofline: the closest-hit (extend) kernel, HBM-bound by design (no dense
 contraction).  Algorithmic bytes per launch = node visits x 128 B (BVH4 node)
@@ -95,7 +93,7 @@ def main():
         scene.set_profiling(False)
         return dt, st
 
-    # per-frame calls (dmt_render once per frame, two frames in flight)
+    # K frames, one dmt_render call each (the library keeps three in flight)
     def per_frame():
         for k in range(args.steps):
             f = frame.copy()
@@ -104,11 +102,7 @@ def main():
             scene.render(f)
 
     run_frames(0, args.warmup)
-    dt1, st1 = timed(per_frame)
-    mr1 = (st1["primary_rays"] + st1["extension_rays"] + st1["shadow_rays"]) / dt1 / 1e6
-    log(f"[bench] per-frame dmt_render x{args.steps}: {dt1 * 1e3:.1f} ms, {mr1:.1f} Mrays/s")
-    # one dmt_render_frames call of K frames (the headline number)
-    dt, st = timed(lambda: run_frames(args.warmup + args.steps, args.steps))
+    dt, st = timed(per_frame)
     rays = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
     mrays = rays / dt / 1e6
     log(f"[bench] {args.steps} frames in {dt * 1e3:.1f} ms: {mrays:.1f} Mrays/s "
@@ -186,8 +180,6 @@ def main():
                      "kernel": "k_extend (closest hit)", "bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_ms": round(avg_launch_ms, 5)},
         "cpu_baseline": cpu,
-        "per_frame_calls": {"value": round(mr1, 2), "unit": "Mrays/s",
-                            "ms_per_step": round(dt1 * 1e3 / args.steps, 4)},
     }
     print(json.dumps(line), flush=True)
     scene.close()

@@ -217,7 +217,7 @@ dmt_status check_frame(const dmt_frame_params* f) {
 FrameArgs frame_args(const dmt_frame_params* f) {
     return make_frame_args(f->pos_fovtan, f->fwd_aspect, f->right_xyz, f->up_xyz, f->width, f->height,
                            f->frame_index, f->spp, f->max_bounces, f->reset_accum, f->mis_enabled,
-                           f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha, 1u);
+                           f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha);
 }
 
 dmt_status harvest_events(dmt_scene* s) {
@@ -390,52 +390,46 @@ void dmt_scene_destroy(dmt_scene* s) {
 
 namespace {
 
-// Renders frame_count frames on one wavefront set.  One frame alternates sets
-// and streams (two frames in flight, accumulation ordered by events); a
-// batch of frames runs on set 0 after everything before it.
+// Renders frame_count consecutive frames, each on the next wavefront set and
+// its stream: frames stay in flight while the accumulation passes are ordered
+// frame after frame by events.
 dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_count) {
     if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
     dmt_status st = check_device(s->dev);
     if (st) return st;
     if ((st = check_frame(f))) return st;
     if (frame_count == 0) return fail(DMT_ERR_INVALID_ARG, "frame_count must be >= 1");
-    if (frame_count > 65535u) return fail(DMT_ERR_LIMIT, "frame_count must be <= 65535");
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
-    FrameArgs F = frame_args(f);
-    F.frame_count = frame_count;
-    if (resized) F.reset_accum = 1u;
     if (f->max_bounces == 0) return DMT_OK;
-    const bool single = frame_count == 1u;
-    const int set = single ? s->next_set : 0;
-    hipStream_t stream = set_stream(s, set);
-    hipEvent_t after = s->last_set >= 0 ? s->acc_done[s->last_set] : nullptr;
-    LaunchOptions o;
-    o.poll_events = s->poll[set];
-    if (single) {
-        o.accum_after = after;
-    } else if (after) {
-        hipError_t e = hipStreamWaitEvent(stream, after, 0);
-        if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
-    }
-    if (s->profiling) {
-        const size_t bound = 2u * (size_t)frame_count * f->spp * f->max_bounces;
-        while (s->ev_pool.size() < s->ev_used + bound) {
-            hipEvent_t e;
-            hipError_t he = hipEventCreate(&e);
-            if (he != hipSuccess) return hip_fail(he, "hipEventCreate");
-            s->ev_pool.push_back(e);
+    for (uint32_t k = 0; k < frame_count; ++k) {
+        FrameArgs F = frame_args(f);
+        F.frame_index = f->frame_index + k;
+        F.reset_accum = (k == 0u && (f->reset_accum || resized)) ? 1u : 0u;
+        const int set = s->next_set;
+        hipStream_t stream = set_stream(s, set);
+        LaunchOptions o;
+        o.poll_events = s->poll[set];
+        o.accum_after = s->last_set >= 0 ? s->acc_done[s->last_set] : nullptr;
+        if (s->profiling) {
+            const size_t bound = 2u * (size_t)f->spp * f->max_bounces;
+            while (s->ev_pool.size() < s->ev_used + bound) {
+                hipEvent_t e;
+                hipError_t he = hipEventCreate(&e);
+                if (he != hipSuccess) return hip_fail(he, "hipEventCreate");
+                s->ev_pool.push_back(e);
+            }
+            o.extend_events = s->ev_pool.data() + s->ev_used;
+            o.max_events = (uint32_t)bound;
         }
-        o.extend_events = s->ev_pool.data() + s->ev_used;
-        o.max_events = (uint32_t)bound;
+        hipError_t e = launch_frame(s->view, F, s->wf[set], s->accum, stream, o);
+        s->ev_used += o.events_used;
+        if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
+        if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "launch_frame");
+        s->last_set = set;
+        s->next_set = (set + 1) % (int)kFrameSets;
+        s->frames++;
     }
-    hipError_t e = launch_frames(s->view, F, s->wf[set], s->accum, stream, o);
-    s->ev_used += o.events_used;
-    if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
-    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "launch_frames");
-    s->last_set = set;
-    s->next_set = (set + 1) % (int)kFrameSets;
-    s->frames += frame_count;
     return DMT_OK;
 }
 
@@ -556,7 +550,7 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     F.reset_accum = 1u;
     LaunchOptions o;
     o.counting = true;
-    if (e == hipSuccess) e = launch_frames(s->view, F, W, scratch, s->dev->stream, o);
+    if (e == hipSuccess) e = launch_frame(s->view, F, W, scratch, s->dev->stream, o);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);

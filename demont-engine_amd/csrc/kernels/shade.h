@@ -21,7 +21,6 @@ struct FrameArgs {
     uint32_t refract_bounces;
     float firefly_clamp;
     float ema_alpha;
-    uint32_t frame_count;       // frames rendered by one launch_frames call
 };
 
 // clampIndirect (PathTrace.slang:3367-3371).

@@ -1,15 +1,13 @@
-// Streaming wavefront path tracer: state, queues and launchers.
+// Wavefront path tracer: state, queues and launchers.
 //
-// Every live path carries its own (bounce, sample, frame).  One iteration is
-//   k_extend -> k_shade -> k_shadow<light> -> k_shadow<env> -> k_finish
-// and k_finish turns each path that ended into (a) the next sample of the same
+// Every live path carries its own bounce and sample index.  One iteration is
+//   k_extend -> k_shade -> k_shadow<light> -> k_shadow<env> -> k_finish [-> k_regen]
+// and k_finish turns each path that ended into the next sample of the same
 // pixel, continuing its PCG state as the reference's per-pixel spp loop does
-// (PathTrace.slang:5722), or (b) the frame's final value and, when more frames
-// were asked for, the first sample of the pixel's next frame.  A pixel's
-// samples and frames therefore run strictly in order and the image equals the
-// frame-by-frame result, while the queues stay full until the batch drains.
-// Queue lengths live in device memory; the host never waits inside a batch
-// except to stop launching once the live queue is empty.
+// (PathTrace.slang:5722), or into the pixel's frame value.  A pixel's samples
+// therefore run strictly in order while the queue stays full until the frame
+// drains.  Queue lengths live in device memory; the host never waits inside a
+// frame except to stop launching once the live queue is empty.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,9 +34,9 @@ constexpr uint32_t kValidBit = 0x40000000u;
 struct PathState {
     float4* T;        // throughput.xyz, prev_lambert_brdf_pdf
     float4* L;        // radiance.xyz, flags (bit0 skip_sky_on_miss, bit1 prev_was_lambert_nee)
-    float4* F;        // sample sum of the frame; the frame value in deferred mode
+    float4* F;        // sample sum of the frame, then the frame value for k_accumulate
     uint32_t* rng;
-    uint32_t* meta;   // bounce | sample << 8 | frame << 16
+    uint32_t* meta;   // bounce | sample << 8
 };
 
 __host__ __device__ inline uint32_t meta_pack(uint32_t b, uint32_t s, uint32_t k) { return b | (s << 8) | (k << 16); }
@@ -76,8 +74,8 @@ struct Wavefront {
     RayQueue rq[2]{};
     uint2* hits = nullptr;    // (t bits, code) per slot
     ShadowQueue sq[2]{};      // 0: light NEE, 1: env NEE
-    uint32_t* finished = nullptr;
-    uint8_t* regen = nullptr;         // per pixel: next sample/frame pending
+    uint32_t* finished = nullptr;     // per slot: pixel of a path that ended, or kMiss
+    uint8_t* regen = nullptr;         // per pixel: next sample pending
     uint32_t* counters = nullptr;
     uint32_t* spill = nullptr;
     uint32_t spill_depth = 0;
@@ -89,7 +87,7 @@ struct Wavefront {
 FrameArgs make_frame_args(const float pos_fovtan[4], const float fwd_aspect[4], const float right[4],
                           const float up[4], uint32_t width, uint32_t height, uint32_t frame_index, uint32_t spp,
                           uint32_t max_bounces, uint32_t reset_accum, uint32_t mis, uint32_t ltree,
-                          uint32_t refract, float clamp, float ema, uint32_t frame_count);
+                          uint32_t refract, float clamp, float ema);
 
 uint32_t grid_for(uint32_t n);
 
@@ -100,18 +98,17 @@ struct LaunchOptions {
     hipEvent_t* extend_events = nullptr;  // 2 per iteration, or null
     uint32_t max_events = 0;
     uint32_t events_used = 0;             // out
-    hipEvent_t accum_after = nullptr;     // deferred accumulation waits for it
+    hipEvent_t accum_after = nullptr;     // the accumulation pass waits for it
     bool counting = false;                // instrumented traversal
     hipEvent_t* poll_events = nullptr;    // kPollSlots events for the stop test
     uint32_t iterations = 0;              // out: iterations launched
 };
 
-// Render F.frame_count frames starting at F.frame_index into accum.  With one
-// frame, the frame's values are folded into accum by a final pass that waits
-// for opts.accum_after (the previous frame on another stream); with several,
-// each pixel folds its frames in order inside the batch.
-hipError_t launch_frames(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
-                         LaunchOptions& opts);
+// Render one frame into accum; the final pass that folds the frame's values
+// into accum waits for opts.accum_after (the previous frame on another
+// stream), keeping per-pixel frame order.
+hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
+                        LaunchOptions& opts);
 
 // Closest-hit queries: rays (8 floats each) in device memory.
 struct HitOut {

@@ -5,8 +5,9 @@
 //   k_shade       one vertex per live path: emits shadow queries, the next ray
 //                 or a "finished" entry
 //   k_shadow<k>   transmittance + contribution of light (k=0) / env (k=1) NEE
-//   k_finish      sample and frame bookkeeping of ended paths; regenerates
-//   k_accumulate  folds one frame's values into the accumulation buffer
+//   k_finish      sample bookkeeping of ended paths; the frame value per pixel
+//   k_regen       camera rays of the next samples, 8x8 tile order
+//   k_accumulate  folds the frame's values into the accumulation buffer
 //
 // Queue appends are wave-aggregated: one atomic per wave64 for each queue,
 // slots from a ballot prefix.  Queue order is not deterministic, the image
@@ -231,41 +232,28 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
     }
 }
 
-// Sample / frame bookkeeping of the paths that ended this iteration.  The next
-// sample or frame of a pixel is only flagged here; k_regen issues the camera
-// rays in 8x8 tile order so the new waves stay coherent.
-// DEFERRED: one frame; its value goes to ps.F for k_accumulate.
-template <bool DEFERRED>
-__global__ void __launch_bounds__(kBlock) k_finish(FrameArgs F, uint32_t in_par, PathState ps, float4* accum,
-                                                   const uint32_t* ctr, const uint32_t* ended, uint8_t* regen) {
+// Sample bookkeeping of the paths that ended this iteration: the next sample
+// of the pixel is flagged (k_regen issues it in 8x8 tile order so the new
+// waves stay coherent); after the last sample the frame's value goes to ps.F
+// for k_accumulate.
+__global__ void __launch_bounds__(kBlock) k_finish(FrameArgs F, uint32_t in_par, PathState ps, const uint32_t* ctr,
+                                                   const uint32_t* ended, uint8_t* regen) {
     const uint32_t n = ctr[kCtrRays + in_par];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = gid; i < n; i += stride) {
         const uint32_t p = ended[i];
         if (p == kMiss) continue;
-        const uint32_t meta = ps.meta[p];
-        const uint32_t s = (meta >> 8) & 0xFFu, k = meta >> 16;
+        const uint32_t s = (ps.meta[p] >> 8) & 0xFFu;
         const f3 L = xyz(ps.L[p]);
         const f3 sum = (s == 0u) ? L : xyz(ps.F[p]) + L;
         if (s + 1u < F.spp) {
             ps.F[p] = make_float4(sum.x, sum.y, sum.z, 0.0f);
-            ps.meta[p] = meta_pack(0u, s + 1u, k);   // the next sample continues the pixel's stream
+            ps.meta[p] = meta_pack(0u, s + 1u, 0u);   // the next sample continues the pixel's stream
             regen[p] = 1u;
-            continue;
-        }
-        const f3 v = frame_value(F, sum);
-        if (DEFERRED) {
+        } else {
+            const f3 v = frame_value(F, sum);
             ps.F[p] = make_float4(v.x, v.y, v.z, 0.0f);
-            continue;
-        }
-        const bool reset = (k == 0u) && F.reset_accum != 0u;
-        accum[p] = accumulate(F, reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : accum[p], reset, v);
-        if (k + 1u < F.frame_count) {
-            const uint32_t x = p % F.width, y = p / F.width;
-            ps.rng[p] = pixel_seed(F, x, y, F.frame_index + k + 1u);
-            ps.meta[p] = meta_pack(0u, 0u, k + 1u);
-            regen[p] = 1u;
         }
     }
 }
@@ -343,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_query(SceneView S, const float
 FrameArgs make_frame_args(const float pf[4], const float fa[4], const float rt[4], const float up[4], uint32_t width,
                           uint32_t height, uint32_t frame_index, uint32_t spp, uint32_t max_bounces,
                           uint32_t reset_accum, uint32_t mis, uint32_t ltree, uint32_t refract, float clamp,
-                          float ema, uint32_t frame_count) {
+                          float ema) {
     FrameArgs F;
     F.pos = f3{pf[0], pf[1], pf[2]};
     F.fovtan = pf[3];
@@ -362,7 +350,6 @@ FrameArgs make_frame_args(const float pf[4], const float fa[4], const float rt[4
     F.refract_bounces = refract;
     F.firefly_clamp = clamp;
     F.ema_alpha = ema;
-    F.frame_count = frame_count;
     return F;
 }
 
@@ -373,8 +360,8 @@ uint32_t grid_for(uint32_t n) {
 }
 
 template <bool COUNT>
-static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
-                            uint32_t par, uint32_t g, bool deferred, LaunchOptions& o) {
+static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W, hipStream_t stream, uint32_t par,
+                            uint32_t g, LaunchOptions& o) {
     const bool ev = o.extend_events && o.events_used + 2u <= o.max_events;
     if (ev) (void)hipEventRecord(o.extend_events[o.events_used], stream);
     hipLaunchKernelGGL(k_extend<COUNT>, dim3(g), dim3(kBlock), 0, stream, S, W.rq[par], W.counters, par, W.hits,
@@ -389,36 +376,31 @@ static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W
                        W.ps, W.spill, W.spill_depth, W.stats);
     hipLaunchKernelGGL((k_shadow<1, COUNT>), dim3(g), dim3(kBlock), 0, stream, S, F, par, W.sq[1], W.counters,
                        W.ps, W.spill, W.spill_depth, W.stats);
-    if (deferred)
-        hipLaunchKernelGGL(k_finish<true>, dim3(g), dim3(kBlock), 0, stream, F, par, W.ps, accum, W.counters,
-                           W.finished, W.regen);
-    else
-        hipLaunchKernelGGL(k_finish<false>, dim3(g), dim3(kBlock), 0, stream, F, par, W.ps, accum, W.counters,
-                           W.finished, W.regen);
-    if (F.spp > 1u || F.frame_count > 1u)
+    hipLaunchKernelGGL(k_finish, dim3(g), dim3(kBlock), 0, stream, F, par, W.ps, W.counters, W.finished, W.regen);
+    if (F.spp > 1u)
         hipLaunchKernelGGL(k_regen, dim3(g), dim3(kBlock), 0, stream, F, par ^ 1u, W.ps, W.counters, W.regen,
                            W.rq[par ^ 1u]);
     return hipGetLastError();
 }
 
-hipError_t launch_frames(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
-                         LaunchOptions& o) {
+hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
+                        LaunchOptions& o) {
     const uint32_t npix = F.width * F.height;
     const uint32_t g = grid_for(npix);
-    const bool deferred = F.frame_count == 1u;
     o.events_used = 0;
     o.iterations = 0;
     hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, W.rq[0], W.counters);
-    // Every path ends after at most max_bounces vertices, so the batch needs at
-    // most frames * spp * max_bounces iterations.  Short batches are launched
-    // in full; long ones stop once a snapshot of the live queue reads zero.
-    const uint64_t bound = (uint64_t)F.frame_count * F.spp * (F.max_bounces ? F.max_bounces : 1u);
+    // Every path ends after at most max_bounces vertices and a pixel's samples
+    // run in order, so the frame needs at most spp * max_bounces iterations.
+    // Short frames are launched in full; long ones stop once a snapshot of the
+    // live queue reads zero.
+    const uint64_t bound = (uint64_t)F.spp * (F.max_bounces ? F.max_bounces : 1u);
     const bool poll = bound > 24u && o.poll_events && W.host_count;
     uint32_t snap = 0;
     for (uint64_t it = 0; it < bound; ++it) {
         const uint32_t par = (uint32_t)(it & 1u);
-        hipError_t e = o.counting ? iteration<true>(S, F, W, accum, stream, par, g, deferred, o)
-                                  : iteration<false>(S, F, W, accum, stream, par, g, deferred, o);
+        hipError_t e = o.counting ? iteration<true>(S, F, W, stream, par, g, o)
+                                  : iteration<false>(S, F, W, stream, par, g, o);
         if (e != hipSuccess) return e;
         o.iterations++;
         if (poll && (it + 1u) % kPollChunk == 0u) {
@@ -435,10 +417,8 @@ hipError_t launch_frames(const SceneView& S, const FrameArgs& F, Wavefront& W, f
             }
         }
     }
-    if (deferred) {
-        if (o.accum_after) (void)hipStreamWaitEvent(stream, o.accum_after, 0);
-        hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum);
-    }
+    if (o.accum_after) (void)hipStreamWaitEvent(stream, o.accum_after, 0);
+    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum);
     return hipGetLastError();
 }
 

@@ -159,11 +159,9 @@ void       dmt_scene_destroy(dmt_scene* scene);
 dmt_status dmt_render(dmt_scene* scene, const dmt_frame_params* frame);
 
 /* Renders frame_count consecutive frames (frame_index, frame_index + 1, ...;
- * reset_accum applies to the first) into the accumulation buffer.  The image
- * is identical to frame_count dmt_render calls; paths are regenerated per
- * pixel as soon as the previous one ends, so queues stay full until the batch
- * drains.  The engine's progressive accumulation loop over Dispatch("pathtrace")
- * (Engine.cpp:12742-12945 -> :9582) issued as one call. */
+ * reset_accum applies to the first) into the accumulation buffer: the
+ * engine's progressive loop over Dispatch("pathtrace") (Engine.cpp:12742-12945
+ * -> :9582) issued as one call.  Identical to frame_count dmt_render calls. */
 dmt_status dmt_render_frames(dmt_scene* scene, const dmt_frame_params* first_frame, uint32_t frame_count);
 
 /* Replaces ReadbackTexture(accum_hdr) (SoftwareDevice.cpp:458-483):
