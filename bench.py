@@ -21,6 +21,15 @@ region.
 cpu_baseline: the CPU oracle (reference semantics, fp32) rendering whole frames
 of the same workload for about 10 s, on this host's cores (hw - 4 threads as
 the reference's software tracer uses, and all cores).
+
+--workload batch: the batched form.  64 scenes of the batched workload
+generator (resolution uniform in {128, 256, 512, 1024} squared, triangles
+log-uniform in 10k..2M, half with 64 spheres; 1 spp, depth 6).  The plain
+form renders each frame as 64 sequential per-scene dmt_render calls; the
+batched form renders each frame of all 64 scenes with one persistent-kernel
+launch replayed from a captured hipGraph, with parameters picked by
+dmt_batch_autotune.  Both run the same frames from frame 0; the images must be
+identical.  value = plain time / batched time over the timed frames.
 """
 import argparse
 import json
@@ -41,8 +50,10 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("c1", "batch"), default="c1")
+    ap.add_argument("--steps", type=int, default=None, help="timed frames (c1: 20, batch: 100)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed frames (c1: 3, batch: 2)")
+    ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -61,6 +72,12 @@ def main():
     if not torch.cuda.is_available():
         log("no GPU visible")
         sys.exit(1)
+    if args.workload == "batch":
+        args.steps = 100 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        return bench_batch(args, np, torch, dmt)
+    args.steps = 20 if args.steps is None else args.steps
+    args.warmup = 3 if args.warmup is None else args.warmup
     t0 = time.time()
     sd = dmt.generate(1, seed=args.seed)
     log(f"[bench] scene: {sd.indices.shape[0]} triangles, {sd.frame.width}x{sd.frame.height}, "
@@ -183,6 +200,101 @@ def main():
     }
     print(json.dumps(line), flush=True)
     scene.close()
+    dev.close()
+
+
+def bench_batch(args, np, torch, dmt):
+    t0 = time.time()
+    dev = dmt.Device(0)
+    scenes, frames = [], []
+    tris = pixels = 0
+    for i in range(args.scenes):
+        sd = dmt.generate_batch_scene(args.seed, i)
+        scenes.append(dmt.Scene(dev, sd))
+        frames.append(sd.frame.copy())
+        tris += int(sd.indices.shape[0])
+        pixels += sd.frame.width * sd.frame.height
+        del sd
+    log(f"[bench] {len(scenes)} scenes, {pixels / 1e6:.2f} M pixels, {tris / 1e6:.2f} M triangles "
+        f"(generate + build + upload {time.time() - t0:.1f}s)")
+
+    def sync():
+        dev.synchronize()
+        torch.cuda.synchronize()
+
+    # plain: each frame is 64 sequential per-scene launches
+    def plain(k0, n):
+        for k in range(k0, k0 + n):
+            for sc, f in zip(scenes, frames):
+                g = f.copy()
+                g.frame_index = f.frame_index + k
+                g.reset_accum = 1 if k == 0 else 0
+                sc.render(g)
+
+    plain(0, args.warmup)
+    sync()
+    for sc in scenes:
+        sc.reset_stats()
+    sync()
+    t0 = time.perf_counter()
+    plain(args.warmup, args.steps)
+    sync()
+    dt_plain = time.perf_counter() - t0
+    rays_plain = 0
+    for sc in scenes:
+        st = sc.stats()
+        rays_plain += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+    images = [sc.readback(f.width, f.height) for sc, f in zip(scenes, frames)]
+    log(f"[bench] plain: {args.steps} frames in {dt_plain * 1e3:.1f} ms ({rays_plain / dt_plain / 1e6:.1f} Mrays/s)")
+
+    batch = dmt.Batch(dev, scenes, frames)
+    t0 = time.time()
+    tuned, tune_ms = batch.autotune(frames_per_trial=2)
+    log(f"[bench] autotune {tuned.as_dict()} {tune_ms:.2f} ms/frame ({time.time() - t0:.1f}s)")
+    batch.render(0, args.warmup)
+    sync()
+    batch.reset_stats()
+    sync()
+    t0 = time.perf_counter()
+    batch.render(args.warmup, args.steps)
+    sync()
+    dt_batch = time.perf_counter() - t0
+    st = batch.stats()
+    rays_batch = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+    same = [bool(np.array_equal(sc.readback(f.width, f.height), img)) for sc, f, img in zip(scenes, frames, images)]
+    log(f"[bench] batch: {args.steps} frames in {dt_batch * 1e3:.1f} ms ({rays_batch / dt_batch / 1e6:.1f} Mrays/s), "
+        f"identical images {sum(same)}/{len(same)}, rays equal {rays_batch == rays_plain}")
+    speedup = dt_plain / dt_batch
+    line = {
+        "metric": "batched vs plain speedup",
+        "value": round(speedup, 3),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt_batch * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded batched-workload generator)",
+        "config": {"workload": f"{len(scenes)} scenes: resolution uniform in {{128,256,512,1024}}^2, triangles "
+                               "log-uniform 10k..2M (icosphere + random tris), odd scenes + 64 spheres; "
+                               "1 spp, max depth 6; one persistent launch per frame from a captured hipGraph "
+                               "vs 64 sequential per-scene launches",
+                   "scenes": len(scenes), "pixels": pixels, "triangles": tris, "seed": args.seed,
+                   "params": tuned.as_dict()},
+        "plain_ms_per_step": round(dt_plain * 1e3 / args.steps, 4),
+        "batch_mrays": round(rays_batch / dt_batch / 1e6, 2),
+        "plain_mrays": round(rays_plain / dt_plain / 1e6, 2),
+        "identical_images": all(same) and rays_batch == rays_plain,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    batch.close()
+    for sc in scenes:
+        sc.close()
     dev.close()
 
 

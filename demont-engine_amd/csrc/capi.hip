@@ -7,6 +7,7 @@
 // LOG_ERROR (src/rhi_software/SoftwareDevice.cpp:337-421, 539-558).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -14,6 +15,7 @@
 
 #include "../../include/dmt.h"
 #include "host/prep.h"
+#include "kernels/persistent.h"
 #include "kernels/wavefront.h"
 
 using namespace dmt;
@@ -98,6 +100,10 @@ struct dmt_scene {
     double extend_ms = 0.0;
     uint64_t extend_launches = 0;
     uint64_t frames = 0;
+    // Batches hold their scenes: dmt_scene_destroy of a member is deferred
+    // to the last dmt_batch_destroy, and the resolution is fixed meanwhile.
+    uint32_t batches = 0;
+    bool destroy_requested = false;
 };
 
 namespace {
@@ -141,25 +147,55 @@ hipError_t sync_all(dmt_device* d) {
     return e;
 }
 
-void free_wavefront(dmt_scene* s) {
+void free_sets(dmt_scene* s) {
     for (void* p : s->wf_buffers) (void)hipFree(p);
     s->wf_buffers.clear();
     for (auto& w : s->wf) w = Wavefront{};
-    s->accum = nullptr;
-    s->acc_w = s->acc_h = 0;
     s->last_set = -1;
     s->next_set = 0;
 }
 
-dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized) {
+void free_accum(dmt_scene* s) {
+    free_sets(s);
+    if (s->accum) (void)hipFree(s->accum);
+    s->accum = nullptr;
+    s->acc_w = s->acc_h = 0;
+}
+
+// The accumulation buffer at w x h (zeroed when it is (re)allocated).
+dmt_status ensure_accum(dmt_scene* s, uint32_t w, uint32_t h, bool* resized) {
     *resized = false;
     if (s->accum && s->acc_w == w && s->acc_h == h) return DMT_OK;
-    hipError_t se = sync_all(s->dev);
-    if (se != hipSuccess) return mark_lost_if_fatal(s->dev, se, "synchronize before resize");
-    free_wavefront(s);
+    if (s->batches > 0) return fail(DMT_ERR_INVALID_ARG, "the scene belongs to a batch at another resolution");
+    hipError_t e = sync_all(s->dev);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize before resize");
+    free_accum(s);
     *resized = true;
     const size_t n = (size_t)w * h;
-    std::vector<std::pair<void**, size_t>> req = {{(void**)&s->accum, n * 16}};
+    e = hipMalloc((void**)&s->accum, n * 16);
+    if (e != hipSuccess) {
+        s->accum = nullptr;
+        return hip_fail(e, "hipMalloc(accum)");
+    }
+    e = hipMemsetAsync(s->accum, 0, n * 16, s->dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
+    if (e != hipSuccess) {
+        free_accum(s);
+        return hip_fail(e, "hipMemsetAsync(accum)");
+    }
+    s->acc_w = w;
+    s->acc_h = h;
+    return DMT_OK;
+}
+
+// The accumulation buffer and the wavefront sets at w x h.
+dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized) {
+    dmt_status st = ensure_accum(s, w, h, resized);
+    if (st) return st;
+    const size_t n = (size_t)w * h;
+    if (s->wf[0].capacity == n) return DMT_OK;
+    free_sets(s);
+    std::vector<std::pair<void**, size_t>> req;
     for (int set = 0; set < (int)kFrameSets; ++set) {
         Wavefront& W = s->wf[set];
         W.capacity = (uint32_t)n;
@@ -189,18 +225,19 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
         hipError_t e = hipMalloc(r.first, r.second);
         if (e != hipSuccess) {
             *r.first = nullptr;
-            free_wavefront(s);
+            free_sets(s);
             return hip_fail(e, "hipMalloc(wavefront)");
         }
         s->wf_buffers.push_back(*r.first);
     }
-    hipError_t e = hipMemsetAsync(s->accum, 0, n * 16, s->dev->stream);
+    hipError_t e = hipSuccess;
     for (int set = 0; set < (int)kFrameSets && e == hipSuccess; ++set)
         e = hipMemsetAsync(s->wf[set].regen, 0, n, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-    s->acc_w = w;
-    s->acc_h = h;
+    if (e != hipSuccess) {
+        free_sets(s);
+        return hip_fail(e, "hipMemsetAsync");
+    }
     return DMT_OK;
 }
 
@@ -368,13 +405,16 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     return DMT_OK;
 }
 
-void dmt_scene_destroy(dmt_scene* s) {
-    if (!s) return;
+}  // extern "C"
+
+namespace {
+
+void scene_free(dmt_scene* s) {
     if (s->dev) {
         (void)hipSetDevice(s->dev->id);
         (void)sync_all(s->dev);
     }
-    free_wavefront(s);
+    free_accum(s);
     for (hipEvent_t e : s->acc_done)
         if (e) (void)hipEventDestroy(e);
     for (auto& row : s->poll)
@@ -386,6 +426,19 @@ void dmt_scene_destroy(dmt_scene* s) {
     dmt_device* d = s->dev;
     delete s;
     if (d && --d->scenes == 0 && d->destroy_requested) device_free(d);
+}
+
+}  // namespace
+
+extern "C" {
+
+void dmt_scene_destroy(dmt_scene* s) {
+    if (!s) return;
+    if (s->batches > 0) {
+        s->destroy_requested = true;
+        return;
+    }
+    scene_free(s);
 }
 
 namespace {
@@ -467,7 +520,9 @@ dmt_status dmt_trace_closest(dmt_scene* s, const float* rays, uint32_t n, dmt_hi
     if (st) return st;
     if (n == 0) return DMT_OK;
     bool resized = false;
-    if (!s->accum && (st = ensure_wavefront(s, 1, 1, &resized))) return st;
+    if (s->wf[0].capacity == 0u &&
+        (st = ensure_wavefront(s, s->accum ? s->acc_w : 1u, s->accum ? s->acc_h : 1u, &resized)))
+        return st;
     float4* d_rays = nullptr;
     HitOut* d_out = nullptr;
     hipError_t e = hipMalloc((void**)&d_rays, (size_t)n * 32u);
@@ -570,6 +625,436 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     out->scene_bytes = s->scene_bytes;
     out->bvh_nodes = s->bvh_nodes;
     out->bvh_depth = s->bvh_depth;
+    return DMT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched persistent form.
+
+struct dmt_batch {
+    dmt_device* dev = nullptr;
+    std::vector<dmt_scene*> scenes;
+    std::vector<dmt_frame_params> frames;
+    dmt_batch_params params{};
+    uint32_t cus = 0;
+    uint32_t max_threads = 0;         // 4 workgroups per CU
+    uint32_t spill_depth = 0;         // per thread, for the smallest LDS stack
+    std::vector<uint32_t> order;      // scenes grouped by XCD queue
+    uint32_t queue_begin[kXcds + 1] = {};
+    BatchScene* d_scenes = nullptr;
+    uint32_t* d_order = nullptr;
+    uint32_t* d_work = nullptr;
+    uint32_t* d_frame = nullptr;
+    float4* d_rec = nullptr;
+    uint32_t* d_spill = nullptr;
+    DeviceStats* d_stats = nullptr;
+    hipGraph_t graph = nullptr;       // one frame: reset, persistent kernel, frame counter
+    hipGraphExec_t exec = nullptr;
+    uint64_t frames_rendered = 0;
+};
+
+namespace {
+
+constexpr uint32_t kBatchMaxScenes = 4096;
+constexpr uint32_t kBatchMaxBlocksPerCu = 4;
+
+dmt_status normalize_params(const dmt_batch_params* in, dmt_batch_params* out) {
+    dmt_batch_params p{};
+    if (in) p = *in;
+    if (p.tile_w == 0) p.tile_w = 8;
+    if (p.tiles_per_fetch == 0) p.tiles_per_fetch = 1;
+    if (p.blocks_per_cu == 0) p.blocks_per_cu = 4;
+    if (p.stack_lds == 0) p.stack_lds = 16;
+    if (p.tile_w != 8 && p.tile_w != 16 && p.tile_w != 32 && p.tile_w != 64)
+        return fail(DMT_ERR_INVALID_ARG, "tile_w must be 8, 16, 32 or 64");
+    if (p.tiles_per_fetch > 16) return fail(DMT_ERR_INVALID_ARG, "tiles_per_fetch must be in 1..16");
+    if (p.blocks_per_cu > kBatchMaxBlocksPerCu) return fail(DMT_ERR_INVALID_ARG, "blocks_per_cu must be in 1..4");
+    if (p.stack_lds != 16 && p.stack_lds != 24 && p.stack_lds != 32)
+        return fail(DMT_ERR_INVALID_ARG, "stack_lds must be 16, 24 or 32");
+    if (p.plain_launches > 1) return fail(DMT_ERR_INVALID_ARG, "plain_launches must be 0 or 1");
+    for (uint32_t r : p.reserved)
+        if (r) return fail(DMT_ERR_INVALID_ARG, "reserved fields must be 0");
+    *out = p;
+    return DMT_OK;
+}
+
+uint32_t log2u(uint32_t v) {
+    uint32_t r = 0;
+    while ((1u << r) < v) ++r;
+    return r;
+}
+
+// Scene table for tile width tile_w; accum[i] replaces scene i's buffer
+// when given (autotuning renders into scratch buffers).
+std::vector<BatchScene> batch_table(const dmt_batch* b, uint32_t tile_w, const std::vector<float4*>* accum) {
+    std::vector<BatchScene> t(b->scenes.size());
+    const uint32_t tile_h = 64u / tile_w;
+    for (size_t i = 0; i < t.size(); ++i) {
+        const dmt_frame_params& f = b->frames[i];
+        BatchScene& e = t[i];
+        std::memset(&e, 0, sizeof(e));
+        e.S = b->scenes[i]->view;
+        e.F = frame_args(&f);
+        e.accum = accum ? (*accum)[i] : b->scenes[i]->accum;
+        e.tiles_x = (f.width + tile_w - 1u) / tile_w;
+        const uint32_t tiles_y = (f.height + tile_h - 1u) / tile_h;
+        e.tiles = f.max_bounces == 0u ? 0u : e.tiles_x * tiles_y;
+    }
+    return t;
+}
+
+BatchArgs batch_args(const dmt_batch* b, const dmt_batch_params& p, const BatchScene* table, DeviceStats* stats) {
+    BatchArgs A{};
+    A.scenes = table;
+    A.order = b->d_order;
+    for (uint32_t q = 0; q <= kXcds; ++q) A.queue_begin[q] = b->queue_begin[q];
+    A.work = b->d_work;
+    A.frame = b->d_frame;
+    A.spill = b->d_spill;
+    A.spill_depth = b->spill_depth;
+    A.threads = b->cus * p.blocks_per_cu * kBlock;
+    A.rec = b->d_rec;
+    A.stats = stats;
+    A.tile_w = p.tile_w;
+    A.tile_shift = log2u(p.tile_w);
+    A.fetch = p.tiles_per_fetch;
+    A.nscenes = (uint32_t)b->scenes.size();
+    return A;
+}
+
+BatchLaunch batch_launch(const dmt_batch* b, const dmt_batch_params& p) {
+    return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds};
+}
+
+void drop_graph(dmt_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+dmt_status upload_table(dmt_batch* b, BatchScene* dst, const std::vector<BatchScene>& t) {
+    hipError_t e = hipMemcpyAsync(dst, t.data(), t.size() * sizeof(BatchScene), hipMemcpyHostToDevice,
+                                  b->dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->dev->stream);
+    if (e != hipSuccess) return mark_lost_if_fatal(b->dev, e, "batch table upload");
+    return DMT_OK;
+}
+
+dmt_status apply_params(dmt_batch* b, const dmt_batch_params& p) {
+    dmt_status st = upload_table(b, b->d_scenes, batch_table(b, p.tile_w, nullptr));
+    if (st) return st;
+    drop_graph(b);
+    b->params = p;
+    return DMT_OK;
+}
+
+dmt_status ensure_graph(dmt_batch* b) {
+    if (b->exec) return DMT_OK;
+    hipStream_t stream = b->dev->stream;
+    const BatchArgs A = batch_args(b, b->params, b->d_scenes, b->d_stats);
+    const BatchLaunch L = batch_launch(b, b->params);
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    const hipError_t le = launch_batch_frame(A, L, stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(stream, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(le != hipSuccess ? le : e, "batch frame capture");
+    }
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        b->exec = nullptr;
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    b->graph = g;
+    return DMT_OK;
+}
+
+void batch_free(dmt_batch* b) {
+    (void)hipSetDevice(b->dev->id);
+    (void)sync_all(b->dev);
+    drop_graph(b);
+    void* bufs[] = {b->d_scenes, b->d_order, b->d_work, b->d_frame, b->d_rec, b->d_spill, b->d_stats};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    for (dmt_scene* s : b->scenes) {
+        if (--s->batches == 0 && s->destroy_requested) scene_free(s);
+    }
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt_frame_params* frames,
+                            uint32_t count, const dmt_batch_params* params, dmt_batch** out) {
+    if (!out) return fail(DMT_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    dmt_status st = check_device(dev);
+    if (st) return st;
+    if (count == 0 || !scenes || !frames) return fail(DMT_ERR_INVALID_ARG, "a batch needs scenes and frames");
+    if (count > kBatchMaxScenes) return fail(DMT_ERR_LIMIT, "at most 4096 scenes per batch");
+    dmt_batch_params p;
+    if ((st = normalize_params(params, &p))) return st;
+    std::vector<dmt_scene*> list(scenes, scenes + count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!list[i]) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is null");
+        if (list[i]->dev != dev) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is on another device");
+        if ((st = check_frame(&frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
+    }
+    {
+        std::vector<dmt_scene*> sorted = list;
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(DMT_ERR_INVALID_ARG, "a scene appears twice in the batch");
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        bool resized = false;
+        if ((st = ensure_accum(list[i], frames[i].width, frames[i].height, &resized)))
+            return fail(st, "scene " + std::to_string(i) + ": " + g_err);
+    }
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->id);
+    if (e != hipSuccess || cus <= 0) return hip_fail(e, "hipDeviceGetAttribute(CU count)");
+
+    auto* b = new (std::nothrow) dmt_batch();
+    if (!b) return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation");
+    b->dev = dev;
+    b->scenes = list;
+    b->frames.assign(frames, frames + count);
+    b->cus = (uint32_t)cus;
+    b->max_threads = b->cus * kBatchMaxBlocksPerCu * kBlock;
+    for (dmt_scene* s : list) s->batches++;
+    uint32_t need = 0;
+    for (dmt_scene* s : list) need = std::max(need, 3u * s->bvh_depth + 4u);
+    b->spill_depth = need > 16u ? need - 16u : 0u;
+
+    // Scenes to XCD queues: largest first onto the least loaded queue.
+    std::vector<uint32_t> idx(count);
+    for (uint32_t i = 0; i < count; ++i) idx[i] = i;
+    auto cost = [&](uint32_t i) { return (uint64_t)frames[i].width * frames[i].height * frames[i].spp; };
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t c) { return cost(a) > cost(c); });
+    std::vector<std::vector<uint32_t>> queue(kXcds);
+    uint64_t load[kXcds] = {};
+    for (uint32_t i : idx) {
+        uint32_t q = 0;
+        for (uint32_t k = 1; k < kXcds; ++k)
+            if (load[k] < load[q]) q = k;
+        queue[q].push_back(i);
+        load[q] += cost(i);
+    }
+    for (uint32_t q = 0; q < kXcds; ++q) {
+        b->queue_begin[q] = (uint32_t)b->order.size();
+        b->order.insert(b->order.end(), queue[q].begin(), queue[q].end());
+    }
+    b->queue_begin[kXcds] = (uint32_t)b->order.size();
+
+    const std::pair<void**, size_t> req[] = {
+        {(void**)&b->d_scenes, count * sizeof(BatchScene)},
+        {(void**)&b->d_order, count * sizeof(uint32_t)},
+        {(void**)&b->d_work, count * sizeof(uint32_t)},
+        {(void**)&b->d_frame, sizeof(uint32_t)},
+        {(void**)&b->d_rec, (size_t)2u * kRecRows * b->max_threads * sizeof(float4)},
+        {(void**)&b->d_spill, (size_t)b->max_threads * b->spill_depth * sizeof(uint32_t)},
+        {(void**)&b->d_stats, sizeof(DeviceStats)},
+    };
+    for (const auto& r : req) {
+        if (r.second == 0) continue;
+        e = hipMalloc(r.first, r.second);
+        if (e != hipSuccess) {
+            *r.first = nullptr;
+            batch_free(b);
+            return hip_fail(e, "hipMalloc(batch)");
+        }
+    }
+    e = hipMemcpy(b->d_order, b->order.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_stats, 0, sizeof(DeviceStats), dev->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_frame, 0, sizeof(uint32_t), dev->stream);
+    if (e != hipSuccess) {
+        batch_free(b);
+        return hip_fail(e, "batch set-up");
+    }
+    if ((st = apply_params(b, p))) {
+        batch_free(b);
+        return st;
+    }
+    *out = b;
+    return DMT_OK;
+}
+
+void dmt_batch_destroy(dmt_batch* b) {
+    if (b) batch_free(b);
+}
+
+dmt_status dmt_batch_set_params(dmt_batch* b, const dmt_batch_params* params) {
+    if (!b || !params) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    dmt_status st = check_device(b->dev);
+    if (st) return st;
+    dmt_batch_params p;
+    if ((st = normalize_params(params, &p))) return st;
+    hipError_t e = sync_all(b->dev);
+    if (e != hipSuccess) return mark_lost_if_fatal(b->dev, e, "synchronize");
+    return apply_params(b, p);
+}
+
+dmt_status dmt_batch_get_params(const dmt_batch* b, dmt_batch_params* out) {
+    if (!b || !out) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    *out = b->params;
+    return DMT_OK;
+}
+
+dmt_status dmt_batch_render(dmt_batch* b, uint32_t first_frame, uint32_t frame_count) {
+    if (!b) return fail(DMT_ERR_INVALID_ARG, "batch is null");
+    dmt_status st = check_device(b->dev);
+    if (st) return st;
+    if (frame_count == 0) return fail(DMT_ERR_INVALID_ARG, "frame_count must be >= 1");
+    hipStream_t stream = b->dev->stream;
+    hipError_t e = hipSuccess;
+    // after the members' per-scene frames still in flight
+    for (dmt_scene* s : b->scenes)
+        if (s->last_set > 0 && e == hipSuccess) e = hipStreamWaitEvent(stream, s->acc_done[s->last_set], 0);
+    if (e != hipSuccess) return mark_lost_if_fatal(b->dev, e, "hipStreamWaitEvent");
+    if (!b->params.plain_launches && (st = ensure_graph(b))) return st;
+    e = batch_set_frame(b->d_frame, first_frame, stream);
+    const BatchArgs A = batch_args(b, b->params, b->d_scenes, b->d_stats);
+    const BatchLaunch L = batch_launch(b, b->params);
+    for (uint32_t k = 0; k < frame_count && e == hipSuccess; ++k)
+        e = b->params.plain_launches ? launch_batch_frame(A, L, stream) : hipGraphLaunch(b->exec, stream);
+    // later per-scene frames wait for the batch's accumulation
+    for (dmt_scene* s : b->scenes) {
+        if (e == hipSuccess) e = hipEventRecord(s->acc_done[0], stream);
+        s->last_set = 0;
+    }
+    if (e != hipSuccess) return mark_lost_if_fatal(b->dev, e, "batch render");
+    b->frames_rendered += frame_count;
+    return DMT_OK;
+}
+
+dmt_status dmt_batch_autotune(dmt_batch* b, uint32_t frames_per_trial, dmt_batch_params* out_best,
+                              float* out_ms) {
+    if (!b) return fail(DMT_ERR_INVALID_ARG, "batch is null");
+    dmt_status st = check_device(b->dev);
+    if (st) return st;
+    if (frames_per_trial == 0) frames_per_trial = 2;
+    hipStream_t stream = b->dev->stream;
+    hipError_t e = sync_all(b->dev);
+    if (e != hipSuccess) return mark_lost_if_fatal(b->dev, e, "synchronize");
+    // scratch accumulation buffers, table and counters
+    size_t pixels = 0;
+    for (const auto& f : b->frames) pixels += (size_t)f.width * f.height;
+    float4* acc = nullptr;
+    BatchScene* table = nullptr;
+    DeviceStats* stats = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    e = hipMalloc((void**)&acc, pixels * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc((void**)&table, b->scenes.size() * sizeof(BatchScene));
+    if (e == hipSuccess) e = hipMalloc((void**)&stats, sizeof(DeviceStats));
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    auto release = [&]() {
+        if (acc) (void)hipFree(acc);
+        if (table) (void)hipFree(table);
+        if (stats) (void)hipFree(stats);
+        for (hipEvent_t x : ev)
+            if (x) (void)hipEventDestroy(x);
+    };
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(e, "autotune scratch");
+    }
+    std::vector<float4*> acc_of(b->scenes.size());
+    {
+        size_t off = 0;
+        for (size_t i = 0; i < b->frames.size(); ++i) {
+            acc_of[i] = acc + off;
+            off += (size_t)b->frames[i].width * b->frames[i].height;
+        }
+    }
+    auto trial = [&](const dmt_batch_params& p, float* ms) -> dmt_status {
+        dmt_status ts = upload_table(b, table, batch_table(b, p.tile_w, &acc_of));
+        if (ts) return ts;
+        const BatchArgs A = batch_args(b, p, table, stats);
+        const BatchLaunch L = batch_launch(b, p);
+        hipError_t te = batch_set_frame(b->d_frame, 0u, stream);
+        if (te == hipSuccess) te = launch_batch_frame(A, L, stream);   // warm-up
+        if (te == hipSuccess) te = hipEventRecord(ev[0], stream);
+        for (uint32_t k = 0; k < frames_per_trial && te == hipSuccess; ++k) te = launch_batch_frame(A, L, stream);
+        if (te == hipSuccess) te = hipEventRecord(ev[1], stream);
+        if (te == hipSuccess) te = hipEventSynchronize(ev[1]);
+        float t = 0.0f;
+        if (te == hipSuccess) te = hipEventElapsedTime(&t, ev[0], ev[1]);
+        if (te != hipSuccess) return mark_lost_if_fatal(b->dev, te, "autotune trial");
+        *ms = t / (float)frames_per_trial;
+        return DMT_OK;
+    };
+    dmt_batch_params best = b->params;
+    float best_ms = 0.0f;
+    st = trial(best, &best_ms);
+    // coordinate descent, one parameter at a time
+    const uint32_t blocks_v[] = {1, 2, 3, 4}, stack_v[] = {16, 24, 32}, tile_v[] = {8, 16, 32, 64},
+                   fetch_v[] = {1, 2, 4, 8};
+    struct Axis {
+        uint32_t dmt_batch_params::*field;
+        const uint32_t* values;
+        size_t n;
+    };
+    const Axis axes[] = {{&dmt_batch_params::blocks_per_cu, blocks_v, 4},
+                         {&dmt_batch_params::stack_lds, stack_v, 3},
+                         {&dmt_batch_params::tile_w, tile_v, 4},
+                         {&dmt_batch_params::tiles_per_fetch, fetch_v, 4}};
+    for (const Axis& ax : axes) {
+        if (st) break;
+        const dmt_batch_params base = best;
+        for (size_t k = 0; k < ax.n && !st; ++k) {
+            if (ax.values[k] == base.*(ax.field)) continue;
+            dmt_batch_params q = base;
+            q.*(ax.field) = ax.values[k];
+            float ms = 0.0f;
+            st = trial(q, &ms);
+            if (!st && ms < best_ms) {
+                best_ms = ms;
+                best = q;
+            }
+        }
+    }
+    release();
+    if (st) return st;
+    if ((st = apply_params(b, best))) return st;
+    if (out_best) *out_best = best;
+    if (out_ms) *out_ms = best_ms;
+    return DMT_OK;
+}
+
+dmt_status dmt_batch_stats(dmt_batch* b, dmt_stats* out) {
+    if (!b || !out) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    dmt_status st = check_device(b->dev);
+    if (st) return st;
+    std::memset(out, 0, sizeof(*out));
+    DeviceStats h{};
+    hipError_t e = sync_all(b->dev);
+    if (e == hipSuccess) e = hipMemcpy(&h, b->d_stats, sizeof(h), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return mark_lost_if_fatal(b->dev, e, "batch stats");
+    out->primary_rays = h.primary;
+    out->extension_rays = h.extension;
+    out->shadow_rays = h.shadow;
+    out->frames = b->frames_rendered;
+    return DMT_OK;
+}
+
+dmt_status dmt_batch_reset_stats(dmt_batch* b) {
+    if (!b) return fail(DMT_ERR_INVALID_ARG, "batch is null");
+    dmt_status st = check_device(b->dev);
+    if (st) return st;
+    hipError_t e = hipMemsetAsync(b->d_stats, 0, sizeof(DeviceStats), b->dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->dev->stream);
+    if (e != hipSuccess) return mark_lost_if_fatal(b->dev, e, "batch stats reset");
+    b->frames_rendered = 0;
     return DMT_OK;
 }
 

@@ -14,10 +14,12 @@
 #include "prep.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <system_error>
 #include <thread>
 
 namespace dmt {
@@ -47,68 +49,77 @@ struct BinNode {
     bool leaf() const { return left < 0; }
 };
 
+constexpr uint32_t kMaxBins = 64;
+// Subtrees at least this large are handed to another thread while spare
+// threads remain; smaller ones are built where they are split.
+constexpr uint32_t kParallelMin = 32768;
+
+// One builder per thread: it owns the nodes of the subtrees it builds; the
+// index array is shared, each builder touching only its own range.  The
+// result does not depend on the thread count: a subtree's split choices see
+// only its own items, and spliced node arrays keep every link.
 struct SahBuilder {
-    const std::vector<BoundsItem>& items;
-    std::vector<uint32_t> idx;
+    const BoundsItem* items;
+    uint32_t* idx;
     std::vector<BinNode> nodes;
     uint32_t bins, max_leaf;
+    std::atomic<int>* spare;
 
-    SahBuilder(const std::vector<BoundsItem>& it, uint32_t b, uint32_t ml) : items(it), bins(b), max_leaf(ml) {
-        idx.resize(items.size());
-        std::iota(idx.begin(), idx.end(), 0u);
-    }
+    SahBuilder(const BoundsItem* it, uint32_t* ix, uint32_t b, uint32_t ml, std::atomic<int>* sp)
+        : items(it), idx(ix), bins(b), max_leaf(ml), spare(sp) {}
 
-    int32_t build(uint32_t first, uint32_t count) {
-        int32_t me = (int32_t)nodes.size();
-        nodes.emplace_back();
-        Box box, cbox;
+    // Chooses the split of idx[first, first + count); returns the partition
+    // point, or first when the range becomes a leaf.
+    uint32_t split(uint32_t first, uint32_t count, Box& box) {
+        Box cbox;
         for (uint32_t i = first; i < first + count; ++i) {
             const BoundsItem& it = items[idx[i]];
             box.grow(it.lo, it.hi);
             cbox.grow_pt(it.c);
         }
-        nodes[me].box = box;
-        if (count <= 1) {
-            nodes[me].first = first;
-            nodes[me].count = count;
-            return me;
+        if (count <= 1) return first;
+        const uint32_t B = bins;
+        Box bb[3][kMaxBins];
+        uint32_t bc[3][kMaxBins] = {};
+        float scale[3];
+        bool live[3];
+        for (int a = 0; a < 3; ++a) {
+            const float ext = cbox.hi[a] - cbox.lo[a];
+            live[a] = ext > 0.0f;
+            scale[a] = live[a] ? float(B) / ext : 0.0f;
         }
-        // Evaluate binned SAH on all three axes.
+        // one pass over the items bins all three axes
+        for (uint32_t i = first; i < first + count; ++i) {
+            const BoundsItem& it = items[idx[i]];
+            for (int a = 0; a < 3; ++a) {
+                if (!live[a]) continue;
+                const uint32_t k = std::min(B - 1, (uint32_t)((it.c[a] - cbox.lo[a]) * scale[a]));
+                bb[a][k].grow(it.lo, it.hi);
+                bc[a][k]++;
+            }
+        }
         float best_cost = INFINITY;
         int best_axis = -1;
         uint32_t best_split = 0;
-        const uint32_t B = bins;
-        std::vector<Box> bb(B);
-        std::vector<uint32_t> bc(B);
-        std::vector<float> right_area(B);
-        std::vector<uint32_t> right_count(B);
+        float right_area[kMaxBins];
+        uint32_t right_count[kMaxBins];
         for (int a = 0; a < 3; ++a) {
-            float ext = cbox.hi[a] - cbox.lo[a];
-            if (!(ext > 0.0f)) continue;
-            float scale = float(B) / ext;
-            std::fill(bb.begin(), bb.end(), Box{});
-            std::fill(bc.begin(), bc.end(), 0u);
-            for (uint32_t i = first; i < first + count; ++i) {
-                const BoundsItem& it = items[idx[i]];
-                uint32_t k = std::min(B - 1, (uint32_t)((it.c[a] - cbox.lo[a]) * scale));
-                bb[k].grow(it.lo, it.hi);
-                bc[k]++;
-            }
+            if (!live[a]) continue;
             Box acc;
             uint32_t n = 0;
             for (uint32_t k = B - 1; k > 0; --k) {
-                acc.grow(bb[k].lo, bb[k].hi);
-                n += bc[k];
+                acc.grow(bb[a][k].lo, bb[a][k].hi);
+                n += bc[a][k];
                 right_area[k] = acc.area();
                 right_count[k] = n;
             }
             Box lacc;
             uint32_t ln = 0;
             for (uint32_t k = 0; k + 1 < B; ++k) {
-                lacc.grow(bb[k].lo, bb[k].hi);
-                ln += bc[k];
+                lacc.grow(bb[a][k].lo, bb[a][k].hi);
+                ln += bc[a][k];
                 if (ln == 0 || right_count[k + 1] == 0) continue;
-                float cost = lacc.area() * float(ln) + right_area[k + 1] * float(right_count[k + 1]);
+                const float cost = lacc.area() * float(ln) + right_area[k + 1] * float(right_count[k + 1]);
                 if (cost < best_cost) {
                     best_cost = cost;
                     best_axis = a;
@@ -116,32 +127,71 @@ struct SahBuilder {
                 }
             }
         }
-        float parent_area = box.area();
-        float leaf_cost = float(count);
-        float split_cost = parent_area > 0.0f ? 1.0f + best_cost / parent_area : INFINITY;
-        if (count <= max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) {
-            nodes[me].first = first;
-            nodes[me].count = count;
+        const float parent_area = box.area();
+        const float leaf_cost = float(count);
+        const float split_cost = parent_area > 0.0f ? 1.0f + best_cost / parent_area : INFINITY;
+        if (count <= max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) return first;
+        if (best_axis < 0) return first + count / 2;   // coincident centroids: split by position
+        const float lo = cbox.lo[best_axis], sc = scale[best_axis];
+        const BoundsItem* its = items;
+        const int ax = best_axis;
+        uint32_t* it = std::partition(idx + first, idx + first + count, [&](uint32_t i) {
+            return std::min(B - 1, (uint32_t)((its[i].c[ax] - lo) * sc)) < best_split;
+        });
+        uint32_t mid = (uint32_t)(it - idx);
+        if (mid == first || mid == first + count) mid = first + count / 2;
+        return mid;
+    }
+
+    int32_t build(uint32_t first, uint32_t count) {
+        const int32_t me = (int32_t)nodes.size();
+        nodes.emplace_back();
+        Box box;
+        const uint32_t mid = split(first, count, box);
+        nodes[(size_t)me].box = box;
+        if (mid == first) {
+            nodes[(size_t)me].first = first;
+            nodes[(size_t)me].count = count;
             return me;
         }
-        uint32_t mid;
-        if (best_axis >= 0) {
-            float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
-            float scale = float(B) / ext;
-            float lo = cbox.lo[best_axis];
-            auto it = std::partition(idx.begin() + first, idx.begin() + first + count, [&](uint32_t i) {
-                uint32_t k = std::min(B - 1, (uint32_t)((items[i].c[best_axis] - lo) * scale));
-                return k < best_split;
-            });
-            mid = (uint32_t)(it - idx.begin());
-            if (mid == first || mid == first + count) mid = first + count / 2;
-        } else {
-            mid = first + count / 2;   // coincident centroids: split by position
+        int32_t l = -1, r = -1;
+        const uint32_t nl = mid - first, nr = first + count - mid;
+        bool forked = false;
+        if (std::min(nl, nr) >= kParallelMin) {
+            if (spare->fetch_sub(1) > 0) {
+                SahBuilder sub(items, idx, bins, max_leaf, spare);
+                int32_t sub_root = -1;
+                std::thread th;
+                try {
+                    th = std::thread([&] { sub_root = sub.build(first, nl); });
+                } catch (const std::system_error&) {
+                    // no thread available: both halves are built here
+                }
+                if (th.joinable()) {
+                    struct Join {
+                        std::thread& t;
+                        ~Join() { t.join(); }
+                    } join{th};
+                    r = build(mid, nr);
+                }
+                forked = sub_root >= 0;
+                if (forked) {
+                    const int32_t off = (int32_t)nodes.size();
+                    for (BinNode n : sub.nodes) {
+                        if (!n.leaf()) { n.left += off; n.right += off; }
+                        nodes.push_back(n);
+                    }
+                    l = sub_root + off;
+                }
+            }
+            spare->fetch_add(1);
         }
-        int32_t l = build(first, mid - first);
-        int32_t r = build(mid, first + count - mid);
-        nodes[me].left = l;
-        nodes[me].right = r;
+        if (!forked) {
+            l = build(first, nl);
+            r = build(mid, nr);
+        }
+        nodes[(size_t)me].left = l;
+        nodes[(size_t)me].right = r;
         return me;
     }
 };
@@ -209,15 +259,20 @@ Bvh4 build_bvh4(const std::vector<BoundsItem>& items, uint32_t bins, uint32_t ma
     Bvh4 out;
     if (items.empty()) return out;
     if (bins < 2) bins = 16;
+    bins = std::min(bins, kMaxBins);
     if (max_leaf < 1) max_leaf = 4;
-    SahBuilder b(items, bins, max_leaf);
+    std::vector<uint32_t> idx(items.size());
+    std::iota(idx.begin(), idx.end(), 0u);
+    const unsigned hw = std::thread::hardware_concurrency();
+    std::atomic<int> spare{(int)std::min(hw ? hw : 1u, 64u) - 1};
+    SahBuilder b(items.data(), idx.data(), bins, max_leaf, &spare);
     b.nodes.reserve(items.size() * 2);
     b.build(0, (uint32_t)items.size());
     Collapser c{b.nodes, out.nodes};
     out.nodes.reserve(b.nodes.size() / 2 + 1);
     c.collapse(0, 1);
     out.depth = c.max_depth;
-    out.order = std::move(b.idx);
+    out.order = std::move(idx);
     return out;
 }
 

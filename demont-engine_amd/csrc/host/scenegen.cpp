@@ -220,6 +220,9 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
     env.assign((size_t)ew * eh * 3, 0.0f);
     if (config == 1u)
         for (auto& t : env) t = float(std::exp(rng.normal()));
+    // config 1 with spheres: config 0's spheres and ground plane join the mesh
+    // (drawn after everything else, so the default stream is unchanged)
+    if (config == 1u && o.spheres > 0) gen_spheres(rng, o.spheres, prims);
     {
         const float lp[3] = {0.0f, 8.0f, 0.0f}, ln[3] = {0.0f, -1.0f, 0.0f}, lu[3] = {1.5f, 0.0f, 0.0f};
         const float I[3] = {10.0f, 10.0f, 10.0f};
@@ -254,6 +257,28 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
     g->frame.firefly_clamp = 10.0f;
     g->frame.ema_alpha = 0.0f;
     *out = g;
+    return DMT_OK;
+}
+
+dmt_status dmt_batch_workload_scene(uint64_t seed, uint32_t index, dmt_gen_options* opt, uint64_t* scene_seed,
+                                    uint32_t* max_bounces) {
+    if (!opt || !scene_seed || !max_bounces) return DMT_ERR_INVALID_ARG;
+    Rng rng(seed * 0x2545F4914F6CDD1Dull + 0x3000u + index);
+    static const uint32_t kRes[4] = {128u, 256u, 512u, 1024u};
+    const uint32_t res = kRes[rng.next() % 4u];
+    const double lt = std::log(1.0e4) + (std::log(2.0e6) - std::log(1.0e4)) * rng.u01();
+    const uint32_t tris = (uint32_t)std::llround(std::exp(lt));
+    int sub = 0;
+    while (sub < 8 && 20u * (1u << (2 * (sub + 1))) <= tris) ++sub;
+    const uint32_t ico = 20u * (1u << (2 * sub));
+    std::memset(opt, 0, sizeof(*opt));
+    opt->subdivisions = sub;
+    opt->random_tris = (int32_t)(tris - ico);
+    opt->spheres = (index & 1u) ? 64 : 0;
+    opt->width = res;
+    opt->height = res;
+    *scene_seed = rng.next();
+    *max_bounces = 6u;
     return DMT_OK;
 }
 

@@ -193,37 +193,52 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
     }
 }
 
-// transmittance (PathTrace.slang:5089-5196): closest hits along the ray,
-// refracting through dielectrics with a (1 - F) * albedo tint.
+// One step i of transmittance (PathTrace.slang:5089-5196) given the closest
+// hit h of the ray (o, d): returns true when t is final (a miss or a hit
+// beyond t_max keeps it, an opaque hit, the step limit or total internal
+// reflection zero it); otherwise tints t by (1 - F) * albedo and moves the
+// ray through the dielectric surface.
+__device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArgs& F, HitRec h, uint32_t i, f3& o,
+                                              f3& d, float& t_max, f3& t) {
+    if (h.code == kMiss || h.t >= t_max) return true;
+    const Surface sf = surface_at(S, o, d, h);
+    if (sf.mat != 2u || i == F.refract_bounces) {
+        t = f3{0.0f, 0.0f, 0.0f};
+        return true;
+    }
+    float ndi = dot(sf.n, d);
+    f3 nf = (ndi < 0.0f) ? sf.n : -sf.n;
+    float ia = (ndi < 0.0f) ? 1.0f : sf.ior;
+    float ib = (ndi < 0.0f) ? sf.ior : 1.0f;
+    float eta = ia / ib;
+    float ci = fabsf(dot(nf, -d));
+    float q = (ia - ib) / (ia + ib);
+    float r0 = q * q;
+    float m = 1.0f - ci;
+    float fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
+    f3 rf = refract(d, nf, eta);
+    if (length(rf) < 1e-3f) {
+        t = f3{0.0f, 0.0f, 0.0f};
+        return true;
+    }
+    t = t * (sf.albedo * (1.0f - fr));
+    float adv = h.t + 1e-3f;
+    o = o + d * adv;
+    d = normalize(rf);
+    t_max = fmaxf(t_max - adv, 0.0f);
+    return false;
+}
+
+// transmittance: closest hits along the ray, refracting through dielectrics,
+// at most refract_bounces + 1 queries.
 template <bool COUNT>
 __device__ __forceinline__ f3 transmittance(const SceneView& S, const FrameArgs& F, f3 o, f3 d, float t_max,
                                             Stack& st, TraceCounters& tc, uint32_t& steps) {
-    const int kmax = (int)F.refract_bounces;
     f3 t{1.0f, 1.0f, 1.0f};
-    for (int i = 0; i <= kmax; ++i) {
+    for (uint32_t i = 0; i <= F.refract_bounces; ++i) {
         steps++;
         HitRec h = trace_closest<COUNT>(S, o, d, st, tc);
-        if (h.code == kMiss || h.t >= t_max) return t;
-        const Surface sf = surface_at(S, o, d, h);
-        if (sf.mat != 2u) return f3{0.0f, 0.0f, 0.0f};
-        if (i == kmax) return f3{0.0f, 0.0f, 0.0f};
-        float ndi = dot(sf.n, d);
-        f3 nf = (ndi < 0.0f) ? sf.n : -sf.n;
-        float ia = (ndi < 0.0f) ? 1.0f : sf.ior;
-        float ib = (ndi < 0.0f) ? sf.ior : 1.0f;
-        float eta = ia / ib;
-        float ci = fabsf(dot(nf, -d));
-        float q = (ia - ib) / (ia + ib);
-        float r0 = q * q;
-        float m = 1.0f - ci;
-        float fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
-        f3 rf = refract(d, nf, eta);
-        if (length(rf) < 1e-3f) return f3{0.0f, 0.0f, 0.0f};
-        t = t * (sf.albedo * (1.0f - fr));
-        float adv = h.t + 1e-3f;
-        o = o + d * adv;
-        d = normalize(rf);
-        t_max = fmaxf(t_max - adv, 0.0f);
+        if (transmit_step(S, F, h, i, o, d, t_max, t)) return t;
     }
     return f3{0.0f, 0.0f, 0.0f};
 }

@@ -1,0 +1,301 @@
+// Persistent batch kernel (see persistent.h).
+#include "persistent.h"
+
+namespace dmt {
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kEnded = 16u;          // pending bit: the path ends after this vertex
+
+// Shadow records of one vertex go to this thread's scratch rows; bit k of
+// `pending` marks record k, bit k + 2 its clampIndirect flag.
+struct ScratchSink {
+    float4* rec;
+    uint32_t threads, tid;
+    uint32_t pending;
+    __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState&) {
+        float4* row = rec + ((size_t)kind * kRecRows) * threads + tid;
+        row[0] = make_float4(r.o.x, r.o.y, r.o.z, r.t_max);
+        row[(size_t)threads] = make_float4(r.d.x, r.d.y, r.d.z, r.w);
+        row[2 * (size_t)threads] = make_float4(r.pre.x, r.pre.y, r.pre.z, r.ndl);
+        row[3 * (size_t)threads] = make_float4(r.brdf.x, r.brdf.y, r.brdf.z, r.pdf);
+        pending |= (1u << kind) | (r.clamp ? (4u << kind) : 0u);
+    }
+};
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ void add_stat(unsigned long long* dst, uint32_t v) {
+    uint32_t s = wave_sum(v);
+    if (__lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// State of one lane's path.  The fields used at every query stay in
+// registers; the rest sits in an LDS column of the thread (`Cold`), so the
+// registers of the traversal and of the vertex shading are not held by it.
+struct Lane {
+    bool active;
+    uint32_t mode;        // 0 extension ray, 1 light-NEE shadow ray, 2 env-NEE shadow ray
+    f3 o, d;              // current ray
+    PathVertexState P;
+    uint32_t b;           // bounce
+    uint32_t pending;     // shadow records of the vertex, their clamp flags, kEnded
+};
+
+struct Cold {
+    float* base;          // this thread's column: field f at base[f * kBlock]
+    enum : uint32_t { kSum = 0, kX = 3, kY = 4, kS = 5, kTr = 6, kTmax = 9, kStep = 10, kNo = 11, kNd = 14, kRows = 17 };
+    __device__ __forceinline__ float get(uint32_t f) const { return base[f * kBlock]; }
+    __device__ __forceinline__ void set(uint32_t f, float v) { base[f * kBlock] = v; }
+    __device__ __forceinline__ uint32_t getu(uint32_t f) const { return __float_as_uint(get(f)); }
+    __device__ __forceinline__ void setu(uint32_t f, uint32_t v) { set(f, __uint_as_float(v)); }
+    __device__ __forceinline__ f3 get3(uint32_t f) const { return f3{get(f), get(f + 1), get(f + 2)}; }
+    __device__ __forceinline__ void set3(uint32_t f, f3 v) { set(f, v.x); set(f + 1, v.y); set(f + 2, v.z); }
+};
+
+__device__ __forceinline__ void start_sample(const FrameArgs& F, Lane& l, uint32_t x, uint32_t y) {
+    l.d = camera_ray(F, x, y, l.P.seed);
+    l.o = F.pos;
+    l.P.T = f3{1.0f, 1.0f, 1.0f};
+    l.P.L = f3{0.0f, 0.0f, 0.0f};
+    l.P.prev_pdf = 0.0f;
+    l.P.skip_sky = false;
+    l.P.prev_lambert = false;
+    l.b = 0u;
+    l.mode = 0u;
+}
+
+__device__ __forceinline__ void start_shadow(const float4* rec, uint32_t threads, uint32_t tid, uint32_t k, Lane& l,
+                                             Cold& c) {
+    const float4* row = rec + ((size_t)k * kRecRows) * threads + tid;
+    const float4 o4 = row[0], d4 = row[(size_t)threads];
+    l.o = xyz(o4);
+    l.d = xyz(d4);
+    c.set(Cold::kTmax, o4.w);
+    c.set3(Cold::kTr, f3{1.0f, 1.0f, 1.0f});
+    c.setu(Cold::kStep, 0u);
+    l.mode = k + 1u;
+}
+
+// After the vertex's shading or one of its shadow records: start the next
+// record (light before env, as the wavefront's shadow kernels run), or close
+// the vertex.
+__device__ __forceinline__ void advance(const FrameArgs& F, const float4* rec, uint32_t threads, uint32_t tid,
+                                        float4* accum, bool reset, Lane& l, Cold& c) {
+    if (l.pending & 1u) {
+        start_shadow(rec, threads, tid, 0u, l, c);
+        return;
+    }
+    if (l.pending & 2u) {
+        const float4* row = rec + ((size_t)1u * kRecRows) * threads + tid;
+        const float4 b4 = row[3 * (size_t)threads];
+        if (b4.w < 0.0f) {
+            // ambient term of the no-env Lambert branch: added without a query
+            l.P.L = l.P.L + xyz(row[2 * (size_t)threads]);
+            l.pending &= ~2u;
+        } else {
+            start_shadow(rec, threads, tid, 1u, l, c);
+            return;
+        }
+    }
+    if (!(l.pending & kEnded)) {
+        l.o = c.get3(Cold::kNo);
+        l.d = c.get3(Cold::kNd);
+        l.b++;
+        l.mode = 0u;
+        return;
+    }
+    // the sample is done
+    const uint32_t s = c.getu(Cold::kS);
+    const f3 sum = (s == 0u) ? l.P.L : c.get3(Cold::kSum) + l.P.L;
+    const uint32_t x = c.getu(Cold::kX), y = c.getu(Cold::kY);
+    if (s + 1u < F.spp) {
+        c.set3(Cold::kSum, sum);
+        c.setu(Cold::kS, s + 1u);
+        start_sample(F, l, x, y);   // the next sample continues the pixel's stream
+        return;
+    }
+    const uint32_t p = y * F.width + x;
+    const f3 v = frame_value(F, sum);
+    accum[p] = accumulate(F, reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : accum[p], reset, v);
+    l.active = false;
+}
+
+template <uint32_t DEPTH>
+__global__ void __launch_bounds__(kBlock) k_persistent(BatchArgs A) {
+    __shared__ uint32_t lds[DEPTH * kBlock];
+    __shared__ float cold[Cold::kRows * kBlock];
+    Cold c{cold + threadIdx.x};
+    const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    Stack stk{lds + threadIdx.x, kBlock, DEPTH, A.spill + (size_t)tid * A.spill_depth, A.spill_depth};
+    TraceCounters tc{0u, 0u, 0u};
+    uint32_t n_primary = 0, n_ext = 0, n_shadow = 0;
+    const uint32_t rel = *A.frame;
+
+    // wave-uniform work position
+    const uint32_t xcd = blockIdx.x % kXcds;
+    uint32_t qoff = 0u, qpos = A.queue_begin[xcd];
+    bool open = false;                 // the current scene may have tiles left
+    uint32_t sc = kNone, pool_lo = 0u, pool_hi = 0u;
+    const BatchScene* bs = A.scenes;
+    uint32_t frame_index = 0u;
+    float4* accum = nullptr;
+    uint32_t tiles_x = 1u, tiles = 0u;
+    bool reset = false;
+
+    Lane l{};
+    l.active = false;
+    const uint32_t tw = A.tile_w, tsh = A.tile_shift;
+    for (;;) {
+        // ---- hand pixels to idle lanes (wave-uniform control flow) ----
+        // The wave may switch scene (bs) in this loop, so the scene's state
+        // is read through bs where it is used, never held across a switch.
+        for (;;) {
+            const unsigned long long idle = __ballot(!l.active);
+            if (idle == 0ull) break;
+            if (pool_lo < pool_hi) {
+                const FrameArgs& F = bs->F;
+                const uint32_t ord = pool_lo + (uint32_t)__popcll(idle & below);
+                if (!l.active && ord < pool_hi) {
+                    const uint32_t tile = ord >> 6, k = ord & 63u;
+                    const uint32_t x = (tile % tiles_x) * tw + (k & (tw - 1u));
+                    const uint32_t y = (tile / tiles_x) * (64u >> tsh) + (k >> tsh);
+                    if (x < F.width && y < F.height) {
+                        l.active = true;
+                        c.setu(Cold::kX, x);
+                        c.setu(Cold::kY, y);
+                        c.setu(Cold::kS, 0u);
+                        l.P.seed = pixel_seed(F, x, y, frame_index);
+                        start_sample(F, l, x, y);
+                    }
+                }
+                pool_lo = min(pool_lo + (uint32_t)__popcll(idle), pool_hi);
+                continue;
+            }
+            if (open) {
+                uint32_t start = 0u;
+                if (lane == 0u) start = atomicAdd(&A.work[sc], A.fetch);
+                start = uniform(__shfl(start, 0));
+                if (start < tiles) {
+                    pool_lo = start * 64u;
+                    pool_hi = min(start + A.fetch, tiles) * 64u;
+                    continue;
+                }
+                open = false;
+            }
+            if (__ballot(l.active) != 0ull) break;   // finish this scene's paths first
+            // next scene with tiles left: this XCD's queue, then the others
+            sc = kNone;
+            while (qoff < kXcds) {
+                const uint32_t q = (xcd + qoff) % kXcds;
+                if (qpos < A.queue_begin[q + 1u]) {
+                    const uint32_t cand = A.order[qpos++];
+                    const uint32_t handed = __hip_atomic_load(&A.work[cand], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (uniform(handed) < A.scenes[cand].tiles) {
+                        sc = cand;
+                        break;
+                    }
+                } else {
+                    ++qoff;
+                    qpos = A.queue_begin[(xcd + qoff) % kXcds];
+                }
+            }
+            if (sc == kNone) break;   // no work left anywhere
+            bs = A.scenes + sc;
+            frame_index = bs->F.frame_index + rel;
+            reset = rel == 0u && bs->F.reset_accum != 0u;
+            accum = bs->accum;
+            tiles_x = bs->tiles_x;
+            tiles = bs->tiles;
+            open = true;
+            pool_lo = pool_hi = 0u;
+        }
+        if (__ballot(l.active) == 0ull) break;
+
+        // ---- one closest-hit query per live lane ----
+        const SceneView& S = bs->S;
+        const FrameArgs& F = bs->F;
+        if (l.active) {
+            const HitRec h = trace_closest<false>(S, l.o, l.d, stk, tc);
+            if (l.mode == 0u) {
+                if (l.b == 0u) n_primary++;
+                else n_ext++;
+                ScratchSink sink{A.rec, A.threads, tid, 0u};
+                VertexResult R;
+                shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
+                const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
+                l.pending = sink.pending | (cont ? 0u : kEnded);
+                c.set3(Cold::kNo, R.no);
+                c.set3(Cold::kNd, R.nd);
+                advance(F, A.rec, A.threads, tid, accum, reset, l, c);
+            } else {
+                n_shadow++;
+                float t_max = c.get(Cold::kTmax);
+                f3 tr = c.get3(Cold::kTr);
+                const uint32_t step = c.getu(Cold::kStep);
+                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr)) {
+                    const uint32_t k = l.mode - 1u;
+                    if (any_pos(tr)) {
+                        const float4* row = A.rec + ((size_t)k * kRecRows) * A.threads + tid;
+                        const float w = row[(size_t)A.threads].w;
+                        const float4 a4 = row[2 * (size_t)A.threads], b4 = row[3 * (size_t)A.threads];
+                        l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w, k == 1u ? w : 1.0f,
+                                                     (l.pending & (4u << k)) != 0u, tr);
+                    }
+                    l.pending &= ~(1u << k);
+                    advance(F, A.rec, A.threads, tid, accum, reset, l, c);
+                } else {
+                    c.set(Cold::kTmax, t_max);
+                    c.set3(Cold::kTr, tr);
+                    c.setu(Cold::kStep, step + 1u);
+                }
+            }
+        }
+    }
+    add_stat(&A.stats->primary, n_primary);
+    add_stat(&A.stats->extension, n_ext);
+    add_stat(&A.stats->shadow, n_shadow);
+}
+
+__global__ void k_batch_begin(uint32_t* work, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) work[i] = 0u;
+}
+
+__global__ void k_batch_end(uint32_t* frame) {
+    if (threadIdx.x == 0) frame[0] = frame[0] + 1u;
+}
+
+__global__ void k_batch_set_frame(uint32_t* frame, uint32_t v) {
+    if (threadIdx.x == 0) frame[0] = v;
+}
+
+}  // namespace
+
+hipError_t batch_set_frame(uint32_t* frame, uint32_t value, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_set_frame, dim3(1), dim3(64), 0, stream, frame, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_frame(const BatchArgs& A, const BatchLaunch& L, hipStream_t stream) {
+    if (L.blocks == 0u || (size_t)L.blocks * kBlock != A.threads) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, stream, A.work, A.nscenes);
+    switch (L.stack_lds) {
+        case 16u: hipLaunchKernelGGL(k_persistent<16u>, dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 24u: hipLaunchKernelGGL(k_persistent<24u>, dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 32u: hipLaunchKernelGGL(k_persistent<32u>, dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, stream, A.frame);
+    return hipGetLastError();
+}
+
+}  // namespace dmt

@@ -110,6 +110,14 @@ class GeneratedScene(C.Structure):
     ]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tiles_per_fetch", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("stack_lds", C.c_uint32), ("plain_launches", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class DmtError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"dmt status {status}: {msg}")
@@ -153,6 +161,17 @@ def lib() -> C.CDLL:
         "dmt_build_bvh4": (st, [fp, u32, C.POINTER(u32), u32, u32, u32, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
         "dmt_build_light_tree": (u32, [fp, u32, fp]),
         "dmt_build_env_cdf": (C.c_double, [fp, u32, u32, fp, fp]),
+        "dmt_batch_workload_scene": (st, [C.c_uint64, u32, C.POINTER(GenOptions), C.POINTER(C.c_uint64),
+                                          C.POINTER(u32)]),
+        "dmt_batch_create": (st, [vp, C.POINTER(vp), C.POINTER(FrameParams), u32, C.POINTER(BatchParams),
+                                  C.POINTER(vp)]),
+        "dmt_batch_destroy": (None, [vp]),
+        "dmt_batch_render": (st, [vp, u32, u32]),
+        "dmt_batch_set_params": (st, [vp, C.POINTER(BatchParams)]),
+        "dmt_batch_get_params": (st, [vp, C.POINTER(BatchParams)]),
+        "dmt_batch_autotune": (st, [vp, u32, C.POINTER(BatchParams), C.POINTER(C.c_float)]),
+        "dmt_batch_stats": (st, [vp, C.POINTER(Stats)]),
+        "dmt_batch_reset_stats": (st, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -235,6 +254,31 @@ def generate(config: int, seed: int = 1, subdivisions: int = -1, random_tris: in
         return sd
     finally:
         lib().dmt_scene_generated_free(out)
+
+
+def batch_workload_scene(seed: int, index: int) -> tuple[GenOptions, int, int]:
+    """Options, scene seed and bounce limit of scene `index` of the batched workload."""
+    opt = GenOptions()
+    sseed = C.c_uint64()
+    mb = C.c_uint32()
+    _check(lib().dmt_batch_workload_scene(seed, index, C.byref(opt), C.byref(sseed), C.byref(mb)))
+    return opt, sseed.value, mb.value
+
+
+def generate_batch_scene(seed: int, index: int, scale: float = 1.0) -> SceneData:
+    """Scene `index` of the batched workload (config 1 generator); `scale` < 1
+    shrinks the resolution and triangle count for quick tests."""
+    opt, sseed, mb = batch_workload_scene(seed, index)
+    res = max(8, int(opt.width * scale))
+    tris = int((20 * 4 ** opt.subdivisions + opt.random_tris) * scale)
+    sub = 0
+    while sub < 8 and 20 * 4 ** (sub + 1) <= tris:
+        sub += 1
+    sd = generate(1, sseed, subdivisions=sub, random_tris=max(0, tris - 20 * 4 ** sub), spheres=opt.spheres,
+                  width=res, height=res)
+    sd.frame.spp = 1
+    sd.frame.max_bounces = mb
+    return sd
 
 
 def look_at(eye, target, fov_deg: float, width: int, height: int, base: FrameParams | None = None) -> FrameParams:
@@ -330,6 +374,56 @@ class Scene:
     def close(self):
         if self._h:
             lib().dmt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many scenes rendered by the persistent kernel (dmt_batch_*)."""
+
+    def __init__(self, device: Device, scenes: list, frames: list, params: BatchParams | None = None):
+        self.device = device
+        self.scenes = list(scenes)   # keeps the Scene objects alive
+        n = len(self.scenes)
+        hs = (C.c_void_p * n)(*[sc._h.value for sc in self.scenes])
+        fs = (FrameParams * n)(*[f.copy() for f in frames])
+        self._h = C.c_void_p()
+        _check(lib().dmt_batch_create(device.handle, hs, fs, n, C.byref(params) if params else None,
+                                      C.byref(self._h)))
+
+    def render(self, first_frame: int, count: int):
+        _check(lib().dmt_batch_render(self._h, first_frame, count))
+
+    def set_params(self, p: BatchParams):
+        _check(lib().dmt_batch_set_params(self._h, C.byref(p)))
+
+    def params(self) -> BatchParams:
+        p = BatchParams()
+        _check(lib().dmt_batch_get_params(self._h, C.byref(p)))
+        return p
+
+    def autotune(self, frames_per_trial: int = 2) -> tuple[BatchParams, float]:
+        p = BatchParams()
+        ms = C.c_float()
+        _check(lib().dmt_batch_autotune(self._h, frames_per_trial, C.byref(p), C.byref(ms)))
+        return p, ms.value
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().dmt_batch_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        _check(lib().dmt_batch_reset_stats(self._h))
+
+    def close(self):
+        if self._h:
+            lib().dmt_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

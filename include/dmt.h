@@ -183,6 +183,55 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* scene, const dmt_frame_params* f
 /* Per-event timing of the closest-hit kernel launches on/off (default off). */
 dmt_status dmt_scene_set_profiling(dmt_scene* scene, int32_t enabled);
 
+/* ---- Batched persistent form ---------------------------------------------
+ * No one-for-one counterpart in the reference: its engine renders one scene
+ * per frame, one PushConstants + Dispatch("pathtrace") each
+ * (Engine.cpp:9577-9582 -> SoftwareDevice.cpp:105-136 -> RunPathTraceKernel).
+ * A batch renders a frame of many scenes, each with its own camera and
+ * resolution, in one persistent-kernel launch fed by a device work queue of
+ * 64-pixel tiles; frames replay a captured hipGraph.  Scene i's image after
+ * dmt_batch_render(b, 0, n) equals dmt_render_frames(scene_i, &frames[i], n).
+ */
+typedef struct dmt_batch dmt_batch;
+
+typedef struct dmt_batch_params {
+    uint32_t tile_w;            /* tile width in pixels, tiles hold 64 pixels:
+                                   8, 16, 32 or 64 (0 = 8)                    */
+    uint32_t tiles_per_fetch;   /* tiles a wave takes per queue atomic, 1..16 (0 = 1) */
+    uint32_t blocks_per_cu;     /* persistent workgroups of 256 threads per CU,
+                                   1..4 (0 = 4)                               */
+    uint32_t stack_lds;         /* LDS traversal-stack entries per thread:
+                                   16, 24 or 32 (0 = 16)                      */
+    uint32_t plain_launches;    /* 0 = replay a captured hipGraph per frame
+                                   (default), 1 = launch the kernels directly */
+    uint32_t reserved[3];
+} dmt_batch_params;
+
+/* Groups count scenes (distinct, on one device) with their frame settings.
+ * Each scene's accumulation buffer is (re)sized to its frame's resolution;
+ * while the batch lives the scene cannot be rendered at another resolution,
+ * and dmt_scene_destroy of a member is deferred to dmt_batch_destroy.
+ * params may be null (defaults). */
+dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt_frame_params* frames,
+                            uint32_t count, const dmt_batch_params* params, dmt_batch** out_batch);
+void       dmt_batch_destroy(dmt_batch* batch);
+/* Renders frames first_frame .. first_frame + frame_count - 1 of every scene:
+ * frame k of scene i uses frame_index = frames[i].frame_index + k and
+ * resets the accumulation at k == 0 when frames[i].reset_accum is set.
+ * Stream-ordered after earlier work on the member scenes. */
+dmt_status dmt_batch_render(dmt_batch* batch, uint32_t first_frame, uint32_t frame_count);
+dmt_status dmt_batch_set_params(dmt_batch* batch, const dmt_batch_params* params);
+dmt_status dmt_batch_get_params(const dmt_batch* batch, dmt_batch_params* out);
+/* Times candidate parameters (coordinate descent over blocks_per_cu,
+ * stack_lds, tile_w, tiles_per_fetch; frames_per_trial timed frames each,
+ * rendered into scratch buffers so the scenes' images are untouched),
+ * keeps the fastest and returns it with its time per frame. */
+dmt_status dmt_batch_autotune(dmt_batch* batch, uint32_t frames_per_trial, dmt_batch_params* out_best,
+                              float* out_ms_per_frame);
+/* Ray counts of the batch frames rendered since the last reset. */
+dmt_status dmt_batch_stats(dmt_batch* batch, dmt_stats* out);
+dmt_status dmt_batch_reset_stats(dmt_batch* batch);
+
 /* Last error message of the calling thread ("" if none). */
 const char* dmt_last_error(void);
 

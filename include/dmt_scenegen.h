@@ -16,6 +16,8 @@
  *             env with lognormal texels exp(N(0,1)); 1920x1080, 1 spp, 8 bounces.
  *   config 4  config 0 with 256 spheres, 256 distinct materials; 512x512,
  *             16 spp, 4 bounces.
+ *   config 1 with options.spheres > 0 adds config 0's spheres and plane (used
+ *             by the batched workload, see dmt_batch_workload_scene).
  * Camera for all: at (0, 2, 12) looking at the origin, vertical fov 50 deg.
  */
 #ifndef DMT_SCENEGEN_H
@@ -46,6 +48,15 @@ typedef struct dmt_generated_scene {
 dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_options* options,
                               dmt_generated_scene** out);
 void       dmt_scene_generated_free(dmt_generated_scene* scene);
+
+/* Scene `index` of the batched workload: resolution drawn uniformly from
+ * {128, 256, 512, 1024} squared; triangle count log-uniform in [10k, 2M],
+ * split into the finest icosphere that fits plus random triangles of the
+ * config 1 generator; odd indices also get config 0's 64 spheres and ground
+ * plane.  Fills the options and seed for dmt_scene_generate(1, ...); the
+ * frame is that scene's camera with 1 spp and *out_max_bounces (6). */
+dmt_status dmt_batch_workload_scene(uint64_t seed, uint32_t index, dmt_gen_options* out_options,
+                                    uint64_t* out_scene_seed, uint32_t* out_max_bounces);
 
 /* Camera basis of the reference Camera (src/renderer/Camera.h) looking from
  * eye to target; fills pos_fovtan / fwd_aspect / right_xyz / up_xyz. */

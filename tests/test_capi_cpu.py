@@ -93,3 +93,42 @@ def test_device_create_reports_missing_device_without_a_gpu():
     with pytest.raises(dmt.DmtError) as e:
         dmt.Device(0)
     assert e.value.status == dmt.DMT_ERR_NO_DEVICE
+
+
+def test_batch_workload_generator():
+    """The batched workload's per-scene draws: resolutions from the four
+    squares, triangle counts log-uniform in [10k, 2M] split into the finest
+    fitting icosphere plus random triangles, spheres on odd scenes."""
+    res, tris = set(), []
+    for i in range(64):
+        o, seed, mb = dmt.batch_workload_scene(1, i)
+        assert o.width == o.height and o.width in (128, 256, 512, 1024)
+        t = 20 * 4 ** o.subdivisions + o.random_tris
+        assert 10_000 <= t <= 2_000_000 and o.random_tris >= 0
+        if o.subdivisions < 8:
+            assert 20 * 4 ** (o.subdivisions + 1) > t
+        assert o.spheres == (64 if i % 2 else 0) and mb == 6
+        res.add(o.width)
+        tris.append(t)
+    assert res == {128, 256, 512, 1024}
+    assert dmt.batch_workload_scene(1, 5)[1] == dmt.batch_workload_scene(1, 5)[1]
+    assert dmt.batch_workload_scene(1, 5)[1] != dmt.batch_workload_scene(2, 5)[1]
+    assert 40_000 < np.median(tris) < 600_000   # log-uniform median is ~141k
+
+
+def test_config1_spheres_option_keeps_the_default_stream():
+    a = dmt.generate(1, seed=3, subdivisions=2, random_tris=50)
+    b = dmt.generate(1, seed=3, subdivisions=2, random_tris=50, spheres=0)
+    c = dmt.generate(1, seed=3, subdivisions=2, random_tris=50, spheres=64)
+    assert np.array_equal(a.positions, b.positions) and np.array_equal(a.env, b.env)
+    assert np.array_equal(a.positions, c.positions) and np.array_equal(a.env, c.env)
+    assert a.prims.shape[0] == 0 and c.prims.shape[0] == 65
+    assert c.prims[0, 8] == 1.0 and np.all(c.prims[1:, 8] == 0.0)   # ground plane, then spheres
+
+
+def test_batch_entry_points_validate_without_a_device():
+    h = C.c_void_p()
+    assert dmt.lib().dmt_batch_create(None, None, None, 0, None, C.byref(h)) == dmt.DMT_ERR_INVALID_ARG
+    assert dmt.lib().dmt_batch_render(None, 0, 1) == dmt.DMT_ERR_INVALID_ARG
+    assert dmt.lib().dmt_batch_autotune(None, 1, None, None) == dmt.DMT_ERR_INVALID_ARG
+    dmt.lib().dmt_batch_destroy(None)
