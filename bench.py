@@ -30,6 +30,14 @@ batched form renders each frame of all 64 scenes with one persistent-kernel
 launch replayed from a captured hipGraph, with parameters picked by
 dmt_batch_autotune.  Both run the same frames from frame 0; the images must be
 identical.  value = plain time / batched time over the timed frames.
+
+--workload adjoint: the differentiable form.  Config 4: 256 spheres with 256
+distinct materials over the ground plane, 512x512, 16 spp, depth 4, Russian
+roulette off.  The target image is rendered with the generator's materials;
+the spheres' parameters are then perturbed by N(0, 0.1) (albedo kept in
+[0.05, 1], emission >= 0.02).  A forward step is one dmt_render of the frame
+(to completion), a backward step one dmt_adjoint with dLoss/dI = 2 (I - target)
+(to completion).  value = backward time / forward time (lower is better).
 """
 import argparse
 import json
@@ -50,7 +58,7 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "batch"), default="c1")
+    ap.add_argument("--workload", choices=("c1", "batch", "adjoint"), default="c1")
     ap.add_argument("--steps", type=int, default=None, help="timed frames (c1: 20, batch: 100)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed frames (c1: 3, batch: 2)")
     ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
@@ -76,6 +84,10 @@ def main():
         args.steps = 100 if args.steps is None else args.steps
         args.warmup = 2 if args.warmup is None else args.warmup
         return bench_batch(args, np, torch, dmt)
+    if args.workload == "adjoint":
+        args.steps = 10 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        return bench_adjoint(args, np, torch, dmt)
     args.steps = 20 if args.steps is None else args.steps
     args.warmup = 3 if args.warmup is None else args.warmup
     t0 = time.time()
@@ -295,6 +307,85 @@ def bench_batch(args, np, torch, dmt):
     batch.close()
     for sc in scenes:
         sc.close()
+    dev.close()
+
+
+def bench_adjoint(args, np, torch, dmt):
+    sd = dmt.generate(4, seed=args.seed)
+    dev = dmt.Device(0)
+    sc = dmt.Scene(dev, sd)
+    f = sd.frame.copy()
+    f.flags = dmt.DMT_FRAME_NO_RUSSIAN_ROULETTE
+    f.reset_accum = 1
+    W, H = f.width, f.height
+    gt = sc.materials()
+    sc.render(f)
+    target = sc.readback(W, H)[..., :3].astype(np.float64)
+    rng = np.random.default_rng(args.seed + 100)
+    init = gt.copy()
+    init[1:] = gt[1:] + rng.normal(0.0, 0.1, size=gt[1:].shape).astype(np.float32)
+    init[1:, 0:3] = np.clip(init[1:, 0:3], 0.05, 1.0)
+    init[1:, 4:7] = np.maximum(init[1:, 4:7], 0.02)
+    sc.set_materials(init.astype(np.float32))
+
+    def sync():
+        dev.synchronize()
+        torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        sc.render(f)
+    sync()
+    fwd = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        sc.render(f)
+        sync()
+        fwd.append(time.perf_counter() - t0)
+    img = sc.readback(W, H)[..., :3].astype(np.float64)
+    loss = float(((img - target) ** 2).sum())
+    dimg = torch.zeros((H, W, 4), device="cuda", dtype=torch.float32)
+    dimg[..., :3] = torch.from_numpy((2.0 * (img - target)).astype(np.float32)).cuda()
+    grads = torch.empty((sd.prims.shape[0], dmt.MATERIAL_FLOATS), device="cuda", dtype=torch.float32)
+    sync()
+    for _ in range(args.warmup):
+        sc.adjoint(f, dimg.data_ptr(), grads.data_ptr())
+    sync()
+    bwd = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        sc.adjoint(f, dimg.data_ptr(), grads.data_ptr())
+        sync()
+        bwd.append(time.perf_counter() - t0)
+    g = grads.cpu().numpy()
+    fwd_ms = 1e3 * float(np.median(fwd))
+    bwd_ms = 1e3 * float(np.median(bwd))
+    log(f"[bench] forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms, loss {loss:.4g}, "
+        f"|grad| max {np.abs(g).max():.4g}, finite {bool(np.isfinite(g).all())}")
+    line = {
+        "metric": "bwd/fwd time ratio",
+        "value": round(bwd_ms / fwd_ms, 3),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(bwd_ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, config 4)",
+        "config": {"workload": "256 spheres with 256 materials + ground plane, quad light, 512x512, 16 spp, "
+                               "max depth 4, Russian roulette off; L2 image loss; gradients of 256 x (3 albedo + "
+                               "1 roughness + 3 emission)",
+                   "width": W, "height": H, "spp": f.spp, "max_bounces": f.max_bounces,
+                   "params": int(sd.prims.shape[0] - 1) * 7},
+        "forward_ms": round(fwd_ms, 4),
+        "backward_ms": round(bwd_ms, 4),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    sc.close()
     dev.close()
 
 

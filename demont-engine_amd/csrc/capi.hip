@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -15,6 +16,7 @@
 
 #include "../../include/dmt.h"
 #include "host/prep.h"
+#include "kernels/adjoint.h"
 #include "kernels/persistent.h"
 #include "kernels/wavefront.h"
 
@@ -104,6 +106,10 @@ struct dmt_scene {
     // to the last dmt_batch_destroy, and the resolution is fixed meanwhile.
     uint32_t batches = 0;
     bool destroy_requested = false;
+    // material parameters: host copy of the prims and the device array
+    std::vector<float> prims_host;
+    float4* prims_dev = nullptr;
+    uint32_t* adj_spill = nullptr;    // traversal spill area of the adjoint kernel
 };
 
 namespace {
@@ -248,13 +254,14 @@ dmt_status check_frame(const dmt_frame_params* f) {
     if (f->spp < 1 || f->spp > DMT_MAX_SPP) return fail(DMT_ERR_INVALID_ARG, "spp must be in 1..32");
     if (f->max_bounces > DMT_MAX_BOUNCES) return fail(DMT_ERR_LIMIT, "max_bounces must be <= 64");
     if (f->refract_bounces > 64) return fail(DMT_ERR_LIMIT, "refract_bounces must be <= 64");
+    if (f->flags & ~DMT_FRAME_NO_RUSSIAN_ROULETTE) return fail(DMT_ERR_INVALID_ARG, "unknown frame flags");
     return DMT_OK;
 }
 
 FrameArgs frame_args(const dmt_frame_params* f) {
     return make_frame_args(f->pos_fovtan, f->fwd_aspect, f->right_xyz, f->up_xyz, f->width, f->height,
                            f->frame_index, f->spp, f->max_bounces, f->reset_accum, f->mis_enabled,
-                           f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha);
+                           f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha, f->flags);
 }
 
 dmt_status harvest_events(dmt_scene* s) {
@@ -359,6 +366,12 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     };
     const float4* f4 = nullptr;
     if ((st = upload(s, (const float4*)P.prims.data(), (size_t)P.prim_count * 7u, &V.prims))) return bail(st);
+    s->prims_dev = const_cast<float4*>(V.prims);   // allocated by upload(); written by dmt_scene_set_materials
+    try {
+        s->prims_host = P.prims;
+    } catch (const std::bad_alloc&) {
+        return bail(fail(DMT_ERR_OUT_OF_MEMORY, "host allocation"));
+    }
     V.prim_count = P.prim_count;
     V.plane_count = (uint32_t)P.plane_ids.size();
     if ((st = upload(s, (const float4*)P.planes.data(), P.plane_ids.size(), &V.planes))) return bail(st);
@@ -421,6 +434,7 @@ void scene_free(dmt_scene* s) {
         for (hipEvent_t e : row)
             if (e) (void)hipEventDestroy(e);
     if (s->host_count) (void)hipHostFree(s->host_count);
+    if (s->adj_spill) (void)hipFree(s->adj_spill);
     for (void* p : s->buffers) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     dmt_device* d = s->dev;
@@ -625,6 +639,79 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     out->scene_bytes = s->scene_bytes;
     out->bvh_nodes = s->bvh_nodes;
     out->bvh_depth = s->bvh_depth;
+    return DMT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Material parameters and the adjoint.
+
+dmt_status dmt_scene_get_materials(dmt_scene* s, float* out, size_t float_count) {
+    if (!s || (!out && float_count)) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    const size_t n = s->prims_host.size() / 28u;
+    if (float_count != n * kMaterialFloats) return fail(DMT_ERR_INVALID_ARG, "float_count must be prim_count * 7");
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = &s->prims_host[i * 28u];
+        float* o = out + i * kMaterialFloats;
+        o[0] = p[4]; o[1] = p[5]; o[2] = p[6]; o[3] = p[7];
+        o[4] = p[16]; o[5] = p[17]; o[6] = p[18];
+    }
+    return DMT_OK;
+}
+
+dmt_status dmt_scene_set_materials(dmt_scene* s, const float* mats, size_t float_count) {
+    if (!s || (!mats && float_count)) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    const size_t n = s->prims_host.size() / 28u;
+    if (float_count != n * kMaterialFloats) return fail(DMT_ERR_INVALID_ARG, "float_count must be prim_count * 7");
+    for (size_t i = 0; i < float_count; ++i)
+        if (!std::isfinite(mats[i])) return fail(DMT_ERR_INVALID_ARG, "material " + std::to_string(i / 7) + " has a non-finite value");
+    if (n == 0) return DMT_OK;
+    // frames in flight read the current values
+    hipError_t e = sync_all(s->dev);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize");
+    for (size_t i = 0; i < n; ++i) {
+        float* p = &s->prims_host[i * 28u];
+        const float* m = mats + i * kMaterialFloats;
+        p[4] = m[0]; p[5] = m[1]; p[6] = m[2]; p[7] = m[3];
+        p[16] = m[4]; p[17] = m[5]; p[18] = m[6];
+    }
+    e = hipMemcpy(s->prims_dev, s->prims_host.data(), s->prims_host.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "material upload");
+    return DMT_OK;
+}
+
+dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_image_grad, void* d_param_grad) {
+    if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if ((st = check_frame(f))) return st;
+    if (f->max_bounces > kAdjMaxVerts) return fail(DMT_ERR_LIMIT, "the adjoint supports max_bounces <= 8");
+    if (f->refract_bounces > kAdjMaxDielectric) return fail(DMT_ERR_LIMIT, "the adjoint supports refract_bounces <= 8");
+    const uint32_t prims = s->view.prim_count;
+    if (prims && (!d_image_grad || !d_param_grad)) return fail(DMT_ERR_INVALID_ARG, "null device pointer");
+    if (prims == 0) return DMT_OK;
+    const uint32_t need = 3u * s->bvh_depth + 4u;
+    const uint32_t spill_depth = need > kLdsDepth ? need - kLdsDepth : 0u;
+    if (spill_depth && !s->adj_spill) {
+        hipError_t e = hipMalloc((void**)&s->adj_spill, (size_t)kMaxGridBlocks * kBlock * spill_depth * 4u);
+        if (e != hipSuccess) {
+            s->adj_spill = nullptr;
+            return hip_fail(e, "hipMalloc(adjoint spill)");
+        }
+    }
+    AdjointArgs A{};
+    A.S = s->view;
+    A.F = frame_args(f);
+    A.dimage = (const float4*)d_image_grad;
+    A.grads = (float*)d_param_grad;
+    A.prim_count = prims;
+    for (uint32_t i = 0; i < prims; ++i)
+        if (s->prims_host[(size_t)i * 28u + 9u] == 1.0f && s->prims_host[(size_t)i * 28u + 8u] == 0.0f) A.metal_prims++;
+    A.spill = s->adj_spill;
+    A.spill_depth = spill_depth;
+    hipError_t e = launch_adjoint(A, s->dev->stream);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "adjoint");
     return DMT_OK;
 }
 

@@ -223,6 +223,12 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
     // config 1 with spheres: config 0's spheres and ground plane join the mesh
     // (drawn after everything else, so the default stream is unchanged)
     if (config == 1u && o.spheres > 0) gen_spheres(rng, o.spheres, prims);
+    // config 4: every sphere also emits, emission U[0.02, 0.3] per channel, so
+    // that all 7 material parameters of a sphere have a two-sided derivative
+    // (a surface emits only when some channel is > 0)
+    if (config == 4u)
+        for (size_t i = 1; i < prims.size() / 28; ++i)
+            for (int k = 0; k < 3; ++k) prims[i * 28 + 16 + k] = rng.uf(0.02f, 0.3f);
     {
         const float lp[3] = {0.0f, 8.0f, 0.0f}, ln[3] = {0.0f, -1.0f, 0.0f}, lu[3] = {1.5f, 0.0f, 0.0f};
         const float I[3] = {10.0f, 10.0f, 10.0f};

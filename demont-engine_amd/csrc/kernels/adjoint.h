@@ -1,0 +1,54 @@
+// Adjoint of one frame by path replay: gradients of an image loss with
+// respect to the material parameters of the analytic primitives.
+//
+// The caller supplies dLoss/dI for every pixel of the frame image
+// I = frame_value(sum of the pixel's samples); each pixel's samples are
+// re-traced from their seeds (no tape) and every radiance term's derivative
+// is scattered to the parameters it depends on:
+//
+// * albedo (reverse mode): a term c = clampIndirect(v) depends on the albedo
+//   of every Lambert / metal vertex before it (through the throughput), on
+//   the albedo of its own vertex (Lambert NEE and the ambient term carry
+//   albedo / pi), and on the albedo of every dielectric its shadow query
+//   passes through (transmittance tint).  Scaling channel ch of v by (1 + e)
+//   changes c by e * (c_ch e_ch - [c clamped, ch = argmax v] c), so each
+//   term adds q = delta * c - [clamped] (delta . c) e_argmax to per-vertex
+//   accumulators; a vertex's accumulator divided by its albedo is the
+//   gradient (flushed when the path ends).
+// * emission: a term T * Le at the hit surface, with the clampIndirect
+//   Jacobian.
+// * roughness: forward mode.  For each metal vertex the sample is replayed
+//   again, in dual numbers from that vertex on, carrying d/d(roughness)
+//   through the reflected ray and everything downstream of it.
+//
+// Sampling decisions are held fixed (the derivative of the estimator with
+// its random numbers fixed, away from decision boundaries); with Russian
+// roulette on, its probability is treated as a constant.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "path.h"
+#include "wavefront.h"
+
+namespace dmt {
+
+constexpr uint32_t kAdjMaxVerts = 8;       // max_bounces limit of the adjoint
+constexpr uint32_t kAdjMaxDielectric = 8;  // refract_bounces limit of the adjoint
+constexpr uint32_t kAdjLdsPrims = 1024;    // per-workgroup LDS accumulation up to this many prims
+constexpr uint32_t kMaterialFloats = 7;    // albedo rgb, roughness, emission rgb
+
+struct AdjointArgs {
+    SceneView S;
+    FrameArgs F;
+    const float4* dimage;   // dLoss/dI per pixel (rgb; w unused)
+    float* grads;           // prim_count x kMaterialFloats
+    uint32_t prim_count;
+    uint32_t metal_prims;   // analytic metal prims (0: no roughness pass)
+    uint32_t* spill;
+    uint32_t spill_depth;
+};
+
+hipError_t launch_adjoint(const AdjointArgs& A, hipStream_t stream);
+
+}  // namespace dmt

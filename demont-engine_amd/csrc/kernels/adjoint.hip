@@ -1,0 +1,367 @@
+// Adjoint (path replay) kernel: see adjoint.h.
+#include "adjoint.h"
+#include "dual.h"
+
+namespace dmt {
+
+namespace {
+
+__device__ __forceinline__ uint32_t argmax3(f3 v) {
+    const float m = maxc(v);
+    return (v.x == m) ? 0u : ((v.y == m) ? 1u : 2u);
+}
+
+__device__ __forceinline__ float comp(f3 v, uint32_t k) { return k == 0u ? v.x : (k == 1u ? v.y : v.z); }
+
+__device__ __forceinline__ bool finite3(f3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+__device__ __forceinline__ void add_grad(float* g, uint32_t i, float v) {
+    if (v != 0.0f && isfinite(v)) atomicAdd(g + i, v);
+}
+
+// Path-replay sink: forms the radiance terms exactly as the renderer does
+// (shadow queries traced at once, in emission order) and scatters each
+// term's derivative.
+struct AdjointSink : NoGradientHooks {
+    const SceneView& S;
+    const FrameArgs& F;
+    Stack& st;
+    TraceCounters& tc;
+    float* G;                          // gradient accumulator (LDS or global)
+    f3 delta;                          // dLoss/dI of the pixel / spp
+    uint32_t n;                        // albedo vertices so far
+    uint32_t prim[kAdjMaxVerts];
+    f3 acc[kAdjMaxVerts];
+
+    // delta . dc / dlog(albedo_ch), per channel ch.
+    __device__ __forceinline__ f3 log_derivative(f3 v, f3 c, bool may_clamp) const {
+        f3 q = delta * c;
+        const bool clamped = may_clamp && F.firefly_clamp > 0.0f && maxc(v) > F.firefly_clamp;
+        if (clamped) {
+            const float dc = dot(delta, c);
+            const uint32_t k = argmax3(v);
+            if (k == 0u) q.x = q.x - dc;
+            else if (k == 1u) q.y = q.y - dc;
+            else q.z = q.z - dc;
+        }
+        return q;
+    }
+
+    __device__ __forceinline__ void add_prefix(f3 q) {
+#pragma unroll
+        for (uint32_t i = 0; i < kAdjMaxVerts; ++i)
+            if (i < n) acc[i] = acc[i] + q;
+    }
+
+    __device__ __forceinline__ void on_radiance(f3 v, f3 c, bool may_clamp) {
+        if (n > 0u) add_prefix(log_derivative(v, c, may_clamp));
+    }
+
+    __device__ __forceinline__ void on_emission(const Surface& sf, f3 T, f3 v, f3 c, bool may_clamp) {
+        if (sf.kind != 1u) return;
+        const float M = F.firefly_clamp;
+        const float m = maxc(v);
+        const bool clamped = may_clamp && M > 0.0f && m > M;
+        const float s = clamped ? (M / m) : 1.0f;
+        f3 g = (delta * T) * s;
+        if (clamped) {
+            const uint32_t k = argmax3(v);
+            const float corr = (dot(delta, c) * comp(T, k)) / m;
+            if (k == 0u) g.x = g.x - corr;
+            else if (k == 1u) g.y = g.y - corr;
+            else g.z = g.z - corr;
+        }
+        const uint32_t base = sf.prim * kMaterialFloats + 4u;
+        add_grad(G, base + 0u, g.x);
+        add_grad(G, base + 1u, g.y);
+        add_grad(G, base + 2u, g.z);
+    }
+
+    __device__ __forceinline__ void on_albedo_vertex(const Surface& sf) {
+        if (sf.kind != 1u || n >= kAdjMaxVerts) return;
+#pragma unroll
+        for (uint32_t i = 0; i < kAdjMaxVerts; ++i) {
+            if (i == n) {
+                prim[i] = sf.prim;
+                acc[i] = f3{0.0f, 0.0f, 0.0f};
+            }
+        }
+        n++;
+    }
+
+    __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState& P) {
+        if (kind == 1 && r.pdf < 0.0f) {   // ambient term: no query, never clamped
+            P.L = P.L + r.pre;
+            add_prefix(delta * r.pre);
+            return;
+        }
+        // transmittance, recording the dielectrics the query passes through
+        uint32_t diel[kAdjMaxDielectric];
+        uint32_t nd = 0;
+        f3 o = r.o, d = r.d, tr{1.0f, 1.0f, 1.0f};
+        float t_max = r.t_max;
+        bool done = false;
+        for (uint32_t i = 0; i <= F.refract_bounces && !done; ++i) {
+            const HitRec h = trace_closest<false>(S, o, d, st, tc);
+            done = transmit_step(S, F, h, i, o, d, t_max, tr);
+#pragma unroll
+            for (uint32_t j = 0; j < kAdjMaxDielectric; ++j)
+                if (!done && j == nd) diel[j] = h.code;
+            if (!done) nd++;
+        }
+        if (!done) tr = f3{0.0f, 0.0f, 0.0f};
+        if (!any_pos(tr)) return;
+        f3 v;
+        if (kind == 0) v = (((r.pre * tr) * r.brdf) * r.ndl) / r.pdf;
+        else v = ((((r.pre * tr) * r.brdf) * r.ndl) * r.w) / r.pdf;
+        const f3 c = r.clamp ? clamp_indirect(F, v) : v;
+        P.L = P.L + c;
+        const f3 q = log_derivative(v, c, r.clamp);
+        add_prefix(q);
+#pragma unroll
+        for (uint32_t j = 0; j < kAdjMaxDielectric; ++j) {
+            if (j < nd) {
+                const uint32_t pid = diel[j];
+                const float4 a = S.prims[(size_t)pid * 7u + 1u];
+                const uint32_t base = pid * kMaterialFloats;
+                if (a.x != 0.0f) add_grad(G, base + 0u, q.x / a.x);
+                if (a.y != 0.0f) add_grad(G, base + 1u, q.y / a.y);
+                if (a.z != 0.0f) add_grad(G, base + 2u, q.z / a.z);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void flush() {
+#pragma unroll
+        for (uint32_t i = 0; i < kAdjMaxVerts; ++i) {
+            if (i < n && finite3(acc[i])) {
+                const float4 a = S.prims[(size_t)prim[i] * 7u + 1u];
+                const uint32_t base = prim[i] * kMaterialFloats;
+                if (a.x != 0.0f) add_grad(G, base + 0u, acc[i].x / a.x);
+                if (a.y != 0.0f) add_grad(G, base + 1u, acc[i].y / a.y);
+                if (a.z != 0.0f) add_grad(G, base + 2u, acc[i].z / a.z);
+            }
+        }
+    }
+};
+
+template <bool LDS>
+__global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs A) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
+    __shared__ float lds_grad[LDS ? kAdjLdsPrims * kMaterialFloats : 1u];
+    const uint32_t nparam = A.prim_count * kMaterialFloats;
+    if (LDS) {
+        for (uint32_t i = threadIdx.x; i < nparam; i += blockDim.x) lds_grad[i] = 0.0f;
+        __syncthreads();
+    }
+    float* G = LDS ? lds_grad : A.grads;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    Stack stk{lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth};
+    TraceCounters tc{0u, 0u, 0u};
+    const FrameArgs& F = A.F;
+    const uint32_t npix = F.width * F.height;
+    for (uint32_t p = gid; p < npix; p += stride) {
+        const uint32_t x = p % F.width, y = p / F.width;
+        const f3 delta = xyz(A.dimage[p]) / (float)F.spp;
+        uint32_t seed = pixel_seed(F, x, y, F.frame_index);
+        for (uint32_t s = 0; s < F.spp; ++s) {
+            f3 d = camera_ray(F, x, y, seed);
+            f3 o = F.pos;
+            PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed};
+            AdjointSink sink{{}, A.S, F, stk, tc, G, delta, 0u, {}, {}};
+            for (uint32_t b = 0; b < F.max_bounces; ++b) {
+                const HitRec h = trace_closest<false>(A.S, o, d, stk, tc);
+                VertexResult R;
+                shade_vertex(A.S, F, b, o, d, h, P, R, sink);
+                if (!(R.alive && (b + 1u) < F.max_bounces)) break;
+                o = R.no;
+                d = R.nd;
+            }
+            seed = P.seed;
+            if (finite3(P.L)) sink.flush();
+        }
+    }
+    if (LDS) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nparam; i += blockDim.x) {
+            const float v = lds_grad[i];
+            if (v != 0.0f) atomicAdd(A.grads + i, v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Roughness: forward-mode tangents.  A metal vertex's roughness bends its
+// reflected ray, which moves every later hit, normal, light sample and shadow
+// query.  For every metal vertex j of a sample, the path is replayed from its
+// seed: float up to vertex j (no shadow queries: terms before j do not depend
+// on its roughness), then in dual numbers with d(roughness_j) = 1, and
+// delta . dL/d(roughness_j) goes to the vertex's primitive.  The hit set, the
+// material of every hit and every sampling decision are held fixed, so this
+// is the derivative away from visibility boundaries.
+
+// Records the metal vertices of a path (no shadow queries).
+struct MetalSink : NoGradientHooks {
+    uint32_t n;
+    uint32_t bounce[kAdjMaxVerts];
+    uint32_t prim[kAdjMaxVerts];
+    __device__ __forceinline__ void on_surface(uint32_t b, Surface& sf) {
+        if (sf.mat != 1u || sf.kind != 1u || n >= kAdjMaxVerts) return;
+#pragma unroll
+        for (uint32_t i = 0; i < kAdjMaxVerts; ++i) {
+            if (i == n) {
+                bounce[i] = b;
+                prim[i] = sf.prim;
+            }
+        }
+        n++;
+    }
+    __device__ __forceinline__ void emit(int, const ShadowRec&, PathVertexState&) {}
+};
+
+struct SkipSink : NoGradientHooks {
+    __device__ __forceinline__ void emit(int, const ShadowRec&, PathVertexState&) {}
+};
+
+// Dual vertices: roughness derivative seeded at bounce seed_b; shadow queries
+// traced at once along the dual ray.
+struct TangentSink : NoGradientHooks {
+    const SceneView& S;
+    const FrameArgs& F;
+    Stack& st;
+    TraceCounters& tc;
+    uint32_t seed_b;
+    __device__ __forceinline__ void on_surface(uint32_t b, SurfaceT<df>& sf) {
+        if (b == seed_b) sf.rough.d = 1.0f;
+    }
+    __device__ __forceinline__ void emit(int kind, const ShadowRecT<df>& r, PathVertexStateT<df>& P) {
+        if (kind == 1 && r.pdf < 0.0f) {
+            P.L = P.L + r.pre;
+            return;
+        }
+        d3 o = r.o, d = r.d;
+        d3 tr = to<df>(f3{1.0f, 1.0f, 1.0f});
+        df t_max = r.t_max;
+        bool done = false;
+        for (uint32_t i = 0; i <= F.refract_bounces && !done; ++i) {
+            const HitRec h = trace_closest<false>(S, value3(o), value3(d), st, tc);
+            done = transmit_step(S, F, h, i, o, d, t_max, tr);
+        }
+        if (!done || !any_pos(tr)) return;
+        P.L = P.L + shadow_value(F, kind, r.pre, r.ndl, r.brdf, r.pdf, r.w, r.clamp, tr);
+    }
+};
+
+template <bool LDS>
+__global__ void __launch_bounds__(kBlock) k_adjoint_rough(AdjointArgs A) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
+    __shared__ float lds_grad[LDS ? kAdjLdsPrims : 1u];
+    if (LDS) {
+        for (uint32_t i = threadIdx.x; i < A.prim_count; i += blockDim.x) lds_grad[i] = 0.0f;
+        __syncthreads();
+    }
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    Stack stk{lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth};
+    TraceCounters tc{0u, 0u, 0u};
+    const FrameArgs& F = A.F;
+    const uint32_t npix = F.width * F.height;
+    for (uint32_t p = gid; p < npix; p += stride) {
+        const f3 delta = xyz(A.dimage[p]) / (float)F.spp;
+        if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) continue;
+        const uint32_t x = p % F.width, y = p / F.width;
+        uint32_t seed = pixel_seed(F, x, y, F.frame_index);
+        for (uint32_t s = 0; s < F.spp; ++s) {
+            const uint32_t sample_seed = seed;
+            // the sample's metal vertices
+            MetalSink ms{{}, 0u, {}, {}};
+            {
+                f3 d = camera_ray(F, x, y, seed);
+                f3 o = F.pos;
+                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed};
+                for (uint32_t b = 0; b < F.max_bounces; ++b) {
+                    const HitRec h = trace_closest<false>(A.S, o, d, stk, tc);
+                    VertexResult R;
+                    shade_vertex(A.S, F, b, o, d, h, P, R, ms);
+                    if (!(R.alive && (b + 1u) < F.max_bounces)) break;
+                    o = R.no;
+                    d = R.nd;
+                }
+                seed = P.seed;
+            }
+            for (uint32_t j = 0; j < ms.n; ++j) {
+                uint32_t bj = 0u, pj = 0u;
+#pragma unroll
+                for (uint32_t k = 0; k < kAdjMaxVerts; ++k)
+                    if (k == j) {
+                        bj = ms.bounce[k];
+                        pj = ms.prim[k];
+                    }
+                uint32_t rs = sample_seed;
+                f3 d = camera_ray(F, x, y, rs);
+                f3 o = F.pos;
+                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, rs};
+                SkipSink skip{};
+                uint32_t b = 0;
+                for (; b < bj; ++b) {
+                    const HitRec h = trace_closest<false>(A.S, o, d, stk, tc);
+                    VertexResult R;
+                    shade_vertex(A.S, F, b, o, d, h, P, R, skip);
+                    o = R.no;
+                    d = R.nd;
+                }
+                d3 od = to<df>(o), dd = to<df>(d);
+                PathVertexStateT<df> PD{to<df>(P.T_), to<df>(f3{0.0f, 0.0f, 0.0f}), df(P.prev_pdf), P.skip_sky,
+                                        P.prev_lambert, P.seed};
+                TangentSink ts{{}, A.S, F, stk, tc, bj};
+                for (; b < F.max_bounces; ++b) {
+                    const HitRec h = trace_closest<false>(A.S, value3(od), value3(dd), stk, tc);
+                    VertexResultT<df> R;
+                    shade_vertex(A.S, F, b, od, dd, h, PD, R, ts);
+                    if (!(R.alive && (b + 1u) < F.max_bounces)) break;
+                    od = R.no;
+                    dd = R.nd;
+                }
+                const float g = dot(delta, deriv3(PD.L));
+                if (LDS) add_grad(lds_grad, pj, g);
+                else add_grad(A.grads, pj * kMaterialFloats + 3u, g);
+            }
+        }
+    }
+    if (LDS) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < A.prim_count; i += blockDim.x) {
+            const float v = lds_grad[i];
+            if (v != 0.0f) atomicAdd(A.grads + i * kMaterialFloats + 3u, v);
+        }
+    }
+}
+
+__global__ void k_fill_zero(float* p, uint32_t n) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
+}
+
+}  // namespace
+
+hipError_t launch_adjoint(const AdjointArgs& A, hipStream_t stream) {
+    const uint32_t nparam = A.prim_count * kMaterialFloats;
+    if (nparam) hipLaunchKernelGGL(k_fill_zero, dim3(min(1024u, (nparam + 255u) / 256u)), dim3(256), 0, stream, A.grads, nparam);
+    const uint32_t npix = A.F.width * A.F.height;
+    if (npix == 0 || A.F.max_bounces == 0 || nparam == 0) return hipGetLastError();
+    const uint32_t blocks = min(kMaxGridBlocks, (npix + kBlock - 1u) / kBlock);
+    if (A.prim_count <= kAdjLdsPrims)
+        hipLaunchKernelGGL(k_adjoint<true>, dim3(blocks), dim3(kBlock), 0, stream, A);
+    else
+        hipLaunchKernelGGL(k_adjoint<false>, dim3(blocks), dim3(kBlock), 0, stream, A);
+    if (A.metal_prims) {
+        if (A.prim_count <= kAdjLdsPrims)
+            hipLaunchKernelGGL(k_adjoint_rough<true>, dim3(blocks), dim3(kBlock), 0, stream, A);
+        else
+            hipLaunchKernelGGL(k_adjoint_rough<false>, dim3(blocks), dim3(kBlock), 0, stream, A);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dmt

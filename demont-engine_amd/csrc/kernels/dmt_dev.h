@@ -16,42 +16,63 @@
 namespace dmt {
 
 // ---------------------------------------------------------------------------
-// vectors
+// scalars and vectors
+//
+// The shading code is written once over a scalar type T: float for rendering,
+// and a dual number (value, derivative) for the adjoint's tangent pass.  The
+// helpers below are the float versions; the dual versions live with the dual
+// type.  Scalar arguments next to a vector are taken as T (non-deduced), so
+// float constants mix with either.
 
-struct f3 {
-    float x, y, z;
+template <class T> struct Id { using type = T; };
+
+__device__ __forceinline__ float sqrt_(float x) { return sqrtf(x); }
+__device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float fmin_(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float fabs_(float x) { return fabsf(x); }
+__device__ __forceinline__ float primal(float x) { return x; }
+
+template <class T> struct V3 {
+    T x, y, z;
 };
+using f3 = V3<float>;
+
 __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
-__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
-__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
-__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ f3 cross(f3 a, f3 b) {
-    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+template <class T> __device__ __forceinline__ V3<T> to(f3 v) { return V3<T>{T(v.x), T(v.y), T(v.z)}; }
+template <class T> __device__ __forceinline__ V3<T> operator+(V3<T> a, V3<T> b) { return V3<T>{a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> __device__ __forceinline__ V3<T> operator-(V3<T> a, V3<T> b) { return V3<T>{a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> __device__ __forceinline__ V3<T> operator-(V3<T> a) { return V3<T>{-a.x, -a.y, -a.z}; }
+template <class T> __device__ __forceinline__ V3<T> operator*(V3<T> a, V3<T> b) { return V3<T>{a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class T> __device__ __forceinline__ V3<T> operator*(V3<T> a, typename Id<T>::type s) {
+    return V3<T>{a.x * s, a.y * s, a.z * s};
 }
-__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ f3 normalize(f3 a) {
-    float inv = 1.0f / sqrtf(dot(a, a));
+template <class T> __device__ __forceinline__ V3<T> operator/(V3<T> a, typename Id<T>::type s) {
+    return V3<T>{a.x / s, a.y / s, a.z / s};
+}
+template <class T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class T> __device__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b) {
+    return V3<T>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> __device__ __forceinline__ T length(V3<T> a) { return sqrt_(dot(a, a)); }
+template <class T> __device__ __forceinline__ V3<T> normalize(V3<T> a) {
+    T inv = 1.0f / sqrt_(dot(a, a));
     return a * inv;
 }
-__device__ __forceinline__ f3 reflect(f3 i, f3 n) {
-    float d = dot(n, i);
+template <class T> __device__ __forceinline__ V3<T> reflect(V3<T> i, V3<T> n) {
+    T d = dot(n, i);
     return i - n * (2.0f * d);
 }
-__device__ __forceinline__ f3 refract(f3 i, f3 n, float eta) {
-    float ni = dot(n, i);
-    float k = 1.0f - (eta * eta) * (1.0f - ni * ni);
-    if (k < 0.0f) return f3{0.0f, 0.0f, 0.0f};
-    return i * eta - n * (eta * ni + sqrtf(k));
+template <class T> __device__ __forceinline__ V3<T> refract(V3<T> i, V3<T> n, float eta) {
+    T ni = dot(n, i);
+    T k = 1.0f - (eta * eta) * (1.0f - ni * ni);
+    if (k < 0.0f) return V3<T>{T(0.0f), T(0.0f), T(0.0f)};
+    return i * T(eta) - n * (eta * ni + sqrt_(k));
 }
-__device__ __forceinline__ float maxc(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
-__device__ __forceinline__ bool any_pos(f3 a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
-__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
-__device__ __forceinline__ f3 lerp3(f3 a, f3 b, float t) { return a + (b - a) * t; }
+template <class T> __device__ __forceinline__ T maxc(V3<T> a) { return fmax_(a.x, fmax_(a.y, a.z)); }
+template <class T> __device__ __forceinline__ bool any_pos(V3<T> a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
+template <class T> __device__ __forceinline__ T clampf(T x, float lo, float hi) { return fmin_(fmax_(x, T(lo)), T(hi)); }
+template <class T> __device__ __forceinline__ V3<T> lerp3(V3<T> a, V3<T> b, T t) { return a + (b - a) * t; }
 // quatRotate (PathTrace.slang:2028-2031).
 __device__ __forceinline__ f3 quat_rotate(float4 q, f3 v) {
     f3 u{q.x, q.y, q.z};
@@ -359,26 +380,28 @@ __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, 
 
 // Surface attributes at a hit (the HitInfo fill of testAnalyticPrim,
 // PathTrace.slang:2085-2150, and of the mesh path, PathTrace.slang:2640-2668).
-struct Surface {
-    f3 n;
+template <class T> struct SurfaceT {
+    V3<T> n;
     f3 albedo;
     f3 emission;
-    float rough, ior;
+    T rough;
+    float ior;
     uint32_t mat;
     uint32_t user_id;
     uint32_t prim;    // analytic input index or triangle id
     uint32_t kind;    // 1 analytic, 2 mesh
+    T t;              // hit distance
 };
+using Surface = SurfaceT<float>;
 
-__device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, HitRec h) {
-    Surface s;
+// The material fields of a hit (no normal): shared by the float and the dual
+// surface evaluation.
+template <class T> __device__ __forceinline__ void surface_material(const SceneView& S, HitRec h, SurfaceT<T>& s) {
     if (h.code & kMeshBit) {
-        const float4* tr = S.tris + (size_t)(h.code & ~kMeshBit) * 3u;
-        const float4 a = tr[0];
-        s.n = normalize(cross(xyz(tr[1]), xyz(tr[2])));
+        const float4 a = S.tris[(size_t)(h.code & ~kMeshBit) * 3u];
         s.albedo = f3{1.0f, 0.85f, 0.45f};   // MESH_ALBEDO (PathTrace.slang:1428)
         s.emission = f3{0.0f, 0.0f, 0.0f};
-        s.rough = 0.05f;
+        s.rough = T(0.05f);
         s.ior = 1.0f;
         s.mat = 1u;                           // MAT_METAL (PathTrace.slang:2655)
         s.user_id = 0u;
@@ -386,14 +409,9 @@ __device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, Hi
         s.kind = 2u;
     } else {
         const float4* p = S.prims + (size_t)h.code * 7u;
-        const float4 v0 = p[0], v1 = p[1], v2 = p[2], v4 = p[4];
-        if (v2.x == 0.0f) {
-            s.n = normalize((o + d * h.t) - xyz(v0));
-        } else {
-            s.n = quat_rotate(p[5], xyz(v0));
-        }
+        const float4 v1 = p[1], v2 = p[2], v4 = p[4];
         s.albedo = xyz(v1);
-        s.rough = v1.w;
+        s.rough = T(v1.w);
         s.mat = (uint32_t)v2.y;
         s.ior = v2.z;
         s.emission = xyz(v4);
@@ -401,7 +419,26 @@ __device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, Hi
         s.prim = h.code;
         s.kind = 1u;
     }
+}
+
+__device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, HitRec h) {
+    Surface s;
+    surface_material(S, h, s);
+    s.t = h.t;
+    if (h.code & kMeshBit) {
+        const float4* tr = S.tris + (size_t)(h.code & ~kMeshBit) * 3u;
+        s.n = normalize(cross(xyz(tr[1]), xyz(tr[2])));
+    } else {
+        const float4* p = S.prims + (size_t)h.code * 7u;
+        const float4 v0 = p[0], v2 = p[2];
+        if (v2.x == 0.0f) {
+            s.n = normalize((o + d * h.t) - xyz(v0));
+        } else {
+            s.n = quat_rotate(p[5], xyz(v0));
+        }
+    }
     return s;
 }
+__device__ __forceinline__ Surface surface_of(const SceneView& S, f3 o, f3 d, HitRec h) { return surface_at(S, o, d, h); }
 
 }  // namespace dmt

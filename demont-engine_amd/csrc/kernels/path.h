@@ -32,74 +32,104 @@ __device__ __forceinline__ uint32_t pixel_seed(const FrameArgs& F, uint32_t x, u
 
 // A shadow query and the factors of its contribution.  kind 0 = light NEE,
 // 1 = env NEE; pdf < 0 marks the no-env ambient term (added without tracing).
-struct ShadowRec {
-    f3 o, d;
-    float t_max;
-    f3 pre;        // throughput * Le (light) or throughput * env colour
-    float ndl;
-    f3 brdf;
-    float pdf;
-    float w;       // MIS weight (env)
+template <class T> struct ShadowRecT {
+    V3<T> o, d;
+    T t_max;
+    V3<T> pre;     // throughput * Le (light) or throughput * env colour
+    T ndl;
+    V3<T> brdf;
+    T pdf;
+    T w;           // MIS weight (env)
     bool clamp;    // clampIndirect applies (light NEE at b > 0; env always)
 };
+using ShadowRec = ShadowRecT<float>;
 
-struct PathVertexState {
-    f3 T, L;
-    float prev_pdf;
+template <class T> struct PathVertexStateT {
+    V3<T> T_, L;
+    T prev_pdf;
     bool skip_sky, prev_lambert;
     uint32_t seed;
 };
 
-struct VertexResult {
+using PathVertexState = PathVertexStateT<float>;
+
+template <class T> struct VertexResultT {
     bool alive;                  // the path scatters (before the depth limit)
-    f3 no, nd;                   // next ray
+    V3<T> no, nd;                // next ray
+};
+using VertexResult = VertexResultT<float>;
+
+// Observation points of shade_vertex for the adjoint (path replay) sinks;
+// the rendering sinks derive from this and ignore them.
+struct NoGradientHooks {
+    // A contribution added to L at this vertex: v before clampIndirect, c after.
+    template <class T> __device__ __forceinline__ void on_radiance(V3<T> v, V3<T> c, bool may_clamp) {}
+    // The emission term of hit surface sf (called before on_radiance for it).
+    template <class T>
+    __device__ __forceinline__ void on_emission(const SurfaceT<T>& sf, V3<T> T_, V3<T> v, V3<T> c, bool may_clamp) {}
+    // The throughput is about to be multiplied by sf.albedo (Lambert, metal);
+    // called before the vertex's own NEE terms, which also carry sf.albedo.
+    template <class T> __device__ __forceinline__ void on_albedo_vertex(const SurfaceT<T>& sf) {}
+    // The surface of bounce b before it is shaded (the tangent pass seeds a
+    // roughness derivative here).
+    template <class T> __device__ __forceinline__ void on_surface(uint32_t b, SurfaceT<T>& sf) {}
 };
 
 // Shades the vertex at bounce b for the ray (o, d) with closest hit h.
 // Shadow queries go to sink.emit(kind, record) as soon as they are formed
 // (light NEE before env NEE, the order their contributions are summed).
-template <class Sink>
-__device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs& F, uint32_t b, f3 o, f3 d,
-                                             HitRec h, PathVertexState& P, VertexResult& R, Sink& sink) {
+template <class T, class Sink>
+__device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs& F, uint32_t b, V3<T> o, V3<T> d,
+                                             HitRec h, PathVertexStateT<T>& P, VertexResultT<T>& R, Sink& sink) {
     const bool mis_on = F.mis_enabled != 0u;
     R.alive = false;
-    R.no = f3{0.0f, 0.0f, 0.0f};
-    R.nd = f3{0.0f, 1.0f, 0.0f};
+    R.no = to<T>(f3{0.0f, 0.0f, 0.0f});
+    R.nd = to<T>(f3{0.0f, 1.0f, 0.0f});
     if (h.code == kMiss) {
         if (mis_on && P.prev_lambert) {
             // transmittance(ro, rd, 1e30) would re-trace this missed ray and
             // return exactly 1 (PathTrace.slang:6081-6084).
-            float pe = pdf_env(S, d);
-            float w = mis_balance(P.prev_pdf, pe);
-            P.L = P.L + clamp_indirect(F, (P.T * sky_color(S, d)) * w);
+            T pe = pdf_env_of(S, d);
+            T w = mis_balance(P.prev_pdf, pe);
+            const V3<T> v = (P.T_ * sky_color(S, d)) * w;
+            const V3<T> c = clamp_indirect(F, v);
+            P.L = P.L + c;
+            sink.on_radiance(v, c, true);
         } else if (!P.skip_sky) {
-            f3 c = P.T * sky_color(S, d);
-            P.L = P.L + ((b == 0u) ? c : clamp_indirect(F, c));
+            V3<T> v = P.T_ * sky_color(S, d);
+            const V3<T> c = (b == 0u) ? v : clamp_indirect(F, v);
+            P.L = P.L + c;
+            sink.on_radiance(v, c, b != 0u);
         }
         return;
     }
-    const Surface sf = surface_at(S, o, d, h);
-    const f3 hp = o + d * h.t;
-    const float ndi = dot(sf.n, d);
-    const f3 nf = (ndi < 0.0f) ? sf.n : -sf.n;
+    SurfaceT<T> sf = surface_of(S, o, d, h);
+    sink.on_surface(b, sf);
+    const V3<T> hp = o + d * sf.t;
+    const T ndi = dot(sf.n, d);
+    const V3<T> nf = (ndi < 0.0f) ? sf.n : -sf.n;
     if (any_pos(sf.emission)) {
-        f3 c = P.T * sf.emission;
-        P.L = P.L + ((b == 0u) ? c : clamp_indirect(F, c));
+        V3<T> v = P.T_ * to<T>(sf.emission);
+        const V3<T> c = (b == 0u) ? v : clamp_indirect(F, v);
+        P.L = P.L + c;
+        sink.on_emission(sf, P.T_, v, c, b != 0u);
+        sink.on_radiance(v, c, b != 0u);
     }
     if (sf.mat == 0u) {
+        sink.on_albedo_vertex(sf);
         if (S.light_count > 0u) {
-            LightSample ls = sample_light(S, F, hp, nf, P.seed);
-            float ndl = dot(nf, ls.dir);
+            LightSampleT<T> ls = sample_light(S, F, hp, nf, P.seed);
+            T ndl = dot(nf, ls.dir);
             if (ls.pdf > 0.0f && ndl > 0.0f) {
-                ShadowRec r;
+                ShadowRecT<T> r;
                 r.o = hp + nf * 1e-3f;
                 r.d = ls.dir;
-                r.t_max = fmaxf(ls.dist - 1e-3f, 0.0f);
-                r.pre = P.T * ls.Le;
+                r.t_max = fmax_(ls.dist - 1e-3f, T(0.0f));
+                r.pre = P.T_ * ls.Le;
                 r.ndl = ndl;
-                r.brdf = sf.albedo / kPi;
+                r.brdf = to<T>(sf.albedo / kPi);
                 r.pdf = ls.pdf;
-                r.w = 1.0f;
+                r.w = T(1.0f);
                 r.clamp = b > 0u;
                 sink.emit(0, r, P);
             }
@@ -108,19 +138,19 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
             float x0 = randf(P.seed);
             float x1 = randf(P.seed);
             EnvSample e = sample_env(S, x0, x1);
-            float ndl = dot(nf, e.dir);
+            T ndl = dot(nf, to<T>(e.dir));
             if (e.pdf > 0.0f && ndl > 0.0f) {
-                float pb = ndl * (1.0f / kPi);
+                T pb = ndl * (1.0f / kPi);
                 bool fire = mis_on && (b + 1u) < F.max_bounces;
-                ShadowRec r;
+                ShadowRecT<T> r;
                 r.o = hp + nf * 1e-3f;
-                r.d = e.dir;
-                r.t_max = 1e30f;
-                r.pre = P.T * e.color;
+                r.d = to<T>(e.dir);
+                r.t_max = T(1e30f);
+                r.pre = P.T_ * to<T>(e.color);
                 r.ndl = ndl;
-                r.brdf = sf.albedo / kPi;
-                r.pdf = e.pdf;
-                r.w = fire ? mis_balance(e.pdf, pb) : 1.0f;
+                r.brdf = to<T>(sf.albedo / kPi);
+                r.pdf = T(e.pdf);
+                r.w = fire ? mis_balance(T(e.pdf), pb) : T(1.0f);
                 r.clamp = true;
                 sink.emit(1, r, P);
             }
@@ -128,35 +158,36 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
         } else {
             // No env map: ambient albedo * skyColorBase(nf) (PathTrace.slang:6667),
             // added after the light NEE term.
-            float tt = clampf(0.5f * (nf.y + 1.0f), 0.0f, 1.0f);
-            ShadowRec r;
+            T tt = clampf(0.5f * (nf.y + 1.0f), 0.0f, 1.0f);
+            ShadowRecT<T> r;
             r.o = hp;
-            r.d = f3{0.0f, 1.0f, 0.0f};
-            r.t_max = 0.0f;
-            r.pre = (P.T * sf.albedo) * lerp3(f3{0.5f, 0.55f, 0.62f}, f3{1.0f, 1.0f, 1.0f}, tt);
-            r.ndl = 0.0f;
-            r.brdf = f3{0.0f, 0.0f, 0.0f};
-            r.pdf = -1.0f;
-            r.w = 0.0f;
+            r.d = to<T>(f3{0.0f, 1.0f, 0.0f});
+            r.t_max = T(0.0f);
+            r.pre = (P.T_ * to<T>(sf.albedo)) * lerp3(to<T>(f3{0.5f, 0.55f, 0.62f}), to<T>(f3{1.0f, 1.0f, 1.0f}), tt);
+            r.ndl = T(0.0f);
+            r.brdf = to<T>(f3{0.0f, 0.0f, 0.0f});
+            r.pdf = T(-1.0f);
+            r.w = T(0.0f);
             r.clamp = false;
             sink.emit(1, r, P);
             P.skip_sky = true;
         }
-        P.T = P.T * sf.albedo;
+        P.T_ = P.T_ * to<T>(sf.albedo);
         R.no = hp + nf * 1e-3f;
         R.nd = cos_sample(nf, P.seed);
         if (mis_on && S.env_present) {
-            P.prev_pdf = fmaxf(0.0f, dot(nf, R.nd)) * (1.0f / kPi);
+            P.prev_pdf = fmax_(T(0.0f), dot(nf, R.nd)) * (1.0f / kPi);
             P.prev_lambert = true;
         } else {
             P.prev_lambert = false;
         }
         R.alive = true;
     } else if (sf.mat == 1u) {
-        f3 r = reflect(d, nf);
-        f3 cs = cos_sample(nf, P.seed);
+        sink.on_albedo_vertex(sf);
+        V3<T> r = reflect(d, nf);
+        V3<T> cs = cos_sample(nf, P.seed);
         r = normalize(r + (cs - nf) * sf.rough);
-        P.T = P.T * sf.albedo;
+        P.T_ = P.T_ * to<T>(sf.albedo);
         R.no = hp + nf * 1e-3f;
         R.nd = r;
         P.skip_sky = false;
@@ -166,13 +197,13 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
         float ia = (ndi < 0.0f) ? 1.0f : sf.ior;
         float ib = (ndi < 0.0f) ? sf.ior : 1.0f;
         float eta = ia / ib;
-        float ci = fabsf(dot(nf, -d));
+        T ci = fabs_(dot(nf, -d));
         float r0 = (ia - ib) / (ia + ib);
         r0 = r0 * r0;
-        float m = 1.0f - ci;
-        float fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
-        f3 refl = reflect(d, nf);
-        f3 refr = refract(d, nf, eta);
+        T m = 1.0f - ci;
+        T fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
+        V3<T> refl = reflect(d, nf);
+        V3<T> refr = refract(d, nf, eta);
         bool take_refl = length(refr) < 1e-3f;
         if (!take_refl) take_refl = randf(P.seed) < fr;
         if (take_refl) {
@@ -186,10 +217,10 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
         }
         R.alive = true;
     }
-    if (b >= 2u) {
-        float p = clampf(maxc(P.T), 0.05f, 1.0f);
+    if (b >= 2u && F.rr_enabled) {
+        T p = clampf(maxc(P.T_), 0.05f, 1.0f);
         if (randf(P.seed) > p) R.alive = false;
-        else P.T = P.T / p;
+        else P.T_ = P.T_ / p;
     }
 }
 
@@ -198,34 +229,35 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
 // beyond t_max keeps it, an opaque hit, the step limit or total internal
 // reflection zero it); otherwise tints t by (1 - F) * albedo and moves the
 // ray through the dielectric surface.
-__device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArgs& F, HitRec h, uint32_t i, f3& o,
-                                              f3& d, float& t_max, f3& t) {
-    if (h.code == kMiss || h.t >= t_max) return true;
-    const Surface sf = surface_at(S, o, d, h);
+template <class T>
+__device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArgs& F, HitRec h, uint32_t i, V3<T>& o,
+                                              V3<T>& d, T& t_max, V3<T>& t) {
+    if (h.code == kMiss || h.t >= primal(t_max)) return true;
+    const SurfaceT<T> sf = surface_of(S, o, d, h);
     if (sf.mat != 2u || i == F.refract_bounces) {
-        t = f3{0.0f, 0.0f, 0.0f};
+        t = to<T>(f3{0.0f, 0.0f, 0.0f});
         return true;
     }
-    float ndi = dot(sf.n, d);
-    f3 nf = (ndi < 0.0f) ? sf.n : -sf.n;
+    T ndi = dot(sf.n, d);
+    V3<T> nf = (ndi < 0.0f) ? sf.n : -sf.n;
     float ia = (ndi < 0.0f) ? 1.0f : sf.ior;
     float ib = (ndi < 0.0f) ? sf.ior : 1.0f;
     float eta = ia / ib;
-    float ci = fabsf(dot(nf, -d));
+    T ci = fabs_(dot(nf, -d));
     float q = (ia - ib) / (ia + ib);
     float r0 = q * q;
-    float m = 1.0f - ci;
-    float fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
-    f3 rf = refract(d, nf, eta);
+    T m = 1.0f - ci;
+    T fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
+    V3<T> rf = refract(d, nf, eta);
     if (length(rf) < 1e-3f) {
-        t = f3{0.0f, 0.0f, 0.0f};
+        t = to<T>(f3{0.0f, 0.0f, 0.0f});
         return true;
     }
-    t = t * (sf.albedo * (1.0f - fr));
-    float adv = h.t + 1e-3f;
+    t = t * (to<T>(sf.albedo) * (1.0f - fr));
+    T adv = sf.t + 1e-3f;
     o = o + d * adv;
     d = normalize(rf);
-    t_max = fmaxf(t_max - adv, 0.0f);
+    t_max = fmax_(t_max - adv, T(0.0f));
     return false;
 }
 
@@ -244,9 +276,10 @@ __device__ __forceinline__ f3 transmittance(const SceneView& S, const FrameArgs&
 }
 
 // Contribution of a shadow record with transmittance tr (tr > 0 somewhere).
-__device__ __forceinline__ f3 shadow_value(const FrameArgs& F, int kind, f3 pre, float ndl, f3 brdf, float pdf,
-                                           float w, bool clamp, f3 tr) {
-    f3 c;
+template <class T>
+__device__ __forceinline__ V3<T> shadow_value(const FrameArgs& F, int kind, V3<T> pre, T ndl, V3<T> brdf, T pdf,
+                                              T w, bool clamp, V3<T> tr) {
+    V3<T> c;
     if (kind == 0) c = (((pre * tr) * brdf) * ndl) / pdf;
     else c = ((((pre * tr) * brdf) * ndl) * w) / pdf;
     return clamp ? clamp_indirect(F, c) : c;

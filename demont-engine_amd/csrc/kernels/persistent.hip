@@ -10,7 +10,7 @@ constexpr uint32_t kEnded = 16u;          // pending bit: the path ends after th
 
 // Shadow records of one vertex go to this thread's scratch rows; bit k of
 // `pending` marks record k, bit k + 2 its clampIndirect flag.
-struct ScratchSink {
+struct ScratchSink : NoGradientHooks {
     float4* rec;
     uint32_t threads, tid;
     uint32_t pending;
@@ -63,7 +63,7 @@ struct Cold {
 __device__ __forceinline__ void start_sample(const FrameArgs& F, Lane& l, uint32_t x, uint32_t y) {
     l.d = camera_ray(F, x, y, l.P.seed);
     l.o = F.pos;
-    l.P.T = f3{1.0f, 1.0f, 1.0f};
+    l.P.T_ = f3{1.0f, 1.0f, 1.0f};
     l.P.L = f3{0.0f, 0.0f, 0.0f};
     l.P.prev_pdf = 0.0f;
     l.P.skip_sky = false;
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_persistent(BatchArgs A) {
             if (l.mode == 0u) {
                 if (l.b == 0u) n_primary++;
                 else n_ext++;
-                ScratchSink sink{A.rec, A.threads, tid, 0u};
+                ScratchSink sink{{}, A.rec, A.threads, tid, 0u};
                 VertexResult R;
                 shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;

@@ -21,17 +21,18 @@ struct FrameArgs {
     uint32_t refract_bounces;
     float firefly_clamp;
     float ema_alpha;
+    uint32_t rr_enabled;      // Russian roulette from bounce 2 on
 };
 
 // clampIndirect (PathTrace.slang:3367-3371).
-__device__ __forceinline__ f3 clamp_indirect(const FrameArgs& F, f3 c) {
+template <class T> __device__ __forceinline__ V3<T> clamp_indirect(const FrameArgs& F, V3<T> c) {
     if (F.firefly_clamp <= 0.0f) return c;
-    float m = maxc(c);
+    T m = maxc(c);
     return (m > F.firefly_clamp) ? c * (F.firefly_clamp / m) : c;
 }
 
 // sampleCosineHemisphere (PathTraceMath.slang:30-42).
-__device__ __forceinline__ f3 cos_sample(f3 n, uint32_t& seed) {
+template <class T> __device__ __forceinline__ V3<T> cos_sample(V3<T> n, uint32_t& seed) {
     float u1 = randf(seed);
     float u2 = randf(seed);
     float r = sqrtf(u1);
@@ -41,9 +42,9 @@ __device__ __forceinline__ f3 cos_sample(f3 n, uint32_t& seed) {
     float x = r * cp;
     float y = r * sp;
     float z = sqrtf(fmaxf(0.0f, 1.0f - u1));
-    f3 ax = fabsf(n.x) > 0.9f ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
-    f3 t = normalize(cross(n, ax));
-    f3 b = cross(n, t);
+    V3<T> ax = to<T>(fabs_(n.x) > 0.9f ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f});
+    V3<T> t = normalize(cross(n, ax));
+    V3<T> b = cross(n, t);
     return normalize((t * x + b * y) + n * z);
 }
 
@@ -53,24 +54,24 @@ __device__ __forceinline__ f3 env_texel(const SceneView& S, uint32_t x, uint32_t
 
 // skyColor (PathTrace.slang:3373-3431): bilinear lat-long lookup; without an
 // env map the base gradient (sun and moon below the horizon).
-__device__ __forceinline__ f3 sky_color(const SceneView& S, f3 rd) {
+template <class T> __device__ __forceinline__ V3<T> sky_color(const SceneView& S, V3<T> rd) {
     if (!S.env_present) {
-        float t = clampf(0.5f * (rd.y + 1.0f), 0.0f, 1.0f);
-        return lerp3(f3{0.5f, 0.55f, 0.62f}, f3{1.0f, 1.0f, 1.0f}, t);
+        T t = clampf(0.5f * (rd.y + 1.0f), 0.0f, 1.0f);
+        return lerp3(to<T>(f3{0.5f, 0.55f, 0.62f}), to<T>(f3{1.0f, 1.0f, 1.0f}), t);
     }
     const int W = (int)S.env_w, H = (int)S.env_h;
-    float u = c_atan2(rd.z, rd.x) * (1.0f / 6.28318530718f) + 0.5f;
-    float v = c_acos(clampf(rd.y, -1.0f, 1.0f)) * (1.0f / 3.14159265359f);
-    float fx = u * (float)W - 0.5f, fy = v * (float)H - 0.5f;
-    int ix = (int)floorf(fx), iy = (int)floorf(fy);
-    float ax = fx - (float)ix, ay = fy - (float)iy;
+    T u = c_atan2(rd.z, rd.x) * (1.0f / 6.28318530718f) + 0.5f;
+    T v = c_acos(clampf(rd.y, -1.0f, 1.0f)) * (1.0f / 3.14159265359f);
+    T fx = u * (float)W - 0.5f, fy = v * (float)H - 0.5f;
+    int ix = (int)floorf(primal(fx)), iy = (int)floorf(primal(fy));
+    T ax = fx - (float)ix, ay = fy - (float)iy;
     int x0 = (ix % W + W) % W;
     int x1 = (x0 + 1) % W;
     int y0 = min(max(iy, 0), H - 1);
     int y1 = min(max(iy + 1, 0), H - 1);
-    f3 c00 = env_texel(S, x0, y0), c10 = env_texel(S, x1, y0);
-    f3 c01 = env_texel(S, x0, y1), c11 = env_texel(S, x1, y1);
-    f3 s = lerp3(lerp3(c00, c10, ax), lerp3(c01, c11, ax), ay);
+    V3<T> c00 = to<T>(env_texel(S, x0, y0)), c10 = to<T>(env_texel(S, x1, y0));
+    V3<T> c01 = to<T>(env_texel(S, x0, y1)), c11 = to<T>(env_texel(S, x1, y1));
+    V3<T> s = lerp3(lerp3(c00, c10, ax), lerp3(c01, c11, ax), ay);
     return s * S.env_intensity;
 }
 
@@ -129,65 +130,73 @@ __device__ __forceinline__ float pdf_env(const SceneView& S, f3 d) {
     return (lum * (float)(W * H)) / (((2.0f * kPi) * kPi) * S.env_total);
 }
 
+// The env pdf is piecewise constant in the direction (one texel's
+// luminance): its derivative is zero, the generic form evaluates the float one.
+template <class T> __device__ __forceinline__ T pdf_env_of(const SceneView& S, V3<T> d) {
+    return T(pdf_env(S, f3{primal(d.x), primal(d.y), primal(d.z)}));
+}
+
 // misBalance (PathTrace.slang:3733-3735).
-__device__ __forceinline__ float mis_balance(float a, float b) { return a / fmaxf(a + b, 1e-12f); }
+template <class T> __device__ __forceinline__ T mis_balance(T a, T b) { return a / fmax_(a + b, T(1e-12f)); }
 
 // lightTreeImportance (PathTrace.slang:3929-3966).
-__device__ __forceinline__ float lt_importance(const float4* nd, f3 p, f3 nf) {
+template <class T> __device__ __forceinline__ T lt_importance(const float4* nd, V3<T> p, V3<T> nf) {
     const float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
     f3 mn = xyz(a), mx = xyz(b);
     f3 ce = (mn + mx) * 0.5f;
-    f3 to = ce - p;
-    float d2 = fmaxf(dot(to, to), 1e-12f);
-    float d = sqrtf(d2);
-    f3 dir = to / d;
+    V3<T> to_c = to<T>(ce) - p;
+    T d2 = fmax_(dot(to_c, to_c), T(1e-12f));
+    T d = sqrt_(d2);
+    V3<T> dir = to_c / d;
     f3 ext = (mx - mn) * 0.5f;
     float rad2 = fmaxf(dot(ext, ext), 1e-12f);
-    float fall = 1.0f / fmaxf(d2, rad2);
-    float orient = 1.0f;
+    T fall = 1.0f / fmax_(d2, T(rad2));
+    T orient = T(1.0f);
     if (c.w > -0.999f) {
-        float ce2 = -dot(xyz(c), dir);
-        float t = (ce2 - (c.w - 0.2f)) / 0.2f;
+        T ce2 = -dot(to<T>(xyz(c)), dir);
+        T t = (ce2 - (c.w - 0.2f)) / 0.2f;
         orient = clampf(t, 0.0f, 1.0f);
     }
-    float recv = fmaxf(0.0f, dot(nf, dir));
-    return fmaxf(((e.x * fall) * orient) * recv, 1e-8f);
+    T recv = fmax_(T(0.0f), dot(nf, dir));
+    return fmax_(((e.x * fall) * orient) * recv, T(1e-8f));
 }
 
-struct LightSample {
-    f3 dir, Le;
-    float pdf, dist;
+template <class T> struct LightSampleT {
+    V3<T> dir, Le;
+    T pdf, dist;
 };
+using LightSample = LightSampleT<float>;
 
 // pickLightFromTree + sampleAnalyticLight (PathTrace.slang:3979-4179).
-__device__ __forceinline__ LightSample sample_light(const SceneView& S, const FrameArgs& F, f3 p, f3 nf,
-                                                   uint32_t& seed) {
-    LightSample s{f3{0.0f, 1.0f, 0.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, 1e30f};
+template <class T>
+__device__ __forceinline__ LightSampleT<T> sample_light(const SceneView& S, const FrameArgs& F, V3<T> p, V3<T> nf,
+                                                       uint32_t& seed) {
+    LightSampleT<T> s{to<T>(f3{0.0f, 1.0f, 0.0f}), to<T>(f3{0.0f, 0.0f, 0.0f}), T(0.0f), T(1e30f)};
     const uint32_t n = S.light_count;
     if (n == 0u) return s;
     uint32_t pick;
-    float pick_pdf;
+    T pick_pdf;
     if (F.light_tree_enabled != 0u && S.ltree_count > 0u) {
         uint32_t idx = 0u, lid = 0u;
-        pick_pdf = 1.0f;
+        pick_pdf = T(1.0f);
         for (uint32_t dd = 0; dd < 32u; ++dd) {
             const float4* nd = S.ltree + (size_t)idx * 4u;
             const float4 v0 = nd[0], v1 = nd[1];
             if (__float_as_uint(v1.w) > 0u) { lid = __float_as_uint(v0.w); break; }
             const uint32_t l = __float_as_uint(v0.w), r = l + 1u;
-            float il = lt_importance(S.ltree + (size_t)l * 4u, p, nf);
-            float ir = lt_importance(S.ltree + (size_t)r * 4u, p, nf);
-            float tot = il + ir;
-            float pl = (tot > 1e-30f) ? (il / tot) : 0.5f;
+            T il = lt_importance(S.ltree + (size_t)l * 4u, p, nf);
+            T ir = lt_importance(S.ltree + (size_t)r * 4u, p, nf);
+            T tot = il + ir;
+            T pl = (tot > 1e-30f) ? (il / tot) : T(0.5f);
             float u = randf(seed);
-            if (u < pl) { idx = l; pick_pdf *= pl; }
-            else { idx = r; pick_pdf *= (1.0f - pl); }
+            if (u < pl) { idx = l; pick_pdf = pick_pdf * pl; }
+            else { idx = r; pick_pdf = pick_pdf * (1.0f - pl); }
         }
         pick = min(lid, n - 1u);
     } else {
         float u = randf(seed);
         pick = min((uint32_t)(u * (float)n), n - 1u);
-        pick_pdf = 1.0f / (float)n;
+        pick_pdf = T(1.0f / (float)n);
     }
     const float4* L = S.lights + (size_t)pick * 5u;
     const float4 l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3], l4 = L[4];
@@ -195,24 +204,24 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, const Fr
     const f3 pos = xyz(l0);
     const f3 I = xyz(l1);
     if (type == 0u) {   // point
-        f3 tl = pos - p;
-        float d2 = fmaxf(dot(tl, tl), 1e-12f);
-        float d = sqrtf(d2);
-        s.dir = tl / d; s.Le = I / d2; s.pdf = pick_pdf; s.dist = d;
+        V3<T> tl = to<T>(pos) - p;
+        T d2 = fmax_(dot(tl, tl), T(1e-12f));
+        T d = sqrt_(d2);
+        s.dir = tl / d; s.Le = to<T>(I) / d2; s.pdf = pick_pdf; s.dist = d;
         return s;
     }
     if (type == 1u) {   // spot
-        f3 tl = pos - p;
-        float d2 = fmaxf(dot(tl, tl), 1e-12f);
-        float d = sqrtf(d2);
-        f3 tld = tl / d;
+        V3<T> tl = to<T>(pos) - p;
+        T d2 = fmax_(dot(tl, tl), T(1e-12f));
+        T d = sqrt_(d2);
+        V3<T> tld = tl / d;
         f3 axis = normalize(quat_rotate(l4, xyz(l2)));
-        float cd = dot(-tld, axis);
+        T cd = dot(-tld, to<T>(axis));
         float ci = l3.x, co = l2.w;
-        float fo = 0.0f;
-        if (cd >= ci) fo = 1.0f;
+        T fo = T(0.0f);
+        if (cd >= ci) fo = T(1.0f);
         else if (cd > co) fo = (cd - co) / fmaxf(ci - co, 1e-6f);
-        s.dir = tld; s.Le = I * (fo / d2); s.pdf = pick_pdf; s.dist = d;
+        s.dir = tld; s.Le = to<T>(I) * (fo / d2); s.pdf = pick_pdf; s.dist = d;
         return s;
     }
     if (type == 2u) {   // sphere
@@ -225,15 +234,15 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, const Fr
         cephes_sincos(phi, &sp, &cp);
         f3 nl{r * cp, r * sp, z};
         f3 lp = pos + nl * l0.w;
-        f3 tl = lp - p;
-        float d2 = fmaxf(dot(tl, tl), 1e-12f);
-        float d = sqrtf(d2);
-        f3 wi = tl / d;
-        float cl = fmaxf(0.0f, dot(nl, -wi));
+        V3<T> tl = to<T>(lp) - p;
+        T d2 = fmax_(dot(tl, tl), T(1e-12f));
+        T d = sqrt_(d2);
+        V3<T> wi = tl / d;
+        T cl = fmax_(T(0.0f), dot(to<T>(nl), -wi));
         if (cl <= 0.0f) return s;
         float area = ((4.0f * kPi) * l0.w) * l0.w;
-        float pdf_sa = d2 / fmaxf(area * cl, 1e-12f);
-        s.dir = wi; s.Le = I; s.pdf = pick_pdf * pdf_sa; s.dist = d;
+        T pdf_sa = d2 / fmax_(area * cl, T(1e-12f));
+        s.dir = wi; s.Le = to<T>(I); s.pdf = pick_pdf * pdf_sa; s.dist = d;
         return s;
     }
     if (type == 3u) {   // quad
@@ -245,15 +254,15 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, const Fr
         f3 nq = normalize(quat_rotate(l4, xyz(l2)));
         f3 va = normalize(cross(nq, uv));
         f3 lp = (pos + uv * x0) + va * (l3.w * x1);
-        f3 tl = lp - p;
-        float d2 = fmaxf(dot(tl, tl), 1e-12f);
-        float d = sqrtf(d2);
-        f3 wi = tl / d;
-        float cl = dot(nq, -wi);
+        V3<T> tl = to<T>(lp) - p;
+        T d2 = fmax_(dot(tl, tl), T(1e-12f));
+        T d = sqrt_(d2);
+        V3<T> wi = tl / d;
+        T cl = dot(to<T>(nq), -wi);
         if (cl <= 0.0f) return s;
         float area = (4.0f * ul) * l3.w;
-        float pdf_sa = d2 / fmaxf(area * cl, 1e-12f);
-        s.dir = wi; s.Le = I; s.pdf = pick_pdf * pdf_sa; s.dist = d;
+        T pdf_sa = d2 / fmax_(area * cl, T(1e-12f));
+        s.dir = wi; s.Le = to<T>(I); s.pdf = pick_pdf * pdf_sa; s.dist = d;
         return s;
     }
     return s;

@@ -87,7 +87,7 @@ struct Wavefront {
 FrameArgs make_frame_args(const float pos_fovtan[4], const float fwd_aspect[4], const float right[4],
                           const float up[4], uint32_t width, uint32_t height, uint32_t frame_index, uint32_t spp,
                           uint32_t max_bounces, uint32_t reset_accum, uint32_t mis, uint32_t ltree,
-                          uint32_t refract, float clamp, float ema);
+                          uint32_t refract, float clamp, float ema, uint32_t flags);
 
 uint32_t grid_for(uint32_t n);
 

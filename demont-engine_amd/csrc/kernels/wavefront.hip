@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint
 }
 
 // Writes a path's shadow queries at its own slot of the iteration.
-struct SlotSink {
+struct SlotSink : NoGradientHooks {
     ShadowQueue* q;
     uint32_t slot, pix;
     bool emitted[2];
@@ -171,14 +171,14 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint
             const uint32_t b = meta & 0xFFu;
             const uint32_t flags = __float_as_uint(Lv.w);
             PathVertexState P{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[pix]};
-            SlotSink sink{sq, i, pix, {false, false}};
+            SlotSink sink{{}, sq, i, pix, {false, false}};
             shade_vertex(S, F, b, xyz(o4), xyz(d4), HitRec{__uint_as_float(hr.x), hr.y}, P, R, sink);
             for (int k = 0; k < 2; ++k)
                 if (!sink.emitted[k]) sq[k].d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
             if (b == 0u) n_primary++;
             else n_ext++;
             const uint32_t nflags = (P.skip_sky ? 1u : 0u) | (P.prev_lambert ? 2u : 0u);
-            ps.T[pix] = make_float4(P.T.x, P.T.y, P.T.z, P.prev_pdf);
+            ps.T[pix] = make_float4(P.T_.x, P.T_.y, P.T_.z, P.prev_pdf);
             ps.L[pix] = make_float4(P.L.x, P.L.y, P.L.z, __uint_as_float(nflags));
             ps.rng[pix] = P.seed;
             cont = R.alive && (b + 1u) < F.max_bounces;
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_query(SceneView S, const float
 FrameArgs make_frame_args(const float pf[4], const float fa[4], const float rt[4], const float up[4], uint32_t width,
                           uint32_t height, uint32_t frame_index, uint32_t spp, uint32_t max_bounces,
                           uint32_t reset_accum, uint32_t mis, uint32_t ltree, uint32_t refract, float clamp,
-                          float ema) {
+                          float ema, uint32_t flags) {
     FrameArgs F;
     F.pos = f3{pf[0], pf[1], pf[2]};
     F.fovtan = pf[3];
@@ -350,6 +350,7 @@ FrameArgs make_frame_args(const float pf[4], const float fa[4], const float rt[4
     F.refract_bounces = refract;
     F.firefly_clamp = clamp;
     F.ema_alpha = ema;
+    F.rr_enabled = (flags & 1u) ? 0u : 1u;   // DMT_FRAME_NO_RUSSIAN_ROULETTE
     return F;
 }
 

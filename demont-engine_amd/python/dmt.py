@@ -24,6 +24,9 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
+DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
+MATERIAL_FLOATS = 7
+
 PRIM_FLOATS = 28
 LIGHT_FLOATS = 20
 
@@ -45,7 +48,7 @@ class FrameParams(C.Structure):
         ("refract_bounces", C.c_uint32),
         ("firefly_clamp", C.c_float),
         ("ema_alpha", C.c_float),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
     def copy(self) -> "FrameParams":
@@ -161,6 +164,9 @@ def lib() -> C.CDLL:
         "dmt_build_bvh4": (st, [fp, u32, C.POINTER(u32), u32, u32, u32, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
         "dmt_build_light_tree": (u32, [fp, u32, fp]),
         "dmt_build_env_cdf": (C.c_double, [fp, u32, u32, fp, fp]),
+        "dmt_scene_get_materials": (st, [vp, fp, C.c_size_t]),
+        "dmt_scene_set_materials": (st, [vp, fp, C.c_size_t]),
+        "dmt_adjoint": (st, [vp, C.POINTER(FrameParams), vp, vp]),
         "dmt_batch_workload_scene": (st, [C.c_uint64, u32, C.POINTER(GenOptions), C.POINTER(C.c_uint64),
                                           C.POINTER(u32)]),
         "dmt_batch_create": (st, [vp, C.POINTER(vp), C.POINTER(FrameParams), u32, C.POINTER(BatchParams),
@@ -370,6 +376,20 @@ class Scene:
         s = Stats()
         _check(lib().dmt_scene_measure_traffic(self._h, C.byref(frame), C.byref(s)))
         return s.as_dict()
+
+    def materials(self) -> np.ndarray:
+        n = self.data.prims.shape[0]
+        out = np.zeros((n, MATERIAL_FLOATS), np.float32)
+        _check(lib().dmt_scene_get_materials(self._h, _fptr(out) if n else None, out.size))
+        return out
+
+    def set_materials(self, mats: np.ndarray):
+        m = np.ascontiguousarray(mats, np.float32)
+        _check(lib().dmt_scene_set_materials(self._h, _fptr(m), m.size))
+
+    def adjoint(self, frame: FrameParams, d_image_grad: int, d_param_grad: int):
+        """dLoss/dparams (device prim_count x 7 floats) from dLoss/dI (device W x H float4)."""
+        _check(lib().dmt_adjoint(self._h, C.byref(frame), C.c_void_p(d_image_grad), C.c_void_p(d_param_grad)))
 
     def close(self):
         if self._h:

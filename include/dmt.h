@@ -92,8 +92,15 @@ typedef struct dmt_frame_params {
     uint32_t refract_bounces;   /* r_refract_bounces (transmittance budget)  */
     float    firefly_clamp;     /* r_firefly_clamp, <= 0 disables            */
     float    ema_alpha;         /* r_accum_ema_alpha                         */
-    uint32_t reserved;
+    uint32_t flags;             /* DMT_FRAME_* bits; 0 = reference behaviour */
 } dmt_frame_params;
+
+/* Russian roulette off (the reference applies it from bounce 2 on,
+ * PathTrace.slang:6908-6913).  Its kill test compares a random number with
+ * a throughput-dependent probability, a decision that flips under parameter
+ * perturbations; differentiable renders checked against finite differences
+ * turn it off. */
+#define DMT_FRAME_NO_RUSSIAN_ROULETTE 1u
 
 typedef struct dmt_build_params {
     uint32_t sah_bins;          /* binned-SAH bin count (0 = default 16)     */
@@ -182,6 +189,34 @@ dmt_status dmt_scene_reset_stats(dmt_scene* scene);
 dmt_status dmt_scene_measure_traffic(dmt_scene* scene, const dmt_frame_params* frame, dmt_stats* out);
 /* Per-event timing of the closest-hit kernel launches on/off (default off). */
 dmt_status dmt_scene_set_profiling(dmt_scene* scene, int32_t enabled);
+
+/* ---- Material parameters and the adjoint -----------------------------------
+ * Per analytic primitive, 7 floats: albedo rgb (v1.xyz), roughness (v1.w),
+ * emission rgb (v4.xyz) of the primitive wire format.  Replaces re-uploading
+ * the prim buffer after a material edit (Engine.cpp:4743-5002 re-packs all
+ * prims); geometry and acceleration data are unchanged. */
+#define DMT_MATERIAL_FLOATS 7u
+dmt_status dmt_scene_get_materials(dmt_scene* scene, float* out, size_t float_count);
+/* Waits for the scene's frames in flight, then updates the device prims. */
+dmt_status dmt_scene_set_materials(dmt_scene* scene, const float* materials, size_t float_count);
+
+/* No counterpart in the reference (it has no differentiable renderer).
+ * Gradient of an image loss with respect to every analytic primitive's
+ * material parameters, for the frame `frame` renders with reset_accum set
+ * (image I = that frame's value): d_image_grad is a device array of
+ * width*height float4 holding dLoss/dI (rgb; w ignored), d_param_grad a
+ * device array of prim_count*7 floats that receives dLoss/dparams
+ * (overwritten).  Path replay: the samples are re-traced from their seeds,
+ * no tape is stored.  Albedo and emission gradients are exact derivatives of
+ * the estimator with its random numbers fixed; the roughness gradient is the
+ * derivative of the estimator away from visibility boundaries (decisions
+ * held fixed).  With Russian roulette on, its probability is treated as a
+ * constant (set DMT_FRAME_NO_RUSSIAN_ROULETTE for a gradient that finite
+ * differences reproduce).  Limits: max_bounces <= 8, refract_bounces <= 8.
+ * Stream-ordered on the device stream; d_image_grad must be complete when
+ * the call is made. */
+dmt_status dmt_adjoint(dmt_scene* scene, const dmt_frame_params* frame, const void* d_image_grad,
+                       void* d_param_grad);
 
 /* ---- Batched persistent form ---------------------------------------------
  * No one-for-one counterpart in the reference: its engine renders one scene

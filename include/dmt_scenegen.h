@@ -14,8 +14,9 @@
  *             seeded value noise (amplitude 0.1), plus 200k random triangles
  *             (edges <= 0.05) uniform in [-5,5]^3; one quad light; 512x256
  *             env with lognormal texels exp(N(0,1)); 1920x1080, 1 spp, 8 bounces.
- *   config 4  config 0 with 256 spheres, 256 distinct materials; 512x512,
- *             16 spp, 4 bounces.
+ *   config 4  config 0 with 256 spheres, 256 distinct materials (each
+ *             sphere also emits U[0.02, 0.3] per channel); 512x512, 16 spp,
+ *             4 bounces.
  *   config 1 with options.spheres > 0 adds config 0's spheres and plane (used
  *             by the batched workload, see dmt_batch_workload_scene).
  * Camera for all: at (0, 2, 12) looking at the origin, vertical fov 50 deg.

@@ -45,7 +45,7 @@ typedef struct orc_frame {
     uint32_t refract_bounces;
     float    firefly_clamp;
     float    ema_alpha;
-    uint32_t reserved;
+    uint32_t flags;      /* bit 0: no Russian roulette (same bits as dmt_frame_params) */
 } orc_frame;
 
 typedef struct orc_hit {
@@ -79,6 +79,19 @@ void orc_scene_destroy(orc_scene* s);
 int orc_render(const orc_scene* s, const orc_frame* f, int precision, int threads,
                uint32_t row_begin, uint32_t row_end, uint32_t row_step, uint32_t col_step,
                float* accum_rgba, orc_counts* counts);
+
+/* fp64 tangent-linear oracle: renders the frame in dual numbers along the
+ * direction of material parameter `param` (0-2 albedo rgb, 3 roughness, 4-6
+ * emission rgb) of analytic prim `prim` and returns
+ * sum_p sum_c dimage[p].c * dI[p].c/dparam in *out_dloss, where I is the frame
+ * value (spp mean) and dimage is width*height float4 (rgb used).  Decisions
+ * follow the fp64 values, so this is the exact derivative of the fp64
+ * estimator with its random numbers fixed.  out_dimage (optional, float4 per
+ * pixel) receives dI/dparam. */
+int orc_tangent(orc_scene* s, const orc_frame* f, uint32_t prim, uint32_t param, const float* dimage,
+                int threads, double* out_dloss, float* out_dimage);
+/* The fp64 frame value (spp mean, sanitised) of every pixel, rgb doubles. */
+int orc_render_f64_image(orc_scene* s, const orc_frame* f, int threads, double* out_rgb);
 
 /* Closest-hit query (traceScene) for n rays: rays are 8 floats each
  * (ox, oy, oz, unused, dx, dy, dz, unused). */

@@ -22,7 +22,7 @@ class OrcSceneDesc(C.Structure):
         ("indices", C.POINTER(C.c_uint32)), ("index_count", C.c_uint32),
         ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
         ("env_intensity", C.c_float),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -71,6 +71,10 @@ def lib():
         L.orc_light_pmf.restype = C.c_int
         L.orc_math_f32.argtypes = [C.c_int, fp, fp, fp, u32]
         L.orc_math_f32.restype = None
+        L.orc_tangent.argtypes = [vp, vp, u32, u32, fp, C.c_int, C.POINTER(C.c_double), fp]
+        L.orc_tangent.restype = C.c_int
+        L.orc_render_f64_image.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double)]
+        L.orc_render_f64_image.restype = C.c_int
         _lib = L
     return _lib
 
@@ -123,6 +127,31 @@ class OracleScene:
         if rc != 0:
             raise ValueError("oracle render failed")
         return accum, cnt.as_dict()
+
+    def tangent(self, frame, prim, param, dimage, threads=0, want_image=False):
+        """fp64 tangent-linear derivative sum_p dimage[p] . dI[p]/d(param of prim)
+        (param 0-2 albedo rgb, 3 roughness, 4-6 emission rgb)."""
+        if threads <= 0:
+            threads = min(16, os.cpu_count() or 1)
+        di = np.ascontiguousarray(dimage, np.float32)
+        out = C.c_double()
+        img = np.zeros((frame.height, frame.width, 4), np.float32) if want_image else None
+        rc = lib().orc_tangent(self._h, C.cast(C.byref(frame), C.c_void_p), prim, param, _fp(di), threads,
+                               C.byref(out), _fp(img) if want_image else None)
+        if rc != 0:
+            raise ValueError("oracle tangent failed")
+        return (out.value, img) if want_image else out.value
+
+    def render_f64(self, frame, threads=0):
+        """fp64 frame value per pixel (H, W, 3) doubles."""
+        if threads <= 0:
+            threads = min(16, os.cpu_count() or 1)
+        out = np.zeros((frame.height, frame.width, 3), np.float64)
+        rc = lib().orc_render_f64_image(self._h, C.cast(C.byref(frame), C.c_void_p), threads,
+                                        out.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise ValueError("oracle f64 render failed")
+        return out
 
     def trace(self, origins, dirs, brute=False):
         n = origins.shape[0]

@@ -24,6 +24,50 @@
 
 namespace orc {
 
+// ---- dual numbers (fp64 tangent-linear mode) ------------------------------
+// Dual{v, d}: value and derivative along one parameter direction.  Arithmetic
+// applies the chain rule; comparisons and casts use the value only, so a
+// PT<Dual> run takes the decisions of the fp64 run and returns the exact
+// derivative of the estimator with its random numbers fixed.
+struct Dual {
+    double v = 0.0, d = 0.0;
+    constexpr Dual() = default;
+    constexpr Dual(double x) : v(x), d(0.0) {}
+    constexpr Dual(double x, double dx) : v(x), d(dx) {}
+    explicit operator double() const { return v; }
+    explicit operator float() const { return (float)v; }
+    explicit operator int() const { return (int)v; }
+    explicit operator uint32_t() const { return (uint32_t)v; }
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
+inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+inline bool operator==(Dual a, Dual b) { return a.v == b.v; }
+inline bool operator!=(Dual a, Dual b) { return a.v != b.v; }
+inline Dual osqrt(Dual a) {
+    const double s = std::sqrt(a.v);
+    return {s, s > 0.0 ? a.d / (2.0 * s) : 0.0};
+}
+inline Dual oabs(Dual a) { return a.v < 0.0 ? -a : a; }
+inline Dual ofloor(Dual a) { return {std::floor(a.v), 0.0}; }
+inline float osqrt(float a) { return std::sqrt(a); }
+inline double osqrt(double a) { return std::sqrt(a); }
+inline float oabs(float a) { return std::fabs(a); }
+inline double oabs(double a) { return std::fabs(a); }
+inline float ofloor(float a) { return std::floor(a); }
+inline double ofloor(double a) { return std::floor(a); }
+inline double value_of(float a) { return a; }
+inline double value_of(double a) { return a; }
+inline double value_of(Dual a) { return a.v; }
+
 template <class R> struct V3 {
     R x, y, z;
 };
@@ -40,11 +84,11 @@ template <class R> inline R dot(V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.
 template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline R length(V3<R> a) { return osqrt(dot(a, a)); }
 // normalize(v) = v * (1 / sqrt(dot(v, v))): one correctly rounded sqrt, one
 // correctly rounded division and three multiplies.
 template <class R> inline V3<R> normalize(V3<R> a) {
-    R inv = R(1) / std::sqrt(dot(a, a));
+    R inv = R(1) / osqrt(dot(a, a));
     return a * inv;
 }
 template <class R> inline V3<R> reflect(V3<R> i, V3<R> n) {
@@ -56,7 +100,7 @@ template <class R> inline V3<R> refract(V3<R> i, V3<R> n, R eta) {
     R ni = dot(n, i);
     R k = R(1) - (eta * eta) * (R(1) - ni * ni);
     if (k < R(0)) return {R(0), R(0), R(0)};
-    return i * eta - n * (eta * ni + std::sqrt(k));
+    return i * eta - n * (eta * ni + osqrt(k));
 }
 template <class R> inline R maxc(V3<R> a) { return std::max(a.x, std::max(a.y, a.z)); }
 template <class R> inline bool any_pos(V3<R> a) { return a.x > R(0) || a.y > R(0) || a.z > R(0); }
@@ -158,9 +202,23 @@ struct MathD {
     static double atan2(double y, double x) { return std::atan2(y, x); }
 };
 
+struct MathDual {
+    static Dual sin(Dual x) { return {std::sin(x.v), std::cos(x.v) * x.d}; }
+    static Dual cos(Dual x) { return {std::cos(x.v), -std::sin(x.v) * x.d}; }
+    static Dual acos(Dual x) {
+        const double s = std::sqrt(std::max(0.0, 1.0 - x.v * x.v));
+        return {std::acos(x.v), s > 0.0 ? -x.d / s : 0.0};
+    }
+    static Dual atan2(Dual y, Dual x) {
+        const double r2 = x.v * x.v + y.v * y.v;
+        return {std::atan2(y.v, x.v), r2 > 0.0 ? (x.v * y.d - y.v * x.d) / r2 : 0.0};
+    }
+};
+
 template <class R> struct MathOf;
 template <> struct MathOf<float> { using M = MathF; };
 template <> struct MathOf<double> { using M = MathD; };
+template <> struct MathOf<Dual> { using M = MathDual; };
 
 // PCG hash + 24-bit uniform (PathTraceMath.slang:16-24).
 inline uint32_t pcg_hash(uint32_t x) {

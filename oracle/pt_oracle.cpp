@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <limits>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 using namespace orc;
@@ -339,6 +340,10 @@ struct orc_scene {
     float env_intensity = 1.0f;
     std::vector<float> marg, cond;
     float env_total = 0.0f;
+    // tangent direction of a PT<Dual> run: material parameter k (albedo rgb,
+    // roughness, emission rgb) of analytic prim tangent_prim
+    uint32_t tangent_prim = 0xFFFFFFFFu;
+    uint32_t tangent_param = 0u;
 };
 
 namespace {
@@ -396,7 +401,7 @@ template <class R> void test_prim(const orc_scene& S, uint32_t i, V3<R> ro, V3<R
         R k = dot(oc, oc) - rad * rad;
         R hh = b * b - k;
         if (hh < R(0)) return;
-        hh = std::sqrt(hh);
+        hh = osqrt(hh);
         R t0 = -b - hh, t1 = -b + hh;
         t = (t0 > R(1e-3)) ? t0 : t1;
         if (!(t > R(1e-3))) return;
@@ -407,7 +412,7 @@ template <class R> void test_prim(const orc_scene& S, uint32_t i, V3<R> ro, V3<R
         R q[4] = {R(p[20]), R(p[21]), R(p[22]), R(p[23])};
         V3<R> pn = quat_rotate(q, ld3<R>(p));
         R denom = dot(rd, pn);
-        if (std::fabs(denom) < R(1e-4)) return;
+        if (oabs(denom) < R(1e-4)) return;
         t = -(dot(ro, pn) + R(p[3])) / denom;
         if (!(t > R(1e-3))) return;
         if (!better(t, i, h, 1u)) return;
@@ -424,6 +429,19 @@ template <class R> void test_prim(const orc_scene& S, uint32_t i, V3<R> ro, V3<R
     h.user_id = f2u(p[11]);
     h.kind = 1u;
     h.prim = i;
+    if constexpr (std::is_same_v<R, Dual>) {
+        if (i == S.tangent_prim) {
+            switch (S.tangent_param) {
+                case 0: h.albedo.x.d = 1.0; break;
+                case 1: h.albedo.y.d = 1.0; break;
+                case 2: h.albedo.z.d = 1.0; break;
+                case 3: h.rough.d = 1.0; break;
+                case 4: h.emission.x.d = 1.0; break;
+                case 5: h.emission.y.d = 1.0; break;
+                default: h.emission.z.d = 1.0; break;
+            }
+        }
+    }
 }
 
 template <class R> struct TriT {
@@ -440,7 +458,7 @@ template <class R> inline TriT<R> tri_test(const orc_scene& S, uint32_t tri, V3<
     V3<R> e1 = v1 - v0, e2 = v2 - v0;
     V3<R> pv = cross(rd, e2);
     R det = dot(e1, pv);
-    if (std::fabs(det) < R(1e-8)) return {false, R(0)};
+    if (oabs(det) < R(1e-8)) return {false, R(0)};
     R inv_det = R(1) / det;
     V3<R> tv = ro - v0;
     R u = dot(tv, pv) * inv_det;
@@ -471,7 +489,7 @@ template <class R> void mesh_fill(const orc_scene& S, uint32_t tri, R t, Hit<R>&
 }
 
 template <class R> inline V3<R> safe_inv(V3<R> rd) {
-    auto f = [](R x) { return std::fabs(x) > R(1e-30) ? x : R(1e-30); };
+    auto f = [](R x) { return oabs(x) > R(1e-30) ? x : R(1e-30); };
     return V3<R>{R(1) / f(rd.x), R(1) / f(rd.y), R(1) / f(rd.z)};
 }
 
@@ -555,12 +573,12 @@ template <class R> struct PT {
     V3<R> cos_sample(V3<R> n, uint32_t& seed) const {
         R u1 = randf<R>(seed);
         R u2 = randf<R>(seed);
-        R r = std::sqrt(u1);
+        R r = osqrt(u1);
         R phi = R(6.28318530718) * u2;
         R x = r * M::cos(phi);
         R y = r * M::sin(phi);
-        R z = std::sqrt(std::max(R(0), R(1) - u1));
-        V3<R> ax = std::fabs(n.x) > R(0.9) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
+        R z = osqrt(std::max(R(0), R(1) - u1));
+        V3<R> ax = oabs(n.x) > R(0.9) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
         V3<R> t = normalize(cross(n, ax));
         V3<R> b = cross(n, t);
         return normalize((t * x + b * y) + n * z);
@@ -583,7 +601,7 @@ template <class R> struct PT {
         R u = M::atan2(rd.z, rd.x) * (R(1) / R(6.28318530718)) + R(0.5);
         R v = M::acos(clampv(rd.y, R(-1), R(1))) * (R(1) / R(3.14159265359));
         R fx = u * R(W) - R(0.5), fy = v * R(H) - R(0.5);
-        R flx = std::floor(fx), fly = std::floor(fy);
+        R flx = ofloor(fx), fly = ofloor(fy);
         int ix = (int)flx, iy = (int)fly;
         R ax = fx - R(ix), ay = fy - R(iy);
         int x0 = (ix % W + W) % W;
@@ -652,7 +670,7 @@ template <class R> struct PT {
         V3<R> ce = (mn + mx) * R(0.5);
         V3<R> to = ce - p;
         R d2 = std::max(dot(to, to), R(1e-12));
-        R d = std::sqrt(d2);
+        R d = osqrt(d2);
         V3<R> dir = to / d;
         V3<R> ext = (mx - mn) * R(0.5);
         R rad2 = std::max(dot(ext, ext), R(1e-12));
@@ -707,14 +725,14 @@ template <class R> struct PT {
         if (type == LIGHT_POINT) {
             V3<R> tl = pos - p;
             R d2 = std::max(dot(tl, tl), R(1e-12));
-            R d = std::sqrt(d2);
+            R d = osqrt(d2);
             s.dir = tl / d; s.Le = I / d2; s.pdf = pick_pdf; s.dist = d;
             return s;
         }
         if (type == LIGHT_SPOT) {
             V3<R> tl = pos - p;
             R d2 = std::max(dot(tl, tl), R(1e-12));
-            R d = std::sqrt(d2);
+            R d = osqrt(d2);
             V3<R> tld = tl / d;
             V3<R> axis = normalize(quat_rotate(q, ld3<R>(L + 8)));
             R cd = dot(-tld, axis);
@@ -729,13 +747,13 @@ template <class R> struct PT {
             R x0 = randf<R>(seed);
             R x1 = randf<R>(seed);
             R z = R(1) - R(2) * x0;
-            R r = std::sqrt(std::max(R(0), R(1) - z * z));
+            R r = osqrt(std::max(R(0), R(1) - z * z));
             R phi = R(6.28318530717958) * x1;
             V3<R> nl{r * M::cos(phi), r * M::sin(phi), z};
             V3<R> lp = pos + nl * radius;
             V3<R> tl = lp - p;
             R d2 = std::max(dot(tl, tl), R(1e-12));
-            R d = std::sqrt(d2);
+            R d = osqrt(d2);
             V3<R> wi = tl / d;
             R cl = std::max(R(0), dot(nl, -wi));
             if (cl <= R(0)) return s;
@@ -755,7 +773,7 @@ template <class R> struct PT {
             V3<R> lp = (pos + uv * x0) + va * (R(L[15]) * x1);
             V3<R> tl = lp - p;
             R d2 = std::max(dot(tl, tl), R(1e-12));
-            R d = std::sqrt(d2);
+            R d = osqrt(d2);
             V3<R> wi = tl / d;
             R cl = dot(nq, -wi);
             if (cl <= R(0)) return s;
@@ -782,7 +800,7 @@ template <class R> struct PT {
             R ia = (ndi < R(0)) ? R(1) : h.ior;
             R ib = (ndi < R(0)) ? h.ior : R(1);
             R eta = ia / ib;
-            R ci = std::fabs(dot(nf, -d));
+            R ci = oabs(dot(nf, -d));
             R q = (ia - ib) / (ia + ib);
             R r0 = q * q;
             R m = R(1) - ci;
@@ -888,7 +906,7 @@ template <class R> struct PT {
                 R ia = (ndi < R(0)) ? R(1) : h.ior;
                 R ib = (ndi < R(0)) ? h.ior : R(1);
                 R eta = ia / ib;
-                R ci = std::fabs(dot(nf, -rd));
+                R ci = oabs(dot(nf, -rd));
                 R r0 = (ia - ib) / (ia + ib);
                 r0 = r0 * r0;
                 R m = R(1) - ci;
@@ -908,7 +926,7 @@ template <class R> struct PT {
                     ro = hp - nf * R(1e-3);
                 }
             }
-            if (b >= 2u) {
+            if (b >= 2u && !(F.flags & 1u)) {
                 R p = clampv(maxc(T), R(0.05), R(1));
                 if (randf<R>(seed) > p) break;
                 T = T / p;
@@ -987,6 +1005,39 @@ void render_rows(const orc_scene& S, const orc_frame& F, int threads, uint32_t r
     for (auto& c : ctrs) {
         total.primary += c.primary; total.extension += c.extension; total.shadow += c.shadow;
         total.nodes += c.nodes; total.tests += c.tests;
+    }
+}
+
+// d(frame value)/d(parameter) of pixel (x, y) by a PT<Dual> run; the
+// sanitising step (NaN/Inf -> 0, max(0, .)) is taken with derivative 1.
+void pixel_tangent(const PT<Dual>& pt, const orc_frame& F, uint32_t x, uint32_t y, double dI[3], double I[3]) {
+    uint32_t seed = pcg_hash(x * 1973u + y * 9277u + F.frame_index * 26699u);
+    using R = Dual;
+    V3<R> fr{R(0), R(0), R(0)};
+    const R w = R(F.width), hgt = R(F.height);
+    V3<R> pos{R(F.pos_fovtan[0]), R(F.pos_fovtan[1]), R(F.pos_fovtan[2])};
+    R fovtan = R(F.pos_fovtan[3]), aspect = R(F.fwd_aspect[3]);
+    V3<R> fwd{R(F.fwd_aspect[0]), R(F.fwd_aspect[1]), R(F.fwd_aspect[2])};
+    V3<R> rt{R(F.right_xyz[0]), R(F.right_xyz[1]), R(F.right_xyz[2])};
+    V3<R> up{R(F.up_xyz[0]), R(F.up_xyz[1]), R(F.up_xyz[2])};
+    const uint32_t spp = std::max(1u, F.spp);
+    for (uint32_t s = 0; s < spp; ++s) {
+        R jx = randf<R>(seed);
+        R jy = randf<R>(seed);
+        jx = jx - R(0.5);
+        jy = jy - R(0.5);
+        R ux = (((R(x) + R(0.5)) + jx) / w) * R(2) - R(1);
+        R uy = (((R(y) + R(0.5)) + jy) / hgt) * R(2) - R(1);
+        uy = -uy;
+        V3<R> rd = normalize((fwd + rt * ((ux * aspect) * fovtan)) + up * (uy * fovtan));
+        fr = fr + pt.path(pos, rd, seed);
+    }
+    fr = fr / R(spp);
+    const Dual c[3] = {fr.x, fr.y, fr.z};
+    for (int k = 0; k < 3; ++k) {
+        const bool ok = std::isfinite(c[k].v) && std::isfinite(c[k].d);
+        dI[k] = ok ? c[k].d : 0.0;
+        I[k] = ok ? std::max(c[k].v, 0.0) : 0.0;
     }
 }
 
@@ -1095,6 +1146,68 @@ int orc_trace(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out) {
 }
 int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out) {
     return trace_impl(s, rays, n, out, true);
+}
+
+int orc_tangent(orc_scene* s, const orc_frame* f, uint32_t prim, uint32_t param, const float* dimage, int threads,
+                double* out_dloss, float* out_dimage) {
+    if (!s || !f || !dimage || !out_dloss || prim >= s->prim_count || param > 6u) return -1;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    s->tangent_prim = prim;
+    s->tangent_param = param;
+    std::atomic<uint32_t> next{0};
+    std::vector<double> part((size_t)threads, 0.0);
+    auto work = [&](int tid) {
+        Ctr c;
+        PT<Dual> pt{*s, *f, c};
+        double acc = 0.0;
+        for (;;) {
+            const uint32_t y = next.fetch_add(1);
+            if (y >= f->height) break;
+            for (uint32_t x = 0; x < f->width; ++x) {
+                double dI[3], I[3];
+                pixel_tangent(pt, *f, x, y, dI, I);
+                const size_t p = ((size_t)y * f->width + x) * 4u;
+                for (int k = 0; k < 3; ++k) acc += (double)dimage[p + k] * dI[k];
+                if (out_dimage)
+                    for (int k = 0; k < 3; ++k) out_dimage[p + k] = (float)dI[k];
+            }
+        }
+        part[(size_t)tid] = acc;
+    };
+    std::vector<std::thread> ts;
+    for (int t = 1; t < threads; ++t) ts.emplace_back(work, t);
+    work(0);
+    for (auto& t : ts) t.join();
+    double total = 0.0;
+    for (double v : part) total += v;
+    *out_dloss = total;
+    s->tangent_prim = 0xFFFFFFFFu;
+    return 0;
+}
+
+int orc_render_f64_image(orc_scene* s, const orc_frame* f, int threads, double* out_rgb) {
+    if (!s || !f || !out_rgb) return -1;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    s->tangent_prim = 0xFFFFFFFFu;
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        Ctr c;
+        PT<Dual> pt{*s, *f, c};
+        for (;;) {
+            const uint32_t y = next.fetch_add(1);
+            if (y >= f->height) break;
+            for (uint32_t x = 0; x < f->width; ++x) {
+                double dI[3], I[3];
+                pixel_tangent(pt, *f, x, y, dI, I);
+                for (int k = 0; k < 3; ++k) out_rgb[((size_t)y * f->width + x) * 3u + k] = I[k];
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 1; t < threads; ++t) ts.emplace_back(work);
+    work();
+    for (auto& t : ts) t.join();
+    return 0;
 }
 
 double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H, float* om, float* oc) {
