@@ -16,7 +16,9 @@ contraction).  Algorithmic bytes per launch = node visits x 128 B (BVH4 node)
 + triangle tests x 48 B (TriRec) + sphere tests x 20 B + rays x 40 B (32 B ray
 record in, 8 B hit out), counted by an instrumented re-render of the same
 frame; divided by the HIP-event duration of the same launches in the timed
-region.
+region.  traffic: the measured bytes per launch beyond L2 (rocprofv3
+FETCH_SIZE + WRITE_SIZE passes of this workload, scaled by a 512 MiB copy),
+read from profiles/k_extend_c1_1080p_pmc.txt.
 
 cpu_baseline: the CPU oracle (reference semantics, fp32) rendering whole frames
 of the same workload for about 10 s, on this host's cores (hw - 4 threads as
@@ -55,6 +57,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_extend_c1_1080p_pmc.txt")
+
+
+def measured_traffic():
+    """HBM bytes per k_extend launch on this workload, from the recorded
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known
+    size; a profiler cannot run inside this process)."""
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +108,17 @@ def main():
         return bench_adjoint(args, np, torch, dmt)
     args.steps = 20 if args.steps is None else args.steps
     args.warmup = 3 if args.warmup is None else args.warmup
+    if args.calibration_copy_mb > 0:
+        # one plain device-to-device copy of known size, so that the HBM
+        # counters of a profiler pass can be scaled to bytes
+        n = args.calibration_copy_mb * (1 << 20) // 4
+        src = torch.arange(n, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibration_copy_mb} MiB done")
+        del src, dst
     t0 = time.time()
     sd = dmt.generate(1, seed=args.seed)
     log(f"[bench] scene: {sd.indices.shape[0]} triangles, {sd.frame.width}x{sd.frame.height}, "
@@ -205,7 +234,7 @@ def main():
                    "triangles": int(sd.indices.shape[0]), "width": frame.width, "height": frame.height,
                    "spp": frame.spp, "max_bounces": frame.max_bounces, "rays_per_frame": rays // args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "kernel": "k_extend (closest hit)", "bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_ms": round(avg_launch_ms, 5)},
         "cpu_baseline": cpu,
