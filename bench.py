@@ -76,7 +76,7 @@ def measured_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "batch", "adjoint"), default="c1")
+    ap.add_argument("--workload", choices=("c1", "c2", "batch", "adjoint"), default="c1")
     ap.add_argument("--steps", type=int, default=None, help="timed frames (c1: 20, batch: 100)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed frames (c1: 3, batch: 2)")
     ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
@@ -119,8 +119,9 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calibration_copy_mb} MiB done")
         del src, dst
+    cfg = 2 if args.workload == "c2" else 1
     t0 = time.time()
-    sd = dmt.generate(1, seed=args.seed)
+    sd = dmt.generate(cfg, seed=args.seed)
     log(f"[bench] scene: {sd.indices.shape[0]} triangles, {sd.frame.width}x{sd.frame.height}, "
         f"{sd.frame.max_bounces} bounces ({time.time() - t0:.1f}s)")
     dev = dmt.Device(0)
@@ -207,7 +208,7 @@ def main():
 
         crays, cdt, nfr = cpu_run(threads)
         cpu = {"value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"{nfr} whole 1920x1080 frames ({crays} rays, {cdt:.1f} s) of the fp32 CPU oracle "
+               "sample": f"{nfr} whole {frame.width}x{frame.height} frames ({crays} rays, {cdt:.1f} s) of the fp32 CPU oracle "
                          f"(reference semantics, same scene, median BVH), {threads} threads of {ncpu} "
                          f"(hw - 4, SoftwareTracer.cpp:867-868)"}
         if threads != ncpu:
@@ -216,8 +217,16 @@ def main():
                                 "sample": f"{nfr2} frames, {cdt2:.1f} s"}
         log(f"[bench] cpu baseline {cpu}")
 
+    if cfg == 1:
+        metric, workload = "Mrays/s @1080p 1M tris 1spp", (
+            "icosphere-8 displaced + 200k random tris (1,510,720 tris), 512x256 lognormal env, 1 quad light, "
+            "1920x1080, 1 spp, max depth 8, binned-SAH BVH4")
+    else:
+        metric, workload = "Mrays/s (many-light mixed scene)", (
+            "4,096 spheres + 500k-tri mesh lit by 4,096 emissive triangles through the light tree, all-zero env, "
+            "1280x720, 4 spp, max depth 6, binned-SAH BVH4")
     line = {
-        "metric": "Mrays/s @1080p 1M tris 1spp",
+        "metric": metric,
         "value": round(mrays, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -228,13 +237,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded generator, config 1)",
-        "config": {"workload": "icosphere-8 displaced + 200k random tris (1,510,720 tris), 512x256 lognormal "
-                               "env, 1 quad light, 1920x1080, 1 spp, max depth 8, binned-SAH BVH4",
+        "data": f"synthetic (seeded generator, config {cfg})",
+        "config": {"workload": workload,
                    "triangles": int(sd.indices.shape[0]), "width": frame.width, "height": frame.height,
                    "spp": frame.spp, "max_bounces": frame.max_bounces, "rays_per_frame": rays // args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic() if cfg == 1 else None,
                      "kernel": "k_extend (closest hit)", "bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_ms": round(avg_launch_ms, 5)},
         "cpu_baseline": cpu,

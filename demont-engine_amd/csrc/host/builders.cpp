@@ -300,7 +300,29 @@ void orient_rotate(const float* q, const float* v, float* out) {
     out[2] = v[2] + 2.0f * (q[0] * cy - q[1] * cx);
 }
 
+// Emissive triangle (light type 4, no counterpart in the reference's light
+// types): vertices a = v0.xyz, b = v2.xyz, c = v3.xyz in world space (no
+// orientation), one-sided along normalize(cross(b - a, c - a)).
+LtLeaf lt_leaf_triangle(const float* L, uint32_t id) {
+    LtLeaf s{};
+    s.light = id;
+    const float* v[3] = {&L[0], &L[8], &L[12]};
+    for (int a = 0; a < 3; ++a) {
+        s.lo[a] = std::min(v[0][a], std::min(v[1][a], v[2][a]));
+        s.hi[a] = std::max(v[0][a], std::max(v[1][a], v[2][a]));
+    }
+    const float e1[3] = {L[8] - L[0], L[9] - L[1], L[10] - L[2]};
+    const float e2[3] = {L[12] - L[0], L[13] - L[1], L[14] - L[2]};
+    const float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int a = 0; a < 3; ++a) s.axis[a] = c[a] / len;
+    s.cos_half = 0.0f;
+    s.power = (0.2126f * L[4] + 0.7152f * L[5] + 0.0722f * L[6]) * (0.5f * len);
+    return s;
+}
+
 LtLeaf lt_leaf(const float* Lp, uint32_t id) {
+    if ((uint32_t)(Lp[7] + 0.5f) == 4u) return lt_leaf_triangle(Lp, id);
     float L[20];
     std::memcpy(L, Lp, sizeof(L));
     orient_rotate(&Lp[16], &Lp[8], &L[8]);

@@ -139,7 +139,14 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
         const float* L = lights + (size_t)i * 20u;
         if (!finite_all(L, 20)) return "light " + std::to_string(i) + " has a non-finite value";
         const uint32_t type = (uint32_t)(L[7] + 0.5f);
-        if (L[7] < 0.0f || type > 3u) return "light " + std::to_string(i) + ": type must be 0..3";
+        if (L[7] < 0.0f || type > 4u) return "light " + std::to_string(i) + ": type must be 0..4";
+        if (type == 4u) {   // emissive triangle: vertices a (v0), b (v2), c (v3)
+            const float e1[3] = {L[8] - L[0], L[9] - L[1], L[10] - L[2]};
+            const float e2[3] = {L[12] - L[0], L[13] - L[1], L[14] - L[2]};
+            const float cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                        cz = e1[0] * e2[1] - e1[1] * e2[0];
+            if (!(cx * cx + cy * cy + cz * cz > 1e-20f)) return "light " + std::to_string(i) + ": degenerate triangle";
+        }
     }
     S.light_tree = build_light_tree(S.lights.data(), light_count);
 

@@ -55,8 +55,10 @@ void put_quad_light(std::vector<float>& out, const float pos[3], const float n[3
     out.insert(out.end(), L, L + 20);
 }
 
-// Spheres over the ground plane, materials by a seeded shuffle of i mod 3.
-void gen_spheres(Rng& rng, int n, std::vector<float>& prims) {
+// Spheres over the ground plane, materials by a seeded shuffle of i mod 3;
+// centres uniform in [-extent, extent]^3, radii uniform in [rmin, rmax].
+void gen_spheres(Rng& rng, int n, std::vector<float>& prims, float extent = 4.0f, float rmin = 0.2f,
+                 float rmax = 1.0f) {
     const float plane_n[3] = {0.0f, 1.0f, 0.0f}, grey[3] = {0.5f, 0.5f, 0.5f}, zero[3] = {0, 0, 0};
     put_prim(prims, plane_n, 5.0f, grey, 0.0f, 1u, 0u, 1.0f, 1u, zero);
     std::vector<uint32_t> mats((size_t)n);
@@ -66,8 +68,8 @@ void gen_spheres(Rng& rng, int n, std::vector<float>& prims) {
         std::swap(mats[(size_t)i], mats[(size_t)j]);
     }
     for (int i = 0; i < n; ++i) {
-        float c[3] = {rng.uf(-4, 4), rng.uf(-4, 4), rng.uf(-4, 4)};
-        float r = rng.uf(0.2f, 1.0f);
+        float c[3] = {rng.uf(-extent, extent), rng.uf(-extent, extent), rng.uf(-extent, extent)};
+        float r = rng.uf(rmin, rmax);
         float alb[3] = {rng.uf(0.1f, 0.9f), rng.uf(0.1f, 0.9f), rng.uf(0.1f, 0.9f)};
         float rough = rng.uf(0.0f, 0.5f);
         uint32_t m = mats[(size_t)i];
@@ -175,8 +177,8 @@ dmt_status dmt_camera_look_at(const float eye[3], const float target[3], float f
 
 dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_options* opt, dmt_generated_scene** out) {
     if (!out) return DMT_ERR_INVALID_ARG;
-    if (config != 0u && config != 1u && config != 4u) return DMT_ERR_UNSUPPORTED;
-    dmt_gen_options o{-1, -1, -1, 0, 0, 0, 0};
+    if (config > 4u || config == 3u) return DMT_ERR_UNSUPPORTED;
+    dmt_gen_options o{-1, -1, -1, 0, 0, 0, 0, -1};
     if (opt) o = *opt;
     Rng rng(seed + 0x1000u * config);
     std::vector<float> prims, lights, pos, env;
@@ -187,6 +189,26 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
         int n = o.spheres >= 0 ? o.spheres : (config == 0u ? 64 : 256);
         gen_spheres(rng, n, prims);
         if (config == 4u) { W = H = 512; spp = 16; }
+    } else if (config == 2u) {
+        // 4,096 spheres + a 500k-triangle mesh lit by 4,096 emissive triangles
+        W = 1280; H = 720; spp = 4; bounces = 6;
+        int n = o.spheres >= 0 ? o.spheres : 4096;
+        gen_spheres(rng, n, prims, 4.5f, 0.05f, 0.3f);
+        int sub = o.subdivisions >= 0 ? o.subdivisions : 7;
+        int nrand = o.random_tris >= 0 ? o.random_tris : 500000 - 20 * (1 << (2 * 7));
+        icosphere(sub, pos, idx);
+        for (auto& v : pos) v *= 2.5f;
+        for (int t = 0; t < nrand; ++t) {
+            float c[3] = {rng.uf(-5, 5), rng.uf(-5, 5), rng.uf(-5, 5)};
+            uint32_t base = (uint32_t)(pos.size() / 3);
+            for (int k = 0; k < 3; ++k) {
+                float off[3] = {rng.uf(-1, 1), rng.uf(-1, 1), rng.uf(-1, 1)};
+                float ol = std::sqrt(off[0] * off[0] + off[1] * off[1] + off[2] * off[2]);
+                float s = ol > 0.0f ? 0.025f * rng.uf(0.2f, 1.0f) / ol : 0.0f;
+                for (int a = 0; a < 3; ++a) pos.push_back(c[a] + off[a] * s);
+            }
+            idx.push_back(base); idx.push_back(base + 1); idx.push_back(base + 2);
+        }
     } else {
         W = 1920; H = 1080; bounces = 8; ew = 512; eh = 256;
         int sub = o.subdivisions >= 0 ? o.subdivisions : 8;
@@ -229,7 +251,32 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
     if (config == 4u)
         for (size_t i = 1; i < prims.size() / 28; ++i)
             for (int k = 0; k < 3; ++k) prims[i * 28 + 16 + k] = rng.uf(0.02f, 0.3f);
-    {
+    if (config == 2u) {
+        // 4,096 emissive triangles (edge ~0.3) facing down, in a slab above the
+        // scene, emission exp(N(0,1)) per channel
+        const int nl = o.lights >= 0 ? o.lights : 4096;
+        for (int i = 0; i < nl; ++i) {
+            float c[3] = {rng.uf(-6, 6), rng.uf(5, 7), rng.uf(-6, 6)};
+            float v[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) v[k][a] = c[a] + rng.uf(-0.2f, 0.2f);
+            const float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+            const float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+            const float ny = e1[2] * e2[0] - e1[0] * e2[2];
+            if (ny > 0.0f)   // emit downwards: swap b and c
+                for (int a = 0; a < 3; ++a) std::swap(v[1][a], v[2][a]);
+            float L[20] = {};
+            for (int a = 0; a < 3; ++a) {
+                L[a] = v[0][a];
+                L[4 + a] = float(std::exp(rng.normal()));
+                L[8 + a] = v[1][a];
+                L[12 + a] = v[2][a];
+            }
+            L[7] = 4.0f;
+            L[19] = 1.0f;
+            lights.insert(lights.end(), L, L + 20);
+        }
+    } else {
         const float lp[3] = {0.0f, 8.0f, 0.0f}, ln[3] = {0.0f, -1.0f, 0.0f}, lu[3] = {1.5f, 0.0f, 0.0f};
         const float I[3] = {10.0f, 10.0f, 10.0f};
         put_quad_light(lights, lp, ln, lu, 1.5f, I);
@@ -278,6 +325,7 @@ dmt_status dmt_batch_workload_scene(uint64_t seed, uint32_t index, dmt_gen_optio
     while (sub < 8 && 20u * (1u << (2 * (sub + 1))) <= tris) ++sub;
     const uint32_t ico = 20u * (1u << (2 * sub));
     std::memset(opt, 0, sizeof(*opt));
+    opt->lights = -1;
     opt->subdivisions = sub;
     opt->random_tris = (int32_t)(tris - ico);
     opt->spheres = (index & 1u) ? 64 : 0;

@@ -245,6 +245,27 @@ __device__ __forceinline__ LightSampleT<T> sample_light(const SceneView& S, cons
         s.dir = wi; s.Le = to<T>(I); s.pdf = pick_pdf * pdf_sa; s.dist = d;
         return s;
     }
+    if (type == 4u) {   // emissive triangle (a = v0, b = v2, c = v3), uniform in area, one-sided
+        float x0 = randf(seed);
+        float x1 = randf(seed);
+        float su = sqrtf(x0);
+        float b0 = 1.0f - su;
+        float b1 = x1 * su;
+        const f3 a = pos, b = xyz(l2), c = xyz(l3);
+        f3 lp = (a * b0 + b * b1) + c * ((1.0f - b0) - b1);
+        f3 cr = cross(b - a, c - a);
+        float area = 0.5f * length(cr);
+        f3 nt = normalize(cr);
+        V3<T> tl = to<T>(lp) - p;
+        T d2 = fmax_(dot(tl, tl), T(1e-12f));
+        T d = sqrt_(d2);
+        V3<T> wi = tl / d;
+        T cl = dot(to<T>(nt), -wi);
+        if (cl <= 0.0f) return s;
+        T pdf_sa = d2 / fmax_(area * cl, T(1e-12f));
+        s.dir = wi; s.Le = to<T>(I); s.pdf = pick_pdf * pdf_sa; s.dist = d;
+        return s;
+    }
     if (type == 3u) {   // quad
         float x0 = randf(seed) * 2.0f - 1.0f;
         float x1 = randf(seed) * 2.0f - 1.0f;

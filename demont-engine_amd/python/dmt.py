@@ -98,7 +98,7 @@ class Stats(C.Structure):
 class GenOptions(C.Structure):
     _fields_ = [("subdivisions", C.c_int32), ("random_tris", C.c_int32), ("spheres", C.c_int32),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("env_width", C.c_uint32),
-                ("env_height", C.c_uint32)]
+                ("env_height", C.c_uint32), ("lights", C.c_int32)]
 
 
 class GeneratedScene(C.Structure):
@@ -234,9 +234,9 @@ class SceneData:
 
 
 def generate(config: int, seed: int = 1, subdivisions: int = -1, random_tris: int = -1, spheres: int = -1,
-             width: int = 0, height: int = 0, env_width: int = 0, env_height: int = 0) -> SceneData:
+             width: int = 0, height: int = 0, env_width: int = 0, env_height: int = 0, lights: int = -1) -> SceneData:
     """Seeded synthetic workload (dmt_scene_generate), copied to numpy."""
-    opt = GenOptions(subdivisions, random_tris, spheres, width, height, env_width, env_height)
+    opt = GenOptions(subdivisions, random_tris, spheres, width, height, env_width, env_height, lights)
     out = C.POINTER(GeneratedScene)()
     _check(lib().dmt_scene_generate(config, seed, C.byref(opt), C.byref(out)))
     g = out.contents

@@ -14,6 +14,13 @@
  *             seeded value noise (amplitude 0.1), plus 200k random triangles
  *             (edges <= 0.05) uniform in [-5,5]^3; one quad light; 512x256
  *             env with lognormal texels exp(N(0,1)); 1920x1080, 1 spp, 8 bounces.
+ *   config 2  4,096 spheres (centres U[-4.5,4.5]^3, radii U[0.05,0.3], config 0
+ *             materials) over the plane y = -5, a 500k-triangle mesh
+ *             (icosphere-7 of radius 2.5 + 172,320 random small triangles in
+ *             [-5,5]^3), lit only by 4,096 emissive triangles (light type 4,
+ *             edges ~0.3, facing down, centres in [-6,6] x [5,7] x [-6,6],
+ *             emission exp(N(0,1)) per channel); all-zero env; 1280x720,
+ *             4 spp, 6 bounces.
  *   config 4  config 0 with 256 spheres, 256 distinct materials (each
  *             sphere also emits U[0.02, 0.3] per channel); 512x512, 16 spp,
  *             4 bounces.
@@ -34,6 +41,7 @@ typedef struct dmt_gen_options {
     int32_t  spheres;        /* analytic spheres (< 0: config default)         */
     uint32_t width, height;  /* 0: config default                              */
     uint32_t env_width, env_height; /* 0: config default                       */
+    int32_t  lights;         /* emissive triangle lights (config 2; < 0: default) */
 } dmt_gen_options;
 
 typedef struct dmt_generated_scene {

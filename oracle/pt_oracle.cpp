@@ -140,7 +140,27 @@ void engine_rotate(const float* q, const float* v, float* o) {
     o[2] = v[2] + 2.0f * (qx * cy - qy * cx);
 }
 
+// Emissive triangle (light type 4; an extension, not a reference light type):
+// vertices v0, v2, v3 in world space, one-sided along cross(b - a, c - a).
+LeafSummary make_leaf_triangle(const float* L, uint32_t id) {
+    LeafSummary s{};
+    s.id = id;
+    for (int i = 0; i < 3; ++i) {
+        s.mn[i] = std::min(L[i], std::min(L[8 + i], L[12 + i]));
+        s.mx[i] = std::max(L[i], std::max(L[8 + i], L[12 + i]));
+    }
+    const float e1[3] = {L[8] - L[0], L[9] - L[1], L[10] - L[2]};
+    const float e2[3] = {L[12] - L[0], L[13] - L[1], L[14] - L[2]};
+    const float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int i = 0; i < 3; ++i) s.axis[i] = c[i] / len;
+    s.cos_half = 0.0f;
+    s.intensity = (0.2126f * L[4] + 0.7152f * L[5] + 0.0722f * L[6]) * (0.5f * len);
+    return s;
+}
+
 LeafSummary make_leaf(const float* Lin, uint32_t id) {
+    if ((uint32_t)(Lin[7] + 0.5f) == 4u) return make_leaf_triangle(Lin, id);
     float L[20];
     std::memcpy(L, Lin, sizeof(L));
     engine_rotate(Lin + 16, Lin + 8, L + 8);
@@ -758,6 +778,27 @@ template <class R> struct PT {
             R cl = std::max(R(0), dot(nl, -wi));
             if (cl <= R(0)) return s;
             R area = ((R(4) * PI) * radius) * radius;
+            R pdf_sa = d2 / std::max(area * cl, R(1e-12));
+            s.dir = wi; s.Le = I; s.pdf = pick_pdf * pdf_sa; s.dist = d;
+            return s;
+        }
+        if (type == 4u) {   // emissive triangle, uniform in area, one-sided
+            R x0 = randf<R>(seed);
+            R x1 = randf<R>(seed);
+            R su = osqrt(x0);
+            R b0 = R(1) - su;
+            R b1 = x1 * su;
+            V3<R> a = pos, b = ld3<R>(L + 8), cc = ld3<R>(L + 12);
+            V3<R> lp = (a * b0 + b * b1) + cc * ((R(1) - b0) - b1);
+            V3<R> cr = cross(b - a, cc - a);
+            R area = R(0.5) * length(cr);
+            V3<R> nt = normalize(cr);
+            V3<R> tl = lp - p;
+            R d2 = std::max(dot(tl, tl), R(1e-12));
+            R d = osqrt(d2);
+            V3<R> wi = tl / d;
+            R cl = dot(nt, -wi);
+            if (cl <= R(0)) return s;
             R pdf_sa = d2 / std::max(area * cl, R(1e-12));
             s.dir = wi; s.Le = I; s.pdf = pick_pdf * pdf_sa; s.dist = d;
             return s;

@@ -63,6 +63,18 @@ def sphere_light(pos, r, I):
     return light(2, pos, I, radius=r)
 
 
+def triangle_light(a, b, c, Le):
+    """Emissive triangle (light type 4): one-sided along cross(b - a, c - a)."""
+    L = np.zeros(20, np.float32)
+    L[0:3] = a
+    L[4:7] = Le
+    L[7] = 4.0
+    L[8:11] = b
+    L[12:15] = c
+    L[19] = 1.0
+    return L
+
+
 def frame(eye, target, fov, w, h, spp=1, bounces=4, frame_index=0):
     f = dmt.look_at(eye, target, fov, w, h)
     f.spp = spp

@@ -145,6 +145,36 @@ def test_render_c1_full_size_rows(device):
     assert st["primary_rays"] == sd.frame.width * sd.frame.height
 
 
+def test_render_c2_small_matches_oracle(device):
+    """Config 2 at reduced size: spheres + mesh lit only by emissive
+    triangles through a 256-leaf light tree, 2 spp, depth 6."""
+    sd = dmt.generate(2, seed=3, spheres=300, subdivisions=4, random_tris=20000, lights=256, width=160, height=90)
+    f = sd.frame.copy()
+    f.spp = 2
+    img, ref, st, cnt = _render_pair(device, sd, f)
+    exact = _check_render(img, ref, st, cnt, "C2-small 160x90 2spp d6, 256 triangle lights")
+    assert (st["primary_rays"], st["extension_rays"], st["shadow_rays"]) == \
+        (cnt["primary"], cnt["extension"], cnt["shadow"])
+    assert exact > 0.999
+
+
+def test_render_c2_full_rows(device):
+    """Full config 2 (4,096 spheres, 500k triangles, 4,096 triangle lights,
+    1280x720, 4 spp, depth 6) on the GPU; the oracle renders every 60th row."""
+    sd = dmt.generate(2, seed=1)
+    scene = dmt.Scene(device, sd)
+    scene.render(sd.frame)
+    img = scene.readback(sd.frame.width, sd.frame.height)
+    oscene = orc.OracleScene.from_scene_data(sd)
+    ref, cnt = oscene.render(sd.frame, 32, 0, rows=(0, sd.frame.height, 60))
+    rows = np.arange(0, sd.frame.height, 60)
+    err = rel_mse(img[rows, :, :3], ref[rows, :, :3])
+    exact = float(np.mean(np.all(img[rows] == ref[rows], axis=-1)))
+    print(f"C2 full: rel-MSE on {len(rows)} rows {err:.3e}, bit-exact {exact:.6f}")
+    assert err <= RADIANCE_REL_MSE
+    assert exact > 0.999
+
+
 @pytest.mark.parametrize("cfg,kw,spp", [(0, dict(seed=6, width=128, height=96), 1),
                                         (4, dict(seed=7, spheres=40, width=64, height=48), 4),
                                         (1, dict(seed=8, subdivisions=4, random_tris=5000, width=160, height=90), 1)])

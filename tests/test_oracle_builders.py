@@ -170,10 +170,12 @@ def random_lights(n, seed, kinds=(0, 1, 2, 3)):
             out.append(scenes.light(1, pos, I, dir=d, cos_outer=0.7, cos_inner=0.9))
         elif k == 2:
             out.append(scenes.sphere_light(pos, rng.uniform(0.05, 1.0), I))
-        else:
+        elif k == 3:
             d = rng.normal(size=3); d /= np.linalg.norm(d)
             u = np.cross(d, [0.3, 0.5, 0.81]); u /= np.linalg.norm(u)
             out.append(scenes.quad_light(pos, d, u * rng.uniform(0.1, 1), rng.uniform(0.1, 1), I))
+        else:
+            out.append(scenes.triangle_light(pos, pos + rng.uniform(-1, 1, 3), pos + rng.uniform(-1, 1, 3), I))
     return np.array(out, np.float32)
 
 
@@ -199,13 +201,29 @@ def _walk(nodes):
 
 @pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 200, 500, 1000])
 def test_light_tree_structure(n):
-    L = random_lights(n, seed=n)
+    L = random_lights(n, seed=n, kinds=(0, 1, 2, 3, 4))
     nodes = orc.light_tree(L)
     assert len(nodes) == 2 * n - 1
     leaves = _walk(nodes)
     assert sorted(leaves) == list(range(n))
     # the product builder is node-for-node identical
     assert np.array_equal(nodes.view(np.uint32), dmt.build_light_tree(L).view(np.uint32))
+
+
+def test_triangle_light_leaf_summary():
+    """Emissive triangle leaf (light type 4, no reference counterpart): the
+    triangle's bounds, its normal as a hemisphere cone, luminance x area."""
+    a, b, c = np.array([0.0, 3.0, 0.0]), np.array([1.0, 3.0, 0.0]), np.array([0.0, 3.0, 2.0])
+    Le = np.array([2.0, 1.0, 0.5], np.float32)
+    nodes = orc.light_tree(np.array([scenes.triangle_light(a, b, c, Le)], np.float32))
+    nd = nodes[0]
+    assert np.allclose(nd[0:3], [0, 3, 0]) and np.allclose(nd[4:7], [1, 3, 2])
+    n = np.cross(b - a, c - a); n /= np.linalg.norm(n)
+    assert np.allclose(nd[8:11], n) and nd[11] == 0.0
+    lum = 0.2126 * 2.0 + 0.7152 * 1.0 + 0.0722 * 0.5
+    assert abs(nd[12] - lum * 1.0) < 1e-5   # area = |(1,0,0) x (0,0,2)| / 2 = 1
+    assert np.array_equal(nodes.view(np.uint32), dmt.build_light_tree(
+        np.array([scenes.triangle_light(a, b, c, Le)], np.float32)).view(np.uint32))
 
 
 def test_light_tree_coincident_lights():

@@ -86,6 +86,26 @@ def test_quad_light_irradiance():
         assert np.allclose(got, expect, rtol=0.02), (x, got, expect)
 
 
+def test_triangle_light_irradiance():
+    """Emissive triangle (light type 4) over a Lambert floor: the radiance
+    equals albedo / pi * Le * Lambert's polygon irradiance of the triangle."""
+    a = np.array([0.8, 0.7, 0.6])
+    Le = 4.0
+    A, B, C = np.array([-0.5, 2.0, -0.3]), np.array([0.7, 2.2, 0.1]), np.array([0.1, 1.8, 0.8])
+    if np.cross(B - A, C - A)[1] > 0:   # face the floor
+        B, C = C, B
+    sd = scenes.scene([_floor(a)], [scenes.triangle_light(A, B, C, (Le, Le, Le))], env=scenes.black_env())
+    for x in (np.array([0.0, 0.0, 0.0]), np.array([1.0, 0.0, -0.7])):
+        got = _probe(sd, x + np.array([0.0, 1.0, 1.5]), x, spp=32, frames=40)
+        E = scenes.polygon_irradiance(x, np.array([0.0, 1.0, 0.0]), [A, B, C])
+        expect = a / math.pi * Le * E
+        assert np.allclose(got, expect, rtol=0.02), (x, got, expect)
+    # one-sided: flipped, the triangle lights nothing below it
+    sd = scenes.scene([_floor(a)], [scenes.triangle_light(A, C, B, (Le, Le, Le))], env=scenes.black_env())
+    got = _probe(sd, (0.0, 1.0, 1.5), (0.0, 0.0, 0.0), spp=8, frames=4)
+    assert np.all(got == 0.0)
+
+
 def test_sphere_light_irradiance():
     a = np.array([0.7, 0.7, 0.7])
     Le, R = 3.0, 0.3
