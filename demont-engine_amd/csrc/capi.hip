@@ -23,7 +23,15 @@
 using namespace dmt;
 
 // Frames of per-frame dmt_render calls kept in flight (one stream each).
-constexpr uint32_t kFrameSets = 3;
+#ifndef DMT_TAIL_AFTER
+#define DMT_TAIL_AFTER 1
+#endif
+constexpr uint32_t kDefaultTailAfter = DMT_TAIL_AFTER;
+constexpr uint32_t kTailBlocksPerCu = 4;   // k_tail grid: resident workgroups per CU
+#ifndef DMT_FRAME_SETS
+#define DMT_FRAME_SETS 3
+#endif
+constexpr uint32_t kFrameSets = DMT_FRAME_SETS;
 
 namespace {
 
@@ -57,6 +65,7 @@ struct dmt_device {
     // teardown to the last dmt_scene_destroy.
     uint32_t scenes = 0;
     bool destroy_requested = false;
+    uint32_t cus = 0;               // compute units
 };
 
 namespace {
@@ -95,6 +104,8 @@ struct dmt_scene {
     uint32_t* host_count = nullptr;   // pinned, kFrameSets * kPollSlots
     int next_set = 0;
     int last_set = -1;
+    // wavefront iterations of a frame before k_tail completes it
+    uint32_t tail_after = kDefaultTailAfter;
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -127,17 +138,24 @@ dmt_status mark_lost_if_fatal(dmt_device* dev, hipError_t e, const char* what) {
     return hip_fail(e, what);
 }
 
+// Copies count elements to a new device buffer followed by `pad` zero
+// elements (leaf primitive arrays: a traversal fetch may read past the last
+// primitive of a leaf).
 template <class T>
-dmt_status upload(dmt_scene* s, const T* src, size_t count, const T** out) {
+dmt_status upload(dmt_scene* s, const T* src, size_t count, const T** out, size_t pad = 0) {
     *out = nullptr;
     if (count == 0) return DMT_OK;
     T* d = nullptr;
-    hipError_t e = dalloc(&d, count);
+    hipError_t e = dalloc(&d, count + pad);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(scene)");
     s->buffers.push_back(d);
-    s->scene_bytes += count * sizeof(T);
+    s->scene_bytes += (count + pad) * sizeof(T);
     e = hipMemcpy(d, src, count * sizeof(T), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpy(scene)");
+    if (pad) {
+        e = hipMemset(d + count, 0, pad * sizeof(T));
+        if (e != hipSuccess) return hip_fail(e, "hipMemset(scene)");
+    }
     *out = d;
     return DMT_OK;
 }
@@ -300,6 +318,13 @@ dmt_status dmt_device_create(int32_t index, dmt_device** out) {
     auto* d = new (std::nothrow) dmt_device();
     if (!d) return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation");
     d->id = index;
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, index);
+    if (e != hipSuccess || cus <= 0) {
+        delete d;
+        return hip_fail(e, "hipDeviceGetAttribute(CU count)");
+    }
+    d->cus = (uint32_t)cus;
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     for (uint32_t k = 0; k + 1 < kFrameSets && e == hipSuccess; ++k)
         e = hipStreamCreateWithFlags(&d->aux[k], hipStreamNonBlocking);
@@ -378,11 +403,11 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     if ((st = upload(s, P.plane_ids.data(), P.plane_ids.size(), &V.plane_ids))) return bail(st);
     if ((st = upload(s, (const float4*)P.sphere_bvh.nodes.data(), P.sphere_bvh.nodes.size() * 8u, &f4))) return bail(st);
     V.sph_nodes = f4;
-    if ((st = upload(s, (const float4*)P.spheres.data(), P.sphere_ids.size(), &V.spheres))) return bail(st);
+    if ((st = upload(s, (const float4*)P.spheres.data(), P.sphere_ids.size(), &V.spheres, kLeafPad))) return bail(st);
     if ((st = upload(s, P.sphere_ids.data(), P.sphere_ids.size(), &V.sphere_ids))) return bail(st);
     if ((st = upload(s, (const float4*)P.tri_bvh.nodes.data(), P.tri_bvh.nodes.size() * 8u, &f4))) return bail(st);
     V.tri_nodes = f4;
-    if ((st = upload(s, (const float4*)P.tris.data(), (size_t)P.tri_count * 3u, &V.tris))) return bail(st);
+    if ((st = upload(s, (const float4*)P.tris.data(), (size_t)P.tri_count * 3u, &V.tris, kLeafPad))) return bail(st);
     if ((st = upload(s, (const float4*)P.lights.data(), (size_t)P.light_count * 5u, &V.lights))) return bail(st);
     V.light_count = P.light_count;
     V.ltree_count = (uint32_t)(P.light_tree.size() / 16u);
@@ -489,6 +514,8 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
             o.extend_events = s->ev_pool.data() + s->ev_used;
             o.max_events = (uint32_t)bound;
         }
+        o.tail_after = s->tail_after;
+        o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
         hipError_t e = launch_frame(s->view, F, s->wf[set], s->accum, stream, o);
         s->ev_used += o.events_used;
         if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
@@ -592,6 +619,18 @@ dmt_status dmt_scene_reset_stats(dmt_scene* s) {
     return DMT_OK;
 }
 
+dmt_status dmt_scene_set_schedule(dmt_scene* s, uint32_t wavefront_iterations) {
+    if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
+    s->tail_after = wavefront_iterations;
+    return DMT_OK;
+}
+
+dmt_status dmt_scene_get_schedule(dmt_scene* s, uint32_t* wavefront_iterations) {
+    if (!s || !wavefront_iterations) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    *wavefront_iterations = s->tail_after;
+    return DMT_OK;
+}
+
 dmt_status dmt_scene_set_profiling(dmt_scene* s, int32_t enabled) {
     if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
     s->profiling = enabled != 0;
@@ -619,6 +658,8 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     F.reset_accum = 1u;
     LaunchOptions o;
     o.counting = true;
+    o.tail_after = s->tail_after;
+    o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
     if (e == hipSuccess) e = launch_frame(s->view, F, W, scratch, s->dev->stream, o);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
@@ -635,6 +676,12 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     out->ext_node_visits = h.ext_nodes;
     out->ext_tri_tests = h.ext_tris;
     out->ext_sphere_tests = h.ext_spheres;
+    out->tail_node_visits = h.tail_nodes;
+    out->tail_tri_tests = h.tail_tris;
+    out->tail_sphere_tests = h.tail_spheres;
+    out->node_visits += h.tail_nodes;
+    out->tri_tests += h.tail_tris;
+    out->sphere_tests += h.tail_spheres;
     out->frames = 1;
     out->scene_bytes = s->scene_bytes;
     out->bvh_nodes = s->bvh_nodes;

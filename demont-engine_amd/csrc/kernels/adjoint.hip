@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs A) {
     float* G = LDS ? lds_grad : A.grads;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    Stack stk{lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth};
+    Stack stk = stack_at(lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     const FrameArgs& F = A.F;
     const uint32_t npix = F.width * F.height;
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_rough(AdjointArgs A) {
     }
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    Stack stk{lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth};
+    Stack stk = stack_at(lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     const FrameArgs& F = A.F;
     const uint32_t npix = F.width * F.height;

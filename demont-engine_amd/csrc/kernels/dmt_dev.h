@@ -204,22 +204,35 @@ struct SceneView {
     uint32_t pad;
 };
 
-// Traversal stack: `depth_lds` entries in LDS (thread-interleaved), the rest
-// in a per-thread global spill area.
+// Traversal stack: `lds_depth` entries in LDS (thread-interleaved), the rest
+// in a per-thread global spill area.  The two pointers carry their address
+// spaces, so a push or pop compiles to an LDS instruction (or a global one on
+// the spill path) rather than a generic flat access, whose latency is that of
+// vector memory even when it lands in LDS.
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+using global_u32 = __attribute__((address_space(1))) uint32_t;
+
 struct Stack {
-    uint32_t* lds;        // this thread's column base
+    lds_u32* lds;         // this thread's column base
     uint32_t lds_stride;  // = block size
     uint32_t lds_depth;
-    uint32_t* spill;      // this thread's spill area
+    global_u32* spill;    // this thread's spill area
     uint32_t spill_depth;
     __device__ __forceinline__ void put(uint32_t i, uint32_t v) {
         if (i < lds_depth) lds[i * lds_stride] = v;
         else spill[i - lds_depth] = v;
     }
     __device__ __forceinline__ uint32_t get(uint32_t i) const {
-        return (i < lds_depth) ? lds[i * lds_stride] : spill[i - lds_depth];
+        if (i < lds_depth) return lds[i * lds_stride];
+        return spill[i - lds_depth];
     }
 };
+
+// column: this thread's first LDS entry (a __shared__ array + thread index).
+__device__ __forceinline__ Stack stack_at(uint32_t* column, uint32_t stride, uint32_t lds_depth, uint32_t* spill,
+                                          uint32_t spill_depth) {
+    return Stack{(lds_u32*)column, stride, lds_depth, (global_u32*)spill, spill_depth};
+}
 
 struct TraceCounters {
     uint32_t nodes;
@@ -235,12 +248,23 @@ struct HitRec {
 // PBRT's robust slab test bound: 1 + 2*gamma(3), gamma(n) = n*eps / (1 - n*eps).
 __device__ __forceinline__ constexpr float robust_hi() { return 1.0f + 2.0f * (3.0f * 5.9604645e-8f / (1.0f - 3.0f * 5.9604645e-8f)); }
 
-// Tests the four children of a node; returns the hit mask and entry distances.
-__device__ __forceinline__ uint32_t node4_test(const float4* np, f3 o, f3 inv, float tbest, float tn[4],
+// Every traversal step fetches kFetch float4 (112 B) from one address: an
+// interior node's six bound rows and its children, or the primitives of a leaf
+// (two triangles, or four spheres).  Lanes of a wave at interior nodes and
+// lanes at leaves then wait on memory together, once per step, instead of once
+// for each kind in turn.  Leaf arrays carry kLeafPad zero float4 past their end
+// so that a fetch at the last primitive stays inside the allocation.
+constexpr uint32_t kFetch = 7u;
+constexpr uint32_t kLeafPad = 8u;
+constexpr uint32_t kLeafFirst = (1u << 27) - 1u;   // leaf reference: first primitive
+
+// Tests the four children of a fetched node; returns the hit mask and entry
+// distances.
+__device__ __forceinline__ uint32_t node4_test(const float4 (&r)[kFetch], f3 o, f3 inv, float tbest, float tn[4],
                                                uint32_t ch[4]) {
-    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    const uint4 c = *reinterpret_cast<const uint4*>(np + 6);
-    ch[0] = c.x; ch[1] = c.y; ch[2] = c.z; ch[3] = c.w;
+    const float4 lx = r[0], hx = r[1], ly = r[2], hy = r[3], lz = r[4], hz = r[5];
+    ch[0] = __float_as_uint(r[6].x); ch[1] = __float_as_uint(r[6].y);
+    ch[2] = __float_as_uint(r[6].z); ch[3] = __float_as_uint(r[6].w);
     const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
     const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
     const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
@@ -270,20 +294,33 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ca, float& tb, uint32
 }
 
 // Generic BVH4 walk: visits children near-to-far, pushes the farther ones.
-// `leaf(first, count)` tests a leaf range and may lower `tbest`.
-template <bool COUNT, class LeafFn>
-__device__ __forceinline__ void bvh4_walk(const float4* nodes, f3 o, f3 inv, float& tbest, Stack& st, LeafFn&& leaf,
-                                          TraceCounters& tc) {
+// A leaf's primitives start at prims + first * STRIDE; `leaf(first, count, r)`
+// tests the ones the fetch `r` holds, may lower `tbest`, and returns how many
+// it consumed (a leaf larger than one fetch takes further steps).
+template <bool COUNT, uint32_t STRIDE, class LeafFn>
+__device__ __forceinline__ void bvh4_walk(const float4* nodes, const float4* prims, f3 o, f3 inv, float& tbest,
+                                          Stack& st, LeafFn&& leaf, TraceCounters& tc) {
     uint32_t sp = 0;
     uint32_t cur = 0u;   // root node (internal reference 0)
     for (;;) {
-        if (cur & kLeaf) {
-            leaf(cur & ((1u << 27) - 1u), ((cur >> 27) & 15u) + 1u);
+        const bool at_leaf = (cur & kLeaf) != 0u;
+        const uint32_t first = cur & kLeafFirst;
+        const float4* p = at_leaf ? prims + (size_t)first * STRIDE : nodes + (size_t)cur * 8u;
+        float4 r[kFetch];
+#pragma unroll
+        for (uint32_t k = 0; k < kFetch; ++k) r[k] = p[k];
+        if (at_leaf) {
+            const uint32_t count = ((cur >> 27) & 15u) + 1u;
+            const uint32_t used = leaf(first, count, r);
+            if (used < count) {
+                cur = kLeaf | ((count - used - 1u) << 27) | (first + used);
+                continue;
+            }
         } else {
             if (COUNT) tc.nodes++;
             float tn[4];
             uint32_t ch[4];
-            uint32_t m = node4_test(nodes + (size_t)cur * 8u, o, inv, tbest, tn, ch);
+            const uint32_t m = node4_test(r, o, inv, tbest, tn, ch);
             if (m) {
                 // sort (t, child) ascending; misses carry +inf and sink.
                 cswap(tn[0], ch[0], tn[1], ch[1]);
@@ -291,7 +328,7 @@ __device__ __forceinline__ void bvh4_walk(const float4* nodes, f3 o, f3 inv, flo
                 cswap(tn[0], ch[0], tn[2], ch[2]);
                 cswap(tn[1], ch[1], tn[3], ch[3]);
                 cswap(tn[1], ch[1], tn[2], ch[2]);
-                uint32_t n = __popc(m);
+                const uint32_t n = __popc(m);
                 for (int i = (int)n - 1; i >= 1; --i) st.put(sp++, ch[i]);
                 cur = ch[0];
                 continue;
@@ -324,10 +361,13 @@ __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, 
     f3 rs{fabsf(d.x) > 1e-30f ? d.x : 1e-30f, fabsf(d.y) > 1e-30f ? d.y : 1e-30f, fabsf(d.z) > 1e-30f ? d.z : 1e-30f};
     f3 inv{1.0f / rs.x, 1.0f / rs.y, 1.0f / rs.z};
     if (S.sph_nodes) {
-        bvh4_walk<COUNT>(S.sph_nodes, o, inv, best, st, [&](uint32_t first, uint32_t count) {
-            for (uint32_t k = 0; k < count; ++k) {
-                const float4 s = S.spheres[first + k];
-                const uint32_t id = S.sphere_ids[first + k];
+        bvh4_walk<COUNT, 1u>(S.sph_nodes, S.spheres, o, inv, best, st, [&](uint32_t first, uint32_t count,
+                                                                             const float4 (&r)[kFetch]) {
+            const uint32_t n = count < 4u ? count : 4u;
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k) {
+                if (k >= n) break;
+                const float4 s = r[k];
                 if (COUNT) tc.spheres++;
                 f3 oc = o - xyz(s);
                 float b = dot(oc, d);
@@ -337,20 +377,27 @@ __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, 
                 h = sqrtf(h);
                 float t0 = -b - h, t1 = -b + h;
                 float t = (t0 > 1e-3f) ? t0 : t1;
-                if (t > 1e-3f && (t < best || (t == best && id < best_id))) {
-                    best = t; best_id = id; code = id;
+                if (t > 1e-3f && t <= best) {   // the id only matters for a candidate
+                    const uint32_t id = S.sphere_ids[first + k];
+                    if (t < best || id < best_id) {
+                        best = t; best_id = id; code = id;
+                    }
                 }
             }
+            return n;
         }, tc);
     }
     if (S.tri_nodes) {
         float tm = best;
         uint32_t tri_id = 0u;   // a triangle at exactly the analytic distance never wins
         uint32_t tri_pos = kMiss;
-        bvh4_walk<COUNT>(S.tri_nodes, o, inv, tm, st, [&](uint32_t first, uint32_t count) {
-            for (uint32_t k = 0; k < count; ++k) {
-                const float4* tr = S.tris + (size_t)(first + k) * 3u;
-                const float4 a = tr[0], e1v = tr[1], e2v = tr[2];
+        bvh4_walk<COUNT, 3u>(S.tri_nodes, S.tris, o, inv, tm, st, [&](uint32_t first, uint32_t count,
+                                                                       const float4 (&r)[kFetch]) {
+            const uint32_t n = count < 2u ? count : 2u;
+#pragma unroll
+            for (uint32_t k = 0; k < 2u; ++k) {
+                if (k >= n) break;
+                const float4 a = r[3u * k], e1v = r[3u * k + 1u], e2v = r[3u * k + 2u];
                 if (COUNT) tc.tris++;
                 const f3 v0 = xyz(a), e1 = xyz(e1v), e2 = xyz(e2v);
                 const f3 pv = cross(d, e2);
@@ -369,6 +416,7 @@ __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, 
                     tm = t; tri_id = id; tri_pos = first + k;
                 }
             }
+            return n;
         }, tc);
         if (tri_pos != kMiss) {
             best = tm;

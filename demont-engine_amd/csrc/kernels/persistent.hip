@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(kBlock) k_persistent(BatchArgs A) {
     const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
-    Stack stk{lds + threadIdx.x, kBlock, DEPTH, A.spill + (size_t)tid * A.spill_depth, A.spill_depth};
+    Stack stk = stack_at(lds + threadIdx.x, kBlock, DEPTH, A.spill + (size_t)tid * A.spill_depth, A.spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     uint32_t n_primary = 0, n_ext = 0, n_shadow = 0;
     const uint32_t rel = *A.frame;

@@ -17,12 +17,18 @@
 namespace dmt {
 
 constexpr uint32_t kBlock = 256;          // threads per workgroup (4 waves)
-constexpr uint32_t kLdsDepth = 24;        // LDS stack entries per thread
+#ifndef DMT_LDS_DEPTH
+#define DMT_LDS_DEPTH 24
+#endif
+// LDS stack entries per thread (build-time tunable: LDS per workgroup bounds
+// the waves a CU holds)
+constexpr uint32_t kLdsDepth = DMT_LDS_DEPTH;
 constexpr uint32_t kMaxGridBlocks = 4096; // grid-stride cap: 1M threads
 
 // counter slots
 constexpr uint32_t kCtrRays = 0;          // + parity: ray queue lengths
 constexpr uint32_t kCtrFinished = 2;
+constexpr uint32_t kCtrTail = 3;          // k_tail: queue entries handed out
 constexpr uint32_t kCounterSlots = 8;
 
 // Shadow records live at the slot of the path that formed them (no
@@ -66,6 +72,9 @@ struct DeviceStats {
     unsigned long long ext_nodes;   // counting traversal, extend kernel
     unsigned long long ext_tris;
     unsigned long long ext_spheres;
+    unsigned long long tail_nodes;  // counting traversal, tail kernel (extension and shadow)
+    unsigned long long tail_tris;
+    unsigned long long tail_spheres;
 };
 
 struct Wavefront {
@@ -91,6 +100,7 @@ FrameArgs make_frame_args(const float pos_fovtan[4], const float fwd_aspect[4], 
 
 uint32_t grid_for(uint32_t n);
 
+constexpr uint32_t kNoTail = 0xFFFFFFFFu;   // every iteration by the per-bounce kernels
 constexpr uint32_t kPollSlots = 4;
 constexpr uint32_t kPollChunk = 4;        // iterations between live-queue snapshots
 
@@ -102,6 +112,8 @@ struct LaunchOptions {
     bool counting = false;                // instrumented traversal
     hipEvent_t* poll_events = nullptr;    // kPollSlots events for the stop test
     uint32_t iterations = 0;              // out: iterations launched
+    uint32_t tail_after = kNoTail;        // iterations before k_tail finishes the frame
+    uint32_t tail_blocks = 0;             // k_tail grid (0: grid_for(pixels))
 };
 
 // Render one frame into accum; the final pass that folds the frame's values

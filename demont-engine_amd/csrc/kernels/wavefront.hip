@@ -64,7 +64,7 @@ __device__ __forceinline__ void add_stat(unsigned long long* dst, uint32_t v) {
 
 __device__ __forceinline__ Stack make_stack(uint32_t* lds, uint32_t* spill, uint32_t spill_depth) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    return Stack{lds + threadIdx.x, kBlock, kLdsDepth, spill + (size_t)gid * spill_depth, spill_depth};
+    return stack_at(lds + threadIdx.x, kBlock, kLdsDepth, spill + (size_t)gid * spill_depth, spill_depth);
 }
 
 __device__ __forceinline__ void start_path(PathState& ps, RayQueue& q, uint32_t slot, uint32_t p, f3 pos, f3 rd,
@@ -108,8 +108,13 @@ __global__ void __launch_bounds__(kBlock) k_generate(FrameArgs F, PathState ps, 
     }
 }
 
+#ifdef DMT_EXT_WAVES
+#define DMT_EXT_ATTR __attribute__((amdgpu_waves_per_eu(DMT_EXT_WAVES, DMT_EXT_WAVES)))
+#else
+#define DMT_EXT_ATTR
+#endif
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint32_t* ctr, uint32_t in_par,
+__global__ void __launch_bounds__(kBlock) DMT_EXT_ATTR k_extend(SceneView S, RayQueue q, uint32_t* ctr, uint32_t in_par,
                                                    uint2* hits, uint32_t* spill, uint32_t spill_depth,
                                                    DeviceStats* st) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
@@ -133,11 +138,11 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint
 
 // Writes a path's shadow queries at its own slot of the iteration.
 struct SlotSink : NoGradientHooks {
-    ShadowQueue* q;
+    ShadowQueue q0, q1;   // light NEE, env NEE
     uint32_t slot, pix;
     bool emitted[2];
     __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState&) {
-        ShadowQueue& s = q[kind];
+        ShadowQueue& s = kind == 0 ? q0 : q1;
         const uint32_t tag = pix | kValidBit | (r.clamp ? kClampBit : 0u);
         s.o[slot] = make_float4(r.o.x, r.o.y, r.o.z, r.t_max);
         s.d[slot] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(tag));
@@ -156,7 +161,6 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint
     const uint32_t n = ctr[kCtrRays + in_par];
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t n_primary = 0, n_ext = 0;
-    ShadowQueue sq[2] = {sl, se};
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {   // block-uniform
         const uint32_t i = base + threadIdx.x;
         bool cont = false;
@@ -171,10 +175,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint
             const uint32_t b = meta & 0xFFu;
             const uint32_t flags = __float_as_uint(Lv.w);
             PathVertexState P{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[pix]};
-            SlotSink sink{{}, sq, i, pix, {false, false}};
+            SlotSink sink{{}, sl, se, i, pix, {false, false}};
             shade_vertex(S, F, b, xyz(o4), xyz(d4), HitRec{__uint_as_float(hr.x), hr.y}, P, R, sink);
-            for (int k = 0; k < 2; ++k)
-                if (!sink.emitted[k]) sq[k].d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+            if (!sink.emitted[0]) sl.d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+            if (!sink.emitted[1]) se.d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
             if (b == 0u) n_primary++;
             else n_ext++;
             const uint32_t nflags = (P.skip_sky ? 1u : 0u) | (P.prev_lambert ? 2u : 0u);
@@ -229,6 +233,202 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
         add_stat(&st->nodes, tc.nodes);
         add_stat(&st->tris, tc.tris);
         add_stat(&st->spheres, tc.spheres);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_tail: the rest of the frame in one launch.
+//
+// Late iterations hold few live paths, and each of their launches lasts as
+// long as its slowest ray, so a frame's tail of short launches costs far more
+// than its share of rays.  k_tail takes every path of the live queue and runs
+// it to the end, then the pixel's remaining samples: one closest-hit query per
+// loop step (the extension ray, or one step of a shadow query), lanes refilled
+// from the queue as their paths end.  Each path's arithmetic is the one the
+// iteration kernels do, in the same order (vertex shading, light-NEE then
+// env-NEE contribution, next vertex; sample sums as k_finish forms them), so
+// the pixel values are the same.
+
+namespace {
+
+constexpr uint32_t kEnded = 16u;   // pending bit: the path ends after this vertex
+
+struct TailLane {
+    bool active;
+    uint32_t mode;        // 0 extension ray, 1 light-NEE shadow ray, 2 env-NEE shadow ray
+    f3 o, d;              // current ray
+    PathVertexState P;
+    uint32_t b;           // bounce
+    uint32_t pending;     // shadow records of the vertex (bits 0, 1), kEnded
+    uint32_t slot;        // queue entry: index of the lane's shadow records
+    uint32_t pix;
+};
+
+// Fields of a lane that are not touched at every query: an LDS column per
+// thread, field f at base[f * kBlock].
+struct TailCold {
+    float* base;
+    enum : uint32_t { kSum = 0, kS = 3, kTr = 4, kTmax = 7, kStep = 8, kNo = 9, kNd = 12, kRows = 15 };
+    __device__ __forceinline__ float get(uint32_t f) const { return base[f * kBlock]; }
+    __device__ __forceinline__ void set(uint32_t f, float v) { base[f * kBlock] = v; }
+    __device__ __forceinline__ uint32_t getu(uint32_t f) const { return __float_as_uint(get(f)); }
+    __device__ __forceinline__ void setu(uint32_t f, uint32_t v) { set(f, __uint_as_float(v)); }
+    __device__ __forceinline__ f3 get3(uint32_t f) const { return f3{get(f), get(f + 1), get(f + 2)}; }
+    __device__ __forceinline__ void set3(uint32_t f, f3 v) { set(f, v.x); set(f + 1, v.y); set(f + 2, v.z); }
+};
+
+__device__ __forceinline__ void tail_start_shadow(const ShadowQueue& q, uint32_t k, TailLane& l, TailCold& c) {
+    const float4 o4 = q.o[l.slot], d4 = q.d[l.slot];
+    l.o = xyz(o4);
+    l.d = xyz(d4);
+    c.set(TailCold::kTmax, o4.w);
+    c.set3(TailCold::kTr, f3{1.0f, 1.0f, 1.0f});
+    c.setu(TailCold::kStep, 0u);
+    l.mode = k + 1u;
+}
+
+// After a vertex or one of its shadow queries: the next shadow query (light
+// before env, as k_shadow<0> runs before k_shadow<1>), the next vertex, the
+// next sample of the pixel, or the pixel's frame value.
+__device__ __forceinline__ void tail_advance(const FrameArgs& F, const ShadowQueue& sl, const ShadowQueue& se,
+                                             const PathState& ps, TailLane& l, TailCold& c) {
+    if (l.pending & 1u) {
+        tail_start_shadow(sl, 0u, l, c);
+        return;
+    }
+    if (l.pending & 2u) {
+        if (se.b[l.slot].w < 0.0f) {
+            // ambient term of the no-env Lambert branch: added without a query
+            l.P.L = l.P.L + xyz(se.a[l.slot]);
+            l.pending &= ~2u;
+        } else {
+            tail_start_shadow(se, 1u, l, c);
+            return;
+        }
+    }
+    if (!(l.pending & kEnded)) {
+        l.o = c.get3(TailCold::kNo);
+        l.d = c.get3(TailCold::kNd);
+        l.b++;
+        l.mode = 0u;
+        return;
+    }
+    const uint32_t s = c.getu(TailCold::kS);
+    const f3 sum = (s == 0u) ? l.P.L : c.get3(TailCold::kSum) + l.P.L;
+    if (s + 1u < F.spp) {
+        // the next sample continues the pixel's stream (k_finish + k_regen)
+        c.set3(TailCold::kSum, sum);
+        c.setu(TailCold::kS, s + 1u);
+        const uint32_t x = l.pix % F.width, y = l.pix / F.width;
+        l.d = camera_ray(F, x, y, l.P.seed);
+        l.o = F.pos;
+        l.P.T_ = f3{1.0f, 1.0f, 1.0f};
+        l.P.L = f3{0.0f, 0.0f, 0.0f};
+        l.P.prev_pdf = 0.0f;
+        l.P.skip_sky = false;
+        l.P.prev_lambert = false;
+        l.b = 0u;
+        l.mode = 0u;
+        return;
+    }
+    const f3 v = frame_value(F, sum);
+    ps.F[l.pix] = make_float4(v.x, v.y, v.z, 0.0f);
+    l.active = false;
+}
+
+}  // namespace
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
+                                                 uint32_t* ctr, PathState ps, ShadowQueue sl, ShadowQueue se,
+                                                 uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    __shared__ float cold[TailCold::kRows * kBlock];
+    TailCold c{cold + threadIdx.x};
+    const uint32_t n = ctr[kCtrRays + in_par];
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    Stack stk = make_stack(lds, spill, spill_depth);
+    TraceCounters tc{0u, 0u, 0u};
+    uint32_t n_primary = 0, n_ext = 0, n_shadow = 0;
+    bool drained = false;
+    TailLane l{};
+    l.active = false;
+    for (;;) {
+        // hand queue entries to idle lanes (wave-uniform)
+        const unsigned long long idle = __ballot(!l.active);
+        if (idle != 0ull && !drained) {
+            const uint32_t want = (uint32_t)__popcll(idle);
+            uint32_t base = 0u;
+            if (lane == 0u) base = atomicAdd(&ctr[kCtrTail], want);
+            base = __shfl(base, 0);
+            const uint32_t i = base + (uint32_t)__popcll(idle & below);
+            if (!l.active && i < n) {
+                const float4 o4 = q.o[i], d4 = q.d[i];
+                l.pix = __float_as_uint(d4.w);
+                l.slot = i;
+                l.o = xyz(o4);
+                l.d = xyz(d4);
+                const float4 Tv = ps.T[l.pix], Lv = ps.L[l.pix];
+                const uint32_t meta = ps.meta[l.pix];
+                const uint32_t flags = __float_as_uint(Lv.w);
+                l.P = PathVertexState{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[l.pix]};
+                l.b = meta & 0xFFu;
+                const uint32_t s = (meta >> 8) & 0xFFu;
+                c.setu(TailCold::kS, s);
+                if (s != 0u) c.set3(TailCold::kSum, xyz(ps.F[l.pix]));
+                l.mode = 0u;
+                l.active = true;
+            }
+            if (base + want >= n) drained = true;
+        }
+        if (__ballot(l.active) == 0ull) break;
+        if (!l.active) continue;
+        const HitRec h = trace_closest<COUNT>(S, l.o, l.d, stk, tc);
+        if (l.mode == 0u) {
+            if (l.b == 0u) n_primary++;
+            else n_ext++;
+            SlotSink sink{{}, sl, se, l.slot, l.pix, {false, false}};
+            VertexResult R;
+            shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
+            const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
+            l.pending = (sink.emitted[0] ? 1u : 0u) | (sink.emitted[1] ? 2u : 0u) | (cont ? 0u : kEnded);
+            if (cont) {
+                c.set3(TailCold::kNo, R.no);
+                c.set3(TailCold::kNd, R.nd);
+            }
+            tail_advance(F, sl, se, ps, l, c);
+        } else {
+            n_shadow++;
+            float t_max = c.get(TailCold::kTmax);
+            f3 tr = c.get3(TailCold::kTr);
+            const uint32_t step = c.getu(TailCold::kStep);
+            if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr)) {
+                const uint32_t k = l.mode - 1u;
+                if (any_pos(tr)) {
+                    // fields picked one by one: a selected reference to either
+                    // queue argument would put both in private memory
+                    const float4 a4 = (k == 0u ? sl.a : se.a)[l.slot], b4 = (k == 0u ? sl.b : se.b)[l.slot];
+                    const uint32_t tag = __float_as_uint((k == 0u ? sl.d : se.d)[l.slot].w);
+                    l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w,
+                                                 k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
+                }
+                l.pending &= ~(1u << k);
+                tail_advance(F, sl, se, ps, l, c);
+            } else {   // transmit_step ends every query by its step refract_bounces
+                c.set(TailCold::kTmax, t_max);
+                c.set3(TailCold::kTr, tr);
+                c.setu(TailCold::kStep, step + 1u);
+            }
+        }
+    }
+    add_stat(&st->primary, n_primary);
+    add_stat(&st->extension, n_ext);
+    add_stat(&st->shadow, n_shadow);
+    if (COUNT) {
+        add_stat(&st->tail_nodes, tc.nodes);
+        add_stat(&st->tail_tris, tc.tris);
+        add_stat(&st->tail_spheres, tc.spheres);
     }
 }
 
@@ -400,6 +600,20 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
     uint32_t snap = 0;
     for (uint64_t it = 0; it < bound; ++it) {
         const uint32_t par = (uint32_t)(it & 1u);
+        if (it == o.tail_after) {
+            // the live queue (and the pixels' later samples) in one launch
+            const uint32_t tg = o.tail_blocks ? (o.tail_blocks < g ? o.tail_blocks : g) : g;
+            if (o.counting)
+                hipLaunchKernelGGL(k_tail<true>, dim3(tg), dim3(kBlock), 0, stream, S, F, par, W.rq[par],
+                                   W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+            else
+                hipLaunchKernelGGL(k_tail<false>, dim3(tg), dim3(kBlock), 0, stream, S, F, par, W.rq[par],
+                                   W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            o.iterations++;
+            break;
+        }
         hipError_t e = o.counting ? iteration<true>(S, F, W, stream, par, g, o)
                                   : iteration<false>(S, F, W, stream, par, g, o);
         if (e != hipSuccess) return e;

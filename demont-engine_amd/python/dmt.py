@@ -24,7 +24,9 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
+ABI_VERSION = 2
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
+SCHEDULE_WAVEFRONT = 0xFFFFFFFF
 MATERIAL_FLOATS = 7
 
 PRIM_FLOATS = 28
@@ -89,6 +91,7 @@ class Stats(C.Structure):
         ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("sphere_tests", C.c_uint64),
         ("ext_node_visits", C.c_uint64), ("ext_tri_tests", C.c_uint64), ("ext_sphere_tests", C.c_uint64),
         ("scene_bytes", C.c_uint64), ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32),
+        ("tail_node_visits", C.c_uint64), ("tail_tri_tests", C.c_uint64), ("tail_sphere_tests", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -137,6 +140,9 @@ def lib() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
     L = C.CDLL(LIB_PATH)
+    v = C.c_uint32()
+    if L.dmt_get_abi_version(C.byref(v)) != 0 or v.value != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI version {v.value}, this binding expects {ABI_VERSION}: rebuild it")
     vp, u32, i32, st = C.c_void_p, C.c_uint32, C.c_int32, C.c_int32
     fp = C.POINTER(C.c_float)
     sigs = {
@@ -157,6 +163,8 @@ def lib() -> C.CDLL:
         "dmt_scene_reset_stats": (st, [vp]),
         "dmt_scene_measure_traffic": (st, [vp, C.POINTER(FrameParams), C.POINTER(Stats)]),
         "dmt_scene_set_profiling": (st, [vp, i32]),
+        "dmt_scene_set_schedule": (st, [vp, u32]),
+        "dmt_scene_get_schedule": (st, [vp, C.POINTER(u32)]),
         "dmt_last_error": (C.c_char_p, []),
         "dmt_scene_generate": (st, [u32, C.c_uint64, C.POINTER(GenOptions), C.POINTER(C.POINTER(GeneratedScene))]),
         "dmt_scene_generated_free": (None, [C.POINTER(GeneratedScene)]),
@@ -371,6 +379,16 @@ class Scene:
 
     def set_profiling(self, on: bool):
         _check(lib().dmt_scene_set_profiling(self._h, 1 if on else 0))
+
+    def set_schedule(self, wavefront_iterations: int):
+        """Per-bounce iterations of a frame before the frame-completion launch
+        (SCHEDULE_WAVEFRONT: per-bounce kernels to the end)."""
+        _check(lib().dmt_scene_set_schedule(self._h, wavefront_iterations))
+
+    def schedule(self) -> int:
+        v = C.c_uint32()
+        _check(lib().dmt_scene_get_schedule(self._h, C.byref(v)))
+        return v.value
 
     def measure_traffic(self, frame: FrameParams) -> dict:
         s = Stats()

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define DMT_ABI_VERSION 1u
+#define DMT_ABI_VERSION 2u
 
 typedef int32_t dmt_status;
 #define DMT_OK                  0
@@ -133,6 +133,9 @@ typedef struct dmt_stats {
     uint64_t scene_bytes;       /* device bytes of the acceleration data     */
     uint32_t bvh_nodes;
     uint32_t bvh_depth;
+    uint64_t tail_node_visits;  /*   frame-completion (tail) kernel: its     */
+    uint64_t tail_tri_tests;    /*   extension and shadow traversals         */
+    uint64_t tail_sphere_tests;
 } dmt_stats;
 
 /* ABI version check. */
@@ -189,6 +192,15 @@ dmt_status dmt_scene_reset_stats(dmt_scene* scene);
 dmt_status dmt_scene_measure_traffic(dmt_scene* scene, const dmt_frame_params* frame, dmt_stats* out);
 /* Per-event timing of the closest-hit kernel launches on/off (default off). */
 dmt_status dmt_scene_set_profiling(dmt_scene* scene, int32_t enabled);
+/* Frame schedule (no counterpart in the reference, whose frame is one
+ * dispatch): the first `wavefront_iterations` bounces of a frame run as
+ * per-bounce kernels over the live path queue, then one frame-completion
+ * launch runs every remaining path, and the pixels' later samples, to the end
+ * with lanes refilled from the queue.  DMT_SCHEDULE_WAVEFRONT keeps per-bounce
+ * kernels to the end.  Images are identical under every schedule. */
+#define DMT_SCHEDULE_WAVEFRONT 0xFFFFFFFFu
+dmt_status dmt_scene_set_schedule(dmt_scene* scene, uint32_t wavefront_iterations);
+dmt_status dmt_scene_get_schedule(dmt_scene* scene, uint32_t* wavefront_iterations);
 
 /* ---- Material parameters and the adjoint -----------------------------------
  * Per analytic primitive, 7 floats: albedo rgb (v1.xyz), roughness (v1.w),

@@ -212,6 +212,59 @@ def test_render_frames_equals_sequential_frames(device, cfg, kw, spp):
     assert np.all(img_a[..., 3] == K)
 
 
+_SCHEDULE_SCENES = [
+    ("C0 1spp", 0, dict(seed=6, width=128, height=96), 1, False),
+    ("C0 no env", 0, dict(seed=4, width=96, height=64), 1, True),
+    ("C4 4spp", 4, dict(seed=7, spheres=40, width=64, height=48), 4, False),
+    ("C1 small", 1, dict(seed=8, subdivisions=4, random_tris=5000, width=160, height=90), 1, False),
+    ("C2 small 2spp", 2, dict(seed=3, spheres=200, subdivisions=3, random_tris=5000, lights=64, width=96,
+                              height=54), 2, False),
+]
+
+
+@pytest.mark.parametrize("label,cfg,kw,spp,no_env", _SCHEDULE_SCENES, ids=[s[0] for s in _SCHEDULE_SCENES])
+def test_schedules_render_identical_images(device, label, cfg, kw, spp, no_env):
+    """The frame-completion launch (k_tail) after any number of per-bounce
+    iterations gives the image, the ray counts and the traversal totals of
+    per-bounce kernels to the end, over three accumulated frames."""
+    sd = dmt.generate(cfg, **kw)
+    if no_env:
+        sd.env = None
+    f = sd.frame.copy()
+    f.spp = spp
+    scene = dmt.Scene(device, sd)
+    results = {}
+    for sched in (dmt.SCHEDULE_WAVEFRONT, 0, 1, 2, 3, 5):
+        scene.set_schedule(sched)
+        assert scene.schedule() == sched
+        scene.reset_stats()
+        for k in range(3):
+            fk = f.copy()
+            fk.frame_index = 11 + k
+            fk.reset_accum = 1 if k == 0 else 0
+            scene.render(fk)
+        img = scene.readback(f.width, f.height)
+        st = scene.stats()
+        tr = scene.measure_traffic(f)
+        results[sched] = (img, (st["primary_rays"], st["extension_rays"], st["shadow_rays"]),
+                          (tr["node_visits"], tr["tri_tests"], tr["sphere_tests"]))
+    ref_img, ref_rays, ref_trav = results[dmt.SCHEDULE_WAVEFRONT]
+    assert ref_rays[0] == 3 * spp * f.width * f.height
+    for sched, (img, rays, trav) in results.items():
+        print(f"{label}: schedule {sched}: image equal {np.array_equal(img, ref_img)}, rays {rays}, traversal {trav}")
+        assert np.array_equal(img, ref_img), f"schedule {sched}"
+        assert rays == ref_rays, f"schedule {sched}"
+        assert trav == ref_trav, f"schedule {sched}"
+    oscene = orc.OracleScene.from_scene_data(sd)
+    ref = None
+    for k in range(3):
+        fk = f.copy()
+        fk.frame_index = 11 + k
+        fk.reset_accum = 1 if k == 0 else 0
+        ref, _ = oscene.render(fk, 32, 0, accum=ref)
+    assert rel_mse(ref_img[..., :3], ref[..., :3]) <= RADIANCE_REL_MSE
+
+
 def test_error_paths_through_the_abi(device):
     sd = dmt.generate(0, seed=1, width=16, height=16)
     sc = dmt.Scene(device, sd)
