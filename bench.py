@@ -177,15 +177,16 @@ def main():
     log(f"[bench] {args.steps} frames in {dt * 1e3:.1f} ms: {mrays:.1f} Mrays/s "
         f"(primary {st['primary_rays']}, extension {st['extension_rays']}, shadow {st['shadow_rays']})")
 
-    # algorithmic traffic of the closest-hit kernel, from an instrumented re-render
+    # Algorithmic bytes of the closest-hit kernel (k_extend) and of the
+    # frame-completion kernel (k_tail), from an instrumented re-render.
     f = frame.copy()
     f.frame_index = args.warmup
     tr = scene.measure_traffic(f)
-    ext_rays = tr["primary_rays"] + tr["extension_rays"]
-    bytes_per_frame = (tr["ext_node_visits"] * 128 + tr["ext_tri_tests"] * 48 + tr["ext_sphere_tests"] * 20
-                       + ext_rays * 40)
+    ext_bytes = (tr["ext_node_visits"] * 128 + tr["ext_tri_tests"] * 48 + tr["ext_sphere_tests"] * 20
+                 + tr["ext_rays"] * 40)
+    tail_bytes = tr["tail_node_visits"] * 128 + tr["tail_tri_tests"] * 48 + tr["tail_sphere_tests"] * 20
     # In the timed region three frames share the GPU, so a launch's event
-    # duration there includes other frames' kernels running beside it.  The
+    # duration there includes other frames' kernels running beside it.  A
     # kernel's own launch duration comes from frames rendered one at a time
     # (each waits for the previous one): the same frame as the instrumented
     # re-render, so its algorithmic bytes are exact.
@@ -198,15 +199,26 @@ def main():
             dev.synchronize()
 
     _, st_iso = timed(isolated)
-    launches_per_frame = max(st_iso["extend_launches"], 1) / args.roofline_frames
-    bytes_per_launch = bytes_per_frame / launches_per_frame
-    avg_launch_ms = max(st_iso["extend_ms"], 1e-9) / max(st_iso["extend_launches"], 1)
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    shared_ms = max(st["extend_ms"], 1e-9) / max(st["extend_launches"], 1)
-    log(f"[bench] extend: {tr['ext_node_visits'] / max(ext_rays, 1):.1f} nodes/ray, "
-        f"{tr['ext_tri_tests'] / max(ext_rays, 1):.1f} tri tests/ray, {bytes_per_launch / 1e6:.1f} MB/launch, "
-        f"avg launch {avg_launch_ms:.3f} ms alone -> {achieved:.0f} GB/s "
-        f"({shared_ms:.3f} ms with three frames in flight -> {bytes_per_launch / (shared_ms * 1e-3) / 1e9:.0f} GB/s)")
+    nf = args.roofline_frames
+    kern = {}
+    for name, key, nbytes in (("k_extend", "extend", ext_bytes), ("k_tail", "tail", tail_bytes)):
+        launches = st_iso[f"{key}_launches"]
+        if launches == 0:
+            continue
+        per_launch = nbytes * nf / launches
+        avg_ms = st_iso[f"{key}_ms"] / launches
+        shared = st[f"{key}_ms"] / max(st[f"{key}_launches"], 1)
+        kern[name] = dict(bytes_per_launch=per_launch, avg_ms=avg_ms, ms_per_frame=st_iso[f"{key}_ms"] / nf,
+                          launches=launches, achieved=per_launch / (avg_ms * 1e-3) / 1e9)
+        log(f"[bench] {name}: {launches / nf:.0f} launch(es)/frame, {per_launch / 1e6:.1f} MB/launch, "
+            f"avg launch {avg_ms:.3f} ms alone -> {kern[name]['achieved']:.0f} GB/s "
+            f"({shared:.3f} ms with three frames in flight)")
+    log(f"[bench] k_extend: {tr['ext_rays']} rays, {tr['ext_node_visits'] / max(tr['ext_rays'], 1):.1f} nodes/ray, "
+        f"{tr['ext_tri_tests'] / max(tr['ext_rays'], 1):.2f} tri tests/ray; k_tail: {tr['tail_node_visits']} node "
+        f"visits, {tr['tail_tri_tests']} tri tests, {tr['tail_sphere_tests']} sphere tests")
+    dom = max(kern, key=lambda k: kern[k]["ms_per_frame"])
+    K = kern[dom]
+    bytes_per_launch, avg_launch_ms, achieved = K["bytes_per_launch"], K["avg_ms"], K["achieved"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -269,11 +281,15 @@ def main():
                    "schedule": ("per-bounce kernels" if schedule == dmt.SCHEDULE_WAVEFRONT else
                                 f"{schedule} per-bounce iteration(s), then one frame-completion launch")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic() if cfg == 1 else None,
-                     "kernel": "k_extend (closest hit)", "bytes_per_launch": int(bytes_per_launch),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic() if (cfg == 1 and dom == "k_extend") else None,
+                     "kernel": dom, "bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_ms": round(avg_launch_ms, 5),
-                     "timing": f"{int(st_iso['extend_launches'])} launches of {args.roofline_frames} frames rendered "
-                               "one at a time, HIP events on the launch stream"},
+                     "timing": f"{int(K['launches'])} launches of {args.roofline_frames} frames rendered "
+                               "one at a time, HIP events on the launch stream",
+                     "other_kernels": {k: {"achieved": round(v["achieved"], 1), "avg_launch_ms": round(v["avg_ms"], 5),
+                                           "bytes_per_launch": int(v["bytes_per_launch"])}
+                                       for k, v in kern.items() if k != dom}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

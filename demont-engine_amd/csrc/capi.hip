@@ -23,14 +23,15 @@
 using namespace dmt;
 
 // Frames of per-frame dmt_render calls kept in flight (one stream each).
-#ifndef DMT_TAIL_AFTER
-#define DMT_TAIL_AFTER 1
-#endif
-constexpr uint32_t kDefaultTailAfter = DMT_TAIL_AFTER;
+// Default frame schedule: the camera rays' bounce by the per-bounce kernels
+// (k_extend, k_shade, ...), the rest of the frame by one k_tail launch.
+constexpr uint32_t kDefaultTailAfter = 1;
 constexpr uint32_t kTailBlocksPerCu = 4;   // k_tail grid: resident workgroups per CU
 #ifndef DMT_FRAME_SETS
-#define DMT_FRAME_SETS 3
+#define DMT_FRAME_SETS 4
 #endif
+// Frames in flight, one HIP stream each (the runtime gives a process four
+// hardware queues by default).
 constexpr uint32_t kFrameSets = DMT_FRAME_SETS;
 
 namespace {
@@ -90,9 +91,9 @@ struct dmt_scene {
     uint32_t bvh_nodes = 0;
     uint64_t scene_bytes = 0;
     // kFrameSets wavefront sets: frame k of per-frame calls runs on set
-    // k % kFrameSets and its stream, so the short late-bounce launches of one
-    // frame overlap the next frames' heavy bounces.  Only the final
-    // accumulation is ordered frame after frame.
+    // k % kFrameSets and its stream, so the latency-bound end of one frame
+    // (the last paths of its k_tail launch) overlaps the next frames' camera
+    // bounce.  Only the final accumulation is ordered frame after frame.
     Wavefront wf[kFrameSets]{};
     std::vector<void*> wf_buffers;
     float4* accum = nullptr;
@@ -110,8 +111,11 @@ struct dmt_scene {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+    std::vector<uint8_t> ev_kind;   // per event pair: 0 k_extend, 1 k_tail
     double extend_ms = 0.0;
     uint64_t extend_launches = 0;
+    double tail_ms = 0.0;
+    uint64_t tail_launches = 0;
     uint64_t frames = 0;
     // Batches hold their scenes: dmt_scene_destroy of a member is deferred
     // to the last dmt_batch_destroy, and the resolution is fixed meanwhile.
@@ -288,8 +292,9 @@ dmt_status harvest_events(dmt_scene* s) {
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize");
     for (size_t k = 0; k + 1 < s->ev_used; k += 2) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, s->ev_pool[k], s->ev_pool[k + 1]) == hipSuccess) s->extend_ms += ms;
-        s->extend_launches++;
+        const bool tail = s->ev_kind[k / 2] != 0u;
+        if (hipEventElapsedTime(&ms, s->ev_pool[k], s->ev_pool[k + 1]) == hipSuccess) (tail ? s->tail_ms : s->extend_ms) += ms;
+        (tail ? s->tail_launches : s->extend_launches)++;
     }
     s->ev_used = 0;
     return DMT_OK;
@@ -511,7 +516,9 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
                 if (he != hipSuccess) return hip_fail(he, "hipEventCreate");
                 s->ev_pool.push_back(e);
             }
+            if (s->ev_kind.size() < s->ev_pool.size() / 2) s->ev_kind.resize(s->ev_pool.size() / 2);
             o.extend_events = s->ev_pool.data() + s->ev_used;
+            o.event_kinds = s->ev_kind.data() + s->ev_used / 2;
             o.max_events = (uint32_t)bound;
         }
         o.tail_after = s->tail_after;
@@ -597,6 +604,8 @@ dmt_status dmt_scene_stats(dmt_scene* s, dmt_stats* out) {
     out->frames = s->frames;
     out->extend_launches = s->extend_launches;
     out->extend_ms = s->extend_ms;
+    out->tail_launches = s->tail_launches;
+    out->tail_ms = s->tail_ms;
     out->scene_bytes = s->scene_bytes;
     out->bvh_nodes = s->bvh_nodes;
     out->bvh_depth = s->bvh_depth;
@@ -610,6 +619,8 @@ dmt_status dmt_scene_reset_stats(dmt_scene* s) {
     if ((st = harvest_events(s))) return st;
     s->extend_ms = 0.0;
     s->extend_launches = 0;
+    s->tail_ms = 0.0;
+    s->tail_launches = 0;
     s->frames = 0;
     {
         hipError_t e = hipMemsetAsync(s->stats, 0, sizeof(DeviceStats), s->dev->stream);
@@ -676,6 +687,7 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     out->ext_node_visits = h.ext_nodes;
     out->ext_tri_tests = h.ext_tris;
     out->ext_sphere_tests = h.ext_spheres;
+    out->ext_rays = h.ext_rays;
     out->tail_node_visits = h.tail_nodes;
     out->tail_tri_tests = h.tail_tris;
     out->tail_sphere_tests = h.tail_spheres;

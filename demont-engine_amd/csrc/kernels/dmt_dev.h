@@ -247,6 +247,11 @@ struct HitRec {
 
 // PBRT's robust slab test bound: 1 + 2*gamma(3), gamma(n) = n*eps / (1 - n*eps).
 __device__ __forceinline__ constexpr float robust_hi() { return 1.0f + 2.0f * (3.0f * 5.9604645e-8f / (1.0f - 3.0f * 5.9604645e-8f)); }
+// 1 / robust_hi() rounded down (0x3F7FFFFA): t * robust_lo() <= t / robust_hi()
+// for t >= 0 (a product that is exactly smaller rounds no higher), so the
+// entry distance shrinks at least as much as by the division, at the cost
+// of one multiply instead of a correctly rounded divide.
+__device__ __forceinline__ constexpr float robust_lo() { return 0.99999964f; }
 
 // Every traversal step fetches kFetch float4 (112 B) from one address: an
 // interior node's six bound rows and its children, or the primitives of a leaf
@@ -278,8 +283,9 @@ __device__ __forceinline__ uint32_t node4_test(const float4 (&r)[kFetch], f3 o, 
         float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
         float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
         tmax = tmax * g;
-        tmin = tmin / g;
-        bool hit = (ch[i] != kEmpty) && tmax >= fmaxf(tmin, 0.0f) && tmin <= tbest;
+        tmin = tmin * robust_lo();   // tmin < 0 only meets max(tmin, 0) and tmin <= tbest
+        // non-short-circuit: straight-line code, no branch per child
+        const bool hit = (ch[i] != kEmpty) & (tmax >= fmaxf(tmin, 0.0f)) & (tmin <= tbest);
         tn[i] = hit ? tmin : __builtin_huge_valf();
         mask |= hit ? (1u << i) : 0u;
     }
@@ -309,6 +315,13 @@ __device__ __forceinline__ void bvh4_walk(const float4* nodes, const float4* pri
         float4 r[kFetch];
 #pragma unroll
         for (uint32_t k = 0; k < kFetch; ++k) r[k] = p[k];
+        // Keep the fetch whole and ahead of the branch: left alone, the
+        // compiler sinks the parts only interior nodes read (the children,
+        // the .w lanes of the bound rows) into the node branch as separate
+        // loads, a second memory wait for the wave.  The empty statement
+        // (no instruction) needs every register of the fetch here.
+#pragma unroll
+        for (uint32_t k = 0; k < kFetch; ++k) asm volatile("" ::"v"(r[k].x), "v"(r[k].y), "v"(r[k].z), "v"(r[k].w));
         if (at_leaf) {
             const uint32_t count = ((cur >> 27) & 15u) + 1u;
             const uint32_t used = leaf(first, count, r);
@@ -328,8 +341,11 @@ __device__ __forceinline__ void bvh4_walk(const float4* nodes, const float4* pri
                 cswap(tn[0], ch[0], tn[2], ch[2]);
                 cswap(tn[1], ch[1], tn[3], ch[3]);
                 cswap(tn[1], ch[1], tn[2], ch[2]);
+                // push the farther hits, farthest first
                 const uint32_t n = __popc(m);
-                for (int i = (int)n - 1; i >= 1; --i) st.put(sp++, ch[i]);
+                if (n > 3u) st.put(sp++, ch[3]);
+                if (n > 2u) st.put(sp++, ch[2]);
+                if (n > 1u) st.put(sp++, ch[1]);
                 cur = ch[0];
                 continue;
             }

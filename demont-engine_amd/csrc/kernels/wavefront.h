@@ -18,10 +18,11 @@ namespace dmt {
 
 constexpr uint32_t kBlock = 256;          // threads per workgroup (4 waves)
 #ifndef DMT_LDS_DEPTH
-#define DMT_LDS_DEPTH 24
+#define DMT_LDS_DEPTH 16
 #endif
-// LDS stack entries per thread (build-time tunable: LDS per workgroup bounds
-// the waves a CU holds)
+// LDS stack entries per thread (build-time tunable).  16 entries = 16 KB per
+// workgroup: k_extend (under 64 VGPRs) then holds 8 waves per SIMD; deeper
+// entries spill to a per-thread global area.
 constexpr uint32_t kLdsDepth = DMT_LDS_DEPTH;
 constexpr uint32_t kMaxGridBlocks = 4096; // grid-stride cap: 1M threads
 
@@ -72,6 +73,7 @@ struct DeviceStats {
     unsigned long long ext_nodes;   // counting traversal, extend kernel
     unsigned long long ext_tris;
     unsigned long long ext_spheres;
+    unsigned long long ext_rays;    // counting traversal: rays of the extend kernel
     unsigned long long tail_nodes;  // counting traversal, tail kernel (extension and shadow)
     unsigned long long tail_tris;
     unsigned long long tail_spheres;
@@ -105,7 +107,8 @@ constexpr uint32_t kPollSlots = 4;
 constexpr uint32_t kPollChunk = 4;        // iterations between live-queue snapshots
 
 struct LaunchOptions {
-    hipEvent_t* extend_events = nullptr;  // 2 per iteration, or null
+    hipEvent_t* extend_events = nullptr;  // a pair around each k_extend / k_tail launch, or null
+    uint8_t* event_kinds = nullptr;       // per pair: 0 k_extend, 1 k_tail
     uint32_t max_events = 0;
     uint32_t events_used = 0;             // out
     hipEvent_t accum_after = nullptr;     // the accumulation pass waits for it

@@ -108,20 +108,18 @@ __global__ void __launch_bounds__(kBlock) k_generate(FrameArgs F, PathState ps, 
     }
 }
 
-#ifdef DMT_EXT_WAVES
-#define DMT_EXT_ATTR __attribute__((amdgpu_waves_per_eu(DMT_EXT_WAVES, DMT_EXT_WAVES)))
-#else
-#define DMT_EXT_ATTR
-#endif
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) DMT_EXT_ATTR k_extend(SceneView S, RayQueue q, uint32_t* ctr, uint32_t in_par,
+__global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint32_t* ctr, uint32_t in_par,
                                                    uint2* hits, uint32_t* spill, uint32_t spill_depth,
                                                    DeviceStats* st) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
     const uint32_t n = ctr[kCtrRays + in_par];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    if (gid == 0) ctr[kCtrRays + (in_par ^ 1u)] = 0u;   // the queue this iteration fills
+    if (gid == 0) {
+        ctr[kCtrRays + (in_par ^ 1u)] = 0u;   // the queue this iteration fills
+        if (COUNT) atomicAdd(&st->ext_rays, (unsigned long long)n);
+    }
     Stack stk = make_stack(lds, spill, spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     for (uint32_t i = gid; i < n; i += stride) {
@@ -564,7 +562,10 @@ template <bool COUNT>
 static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W, hipStream_t stream, uint32_t par,
                             uint32_t g, LaunchOptions& o) {
     const bool ev = o.extend_events && o.events_used + 2u <= o.max_events;
-    if (ev) (void)hipEventRecord(o.extend_events[o.events_used], stream);
+    if (ev) {
+        o.event_kinds[o.events_used / 2u] = 0u;
+        (void)hipEventRecord(o.extend_events[o.events_used], stream);
+    }
     hipLaunchKernelGGL(k_extend<COUNT>, dim3(g), dim3(kBlock), 0, stream, S, W.rq[par], W.counters, par, W.hits,
                        W.spill, W.spill_depth, W.stats);
     if (ev) {
@@ -603,12 +604,21 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
         if (it == o.tail_after) {
             // the live queue (and the pixels' later samples) in one launch
             const uint32_t tg = o.tail_blocks ? (o.tail_blocks < g ? o.tail_blocks : g) : g;
+            const bool ev = o.extend_events && o.events_used + 2u <= o.max_events;
+            if (ev) {
+                o.event_kinds[o.events_used / 2u] = 1u;
+                (void)hipEventRecord(o.extend_events[o.events_used], stream);
+            }
             if (o.counting)
                 hipLaunchKernelGGL(k_tail<true>, dim3(tg), dim3(kBlock), 0, stream, S, F, par, W.rq[par],
                                    W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
             else
                 hipLaunchKernelGGL(k_tail<false>, dim3(tg), dim3(kBlock), 0, stream, S, F, par, W.rq[par],
                                    W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+            if (ev) {
+                (void)hipEventRecord(o.extend_events[o.events_used + 1u], stream);
+                o.events_used += 2u;
+            }
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
             o.iterations++;

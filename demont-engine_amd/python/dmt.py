@@ -92,6 +92,7 @@ class Stats(C.Structure):
         ("ext_node_visits", C.c_uint64), ("ext_tri_tests", C.c_uint64), ("ext_sphere_tests", C.c_uint64),
         ("scene_bytes", C.c_uint64), ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32),
         ("tail_node_visits", C.c_uint64), ("tail_tri_tests", C.c_uint64), ("tail_sphere_tests", C.c_uint64),
+        ("ext_rays", C.c_uint64), ("tail_launches", C.c_uint64), ("tail_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -136,6 +136,9 @@ typedef struct dmt_stats {
     uint64_t tail_node_visits;  /*   frame-completion (tail) kernel: its     */
     uint64_t tail_tri_tests;    /*   extension and shadow traversals         */
     uint64_t tail_sphere_tests;
+    uint64_t ext_rays;          /*   rays of the closest-hit kernel          */
+    uint64_t tail_launches;     /* frame-completion kernel launches (timed   */
+    double   tail_ms;           /*   like extend_launches / extend_ms)       */
 } dmt_stats;
 
 /* ABI version check. */
@@ -190,7 +193,8 @@ dmt_status dmt_scene_reset_stats(dmt_scene* scene);
 /* Re-renders one frame with the counting traversal (outside any timed region)
  * and fills node_visits / tri_tests / sphere_tests. */
 dmt_status dmt_scene_measure_traffic(dmt_scene* scene, const dmt_frame_params* frame, dmt_stats* out);
-/* Per-event timing of the closest-hit kernel launches on/off (default off). */
+/* Per-event timing of the closest-hit and frame-completion kernel launches
+ * on/off (default off). */
 dmt_status dmt_scene_set_profiling(dmt_scene* scene, int32_t enabled);
 /* Frame schedule (no counterpart in the reference, whose frame is one
  * dispatch): the first `wavefront_iterations` bounces of a frame run as
