@@ -26,7 +26,8 @@ using namespace dmt;
 // Default frame schedule: the camera rays' bounce by the per-bounce kernels
 // (k_extend, k_shade, ...), the rest of the frame by one k_tail launch.
 constexpr uint32_t kDefaultTailAfter = 1;
-constexpr uint32_t kTailBlocksPerCu = 4;   // k_tail grid: resident workgroups per CU
+constexpr uint32_t kTailBlocksPerCu = kTailWavesPerSimd;   // k_tail grid: resident workgroups per CU
+constexpr uint32_t kExtendBlocksPerCu = 8;   // k_extend grid: 8 waves per SIMD, rays handed out dynamically
 #ifndef DMT_FRAME_SETS
 #define DMT_FRAME_SETS 4
 #endif
@@ -273,6 +274,7 @@ dmt_status check_frame(const dmt_frame_params* f) {
     if (!f) return fail(DMT_ERR_INVALID_ARG, "frame params are null");
     if (f->width == 0 || f->height == 0) return fail(DMT_ERR_INVALID_ARG, "width and height must be > 0");
     if ((uint64_t)f->width * f->height > (1ull << 26)) return fail(DMT_ERR_LIMIT, "more than 2^26 pixels");
+    if (f->width > 65535u || f->height > 65535u) return fail(DMT_ERR_LIMIT, "width and height must be <= 65535");
     if (f->spp < 1 || f->spp > DMT_MAX_SPP) return fail(DMT_ERR_INVALID_ARG, "spp must be in 1..32");
     if (f->max_bounces > DMT_MAX_BOUNCES) return fail(DMT_ERR_LIMIT, "max_bounces must be <= 64");
     if (f->refract_bounces > 64) return fail(DMT_ERR_LIMIT, "refract_bounces must be <= 64");
@@ -523,6 +525,7 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
         }
         o.tail_after = s->tail_after;
         o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
+        o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
         hipError_t e = launch_frame(s->view, F, s->wf[set], s->accum, stream, o);
         s->ev_used += o.events_used;
         if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
@@ -671,6 +674,7 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     o.counting = true;
     o.tail_after = s->tail_after;
     o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
+    o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
     if (e == hipSuccess) e = launch_frame(s->view, F, W, scratch, s->dev->stream, o);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
@@ -804,7 +808,7 @@ struct dmt_batch {
 namespace {
 
 constexpr uint32_t kBatchMaxScenes = 4096;
-constexpr uint32_t kBatchMaxBlocksPerCu = 4;
+constexpr uint32_t kBatchMaxBlocksPerCu = 5;
 
 dmt_status normalize_params(const dmt_batch_params* in, dmt_batch_params* out) {
     dmt_batch_params p{};
@@ -816,7 +820,7 @@ dmt_status normalize_params(const dmt_batch_params* in, dmt_batch_params* out) {
     if (p.tile_w != 8 && p.tile_w != 16 && p.tile_w != 32 && p.tile_w != 64)
         return fail(DMT_ERR_INVALID_ARG, "tile_w must be 8, 16, 32 or 64");
     if (p.tiles_per_fetch > 16) return fail(DMT_ERR_INVALID_ARG, "tiles_per_fetch must be in 1..16");
-    if (p.blocks_per_cu > kBatchMaxBlocksPerCu) return fail(DMT_ERR_INVALID_ARG, "blocks_per_cu must be in 1..4");
+    if (p.blocks_per_cu > kBatchMaxBlocksPerCu) return fail(DMT_ERR_INVALID_ARG, "blocks_per_cu must be in 1..5");
     if (p.stack_lds != 16 && p.stack_lds != 24 && p.stack_lds != 32)
         return fail(DMT_ERR_INVALID_ARG, "stack_lds must be 16, 24 or 32");
     if (p.plain_launches > 1) return fail(DMT_ERR_INVALID_ARG, "plain_launches must be 0 or 1");
@@ -1000,7 +1004,6 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         b->order.insert(b->order.end(), queue[q].begin(), queue[q].end());
     }
     b->queue_begin[kXcds] = (uint32_t)b->order.size();
-
     const std::pair<void**, size_t> req[] = {
         {(void**)&b->d_scenes, count * sizeof(BatchScene)},
         {(void**)&b->d_order, count * sizeof(uint32_t)},
@@ -1143,14 +1146,14 @@ dmt_status dmt_batch_autotune(dmt_batch* b, uint32_t frames_per_trial, dmt_batch
     float best_ms = 0.0f;
     st = trial(best, &best_ms);
     // coordinate descent, one parameter at a time
-    const uint32_t blocks_v[] = {1, 2, 3, 4}, stack_v[] = {16, 24, 32}, tile_v[] = {8, 16, 32, 64},
+    const uint32_t blocks_v[] = {1, 2, 3, 4, 5}, stack_v[] = {16, 24, 32}, tile_v[] = {8, 16, 32, 64},
                    fetch_v[] = {1, 2, 4, 8};
     struct Axis {
         uint32_t dmt_batch_params::*field;
         const uint32_t* values;
         size_t n;
     };
-    const Axis axes[] = {{&dmt_batch_params::blocks_per_cu, blocks_v, 4},
+    const Axis axes[] = {{&dmt_batch_params::blocks_per_cu, blocks_v, 5},
                          {&dmt_batch_params::stack_lds, stack_v, 3},
                          {&dmt_batch_params::tile_w, tile_v, 4},
                          {&dmt_batch_params::tiles_per_fetch, fetch_v, 4}};

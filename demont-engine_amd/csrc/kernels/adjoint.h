@@ -37,6 +37,9 @@ constexpr uint32_t kAdjMaxVerts = 8;       // max_bounces limit of the adjoint
 constexpr uint32_t kAdjMaxDielectric = 8;  // refract_bounces limit of the adjoint
 constexpr uint32_t kAdjLdsPrims = 1024;    // per-workgroup LDS accumulation up to this many prims
 constexpr uint32_t kMaterialFloats = 7;    // albedo rgb, roughness, emission rgb
+// Register budget of the adjoint kernels: three waves per SIMD (168 VGPRs,
+// a few values spilled outside the traversal loop) instead of two.
+constexpr uint32_t kAdjWavesPerSimd = 3;
 
 struct AdjointArgs {
     SceneView S;

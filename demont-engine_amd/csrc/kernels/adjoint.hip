@@ -146,7 +146,8 @@ struct AdjointSink : NoGradientHooks {
 };
 
 template <bool LDS>
-__global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs A) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
+k_adjoint(AdjointArgs A) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
     __shared__ float lds_grad[LDS ? kAdjLdsPrims * kMaterialFloats : 1u];
     const uint32_t nparam = A.prim_count * kMaterialFloats;
@@ -254,7 +255,8 @@ struct TangentSink : NoGradientHooks {
 };
 
 template <bool LDS>
-__global__ void __launch_bounds__(kBlock) k_adjoint_rough(AdjointArgs A) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
+k_adjoint_rough(AdjointArgs A) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
     __shared__ float lds_grad[LDS ? kAdjLdsPrims : 1u];
     if (LDS) {

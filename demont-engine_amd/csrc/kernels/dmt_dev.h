@@ -262,6 +262,9 @@ __device__ __forceinline__ constexpr float robust_lo() { return 0.99999964f; }
 constexpr uint32_t kFetch = 7u;
 constexpr uint32_t kLeafPad = 8u;
 constexpr uint32_t kLeafFirst = (1u << 27) - 1u;   // leaf reference: first primitive
+// trace_queue hands out rays to a wave once all its lanes are idle: partial
+// refills cost an atomic and two dependent loads each and measured slower.
+constexpr uint32_t kRefillLanes = 64u;
 
 // Tests the four children of a fetched node; returns the hit mask and entry
 // distances.
@@ -355,14 +358,23 @@ __device__ __forceinline__ void bvh4_walk(const float4* nodes, const float4* pri
     }
 }
 
-// traceScene (PathTrace.slang:2331-2693): planes, then the sphere BVH, then
-// the triangle BVH limited to the analytic hit distance.  Ties resolve to
-// the lower primitive index; analytic hits win ties against triangles.
-template <bool COUNT>
-__device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, Stack& st, TraceCounters& tc) {
-    float best = 1e30f;
-    uint32_t best_id = 0xFFFFFFFFu;   // analytic input index of the best hit
-    uint32_t code = kMiss;
+// Closest analytic hit so far: distance, input index (tie-break), hit code.
+struct AnalyticBest {
+    float t;
+    uint32_t id;
+    uint32_t code;
+};
+
+// Closest triangle hit so far, limited to the analytic distance.
+struct TriBest {
+    float t;
+    uint32_t id;    // a triangle at exactly the analytic distance never wins
+    uint32_t pos;   // leaf position, kMiss for none
+};
+
+// Planes first (intersectPlane, |n.d| >= 1e-4, t > 1e-3).
+__device__ __forceinline__ AnalyticBest planes_closest(const SceneView& S, f3 o, f3 d) {
+    AnalyticBest b{1e30f, 0xFFFFFFFFu, kMiss};
     for (uint32_t i = 0; i < S.plane_count; ++i) {
         const float4 p = S.planes[i];
         const uint32_t id = S.plane_ids[i];
@@ -370,76 +382,213 @@ __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, 
         float denom = dot(d, n);
         if (fabsf(denom) < 1e-4f) continue;
         float t = -(dot(o, n) + p.w) / denom;
-        if (t > 1e-3f && (t < best || (t == best && id < best_id))) {
-            best = t; best_id = id; code = id;
+        if (t > 1e-3f && (t < b.t || (t == b.t && id < b.id))) {
+            b.t = t; b.id = id; b.code = id;
         }
     }
+    return b;
+}
+
+__device__ __forceinline__ f3 safe_inverse(f3 d) {
     f3 rs{fabsf(d.x) > 1e-30f ? d.x : 1e-30f, fabsf(d.y) > 1e-30f ? d.y : 1e-30f, fabsf(d.z) > 1e-30f ? d.z : 1e-30f};
-    f3 inv{1.0f / rs.x, 1.0f / rs.y, 1.0f / rs.z};
-    if (S.sph_nodes) {
-        bvh4_walk<COUNT, 1u>(S.sph_nodes, S.spheres, o, inv, best, st, [&](uint32_t first, uint32_t count,
-                                                                             const float4 (&r)[kFetch]) {
-            const uint32_t n = count < 4u ? count : 4u;
+    return f3{1.0f / rs.x, 1.0f / rs.y, 1.0f / rs.z};
+}
+
+// Up to four spheres of a leaf held in the fetch (intersectSphere: closest
+// root > 1e-3); returns how many it tested.
+template <bool COUNT>
+__device__ __forceinline__ uint32_t sphere_leaf(const SceneView& S, f3 o, f3 d, uint32_t first, uint32_t count,
+                                                const float4 (&r)[kFetch], AnalyticBest& b, TraceCounters& tc) {
+    const uint32_t n = count < 4u ? count : 4u;
 #pragma unroll
-            for (uint32_t k = 0; k < 4u; ++k) {
-                if (k >= n) break;
-                const float4 s = r[k];
-                if (COUNT) tc.spheres++;
-                f3 oc = o - xyz(s);
-                float b = dot(oc, d);
-                float kk = dot(oc, oc) - s.w * s.w;
-                float h = b * b - kk;
-                if (h < 0.0f) continue;
-                h = sqrtf(h);
-                float t0 = -b - h, t1 = -b + h;
-                float t = (t0 > 1e-3f) ? t0 : t1;
-                if (t > 1e-3f && t <= best) {   // the id only matters for a candidate
-                    const uint32_t id = S.sphere_ids[first + k];
-                    if (t < best || id < best_id) {
-                        best = t; best_id = id; code = id;
-                    }
-                }
+    for (uint32_t k = 0; k < 4u; ++k) {
+        if (k >= n) break;
+        const float4 s = r[k];
+        if (COUNT) tc.spheres++;
+        f3 oc = o - xyz(s);
+        float bb = dot(oc, d);
+        float kk = dot(oc, oc) - s.w * s.w;
+        float h = bb * bb - kk;
+        if (h < 0.0f) continue;
+        h = sqrtf(h);
+        float t0 = -bb - h, t1 = -bb + h;
+        float t = (t0 > 1e-3f) ? t0 : t1;
+        if (t > 1e-3f && t <= b.t) {   // the id only matters for a candidate
+            const uint32_t id = S.sphere_ids[first + k];
+            if (t < b.t || id < b.id) {
+                b.t = t; b.id = id; b.code = id;
             }
-            return n;
-        }, tc);
-    }
-    if (S.tri_nodes) {
-        float tm = best;
-        uint32_t tri_id = 0u;   // a triangle at exactly the analytic distance never wins
-        uint32_t tri_pos = kMiss;
-        bvh4_walk<COUNT, 3u>(S.tri_nodes, S.tris, o, inv, tm, st, [&](uint32_t first, uint32_t count,
-                                                                       const float4 (&r)[kFetch]) {
-            const uint32_t n = count < 2u ? count : 2u;
-#pragma unroll
-            for (uint32_t k = 0; k < 2u; ++k) {
-                if (k >= n) break;
-                const float4 a = r[3u * k], e1v = r[3u * k + 1u], e2v = r[3u * k + 2u];
-                if (COUNT) tc.tris++;
-                const f3 v0 = xyz(a), e1 = xyz(e1v), e2 = xyz(e2v);
-                const f3 pv = cross(d, e2);
-                const float det = dot(e1, pv);
-                if (fabsf(det) < 1e-8f) continue;
-                const float inv_det = 1.0f / det;
-                const f3 tv = o - v0;
-                const float u = dot(tv, pv) * inv_det;
-                if (u < 0.0f || u > 1.0f) continue;
-                const f3 qv = cross(tv, e1);
-                const float v = dot(d, qv) * inv_det;
-                if (v < 0.0f || u + v > 1.0f) continue;
-                const float t = dot(e2, qv) * inv_det;
-                const uint32_t id = __float_as_uint(a.w);
-                if (t > 1e-3f && (t < tm || (t == tm && id < tri_id))) {
-                    tm = t; tri_id = id; tri_pos = first + k;
-                }
-            }
-            return n;
-        }, tc);
-        if (tri_pos != kMiss) {
-            best = tm;
-            code = kMeshBit | tri_pos;
         }
     }
-    return HitRec{best, code};
+    return n;
+}
+
+// Up to two triangles of a leaf held in the fetch (two-sided Moller-Trumbore,
+// |det| >= 1e-8, 1e-3 < t); returns how many it tested.
+template <bool COUNT>
+__device__ __forceinline__ uint32_t tri_leaf(f3 o, f3 d, uint32_t first, uint32_t count, const float4 (&r)[kFetch],
+                                             TriBest& b, TraceCounters& tc) {
+    const uint32_t n = count < 2u ? count : 2u;
+#pragma unroll
+    for (uint32_t k = 0; k < 2u; ++k) {
+        if (k >= n) break;
+        const float4 a = r[3u * k], e1v = r[3u * k + 1u], e2v = r[3u * k + 2u];
+        if (COUNT) tc.tris++;
+        const f3 v0 = xyz(a), e1 = xyz(e1v), e2 = xyz(e2v);
+        const f3 pv = cross(d, e2);
+        const float det = dot(e1, pv);
+        if (fabsf(det) < 1e-8f) continue;
+        const float inv_det = 1.0f / det;
+        const f3 tv = o - v0;
+        const float u = dot(tv, pv) * inv_det;
+        if (u < 0.0f || u > 1.0f) continue;
+        const f3 qv = cross(tv, e1);
+        const float v = dot(d, qv) * inv_det;
+        if (v < 0.0f || u + v > 1.0f) continue;
+        const float t = dot(e2, qv) * inv_det;
+        const uint32_t id = __float_as_uint(a.w);
+        if (t > 1e-3f && (t < b.t || (t == b.t && id < b.id))) {
+            b.t = t; b.id = id; b.pos = first + k;
+        }
+    }
+    return n;
+}
+
+__device__ __forceinline__ HitRec final_hit(const AnalyticBest& a, const TriBest& t) {
+    return (t.pos != kMiss) ? HitRec{t.t, kMeshBit | t.pos} : HitRec{a.t, a.code};
+}
+
+// traceScene (PathTrace.slang:2331-2693): planes, then the sphere BVH, then
+// the triangle BVH limited to the analytic hit distance.  Ties resolve to
+// the lower primitive index; analytic hits win ties against triangles.
+template <bool COUNT>
+__device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, Stack& st, TraceCounters& tc) {
+    AnalyticBest ab = planes_closest(S, o, d);
+    const f3 inv = safe_inverse(d);
+    if (S.sph_nodes) {
+        bvh4_walk<COUNT, 1u>(S.sph_nodes, S.spheres, o, inv, ab.t, st, [&](uint32_t first, uint32_t count,
+                                                                          const float4 (&r)[kFetch]) {
+            return sphere_leaf<COUNT>(S, o, d, first, count, r, ab, tc);
+        }, tc);
+    }
+    TriBest tb{ab.t, 0u, kMiss};
+    if (S.tri_nodes) {
+        bvh4_walk<COUNT, 3u>(S.tri_nodes, S.tris, o, inv, tb.t, st, [&](uint32_t first, uint32_t count,
+                                                                         const float4 (&r)[kFetch]) {
+            return tri_leaf<COUNT>(o, d, first, count, r, tb, tc);
+        }, tc);
+    }
+    return final_hit(ab, tb);
+}
+
+// Closest hits of the rays 0 .. n-1 of a queue, every lane on its own ray,
+// rays handed out dynamically (persistent traversal with dynamic ray fetch,
+// Aila and Laine 2009): each lane starts on ray `first` (its thread index;
+// rays at or beyond n are none); once kRefillLanes lanes of the wave are
+// idle, they take the next rays from next_base + atomicAdd(*counter, ...),
+// so *counter must read 0 at launch.  A persistent grid of a few waves per
+// SIMD then balances the rays' uneven lengths across the chip.  The ray's
+// two walks (sphere BVH, then triangle BVH) are one loop with a per-lane
+// phase; every step visits one node or one leaf fetch.  Hits are identical
+// to trace_closest's (the same tests and tie rules).
+template <bool COUNT, class LoadRay, class StoreHit>
+__device__ __forceinline__ void trace_queue(const SceneView& S, uint32_t first, uint32_t next_base, uint32_t n,
+                                            uint32_t* counter, Stack& st, TraceCounters& tc, LoadRay&& load_ray,
+                                            StoreHit&& store_hit) {
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t first_phase = S.sph_nodes ? 0u : (S.tri_nodes ? 1u : 2u);
+    uint32_t ray = kMiss, phase = 2u, cur = 0u, sp = 0u;
+    f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f}, inv{0.0f, 0.0f, 0.0f};
+    AnalyticBest ab{1e30f, 0xFFFFFFFFu, kMiss};
+    TriBest tb{1e30f, 0u, kMiss};
+    // Starts ray i in this lane; a scene without hierarchies is done at once.
+    auto begin = [&](uint32_t i) {
+        load_ray(i, o, d);
+        inv = safe_inverse(d);
+        ab = planes_closest(S, o, d);
+        tb = TriBest{ab.t, 0u, kMiss};
+        phase = first_phase;
+        cur = 0u;
+        sp = 0u;
+        ray = i;
+        if (phase == 2u) {
+            store_hit(i, final_hit(ab, tb));
+            ray = kMiss;
+        }
+    };
+    if (first < n) begin(first);
+    bool drained = next_base >= n;
+    for (;;) {
+        const unsigned long long idle = __ballot(ray == kMiss);
+        if (!drained && (uint32_t)__popcll(idle) >= kRefillLanes) {
+            const uint32_t want = (uint32_t)__popcll(idle);
+            uint32_t base = 0u;
+            if (idle & (1ull << lane) ? (idle & below) == 0ull : false) base = atomicAdd(counter, want);
+            base = next_base + __shfl(base, __ffsll((long long)idle) - 1);
+            if (ray == kMiss) {
+                const uint32_t i = base + (uint32_t)__popcll(idle & below);
+                if (i < n) begin(i);
+            }
+            if (base + want >= n) drained = true;
+        }
+        if (__ballot(ray != kMiss) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        if (ray == kMiss) continue;
+        // one step of the lane's current walk
+        const bool tri = phase == 1u;
+        const bool at_leaf = (cur & kLeaf) != 0u;
+        const uint32_t lf = cur & kLeafFirst;
+        const float4* p = at_leaf ? (tri ? S.tris + (size_t)lf * 3u : S.spheres + lf)
+                                  : (tri ? S.tri_nodes : S.sph_nodes) + (size_t)cur * 8u;
+        float4 r[kFetch];
+#pragma unroll
+        for (uint32_t k = 0; k < kFetch; ++k) r[k] = p[k];
+#pragma unroll
+        for (uint32_t k = 0; k < kFetch; ++k) asm volatile("" ::"v"(r[k].x), "v"(r[k].y), "v"(r[k].z), "v"(r[k].w));
+        bool pop = true;
+        if (at_leaf) {
+            const uint32_t count = ((cur >> 27) & 15u) + 1u;
+            const uint32_t used = tri ? tri_leaf<COUNT>(o, d, lf, count, r, tb, tc)
+                                      : sphere_leaf<COUNT>(S, o, d, lf, count, r, ab, tc);
+            if (used < count) {
+                cur = kLeaf | ((count - used - 1u) << 27) | (lf + used);
+                pop = false;
+            }
+        } else {
+            if (COUNT) tc.nodes++;
+            float tn[4];
+            uint32_t ch[4];
+            const uint32_t m = node4_test(r, o, inv, tri ? tb.t : ab.t, tn, ch);
+            if (m) {
+                cswap(tn[0], ch[0], tn[1], ch[1]);
+                cswap(tn[2], ch[2], tn[3], ch[3]);
+                cswap(tn[0], ch[0], tn[2], ch[2]);
+                cswap(tn[1], ch[1], tn[3], ch[3]);
+                cswap(tn[1], ch[1], tn[2], ch[2]);
+                const uint32_t nh = __popc(m);
+                if (nh > 3u) st.put(sp++, ch[3]);
+                if (nh > 2u) st.put(sp++, ch[2]);
+                if (nh > 1u) st.put(sp++, ch[1]);
+                cur = ch[0];
+                pop = false;
+            }
+        }
+        if (pop) {
+            if (sp > 0u) {
+                cur = st.get(--sp);
+            } else if (phase == 0u && S.tri_nodes) {   // the sphere walk is done
+                tb = TriBest{ab.t, 0u, kMiss};
+                phase = 1u;
+                cur = 0u;
+            } else {
+                store_hit(ray, final_hit(ab, tb));
+                ray = kMiss;
+            }
+        }
+    }
 }
 
 // Surface attributes at a hit (the HitInfo fill of testAnalyticPrim,

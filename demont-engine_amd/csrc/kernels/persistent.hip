@@ -51,7 +51,8 @@ struct Lane {
 
 struct Cold {
     float* base;          // this thread's column: field f at base[f * kBlock]
-    enum : uint32_t { kSum = 0, kX = 3, kY = 4, kS = 5, kTr = 6, kTmax = 9, kStep = 10, kNo = 11, kNd = 14, kRows = 17 };
+    // kXY: x | y << 16 (frames are at most 65535 pixels wide and high)
+    enum : uint32_t { kSum = 0, kXY = 3, kS = 4, kTr = 5, kTmax = 8, kStep = 9, kNo = 10, kNd = 13, kRows = 16 };
     __device__ __forceinline__ float get(uint32_t f) const { return base[f * kBlock]; }
     __device__ __forceinline__ void set(uint32_t f, float v) { base[f * kBlock] = v; }
     __device__ __forceinline__ uint32_t getu(uint32_t f) const { return __float_as_uint(get(f)); }
@@ -115,7 +116,7 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const float4* rec, u
     // the sample is done
     const uint32_t s = c.getu(Cold::kS);
     const f3 sum = (s == 0u) ? l.P.L : c.get3(Cold::kSum) + l.P.L;
-    const uint32_t x = c.getu(Cold::kX), y = c.getu(Cold::kY);
+    const uint32_t xy = c.getu(Cold::kXY), x = xy & 0xFFFFu, y = xy >> 16;
     if (s + 1u < F.spp) {
         c.set3(Cold::kSum, sum);
         c.setu(Cold::kS, s + 1u);
@@ -128,8 +129,13 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const float4* rec, u
     l.active = false;
 }
 
+// Workgroups per CU that the LDS of one (stack + cold column) allows, and
+// the occupancy the register budget is set for.
+template <uint32_t DEPTH> constexpr uint32_t persistent_waves() { return (160u * 1024u) / ((DEPTH + Cold::kRows) * kBlock * 4u); }
+
 template <uint32_t DEPTH>
-__global__ void __launch_bounds__(kBlock) k_persistent(BatchArgs A) {
+__global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<DEPTH>()))) k_persistent(BatchArgs A) {
     __shared__ uint32_t lds[DEPTH * kBlock];
     __shared__ float cold[Cold::kRows * kBlock];
     Cold c{cold + threadIdx.x};
@@ -171,8 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_persistent(BatchArgs A) {
                     const uint32_t y = (tile / tiles_x) * (64u >> tsh) + (k >> tsh);
                     if (x < F.width && y < F.height) {
                         l.active = true;
-                        c.setu(Cold::kX, x);
-                        c.setu(Cold::kY, y);
+                        c.setu(Cold::kXY, x | (y << 16));
                         c.setu(Cold::kS, 0u);
                         l.P.seed = pixel_seed(F, x, y, frame_index);
                         start_sample(F, l, x, y);

@@ -30,6 +30,7 @@ constexpr uint32_t kMaxGridBlocks = 4096; // grid-stride cap: 1M threads
 constexpr uint32_t kCtrRays = 0;          // + parity: ray queue lengths
 constexpr uint32_t kCtrFinished = 2;
 constexpr uint32_t kCtrTail = 3;          // k_tail: queue entries handed out
+constexpr uint32_t kCtrExtend = 4;        // + parity: k_extend rays handed out beyond the first grid pass
 constexpr uint32_t kCounterSlots = 8;
 
 // Shadow records live at the slot of the path that formed them (no
@@ -103,6 +104,7 @@ FrameArgs make_frame_args(const float pos_fovtan[4], const float fwd_aspect[4], 
 uint32_t grid_for(uint32_t n);
 
 constexpr uint32_t kNoTail = 0xFFFFFFFFu;   // every iteration by the per-bounce kernels
+constexpr uint32_t kTailWavesPerSimd = 5;    // k_tail occupancy target (and workgroups per CU)
 constexpr uint32_t kPollSlots = 4;
 constexpr uint32_t kPollChunk = 4;        // iterations between live-queue snapshots
 
@@ -117,6 +119,7 @@ struct LaunchOptions {
     uint32_t iterations = 0;              // out: iterations launched
     uint32_t tail_after = kNoTail;        // iterations before k_tail finishes the frame
     uint32_t tail_blocks = 0;             // k_tail grid (0: grid_for(pixels))
+    uint32_t extend_blocks = 0;           // k_extend grid cap (0: grid_for(pixels))
 };
 
 // Render one frame into accum; the final pass that folds the frame's values

@@ -117,16 +117,22 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     if (gid == 0) {
-        ctr[kCtrRays + (in_par ^ 1u)] = 0u;   // the queue this iteration fills
+        ctr[kCtrRays + (in_par ^ 1u)] = 0u;     // the queue this iteration fills
+        ctr[kCtrExtend + (in_par ^ 1u)] = 0u;   // the next iteration's ray counter
         if (COUNT) atomicAdd(&st->ext_rays, (unsigned long long)n);
     }
     Stack stk = make_stack(lds, spill, spill_depth);
     TraceCounters tc{0u, 0u, 0u};
-    for (uint32_t i = gid; i < n; i += stride) {
-        const float4 o = q.o[i], d = q.d[i];
-        HitRec h = trace_closest<COUNT>(S, xyz(o), xyz(d), stk, tc);
-        hits[i] = make_uint2(__float_as_uint(h.t), h.code);
-    }
+    // lane i starts on ray i (slots are in 8x8 tile order: a wave starts on
+    // one tile), then takes rays stride, stride + 1, ... as its rays finish
+    trace_queue<COUNT>(
+        S, gid, stride, n, &ctr[kCtrExtend + in_par], stk, tc,
+        [&](uint32_t i, f3& o, f3& d) {
+            const float4 a = q.o[i], b = q.d[i];
+            o = xyz(a);
+            d = xyz(b);
+        },
+        [&](uint32_t i, HitRec h) { hits[i] = make_uint2(__float_as_uint(h.t), h.code); });
     if (COUNT) {
         add_stat(&st->ext_nodes, tc.nodes);
         add_stat(&st->ext_tris, tc.tris);
@@ -336,8 +342,11 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const ShadowQue
 
 }  // namespace
 
+// Five waves per SIMD (the LDS of a workgroup, stack and cold fields, allows
+// five per CU): 96 VGPRs, a few spilled values outside the traversal loop.
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerSimd, kTailWavesPerSimd)))
+k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
                                                  uint32_t* ctr, PathState ps, ShadowQueue sl, ShadowQueue se,
                                                  uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
@@ -566,7 +575,8 @@ static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W
         o.event_kinds[o.events_used / 2u] = 0u;
         (void)hipEventRecord(o.extend_events[o.events_used], stream);
     }
-    hipLaunchKernelGGL(k_extend<COUNT>, dim3(g), dim3(kBlock), 0, stream, S, W.rq[par], W.counters, par, W.hits,
+    const uint32_t ge = o.extend_blocks ? (o.extend_blocks < g ? o.extend_blocks : g) : g;
+    hipLaunchKernelGGL(k_extend<COUNT>, dim3(ge), dim3(kBlock), 0, stream, S, W.rq[par], W.counters, par, W.hits,
                        W.spill, W.spill_depth, W.stats);
     if (ev) {
         (void)hipEventRecord(o.extend_events[o.events_used + 1u], stream);

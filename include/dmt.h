@@ -250,7 +250,8 @@ typedef struct dmt_batch_params {
                                    8, 16, 32 or 64 (0 = 8)                    */
     uint32_t tiles_per_fetch;   /* tiles a wave takes per queue atomic, 1..16 (0 = 1) */
     uint32_t blocks_per_cu;     /* persistent workgroups of 256 threads per CU,
-                                   1..4 (0 = 4)                               */
+                                   1..5 (0 = 4); stack_lds 16 holds 5 per CU,
+                                   24 holds 4, 32 holds 3                     */
     uint32_t stack_lds;         /* LDS traversal-stack entries per thread:
                                    16, 24 or 32 (0 = 16)                      */
     uint32_t plain_launches;    /* 0 = replay a captured hipGraph per frame

@@ -154,7 +154,7 @@ def test_autotune_keeps_images_and_picks_valid_params(device, cases, reference):
     p, ms = b.autotune(frames_per_trial=1)
     print(f"autotuned {p.as_dict()} at {ms:.3f} ms/frame")
     assert p.tile_w in (8, 16, 32, 64) and p.stack_lds in (16, 24, 32)
-    assert 1 <= p.blocks_per_cu <= 4 and 1 <= p.tiles_per_fetch <= 16 and ms > 0.0
+    assert 1 <= p.blocks_per_cu <= 5 and 1 <= p.tiles_per_fetch <= 16 and ms > 0.0
     assert b.params().as_dict() == p.as_dict()
     after = [sc.readback(f.width, f.height) for sc, f in zip(scenes, frames)]
     for x, y in zip(before, after):
@@ -174,7 +174,7 @@ def test_batch_error_paths(device, cases):
     with pytest.raises(dmt.DmtError) as e:
         dmt.Batch(device, [a, a], [f, f])
     assert e.value.status == dmt.DMT_ERR_INVALID_ARG
-    for bad in (dict(tile_w=12), dict(stack_lds=20), dict(blocks_per_cu=5), dict(tiles_per_fetch=17)):
+    for bad in (dict(tile_w=12), dict(stack_lds=20), dict(blocks_per_cu=6), dict(tiles_per_fetch=17)):
         with pytest.raises(dmt.DmtError) as e:
             dmt.Batch(device, [a, c], [f, f2], dmt.BatchParams(**bad))
         assert e.value.status == dmt.DMT_ERR_INVALID_ARG
