@@ -481,39 +481,105 @@ __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, 
     return final_hit(ab, tb);
 }
 
+// A closest-hit query in progress: trace_closest's walk state, kept across
+// loop steps so that a lane advances its query one node or leaf at a time
+// (the sphere BVH walk, then the triangle BVH walk, as one loop with a
+// phase).  The tests and tie rules are trace_closest's, so the hit is too.
+struct Walk {
+    f3 inv;
+    AnalyticBest ab;
+    TriBest tb;
+    uint32_t phase;   // 0 sphere BVH, 1 triangle BVH
+    uint32_t cur, sp;
+};
+
+// Starts the query (o, d); returns true when it is already done (no
+// hierarchies: the planes decide).
+__device__ __forceinline__ bool walk_begin(const SceneView& S, f3 o, f3 d, Walk& w) {
+    w.inv = safe_inverse(d);
+    w.ab = planes_closest(S, o, d);
+    w.tb = TriBest{w.ab.t, 0u, kMiss};
+    w.phase = S.sph_nodes ? 0u : 1u;
+    w.cur = 0u;
+    w.sp = 0u;
+    return !S.sph_nodes && !S.tri_nodes;
+}
+
+// One step of the query: a node or one leaf fetch.  Returns true when the
+// query is done; its hit is then final_hit(w.ab, w.tb).
+template <bool COUNT>
+__device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& w, Stack& st, TraceCounters& tc) {
+    const bool tri = w.phase == 1u;
+    const bool at_leaf = (w.cur & kLeaf) != 0u;
+    const uint32_t lf = w.cur & kLeafFirst;
+    const float4* p = at_leaf ? (tri ? S.tris + (size_t)lf * 3u : S.spheres + lf)
+                              : (tri ? S.tri_nodes : S.sph_nodes) + (size_t)w.cur * 8u;
+    float4 r[kFetch];
+#pragma unroll
+    for (uint32_t k = 0; k < kFetch; ++k) r[k] = p[k];
+#pragma unroll
+    for (uint32_t k = 0; k < kFetch; ++k) asm volatile("" ::"v"(r[k].x), "v"(r[k].y), "v"(r[k].z), "v"(r[k].w));
+    if (at_leaf) {
+        const uint32_t count = ((w.cur >> 27) & 15u) + 1u;
+        const uint32_t used = tri ? tri_leaf<COUNT>(o, d, lf, count, r, w.tb, tc)
+                                  : sphere_leaf<COUNT>(S, o, d, lf, count, r, w.ab, tc);
+        if (used < count) {
+            w.cur = kLeaf | ((count - used - 1u) << 27) | (lf + used);
+            return false;
+        }
+    } else {
+        if (COUNT) tc.nodes++;
+        float tn[4];
+        uint32_t ch[4];
+        const uint32_t m = node4_test(r, o, w.inv, tri ? w.tb.t : w.ab.t, tn, ch);
+        if (m) {
+            cswap(tn[0], ch[0], tn[1], ch[1]);
+            cswap(tn[2], ch[2], tn[3], ch[3]);
+            cswap(tn[0], ch[0], tn[2], ch[2]);
+            cswap(tn[1], ch[1], tn[3], ch[3]);
+            cswap(tn[1], ch[1], tn[2], ch[2]);
+            const uint32_t nh = __popc(m);
+            if (nh > 3u) st.put(w.sp++, ch[3]);
+            if (nh > 2u) st.put(w.sp++, ch[2]);
+            if (nh > 1u) st.put(w.sp++, ch[1]);
+            w.cur = ch[0];
+            return false;
+        }
+    }
+    if (w.sp > 0u) {
+        w.cur = st.get(--w.sp);
+        return false;
+    }
+    if (w.phase == 0u && S.tri_nodes) {   // the sphere walk is done
+        w.tb = TriBest{w.ab.t, 0u, kMiss};
+        w.phase = 1u;
+        w.cur = 0u;
+        return false;
+    }
+    return true;
+}
+
 // Closest hits of the rays 0 .. n-1 of a queue, every lane on its own ray,
 // rays handed out dynamically (persistent traversal with dynamic ray fetch,
 // Aila and Laine 2009): each lane starts on ray `first` (its thread index;
 // rays at or beyond n are none); once kRefillLanes lanes of the wave are
 // idle, they take the next rays from next_base + atomicAdd(*counter, ...),
 // so *counter must read 0 at launch.  A persistent grid of a few waves per
-// SIMD then balances the rays' uneven lengths across the chip.  The ray's
-// two walks (sphere BVH, then triangle BVH) are one loop with a per-lane
-// phase; every step visits one node or one leaf fetch.  Hits are identical
-// to trace_closest's (the same tests and tie rules).
+// SIMD then balances the rays' uneven lengths across the chip.
 template <bool COUNT, class LoadRay, class StoreHit>
 __device__ __forceinline__ void trace_queue(const SceneView& S, uint32_t first, uint32_t next_base, uint32_t n,
                                             uint32_t* counter, Stack& st, TraceCounters& tc, LoadRay&& load_ray,
                                             StoreHit&& store_hit) {
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
-    const uint32_t first_phase = S.sph_nodes ? 0u : (S.tri_nodes ? 1u : 2u);
-    uint32_t ray = kMiss, phase = 2u, cur = 0u, sp = 0u;
-    f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f}, inv{0.0f, 0.0f, 0.0f};
-    AnalyticBest ab{1e30f, 0xFFFFFFFFu, kMiss};
-    TriBest tb{1e30f, 0u, kMiss};
-    // Starts ray i in this lane; a scene without hierarchies is done at once.
+    uint32_t ray = kMiss;
+    f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
+    Walk w{};
     auto begin = [&](uint32_t i) {
         load_ray(i, o, d);
-        inv = safe_inverse(d);
-        ab = planes_closest(S, o, d);
-        tb = TriBest{ab.t, 0u, kMiss};
-        phase = first_phase;
-        cur = 0u;
-        sp = 0u;
         ray = i;
-        if (phase == 2u) {
-            store_hit(i, final_hit(ab, tb));
+        if (walk_begin(S, o, d, w)) {
+            store_hit(i, final_hit(w.ab, w.tb));
             ray = kMiss;
         }
     };
@@ -537,56 +603,9 @@ __device__ __forceinline__ void trace_queue(const SceneView& S, uint32_t first, 
             continue;
         }
         if (ray == kMiss) continue;
-        // one step of the lane's current walk
-        const bool tri = phase == 1u;
-        const bool at_leaf = (cur & kLeaf) != 0u;
-        const uint32_t lf = cur & kLeafFirst;
-        const float4* p = at_leaf ? (tri ? S.tris + (size_t)lf * 3u : S.spheres + lf)
-                                  : (tri ? S.tri_nodes : S.sph_nodes) + (size_t)cur * 8u;
-        float4 r[kFetch];
-#pragma unroll
-        for (uint32_t k = 0; k < kFetch; ++k) r[k] = p[k];
-#pragma unroll
-        for (uint32_t k = 0; k < kFetch; ++k) asm volatile("" ::"v"(r[k].x), "v"(r[k].y), "v"(r[k].z), "v"(r[k].w));
-        bool pop = true;
-        if (at_leaf) {
-            const uint32_t count = ((cur >> 27) & 15u) + 1u;
-            const uint32_t used = tri ? tri_leaf<COUNT>(o, d, lf, count, r, tb, tc)
-                                      : sphere_leaf<COUNT>(S, o, d, lf, count, r, ab, tc);
-            if (used < count) {
-                cur = kLeaf | ((count - used - 1u) << 27) | (lf + used);
-                pop = false;
-            }
-        } else {
-            if (COUNT) tc.nodes++;
-            float tn[4];
-            uint32_t ch[4];
-            const uint32_t m = node4_test(r, o, inv, tri ? tb.t : ab.t, tn, ch);
-            if (m) {
-                cswap(tn[0], ch[0], tn[1], ch[1]);
-                cswap(tn[2], ch[2], tn[3], ch[3]);
-                cswap(tn[0], ch[0], tn[2], ch[2]);
-                cswap(tn[1], ch[1], tn[3], ch[3]);
-                cswap(tn[1], ch[1], tn[2], ch[2]);
-                const uint32_t nh = __popc(m);
-                if (nh > 3u) st.put(sp++, ch[3]);
-                if (nh > 2u) st.put(sp++, ch[2]);
-                if (nh > 1u) st.put(sp++, ch[1]);
-                cur = ch[0];
-                pop = false;
-            }
-        }
-        if (pop) {
-            if (sp > 0u) {
-                cur = st.get(--sp);
-            } else if (phase == 0u && S.tri_nodes) {   // the sphere walk is done
-                tb = TriBest{ab.t, 0u, kMiss};
-                phase = 1u;
-                cur = 0u;
-            } else {
-                store_hit(ray, final_hit(ab, tb));
-                ray = kMiss;
-            }
+        if (walk_step<COUNT>(S, o, d, w, st, tc)) {
+            store_hit(ray, final_hit(w.ab, w.tb));
+            ray = kMiss;
         }
     }
 }

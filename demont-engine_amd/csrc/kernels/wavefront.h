@@ -104,7 +104,10 @@ FrameArgs make_frame_args(const float pos_fovtan[4], const float fwd_aspect[4], 
 uint32_t grid_for(uint32_t n);
 
 constexpr uint32_t kNoTail = 0xFFFFFFFFu;   // every iteration by the per-bounce kernels
-constexpr uint32_t kTailWavesPerSimd = 5;    // k_tail occupancy target (and workgroups per CU)
+#ifndef DMT_TAIL_WAVES
+#define DMT_TAIL_WAVES 5
+#endif
+constexpr uint32_t kTailWavesPerSimd = DMT_TAIL_WAVES;   // k_tail occupancy target (and workgroups per CU)
 constexpr uint32_t kPollSlots = 4;
 constexpr uint32_t kPollChunk = 4;        // iterations between live-queue snapshots
 

@@ -257,6 +257,20 @@ namespace {
 
 constexpr uint32_t kEnded = 16u;   // pending bit: the path ends after this vertex
 
+// k_tail lane states
+constexpr uint32_t kLaneIdle = 0u;    // no path
+constexpr uint32_t kLaneWalk = 1u;    // a closest-hit query in progress
+constexpr uint32_t kLaneReady = 2u;   // the query is done; its hit waits for the wave's shading step
+
+#ifndef DMT_TAIL_SHADE_LANES
+#define DMT_TAIL_SHADE_LANES 32u
+#endif
+#ifndef DMT_TAIL_REFILL_LANES
+#define DMT_TAIL_REFILL_LANES 16u
+#endif
+constexpr uint32_t kTailShadeLanes = DMT_TAIL_SHADE_LANES;    // shade once this many lanes are ready
+constexpr uint32_t kTailRefillLanes = DMT_TAIL_REFILL_LANES;  // take new paths once this many are idle
+
 struct TailLane {
     bool active;
     uint32_t mode;        // 0 extension ray, 1 light-NEE shadow ray, 2 env-NEE shadow ray
@@ -361,16 +375,26 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
     bool drained = false;
     TailLane l{};
     l.active = false;
+    // Each lane advances its query one node or leaf per loop step (Walk);
+    // a lane whose query is done waits (kLaneReady) until enough of the wave
+    // is done too, and the wave then shades those lanes together.  So the
+    // traversal steps keep most lanes busy instead of waiting, query by
+    // query, for the wave's longest ray, and the shading code runs for many
+    // lanes at once.
+    Walk w{};
+    HitRec h{0.0f, kMiss};
+    uint32_t state = kLaneIdle;
     for (;;) {
         // hand queue entries to idle lanes (wave-uniform)
-        const unsigned long long idle = __ballot(!l.active);
-        if (idle != 0ull && !drained) {
+        const unsigned long long idle = __ballot(state == kLaneIdle);
+        const unsigned long long walking = __ballot(state == kLaneWalk);
+        if (idle != 0ull && !drained && ((uint32_t)__popcll(idle) >= kTailRefillLanes || walking == 0ull)) {
             const uint32_t want = (uint32_t)__popcll(idle);
             uint32_t base = 0u;
-            if (lane == 0u) base = atomicAdd(&ctr[kCtrTail], want);
-            base = __shfl(base, 0);
+            if (idle & (1ull << lane) ? (idle & below) == 0ull : false) base = atomicAdd(&ctr[kCtrTail], want);
+            base = __shfl(base, __ffsll((long long)idle) - 1);
             const uint32_t i = base + (uint32_t)__popcll(idle & below);
-            if (!l.active && i < n) {
+            if (state == kLaneIdle && i < n) {
                 const float4 o4 = q.o[i], d4 = q.d[i];
                 l.pix = __float_as_uint(d4.w);
                 l.slot = i;
@@ -386,47 +410,74 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
                 if (s != 0u) c.set3(TailCold::kSum, xyz(ps.F[l.pix]));
                 l.mode = 0u;
                 l.active = true;
+                state = kLaneWalk;
+                if (walk_begin(S, l.o, l.d, w)) {
+                    h = final_hit(w.ab, w.tb);
+                    state = kLaneReady;
+                }
             }
             if (base + want >= n) drained = true;
         }
-        if (__ballot(l.active) == 0ull) break;
-        if (!l.active) continue;
-        const HitRec h = trace_closest<COUNT>(S, l.o, l.d, stk, tc);
-        if (l.mode == 0u) {
-            if (l.b == 0u) n_primary++;
-            else n_ext++;
-            SlotSink sink{{}, sl, se, l.slot, l.pix, {false, false}};
-            VertexResult R;
-            shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
-            const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
-            l.pending = (sink.emitted[0] ? 1u : 0u) | (sink.emitted[1] ? 2u : 0u) | (cont ? 0u : kEnded);
-            if (cont) {
-                c.set3(TailCold::kNo, R.no);
-                c.set3(TailCold::kNd, R.nd);
-            }
-            tail_advance(F, sl, se, ps, l, c);
-        } else {
-            n_shadow++;
-            float t_max = c.get(TailCold::kTmax);
-            f3 tr = c.get3(TailCold::kTr);
-            const uint32_t step = c.getu(TailCold::kStep);
-            if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr)) {
-                const uint32_t k = l.mode - 1u;
-                if (any_pos(tr)) {
-                    // fields picked one by one: a selected reference to either
-                    // queue argument would put both in private memory
-                    const float4 a4 = (k == 0u ? sl.a : se.a)[l.slot], b4 = (k == 0u ? sl.b : se.b)[l.slot];
-                    const uint32_t tag = __float_as_uint((k == 0u ? sl.d : se.d)[l.slot].w);
-                    l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w,
-                                                 k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
+        const unsigned long long ready = __ballot(state == kLaneReady);
+        const unsigned long long walk = __ballot(state == kLaneWalk);
+        if (ready == 0ull && walk == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        // the finished queries of the wave, together: the vertex, or one step
+        // of a shadow query; then the lane's next query
+        if (ready != 0ull && ((uint32_t)__popcll(ready) >= kTailShadeLanes || walk == 0ull) &&
+            state == kLaneReady) {
+            if (l.mode == 0u) {
+                if (l.b == 0u) n_primary++;
+                else n_ext++;
+                SlotSink sink{{}, sl, se, l.slot, l.pix, {false, false}};
+                VertexResult R;
+                shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
+                const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
+                l.pending = (sink.emitted[0] ? 1u : 0u) | (sink.emitted[1] ? 2u : 0u) | (cont ? 0u : kEnded);
+                if (cont) {
+                    c.set3(TailCold::kNo, R.no);
+                    c.set3(TailCold::kNd, R.nd);
                 }
-                l.pending &= ~(1u << k);
                 tail_advance(F, sl, se, ps, l, c);
-            } else {   // transmit_step ends every query by its step refract_bounces
-                c.set(TailCold::kTmax, t_max);
-                c.set3(TailCold::kTr, tr);
-                c.setu(TailCold::kStep, step + 1u);
+            } else {
+                n_shadow++;
+                float t_max = c.get(TailCold::kTmax);
+                f3 tr = c.get3(TailCold::kTr);
+                const uint32_t step = c.getu(TailCold::kStep);
+                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr)) {
+                    const uint32_t k = l.mode - 1u;
+                    if (any_pos(tr)) {
+                        // fields picked one by one: a selected reference to either
+                        // queue argument would put both in private memory
+                        const float4 a4 = (k == 0u ? sl.a : se.a)[l.slot], b4 = (k == 0u ? sl.b : se.b)[l.slot];
+                        const uint32_t tag = __float_as_uint((k == 0u ? sl.d : se.d)[l.slot].w);
+                        l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w,
+                                                     k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
+                    }
+                    l.pending &= ~(1u << k);
+                    tail_advance(F, sl, se, ps, l, c);
+                } else {   // transmit_step ends every query by its step refract_bounces
+                    c.set(TailCold::kTmax, t_max);
+                    c.set3(TailCold::kTr, tr);
+                    c.setu(TailCold::kStep, step + 1u);
+                }
             }
+            if (l.active) {
+                state = kLaneWalk;
+                if (walk_begin(S, l.o, l.d, w)) {
+                    h = final_hit(w.ab, w.tb);
+                    state = kLaneReady;
+                }
+            } else {
+                state = kLaneIdle;
+            }
+        }
+        // one traversal step of every lane with a query in progress
+        if (state == kLaneWalk && walk_step<COUNT>(S, l.o, l.d, w, stk, tc)) {
+            h = final_hit(w.ab, w.tb);
+            state = kLaneReady;
         }
     }
     add_stat(&st->primary, n_primary);
