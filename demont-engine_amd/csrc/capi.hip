@@ -23,9 +23,10 @@
 using namespace dmt;
 
 // Frames of per-frame dmt_render calls kept in flight (one stream each).
-// Default frame schedule: the camera rays' bounce by the per-bounce kernels
-// (k_extend, k_shade, ...), the rest of the frame by one k_tail launch.
-constexpr uint32_t kDefaultTailAfter = 1;
+// Default frame schedule: the whole frame, camera rays included, by one
+// k_tail launch (measured faster than the camera bounce by the per-bounce
+// kernels on both the 1080p mesh and the many-light scene).
+constexpr uint32_t kDefaultTailAfter = 0;
 constexpr uint32_t kTailBlocksPerCu = kTailWavesPerSimd;   // k_tail grid: resident workgroups per CU
 constexpr uint32_t kExtendBlocksPerCu = 8;   // k_extend grid: 8 waves per SIMD, rays handed out dynamically
 #ifndef DMT_FRAME_SETS

@@ -8,6 +8,13 @@ namespace {
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kEnded = 16u;          // pending bit: the path ends after this vertex
 
+// lane states
+constexpr uint32_t kLaneIdle = 0u;    // no path
+constexpr uint32_t kLaneWalk = 1u;    // a closest-hit query in progress
+constexpr uint32_t kLaneReady = 2u;   // the query is done; its hit waits for the wave's shading step
+constexpr uint32_t kPersistShadeLanes = 32u;    // shade once this many lanes are ready
+constexpr uint32_t kPersistRefillLanes = 16u;   // take pixels once this many lanes are idle
+
 // Shadow records of one vertex go to this thread's scratch rows; bit k of
 // `pending` marks record k, bit k + 2 its clampIndirect flag.
 struct ScratchSink : NoGradientHooks {
@@ -36,6 +43,22 @@ __device__ __forceinline__ void add_stat(unsigned long long* dst, uint32_t v) {
 }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// The scene table is constant while the kernel runs.  Read through the
+// constant address space, a wave-uniform scene's fields are scalar loads
+// that the compiler may issue again where they are used (like kernel
+// arguments) instead of holding them in registers across the loop.
+template <class T> using ConstPtr = __attribute__((address_space(4))) const T*;
+
+template <class T> __device__ __forceinline__ T const_copy(ConstPtr<T> src) {
+    static_assert(sizeof(T) % 4u == 0u, "dword-sized struct");
+    T out;
+    const ConstPtr<uint32_t> s = (ConstPtr<uint32_t>)src;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4u; ++i) d[i] = s[i];
+    return out;
+}
 
 // State of one lane's path.  The fields used at every query stay in
 // registers; the rest sits in an LDS column of the thread (`Cold`), so the
@@ -152,7 +175,8 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
     uint32_t qoff = 0u, qpos = A.queue_begin[xcd];
     bool open = false;                 // the current scene may have tiles left
     uint32_t sc = kNone, pool_lo = 0u, pool_hi = 0u;
-    const BatchScene* bs = A.scenes;
+    const ConstPtr<BatchScene> table = (ConstPtr<BatchScene>)A.scenes;
+    ConstPtr<BatchScene> bs = table;
     uint32_t frame_index = 0u;
     float4* accum = nullptr;
     uint32_t tiles_x = 1u, tiles = 0u;
@@ -161,15 +185,25 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
     Lane l{};
     l.active = false;
     const uint32_t tw = A.tile_w, tsh = A.tile_shift;
+    // Each lane advances its query one node or leaf per loop step (Walk); a
+    // lane whose query is done waits (kLaneReady) until enough of the wave is
+    // done too, and the wave then shades those lanes together.
+    Walk w{};
+    HitRec h{0.0f, kMiss};
+    uint32_t state = kLaneIdle;
     for (;;) {
         // ---- hand pixels to idle lanes (wave-uniform control flow) ----
         // The wave may switch scene (bs) in this loop, so the scene's state
         // is read through bs where it is used, never held across a switch.
-        for (;;) {
+        // It takes pixels once enough lanes are idle, or none is walking.
+        const bool take = (uint32_t)__popcll(__ballot(!l.active)) >= kPersistRefillLanes ||
+                          __ballot(state == kLaneWalk) == 0ull;
+        for (; take;) {
             const unsigned long long idle = __ballot(!l.active);
             if (idle == 0ull) break;
             if (pool_lo < pool_hi) {
-                const FrameArgs& F = bs->F;
+                const FrameArgs F = const_copy(&bs->F);
+                const SceneView S = const_copy(&bs->S);
                 const uint32_t ord = pool_lo + (uint32_t)__popcll(idle & below);
                 if (!l.active && ord < pool_hi) {
                     const uint32_t tile = ord >> 6, k = ord & 63u;
@@ -181,6 +215,11 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                         c.setu(Cold::kS, 0u);
                         l.P.seed = pixel_seed(F, x, y, frame_index);
                         start_sample(F, l, x, y);
+                        state = kLaneWalk;
+                        if (walk_begin(S, l.o, l.d, w)) {
+                            h = final_hit(w.ab, w.tb);
+                            state = kLaneReady;
+                        }
                     }
                 }
                 pool_lo = min(pool_lo + (uint32_t)__popcll(idle), pool_hi);
@@ -203,9 +242,9 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
             while (qoff < kXcds) {
                 const uint32_t q = (xcd + qoff) % kXcds;
                 if (qpos < A.queue_begin[q + 1u]) {
-                    const uint32_t cand = A.order[qpos++];
+                    const uint32_t cand = uniform(A.order[qpos++]);
                     const uint32_t handed = __hip_atomic_load(&A.work[cand], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (uniform(handed) < A.scenes[cand].tiles) {
+                    if (uniform(handed) < table[cand].tiles) {
                         sc = cand;
                         break;
                     }
@@ -215,7 +254,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                 }
             }
             if (sc == kNone) break;   // no work left anywhere
-            bs = A.scenes + sc;
+            bs = table + sc;
             frame_index = bs->F.frame_index + rel;
             reset = rel == 0u && bs->F.reset_accum != 0u;
             accum = bs->accum;
@@ -224,13 +263,19 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
             open = true;
             pool_lo = pool_hi = 0u;
         }
-        if (__ballot(l.active) == 0ull) break;
+        const unsigned long long ready = __ballot(state == kLaneReady);
+        const unsigned long long walk = __ballot(state == kLaneWalk);
+        if (ready == 0ull && walk == 0ull) {
+            if (__ballot(l.active) == 0ull) break;   // no work left anywhere
+            continue;
+        }
 
-        // ---- one closest-hit query per live lane ----
-        const SceneView& S = bs->S;
-        const FrameArgs& F = bs->F;
-        if (l.active) {
-            const HitRec h = trace_closest<false>(S, l.o, l.d, stk, tc);
+        // the wave's scene, read again each step (nothing held across steps)
+        const SceneView S = const_copy(&bs->S);
+        const FrameArgs F = const_copy(&bs->F);
+        // ---- the finished queries of the wave, together ----
+        if (ready != 0ull && ((uint32_t)__popcll(ready) >= kPersistShadeLanes || walk == 0ull) &&
+            state == kLaneReady) {
             if (l.mode == 0u) {
                 if (l.b == 0u) n_primary++;
                 else n_ext++;
@@ -251,9 +296,9 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         const float4* row = A.rec + ((size_t)k * kRecRows) * A.threads + tid;
-                        const float w = row[(size_t)A.threads].w;
+                        const float wt = row[(size_t)A.threads].w;
                         const float4 a4 = row[2 * (size_t)A.threads], b4 = row[3 * (size_t)A.threads];
-                        l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w, k == 1u ? w : 1.0f,
+                        l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w, k == 1u ? wt : 1.0f,
                                                      (l.pending & (4u << k)) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
@@ -264,6 +309,20 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                     c.setu(Cold::kStep, step + 1u);
                 }
             }
+            if (l.active) {
+                state = kLaneWalk;
+                if (walk_begin(S, l.o, l.d, w)) {
+                    h = final_hit(w.ab, w.tb);
+                    state = kLaneReady;
+                }
+            } else {
+                state = kLaneIdle;
+            }
+        }
+        // ---- one traversal step of every lane with a query in progress ----
+        if (state == kLaneWalk && walk_step<false>(S, l.o, l.d, w, stk, tc)) {
+            h = final_hit(w.ab, w.tb);
+            state = kLaneReady;
         }
     }
     add_stat(&A.stats->primary, n_primary);

@@ -13,6 +13,7 @@
 // slots from a ballot prefix.  Queue order is not deterministic, the image
 // is: every path's arithmetic is done by one lane in a fixed order.
 #include "path.h"
+#include "tail.h"
 #include "wavefront.h"
 
 namespace dmt {
@@ -67,16 +68,6 @@ __device__ __forceinline__ Stack make_stack(uint32_t* lds, uint32_t* spill, uint
     return stack_at(lds + threadIdx.x, kBlock, kLdsDepth, spill + (size_t)gid * spill_depth, spill_depth);
 }
 
-__device__ __forceinline__ void start_path(PathState& ps, RayQueue& q, uint32_t slot, uint32_t p, f3 pos, f3 rd,
-                                           uint32_t seed, uint32_t meta) {
-    q.o[slot] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-    q.d[slot] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(p));
-    ps.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-    ps.rng[p] = seed;
-    ps.meta[p] = meta;
-}
-
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(kBlock) k_generate(FrameArgs F, PathState ps, RayQueue q, uint32_t* ctr) {
@@ -86,26 +77,8 @@ __global__ void __launch_bounds__(kBlock) k_generate(FrameArgs F, PathState ps, 
         for (uint32_t k = 0; k < kCounterSlots; ++k) ctr[k] = 0u;
         ctr[kCtrRays + 0] = F.width * F.height;
     }
-    // Slot q walks the image in 8x8 tiles so a wave64 traces one tile of
-    // neighbouring camera rays; the path state stays indexed by pixel.
-    const uint32_t tiles_x = (F.width + 7u) / 8u;
-    const uint32_t tiles_y = (F.height + 7u) / 8u;
-    const uint32_t nslots = tiles_x * tiles_y * 64u;
-    for (uint32_t qs = gid; qs < nslots; qs += stride) {
-        const uint32_t tile = qs >> 6, lane = qs & 63u;
-        const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-        const uint32_t wx = min(8u, F.width - tx * 8u), hy = min(8u, F.height - ty * 8u);
-        const uint32_t lx = lane & 7u, ly = lane >> 3;
-        if (lx >= wx || ly >= hy) continue;
-        const uint32_t x = tx * 8u + lx, y = ty * 8u + ly;
-        // slot = pixels of the full tile rows above + full tiles to the left
-        // in this tile row + position inside the (possibly clipped) tile
-        const uint32_t slot = 8u * ty * F.width + tx * 8u * hy + ly * wx + lx;
-        const uint32_t p = y * F.width + x;
-        uint32_t seed = pixel_seed(F, x, y, F.frame_index);
-        f3 rd = camera_ray(F, x, y, seed);
-        start_path(ps, q, slot, p, F.pos, rd, seed, meta_pack(0u, 0u, 0u));
-    }
+    const uint32_t nslots = camera_tiles(F) * 64u;
+    for (uint32_t qs = gid; qs < nslots; qs += stride) camera_path(F, F.frame_index, ps, q, qs >> 6, qs & 63u);
 }
 
 template <bool COUNT>
@@ -139,23 +112,6 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView S, RayQueue q, uint
         add_stat(&st->ext_spheres, tc.spheres);
     }
 }
-
-// Writes a path's shadow queries at its own slot of the iteration.
-struct SlotSink : NoGradientHooks {
-    ShadowQueue q0, q1;   // light NEE, env NEE
-    uint32_t slot, pix;
-    bool emitted[2];
-    __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState&) {
-        ShadowQueue& s = kind == 0 ? q0 : q1;
-        const uint32_t tag = pix | kValidBit | (r.clamp ? kClampBit : 0u);
-        s.o[slot] = make_float4(r.o.x, r.o.y, r.o.z, r.t_max);
-        s.d[slot] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(tag));
-        s.a[slot] = make_float4(r.pre.x, r.pre.y, r.pre.z, r.ndl);
-        s.b[slot] = make_float4(r.brdf.x, r.brdf.y, r.brdf.z, r.pdf);
-        if (kind == 1) s.w[slot] = r.w;
-        emitted[kind] = true;
-    }
-};
 
 __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint32_t in_par, RayQueue qin,
                                                   RayQueue qout, uint32_t* ctr, const uint2* hits, PathState ps,
@@ -241,131 +197,32 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
 }
 
 // ---------------------------------------------------------------------------
-// k_tail: the rest of the frame in one launch.
+// k_tail: the rest of the frame (or all of it) in one launch.
 //
-// Late iterations hold few live paths, and each of their launches lasts as
-// long as its slowest ray, so a frame's tail of short launches costs far more
-// than its share of rays.  k_tail takes every path of the live queue and runs
-// it to the end, then the pixel's remaining samples: one closest-hit query per
-// loop step (the extension ray, or one step of a shadow query), lanes refilled
-// from the queue as their paths end.  Each path's arithmetic is the one the
-// iteration kernels do, in the same order (vertex shading, light-NEE then
-// env-NEE contribution, next vertex; sample sums as k_finish forms them), so
-// the pixel values are the same.
-
-namespace {
-
-constexpr uint32_t kEnded = 16u;   // pending bit: the path ends after this vertex
-
-// k_tail lane states
-constexpr uint32_t kLaneIdle = 0u;    // no path
-constexpr uint32_t kLaneWalk = 1u;    // a closest-hit query in progress
-constexpr uint32_t kLaneReady = 2u;   // the query is done; its hit waits for the wave's shading step
-
-#ifndef DMT_TAIL_SHADE_LANES
-#define DMT_TAIL_SHADE_LANES 32u
-#endif
-#ifndef DMT_TAIL_REFILL_LANES
-#define DMT_TAIL_REFILL_LANES 16u
-#endif
-constexpr uint32_t kTailShadeLanes = DMT_TAIL_SHADE_LANES;    // shade once this many lanes are ready
-constexpr uint32_t kTailRefillLanes = DMT_TAIL_REFILL_LANES;  // take new paths once this many are idle
-
-struct TailLane {
-    bool active;
-    uint32_t mode;        // 0 extension ray, 1 light-NEE shadow ray, 2 env-NEE shadow ray
-    f3 o, d;              // current ray
-    PathVertexState P;
-    uint32_t b;           // bounce
-    uint32_t pending;     // shadow records of the vertex (bits 0, 1), kEnded
-    uint32_t slot;        // queue entry: index of the lane's shadow records
-    uint32_t pix;
-};
-
-// Fields of a lane that are not touched at every query: an LDS column per
-// thread, field f at base[f * kBlock].
-struct TailCold {
-    float* base;
-    enum : uint32_t { kSum = 0, kS = 3, kTr = 4, kTmax = 7, kStep = 8, kNo = 9, kNd = 12, kRows = 15 };
-    __device__ __forceinline__ float get(uint32_t f) const { return base[f * kBlock]; }
-    __device__ __forceinline__ void set(uint32_t f, float v) { base[f * kBlock] = v; }
-    __device__ __forceinline__ uint32_t getu(uint32_t f) const { return __float_as_uint(get(f)); }
-    __device__ __forceinline__ void setu(uint32_t f, uint32_t v) { set(f, __uint_as_float(v)); }
-    __device__ __forceinline__ f3 get3(uint32_t f) const { return f3{get(f), get(f + 1), get(f + 2)}; }
-    __device__ __forceinline__ void set3(uint32_t f, f3 v) { set(f, v.x); set(f + 1, v.y); set(f + 2, v.z); }
-};
-
-__device__ __forceinline__ void tail_start_shadow(const ShadowQueue& q, uint32_t k, TailLane& l, TailCold& c) {
-    const float4 o4 = q.o[l.slot], d4 = q.d[l.slot];
-    l.o = xyz(o4);
-    l.d = xyz(d4);
-    c.set(TailCold::kTmax, o4.w);
-    c.set3(TailCold::kTr, f3{1.0f, 1.0f, 1.0f});
-    c.setu(TailCold::kStep, 0u);
-    l.mode = k + 1u;
-}
-
-// After a vertex or one of its shadow queries: the next shadow query (light
-// before env, as k_shadow<0> runs before k_shadow<1>), the next vertex, the
-// next sample of the pixel, or the pixel's frame value.
-__device__ __forceinline__ void tail_advance(const FrameArgs& F, const ShadowQueue& sl, const ShadowQueue& se,
-                                             const PathState& ps, TailLane& l, TailCold& c) {
-    if (l.pending & 1u) {
-        tail_start_shadow(sl, 0u, l, c);
-        return;
-    }
-    if (l.pending & 2u) {
-        if (se.b[l.slot].w < 0.0f) {
-            // ambient term of the no-env Lambert branch: added without a query
-            l.P.L = l.P.L + xyz(se.a[l.slot]);
-            l.pending &= ~2u;
-        } else {
-            tail_start_shadow(se, 1u, l, c);
-            return;
-        }
-    }
-    if (!(l.pending & kEnded)) {
-        l.o = c.get3(TailCold::kNo);
-        l.d = c.get3(TailCold::kNd);
-        l.b++;
-        l.mode = 0u;
-        return;
-    }
-    const uint32_t s = c.getu(TailCold::kS);
-    const f3 sum = (s == 0u) ? l.P.L : c.get3(TailCold::kSum) + l.P.L;
-    if (s + 1u < F.spp) {
-        // the next sample continues the pixel's stream (k_finish + k_regen)
-        c.set3(TailCold::kSum, sum);
-        c.setu(TailCold::kS, s + 1u);
-        const uint32_t x = l.pix % F.width, y = l.pix / F.width;
-        l.d = camera_ray(F, x, y, l.P.seed);
-        l.o = F.pos;
-        l.P.T_ = f3{1.0f, 1.0f, 1.0f};
-        l.P.L = f3{0.0f, 0.0f, 0.0f};
-        l.P.prev_pdf = 0.0f;
-        l.P.skip_sky = false;
-        l.P.prev_lambert = false;
-        l.b = 0u;
-        l.mode = 0u;
-        return;
-    }
-    const f3 v = frame_value(F, sum);
-    ps.F[l.pix] = make_float4(v.x, v.y, v.z, 0.0f);
-    l.active = false;
-}
-
-}  // namespace
-
+// A persistent grid takes the paths of the live queue (the camera rays of
+// k_generate, or the paths still alive after the per-bounce kernels) and
+// runs each to its end, then the pixel's remaining samples, writing the
+// pixel's frame value to PathState::F.  Every lane advances its closest-hit
+// query one BVH node or leaf per loop step (Walk); a lane whose query is done
+// waits until enough of the wave is done too, and the wave then processes
+// those lanes together: the vertex (shade_vertex), or one step of a shadow
+// query (transmit_step).  The traversal steps keep most lanes busy instead of
+// every query waiting for the wave's longest ray, and the shading code runs
+// for many lanes at once; idle lanes take new paths from the queue.  Each
+// path's arithmetic is the one the per-bounce kernels do, in the same order
+// (vertex shading, light-NEE then env-NEE contribution, next vertex; sample
+// sums as k_finish forms them), so the pixel values are the same.  The
+// scene's data stay kernel arguments, read where they are used.
+//
 // Five waves per SIMD (the LDS of a workgroup, stack and cold fields, allows
 // five per CU): 96 VGPRs, a few spilled values outside the traversal loop.
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerSimd, kTailWavesPerSimd)))
-k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
-                                                 uint32_t* ctr, PathState ps, ShadowQueue sl, ShadowQueue se,
-                                                 uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
+k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, PathState ps, ShadowQueue sl,
+       ShadowQueue se, uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
-    __shared__ float cold[TailCold::kRows * kBlock];
-    TailCold c{cold + threadIdx.x};
+    __shared__ float cold[tail::TailCold::kRows * kBlock];
+    tail::TailCold c{cold + threadIdx.x};
     const uint32_t n = ctr[kCtrRays + in_par];
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -373,28 +230,28 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
     TraceCounters tc{0u, 0u, 0u};
     uint32_t n_primary = 0, n_ext = 0, n_shadow = 0;
     bool drained = false;
-    TailLane l{};
+    tail::TailLane l{};
     l.active = false;
     // Each lane advances its query one node or leaf per loop step (Walk);
-    // a lane whose query is done waits (kLaneReady) until enough of the wave
+    // a lane whose query is done waits (tail::kLaneReady) until enough of the wave
     // is done too, and the wave then shades those lanes together.  So the
     // traversal steps keep most lanes busy instead of waiting, query by
     // query, for the wave's longest ray, and the shading code runs for many
     // lanes at once.
     Walk w{};
     HitRec h{0.0f, kMiss};
-    uint32_t state = kLaneIdle;
+    uint32_t state = tail::kLaneIdle;
     for (;;) {
         // hand queue entries to idle lanes (wave-uniform)
-        const unsigned long long idle = __ballot(state == kLaneIdle);
-        const unsigned long long walking = __ballot(state == kLaneWalk);
-        if (idle != 0ull && !drained && ((uint32_t)__popcll(idle) >= kTailRefillLanes || walking == 0ull)) {
+        const unsigned long long idle = __ballot(state == tail::kLaneIdle);
+        const unsigned long long walking = __ballot(state == tail::kLaneWalk);
+        if (idle != 0ull && !drained && ((uint32_t)__popcll(idle) >= tail::kTailRefillLanes || walking == 0ull)) {
             const uint32_t want = (uint32_t)__popcll(idle);
             uint32_t base = 0u;
             if (idle & (1ull << lane) ? (idle & below) == 0ull : false) base = atomicAdd(&ctr[kCtrTail], want);
             base = __shfl(base, __ffsll((long long)idle) - 1);
             const uint32_t i = base + (uint32_t)__popcll(idle & below);
-            if (state == kLaneIdle && i < n) {
+            if (state == tail::kLaneIdle && i < n) {
                 const float4 o4 = q.o[i], d4 = q.d[i];
                 l.pix = __float_as_uint(d4.w);
                 l.slot = i;
@@ -406,28 +263,28 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
                 l.P = PathVertexState{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[l.pix]};
                 l.b = meta & 0xFFu;
                 const uint32_t s = (meta >> 8) & 0xFFu;
-                c.setu(TailCold::kS, s);
-                if (s != 0u) c.set3(TailCold::kSum, xyz(ps.F[l.pix]));
+                c.setu(tail::TailCold::kS, s);
+                if (s != 0u) c.set3(tail::TailCold::kSum, xyz(ps.F[l.pix]));
                 l.mode = 0u;
                 l.active = true;
-                state = kLaneWalk;
+                state = tail::kLaneWalk;
                 if (walk_begin(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
-                    state = kLaneReady;
+                    state = tail::kLaneReady;
                 }
             }
             if (base + want >= n) drained = true;
         }
-        const unsigned long long ready = __ballot(state == kLaneReady);
-        const unsigned long long walk = __ballot(state == kLaneWalk);
+        const unsigned long long ready = __ballot(state == tail::kLaneReady);
+        const unsigned long long walk = __ballot(state == tail::kLaneWalk);
         if (ready == 0ull && walk == 0ull) {
             if (drained) break;
             continue;
         }
         // the finished queries of the wave, together: the vertex, or one step
         // of a shadow query; then the lane's next query
-        if (ready != 0ull && ((uint32_t)__popcll(ready) >= kTailShadeLanes || walk == 0ull) &&
-            state == kLaneReady) {
+        if (ready != 0ull && ((uint32_t)__popcll(ready) >= tail::kTailShadeLanes || walk == 0ull) &&
+            state == tail::kLaneReady) {
             if (l.mode == 0u) {
                 if (l.b == 0u) n_primary++;
                 else n_ext++;
@@ -435,17 +292,17 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
                 VertexResult R;
                 shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
-                l.pending = (sink.emitted[0] ? 1u : 0u) | (sink.emitted[1] ? 2u : 0u) | (cont ? 0u : kEnded);
+                l.pending = (sink.emitted[0] ? 1u : 0u) | (sink.emitted[1] ? 2u : 0u) | (cont ? 0u : tail::kEnded);
                 if (cont) {
-                    c.set3(TailCold::kNo, R.no);
-                    c.set3(TailCold::kNd, R.nd);
+                    c.set3(tail::TailCold::kNo, R.no);
+                    c.set3(tail::TailCold::kNd, R.nd);
                 }
-                tail_advance(F, sl, se, ps, l, c);
+                tail::tail_advance(F, sl, se, ps, l, c);
             } else {
                 n_shadow++;
-                float t_max = c.get(TailCold::kTmax);
-                f3 tr = c.get3(TailCold::kTr);
-                const uint32_t step = c.getu(TailCold::kStep);
+                float t_max = c.get(tail::TailCold::kTmax);
+                f3 tr = c.get3(tail::TailCold::kTr);
+                const uint32_t step = c.getu(tail::TailCold::kStep);
                 if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
@@ -457,36 +314,36 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q,
                                                      k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    tail_advance(F, sl, se, ps, l, c);
+                    tail::tail_advance(F, sl, se, ps, l, c);
                 } else {   // transmit_step ends every query by its step refract_bounces
-                    c.set(TailCold::kTmax, t_max);
-                    c.set3(TailCold::kTr, tr);
-                    c.setu(TailCold::kStep, step + 1u);
+                    c.set(tail::TailCold::kTmax, t_max);
+                    c.set3(tail::TailCold::kTr, tr);
+                    c.setu(tail::TailCold::kStep, step + 1u);
                 }
             }
             if (l.active) {
-                state = kLaneWalk;
+                state = tail::kLaneWalk;
                 if (walk_begin(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
-                    state = kLaneReady;
+                    state = tail::kLaneReady;
                 }
             } else {
-                state = kLaneIdle;
+                state = tail::kLaneIdle;
             }
         }
         // one traversal step of every lane with a query in progress
-        if (state == kLaneWalk && walk_step<COUNT>(S, l.o, l.d, w, stk, tc)) {
+        if (state == tail::kLaneWalk && walk_step<COUNT>(S, l.o, l.d, w, stk, tc)) {
             h = final_hit(w.ab, w.tb);
-            state = kLaneReady;
+            state = tail::kLaneReady;
         }
     }
-    add_stat(&st->primary, n_primary);
-    add_stat(&st->extension, n_ext);
-    add_stat(&st->shadow, n_shadow);
+    tail::add_stat(&st->primary, n_primary);
+    tail::add_stat(&st->extension, n_ext);
+    tail::add_stat(&st->shadow, n_shadow);
     if (COUNT) {
-        add_stat(&st->tail_nodes, tc.nodes);
-        add_stat(&st->tail_tris, tc.tris);
-        add_stat(&st->tail_spheres, tc.spheres);
+        tail::add_stat(&st->tail_nodes, tc.nodes);
+        tail::add_stat(&st->tail_tris, tc.tris);
+        tail::add_stat(&st->tail_spheres, tc.spheres);
     }
 }
 
