@@ -1,4 +1,4 @@
-"""Benchmark: Mrays/s of the wavefront path tracer on the 1.51 M-triangle 1080p workload.
+"""Benchmark: Mrays/s of the path tracer on the 1.51 M-triangle 1080p workload.
 
 Workload (config 1 of the synthetic generators): icosphere subdivided 8x
 (1,310,720 triangles, radius 3, value-noise displacement 0.1) + 200,000 random
@@ -6,21 +6,26 @@ small triangles in [-5,5]^3, one quad light, 512x256 lognormal env map,
 1920x1080, 1 spp, 8 bounces, binned-SAH BVH4 built on the host (not timed).
 
 A step is one frame (1920x1080, 1 spp, up to 8 bounces, accumulated into the
-running mean), issued as one dmt_render call; the library keeps three frames in
-flight on three HIP streams (the accumulation passes stay ordered frame after
-frame, so the image equals the sequential result).  value = (primary +
-extension + shadow rays) / wall time of the K frames, inputs resident in HBM.
+running mean), issued as one dmt_render call: camera rays into the path
+queue (k_generate), the frame-completion kernel (k_tail) that runs every path
+to its end, the accumulation pass (k_accumulate).  The library keeps four
+frames in flight on four HIP streams (the accumulation passes stay ordered
+frame after frame, so the image equals the sequential result).  value =
+(primary + extension + shadow rays) / wall time of the K frames, inputs
+resident in HBM.
 
-roofline: the closest-hit (extend) kernel, HBM-bound by design (no dense
+roofline: the kernel with the most device time per frame (k_tail on the
+default schedule; k_extend, the closest-hit kernel, when the camera bounce
+runs through the per-bounce kernels), HBM-bound by design (no dense
 contraction).  Algorithmic bytes per launch = node visits x 128 B (BVH4 node)
-+ triangle tests x 48 B (TriRec) + sphere tests x 20 B + rays x 40 B (32 B ray
-record in, 8 B hit out), counted by an instrumented re-render of the same
-frame; divided by the HIP-event duration of that kernel's launches in frames
-rendered one at a time after the timed region (in the timed region three
-frames share the GPU, and a launch's duration there includes the kernels of
-the other frames).  traffic: the measured bytes per launch beyond L2 (rocprofv3
-FETCH_SIZE + WRITE_SIZE passes of this workload, scaled by a 512 MiB copy),
-read from profiles/k_extend_c1_1080p_pmc.txt.
++ triangle tests x 48 B (TriRec) + sphere tests x 20 B (+ rays x 40 B for
+k_extend: 32 B ray record in, 8 B hit out), counted by an instrumented
+re-render of the same frame; divided by the HIP-event duration of that
+kernel's launches in frames rendered one at a time after the timed region (in
+the timed region four frames share the GPU, and a launch's duration there
+includes the kernels of the other frames).  traffic: the measured bytes per
+launch beyond L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE passes of this workload,
+scaled by a 512 MiB copy), read from profiles/<kernel>_c1_1080p_pmc.txt.
 
 cpu_baseline: the CPU oracle (reference semantics, fp32) rendering whole frames
 of the same workload for about 10 s, on this host's cores (hw - 4 threads as
@@ -29,11 +34,13 @@ the reference's software tracer uses, and all cores).
 --workload batch: the batched form.  64 scenes of the batched workload
 generator (resolution uniform in {128, 256, 512, 1024} squared, triangles
 log-uniform in 10k..2M, half with 64 spheres; 1 spp, depth 6).  The plain
-form renders each frame as 64 sequential per-scene dmt_render calls; the
-batched form renders each frame of all 64 scenes with one persistent-kernel
-launch replayed from a captured hipGraph, with parameters picked by
-dmt_batch_autotune.  Both run the same frames from frame 0; the images must be
-identical.  value = plain time / batched time over the timed frames.
+form renders each frame as 64 sequential per-scene dmt_render calls, each to
+completion; the batched form renders each frame of all 64 scenes with one
+persistent-kernel launch replayed from a captured hipGraph, with parameters
+picked by dmt_batch_autotune.  All run the same frames from frame 0; the
+images must be identical.  value = plain time / batched time over the timed
+frames.  vs_pipelined_plain compares with the same dmt_render calls issued
+back to back, the library keeping up to four frames in flight.
 
 --workload adjoint: the differentiable form.  Config 4: 256 spheres with 256
 distinct materials over the ground plane, 512x512, 16 spp, depth 4, Russian
@@ -59,15 +66,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_extend_c1_1080p_pmc.txt")
-
-
-def measured_traffic():
-    """HBM bytes per k_extend launch on this workload, from the recorded
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known
-    size; a profiler cannot run inside this process)."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 1080p workload, from the
+    recorded rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel
+    of known size; a profiler cannot run inside this process)."""
     try:
-        for line in open(PMC_SUMMARY):
+        for line in open(os.path.join(ROOT, "profiles", f"{kernel}_c1_1080p_pmc.txt")):
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":")[1])
     except OSError:
@@ -212,7 +216,7 @@ def main():
                           launches=launches, achieved=per_launch / (avg_ms * 1e-3) / 1e9)
         log(f"[bench] {name}: {launches / nf:.0f} launch(es)/frame, {per_launch / 1e6:.1f} MB/launch, "
             f"avg launch {avg_ms:.3f} ms alone -> {kern[name]['achieved']:.0f} GB/s "
-            f"({shared:.3f} ms with three frames in flight)")
+            f"({shared:.3f} ms with four frames in flight)")
     log(f"[bench] k_extend: {tr['ext_rays']} rays, {tr['ext_node_visits'] / max(tr['ext_rays'], 1):.1f} nodes/ray, "
         f"{tr['ext_tri_tests'] / max(tr['ext_rays'], 1):.2f} tri tests/ray; k_tail: {tr['tail_node_visits']} node "
         f"visits, {tr['tail_tri_tests']} tri tests, {tr['tail_sphere_tests']} sphere tests")
@@ -282,7 +286,7 @@ def main():
                                 f"{schedule} per-bounce iteration(s), then one frame-completion launch")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic() if (cfg == 1 and dom == "k_extend") else None,
+                     "traffic": measured_traffic(dom) if cfg == 1 else None,
                      "kernel": dom, "bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_ms": round(avg_launch_ms, 5),
                      "timing": f"{int(K['launches'])} launches of {args.roofline_frames} frames rendered "
@@ -316,30 +320,43 @@ def bench_batch(args, np, torch, dmt):
         dev.synchronize()
         torch.cuda.synchronize()
 
-    # plain: each frame is 64 sequential per-scene launches
-    def plain(k0, n):
+    # plain: each frame is one dmt_render per scene.  Sequential: every
+    # render runs to completion before the next starts (host sync after each
+    # scene's frame).  Pipelined: the library's own frames in flight (up to
+    # four renders overlap on the device).
+    def plain(k0, n, each=None):
         for k in range(k0, k0 + n):
             for sc, f in zip(scenes, frames):
                 g = f.copy()
                 g.frame_index = f.frame_index + k
                 g.reset_accum = 1 if k == 0 else 0
                 sc.render(g)
+                if each:
+                    each()
 
-    plain(0, args.warmup)
-    sync()
-    for sc in scenes:
-        sc.reset_stats()
-    sync()
-    t0 = time.perf_counter()
-    plain(args.warmup, args.steps)
-    sync()
-    dt_plain = time.perf_counter() - t0
-    rays_plain = 0
-    for sc in scenes:
-        st = sc.stats()
-        rays_plain += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
-    images = [sc.readback(f.width, f.height) for sc, f in zip(scenes, frames)]
-    log(f"[bench] plain: {args.steps} frames in {dt_plain * 1e3:.1f} ms ({rays_plain / dt_plain / 1e6:.1f} Mrays/s)")
+    def timed_plain(each):
+        plain(0, args.warmup, each)
+        sync()
+        for sc in scenes:
+            sc.reset_stats()
+        sync()
+        t0 = time.perf_counter()
+        plain(args.warmup, args.steps, each)
+        sync()
+        dt = time.perf_counter() - t0
+        rays = 0
+        for sc in scenes:
+            st = sc.stats()
+            rays += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+        return dt, rays, [sc.readback(f.width, f.height) for sc, f in zip(scenes, frames)]
+
+    dt_pipe, rays_plain, images = timed_plain(None)
+    log(f"[bench] plain, pipelined: {args.steps} frames in {dt_pipe * 1e3:.1f} ms "
+        f"({rays_plain / dt_pipe / 1e6:.1f} Mrays/s)")
+    dt_plain, rays_seq, images_seq = timed_plain(dev.synchronize)
+    same_seq = all(np.array_equal(a, b) for a, b in zip(images, images_seq)) and rays_seq == rays_plain
+    log(f"[bench] plain, sequential: {args.steps} frames in {dt_plain * 1e3:.1f} ms "
+        f"({rays_plain / dt_plain / 1e6:.1f} Mrays/s), images equal to the pipelined renders {same_seq}")
 
     batch = dmt.Batch(dev, scenes, frames)
     t0 = time.time()
@@ -375,13 +392,15 @@ def bench_batch(args, np, torch, dmt):
         "config": {"workload": f"{len(scenes)} scenes: resolution uniform in {{128,256,512,1024}}^2, triangles "
                                "log-uniform 10k..2M (icosphere + random tris), odd scenes + 64 spheres; "
                                "1 spp, max depth 6; one persistent launch per frame from a captured hipGraph "
-                               "vs 64 sequential per-scene launches",
+                               "vs 64 sequential per-scene renders (each to completion)",
                    "scenes": len(scenes), "pixels": pixels, "triangles": tris, "seed": args.seed,
                    "params": tuned.as_dict()},
         "plain_ms_per_step": round(dt_plain * 1e3 / args.steps, 4),
         "batch_mrays": round(rays_batch / dt_batch / 1e6, 2),
         "plain_mrays": round(rays_plain / dt_plain / 1e6, 2),
-        "identical_images": all(same) and rays_batch == rays_plain,
+        "vs_pipelined_plain": round(dt_pipe / dt_batch, 3),
+        "pipelined_plain_ms_per_step": round(dt_pipe * 1e3 / args.steps, 4),
+        "identical_images": all(same) and rays_batch == rays_plain and same_seq,
         "roofline": None,
         "cpu_baseline": None,
     }
