@@ -17,6 +17,7 @@
 #include "../../include/dmt.h"
 #include "host/prep.h"
 #include "kernels/adjoint.h"
+#include "kernels/output.h"
 #include "kernels/persistent.h"
 #include "kernels/wavefront.h"
 
@@ -100,6 +101,7 @@ struct dmt_scene {
     std::vector<void*> wf_buffers;
     float4* accum = nullptr;
     uint32_t acc_w = 0, acc_h = 0;
+    bool accum_external = false;      // caller-owned (dmt_scene_bind_accum)
     DeviceStats* stats = nullptr;
     DeviceStats* count_stats = nullptr;
     hipEvent_t acc_done[kFrameSets] = {};
@@ -187,8 +189,9 @@ void free_sets(dmt_scene* s) {
 
 void free_accum(dmt_scene* s) {
     free_sets(s);
-    if (s->accum) (void)hipFree(s->accum);
+    if (s->accum && !s->accum_external) (void)hipFree(s->accum);
     s->accum = nullptr;
+    s->accum_external = false;
     s->acc_w = s->acc_h = 0;
 }
 
@@ -197,6 +200,9 @@ dmt_status ensure_accum(dmt_scene* s, uint32_t w, uint32_t h, bool* resized) {
     *resized = false;
     if (s->accum && s->acc_w == w && s->acc_h == h) return DMT_OK;
     if (s->batches > 0) return fail(DMT_ERR_INVALID_ARG, "the scene belongs to a batch at another resolution");
+    if (s->accum_external)
+        return fail(DMT_ERR_INVALID_ARG, "the bound accumulation buffer is " + std::to_string(s->acc_w) + "x" +
+                                             std::to_string(s->acc_h));
     hipError_t e = sync_all(s->dev);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize before resize");
     free_accum(s);
@@ -563,6 +569,62 @@ dmt_status dmt_accum_device_ptr(dmt_scene* s, void** out) {
     if (!s || !out) return fail(DMT_ERR_INVALID_ARG, "null pointer");
     *out = s->accum;
     return s->accum ? DMT_OK : fail(DMT_ERR_INVALID_ARG, "nothing rendered yet");
+}
+
+dmt_status dmt_scene_bind_accum(dmt_scene* s, void* d_rgba, uint32_t w, uint32_t h) {
+    if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if (s->batches > 0) return fail(DMT_ERR_INVALID_ARG, "the scene belongs to a batch");
+    if (d_rgba) {
+        if (w == 0 || h == 0) return fail(DMT_ERR_INVALID_ARG, "width and height must be >= 1");
+        if (w > 65535u || h > 65535u) return fail(DMT_ERR_LIMIT, "width and height must be <= 65535");
+        if (reinterpret_cast<uintptr_t>(d_rgba) % 16u) return fail(DMT_ERR_INVALID_ARG, "d_rgba must be 16-byte aligned");
+        hipPointerAttribute_t attr{};
+        hipError_t pe = hipPointerGetAttributes(&attr, d_rgba);
+        if (pe != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(DMT_ERR_INVALID_ARG, "d_rgba is not a device allocation");
+        }
+        if (attr.type != hipMemoryTypeDevice || attr.device != s->dev->id)
+            return fail(DMT_ERR_INVALID_ARG, "d_rgba is not device memory of the scene's device");
+    }
+    // frames in flight write the current buffer
+    hipError_t e = sync_all(s->dev);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize");
+    if (!d_rgba) {
+        if (s->accum_external) free_accum(s);
+        return DMT_OK;
+    }
+    if (s->accum && s->acc_w == w && s->acc_h == h) {
+        // same resolution: the wavefront sets stay
+        if (!s->accum_external) (void)hipFree(s->accum);
+    } else {
+        free_accum(s);
+        s->acc_w = w;
+        s->acc_h = h;
+    }
+    s->accum = static_cast<float4*>(d_rgba);
+    s->accum_external = true;
+    return DMT_OK;
+}
+
+dmt_status dmt_tonemap(dmt_scene* s, const dmt_tonemap_params* p, void* d_out) {
+    if (!s || !p || !d_out) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if (!s->accum) return fail(DMT_ERR_INVALID_ARG, "nothing rendered yet");
+    if (reinterpret_cast<uintptr_t>(d_out) % 16u) return fail(DMT_ERR_INVALID_ARG, "d_out_rgba must be 16-byte aligned");
+    const float exposure = (std::isfinite(p->exposure) && p->exposure > 0.0f) ? p->exposure : 1.0f;
+    hipStream_t stream = s->dev->stream;
+    hipError_t e = hipSuccess;
+    if (s->last_set >= 0) e = hipStreamWaitEvent(stream, s->acc_done[s->last_set], 0);
+    if (e == hipSuccess)
+        e = launch_tonemap(s->accum, s->acc_w * s->acc_h, exposure, p->op, static_cast<float4*>(d_out), stream);
+    // the next frame's accumulation pass waits for this read
+    if (e == hipSuccess && s->last_set >= 0) e = hipEventRecord(s->acc_done[s->last_set], stream);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "tonemap");
+    return DMT_OK;
 }
 
 dmt_status dmt_trace_closest(dmt_scene* s, const float* rays, uint32_t n, dmt_hit* out) {

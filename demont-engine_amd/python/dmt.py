@@ -24,7 +24,7 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
 SCHEDULE_WAVEFRONT = 0xFFFFFFFF
 MATERIAL_FLOATS = 7
@@ -125,6 +125,10 @@ class BatchParams(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class TonemapParams(C.Structure):
+    _fields_ = [("exposure", C.c_float), ("op", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
 class DmtError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"dmt status {status}: {msg}")
@@ -159,6 +163,8 @@ def lib() -> C.CDLL:
         "dmt_render_frames": (st, [vp, C.POINTER(FrameParams), u32]),
         "dmt_readback_accum": (st, [vp, fp, C.c_size_t]),
         "dmt_accum_device_ptr": (st, [vp, C.POINTER(vp)]),
+        "dmt_scene_bind_accum": (st, [vp, vp, u32, u32]),
+        "dmt_tonemap": (st, [vp, C.POINTER(TonemapParams), vp]),
         "dmt_trace_closest": (st, [vp, fp, u32, vp]),
         "dmt_scene_stats": (st, [vp, C.POINTER(Stats)]),
         "dmt_scene_reset_stats": (st, [vp]),
@@ -360,6 +366,15 @@ class Scene:
         p = C.c_void_p()
         _check(lib().dmt_accum_device_ptr(self._h, C.byref(p)))
         return p.value or 0
+
+    def bind_accum(self, d_rgba: int, width: int, height: int):
+        """Render into caller-owned device memory (width*height float4); 0 unbinds."""
+        _check(lib().dmt_scene_bind_accum(self._h, C.c_void_p(d_rgba or None), width, height))
+
+    def tonemap(self, d_out: int, exposure: float = 1.0, op: int = 0):
+        """Display image (sRGB of tonemapped mean * exposure, alpha 1) into device memory."""
+        p = TonemapParams(exposure, op)
+        _check(lib().dmt_tonemap(self._h, C.byref(p), C.c_void_p(d_out)))
 
     def trace(self, origins: np.ndarray, dirs: np.ndarray) -> np.ndarray:
         n = origins.shape[0]

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define DMT_ABI_VERSION 2u
+#define DMT_ABI_VERSION 3u
 
 typedef int32_t dmt_status;
 #define DMT_OK                  0
@@ -182,6 +182,31 @@ dmt_status dmt_render_frames(dmt_scene* scene, const dmt_frame_params* first_fra
 dmt_status dmt_readback_accum(dmt_scene* scene, float* rgba, size_t float_count);
 /* Device address of the accumulation buffer (width*height float4). */
 dmt_status dmt_accum_device_ptr(dmt_scene* scene, void** out_ptr);
+
+/* Caller-owned accumulation memory, for an engine whose accumulation texture
+ * outlives scene rebuilds: the reference engine binds one accum texture at
+ * storage-texture slot 1 for every frame (Engine.cpp:6867-6870) and its
+ * contents, not the scene, carry the running mean.  d_rgba is width*height
+ * float4 of device memory on the scene's device, 16-byte aligned; its
+ * contents are the history (nothing is cleared), and every frame rendered
+ * must have this width and height.  NULL returns the scene to a buffer of its
+ * own (allocated and zeroed at the next render).  Waits for the frames in
+ * flight on the device.  Not allowed while the scene belongs to a batch. */
+dmt_status dmt_scene_bind_accum(dmt_scene* scene, void* d_rgba, uint32_t width, uint32_t height);
+
+/* Display output (r_tonemap_op, exposure_state[0]). */
+typedef struct dmt_tonemap_params {
+    float    exposure;          /* scales the mean; <= 0 or non-finite -> 1  */
+    uint32_t op;                /* 0 ACES, 1 AgX, 2 Khronos PBR Neutral,
+                                   3 Reinhard, 4 linear clamp; others -> 0   */
+    uint32_t reserved[2];
+} dmt_tonemap_params;
+/* Replaces the output stage of RunPathTraceKernel (SoftwareTracer.cpp:788-801,
+ * 995-1007), which writes the slot-0 output texture: rgb = sRGB OETF of
+ * tonemap(mean * exposure), alpha = 1, as width*height float4 into device
+ * memory d_out_rgba.  Stream-ordered after the scene's last frame; later
+ * frames wait for it before they update the accumulation buffer. */
+dmt_status dmt_tonemap(dmt_scene* scene, const dmt_tonemap_params* params, void* d_out_rgba);
 
 /* traceScene() closest-hit query (PathTrace.slang:2331-2693) for n rays,
  * 8 floats per ray (ox, oy, oz, -, dx, dy, dz, -); host in, host out. */

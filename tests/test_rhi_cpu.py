@@ -1,0 +1,215 @@
+"""The RHI adapter without a GPU: PtPush decoding (field offsets, feature
+gates, short pushes), the host-side resource rules the software device sets
+(WriteBuffer range checks, BLAS validation, TLAS with unknown BLAS), the
+exported C++ interface, and the reference-side binding compiled against the
+reference's own RHI headers."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import _dmt_rhi as rhi
+import dmt
+from rhi_engine import NO_TEX_TILE, OFF, PTPUSH_SIZE, pack_ptpush
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_SRC = "/root/reference/src"
+
+
+def _frame():
+    sd = dmt.generate(0, seed=2, width=64, height=48)
+    f = sd.frame.copy()
+    f.frame_index = 17
+    f.reset_accum = 1
+    f.spp = 3
+    f.max_bounces = 5
+    f.refract_bounces = 6
+    f.firefly_clamp = 7.5
+    f.ema_alpha = 0.25
+    f.mis_enabled = 0
+    f.light_tree_enabled = 1
+    return f
+
+
+def test_ptpush_size_matches_the_engine_layout():
+    # the engine asserts these offsets beside its PtPush (Engine.cpp:9409-9423)
+    assert PTPUSH_SIZE == 1792
+    assert (OFF["sun_and_mode"], OFF["accum_params"], OFF["hosek_params"], OFF["tonemap_params"]) == (240, 784, 1184, 1600)
+
+
+def test_decode_reads_every_frame_field():
+    f = _frame()
+    push = pack_ptpush(f, prim_count=65, light_count=3, env_present=True, env_intensity=2.5, tlas_present=True,
+                       tonemap_op=2)
+    d = rhi.decode_pathtrace_push(push)
+    assert d["pos_fovtan"] == pytest.approx(list(f.pos_fovtan), abs=0)
+    assert d["fwd_aspect"] == pytest.approx(list(f.fwd_aspect), abs=0)
+    assert d["right_xyz"] == pytest.approx(list(f.right_xyz), abs=0)
+    assert d["up_xyz"] == pytest.approx(list(f.up_xyz), abs=0)
+    assert (d["frame_index"], d["spp"], d["max_bounces"], d["reset_accum"]) == (17, 3, 5, 1)
+    assert (d["mis_enabled"], d["light_tree_enabled"], d["refract_bounces"]) == (0, 1, 6)
+    assert (d["firefly_clamp"], d["ema_alpha"], d["flags"]) == (7.5, 0.25, 0)
+    assert (d["prim_count"], d["light_count"], d["env_map_present"], d["tlas_present"]) == (65, 3, 1, 1)
+    assert (d["env_intensity"], d["tonemap_op"], d["motion_blur"], d["ignored"]) == (2.5, 2, 0, "")
+
+
+def test_decode_refract_budget_and_light_tree_gate():
+    f = _frame()
+    for budget, want in ((0.0, 0), (0.5, 0), (1.0, 1), (4.9, 4), (16.0, 16)):
+        push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True,
+                           overrides={"w2j_row1": ("4f", (0.0, 1.0, 0.0, budget))})
+        assert rhi.decode_pathtrace_push(push)["refract_bounces"] == want
+    # the engine clears light_tree_enabled when the cvar is off or there are no nodes (Engine.cpp:8133-8137)
+    push = pack_ptpush(f, prim_count=1, light_count=0, env_present=True)
+    assert rhi.decode_pathtrace_push(push)["light_tree_enabled"] == 0
+
+
+GATES = [
+    ("sun_and_mode", "4f", (0.0, -0.5, -0.86, 2.0), "procedural sky"),
+    ("sun_and_mode", "4f", (0.0, -0.5, -0.86, 3.0), "Hosek-Wilkie"),
+    ("hdri_lights_count", "I", 2, "HDRI cluster lights"),
+    ("dof_params", "4f", (0.05, 4.0, 0.0, 0.0), "depth of field"),
+    ("spectral_params", "4f", (1.0, 0.0042, 0.0, 0.0), "spectral dispersion"),
+    ("vol_params", "4f", (0.01, 0.0, 1.0, 8.0), "participating media"),
+    ("clouds_p1", "4f", (100.0, 200.0, 0.5, 0.2), "participating media"),
+    ("clouds_p3", "4f", (0.0, 0.0, 0.0, 1.0), "participating media"),
+    ("sdf_params", "4I", (2, 128, 0, 0), "SDF clusters"),
+    ("restir_enabled", "I", 1, "ReSTIR"),
+    ("motion_blur_params", "4f", (1.0, 0.016, 0.0, 0.0), "motion blur"),
+    ("mat_override_count", "4I", (1, 0, 0, 0), "material overrides"),
+    ("smoke_params", "4f", (1.0, 0.0, 0.0, 0.0), "smoke volumes"),
+    ("denoiser_enabled", "I", 1, "denoiser G-buffer outputs"),
+    ("write_hdr_aux", "I", 1, "denoiser G-buffer outputs"),
+    ("write_shadow_vis", "I", 1, "denoiser G-buffer outputs"),
+    ("mesh_tex_indices", "4I", (3, NO_TEX_TILE, NO_TEX_TILE, NO_TEX_TILE), "PBR mesh textures"),
+]
+
+
+@pytest.mark.parametrize("field,fmt,value,what", GATES, ids=[g[0] + "-" + g[3] for g in GATES])
+def test_strict_refuses_and_lenient_lists_each_feature(field, fmt, value, what):
+    f = _frame()
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, tlas_present=True,
+                       overrides={field: (fmt, value)})
+    with pytest.raises(ValueError, match=what):
+        rhi.decode_pathtrace_push(push, strict=True)
+    d = rhi.decode_pathtrace_push(push, strict=False)
+    assert what in d["ignored"]
+    assert d["spp"] == 3   # the frame itself still decodes
+
+
+def test_sky_without_env_map_needs_sun_and_moon_below_the_horizon():
+    f = _frame()
+    below = pack_ptpush(f, prim_count=1, light_count=1, env_present=False)
+    assert rhi.decode_pathtrace_push(below)["env_map_present"] == 0
+    # sunDisc / moonDisc vanish below y = -0.1 and sun / moon NEE below y = 0
+    # (PathTrace.slang:3113, 3213, 6555, 6626): -0.1 itself is refused
+    for field, y, what in (("sun_and_mode", -0.1, "sun above"), ("sun_and_mode", 0.3, "sun above"),
+                           ("moon_dir_phase", -0.1, "moon above"), ("moon_dir_phase", 0.9, "moon above")):
+        push = pack_ptpush(f, prim_count=1, light_count=1, env_present=False,
+                           overrides={field: ("4f", (0.0, y, 0.5, 0.0 if field == "sun_and_mode" else 1.0))})
+        with pytest.raises(ValueError, match=what):
+            rhi.decode_pathtrace_push(push)
+    # with the env map the HDRI is the sky: the sun's position does not matter
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True,
+                       overrides={"sun_and_mode": ("4f", (0.0, 0.9, 0.4, 1.0))})
+    assert rhi.decode_pathtrace_push(push)["env_map_present"] == 1
+    # HDRI cluster lights only count with an env map (PathTrace.slang:3680)
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=False, overrides={"hdri_lights_count": ("I", 4)})
+    assert rhi.decode_pathtrace_push(push)["ignored"] == ""
+    # PBR mesh tiles only count with a mesh
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, tlas_present=False,
+                       overrides={"mesh_tex_indices": ("4I", (0, 0, 0, 0))})
+    assert rhi.decode_pathtrace_push(push)["ignored"] == ""
+
+
+def test_short_push_is_refused_and_missing_fields_read_as_zero():
+    f = _frame()
+    full = pack_ptpush(f, prim_count=9, light_count=1, env_present=True)
+    for n in (0, 64, 111):
+        with pytest.raises(ValueError, match="shorter than the 112 B"):
+            rhi.decode_pathtrace_push(full[:n])
+    d = rhi.decode_pathtrace_push(full[:112])   # the head only (SoftwareTracer.cpp:38-56)
+    assert (d["prim_count"], d["spp"], d["env_map_present"]) == (9, 3, 1)
+    assert (d["light_count"], d["mis_enabled"], d["light_tree_enabled"], d["refract_bounces"]) == (0, 0, 0, 0)
+    assert (d["firefly_clamp"], d["ema_alpha"], d["tonemap_op"]) == (0.0, 0.0, 0)
+    # a head-only push without the env map leaves the sun at y = 0: refused
+    noenv = pack_ptpush(f, prim_count=9, light_count=1, env_present=False)
+    with pytest.raises(ValueError, match="sun above"):
+        rhi.decode_pathtrace_push(noenv[:112])
+    # bytes past the 2048-byte push budget are dropped, not read
+    assert rhi.decode_pathtrace_push(full + bytes(4096))["prim_count"] == 9
+
+
+def test_host_resources_without_a_device():
+    d = rhi.HipDevice(strict=True)
+    if not d.is_device_lost():
+        pytest.skip("a HIP device is present: covered by the GPU tests")
+    assert "dmt_device_create" in d.last_error()
+    assert d.supports_hardware_rt()
+    # buffers are host copies: written and read back, out-of-range writes refused
+    b = d.create_buffer(64)
+    assert b
+    d.write_buffer(b, np.arange(8, dtype=np.float32), 16)
+    raw = d.readback_buffer(b, 64)
+    assert np.array_equal(np.frombuffer(raw, np.float32)[4:12], np.arange(8, dtype=np.float32))
+    d.write_buffer(b, np.zeros(8, np.float32), 40)
+    assert "exceeds buffer size 64" in d.last_error()
+    assert np.array_equal(np.frombuffer(d.readback_buffer(b, 64), np.float32), np.frombuffer(raw, np.float32))
+    assert d.readback_buffer(b, 65) is None
+    # BLAS: empty meshes and out-of-range indices are refused
+    pos = np.zeros((3, 3), np.float32)
+    assert d.create_blas(pos, np.array([0, 1, 2], np.uint32))
+    assert d.create_blas(np.zeros((0, 3), np.float32), np.zeros(0, np.uint32)) == 0
+    assert d.create_blas(pos, np.array([0, 1, 3], np.uint32)) == 0 and "out of range" in d.last_error()
+    assert d.create_blas(pos, np.array([0, 1], np.uint32)) == 0
+    # TLAS: unknown BLAS ids are skipped, an empty instance list gives no TLAS
+    blas = d.create_blas(pos, np.array([0, 1, 2], np.uint32))
+    assert d.create_tlas([(blas, None, 0, 0xFF), (999999, None, 0, 0xFF)])
+    assert d.create_tlas([]) == 0
+    with pytest.raises(ValueError):
+        d.create_tlas([(blas, [1.0] * 11, 0, 0xFF)])
+    # textures need device memory
+    assert d.create_texture(4, 4) == 0
+    assert d.allocated_bytes() > 0
+    d.close()
+
+
+def _nm_demangled(path):
+    if not shutil.which("nm"):
+        pytest.skip("nm not available")
+    out = subprocess.run(["nm", "-DC", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return out
+
+
+def test_adapter_exports_the_device_interface():
+    lib = os.path.join(ROOT, "demont-engine_amd", "libdmt_rhi.so")
+    syms = _nm_demangled(lib)
+    methods = ["CreateBuffer", "CreateTexture", "CreateComputePipeline", "CreateBLAS", "CreateTLAS", "DestroyBuffer",
+               "DestroyTexture", "DestroyPipeline", "DestroyAccelStruct", "WriteBuffer", "WriteTexture", "BeginFrame",
+               "EndFrame", "AcquireCommandBuffer", "Submit", "WaitIdle", "Resize", "ReadbackTexture", "ReadbackBuffer",
+               "DeviceName", "CurrentAllocatedBytes", "IsDeviceLost", "LastError", "ActiveScene"]
+    for m in methods:
+        assert f"dmt::rhi::HipDevice::{m}(" in syms or f"dmt::rhi::HipDevice::{m}[abi:" in syms, m
+    for m in ["BindComputePipeline", "BindBuffer", "BindStorageTexture", "BindAccelStruct", "PushConstants",
+              "Dispatch", "CopyBufferToTexture", "ClearStorageTexture", "Barrier"]:
+        assert f"dmt::rhi::HipCommandBuffer::{m}(" in syms, m
+    assert "dmt::rhi::DecodePathtracePush(" in syms
+    assert "dmt::rhi::CreateHipDevice(" in syms
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_SRC, "rhi")), reason="reference RHI headers not present")
+def test_reference_side_binding_compiles_against_the_reference_headers(tmp_path):
+    """pt::rhi::HipDevice (csrc/rhi/pt_rhi_hip_device.cpp) implements every pure
+    virtual of the reference's Device and CommandBuffer and links against
+    libdmt_rhi.so with no undefined symbol."""
+    src = os.path.join(ROOT, "demont-engine_amd", "csrc", "rhi", "pt_rhi_hip_device.cpp")
+    obj, so = str(tmp_path / "binding.o"), str(tmp_path / "binding.so")
+    subprocess.run(["g++", "-std=c++2b", "-Wall", "-Wextra", "-Werror", "-fPIC", "-c", src, "-I", REF_SRC, "-I",
+                    os.path.join(ROOT, "include"), "-o", obj], check=True)
+    libdir = os.path.join(ROOT, "demont-engine_amd")
+    subprocess.run(["g++", "-shared", "-o", so, obj, "-L", libdir, "-ldmt_rhi", "-Wl,--no-undefined",
+                    "-Wl,-rpath," + libdir], check=True)
+    assert "pt::rhi::CreateHipDevice(" in _nm_demangled(so)
