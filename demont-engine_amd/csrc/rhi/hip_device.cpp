@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <unordered_map>
 #include <vector>
 
@@ -186,6 +187,27 @@ bool same_geometry(const std::vector<float>& a, const std::vector<float>& b) {
     return true;
 }
 
+// Appends BLAS b to the flattened mesh t, each vertex through the row-major 3x4
+// transform m; an identity transform keeps the vertices bit for bit.
+void append_instance(Accel& t, const Accel& b, const float* m) {
+    static const float kIdentity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    const bool identity = std::memcmp(m, kIdentity, sizeof(kIdentity)) == 0;
+    const std::size_t base = t.positions.size() / 3u;
+    t.positions.reserve(t.positions.size() + b.positions.size());
+    for (std::size_t v = 0; v < b.positions.size(); v += 3) {
+        const float x = b.positions[v], y = b.positions[v + 1], z = b.positions[v + 2];
+        if (identity) {
+            t.positions.insert(t.positions.end(), {x, y, z});
+        } else {
+            t.positions.insert(t.positions.end(), {m[0] * x + m[1] * y + m[2] * z + m[3],
+                                                   m[4] * x + m[5] * y + m[6] * z + m[7],
+                                                   m[8] * x + m[9] * y + m[10] * z + m[11]});
+        }
+    }
+    t.indices.reserve(t.indices.size() + b.indices.size());
+    for (std::uint32_t ix : b.indices) t.indices.push_back((std::uint32_t)base + ix);
+}
+
 }  // namespace
 
 struct HipDevice::Impl {
@@ -325,7 +347,12 @@ std::unique_ptr<HipDevice> CreateHipDevice(const HipDeviceOptions& options) {
 BufferHandle HipDevice::CreateBuffer(const BufferDesc& d) {
     std::lock_guard lock(impl_->mu);
     Buffer b;
-    b.data.assign(d.size, 0);
+    try {
+        b.data.assign(d.size, 0);
+    } catch (const std::bad_alloc&) {
+        impl_->report("CreateBuffer: out of host memory for " + std::to_string(d.size) + " B");
+        return BufferHandle{0};
+    }
     const std::uint64_t id = impl_->next_id++;
     impl_->buffers.emplace(id, std::move(b));
     return BufferHandle{id};
@@ -369,8 +396,13 @@ AccelStructHandle HipDevice::CreateBLAS(const BLASDesc& d) {
             return AccelStructHandle{0};
         }
     Accel a;
-    a.positions.assign(d.vertex_positions, d.vertex_positions + (std::size_t)d.vertex_count * 3u);
-    a.indices.assign(d.indices, d.indices + d.index_count);
+    try {
+        a.positions.assign(d.vertex_positions, d.vertex_positions + (std::size_t)d.vertex_count * 3u);
+        a.indices.assign(d.indices, d.indices + d.index_count);
+    } catch (const std::bad_alloc&) {
+        I.report("CreateBLAS: out of host memory");
+        return AccelStructHandle{0};
+    }
     const std::uint64_t id = I.next_id++;
     I.accels.emplace(id, std::move(a));
     return AccelStructHandle{id};
@@ -382,37 +414,27 @@ AccelStructHandle HipDevice::CreateTLAS(const TLASDesc& d) {
     if (d.instances.empty()) return AccelStructHandle{0};
     // The instances flattened into one mesh, each vertex through its row-major
     // 3x4 transform (the software device instances BLAS scenes with the same
-    // matrix, SoftwareDevice.cpp:379-421).  An identity transform keeps the
-    // BLAS's vertices bit for bit.
+    // matrix, SoftwareDevice.cpp:379-421).
     Accel t;
     t.tlas = true;
-    for (const TLASInstance& in : d.instances) {
-        const Accel* b = I.accel(in.blas.id);
-        if (!b || b->tlas) {
-            std::fprintf(stderr, "[dmt_rhi] CreateTLAS: instance references unknown BLAS id=%llu\n",
-                         (unsigned long long)in.blas.id);
-            continue;
-        }
-        if (in.mask == 0u) continue;   // never intersected by a ray mask of 0xFF
-        const std::size_t base = t.positions.size() / 3u;
-        if (base + b->positions.size() / 3u > 0xFFFFFFFFull) {
-            I.report("CreateTLAS: more than 2^32 vertices");
-            return AccelStructHandle{0};
-        }
-        static const float kIdentity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-        const float* m = in.transform;
-        const bool identity = std::memcmp(m, kIdentity, sizeof(kIdentity)) == 0;
-        for (std::size_t v = 0; v < b->positions.size(); v += 3) {
-            const float x = b->positions[v], y = b->positions[v + 1], z = b->positions[v + 2];
-            if (identity) {
-                t.positions.insert(t.positions.end(), {x, y, z});
-            } else {
-                t.positions.insert(t.positions.end(), {m[0] * x + m[1] * y + m[2] * z + m[3],
-                                                       m[4] * x + m[5] * y + m[6] * z + m[7],
-                                                       m[8] * x + m[9] * y + m[10] * z + m[11]});
+    try {
+        for (const TLASInstance& in : d.instances) {
+            const Accel* b = I.accel(in.blas.id);
+            if (!b || b->tlas) {
+                std::fprintf(stderr, "[dmt_rhi] CreateTLAS: instance references unknown BLAS id=%llu\n",
+                             (unsigned long long)in.blas.id);
+                continue;
             }
+            if (in.mask == 0u) continue;   // never intersected by a ray mask of 0xFF
+            if (t.positions.size() / 3u + b->positions.size() / 3u > 0xFFFFFFFFull) {
+                I.report("CreateTLAS: more than 2^32 vertices");
+                return AccelStructHandle{0};
+            }
+            append_instance(t, *b, in.transform);
         }
-        for (std::uint32_t ix : b->indices) t.indices.push_back((std::uint32_t)base + ix);
+    } catch (const std::bad_alloc&) {
+        I.report("CreateTLAS: out of host memory");
+        return AccelStructHandle{0};
     }
     const std::uint64_t id = I.next_id++;
     I.accels.emplace(id, std::move(t));
@@ -463,7 +485,13 @@ bool HipDevice::WriteTexture(TextureHandle h, const void* src, std::size_t size)
     Texture* t = I.texture(h.id);
     if (!t || !src) return false;
     const std::size_t channels = (std::size_t)t->w * t->h * 4u;
-    std::vector<float> rgba(channels, 0.0f);
+    std::vector<float> rgba;
+    try {
+        rgba.assign(channels, 0.0f);
+    } catch (const std::bad_alloc&) {
+        I.report("WriteTexture: out of host memory");
+        return false;
+    }
     if (t->format == TextureFormat::RGBA8_UNORM) {
         // one byte per channel, read as byte / 255 (SoftwareDevice.cpp:497-523)
         if (size != channels) return false;
@@ -719,6 +747,14 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
 }
 
 void HipDevice::dispatch(const HipCommandBuffer& cb) {
+    try {
+        dispatch_impl(cb);
+    } catch (const std::bad_alloc&) {
+        impl_->report("pathtrace: out of host memory");
+    }
+}
+
+void HipDevice::dispatch_impl(const HipCommandBuffer& cb) {
     std::lock_guard lock(impl_->mu);
     Impl& I = *impl_;
     auto pl = I.pipelines.find(cb.pipeline_.id);

@@ -253,6 +253,7 @@ public:
 private:
     friend class HipCommandBuffer;
     void dispatch(const HipCommandBuffer& cb);
+    void dispatch_impl(const HipCommandBuffer& cb);
     void clear_texture(TextureHandle t, const float rgba[4]);
     void copy_buffer_to_texture(BufferHandle src, TextureHandle dst);
     struct Impl;
