@@ -615,15 +615,51 @@ dmt_status dmt_tonemap(dmt_scene* s, const dmt_tonemap_params* p, void* d_out) {
     if (st) return st;
     if (!s->accum) return fail(DMT_ERR_INVALID_ARG, "nothing rendered yet");
     if (reinterpret_cast<uintptr_t>(d_out) % 16u) return fail(DMT_ERR_INVALID_ARG, "d_out_rgba must be 16-byte aligned");
+    if (p->mode > DMT_TONEMAP_HDR) return fail(DMT_ERR_INVALID_ARG, "unknown tonemap mode");
     const float exposure = (std::isfinite(p->exposure) && p->exposure > 0.0f) ? p->exposure : 1.0f;
     hipStream_t stream = s->dev->stream;
     hipError_t e = hipSuccess;
     if (s->last_set >= 0) e = hipStreamWaitEvent(stream, s->acc_done[s->last_set], 0);
     if (e == hipSuccess)
-        e = launch_tonemap(s->accum, s->acc_w * s->acc_h, exposure, p->op, static_cast<float4*>(d_out), stream);
+        e = launch_tonemap(s->accum, s->acc_w * s->acc_h, exposure, p->op, p->mode, static_cast<float4*>(d_out),
+                           stream);
     // the next frame's accumulation pass waits for this read
     if (e == hipSuccess && s->last_set >= 0) e = hipEventRecord(s->acc_done[s->last_set], stream);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "tonemap");
+    return DMT_OK;
+}
+
+dmt_status dmt_render_gbuffer(dmt_scene* s, const dmt_frame_params* f, const dmt_gbuffer_params* p) {
+    if (!s || !p) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if ((st = check_frame(f))) return st;
+    GbufferArgs A{};
+    A.F = frame_args(f);
+    std::memcpy(A.vp_curr, p->curr_view_proj, sizeof(A.vp_curr));
+    std::memcpy(A.vp_prev, p->prev_view_proj, sizeof(A.vp_prev));
+    A.stride = p->pixel_stride;
+    float* const outs[kGbOutputs] = {p->depth,           p->motion,    p->normal,
+                                     p->albedo,          p->specular_albedo, p->roughness,
+                                     p->specular_hit_distance, p->cloud_transmittance};
+    static constexpr uint32_t kFloats[kGbOutputs] = {1u, 2u, 4u, 4u, 4u, 1u, 1u, 1u};
+    bool any = false;
+    for (uint32_t k = 0; k < kGbOutputs; ++k) {
+        A.out[k] = outs[k];
+        if (!outs[k]) continue;
+        any = true;
+        if (A.stride < kFloats[k])
+            return fail(DMT_ERR_INVALID_ARG, "pixel_stride " + std::to_string(A.stride) + " is below the " +
+                                                 std::to_string(kFloats[k]) + " floats output " + std::to_string(k) +
+                                                 " writes");
+        if (reinterpret_cast<uintptr_t>(outs[k]) % 4u) return fail(DMT_ERR_INVALID_ARG, "outputs must be float-aligned");
+    }
+    if (!any) return DMT_OK;
+    bool resized = false;
+    if (s->wf[0].capacity == 0u && (st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
+    // on the device stream, as frame set 0, whose traversal spill area it shares
+    hipError_t e = launch_gbuffer(s->view, A, s->wf[0], s->dev->stream);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "gbuffer");
     return DMT_OK;
 }
 

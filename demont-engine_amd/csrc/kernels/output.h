@@ -1,5 +1,7 @@
 // Display output of a frame: exposure, tone mapping and the sRGB transfer
-// function applied to the accumulated mean, written as RGBA32F with alpha 1.
+// function applied to the accumulated mean, written as RGBA32F with alpha 1
+// (or, for a denoiser's colour input, the tone-mapped linear image or the
+// clamped HDR mean).
 // This is the output stage of the reference's software path-trace kernel
 // (src/rhi_software/SoftwareTracer.cpp:995-1007: SrgbOetf(TonemapDispatch(
 // avg * exposure, op)) into the slot-0 output texture), run as its own
@@ -21,7 +23,15 @@ enum TonemapOp : uint32_t {
     kTonemapLinear = 4,
 };
 
-hipError_t launch_tonemap(const float4* accum, uint32_t pixels, float exposure, uint32_t op, float4* out,
-                          hipStream_t stream);
+// What the pass writes: the display image, the tone-mapped linear image, or
+// the clamped HDR mean.
+enum TonemapMode : uint32_t {
+    kModeDisplay = 0,
+    kModeLinear = 1,
+    kModeHdr = 2,
+};
+
+hipError_t launch_tonemap(const float4* accum, uint32_t pixels, float exposure, uint32_t op, uint32_t mode,
+                          float4* out, hipStream_t stream);
 
 }  // namespace dmt

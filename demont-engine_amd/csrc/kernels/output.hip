@@ -75,10 +75,18 @@ __device__ __forceinline__ float srgb_oetf(float x) {
 }
 
 __global__ __launch_bounds__(256) void k_tonemap(const float4* __restrict__ accum, uint32_t pixels, float exposure,
-                                                 uint32_t op, float4* __restrict__ out) {
+                                                 uint32_t op, uint32_t mode, float4* __restrict__ out) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pixels; i += stride) {
-        const float4 a = accum[i];
+        float4 a = accum[i];
+        if (mode != kModeDisplay) {
+            // hdr_src = max(mean, 0) (PathTrace.slang:7800-7801)
+            a = make_float4(fmaxf(a.x, 0.0f), fmaxf(a.y, 0.0f), fmaxf(a.z, 0.0f), a.w);
+            if (mode == kModeHdr) {
+                out[i] = make_float4(a.x, a.y, a.z, 1.0f);
+                continue;
+            }
+        }
         c3 c{a.x * exposure, a.y * exposure, a.z * exposure};
         switch (op) {
             case kTonemapAgx: c = agx(c); break;
@@ -89,17 +97,18 @@ __global__ __launch_bounds__(256) void k_tonemap(const float4* __restrict__ accu
             case kTonemapLinear: c = c3{clamp01(c.x), clamp01(c.y), clamp01(c.z)}; break;
             default: c = c3{aces(c.x), aces(c.y), aces(c.z)}; break;
         }
-        out[i] = make_float4(srgb_oetf(c.x), srgb_oetf(c.y), srgb_oetf(c.z), 1.0f);
+        out[i] = mode == kModeLinear ? make_float4(c.x, c.y, c.z, 1.0f)
+                                     : make_float4(srgb_oetf(c.x), srgb_oetf(c.y), srgb_oetf(c.z), 1.0f);
     }
 }
 
 }  // namespace
 
-hipError_t launch_tonemap(const float4* accum, uint32_t pixels, float exposure, uint32_t op, float4* out,
-                          hipStream_t stream) {
+hipError_t launch_tonemap(const float4* accum, uint32_t pixels, float exposure, uint32_t op, uint32_t mode,
+                          float4* out, hipStream_t stream) {
     if (pixels == 0) return hipSuccess;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)pixels + 255u) / 256u, 1u << 16);
-    hipLaunchKernelGGL(k_tonemap, dim3(blocks), dim3(256), 0, stream, accum, pixels, exposure, op, out);
+    hipLaunchKernelGGL(k_tonemap, dim3(blocks), dim3(256), 0, stream, accum, pixels, exposure, op, mode, out);
     return hipGetLastError();
 }
 

@@ -140,4 +140,27 @@ struct HitOut {
 hipError_t launch_trace_query(const SceneView& S, const float4* rays, uint32_t n, HitOut* out, Wavefront& W,
                               hipStream_t stream);
 
+// The denoiser inputs of a frame (PathTrace.slang:5463-5648): one closest-hit
+// query per pixel along the pixel-centre camera ray (no jitter), and per
+// pixel the linear depth, the reprojected motion and the primary surface's
+// normal, albedo, specular F0, roughness and specular hit distance.
+enum GbufferOut : uint32_t {
+    kGbDepth = 0,            // 1 float
+    kGbMotion = 1,           // 2
+    kGbNormal = 2,           // 4
+    kGbAlbedo = 3,           // 4
+    kGbSpecularAlbedo = 4,   // 4
+    kGbRoughness = 5,        // 1
+    kGbSpecularDistance = 6, // 1
+    kGbCloudTrans = 7,       // 1
+    kGbOutputs = 8,
+};
+struct GbufferArgs {
+    FrameArgs F;
+    float vp_curr[16], vp_prev[16];   // column-major: clip = M * (p, 1)
+    float* out[kGbOutputs];           // null: not written
+    uint32_t stride;                  // floats from one pixel to the next
+};
+hipError_t launch_gbuffer(const SceneView& S, const GbufferArgs& A, Wavefront& W, hipStream_t stream);
+
 }  // namespace dmt

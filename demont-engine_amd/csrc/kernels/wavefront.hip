@@ -440,6 +440,79 @@ __global__ void __launch_bounds__(kBlock) k_trace_query(SceneView S, const float
     }
 }
 
+// clip = M * (p, 1) with M column-major (mul(M, v) of PathTrace.slang:5517-5518
+// on the engine's glm::value_ptr layout, Engine.cpp:8258-8259).
+__device__ __forceinline__ float4 project(const float* m, f3 p) {
+    return make_float4(((m[0] * p.x + m[4] * p.y) + m[8] * p.z) + m[12], ((m[1] * p.x + m[5] * p.y) + m[9] * p.z) + m[13],
+                       ((m[2] * p.x + m[6] * p.y) + m[10] * p.z) + m[14],
+                       ((m[3] * p.x + m[7] * p.y) + m[11] * p.z) + m[15]);
+}
+
+__device__ __forceinline__ float2 to_pixel(float4 c, float w, float h) {
+    const float cw = fmaxf(c.w, 1e-6f);
+    return make_float2(((c.x / cw) * 0.5f + 0.5f) * w, ((-c.y / cw) * 0.5f + 0.5f) * h);
+}
+
+// One pixel-centre ray per thread (grid-stride): the G-buffer writes of
+// PathTrace.slang:5463-5648 with the sky sentinels of a miss.
+__global__ void __launch_bounds__(kBlock) k_gbuffer(SceneView S, GbufferArgs A, uint32_t* spill, uint32_t spill_depth) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    const FrameArgs& F = A.F;
+    const uint32_t n = F.width * F.height;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    Stack stk = make_stack(lds, spill, spill_depth);
+    TraceCounters tc{0u, 0u, 0u};
+    const f3 fwdn = normalize(F.fwd);
+    const float W = (float)F.width, H = (float)F.height;
+    for (uint32_t p = gid; p < n; p += stride) {
+        const uint32_t x = p % F.width, y = p / F.width;
+        const float ux = (((float)x + 0.5f) / W) * 2.0f - 1.0f;
+        float uy = (((float)y + 0.5f) / H) * 2.0f - 1.0f;
+        uy = -uy;
+        const f3 d = normalize((F.fwd + F.right * ((ux * F.aspect) * F.fovtan)) + F.up * (uy * F.fovtan));
+        const HitRec h = trace_closest<false>(S, F.pos, d, stk, tc);
+        f3 world, nrm{0.0f, 0.0f, 0.0f}, alb{0.0f, 0.0f, 0.0f}, f0{0.0f, 0.0f, 0.0f};
+        float depth, rough = 1.0f, dist = 0.0f;
+        if (h.code != kMiss) {
+            const Surface sf = surface_at(S, F.pos, d, h);
+            world = F.pos + d * h.t;
+            nrm = sf.n;
+            alb = sf.albedo;
+            depth = h.t * dot(d, fwdn);
+            const float r = fminf(fmaxf(sf.rough, 0.0f), 1.0f);
+            if (sf.mat == 1u) f0 = sf.albedo;                            // metal
+            else if (sf.mat == 2u) f0 = f3{0.04f, 0.04f, 0.04f};         // dielectric
+            rough = sf.mat == 0u ? 1.0f : r;                             // Lambert: no specular lobe
+            dist = h.t * (1.0f - r);
+        } else {
+            world = F.pos + d * 1000.0f;   // far point: the sky still moves
+            depth = 1.0e10f;
+        }
+        const float2 cur = to_pixel(project(A.vp_curr, world), W, H);
+        const float2 pre = to_pixel(project(A.vp_prev, world), W, H);
+        float* const* o = A.out;
+        const size_t b = (size_t)p * A.stride;
+        if (o[kGbDepth]) o[kGbDepth][b] = depth;
+        if (o[kGbMotion]) {
+            o[kGbMotion][b] = pre.x - cur.x;
+            o[kGbMotion][b + 1] = pre.y - cur.y;
+        }
+        const f3 v4[3] = {nrm, alb, f0};
+        for (int k = 0; k < 3; ++k) {
+            float* q = o[kGbNormal + k];
+            if (!q) continue;
+            q[b] = v4[k].x;
+            q[b + 1] = v4[k].y;
+            q[b + 2] = v4[k].z;
+            q[b + 3] = 0.0f;
+        }
+        if (o[kGbRoughness]) o[kGbRoughness][b] = rough;
+        if (o[kGbSpecularDistance]) o[kGbSpecularDistance][b] = dist;
+        if (o[kGbCloudTrans]) o[kGbCloudTrans][b] = 1.0f;   // no clouds: frame_cloud_trans stays 1
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -562,6 +635,13 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
     }
     if (o.accum_after) (void)hipStreamWaitEvent(stream, o.accum_after, 0);
     hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum);
+    return hipGetLastError();
+}
+
+hipError_t launch_gbuffer(const SceneView& S, const GbufferArgs& A, Wavefront& W, hipStream_t stream) {
+    const uint32_t n = A.F.width * A.F.height;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gbuffer, dim3(grid_for(n)), dim3(kBlock), 0, stream, S, A, W.spill, W.spill_depth);
     return hipGetLastError();
 }
 

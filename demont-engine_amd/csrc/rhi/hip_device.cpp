@@ -92,6 +92,19 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     r.env_intensity = f32(p, kEnvIntensity);
     r.tonemap_op = u32(p, kTonemapParams);
     r.motion_blur = f32(p, kMotionBlurParams) != 0.0f;
+    r.denoiser = u32(p, kDenoiserEnabled) != 0u;
+    r.write_normal = u32(p, kWriteNormalGbuffer) != 0u;
+    r.write_albedo = u32(p, kWriteAlbedoGbuffer) != 0u;
+    r.write_specular_albedo = u32(p, kWriteSpecularAlbedo) != 0u;
+    r.write_roughness = u32(p, kWriteRoughness) != 0u;
+    r.write_specular_distance = u32(p, kWriteSpecularHitDistance) != 0u;
+    r.write_hdr_aux = u32(p, kWriteHdrAux) != 0u;
+    r.hdr_pipeline = f32(p, kW2jRow0 + 12) > 0.5f;   // r_hdr_pipeline (Engine.cpp:8667-8669)
+    r.clouds_mode = u32(p, kCloudsRuntime);
+    for (int i = 0; i < 16; ++i) {
+        r.curr_view_proj[i] = f32(p, kCurrViewProj + 4 * i);
+        r.prev_view_proj[i] = f32(p, kPrevViewProj + 4 * i);
+    }
 
     std::string feats;
     auto need = [&feats](bool on, const char* what) {
@@ -121,10 +134,12 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     need(r.motion_blur, "motion blur");                                                      // :1029
     need(u32(p, kMatOverrideCount) != 0u, "material overrides");                             // :1164
     need(f32(p, kSmokeParams) != 0.0f, "smoke volumes");                                     // :981
-    need(u32(p, kDenoiserEnabled) || u32(p, kWriteHdrAux) || u32(p, kWriteNormalGbuffer) ||
-             u32(p, kWriteAlbedoGbuffer) || u32(p, kWriteShadowVis) || u32(p, kWriteSpecularAlbedo) ||
-             u32(p, kWriteRoughness) || u32(p, kWriteSpecularHitDistance),
-         "denoiser G-buffer outputs");
+    need(u32(p, kWriteShadowVis) != 0u, "SIGMA shadow visibility (write_shadow_vis)");       // :5693-5702
+    // the denoiser's colour input is the frame's radiance (:7800); this
+    // backend forms it from the mean, which is that radiance when the frame
+    // resets the accumulation (the engine resets every frame while a denoiser
+    // runs, Engine.cpp:7834)
+    need(r.denoiser && r.write_hdr_aux && !F.reset_accum, "denoiser colour input without reset_accum");
     if (r.tlas_present) {
         bool tex = false;
         for (int k = 0; k < 4; ++k) tex |= u32(p, kMeshTexIndices + 4 * k) != kPbrNoTexTile;
@@ -786,27 +801,76 @@ void HipDevice::dispatch_impl(const HipCommandBuffer& cb) {
     }
     P.frame.width = acc->w;
     P.frame.height = acc->h;
+    // the denoiser's textures, checked before any work so a refused dispatch changes nothing
+    struct Aux {
+        std::uint32_t slot;
+        bool on;
+        Texture* t;
+    };
+    Aux aux[] = {{slot::kDepthTexture, P.denoiser != 0u, nullptr},
+                 {slot::kMotionTexture, P.denoiser != 0u, nullptr},
+                 {slot::kNormalTexture, P.denoiser && P.write_normal, nullptr},
+                 {slot::kAlbedoTexture, P.denoiser && P.write_albedo, nullptr},
+                 {slot::kSpecularAlbedoTexture, P.denoiser && P.write_specular_albedo, nullptr},
+                 {slot::kRoughnessTexture, P.denoiser && P.write_roughness, nullptr},
+                 {slot::kSpecularDistanceTexture, P.denoiser && P.write_specular_distance, nullptr},
+                 {slot::kCloudTransTexture, P.denoiser && P.clouds_mode == 0u, nullptr},
+                 {slot::kDenoiseColorTexture, P.write_hdr_aux != 0u, nullptr}};
+    for (Aux& a : aux) {
+        if (!a.on) continue;
+        a.t = I.texture(cb.textures_[a.slot]);
+        if (!a.t || !a.t->dev) continue;   // not bound: not written
+        if (a.t->w != acc->w || a.t->h != acc->h) {
+            I.report("pathtrace: storage texture at slot " + std::to_string(a.slot) + " is " + std::to_string(a.t->w) +
+                     "x" + std::to_string(a.t->h) + ", the frame " + std::to_string(acc->w) + "x" +
+                     std::to_string(acc->h));
+            return;
+        }
+    }
     if (!I.ensure_scene(cb, P)) return;
     if (I.scene_accum != acc->dev) {
         if (!I.check(dmt_scene_bind_accum(I.scene, acc->dev, acc->w, acc->h), "dmt_scene_bind_accum")) return;
         I.scene_accum = acc->dev;
     }
+    auto written = [](Texture* t) {
+        t->host_valid = false;
+        t->gen++;
+    };
+    if (P.denoiser) {
+        dmt_gbuffer_params gp{};
+        std::memcpy(gp.curr_view_proj, P.curr_view_proj, sizeof(gp.curr_view_proj));
+        std::memcpy(gp.prev_view_proj, P.prev_view_proj, sizeof(gp.prev_view_proj));
+        gp.pixel_stride = 4;   // RGBA32F textures
+        float** dst[] = {&gp.depth, &gp.motion, &gp.normal, &gp.albedo, &gp.specular_albedo, &gp.roughness,
+                         &gp.specular_hit_distance, &gp.cloud_transmittance};
+        for (int k = 0; k < 8; ++k)
+            if (aux[k].on && aux[k].t && aux[k].t->dev) *dst[k] = aux[k].t->dev;
+        if (!I.check(dmt_render_gbuffer(I.scene, &P.frame, &gp), "dmt_render_gbuffer")) return;
+        for (int k = 0; k < 8; ++k)
+            if (*dst[k]) written(aux[k].t);
+    }
     if (!I.check(dmt_render(I.scene, &P.frame), "dmt_render")) return;
-    acc->host_valid = false;
-    acc->gen++;
+    written(acc);
+    // exposure_state[0] scales the mean; invalid values read as 1
+    // (SoftwareTracer.cpp:788-801)
+    float exposure = 1.0f;
+    if (const Buffer* eb = I.buffer(cb.buffers_[slot::kExposureBuffer]);
+        eb && eb->data.size() >= cb.buffer_offsets_[slot::kExposureBuffer] + 4u)
+        std::memcpy(&exposure, eb->data.data() + cb.buffer_offsets_[slot::kExposureBuffer], 4);
+    dmt_tonemap_params tp{};
+    tp.exposure = exposure;
+    tp.op = P.tonemap_op;
+    if (Texture* dc = aux[8].t; dc && dc->dev) {
+        // the denoiser's colour input: raw HDR or tone-mapped, no OETF
+        // (PathTrace.slang:7797-7811)
+        tp.mode = P.hdr_pipeline ? DMT_TONEMAP_HDR : DMT_TONEMAP_LINEAR;
+        if (!I.check(dmt_tonemap(I.scene, &tp, dc->dev), "dmt_tonemap(denoiser colour)")) return;
+        written(dc);
+    }
     if (out && out->dev) {
-        // exposure_state[0] scales the mean; invalid values read as 1
-        // (SoftwareTracer.cpp:788-801)
-        float exposure = 1.0f;
-        if (const Buffer* eb = I.buffer(cb.buffers_[slot::kExposureBuffer]);
-            eb && eb->data.size() >= cb.buffer_offsets_[slot::kExposureBuffer] + 4u)
-            std::memcpy(&exposure, eb->data.data() + cb.buffer_offsets_[slot::kExposureBuffer], 4);
-        dmt_tonemap_params tp{};
-        tp.exposure = exposure;
-        tp.op = P.tonemap_op;
+        tp.mode = DMT_TONEMAP_DISPLAY;
         if (!I.check(dmt_tonemap(I.scene, &tp, out->dev), "dmt_tonemap")) return;
-        out->host_valid = false;
-        out->gen++;
+        written(out);
     }
 }
 

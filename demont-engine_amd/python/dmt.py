@@ -126,7 +126,18 @@ class BatchParams(C.Structure):
 
 
 class TonemapParams(C.Structure):
-    _fields_ = [("exposure", C.c_float), ("op", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("exposure", C.c_float), ("op", C.c_uint32), ("mode", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+TONEMAP_DISPLAY, TONEMAP_LINEAR, TONEMAP_HDR = 0, 1, 2
+
+
+class GbufferParams(C.Structure):
+    _fields_ = [("curr_view_proj", C.c_float * 16), ("prev_view_proj", C.c_float * 16),
+                ("pixel_stride", C.c_uint32), ("reserved", C.c_uint32),
+                ("depth", C.c_void_p), ("motion", C.c_void_p), ("normal", C.c_void_p), ("albedo", C.c_void_p),
+                ("specular_albedo", C.c_void_p), ("roughness", C.c_void_p), ("specular_hit_distance", C.c_void_p),
+                ("cloud_transmittance", C.c_void_p)]
 
 
 class DmtError(RuntimeError):
@@ -165,6 +176,7 @@ def lib() -> C.CDLL:
         "dmt_accum_device_ptr": (st, [vp, C.POINTER(vp)]),
         "dmt_scene_bind_accum": (st, [vp, vp, u32, u32]),
         "dmt_tonemap": (st, [vp, C.POINTER(TonemapParams), vp]),
+        "dmt_render_gbuffer": (st, [vp, C.POINTER(FrameParams), C.POINTER(GbufferParams)]),
         "dmt_trace_closest": (st, [vp, fp, u32, vp]),
         "dmt_scene_stats": (st, [vp, C.POINTER(Stats)]),
         "dmt_scene_reset_stats": (st, [vp]),
@@ -371,10 +383,23 @@ class Scene:
         """Render into caller-owned device memory (width*height float4); 0 unbinds."""
         _check(lib().dmt_scene_bind_accum(self._h, C.c_void_p(d_rgba or None), width, height))
 
-    def tonemap(self, d_out: int, exposure: float = 1.0, op: int = 0):
-        """Display image (sRGB of tonemapped mean * exposure, alpha 1) into device memory."""
-        p = TonemapParams(exposure, op)
+    def tonemap(self, d_out: int, exposure: float = 1.0, op: int = 0, mode: int = TONEMAP_DISPLAY):
+        """Display image (sRGB of tonemapped mean * exposure, alpha 1), or the
+        denoiser colour input (mode LINEAR / HDR), into device memory."""
+        p = TonemapParams(exposure, op, mode)
         _check(lib().dmt_tonemap(self._h, C.byref(p), C.c_void_p(d_out)))
+
+    def gbuffer(self, frame: FrameParams, vp_curr, vp_prev, stride: int = 4, **outs):
+        """Denoiser inputs into device memory: outs maps depth / motion / normal /
+        albedo / specular_albedo / roughness / specular_hit_distance /
+        cloud_transmittance to device addresses (stride floats per pixel)."""
+        p = GbufferParams()
+        p.curr_view_proj[:] = [float(v) for v in np.asarray(vp_curr, np.float32).reshape(16)]
+        p.prev_view_proj[:] = [float(v) for v in np.asarray(vp_prev, np.float32).reshape(16)]
+        p.pixel_stride = stride
+        for k, v in outs.items():
+            setattr(p, k, v)
+        _check(lib().dmt_render_gbuffer(self._h, C.byref(frame), C.byref(p)))
 
     def trace(self, origins: np.ndarray, dirs: np.ndarray) -> np.ndarray:
         n = origins.shape[0]

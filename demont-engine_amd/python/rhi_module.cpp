@@ -45,6 +45,17 @@ py::dict decoded(const PathtracePush& P) {
     d["env_intensity"] = P.env_intensity;
     d["tonemap_op"] = P.tonemap_op;
     d["motion_blur"] = P.motion_blur;
+    d["denoiser"] = P.denoiser;
+    d["write_normal"] = P.write_normal;
+    d["write_albedo"] = P.write_albedo;
+    d["write_specular_albedo"] = P.write_specular_albedo;
+    d["write_roughness"] = P.write_roughness;
+    d["write_specular_distance"] = P.write_specular_distance;
+    d["write_hdr_aux"] = P.write_hdr_aux;
+    d["hdr_pipeline"] = P.hdr_pipeline;
+    d["clouds_mode"] = P.clouds_mode;
+    d["curr_view_proj"] = std::vector<float>(P.curr_view_proj, P.curr_view_proj + 16);
+    d["prev_view_proj"] = std::vector<float>(P.prev_view_proj, P.prev_view_proj + 16);
     d["ignored"] = P.ignored;
     return d;
 }
@@ -60,9 +71,18 @@ public:
         dev_ = std::make_unique<HipDevice>(o);
         cb_ = dev_->AcquireCommandBuffer();
     }
-    HipDevice& d() { return *dev_; }
-    HipCommandBuffer& cb() { return *cb_; }
-    void close() { dev_.reset(); }
+    HipDevice& d() {
+        if (!dev_) throw py::value_error("the device is closed");
+        return *dev_;
+    }
+    HipCommandBuffer& cb() {
+        if (!dev_) throw py::value_error("the device is closed");
+        return *cb_;
+    }
+    void close() {
+        dev_.reset();
+        cb_ = nullptr;
+    }
 
 private:
     std::unique_ptr<HipDevice> dev_;

@@ -195,18 +195,59 @@ dmt_status dmt_accum_device_ptr(dmt_scene* scene, void** out_ptr);
 dmt_status dmt_scene_bind_accum(dmt_scene* scene, void* d_rgba, uint32_t width, uint32_t height);
 
 /* Display output (r_tonemap_op, exposure_state[0]). */
+#define DMT_TONEMAP_DISPLAY  0u   /* sRGB OETF of tonemap(mean * exposure)     */
+#define DMT_TONEMAP_LINEAR   1u   /* tonemap(max(mean, 0) * exposure), no OETF */
+#define DMT_TONEMAP_HDR      2u   /* max(mean, 0): the raw HDR image           */
 typedef struct dmt_tonemap_params {
     float    exposure;          /* scales the mean; <= 0 or non-finite -> 1  */
     uint32_t op;                /* 0 ACES, 1 AgX, 2 Khronos PBR Neutral,
                                    3 Reinhard, 4 linear clamp; others -> 0   */
-    uint32_t reserved[2];
+    uint32_t mode;              /* DMT_TONEMAP_*                              */
+    uint32_t reserved;
 } dmt_tonemap_params;
-/* Replaces the output stage of RunPathTraceKernel (SoftwareTracer.cpp:788-801,
- * 995-1007), which writes the slot-0 output texture: rgb = sRGB OETF of
- * tonemap(mean * exposure), alpha = 1, as width*height float4 into device
- * memory d_out_rgba.  Stream-ordered after the scene's last frame; later
- * frames wait for it before they update the accumulation buffer. */
+/* DMT_TONEMAP_DISPLAY replaces the output stage of RunPathTraceKernel
+ * (SoftwareTracer.cpp:788-801, 995-1007), which writes the slot-0 output
+ * texture; LINEAR and HDR are the denoiser's colour input of the write_hdr_aux
+ * path (PathTrace.slang:7797-7811, r_hdr_pipeline off / on) when each frame
+ * resets the accumulation (the engine resets every frame while a denoiser
+ * runs), so the mean is that frame's radiance.  rgb as the mode says,
+ * alpha = 1, as width*height float4 into device memory d_out_rgba.
+ * Stream-ordered after the scene's last frame; later frames wait for it
+ * before they update the accumulation buffer. */
 dmt_status dmt_tonemap(dmt_scene* scene, const dmt_tonemap_params* params, void* d_out_rgba);
+
+/* Denoiser inputs of a frame (PathTrace.slang:5463-5648, written when the
+ * engine sets denoiser_enabled): a closest-hit query along every pixel-centre
+ * camera ray (no jitter) and, per pixel, into device memory (NULL: not
+ * written; `pixel_stride` floats from one pixel to the next, >= the floats an
+ * output writes, 4 for RGBA32F textures):
+ *   depth                 1  view depth t * dot(d, normalize(forward)); 1e10 on a miss
+ *   motion                2  previous - current pixel position of the hit point
+ *                            (of the point 1000 along the ray on a miss)
+ *   normal, albedo        4  the hit's normal / albedo, 0; zero on a miss
+ *   specular_albedo       4  F0: metal albedo, dielectric 0.04, Lambert 0; 0
+ *   roughness             1  saturate(roughness); 1 for Lambert and misses
+ *   specular_hit_distance 1  t * (1 - saturate(roughness)); 0 on a miss
+ *   cloud_transmittance   1  1 (no volumetric clouds in this backend)
+ * Pixel positions are ((x / w) * 0.5 + 0.5) * width, ((-y / w) * 0.5 + 0.5) *
+ * height of clip = M * (p, 1), M column-major as the engine packs
+ * curr_view_proj / prev_view_proj (Engine.cpp:8257-8259).  Stream-ordered on
+ * the device stream. */
+typedef struct dmt_gbuffer_params {
+    float    curr_view_proj[16];
+    float    prev_view_proj[16];
+    uint32_t pixel_stride;
+    uint32_t reserved;
+    float*   depth;
+    float*   motion;
+    float*   normal;
+    float*   albedo;
+    float*   specular_albedo;
+    float*   roughness;
+    float*   specular_hit_distance;
+    float*   cloud_transmittance;
+} dmt_gbuffer_params;
+dmt_status dmt_render_gbuffer(dmt_scene* scene, const dmt_frame_params* frame, const dmt_gbuffer_params* params);
 
 /* traceScene() closest-hit query (PathTrace.slang:2331-2693) for n rays,
  * 8 floats per ray (ox, oy, oz, -, dx, dy, dz, -); host in, host out. */

@@ -26,7 +26,12 @@
  *   - the frame renders into the accum texture's device memory (slot 1, bound
  *     with dmt_scene_bind_accum, so the running mean survives a rebuild);
  *   - the output texture (slot 0) receives sRGB(tonemap(mean * exposure))
- *     through dmt_tonemap, exposure = exposure_state[0] of buffer slot 6.
+ *     through dmt_tonemap, exposure = exposure_state[0] of buffer slot 6;
+ *   - with denoiser_enabled, the bound G-buffer textures (depth 3, motion 4,
+ *     normal 8, albedo 9, cloud transmittance 10, specular albedo 11,
+ *     roughness 12, specular hit distance 13, each as its write flag says)
+ *     receive dmt_render_gbuffer's outputs, and with write_hdr_aux the
+ *     denoiser colour texture (slot 2) the frame's mean, raw or tone-mapped.
  * The light-tree nodes (slot 13) and env CDFs (slots 4/5) the engine uploads
  * are not read: the scene builds both from the lights and the env map with the
  * reference's own construction.  Work is stream-ordered on the device;
@@ -108,11 +113,21 @@ struct FrameContext {
     std::uint32_t frame_index = 0;
 };
 
-/* Bind slots of the engine's "pathtrace" pipeline (Engine.cpp:6867-7217). */
+/* Bind slots of the engine's "pathtrace" pipeline (Engine.cpp:6867-7217;
+ * storage textures and buffers are separate slot spaces). */
 namespace slot {
 constexpr std::uint32_t kOutputTexture = 0;    /* BindStorageTexture(0, swapchain image) */
 constexpr std::uint32_t kAccumTexture = 1;     /* BindStorageTexture(1, accum)           */
+constexpr std::uint32_t kDenoiseColorTexture = 2;   /* denoiser colour input (write_hdr_aux) */
+constexpr std::uint32_t kDepthTexture = 3;     /* denoiser inputs (denoiser_enabled) ...  */
+constexpr std::uint32_t kMotionTexture = 4;
 constexpr std::uint32_t kEnvTexture = 5;       /* BindStorageTexture(5, env map)         */
+constexpr std::uint32_t kNormalTexture = 8;    /* write_normal_gbuffer                   */
+constexpr std::uint32_t kAlbedoTexture = 9;    /* write_albedo_gbuffer                   */
+constexpr std::uint32_t kCloudTransTexture = 10;
+constexpr std::uint32_t kSpecularAlbedoTexture = 11;
+constexpr std::uint32_t kRoughnessTexture = 12;
+constexpr std::uint32_t kSpecularDistanceTexture = 13;
 constexpr std::uint32_t kTlas = 2;             /* BindAccelStruct(2, scene TLAS)         */
 constexpr std::uint32_t kPrimBuffer = 3;       /* BindBuffer(3, primitives)              */
 constexpr std::uint32_t kExposureBuffer = 6;   /* BindBuffer(6, exposure_state)          */
@@ -131,16 +146,16 @@ namespace ptpush {
 constexpr std::size_t kPosFovTan = 0, kFwdAspect = 16, kRightXyz = 32, kUpXyz = 48;
 constexpr std::size_t kFrameIndex = 64, kResetAccum = 68, kMaxBounces = 72, kTlasPresent = 76;
 constexpr std::size_t kPrimCount = 80, kSpp = 84, kDenoiserEnabled = 88, kEnvMapPresent = 92;
-constexpr std::size_t kEnvIntensity = 104;
+constexpr std::size_t kEnvIntensity = 104, kCurrViewProj = 112, kPrevViewProj = 176;
 constexpr std::size_t kHeadBytes = 112;        /* PtPushHead (SoftwareTracer.cpp:38-56) */
-constexpr std::size_t kSunAndMode = 240, kW2jRow1 = 288, kW2jRow2 = 304;
+constexpr std::size_t kSunAndMode = 240, kW2jRow0 = 272, kW2jRow1 = 288, kW2jRow2 = 304;
 constexpr std::size_t kDofParams = 320, kSpectralParams = 352, kVolParams = 368;
 constexpr std::size_t kCloudsP1 = 384, kCloudsP3 = 416, kMoonDirPhase = 448;
 constexpr std::size_t kHdriLightsCount = 752, kWriteNormalGbuffer = 756, kWriteAlbedoGbuffer = 760;
 constexpr std::size_t kWriteHdrAux = 764, kMisEnabled = 768, kAccumParams = 784, kSdfParams = 816;
 constexpr std::size_t kWriteShadowVis = 856, kLightCount = 880, kLightTreeEnabled = 888;
 constexpr std::size_t kWriteSpecularAlbedo = 896, kWriteRoughness = 900, kWriteSpecularHitDistance = 904;
-constexpr std::size_t kRestirEnabled = 944, kSmokeParams = 960, kMotionBlurParams = 1024;
+constexpr std::size_t kRestirEnabled = 944, kSmokeParams = 960, kMotionBlurParams = 1024, kCloudsRuntime = 1072;
 constexpr std::size_t kMeshTexIndices = 1152, kMatOverrideCount = 1200, kTonemapParams = 1600;
 constexpr std::size_t kSize = 1792;            /* sizeof(PtPush) with kMaxHdriLights = kMaxMatOverrides = 8 */
 }  // namespace ptpush
@@ -155,6 +170,15 @@ struct PathtracePush {
     float env_intensity = 1.0f;
     std::uint32_t tonemap_op = 0;   /* r_tonemap_op; values above 4 select ACES */
     std::uint32_t motion_blur = 0;  /* motion_blur_params.x != 0 */
+    /* denoiser inputs (PathTrace.slang:5463-5648, 7797-7811) */
+    std::uint32_t denoiser = 0;               /* denoiser_enabled: depth + motion */
+    std::uint32_t write_normal = 0, write_albedo = 0, write_specular_albedo = 0;
+    std::uint32_t write_roughness = 0, write_specular_distance = 0;
+    std::uint32_t write_hdr_aux = 0;          /* the denoiser's colour input */
+    std::uint32_t hdr_pipeline = 0;           /* w2j_row0.w > 0.5: that input raw, not tone-mapped */
+    std::uint32_t clouds_mode = 0;            /* clouds_runtime.x (0: cloud transmittance written) */
+    float curr_view_proj[16]{};
+    float prev_view_proj[16]{};
     std::string ignored;            /* lenient mode: features asked for and not rendered */
 };
 
@@ -164,8 +188,9 @@ struct PathtracePush {
  * turns on a feature this backend does not render: a procedural / Hosek sky
  * (sun_and_mode.w 2 or 3), the sun or moon above the horizon without an env
  * map, HDRI cluster lights, depth of field, spectral dispersion, participating
- * media, SDF clusters, ReSTIR, motion blur, material overrides, smoke,
- * denoiser G-buffer outputs, or PBR mesh textures.  Lenient decoding renders
+ * media, SDF clusters, ReSTIR, motion blur, material overrides, smoke, the
+ * SIGMA shadow-visibility output, a per-frame denoiser colour input without
+ * reset_accum, or PBR mesh textures.  Lenient decoding renders
  * without them and lists them in out->ignored.  Fields past the pushed size
  * read as zero. */
 bool DecodePathtracePush(const void* data, std::size_t size, bool strict, PathtracePush* out, std::string* error);

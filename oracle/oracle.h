@@ -99,6 +99,12 @@ int orc_trace(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
 /* Same through an exhaustive linear scan (no BVH). */
 int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
 
+/* Denoiser G-buffer of a frame (PathTrace.slang:5463-5648): the pixel-centre
+ * camera ray of every pixel traced once; 16 floats per pixel: depth, motion.xy,
+ * normal.xyz, albedo.xyz, specular albedo (F0).xyz, roughness, specular hit
+ * distance, 0, 0.  vp_curr / vp_prev are column-major (clip = M * (p, 1)). */
+int orc_gbuffer(const orc_scene* s, const orc_frame* f, const float* vp_curr, const float* vp_prev, float* out);
+
 /* Host builders, exposed for tests. */
 double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H,
                          float* out_marginal, float* out_conditional);

@@ -56,6 +56,8 @@ def lib():
         L.orc_trace.argtypes = [vp, fp, u32, vp]
         L.orc_trace.restype = C.c_int
         L.orc_trace_bruteforce.argtypes = [vp, fp, u32, vp]
+        L.orc_gbuffer.restype = C.c_int
+        L.orc_gbuffer.argtypes = [vp, vp, fp, fp, fp]
         L.orc_trace_bruteforce.restype = C.c_int
         L.orc_build_env_cdf.argtypes = [fp, u32, u32, fp, fp]
         L.orc_build_env_cdf.restype = C.c_double
@@ -161,6 +163,17 @@ class OracleScene:
         out = np.zeros(n, HIT_DTYPE)
         fn = lib().orc_trace_bruteforce if brute else lib().orc_trace
         fn(self._h, _fp(rays), n, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def gbuffer(self, frame, vp_curr, vp_prev):
+        """(H, W, 16) float32: depth, motion.xy, normal.xyz, albedo.xyz, F0.xyz,
+        roughness, specular hit distance, 0, 0 (column-major view-projections)."""
+        out = np.zeros((frame.height, frame.width, 16), np.float32)
+        a = np.ascontiguousarray(vp_curr, np.float32).reshape(16)
+        b = np.ascontiguousarray(vp_prev, np.float32).reshape(16)
+        rc = lib().orc_gbuffer(self._h, C.cast(C.byref(frame), C.c_void_p), _fp(a), _fp(b), _fp(out))
+        if rc != 0:
+            raise ValueError("oracle gbuffer failed")
         return out
 
     def env_sample(self, xi):

@@ -1189,6 +1189,73 @@ int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_
     return trace_impl(s, rays, n, out, true);
 }
 
+// G-buffer pass (PathTrace.slang:5463-5648): the pixel-centre ray (5464-5469),
+// the hit's linear depth and surface fields or the sky sentinels (5495-5519),
+// pixel motion by reprojection through the two view-projections (5524-5528),
+// the MetalFX specular guidance fields (5611-5646).
+int orc_gbuffer(const orc_scene* s, const orc_frame* f, const float* vp_curr, const float* vp_prev, float* out) {
+    if (!s || !f || !vp_curr || !vp_prev || !out) return -1;
+    using V = V3<float>;
+    const float W = (float)f->width, H = (float)f->height;
+    const V pos{f->pos_fovtan[0], f->pos_fovtan[1], f->pos_fovtan[2]};
+    const float fovtan = f->pos_fovtan[3], aspect = f->fwd_aspect[3];
+    const V fwd{f->fwd_aspect[0], f->fwd_aspect[1], f->fwd_aspect[2]};
+    const V rt{f->right_xyz[0], f->right_xyz[1], f->right_xyz[2]};
+    const V up{f->up_xyz[0], f->up_xyz[1], f->up_xyz[2]};
+    const V fwdn = normalize(fwd);
+    // mul(M, float4(p, 1)) for the column-major engine matrix
+    auto project = [](const float* m, V p, float o[4]) {
+        for (int i = 0; i < 4; ++i) o[i] = ((m[i] * p.x + m[4 + i] * p.y) + m[8 + i] * p.z) + m[12 + i];
+    };
+    auto pixel = [W, H](const float c[4], float& px, float& py) {
+        const float cw = std::max(c[3], 1e-6f);
+        px = ((c[0] / cw) * 0.5f + 0.5f) * W;
+        py = ((-c[1] / cw) * 0.5f + 0.5f) * H;
+    };
+    Ctr c;
+    for (uint32_t y = 0; y < f->height; ++y) {
+        for (uint32_t x = 0; x < f->width; ++x) {
+            const float ux = (((float)x + 0.5f) / W) * 2.0f - 1.0f;
+            float uy = (((float)y + 0.5f) / H) * 2.0f - 1.0f;
+            uy = -uy;
+            const V d = normalize((fwd + rt * ((ux * aspect) * fovtan)) + up * (uy * fovtan));
+            const Hit<float> h = trace_scene<float>(*s, pos, d, c, false);
+            float* o = out + ((size_t)y * f->width + x) * 16u;
+            for (int k = 0; k < 16; ++k) o[k] = 0.0f;
+            V world;
+            float depth, rough = 1.0f, dist = 0.0f;
+            if (h.hit) {
+                world = pos + d * h.t;
+                depth = h.t * dot(d, fwdn);
+                o[3] = h.n.x; o[4] = h.n.y; o[5] = h.n.z;
+                o[6] = h.albedo.x; o[7] = h.albedo.y; o[8] = h.albedo.z;
+                const float r = clampv(h.rough, 0.0f, 1.0f);
+                if (h.mat == MAT_METAL) {
+                    o[9] = h.albedo.x; o[10] = h.albedo.y; o[11] = h.albedo.z;
+                } else if (h.mat == MAT_DIELECTRIC) {
+                    o[9] = o[10] = o[11] = 0.04f;
+                }
+                rough = h.mat == MAT_LAMBERT ? 1.0f : r;
+                dist = h.t * (1.0f - r);
+            } else {
+                world = pos + d * 1000.0f;
+                depth = 1.0e10f;
+            }
+            float pc[4], pp[4], cx, cy, qx, qy;
+            project(vp_curr, world, pc);
+            project(vp_prev, world, pp);
+            pixel(pc, cx, cy);
+            pixel(pp, qx, qy);
+            o[0] = depth;
+            o[1] = qx - cx;
+            o[2] = qy - cy;
+            o[12] = rough;
+            o[13] = dist;
+        }
+    }
+    return 0;
+}
+
 int orc_tangent(orc_scene* s, const orc_frame* f, uint32_t prim, uint32_t param, const float* dimage, int threads,
                 double* out_dloss, float* out_dimage) {
     if (!s || !f || !dimage || !out_dloss || prim >= s->prim_count || param > 6u) return -1;

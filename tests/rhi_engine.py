@@ -25,7 +25,8 @@ NO_TEX_TILE = 0xFFFFFFFF
 # PtPush byte offsets (Engine.cpp:7243-7820)
 OFF = dict(pos_fovtan=0, fwd_aspect=16, right_xyz=32, up_xyz=48, frame_index=64, reset_accum=68,
            max_bounces=72, tlas_present=76, prim_count=80, spp=84, denoiser_enabled=88, env_map_present=92,
-           halton_jitter=96, env_intensity=104, env_total_luminance=108, sun_and_mode=240, exposure_pad=256,
+           halton_jitter=96, env_intensity=104, env_total_luminance=108, curr_view_proj=112, prev_view_proj=176,
+           sun_and_mode=240, exposure_pad=256,
            w2j_row0=272, w2j_row1=288, w2j_row2=304, dof_params=320, spectral_params=352, vol_params=368,
            clouds_p1=384, clouds_p3=416, moon_dir_phase=448, hdri_lights_count=752, write_normal_gbuffer=756,
            write_albedo_gbuffer=760, write_hdr_aux=764, mis_enabled=768, accum_params=784, bvh_params=800,
@@ -120,7 +121,8 @@ class EngineSim:
                            env_present=self.env_present, env_intensity=self.sd.env_intensity,
                            tlas_present=bool(self.tlas), **kw)
 
-    def frame(self, frame: "dmt.FrameParams", push: bytes | None = None, **kw):
+    def frame(self, frame: "dmt.FrameParams", push: bytes | None = None, textures: dict | None = None, **kw):
+        """One engine frame; `textures` binds further storage textures {slot: handle}."""
         d = self.dev
         out, w, h, _ = d.begin_frame()
         d.bind_pipeline(self.pipeline)
@@ -134,6 +136,8 @@ class EngineSim:
         d.bind_buffer(6, self.exposure)
         d.bind_buffer(12, self.lights)
         d.bind_buffer(13, self.tree)
+        for slot, tex in (textures or {}).items():
+            d.bind_texture(slot, tex)
         d.push_constants(push if push is not None else self.push(frame, **kw))
         d.dispatch((w + 7) // 8, (h + 7) // 8, 1)
         d.submit()
@@ -153,9 +157,9 @@ class EngineSim:
         return self.dev.readback_texture(self.accum)
 
 
-def tonemap_reference(mean_rgb: np.ndarray, exposure: float, op: int) -> np.ndarray:
+def tonemap_reference(mean_rgb: np.ndarray, exposure: float, op: int, oetf: bool = True) -> np.ndarray:
     """The display transform in float64 (SoftwareTracer.cpp:263-341, 995-1007):
-    sRGB OETF of tonemap(mean * exposure)."""
+    sRGB OETF of tonemap(mean * exposure); without the OETF when oetf is false."""
     c = mean_rgb.astype(np.float64) * exposure
     if op == 1:
         ir = np.array([[0.842479062253094, 0.0784335999999992, 0.0792237451477643],
@@ -190,4 +194,44 @@ def tonemap_reference(mean_rgb: np.ndarray, exposure: float, op: int) -> np.ndar
         t = np.clip(c, 0.0, 1.0)
     else:
         t = np.clip((c * (2.51 * c + 0.03)) / (c * (2.43 * c + 0.59) + 0.14), 0.0, 1.0)
+    if not oetf:
+        return t
     return np.where(t <= 0.0031308, t * 12.92, 1.055 * np.power(np.maximum(t, 0.0), 1.0 / 2.4) - 0.055)
+
+
+def view_proj(frame, pos=None, near=0.05, far=500.0):
+    """Column-major (glm::value_ptr order) proj * view of the frame's camera in
+    float64: the right-handed look-at of the camera basis and the zero-to-one
+    depth perspective with the frame's tan(fov/2) and aspect (the engine's
+    glm::lookAtRH / perspectiveRH_ZO, Engine.cpp:7239-7241)."""
+    p = np.array(frame.pos_fovtan[:3], np.float64) if pos is None else np.asarray(pos, np.float64)
+    f = np.array(frame.fwd_aspect[:3], np.float64)
+    r = np.array(frame.right_xyz[:3], np.float64)
+    u = np.array(frame.up_xyz[:3], np.float64)
+    V = np.eye(4)
+    V[0, :3], V[1, :3], V[2, :3] = r, u, -f
+    V[:3, 3] = -V[:3, :3] @ p
+    t, a = float(frame.pos_fovtan[3]), float(frame.fwd_aspect[3])
+    P = np.zeros((4, 4))
+    P[0, 0], P[1, 1] = 1.0 / (a * t), 1.0 / t
+    P[2, 2], P[2, 3], P[3, 2] = far / (near - far), near * far / (near - far), -1.0
+    return (P @ V).T.reshape(16)
+
+
+def pixel_dirs(frame):
+    """Pixel-centre camera directions in float64."""
+    w, h = frame.width, frame.height
+    x, y = np.meshgrid(np.arange(w) + 0.5, np.arange(h) + 0.5)
+    ux = x / w * 2.0 - 1.0
+    uy = -(y / h * 2.0 - 1.0)
+    t, a = frame.pos_fovtan[3], frame.fwd_aspect[3]
+    d = (np.array(frame.fwd_aspect[:3]) + np.multiply.outer(ux * a * t, frame.right_xyz[:3])
+         + np.multiply.outer(uy * t, frame.up_xyz[:3]))
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def to_pixel(M, pts, w, h):
+    M = M.reshape(4, 4).T
+    c = pts @ M[:, :3].T + M[:, 3]
+    cw = np.maximum(c[..., 3], 1e-6)
+    return np.stack([(c[..., 0] / cw * 0.5 + 0.5) * w, (-c[..., 1] / cw * 0.5 + 0.5) * h], axis=-1)

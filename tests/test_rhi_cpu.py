@@ -81,9 +81,7 @@ GATES = [
     ("motion_blur_params", "4f", (1.0, 0.016, 0.0, 0.0), "motion blur"),
     ("mat_override_count", "4I", (1, 0, 0, 0), "material overrides"),
     ("smoke_params", "4f", (1.0, 0.0, 0.0, 0.0), "smoke volumes"),
-    ("denoiser_enabled", "I", 1, "denoiser G-buffer outputs"),
-    ("write_hdr_aux", "I", 1, "denoiser G-buffer outputs"),
-    ("write_shadow_vis", "I", 1, "denoiser G-buffer outputs"),
+    ("write_shadow_vis", "I", 1, "SIGMA shadow visibility"),
     ("mesh_tex_indices", "4I", (3, NO_TEX_TILE, NO_TEX_TILE, NO_TEX_TILE), "PBR mesh textures"),
 ]
 
@@ -98,6 +96,32 @@ def test_strict_refuses_and_lenient_lists_each_feature(field, fmt, value, what):
     d = rhi.decode_pathtrace_push(push, strict=False)
     assert what in d["ignored"]
     assert d["spp"] == 3   # the frame itself still decodes
+
+
+def test_decode_reads_the_denoiser_fields():
+    f = _frame()
+    vp = tuple(float(i) * 0.5 for i in range(16))
+    pv = tuple(float(i) * -0.25 for i in range(16))
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides={
+        "denoiser_enabled": ("I", 1), "write_normal_gbuffer": ("I", 1), "write_albedo_gbuffer": ("I", 0),
+        "write_specular_albedo_gbuffer": ("I", 1), "write_roughness_gbuffer": ("I", 1),
+        "write_specular_hit_distance_gbuffer": ("I", 0), "write_hdr_aux": ("I", 1),
+        "w2j_row0": ("4f", (1.0, 0.0, 0.0, 0.0)), "curr_view_proj": ("16f", vp), "prev_view_proj": ("16f", pv)})
+    d = rhi.decode_pathtrace_push(push)
+    assert (d["denoiser"], d["write_normal"], d["write_albedo"], d["write_specular_albedo"]) == (1, 1, 0, 1)
+    assert (d["write_roughness"], d["write_specular_distance"], d["write_hdr_aux"], d["hdr_pipeline"]) == (1, 0, 1, 0)
+    assert d["curr_view_proj"] == list(vp) and d["prev_view_proj"] == list(pv)
+    # the per-frame colour input is the mean only when the frame resets it
+    g = f.copy()
+    g.reset_accum = 0
+    push = pack_ptpush(g, prim_count=1, light_count=1, env_present=True,
+                       overrides={"denoiser_enabled": ("I", 1), "write_hdr_aux": ("I", 1)})
+    with pytest.raises(ValueError, match="without reset_accum"):
+        rhi.decode_pathtrace_push(push)
+    assert "without reset_accum" in rhi.decode_pathtrace_push(push, strict=False)["ignored"]
+    # without the denoiser the colour input is the running mean: no reset needed
+    push = pack_ptpush(g, prim_count=1, light_count=1, env_present=True, overrides={"write_hdr_aux": ("I", 1)})
+    assert rhi.decode_pathtrace_push(push)["write_hdr_aux"] == 1
 
 
 def test_sky_without_env_map_needs_sun_and_moon_below_the_horizon():
