@@ -18,6 +18,7 @@
 #include "host/prep.h"
 #include "kernels/adjoint.h"
 #include "kernels/output.h"
+#include "kernels/refit.h"
 #include "kernels/persistent.h"
 #include "kernels/wavefront.h"
 
@@ -128,6 +129,15 @@ struct dmt_scene {
     // material parameters: host copy of the prims and the device array
     std::vector<float> prims_host;
     float4* prims_dev = nullptr;
+    // mesh refit (dmt_scene_update_mesh): vertex indices per leaf slot, the
+    // triangle BVH's internal nodes by level (deepest level first), the
+    // device copy of the positions
+    uint32_t mesh_vertices = 0;
+    uint32_t mesh_tris = 0;
+    uint32_t* leaf_idx = nullptr;
+    uint32_t* refit_nodes = nullptr;
+    std::vector<uint32_t> refit_level_start;   // offsets into refit_nodes, one past the end last
+    float* positions_dev = nullptr;
     uint32_t* adj_spill = nullptr;    // traversal spill area of the adjoint kernel
 };
 
@@ -436,6 +446,38 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         V.env_intensity = P.env_intensity;
         V.env_present = 1u;
     }
+    if (P.tri_count) {
+        // what a refit needs: the vertex indices of every leaf slot, and the
+        // internal nodes grouped by level, deepest first
+        std::vector<uint32_t> idx((size_t)P.tri_count * 3u);
+        for (uint32_t k = 0; k < P.tri_count; ++k)
+            for (uint32_t a = 0; a < 3u; ++a) idx[3u * k + a] = desc->indices[3u * P.tri_bvh.order[k] + a];
+        std::vector<std::vector<uint32_t>> levels(1, std::vector<uint32_t>{0u});
+        while (true) {
+            std::vector<uint32_t> next;
+            for (uint32_t n : levels.back())
+                for (uint32_t c : P.tri_bvh.nodes[n].child)
+                    if (c != kEmptyChild && !(c & kLeafBit)) next.push_back(c);
+            if (next.empty()) break;
+            levels.push_back(std::move(next));
+        }
+        std::vector<uint32_t> order;
+        order.reserve(P.tri_bvh.nodes.size());
+        for (size_t l = levels.size(); l-- > 0;) {
+            s->refit_level_start.push_back((uint32_t)order.size());
+            order.insert(order.end(), levels[l].begin(), levels[l].end());
+        }
+        s->refit_level_start.push_back((uint32_t)order.size());
+        hipError_t e = dalloc(&s->leaf_idx, idx.size());
+        if (e == hipSuccess) s->buffers.push_back(s->leaf_idx);
+        if (e == hipSuccess) e = hipMemcpy(s->leaf_idx, idx.data(), idx.size() * 4u, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = dalloc(&s->refit_nodes, order.size());
+        if (e == hipSuccess) s->buffers.push_back(s->refit_nodes);
+        if (e == hipSuccess) e = hipMemcpy(s->refit_nodes, order.data(), order.size() * 4u, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(hip_fail(e, "refit tables"));
+        s->mesh_vertices = desc->vertex_count;
+        s->mesh_tris = P.tri_count;
+    }
     s->bvh_depth = std::max(P.tri_bvh.depth, P.sphere_bvh.depth);
     s->bvh_nodes = (uint32_t)(P.tri_bvh.nodes.size() + P.sphere_bvh.nodes.size());
     hipError_t e = hipMalloc((void**)&s->count_stats, sizeof(DeviceStats));
@@ -626,6 +668,41 @@ dmt_status dmt_tonemap(dmt_scene* s, const dmt_tonemap_params* p, void* d_out) {
     // the next frame's accumulation pass waits for this read
     if (e == hipSuccess && s->last_set >= 0) e = hipEventRecord(s->acc_done[s->last_set], stream);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "tonemap");
+    return DMT_OK;
+}
+
+dmt_status dmt_scene_update_mesh(dmt_scene* s, const float* positions, uint32_t vertex_count) {
+    if (!s || (!positions && vertex_count)) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if (s->mesh_tris == 0) return fail(DMT_ERR_INVALID_ARG, "the scene has no mesh");
+    if (vertex_count != s->mesh_vertices)
+        return fail(DMT_ERR_INVALID_ARG, "vertex_count must be the mesh's " + std::to_string(s->mesh_vertices));
+    const size_t nf = (size_t)vertex_count * 3u;
+    for (size_t i = 0; i < nf; ++i)
+        if (!std::isfinite(positions[i])) return fail(DMT_ERR_INVALID_ARG, "positions contain a non-finite value");
+    hipError_t e = hipSuccess;
+    if (!s->positions_dev) {
+        e = dalloc(&s->positions_dev, nf);
+        if (e != hipSuccess) {
+            s->positions_dev = nullptr;
+            return hip_fail(e, "hipMalloc(positions)");
+        }
+        s->buffers.push_back(s->positions_dev);
+    }
+    // frames in flight read the triangles and the tree
+    e = sync_all(s->dev);
+    if (e == hipSuccess) e = hipMemcpy(s->positions_dev, positions, nf * 4u, hipMemcpyHostToDevice);
+    hipStream_t stream = s->dev->stream;
+    if (e == hipSuccess)
+        e = launch_refit_tris(s->positions_dev, s->leaf_idx, s->mesh_tris, const_cast<float4*>(s->view.tris), stream);
+    for (size_t l = 0; e == hipSuccess && l + 1 < s->refit_level_start.size(); ++l) {
+        const uint32_t b = s->refit_level_start[l], n = s->refit_level_start[l + 1] - b;
+        e = launch_refit_nodes(const_cast<float4*>(s->view.tri_nodes), s->refit_nodes + b, n, s->positions_dev,
+                               s->leaf_idx, stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "mesh refit");
     return DMT_OK;
 }
 

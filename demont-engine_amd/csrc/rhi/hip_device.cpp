@@ -245,7 +245,8 @@ struct HipDevice::Impl {
     SceneKey key{};
     std::vector<float> scene_prims;
     float* scene_accum = nullptr;        // texture memory the scene renders into
-    std::uint64_t builds = 0, material_updates = 0;
+    std::uint64_t builds = 0, material_updates = 0, mesh_refits = 0;
+    std::vector<std::uint32_t> scene_indices;   // the mesh triangles the scene was built with
 
     void report(const std::string& m) {
         error = m;
@@ -622,6 +623,7 @@ const std::string& HipDevice::LastError() const { return impl_->error; }
 dmt_scene* HipDevice::ActiveScene() const { return impl_->scene; }
 std::uint64_t HipDevice::SceneBuilds() const { return impl_->builds; }
 std::uint64_t HipDevice::MaterialUpdates() const { return impl_->material_updates; }
+std::uint64_t HipDevice::MeshRefits() const { return impl_->mesh_refits; }
 const std::string& HipDevice::IgnoredFeatures() const { return impl_->ignored; }
 
 // ---- the pathtrace dispatch ------------------------------------------------------
@@ -677,6 +679,19 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
         k.env_intensity = bits(P.env_intensity);
     }
     if (scene && k == key) return true;
+
+    // A new TLAS over the same triangles (instances moved): refit the mesh's
+    // tree on the device instead of a rebuild.
+    SceneKey only_tlas = key;
+    only_tlas.tlas = k.tlas;
+    if (scene && tlas && key.tlas && only_tlas == k && !tlas->indices.empty() && tlas->indices == scene_indices) {
+        if (!check(dmt_scene_update_mesh(scene, tlas->positions.data(), (std::uint32_t)(tlas->positions.size() / 3u)),
+                   "dmt_scene_update_mesh"))
+            return false;
+        key = k;
+        mesh_refits++;
+        return true;
+    }
 
     // The primitives as the scene takes them: v3 (previous centre) reads as v0
     // without motion blur, as the shader does with motion_blur_params.x = 0
@@ -756,6 +771,8 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
     scene = ns;
     scene_accum = nullptr;
     scene_prims = std::move(pr);
+    if (tlas) scene_indices = tlas->indices;
+    else scene_indices.clear();
     key = k;
     builds++;
     return true;

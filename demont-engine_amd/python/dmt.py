@@ -177,6 +177,7 @@ def lib() -> C.CDLL:
         "dmt_scene_bind_accum": (st, [vp, vp, u32, u32]),
         "dmt_tonemap": (st, [vp, C.POINTER(TonemapParams), vp]),
         "dmt_render_gbuffer": (st, [vp, C.POINTER(FrameParams), C.POINTER(GbufferParams)]),
+        "dmt_scene_update_mesh": (st, [vp, fp, u32]),
         "dmt_trace_closest": (st, [vp, fp, u32, vp]),
         "dmt_scene_stats": (st, [vp, C.POINTER(Stats)]),
         "dmt_scene_reset_stats": (st, [vp]),
@@ -378,6 +379,11 @@ class Scene:
         p = C.c_void_p()
         _check(lib().dmt_accum_device_ptr(self._h, C.byref(p)))
         return p.value or 0
+
+    def update_mesh(self, positions: np.ndarray):
+        """New vertex positions of the mesh; the triangle BVH is refit on the device."""
+        p = np.ascontiguousarray(positions, np.float32)
+        _check(lib().dmt_scene_update_mesh(self._h, _fptr(p), p.size // 3))
 
     def bind_accum(self, d_rgba: int, width: int, height: int):
         """Render into caller-owned device memory (width*height float4); 0 unbinds."""

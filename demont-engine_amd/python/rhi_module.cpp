@@ -233,5 +233,6 @@ PYBIND11_MODULE(_dmt_rhi, m) {
         .def("ignored_features", [](PyDevice& s) { return s.d().IgnoredFeatures(); })
         .def("scene_builds", [](PyDevice& s) { return s.d().SceneBuilds(); })
         .def("material_updates", [](PyDevice& s) { return s.d().MaterialUpdates(); })
+        .def("mesh_refits", [](PyDevice& s) { return s.d().MeshRefits(); })
         .def("active_scene", [](PyDevice& s) { return reinterpret_cast<uintptr_t>(s.d().ActiveScene()); });
 }

@@ -272,6 +272,18 @@ dmt_status dmt_scene_set_profiling(dmt_scene* scene, int32_t enabled);
 dmt_status dmt_scene_set_schedule(dmt_scene* scene, uint32_t wavefront_iterations);
 dmt_status dmt_scene_get_schedule(dmt_scene* scene, uint32_t* wavefront_iterations);
 
+/* Moving meshes: new positions for the scene's mesh (the same vertex count and
+ * triangles) with the triangle BVH refit on the device -- the triangle records
+ * re-formed and every node's boxes recomputed bottom-up from the new vertices,
+ * the tree's topology kept.  Replaces the host rebuild the reference engine
+ * runs when a mesh changes (RebuildMeshResources, Engine.cpp:3752-3900;
+ * CreateBLAS/CreateTLAS, SoftwareDevice.cpp:337-421) where the triangles
+ * stay the same.  Images are those of a scene built from the new positions
+ * (the closest hit does not depend on the tree); traversal cost grows as the
+ * vertices move away from where the tree was built, and dmt_scene_create
+ * restores it.  Waits for the scene's frames in flight. */
+dmt_status dmt_scene_update_mesh(dmt_scene* scene, const float* positions, uint32_t vertex_count);
+
 /* ---- Material parameters and the adjoint -----------------------------------
  * Per analytic primitive, 7 floats: albedo rgb (v1.xyz), roughness (v1.w),
  * emission rgb (v4.xyz) of the primitive wire format.  Replaces re-uploading

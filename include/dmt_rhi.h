@@ -22,7 +22,9 @@
  *     2, its instances flattened into one triangle mesh) and the env map
  *     (storage texture slot 5, RGBA32F) form a dmt_scene, built once and
  *     rebuilt only when one of them changes (a material-only change of the
- *     primitives goes through dmt_scene_set_materials instead);
+ *     primitives goes through dmt_scene_set_materials instead, and a new TLAS
+ *     over the same triangles -- moved instances -- through
+ *     dmt_scene_update_mesh, a refit of the mesh's tree on the device);
  *   - the frame renders into the accum texture's device memory (slot 1, bound
  *     with dmt_scene_bind_accum, so the running mean survives a rebuild);
  *   - the output texture (slot 0) receives sRGB(tonemap(mean * exposure))
@@ -273,6 +275,7 @@ public:
     dmt_scene* ActiveScene() const;                /* scene of the last pathtrace dispatch */
     std::uint64_t SceneBuilds() const;             /* dmt_scene_create calls so far */
     std::uint64_t MaterialUpdates() const;         /* dmt_scene_set_materials calls so far */
+    std::uint64_t MeshRefits() const;              /* dmt_scene_update_mesh calls so far */
     const std::string& IgnoredFeatures() const;    /* lenient mode: of the last dispatch */
 
 private:

@@ -214,7 +214,7 @@ def test_adapter_exports_the_device_interface():
     methods = ["CreateBuffer", "CreateTexture", "CreateComputePipeline", "CreateBLAS", "CreateTLAS", "DestroyBuffer",
                "DestroyTexture", "DestroyPipeline", "DestroyAccelStruct", "WriteBuffer", "WriteTexture", "BeginFrame",
                "EndFrame", "AcquireCommandBuffer", "Submit", "WaitIdle", "Resize", "ReadbackTexture", "ReadbackBuffer",
-               "DeviceName", "CurrentAllocatedBytes", "IsDeviceLost", "LastError", "ActiveScene"]
+               "DeviceName", "CurrentAllocatedBytes", "IsDeviceLost", "LastError", "ActiveScene", "MeshRefits"]
     for m in methods:
         assert f"dmt::rhi::HipDevice::{m}(" in syms or f"dmt::rhi::HipDevice::{m}[abi:" in syms, m
     for m in ["BindComputePipeline", "BindBuffer", "BindStorageTexture", "BindAccelStruct", "PushConstants",
