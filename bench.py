@@ -166,7 +166,7 @@ def main():
         scene.set_profiling(False)
         return dt, st
 
-    # K frames, one dmt_render call each (the library keeps three in flight)
+    # K frames, one dmt_render call each (the library keeps four in flight)
     def per_frame():
         for k in range(args.steps):
             f = frame.copy()
@@ -189,7 +189,7 @@ def main():
     ext_bytes = (tr["ext_node_visits"] * 128 + tr["ext_tri_tests"] * 48 + tr["ext_sphere_tests"] * 20
                  + tr["ext_rays"] * 40)
     tail_bytes = tr["tail_node_visits"] * 128 + tr["tail_tri_tests"] * 48 + tr["tail_sphere_tests"] * 20
-    # In the timed region three frames share the GPU, so a launch's event
+    # In the timed region four frames share the GPU, so a launch's event
     # duration there includes other frames' kernels running beside it.  A
     # kernel's own launch duration comes from frames rendered one at a time
     # (each waits for the previous one): the same frame as the instrumented
