@@ -18,6 +18,7 @@
 #include "host/prep.h"
 #include "kernels/adjoint.h"
 #include "kernels/output.h"
+#include "kernels/lbvh.h"
 #include "kernels/refit.h"
 #include "kernels/persistent.h"
 #include "kernels/wavefront.h"
@@ -392,13 +393,15 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     dmt_status st = check_device(dev);
     if (st) return st;
     if (!desc) return fail(DMT_ERR_INVALID_ARG, "scene desc is null");
+    if (bp && (bp->flags & ~DMT_BUILD_DEVICE)) return fail(DMT_ERR_INVALID_ARG, "unknown build flags");
+    const bool device_build = bp && (bp->flags & DMT_BUILD_DEVICE);
     PreparedScene P;
     std::string msg;
     try {
         msg = prepare_scene(desc->prims, desc->prim_count, desc->lights, desc->light_count, desc->positions,
                             desc->vertex_count, desc->indices, desc->index_count, desc->env_rgb, desc->env_width,
                             desc->env_height, desc->env_intensity, bp ? bp->sah_bins : 0u,
-                            bp ? bp->max_leaf : 0u, P);
+                            bp ? bp->max_leaf : 0u, P, !device_build);
     } catch (const std::bad_alloc&) {
         return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation during scene build");
     }
@@ -429,9 +432,46 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     V.sph_nodes = f4;
     if ((st = upload(s, (const float4*)P.spheres.data(), P.sphere_ids.size(), &V.spheres, kLeafPad))) return bail(st);
     if ((st = upload(s, P.sphere_ids.data(), P.sphere_ids.size(), &V.sphere_ids))) return bail(st);
-    if ((st = upload(s, (const float4*)P.tri_bvh.nodes.data(), P.tri_bvh.nodes.size() * 8u, &f4))) return bail(st);
-    V.tri_nodes = f4;
-    if ((st = upload(s, (const float4*)P.tris.data(), (size_t)P.tri_count * 3u, &V.tris, kLeafPad))) return bail(st);
+    uint32_t tri_nodes = (uint32_t)P.tri_bvh.nodes.size(), tri_depth = P.tri_bvh.depth;
+    std::vector<uint32_t> level_start;   // device build: first node of each level, node count last
+    if (device_build && P.tri_count) {
+        float* pos = nullptr;
+        uint32_t* idx = nullptr;
+        hipError_t e = dalloc(&pos, (size_t)desc->vertex_count * 3u);
+        if (e == hipSuccess) s->buffers.push_back(pos);
+        if (e == hipSuccess) e = dalloc(&idx, (size_t)P.tri_count * 3u);
+        if (e == hipSuccess)
+            e = hipMemcpy(pos, desc->positions, (size_t)desc->vertex_count * 12u, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(idx, desc->indices, (size_t)P.tri_count * 12u, hipMemcpyHostToDevice);
+        LbvhOutput L;
+        if (e == hipSuccess) {
+            LbvhInput in{};
+            in.positions = pos;
+            in.indices = idx;
+            in.tri_count = P.tri_count;
+            for (int a = 0; a < 3; ++a) {
+                in.cmin[a] = P.tri_cmin[a];
+                in.cmax[a] = P.tri_cmax[a];
+            }
+            in.max_leaf = (bp && bp->max_leaf) ? bp->max_leaf : 4u;
+            e = build_lbvh(in, kLeafPad, L, dev->stream);
+        }
+        if (idx) (void)hipFree(idx);
+        if (e != hipSuccess) return bail(hip_fail(e, "device BVH build"));
+        for (void* p : {(void*)L.nodes, (void*)L.tris, (void*)L.leaf_idx}) s->buffers.push_back(p);
+        V.tri_nodes = L.nodes;
+        V.tris = L.tris;
+        s->leaf_idx = L.leaf_idx;
+        s->positions_dev = pos;   // the device copy a later refit overwrites
+        s->scene_bytes += (size_t)L.node_count * 128u + ((size_t)P.tri_count * 3u + kLeafPad) * 16u;
+        tri_nodes = L.node_count;
+        tri_depth = (uint32_t)L.level_start.size() - 1u;
+        level_start = std::move(L.level_start);
+    } else {
+        if ((st = upload(s, (const float4*)P.tri_bvh.nodes.data(), P.tri_bvh.nodes.size() * 8u, &f4))) return bail(st);
+        V.tri_nodes = f4;
+        if ((st = upload(s, (const float4*)P.tris.data(), (size_t)P.tri_count * 3u, &V.tris, kLeafPad))) return bail(st);
+    }
     if ((st = upload(s, (const float4*)P.lights.data(), (size_t)P.light_count * 5u, &V.lights))) return bail(st);
     V.light_count = P.light_count;
     V.ltree_count = (uint32_t)(P.light_tree.size() / 16u);
@@ -446,7 +486,23 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         V.env_intensity = P.env_intensity;
         V.env_present = 1u;
     }
-    if (P.tri_count) {
+    if (P.tri_count && device_build) {
+        // the device build allocates its nodes level by level: the refit's
+        // lists are those index ranges, deepest first
+        std::vector<uint32_t> order;
+        order.reserve(tri_nodes);
+        for (size_t l = level_start.size() - 1u; l-- > 0;) {
+            s->refit_level_start.push_back((uint32_t)order.size());
+            for (uint32_t n = level_start[l]; n < level_start[l + 1u]; ++n) order.push_back(n);
+        }
+        s->refit_level_start.push_back((uint32_t)order.size());
+        hipError_t e = dalloc(&s->refit_nodes, order.size());
+        if (e == hipSuccess) s->buffers.push_back(s->refit_nodes);
+        if (e == hipSuccess) e = hipMemcpy(s->refit_nodes, order.data(), order.size() * 4u, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(hip_fail(e, "refit tables"));
+        s->mesh_vertices = desc->vertex_count;
+        s->mesh_tris = P.tri_count;
+    } else if (P.tri_count) {
         // what a refit needs: the vertex indices of every leaf slot, and the
         // internal nodes grouped by level, deepest first
         std::vector<uint32_t> idx((size_t)P.tri_count * 3u);
@@ -478,8 +534,8 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         s->mesh_vertices = desc->vertex_count;
         s->mesh_tris = P.tri_count;
     }
-    s->bvh_depth = std::max(P.tri_bvh.depth, P.sphere_bvh.depth);
-    s->bvh_nodes = (uint32_t)(P.tri_bvh.nodes.size() + P.sphere_bvh.nodes.size());
+    s->bvh_depth = std::max(tri_depth, P.sphere_bvh.depth);
+    s->bvh_nodes = (uint32_t)(tri_nodes + P.sphere_bvh.nodes.size());
     hipError_t e = hipMalloc((void**)&s->count_stats, sizeof(DeviceStats));
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(stats)"));
     s->buffers.push_back(s->count_stats);

@@ -39,7 +39,8 @@ void quat_rotate(const float* q, const float* v, float* out) {
 std::string prepare_scene(const float* prims, uint32_t prim_count, const float* lights, uint32_t light_count,
                           const float* positions, uint32_t vertex_count, const uint32_t* indices,
                           uint32_t index_count, const float* env_rgb, uint32_t env_w, uint32_t env_h,
-                          float env_intensity, uint32_t bins, uint32_t max_leaf, PreparedScene& S) {
+                          float env_intensity, uint32_t bins, uint32_t max_leaf, PreparedScene& S,
+                          bool host_tri_bvh) {
     if (prim_count && !prims) return "prims is null";
     if (light_count && !lights) return "lights is null";
     if (index_count % 3u) return "index_count must be a multiple of 3";
@@ -111,24 +112,28 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
                 b.lo[a] = std::fmin(v0[a], std::fmin(v1[a], v2[a]));
                 b.hi[a] = std::fmax(v0[a], std::fmax(v1[a], v2[a]));
                 b.c[a] = (v0[a] + v1[a] + v2[a]) * (1.0f / 3.0f);
+                S.tri_cmin[a] = t ? std::fmin(S.tri_cmin[a], b.c[a]) : b.c[a];
+                S.tri_cmax[a] = t ? std::fmax(S.tri_cmax[a], b.c[a]) : b.c[a];
             }
         }
-        S.tri_bvh = build_bvh4(items, bins, max_leaf ? max_leaf : 4u);
         S.tri_count = tris;
-        S.tris.resize(tris);
-        for (uint32_t k = 0; k < tris; ++k) {
-            const uint32_t t = S.tri_bvh.order[k];
-            const float* v0 = positions + (size_t)indices[3 * t] * 3u;
-            const float* v1 = positions + (size_t)indices[3 * t + 1] * 3u;
-            const float* v2 = positions + (size_t)indices[3 * t + 2] * 3u;
-            TriRec& r = S.tris[k];
-            for (int a = 0; a < 3; ++a) {
-                r.v0[a] = v0[a];
-                r.e1[a] = v1[a] - v0[a];
-                r.e2[a] = v2[a] - v0[a];
+        if (host_tri_bvh) {   // else the device builds the tree and the records
+            S.tri_bvh = build_bvh4(items, bins, max_leaf ? max_leaf : 4u);
+            S.tris.resize(tris);
+            for (uint32_t k = 0; k < tris; ++k) {
+                const uint32_t t = S.tri_bvh.order[k];
+                const float* v0 = positions + (size_t)indices[3 * t] * 3u;
+                const float* v1 = positions + (size_t)indices[3 * t + 1] * 3u;
+                const float* v2 = positions + (size_t)indices[3 * t + 2] * 3u;
+                TriRec& r = S.tris[k];
+                for (int a = 0; a < 3; ++a) {
+                    r.v0[a] = v0[a];
+                    r.e1[a] = v1[a] - v0[a];
+                    r.e2[a] = v2[a] - v0[a];
+                }
+                r.id = t;
+                r.pad1 = r.pad2 = 0.0f;
             }
-            r.id = t;
-            r.pad1 = r.pad2 = 0.0f;
         }
     }
 

@@ -70,8 +70,10 @@ struct PreparedScene {
     std::vector<uint32_t> sphere_ids;     // input index, leaf order
     // mesh
     uint32_t tri_count = 0;
-    Bvh4 tri_bvh;
-    std::vector<TriRec> tris;             // leaf order
+    Bvh4 tri_bvh;                         // empty when the tree is built on the device
+    std::vector<TriRec> tris;             // leaf order (likewise)
+    float tri_cmin[3] = {0.0f, 0.0f, 0.0f};   // triangle centroid bounds
+    float tri_cmax[3] = {0.0f, 0.0f, 0.0f};
     // lights
     std::vector<float> lights;            // 20 floats each
     uint32_t light_count = 0;
@@ -89,6 +91,7 @@ struct PreparedScene {
 std::string prepare_scene(const float* prims, uint32_t prim_count, const float* lights, uint32_t light_count,
                           const float* positions, uint32_t vertex_count, const uint32_t* indices,
                           uint32_t index_count, const float* env_rgb, uint32_t env_w, uint32_t env_h,
-                          float env_intensity, uint32_t bins, uint32_t max_leaf, PreparedScene& out);
+                          float env_intensity, uint32_t bins, uint32_t max_leaf, PreparedScene& out,
+                          bool host_tri_bvh = true);
 
 }  // namespace dmt

@@ -72,7 +72,10 @@ class SceneDesc(C.Structure):
 
 
 class BuildParams(C.Structure):
-    _fields_ = [("sah_bins", C.c_uint32), ("max_leaf", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("sah_bins", C.c_uint32), ("max_leaf", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+BUILD_DEVICE = 1
 
 
 class Hit(C.Structure):
@@ -356,12 +359,13 @@ class Device:
 
 
 class Scene:
-    def __init__(self, device: Device, data: SceneData, sah_bins: int = 0, max_leaf: int = 0):
+    def __init__(self, device: Device, data: SceneData, sah_bins: int = 0, max_leaf: int = 0,
+                 device_build: bool = False):
         self.device = device
         self.data = data
         self._h = C.c_void_p()
         d = data.desc()
-        bp = BuildParams(sah_bins, max_leaf)
+        bp = BuildParams(sah_bins, max_leaf, BUILD_DEVICE if device_build else 0)
         _check(lib().dmt_scene_create(device.handle, C.byref(d), C.byref(bp), C.byref(self._h)))
 
     def render(self, frame: FrameParams):

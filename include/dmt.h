@@ -105,8 +105,17 @@ typedef struct dmt_frame_params {
 typedef struct dmt_build_params {
     uint32_t sah_bins;          /* binned-SAH bin count (0 = default 16)     */
     uint32_t max_leaf;          /* triangles per leaf (0 = default 4, <= 8)  */
-    uint32_t reserved[2];
+    uint32_t flags;             /* DMT_BUILD_* bits                          */
+    uint32_t reserved;
 } dmt_build_params;
+/* Build the triangle BVH on the device: a linear BVH over the Morton order of
+ * the triangle centroids (Karras 2012), collapsed to 4-wide nodes -- for
+ * meshes whose triangles change, where the host's binned-SAH build is the
+ * slow step (no counterpart in the reference, whose builds run on the host:
+ * TriangleBvh::Build, src/renderer/TriangleBvh.cpp:33-169; Embree scenes,
+ * SoftwareDevice.cpp:337-421).  Same images (the closest hit does not depend
+ * on the tree); the tree traverses somewhat slower than the SAH one. */
+#define DMT_BUILD_DEVICE 1u
 
 typedef struct dmt_hit {
     float    t;

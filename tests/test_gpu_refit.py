@@ -169,3 +169,81 @@ def test_engine_tlas_with_moved_instances_refits(device):
     assert np.array_equal(got, sc.readback(g.width, g.height))
     sc.close()
     dev.close()
+
+
+def _device_build_cases():
+    yield "mesh + env", _mesh_scene(seed=61)
+    yield "spheres + mesh + lights", dmt.generate(2, seed=62, spheres=120, subdivisions=3, random_tris=2500, lights=40,
+                                                  width=48, height=32)
+    sd = _mesh_scene(seed=63, w=32, h=24)
+    yield "one triangle", dmt.SceneData(prims=sd.prims, lights=sd.lights, positions=sd.positions,
+                                        indices=sd.indices[:1], env=sd.env, frame=sd.frame)
+    yield "three triangles", dmt.SceneData(prims=sd.prims, lights=sd.lights, positions=sd.positions,
+                                           indices=sd.indices[100:103], env=sd.env, frame=sd.frame)
+    dup = np.repeat(sd.indices[:40], 50, axis=0)      # equal Morton codes everywhere
+    yield "duplicated triangles", dmt.SceneData(prims=sd.prims, lights=sd.lights, positions=sd.positions, indices=dup,
+                                                env=sd.env, frame=sd.frame)
+
+
+DEVICE_CASES = list(_device_build_cases())
+
+
+@pytest.mark.parametrize("idx", range(len(DEVICE_CASES)), ids=[c[0] for c in DEVICE_CASES])
+def test_device_build_renders_the_host_built_images(device, idx):
+    label, sd = DEVICE_CASES[idx]
+    host = dmt.Scene(device, sd)
+    dev = dmt.Scene(device, sd, device_build=True)
+    a = _render(device, host, sd.frame, 2)
+    b = _render(device, dev, sd.frame, 2)
+    assert np.array_equal(a, b), label
+    rng = np.random.default_rng(idx)
+    o = rng.uniform(-6, 6, (3000, 3)).astype(np.float32)
+    d = rng.normal(size=(3000, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    h = dev.trace(o, d)
+    ref = orc.OracleScene.from_scene_data(sd).trace(o, d, brute=True)
+    for k in ("t", "hit", "prim"):
+        assert np.array_equal(h[k], ref[k]), (label, k)
+    sh, sd_ = host.stats(), dev.stats()
+    print(f"{label}: {sd.indices.shape[0]} triangles, nodes host {sh['bvh_nodes']} / device {sd_['bvh_nodes']}, "
+          f"depth {sh['bvh_depth']} / {sd_['bvh_depth']}")
+    host.close()
+    dev.close()
+
+
+def test_device_build_then_refit(device):
+    sd = _mesh_scene(seed=64)
+    sc = dmt.Scene(device, sd, device_build=True)
+    pos = dict(_deformations(sd.positions))["smooth wave"]
+    sc.update_mesh(pos)
+    got = _render(device, sc, sd.frame, 1)
+    fresh = dmt.Scene(device, _with_positions(sd, pos))
+    assert np.array_equal(got, _render(device, fresh, sd.frame, 1))
+    sc.close()
+    fresh.close()
+
+
+def test_device_build_time_and_traversal_on_the_large_mesh(device):
+    sd = dmt.generate(1, seed=1)          # 1.51 M triangles, 1920x1080
+    out = {}
+    for label, kw in (("host SAH", {}), ("device LBVH", {"device_build": True})):
+        t0 = time.perf_counter()
+        sc = dmt.Scene(device, sd, **kw)
+        build = time.perf_counter() - t0
+        f = sd.frame.copy()
+        f.reset_accum = 1
+        sc.render_frames(f, 2)
+        device.synchronize()
+        t0 = time.perf_counter()
+        g = sd.frame.copy()
+        g.frame_index = 2
+        sc.render_frames(g, 10)
+        device.synchronize()
+        dt = (time.perf_counter() - t0) / 10
+        tr = sc.measure_traffic(f)
+        st = sc.stats()
+        out[label] = sc.readback(f.width, f.height)
+        print(f"{label}: scene build {build * 1e3:.0f} ms, {st['bvh_nodes']} nodes depth {st['bvh_depth']}, "
+              f"{dt * 1e3:.3f} ms/frame, {tr['node_visits']} node visits")
+        sc.close()
+    assert np.array_equal(out["host SAH"], out["device LBVH"])
