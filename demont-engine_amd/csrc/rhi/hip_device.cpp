@@ -765,8 +765,10 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
         std::memcpy(light_copy.data(), lights->data.data() + k.lights_off, light_copy.size() * 4u);
         d.lights = light_copy.data();
     }
+    dmt_build_params bp{};
+    bp.flags = opt.device_bvh_build ? DMT_BUILD_DEVICE : 0u;
     dmt_scene* ns = nullptr;
-    if (!check(dmt_scene_create(dev, &d, nullptr, &ns), "dmt_scene_create")) return false;
+    if (!check(dmt_scene_create(dev, &d, &bp, &ns), "dmt_scene_create")) return false;
     if (scene) dmt_scene_destroy(scene);   // waits for its frames
     scene = ns;
     scene_accum = nullptr;

@@ -62,12 +62,13 @@ py::dict decoded(const PathtracePush& P) {
 
 class PyDevice {
 public:
-    PyDevice(int device_index, uint32_t width, uint32_t height, bool strict) {
+    PyDevice(int device_index, uint32_t width, uint32_t height, bool strict, bool device_bvh_build) {
         HipDeviceOptions o;
         o.device_index = device_index;
         o.width = width;
         o.height = height;
         o.strict = strict;
+        o.device_bvh_build = device_bvh_build;
         dev_ = std::make_unique<HipDevice>(o);
         cb_ = dev_->AcquireCommandBuffer();
     }
@@ -107,8 +108,8 @@ PYBIND11_MODULE(_dmt_rhi, m) {
         py::arg("push"), py::arg("strict") = true);
 
     py::class_<PyDevice>(m, "HipDevice")
-        .def(py::init<int, uint32_t, uint32_t, bool>(), py::arg("device_index") = 0, py::arg("width") = 1280,
-             py::arg("height") = 720, py::arg("strict") = true)
+        .def(py::init<int, uint32_t, uint32_t, bool, bool>(), py::arg("device_index") = 0, py::arg("width") = 1280,
+             py::arg("height") = 720, py::arg("strict") = true, py::arg("device_bvh_build") = false)
         .def("close", &PyDevice::close)
         .def("create_buffer", [](PyDevice& s, size_t size) { return s.d().CreateBuffer(BufferDesc{size}).id; })
         .def(

@@ -202,6 +202,8 @@ struct HipDeviceOptions {
     std::uint32_t width = 1280;     /* initial swapchain size (the software device's 1280x720 sentinel) */
     std::uint32_t height = 720;
     bool strict = true;             /* see DecodePathtracePush */
+    bool device_bvh_build = false;  /* mesh trees built on the device (DMT_BUILD_DEVICE): faster
+                                       rebuilds after mesh edits, somewhat slower traversal */
 };
 
 class HipDevice;

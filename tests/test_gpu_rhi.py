@@ -49,13 +49,14 @@ def _direct(device, sd, frame, count, materials=None):
     return img
 
 
+@pytest.mark.parametrize("device_bvh", [False, True], ids=["host-bvh", "device-bvh"])
 @pytest.mark.parametrize("idx", range(len(SCENES)), ids=[s[0] for s in SCENES])
-def test_engine_frames_equal_direct_render(device, idx):
+def test_engine_frames_equal_direct_render(device, idx, device_bvh):
     label, sd = SCENES[idx]
     f = sd.frame.copy()
     f.frame_index = 3
     f.reset_accum = 1
-    dev = rhi.HipDevice()
+    dev = rhi.HipDevice(device_bvh_build=device_bvh)
     assert not dev.is_device_lost(), dev.last_error()
     eng = EngineSim(dev, sd)
     eng.frames(f, 3)
