@@ -319,10 +319,13 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                 state = kLaneIdle;
             }
         }
-        // ---- one traversal step of every lane with a query in progress ----
-        if (state == kLaneWalk && walk_step<false>(S, l.o, l.d, w, stk, tc)) {
-            h = final_hit(w.ab, w.tb);
-            state = kLaneReady;
+        // ---- traversal steps of every lane with a query in progress ----
+#pragma unroll
+        for (uint32_t k = 0; k < kTailSteps; ++k) {
+            if (state == kLaneWalk && walk_step<false>(S, l.o, l.d, w, stk, tc)) {
+                h = final_hit(w.ab, w.tb);
+                state = kLaneReady;
+            }
         }
     }
     add_stat(&A.stats->primary, n_primary);

@@ -108,6 +108,14 @@ constexpr uint32_t kNoTail = 0xFFFFFFFFu;   // every iteration by the per-bounce
 #define DMT_TAIL_WAVES 5
 #endif
 constexpr uint32_t kTailWavesPerSimd = DMT_TAIL_WAVES;   // k_tail occupancy target (and workgroups per CU)
+// Traversal steps per pass of the step-granular loops (k_tail, k_persistent)
+// between the wave-level bookkeeping (ballots, refill, batched shading): three
+// measured 1.5 % (1.51 M-triangle 1080p) to 4.6 % (many-light scene) faster
+// than one; a lane that finishes in an earlier step waits out the pass.
+#ifndef DMT_TAIL_STEPS
+#define DMT_TAIL_STEPS 3
+#endif
+constexpr uint32_t kTailSteps = DMT_TAIL_STEPS;
 constexpr uint32_t kPollSlots = 4;
 constexpr uint32_t kPollChunk = 4;        // iterations between live-queue snapshots
 

@@ -331,10 +331,13 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, Pat
                 state = tail::kLaneIdle;
             }
         }
-        // one traversal step of every lane with a query in progress
-        if (state == tail::kLaneWalk && walk_step<COUNT>(S, l.o, l.d, w, stk, tc)) {
-            h = final_hit(w.ab, w.tb);
-            state = tail::kLaneReady;
+        // traversal steps of every lane with a query in progress
+#pragma unroll
+        for (uint32_t k = 0; k < kTailSteps; ++k) {
+            if (state == tail::kLaneWalk && walk_step<COUNT>(S, l.o, l.d, w, stk, tc)) {
+                h = final_hit(w.ab, w.tb);
+                state = tail::kLaneReady;
+            }
         }
     }
     tail::add_stat(&st->primary, n_primary);
