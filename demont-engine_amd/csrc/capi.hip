@@ -297,13 +297,53 @@ dmt_status check_frame(const dmt_frame_params* f) {
     if (f->max_bounces > DMT_MAX_BOUNCES) return fail(DMT_ERR_LIMIT, "max_bounces must be <= 64");
     if (f->refract_bounces > 64) return fail(DMT_ERR_LIMIT, "refract_bounces must be <= 64");
     if (f->flags & ~DMT_FRAME_NO_RUSSIAN_ROULETTE) return fail(DMT_ERR_INVALID_ARG, "unknown frame flags");
+    const float lens[6] = {f->dof[0], f->dof[1], f->dof[2], f->dof[3], f->cat_eye, f->cauchy_c};
+    for (float v : lens)
+        if (!std::isfinite(v)) return fail(DMT_ERR_INVALID_ARG, "lens and spectral parameters must be finite");
+    if (f->dof[0] > 0.0f && f->dof[2] > 64.0f) return fail(DMT_ERR_LIMIT, "at most 64 aperture blades");
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(f->mesh_shift_prev[a]) || !std::isfinite(f->mesh_shift_curr[a]))
+            return fail(DMT_ERR_INVALID_ARG, "mesh translations must be finite");
+    if (f->motion_blur > 1u || f->spectral > 1u || f->mesh_shift_active > 1u)
+        return fail(DMT_ERR_INVALID_ARG, "motion_blur, spectral and mesh_shift_active are 0 or 1");
+    if (f->reserved) return fail(DMT_ERR_INVALID_ARG, "reserved must be 0");
     return DMT_OK;
 }
 
 FrameArgs frame_args(const dmt_frame_params* f) {
-    return make_frame_args(f->pos_fovtan, f->fwd_aspect, f->right_xyz, f->up_xyz, f->width, f->height,
-                           f->frame_index, f->spp, f->max_bounces, f->reset_accum, f->mis_enabled,
-                           f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha, f->flags);
+    FrameArgs F = make_frame_args(f->pos_fovtan, f->fwd_aspect, f->right_xyz, f->up_xyz, f->width, f->height,
+                                  f->frame_index, f->spp, f->max_bounces, f->reset_accum, f->mis_enabled,
+                                  f->light_tree_enabled, f->refract_bounces, f->firefly_clamp, f->ema_alpha,
+                                  f->flags);
+    F.cauchy_c = f->cauchy_c;
+    return F;
+}
+
+// A frame that draws more than the jitter per sample (lens point, shutter
+// time, hero wavelength) or translates the mesh renders through k_paths.
+bool uses_features(const dmt_frame_params* f) {
+    return f->dof[0] > 0.0f || f->motion_blur || f->spectral || f->mesh_shift_active;
+}
+
+FeatureArgs feature_args(const dmt_frame_params* f) {
+    FeatureArgs X{};
+    X.aperture = f->dof[0];
+    X.focal = f->dof[1];
+    X.blades = (int)(f->dof[2] + 0.5f);   // int(dof_params.z + 0.5), PathTrace.slang:5789
+    X.rot = f->dof[3];
+    X.cat_eye = f->cat_eye;
+    X.motion_blur = f->motion_blur;
+    X.spectral = f->spectral;
+    return X;
+}
+
+// The scene as a frame sees it: the mesh translation is per frame.
+SceneView frame_view(const dmt_scene* s, const dmt_frame_params* f) {
+    SceneView V = s->view;
+    V.mesh_shift = f->mesh_shift_active;
+    V.mesh_prev = f3{f->mesh_shift_prev[0], f->mesh_shift_prev[1], f->mesh_shift_prev[2]};
+    V.mesh_curr = f3{f->mesh_shift_curr[0], f->mesh_shift_curr[1], f->mesh_shift_curr[2]};
+    return V;
 }
 
 dmt_status harvest_events(dmt_scene* s) {
@@ -432,6 +472,7 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     V.sph_nodes = f4;
     if ((st = upload(s, (const float4*)P.spheres.data(), P.sphere_ids.size(), &V.spheres, kLeafPad))) return bail(st);
     if ((st = upload(s, P.sphere_ids.data(), P.sphere_ids.size(), &V.sphere_ids))) return bail(st);
+    if ((st = upload(s, (const float4*)P.sphere_prev.data(), P.sphere_ids.size(), &V.sph_prev))) return bail(st);
     uint32_t tri_nodes = (uint32_t)P.tri_bvh.nodes.size(), tri_depth = P.tri_bvh.depth;
     std::vector<uint32_t> level_start;   // device build: first node of each level, node count last
     if (device_build && P.tri_count) {
@@ -631,7 +672,9 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
         o.tail_after = s->tail_after;
         o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
         o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
-        hipError_t e = launch_frame(s->view, F, s->wf[set], s->accum, stream, o);
+        const FeatureArgs X = feature_args(f);
+        if (uses_features(f)) o.features = &X;
+        hipError_t e = launch_frame(frame_view(s, f), F, s->wf[set], s->accum, stream, o);
         s->ev_used += o.events_used;
         if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
         if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "launch_frame");
@@ -791,7 +834,7 @@ dmt_status dmt_render_gbuffer(dmt_scene* s, const dmt_frame_params* f, const dmt
     bool resized = false;
     if (s->wf[0].capacity == 0u && (st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
     // on the device stream, as frame set 0, whose traversal spill area it shares
-    hipError_t e = launch_gbuffer(s->view, A, s->wf[0], s->dev->stream);
+    hipError_t e = launch_gbuffer(frame_view(s, f), A, s->wf[0], s->dev->stream);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "gbuffer");
     return DMT_OK;
 }
@@ -907,7 +950,9 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     o.tail_after = s->tail_after;
     o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
     o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
-    if (e == hipSuccess) e = launch_frame(s->view, F, W, scratch, s->dev->stream, o);
+    const FeatureArgs X = feature_args(f);
+    if (uses_features(f)) o.features = &X;
+    if (e == hipSuccess) e = launch_frame(frame_view(s, f), F, W, scratch, s->dev->stream, o);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
@@ -981,6 +1026,8 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
     dmt_status st = check_device(s->dev);
     if (st) return st;
     if ((st = check_frame(f))) return st;
+    if (uses_features(f))
+        return fail(DMT_ERR_UNSUPPORTED, "the adjoint renders without lens, motion blur, spectral or mesh translation");
     if (f->max_bounces > kAdjMaxVerts) return fail(DMT_ERR_LIMIT, "the adjoint supports max_bounces <= 8");
     if (f->refract_bounces > kAdjMaxDielectric) return fail(DMT_ERR_LIMIT, "the adjoint supports refract_bounces <= 8");
     const uint32_t prims = s->view.prim_count;
@@ -1189,6 +1236,9 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         if (!list[i]) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is null");
         if (list[i]->dev != dev) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is on another device");
         if ((st = check_frame(&frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
+        if (uses_features(&frames[i]))
+            return fail(DMT_ERR_UNSUPPORTED, "frame " + std::to_string(i) +
+                                                 ": batches render without lens, motion blur, spectral or mesh translation");
     }
     {
         std::vector<dmt_scene*> sorted = list;

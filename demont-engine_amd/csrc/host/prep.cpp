@@ -69,12 +69,17 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
             S.planes.insert(S.planes.end(), {n[0], n[1], n[2], p[3]});
             S.plane_ids.push_back(i);
         } else {
-            if (p[12] != p[0] || p[13] != p[1] || p[14] != p[2])
-                return "prim " + std::to_string(i) + ": motion blur (v3 != v0) is not supported";
+            // The box holds the sphere at its previous centre (v3) and its
+            // current one (v0), between which motion blur moves it
+            // (PathTrace.slang:2093; Engine.cpp:4861-4898 sweeps the same
+            // volume), widened by 2^-20 of the coordinates so that the
+            // rounding of the shutter-time lerp stays inside.
             BoundsItem b{};
             for (int a = 0; a < 3; ++a) {
-                b.lo[a] = p[a] - p[3];
-                b.hi[a] = p[a] + p[3];
+                const float lo = std::fmin(p[a], p[12 + a]), hi = std::fmax(p[a], p[12 + a]);
+                const float pad = (std::fmax(std::fabs(lo), std::fabs(hi)) + p[3]) * 9.5367431640625e-7f;
+                b.lo[a] = (lo - p[3]) - pad;
+                b.hi[a] = (hi + p[3]) + pad;
                 b.c[a] = p[a];
             }
             sph.push_back(b);
@@ -84,14 +89,17 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
     if (!sph.empty()) {
         S.sphere_bvh = build_bvh4(sph, bins, max_leaf ? max_leaf : 4u);
         S.spheres.resize(sph.size() * 4);
+        S.sphere_prev.resize(sph.size() * 4);
         S.sphere_ids.resize(sph.size());
         for (size_t k = 0; k < sph.size(); ++k) {
             uint32_t id = sph_ids[S.sphere_bvh.order[k]];
             const float* p = prims + (size_t)id * 28u;
-            S.spheres[4 * k + 0] = p[0];
-            S.spheres[4 * k + 1] = p[1];
-            S.spheres[4 * k + 2] = p[2];
+            for (int a = 0; a < 3; ++a) {
+                S.spheres[4 * k + a] = p[a];
+                S.sphere_prev[4 * k + a] = p[12 + a];
+            }
             S.spheres[4 * k + 3] = p[3];
+            S.sphere_prev[4 * k + 3] = p[3];
             S.sphere_ids[k] = id;
         }
     }

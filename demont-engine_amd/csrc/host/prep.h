@@ -67,6 +67,7 @@ struct PreparedScene {
     std::vector<uint32_t> plane_ids;      // input index per plane
     Bvh4 sphere_bvh;
     std::vector<float> spheres;           // float4 (c.xyz, r) in leaf order
+    std::vector<float> sphere_prev;       // float4 (previous c.xyz, r) in leaf order
     std::vector<uint32_t> sphere_ids;     // input index, leaf order
     // mesh
     uint32_t tri_count = 0;

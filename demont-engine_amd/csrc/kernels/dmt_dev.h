@@ -73,6 +73,12 @@ template <class T> __device__ __forceinline__ T maxc(V3<T> a) { return fmax_(a.x
 template <class T> __device__ __forceinline__ bool any_pos(V3<T> a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
 template <class T> __device__ __forceinline__ T clampf(T x, float lo, float hi) { return fmin_(fmax_(x, T(lo)), T(hi)); }
 template <class T> __device__ __forceinline__ V3<T> lerp3(V3<T> a, V3<T> b, T t) { return a + (b - a) * t; }
+// The shutter-time lerp of motion blur, a (1 - t) + b t: exactly b at t = 1,
+// so a sample without motion blur (t = 1) sees the current positions.
+__device__ __forceinline__ f3 mix3(f3 a, f3 b, float t) {
+    const float u = 1.0f - t;
+    return f3{a.x * u + b.x * t, a.y * u + b.y * t, a.z * u + b.z * t};
+}
 // quatRotate (PathTrace.slang:2028-2031).
 __device__ __forceinline__ f3 quat_rotate(float4 q, f3 v) {
     f3 u{q.x, q.y, q.z};
@@ -158,6 +164,23 @@ __device__ __forceinline__ float c_atan2(float y, float x) {
     return w + c_atan(y / x);
 }
 
+// Cephes expf: n = floor(x log2(e) + 1/2), x - n ln 2 in two parts, the
+// degree-6 minimax polynomial, the exact scaling by 2^n.  Below -87 the result
+// is 0 (no denormals are formed), above 88.72 it is +inf.
+__device__ __forceinline__ float c_exp(float xin) {
+    if (xin > 88.72283905206835f) return __int_as_float(0x7f800000);
+    if (xin < -87.0f) return 0.0f;
+    float z = floorf(1.44269504088896341f * xin + 0.5f);
+    float x = xin - z * 0.693359375f;
+    x = x - z * -2.12194440e-4f;
+    const int n = (int)z;
+    const float xx = x * x;
+    float p = (((((1.9875691500e-4f * x + 1.3981999507e-3f) * x + 8.3334519073e-3f) * x + 4.1665795894e-2f) * x +
+                1.6666665459e-1f) * x + 5.0000001201e-1f);
+    p = (p * xx + x) + 1.0f;
+    return p * __uint_as_float((uint32_t)(n + 127) << 23);
+}
+
 // ---------------------------------------------------------------------------
 // RNG (PathTraceMath.slang:16-24)
 
@@ -201,7 +224,9 @@ struct SceneView {
     float env_total;
     float env_intensity;
     uint32_t env_present;
-    uint32_t pad;
+    uint32_t mesh_shift;          // the mesh is translated (set per frame)
+    const float4* sph_prev;       // (previous c.xyz, r), leaf order
+    f3 mesh_prev, mesh_curr;      // its translation at shutter times 0 and 1
 };
 
 // Traversal stack: `lds_depth` entries in LDS (thread-interleaved), the rest
@@ -395,17 +420,22 @@ __device__ __forceinline__ f3 safe_inverse(f3 d) {
 }
 
 // Up to four spheres of a leaf held in the fetch (intersectSphere: closest
-// root > 1e-3); returns how many it tested.
-template <bool COUNT>
+// root > 1e-3); returns how many it tested.  With MOTION, a ray at shutter
+// time < 1 sees each centre at lerp(previous, current, time)
+// (PathTrace.slang:2093).
+template <bool COUNT, bool MOTION = false>
 __device__ __forceinline__ uint32_t sphere_leaf(const SceneView& S, f3 o, f3 d, uint32_t first, uint32_t count,
-                                                const float4 (&r)[kFetch], AnalyticBest& b, TraceCounters& tc) {
+                                                const float4 (&r)[kFetch], AnalyticBest& b, TraceCounters& tc,
+                                                float time = 1.0f) {
     const uint32_t n = count < 4u ? count : 4u;
 #pragma unroll
     for (uint32_t k = 0; k < 4u; ++k) {
         if (k >= n) break;
         const float4 s = r[k];
         if (COUNT) tc.spheres++;
-        f3 oc = o - xyz(s);
+        f3 c = xyz(s);
+        if (MOTION && time < 1.0f) c = mix3(xyz(S.sph_prev[first + k]), c, time);
+        f3 oc = o - c;
         float bb = dot(oc, d);
         float kk = dot(oc, oc) - s.w * s.w;
         float h = bb * bb - kk;
@@ -461,21 +491,27 @@ __device__ __forceinline__ HitRec final_hit(const AnalyticBest& a, const TriBest
 // traceScene (PathTrace.slang:2331-2693): planes, then the sphere BVH, then
 // the triangle BVH limited to the analytic hit distance.  Ties resolve to
 // the lower primitive index; analytic hits win ties against triangles.
-template <bool COUNT>
-__device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, Stack& st, TraceCounters& tc) {
+// With MOTION the query is at shutter time `time`: moving spheres at their
+// lerped centres, and a translated mesh intersected from the origin shifted
+// by -lerp(mesh_prev, mesh_curr, time) (:2483-2487).
+template <bool COUNT, bool MOTION = false>
+__device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, Stack& st, TraceCounters& tc,
+                                                float time = 1.0f) {
     AnalyticBest ab = planes_closest(S, o, d);
     const f3 inv = safe_inverse(d);
     if (S.sph_nodes) {
         bvh4_walk<COUNT, 1u>(S.sph_nodes, S.spheres, o, inv, ab.t, st, [&](uint32_t first, uint32_t count,
                                                                           const float4 (&r)[kFetch]) {
-            return sphere_leaf<COUNT>(S, o, d, first, count, r, ab, tc);
+            return sphere_leaf<COUNT, MOTION>(S, o, d, first, count, r, ab, tc, time);
         }, tc);
     }
     TriBest tb{ab.t, 0u, kMiss};
     if (S.tri_nodes) {
-        bvh4_walk<COUNT, 3u>(S.tri_nodes, S.tris, o, inv, tb.t, st, [&](uint32_t first, uint32_t count,
-                                                                         const float4 (&r)[kFetch]) {
-            return tri_leaf<COUNT>(o, d, first, count, r, tb, tc);
+        f3 om = o;
+        if (MOTION && S.mesh_shift) om = o - mix3(S.mesh_prev, S.mesh_curr, time);
+        bvh4_walk<COUNT, 3u>(S.tri_nodes, S.tris, om, inv, tb.t, st, [&](uint32_t first, uint32_t count,
+                                                                          const float4 (&r)[kFetch]) {
+            return tri_leaf<COUNT>(om, d, first, count, r, tb, tc);
         }, tc);
     }
     return final_hit(ab, tb);
@@ -653,7 +689,9 @@ template <class T> __device__ __forceinline__ void surface_material(const SceneV
     }
 }
 
-__device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, HitRec h) {
+// The normal of a sphere hit at shutter time `time` is taken about the
+// centre the ray saw (PathTrace.slang:2093-2097).
+__device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, HitRec h, float time = 1.0f) {
     Surface s;
     surface_material(S, h, s);
     s.t = h.t;
@@ -664,13 +702,17 @@ __device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, Hi
         const float4* p = S.prims + (size_t)h.code * 7u;
         const float4 v0 = p[0], v2 = p[2];
         if (v2.x == 0.0f) {
-            s.n = normalize((o + d * h.t) - xyz(v0));
+            f3 c = xyz(v0);
+            if (time < 1.0f) c = mix3(xyz(p[3]), c, time);
+            s.n = normalize((o + d * h.t) - c);
         } else {
             s.n = quat_rotate(p[5], xyz(v0));
         }
     }
     return s;
 }
-__device__ __forceinline__ Surface surface_of(const SceneView& S, f3 o, f3 d, HitRec h) { return surface_at(S, o, d, h); }
+__device__ __forceinline__ Surface surface_of(const SceneView& S, f3 o, f3 d, HitRec h, float time = 1.0f) {
+    return surface_at(S, o, d, h, time);
+}
 
 }  // namespace dmt

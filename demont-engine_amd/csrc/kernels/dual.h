@@ -62,8 +62,9 @@ __device__ __forceinline__ f3 deriv3(d3 a) { return f3{a.x.d, a.y.d, a.z.d}; }
 // Surface of a hit along a dual ray: the hit distance keeps the traversal's
 // value h.t and takes its derivative from the primitive's implicit equation
 // g(o + t d) = 0, dt = -(dg . do + t dg . dd) / (dg . d), with dg the sphere's
-// radius vector or the plane / triangle normal at the hit.
-__device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 d, HitRec h) {
+// radius vector or the plane / triangle normal at the hit.  The adjoint
+// renders without motion blur: `time` is always 1.
+__device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 d, HitRec h, float time = 1.0f) {
     SurfaceT<df> s;
     surface_material(S, h, s);
     const f3 ov = value3(o), dv = value3(d), od = deriv3(o), dd = deriv3(d);

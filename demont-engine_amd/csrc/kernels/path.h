@@ -30,6 +30,124 @@ __device__ __forceinline__ uint32_t pixel_seed(const FrameArgs& F, uint32_t x, u
     return pcg_hash(x * 1973u + y * 9277u + frame * 26699u);
 }
 
+// cauchyIOR (PathTraceMath.slang:111-119): n = B + C / lambda^2 through
+// (base, 587.6 nm), lambda in micrometres.
+__device__ __forceinline__ float cauchy_ior(float base, float cc, float lambda_nm) {
+    const float lum = lambda_nm * 0.001f;
+    const float l2 = lum * lum;
+    const float ld2 = 0.5876f * 0.5876f;
+    const float B = base - cc / ld2;
+    return B + cc / l2;
+}
+
+// wymanGauss, cieXYZFit, xyzToLinearSRGB, spectralWeightSRGB
+// (PathTraceMath.slang:140-192): the Wyman-Sloan-Shirley fit of the CIE 1931
+// matching functions, to linear sRGB, over the band means.
+__device__ __forceinline__ float wyman(float x, float mu, float s1, float s2) {
+    const float t = (x - mu) * ((x < mu) ? s1 : s2);
+    return c_exp((-0.5f * t) * t);
+}
+__device__ __forceinline__ f3 spectral_weight(float l) {
+    const float X = (1.056f * wyman(l, 599.8f, 0.0264f, 0.0323f) + 0.362f * wyman(l, 442.0f, 0.0624f, 0.0374f)) -
+                    0.065f * wyman(l, 501.1f, 0.0490f, 0.0382f);
+    const float Y = 0.821f * wyman(l, 568.8f, 0.0213f, 0.0247f) + 0.286f * wyman(l, 530.9f, 0.0613f, 0.0322f);
+    const float Z = 1.217f * wyman(l, 437.0f, 0.0845f, 0.0278f) + 0.681f * wyman(l, 459.0f, 0.0385f, 0.0725f);
+    const float r = (3.2404542f * X - 1.5371385f * Y) - 0.4985314f * Z;
+    const float g = (-0.9692660f * X + 1.8760108f * Y) + 0.0415560f * Z;
+    const float b = (0.0556434f * X - 0.2040259f * Y) + 1.0572252f * Z;
+    return f3{r * 2.730792f, g * 3.443936f, b * 3.606855f};
+}
+
+// ptConcentricDisk (PathTraceMath.slang:223-236).
+__device__ __forceinline__ void concentric_disk(float u1, float u2, float& x, float& y) {
+    const float a = 2.0f * u1 - 1.0f;
+    const float b = 2.0f * u2 - 1.0f;
+    const float qpi = 3.14159265358979f / 4.0f, hpi = 3.14159265358979f / 2.0f;
+    float r, phi;
+    if (fabsf(a) > fabsf(b)) {
+        r = a;
+        phi = qpi * (b / fmaxf(fabsf(a), 1e-6f));
+    } else {
+        r = b;
+        phi = hpi - qpi * (a / fmaxf(fabsf(b), 1e-6f));
+    }
+    float sn, cs;
+    cephes_sincos(phi, &sn, &cs);
+    x = r * cs;
+    y = r * sn;
+}
+
+// ptPolygonSample (PathTraceMath.slang:268-286): a wedge of the regular
+// N-gon picked by u1, then the sqrt-barycentric point of that triangle.
+__device__ __forceinline__ void polygon_sample(float u1, float u2, int blades, float rot, float& x, float& y) {
+    const float n = (float)blades;
+    const float wedge = 6.28318530717958f / n;
+    const float kf = floorf(u1 * n);
+    const float ku = u1 * n - kf;
+    const float a0 = rot + kf * wedge;
+    const float a1 = a0 + wedge;
+    float s0, c0, s1, c1;
+    cephes_sincos(a0, &s0, &c0);
+    cephes_sincos(a1, &s1, &c1);
+    const float su = sqrtf(ku);
+    const float w1 = 1.0f - u2;
+    x = su * (w1 * c0 + u2 * c1);
+    y = su * (w1 * s0 + u2 * s1);
+}
+
+// ptCatEye (PathTraceMath.slang:296-310): the radial component of the lens
+// sample narrowed with the image eccentricity |uv|.
+__device__ __forceinline__ void cat_eye(float& x, float& y, float ux, float uy, float strength) {
+    const float len = sqrtf(ux * ux + uy * uy);
+    const float ecc = fminf(fmaxf(len, 0.0f), 1.0f);
+    const float squeeze = strength * ecc;
+    if (squeeze < 1e-5f) return;
+    float rx = 1.0f, ry = 0.0f;
+    if (ecc > 1e-5f) { rx = ux / len; ry = uy / len; }
+    const float tx = -ry, ty = rx;
+    float ar = x * rx + y * ry;
+    const float at = x * tx + y * ty;
+    ar = ar * (1.0f - 0.7f * squeeze);
+    x = rx * ar + tx * at;
+    y = ry * ar + ty * at;
+}
+
+// Camera ray of sample s of pixel (x, y) with the camera-sample features
+// (PathTrace.slang:5729-5816): the jitter, then, each only when its feature
+// is on, the hero wavelength (stratified over the spp samples), the shutter
+// time and the lens point.  Without them: camera_ray, time 1, lambda -1.
+__device__ __forceinline__ void camera_sample(const FrameArgs& F, const FeatureArgs& X, uint32_t x, uint32_t y,
+                                              uint32_t s, uint32_t& seed, f3& o, f3& d, float& time, float& lambda) {
+    float jx = randf(seed);
+    float jy = randf(seed);
+    jx = jx - 0.5f;
+    jy = jy - 0.5f;
+    const float ux = ((((float)x + 0.5f) + jx) / (float)F.width) * 2.0f - 1.0f;
+    float uy = ((((float)y + 0.5f) + jy) / (float)F.height) * 2.0f - 1.0f;
+    uy = -uy;
+    lambda = -1.0f;
+    if (X.spectral) {
+        const float u = ((float)s + randf(seed)) / (float)F.spp;
+        lambda = 380.0f + u * (730.0f - 380.0f);
+    }
+    time = 1.0f;
+    if (X.motion_blur) time = randf(seed);
+    o = F.pos;
+    d = normalize((F.fwd + F.right * ((ux * F.aspect) * F.fovtan)) + F.up * (uy * F.fovtan));
+    if (X.aperture > 0.0f) {
+        const float u1 = randf(seed);
+        const float u2 = randf(seed);
+        float ax, ay;
+        if (X.blades >= 3) polygon_sample(u1, u2, X.blades, X.rot, ax, ay);
+        else concentric_disk(u1, u2, ax, ay);
+        if (X.cat_eye > 0.0f) cat_eye(ax, ay, ux, uy, X.cat_eye);
+        const f3 lens = F.right * (ax * X.aperture) + F.up * (ay * X.aperture);
+        const f3 fp = o + d * X.focal;
+        o = o + lens;
+        d = normalize(fp - o);
+    }
+}
+
 // A shadow query and the factors of its contribution.  kind 0 = light NEE,
 // 1 = env NEE; pdf < 0 marks the no-env ambient term (added without tracing).
 template <class T> struct ShadowRecT {
@@ -49,6 +167,8 @@ template <class T> struct PathVertexStateT {
     T prev_pdf;
     bool skip_sky, prev_lambert;
     uint32_t seed;
+    float time = 1.0f;      // shutter time of the sample (1: no motion blur)
+    float lambda = -1.0f;   // hero wavelength in nm (< 0: RGB)
 };
 
 using PathVertexState = PathVertexStateT<float>;
@@ -103,7 +223,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
         }
         return;
     }
-    SurfaceT<T> sf = surface_of(S, o, d, h);
+    SurfaceT<T> sf = surface_of(S, o, d, h, P.time);
     sink.on_surface(b, sf);
     const V3<T> hp = o + d * sf.t;
     const T ndi = dot(sf.n, d);
@@ -194,8 +314,10 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
         P.prev_lambert = false;
         R.alive = true;
     } else {
-        float ia = (ndi < 0.0f) ? 1.0f : sf.ior;
-        float ib = (ndi < 0.0f) ? sf.ior : 1.0f;
+        // the hero wavelength's own index (PathTrace.slang:6721-6724)
+        const float ior = (P.lambda >= 0.0f) ? cauchy_ior(sf.ior, F.cauchy_c, P.lambda) : sf.ior;
+        float ia = (ndi < 0.0f) ? 1.0f : ior;
+        float ib = (ndi < 0.0f) ? ior : 1.0f;
         float eta = ia / ib;
         T ci = fabs_(dot(nf, -d));
         float r0 = (ia - ib) / (ia + ib);
@@ -231,9 +353,9 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
 // ray through the dielectric surface.
 template <class T>
 __device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArgs& F, HitRec h, uint32_t i, V3<T>& o,
-                                              V3<T>& d, T& t_max, V3<T>& t) {
+                                              V3<T>& d, T& t_max, V3<T>& t, float time = 1.0f) {
     if (h.code == kMiss || h.t >= primal(t_max)) return true;
-    const SurfaceT<T> sf = surface_of(S, o, d, h);
+    const SurfaceT<T> sf = surface_of(S, o, d, h, time);
     if (sf.mat != 2u || i == F.refract_bounces) {
         t = to<T>(f3{0.0f, 0.0f, 0.0f});
         return true;
@@ -262,15 +384,15 @@ __device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArg
 }
 
 // transmittance: closest hits along the ray, refracting through dielectrics,
-// at most refract_bounces + 1 queries.
-template <bool COUNT>
+// at most refract_bounces + 1 queries (at shutter time `time` with MOTION).
+template <bool COUNT, bool MOTION = false>
 __device__ __forceinline__ f3 transmittance(const SceneView& S, const FrameArgs& F, f3 o, f3 d, float t_max,
-                                            Stack& st, TraceCounters& tc, uint32_t& steps) {
+                                            Stack& st, TraceCounters& tc, uint32_t& steps, float time = 1.0f) {
     f3 t{1.0f, 1.0f, 1.0f};
     for (uint32_t i = 0; i <= F.refract_bounces; ++i) {
         steps++;
-        HitRec h = trace_closest<COUNT>(S, o, d, st, tc);
-        if (transmit_step(S, F, h, i, o, d, t_max, t)) return t;
+        HitRec h = trace_closest<COUNT, MOTION>(S, o, d, st, tc, time);
+        if (transmit_step(S, F, h, i, o, d, t_max, t, time)) return t;
     }
     return f3{0.0f, 0.0f, 0.0f};
 }

@@ -22,6 +22,16 @@ struct FrameArgs {
     float firefly_clamp;
     float ema_alpha;
     uint32_t rr_enabled;      // Russian roulette from bounce 2 on
+    float cauchy_c;           // Cauchy C (um^2) of the spectral dielectric index
+};
+
+// Camera-sample features of a frame (all off: the plain camera ray).
+struct FeatureArgs {
+    float aperture, focal, rot, cat_eye;   // thin lens (dof_params, dof_bokeh_params.x)
+    int blades;                            // < 3: round aperture
+    uint32_t motion_blur;                  // draw a shutter time per sample
+    uint32_t spectral;                     // draw a hero wavelength per sample
+    uint32_t pad;
 };
 
 // clampIndirect (PathTrace.slang:3367-3371).

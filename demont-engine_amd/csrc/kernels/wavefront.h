@@ -131,6 +131,9 @@ struct LaunchOptions {
     uint32_t tail_after = kNoTail;        // iterations before k_tail finishes the frame
     uint32_t tail_blocks = 0;             // k_tail grid (0: grid_for(pixels))
     uint32_t extend_blocks = 0;           // k_extend grid cap (0: grid_for(pixels))
+    // Camera-sample features (thin lens, shutter time, hero wavelength) or a
+    // translated mesh: the frame renders through the per-pixel kernel k_paths.
+    const FeatureArgs* features = nullptr;
 };
 
 // Render one frame into accum; the final pass that folds the frame's values

@@ -350,6 +350,81 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, Pat
     }
 }
 
+// Frames with camera-sample features (thin lens, motion blur, hero
+// wavelength) or a translated mesh: one thread per pixel runs the pixel's
+// samples in order, each path to its end, its shadow queries traced in place
+// after the vertex that formed them (light NEE, then env NEE or the ambient
+// term: the order in which the wavefront form adds them).  The sample's
+// shutter time and wavelength ride in its path state; the frame value is the
+// spectrally weighted sample mean (PathTrace.slang:7693-7707).
+struct InlineSink : NoGradientHooks {
+    ShadowRec rec[2];
+    bool emitted[2];
+    __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState&) {
+        rec[kind] = r;
+        emitted[kind] = true;
+    }
+};
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_paths(SceneView S, FrameArgs F, FeatureArgs X, PathState ps,
+                                                  uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    Stack stk = make_stack(lds, spill, spill_depth);
+    TraceCounters tc{0u, 0u, 0u};
+    uint32_t n_primary = 0, n_ext = 0, n_shadow = 0;
+    const uint32_t npix = F.width * F.height;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += stride) {
+        const uint32_t x = p % F.width, y = p / F.width;
+        uint32_t seed = pixel_seed(F, x, y, F.frame_index);
+        f3 sum{0.0f, 0.0f, 0.0f};
+        for (uint32_t s = 0; s < F.spp; ++s) {
+            f3 o, d;
+            float time, lambda;
+            camera_sample(F, X, x, y, s, seed, o, d, time, lambda);
+            PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed, time, lambda};
+            for (uint32_t b = 0; b < F.max_bounces; ++b) {
+                if (b == 0u) n_primary++;
+                else n_ext++;
+                const HitRec h = trace_closest<COUNT, true>(S, o, d, stk, tc, time);
+                InlineSink sink{{}, {}, {false, false}};
+                VertexResult R;
+                shade_vertex(S, F, b, o, d, h, P, R, sink);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    if (!sink.emitted[k]) continue;
+                    const ShadowRec& r = sink.rec[k];
+                    if (k == 1 && r.pdf < 0.0f) {   // the no-env ambient term: no query
+                        P.L = P.L + r.pre;
+                        continue;
+                    }
+                    uint32_t steps = 0;
+                    const f3 tr = transmittance<COUNT, true>(S, F, r.o, r.d, r.t_max, stk, tc, steps, time);
+                    n_shadow += steps;
+                    if (any_pos(tr))
+                        P.L = P.L + shadow_value(F, k, r.pre, r.ndl, r.brdf, r.pdf, k == 1 ? r.w : 1.0f, r.clamp, tr);
+                }
+                if (!(R.alive && (b + 1u) < F.max_bounces)) break;
+                o = R.no;
+                d = R.nd;
+            }
+            seed = P.seed;
+            sum = sum + ((lambda >= 0.0f) ? P.L * spectral_weight(lambda) : P.L);
+        }
+        const f3 v = frame_value(F, sum);
+        ps.F[p] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
+    add_stat(&st->primary, n_primary);
+    add_stat(&st->extension, n_ext);
+    add_stat(&st->shadow, n_shadow);
+    if (COUNT) {
+        add_stat(&st->nodes, tc.nodes);
+        add_stat(&st->tris, tc.tris);
+        add_stat(&st->spheres, tc.spheres);
+    }
+}
+
 // Sample bookkeeping of the paths that ended this iteration: the next sample
 // of the pixel is flagged (k_regen issues it in 8x8 tile order so the new
 // waves stay coherent); after the last sample the frame's value goes to ps.F
@@ -474,7 +549,9 @@ __global__ void __launch_bounds__(kBlock) k_gbuffer(SceneView S, GbufferArgs A, 
         float uy = (((float)y + 0.5f) / H) * 2.0f - 1.0f;
         uy = -uy;
         const f3 d = normalize((F.fwd + F.right * ((ux * F.aspect) * F.fovtan)) + F.up * (uy * F.fovtan));
-        const HitRec h = trace_closest<false>(S, F.pos, d, stk, tc);
+        // shutter time 1: a translated mesh at its current translation
+        // (PathTrace.slang:5470-5478)
+        const HitRec h = trace_closest<false, true>(S, F.pos, d, stk, tc, 1.0f);
         f3 world, nrm{0.0f, 0.0f, 0.0f}, alb{0.0f, 0.0f, 0.0f}, f0{0.0f, 0.0f, 0.0f};
         float depth, rough = 1.0f, dist = 0.0f;
         if (h.code != kMiss) {
@@ -542,6 +619,7 @@ FrameArgs make_frame_args(const float pf[4], const float fa[4], const float rt[4
     F.firefly_clamp = clamp;
     F.ema_alpha = ema;
     F.rr_enabled = (flags & 1u) ? 0u : 1u;   // DMT_FRAME_NO_RUSSIAN_ROULETTE
+    F.cauchy_c = 0.0f;
     return F;
 }
 
@@ -585,6 +663,20 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
     const uint32_t g = grid_for(npix);
     o.events_used = 0;
     o.iterations = 0;
+    if (o.features) {
+        if (o.counting)
+            hipLaunchKernelGGL(k_paths<true>, dim3(g), dim3(kBlock), 0, stream, S, F, *o.features, W.ps, W.spill,
+                               W.spill_depth, W.stats);
+        else
+            hipLaunchKernelGGL(k_paths<false>, dim3(g), dim3(kBlock), 0, stream, S, F, *o.features, W.ps, W.spill,
+                               W.spill_depth, W.stats);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        o.iterations = 1;
+        if (o.accum_after) (void)hipStreamWaitEvent(stream, o.accum_after, 0);
+        hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, W.rq[0], W.counters);
     // Every path ends after at most max_bounces vertices and a pixel's samples
     // run in order, so the frame needs at most spp * max_bounces iterations.

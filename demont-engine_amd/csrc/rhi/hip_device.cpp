@@ -91,7 +91,19 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     r.env_map_present = u32(p, kEnvMapPresent) != 0u;
     r.env_intensity = f32(p, kEnvIntensity);
     r.tonemap_op = u32(p, kTonemapParams);
-    r.motion_blur = f32(p, kMotionBlurParams) != 0.0f;
+    // camera-sample features: thin lens (PathTrace.slang:5786-5816), shutter
+    // time (:5771), hero wavelength (:5756), mesh translation (:2483)
+    for (int i = 0; i < 4; ++i) F.dof[i] = f32(p, kDofParams + 4 * i);
+    F.cat_eye = f32(p, kDofBokehParams);
+    r.motion_blur = f32(p, kMotionBlurParams) > 0.5f;
+    F.motion_blur = r.motion_blur;
+    F.spectral = f32(p, kSpectralParams) > 0.5f;
+    F.cauchy_c = f32(p, kSpectralParams + 4);
+    for (int i = 0; i < 3; ++i) {
+        F.mesh_shift_prev[i] = f32(p, kMeshMotionPrev + 4 * i);
+        F.mesh_shift_curr[i] = f32(p, kMeshMotionCurr + 4 * i);
+    }
+    F.mesh_shift_active = f32(p, kMeshMotionPrev + 12) > 0.5f;
     r.denoiser = u32(p, kDenoiserEnabled) != 0u;
     r.write_normal = u32(p, kWriteNormalGbuffer) != 0u;
     r.write_albedo = u32(p, kWriteAlbedoGbuffer) != 0u;
@@ -125,13 +137,10 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
         need(!(f32(p, kMoonDirPhase + 4) < -0.10f), "moon above the horizon without an env map");
     }
     need(r.env_map_present && u32(p, kHdriLightsCount) != 0u, "HDRI cluster lights");       // :3680, :3752
-    need(f32(p, kDofParams) > 0.0f, "depth of field");                                       // :5786
-    need(f32(p, kSpectralParams) > 0.5f, "spectral dispersion");                             // :5756
     need(f32(p, kVolParams) > 0.0f || f32(p, kCloudsP1 + 12) > 0.0f || f32(p, kCloudsP3 + 12) > 0.0f,
          "participating media (fog, clouds, Rayleigh)");                                     // :7011
     need(u32(p, kSdfParams) != 0u, "SDF clusters");                                          // :2224
     need(u32(p, kRestirEnabled) != 0u, "ReSTIR");                                            // :6315
-    need(r.motion_blur, "motion blur");                                                      // :1029
     need(u32(p, kMatOverrideCount) != 0u, "material overrides");                             // :1164
     need(f32(p, kSmokeParams) != 0.0f, "smoke volumes");                                     // :981
     need(u32(p, kWriteShadowVis) != 0u, "SIGMA shadow visibility (write_shadow_vis)");       // :5693-5702
@@ -693,10 +702,9 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
         return true;
     }
 
-    // The primitives as the scene takes them: v3 (previous centre) reads as v0
-    // without motion blur, as the shader does with motion_blur_params.x = 0
-    // (PathTrace.slang:1029-1033); PBR texture tiles in v6 are outside this
-    // backend (strict: refused).
+    // The primitives as the scene takes them (v3, the previous centre, is the
+    // start of the motion-blur shutter); PBR texture tiles in v6 are outside
+    // this backend (strict: refused).
     std::vector<float> pr((std::size_t)P.prim_count * kPrimFloats);
     if (P.prim_count) std::memcpy(pr.data(), prims->data.data() + k.prims_off, pr.size() * 4u);
     for (std::uint32_t i = 0; i < P.prim_count; ++i) {
@@ -712,9 +720,6 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
             if (ignored.find("PBR primitive textures") == std::string::npos)
                 ignored += ignored.empty() ? "PBR primitive textures" : ", PBR primitive textures";
         }
-        q[12] = q[0];
-        q[13] = q[1];
-        q[14] = q[2];
     }
 
     SceneKey only_prims = key;

@@ -24,7 +24,7 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
 SCHEDULE_WAVEFRONT = 0xFFFFFFFF
 MATERIAL_FLOATS = 7
@@ -51,6 +51,15 @@ class FrameParams(C.Structure):
         ("firefly_clamp", C.c_float),
         ("ema_alpha", C.c_float),
         ("flags", C.c_uint32),
+        ("dof", C.c_float * 4),
+        ("cat_eye", C.c_float),
+        ("motion_blur", C.c_uint32),
+        ("spectral", C.c_uint32),
+        ("cauchy_c", C.c_float),
+        ("mesh_shift_prev", C.c_float * 3),
+        ("mesh_shift_active", C.c_uint32),
+        ("mesh_shift_curr", C.c_float * 3),
+        ("reserved", C.c_uint32),
     ]
 
     def copy(self) -> "FrameParams":

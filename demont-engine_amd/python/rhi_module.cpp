@@ -45,6 +45,13 @@ py::dict decoded(const PathtracePush& P) {
     d["env_intensity"] = P.env_intensity;
     d["tonemap_op"] = P.tonemap_op;
     d["motion_blur"] = P.motion_blur;
+    d["dof"] = std::vector<float>(F.dof, F.dof + 4);
+    d["cat_eye"] = F.cat_eye;
+    d["spectral"] = F.spectral;
+    d["cauchy_c"] = F.cauchy_c;
+    d["mesh_shift_prev"] = std::vector<float>(F.mesh_shift_prev, F.mesh_shift_prev + 3);
+    d["mesh_shift_curr"] = std::vector<float>(F.mesh_shift_curr, F.mesh_shift_curr + 3);
+    d["mesh_shift_active"] = F.mesh_shift_active;
     d["denoiser"] = P.denoiser;
     d["write_normal"] = P.write_normal;
     d["write_albedo"] = P.write_albedo;

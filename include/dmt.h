@@ -23,7 +23,7 @@
  *     v1 = albedo.rgb, roughness
  *     v2 = type (0 sphere, 1 plane), material (0 Lambert, 1 metal, 2 dielectric),
  *          ior, user id (uint32 bits)
- *     v3 = previous centre (motion blur; must equal v0 -- no motion blur)
+ *     v3 = previous centre (the motion-blur shutter lerps from it to v0)
  *     v4 = emission.rgb, 0
  *     v5 = orientation quaternion xyzw (planes: normal = quatRotate(q, v0.xyz))
  *     v6 = texture tiles (ignored: no PBR atlas)
@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define DMT_ABI_VERSION 3u
+#define DMT_ABI_VERSION 4u
 
 typedef int32_t dmt_status;
 #define DMT_OK                  0
@@ -93,6 +93,26 @@ typedef struct dmt_frame_params {
     float    firefly_clamp;     /* r_firefly_clamp, <= 0 disables            */
     float    ema_alpha;         /* r_accum_ema_alpha                         */
     uint32_t flags;             /* DMT_FRAME_* bits; 0 = reference behaviour */
+    /* Camera-sample features, all off when zero.  Per sample, after the
+     * jitter, the reference draws the hero wavelength (spectral), then the
+     * shutter time (motion_blur), then the lens point (dof[0] > 0)
+     * (PathTrace.slang:5729-5816).  A frame that uses any of them renders
+     * through the per-pixel feature kernel. */
+    float    dof[4];            /* dof_params: aperture radius (0 = pinhole),
+                                   focal distance, blade count (< 3 = round),
+                                   blade rotation in radians               */
+    float    cat_eye;           /* dof_bokeh_params.x, 0..1                  */
+    uint32_t motion_blur;       /* motion_blur_params.x: a shutter time t per
+                                   sample; spheres at lerp(prev_pos, pos, t) */
+    uint32_t spectral;          /* spectral_params.x: hero wavelength per
+                                   sample, dielectric index by Cauchy's law  */
+    float    cauchy_c;          /* spectral_params.y, Cauchy C in um^2       */
+    float    mesh_shift_prev[3];/* mesh_motion_prev.xyz                      */
+    uint32_t mesh_shift_active; /* mesh_motion_prev.w > 0.5: the mesh is drawn
+                                   translated by lerp(prev, curr, t), t = 1
+                                   without motion blur (PathTrace.slang:2483) */
+    float    mesh_shift_curr[3];/* mesh_motion_curr.xyz                      */
+    uint32_t reserved;          /* 0                                         */
 } dmt_frame_params;
 
 /* Russian roulette off (the reference applies it from bounce 2 on,
@@ -315,8 +335,9 @@ dmt_status dmt_scene_set_materials(dmt_scene* scene, const float* materials, siz
  * derivative of the estimator away from visibility boundaries (decisions
  * held fixed).  With Russian roulette on, its probability is treated as a
  * constant (set DMT_FRAME_NO_RUSSIAN_ROULETTE for a gradient that finite
- * differences reproduce).  Limits: max_bounces <= 8, refract_bounces <= 8.
- * Stream-ordered on the device stream; d_image_grad must be complete when
+ * differences reproduce).  Limits: max_bounces <= 8, refract_bounces <= 8;
+ * the camera-sample features (dof, motion_blur, spectral, mesh_shift_active)
+ * are DMT_ERR_UNSUPPORTED.  Stream-ordered on the device stream; d_image_grad must be complete when
  * the call is made. */
 dmt_status dmt_adjoint(dmt_scene* scene, const dmt_frame_params* frame, const void* d_image_grad,
                        void* d_param_grad);
@@ -350,7 +371,8 @@ typedef struct dmt_batch_params {
  * Each scene's accumulation buffer is (re)sized to its frame's resolution;
  * while the batch lives the scene cannot be rendered at another resolution,
  * and dmt_scene_destroy of a member is deferred to dmt_batch_destroy.
- * params may be null (defaults). */
+ * params may be null (defaults).  Frames that use the camera-sample features
+ * (dof, motion_blur, spectral, mesh_shift_active) are DMT_ERR_UNSUPPORTED. */
 dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt_frame_params* frames,
                             uint32_t count, const dmt_batch_params* params, dmt_batch** out_batch);
 void       dmt_batch_destroy(dmt_batch* batch);

@@ -151,13 +151,14 @@ constexpr std::size_t kPrimCount = 80, kSpp = 84, kDenoiserEnabled = 88, kEnvMap
 constexpr std::size_t kEnvIntensity = 104, kCurrViewProj = 112, kPrevViewProj = 176;
 constexpr std::size_t kHeadBytes = 112;        /* PtPushHead (SoftwareTracer.cpp:38-56) */
 constexpr std::size_t kSunAndMode = 240, kW2jRow0 = 272, kW2jRow1 = 288, kW2jRow2 = 304;
-constexpr std::size_t kDofParams = 320, kSpectralParams = 352, kVolParams = 368;
+constexpr std::size_t kDofParams = 320, kDofBokehParams = 336, kSpectralParams = 352, kVolParams = 368;
 constexpr std::size_t kCloudsP1 = 384, kCloudsP3 = 416, kMoonDirPhase = 448;
 constexpr std::size_t kHdriLightsCount = 752, kWriteNormalGbuffer = 756, kWriteAlbedoGbuffer = 760;
 constexpr std::size_t kWriteHdrAux = 764, kMisEnabled = 768, kAccumParams = 784, kSdfParams = 816;
 constexpr std::size_t kWriteShadowVis = 856, kLightCount = 880, kLightTreeEnabled = 888;
 constexpr std::size_t kWriteSpecularAlbedo = 896, kWriteRoughness = 900, kWriteSpecularHitDistance = 904;
 constexpr std::size_t kRestirEnabled = 944, kSmokeParams = 960, kMotionBlurParams = 1024, kCloudsRuntime = 1072;
+constexpr std::size_t kMeshMotionPrev = 1088, kMeshMotionCurr = 1104;
 constexpr std::size_t kMeshTexIndices = 1152, kMatOverrideCount = 1200, kTonemapParams = 1600;
 constexpr std::size_t kSize = 1792;            /* sizeof(PtPush) with kMaxHdriLights = kMaxMatOverrides = 8 */
 }  // namespace ptpush
@@ -171,7 +172,7 @@ struct PathtracePush {
     std::uint32_t env_map_present = 0;
     float env_intensity = 1.0f;
     std::uint32_t tonemap_op = 0;   /* r_tonemap_op; values above 4 select ACES */
-    std::uint32_t motion_blur = 0;  /* motion_blur_params.x != 0 */
+    std::uint32_t motion_blur = 0;  /* motion_blur_params.x > 0.5 (also frame.motion_blur) */
     /* denoiser inputs (PathTrace.slang:5463-5648, 7797-7811) */
     std::uint32_t denoiser = 0;               /* denoiser_enabled: depth + motion */
     std::uint32_t write_normal = 0, write_albedo = 0, write_specular_albedo = 0;
@@ -189,8 +190,8 @@ struct PathtracePush {
  * tracer's no-op, SoftwareTracer.cpp:686-690) and, when strict, for a push that
  * turns on a feature this backend does not render: a procedural / Hosek sky
  * (sun_and_mode.w 2 or 3), the sun or moon above the horizon without an env
- * map, HDRI cluster lights, depth of field, spectral dispersion, participating
- * media, SDF clusters, ReSTIR, motion blur, material overrides, smoke, the
+ * map, HDRI cluster lights, participating
+ * media, SDF clusters, ReSTIR, material overrides, smoke, the
  * SIGMA shadow-visibility output, a per-frame denoiser colour input without
  * reset_accum, or PBR mesh textures.  Lenient decoding renders
  * without them and lists them in out->ignored.  Fields past the pushed size

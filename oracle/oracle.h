@@ -46,6 +46,18 @@ typedef struct orc_frame {
     float    firefly_clamp;
     float    ema_alpha;
     uint32_t flags;      /* bit 0: no Russian roulette (same bits as dmt_frame_params) */
+    /* Camera-sample features (same fields as dmt_frame_params):
+     * thin lens (PathTrace.slang:5779-5816), shutter time (:5761-5773),
+     * hero wavelength (:5737-5759), mesh translation (:2461-2487). */
+    float    dof[4];     /* aperture radius (0 = pinhole), focal distance, blades, blade rotation (rad) */
+    float    cat_eye;    /* dof_bokeh_params.x */
+    uint32_t motion_blur;
+    uint32_t spectral;
+    float    cauchy_c;   /* spectral_params.y, um^2 */
+    float    mesh_shift_prev[3];
+    uint32_t mesh_shift_active;
+    float    mesh_shift_curr[3];
+    uint32_t reserved;
 } orc_frame;
 
 typedef struct orc_hit {
@@ -98,6 +110,11 @@ int orc_render_f64_image(orc_scene* s, const orc_frame* f, int threads, double* 
 int orc_trace(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
 /* Same through an exhaustive linear scan (no BVH). */
 int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
+/* traceScene at shutter time `time` (spheres at lerp(v3, v0, time)); with
+ * shift_prev and shift_curr non-null the mesh is translated by
+ * lerp(shift_prev, shift_curr, time).  brute != 0: exhaustive scan. */
+int orc_trace_at(const orc_scene* s, const float* rays, uint32_t n, float time, const float* shift_prev,
+                 const float* shift_curr, int brute, orc_hit* out);
 
 /* Denoiser G-buffer of a frame (PathTrace.slang:5463-5648): the pixel-centre
  * camera ray of every pixel traced once; 16 floats per pixel: depth, motion.xy,
@@ -122,8 +139,17 @@ int orc_env_pdf(const orc_scene* s, const float* dirs, uint32_t n, float* out);
 /* Exhaustive light-tree pick PMF at point p with facing normal nf. */
 int orc_light_pmf(const orc_scene* s, const float* p, const float* nf, float* pmf);
 
-/* Math kernels exposed for the pinning tests (fp32 Cephes restatement). */
+/* Math kernels exposed for the pinning tests (fp32 Cephes restatement).
+ * fn: 0 sin, 1 cos, 2 acos, 3 atan2(a, b), 4 exp,
+ *     5/6/7 spectral weight r/g/b at wavelength a (nm),
+ *     8 Cauchy index at wavelength a (nm) of base index b (C = 0.0042 um^2). */
 void orc_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n);
+
+/* Lens samples (fp32): u 2 floats each -> 2 floats.  blades < 3: the
+ * concentric disk (ptConcentricDisk); else the area-uniform polygon
+ * (ptPolygonSample) rotated by rot.  cat_eye > 0 then applies ptCatEye with
+ * the screen position uv (2 floats per sample; may be null when cat_eye is 0). */
+int orc_lens_sample(const float* u, const float* uv, uint32_t n, int blades, float rot, float cat_eye, float* out);
 
 #ifdef __cplusplus
 }

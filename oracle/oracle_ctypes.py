@@ -77,6 +77,10 @@ def lib():
         L.orc_tangent.restype = C.c_int
         L.orc_render_f64_image.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double)]
         L.orc_render_f64_image.restype = C.c_int
+        L.orc_trace_at.argtypes = [vp, fp, u32, C.c_float, fp, fp, C.c_int, vp]
+        L.orc_trace_at.restype = C.c_int
+        L.orc_lens_sample.argtypes = [fp, fp, u32, C.c_int, C.c_float, C.c_float, fp]
+        L.orc_lens_sample.restype = C.c_int
         _lib = L
     return _lib
 
@@ -165,6 +169,24 @@ class OracleScene:
         fn(self._h, _fp(rays), n, out.ctypes.data_as(C.c_void_p))
         return out
 
+    def trace_at(self, origins, dirs, time, shift=None, brute=False):
+        """Closest hits at shutter time `time`; shift = (prev xyz, curr xyz) of
+        a mesh translation or None."""
+        n = origins.shape[0]
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = origins
+        rays[:, 4:7] = dirs
+        out = np.zeros(n, HIT_DTYPE)
+        sp = sc = None
+        if shift is not None:
+            sp = np.ascontiguousarray(shift[0], np.float32)
+            sc = np.ascontiguousarray(shift[1], np.float32)
+        rc = lib().orc_trace_at(self._h, _fp(rays), n, float(time), _fp(sp), _fp(sc), 1 if brute else 0,
+                                out.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise ValueError("oracle trace failed")
+        return out
+
     def gbuffer(self, frame, vp_curr, vp_prev):
         """(H, W, 16) float32: depth, motion.xy, normal.xyz, albedo.xyz, F0.xyz,
         roughness, specular hit distance, 0, 0 (column-major view-projections)."""
@@ -237,5 +259,21 @@ def math_f32(fn, a, b=None):
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b if b is not None else np.zeros_like(a), np.float32)
     out = np.zeros_like(a)
-    lib().orc_math_f32({"sin": 0, "cos": 1, "acos": 2, "atan2": 3}[fn], _fp(a), _fp(b), _fp(out), a.size)
+    code = {"sin": 0, "cos": 1, "acos": 2, "atan2": 3, "exp": 4, "spectral_r": 5, "spectral_g": 6,
+            "spectral_b": 7, "cauchy": 8}[fn]
+    lib().orc_math_f32(code, _fp(a), _fp(b), _fp(out), a.size)
+    return out
+
+
+def lens_sample(u, blades=0, rot=0.0, cat_eye=0.0, uv=None):
+    """Lens points (n, 2) of the uniforms u (n, 2): the concentric disk
+    (blades < 3) or the rotated regular polygon, then the cat's-eye squeeze
+    at screen positions uv (n, 2) when cat_eye > 0."""
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    uvv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2) if uv is not None else None
+    out = np.zeros_like(u)
+    rc = lib().orc_lens_sample(_fp(u), _fp(uvv) if uvv is not None else None, u.shape[0], int(blades), float(rot),
+                               float(cat_eye), _fp(out))
+    if rc != 0:
+        raise ValueError("lens sample rejected")
     return out

@@ -106,6 +106,13 @@ template <class R> inline R maxc(V3<R> a) { return std::max(a.x, std::max(a.y, a
 template <class R> inline bool any_pos(V3<R> a) { return a.x > R(0) || a.y > R(0) || a.z > R(0); }
 template <class R> inline R clampv(R x, R lo, R hi) { return std::min(std::max(x, lo), hi); }
 template <class R> inline V3<R> lerp3(V3<R> a, V3<R> b, R t) { return a + (b - a) * t; }
+// The shutter-time lerp of motion blur, in the defining form of GLSL's mix,
+// a (1 - t) + b t: at t = 1 it returns b exactly, which the reference relies
+// on for bit-identical images with motion blur off (PathTrace.slang:2048-2053).
+template <class R> inline V3<R> mix3(V3<R> a, V3<R> b, R t) {
+    const R u = R(1) - t;
+    return {a.x * u + b.x * t, a.y * u + b.y * t, a.z * u + b.z * t};
+}
 // quatRotate (PathTrace.slang:2028-2031): v + 2 * cross(u, cross(u, v) + w * v).
 template <class R> inline V3<R> quat_rotate(const R q[4], V3<R> v) {
     V3<R> u{q[0], q[1], q[2]};
@@ -193,6 +200,29 @@ struct MathF {
         else if (code == 3) w = -3.14159265358979f;
         return w + atan(y / x);
     }
+    // Cephes expf: n = floor(x log2(e) + 1/2), x - n ln 2 in two parts
+    // (C1 + C2), the degree-6 minimax polynomial, then the exact scaling by
+    // 2^n.  Below -87 (where e^x nears the smallest normal float) the result
+    // is 0, so no denormal is ever formed; above 88.72 it is +inf.
+    static float exp(float xin) {
+        if (xin > 88.72283905206835f) return INFINITY;
+        if (xin < -87.0f) return 0.0f;
+        const float LOG2EF = 1.44269504088896341f;
+        const float C1 = 0.693359375f;
+        const float C2 = -2.12194440e-4f;
+        float z = std::floor(LOG2EF * xin + 0.5f);
+        float x = xin - z * C1;
+        x = x - z * C2;
+        const int n = (int)z;
+        const float xx = x * x;
+        float p = (((((1.9875691500e-4f * x + 1.3981999507e-3f) * x + 8.3334519073e-3f) * x + 4.1665795894e-2f) * x
+                    + 1.6666665459e-1f) * x + 5.0000001201e-1f);
+        p = (p * xx + x) + 1.0f;
+        uint32_t bits = (uint32_t)(n + 127) << 23;
+        float scale;
+        std::memcpy(&scale, &bits, 4);
+        return p * scale;
+    }
 };
 
 struct MathD {
@@ -200,6 +230,7 @@ struct MathD {
     static double cos(double x) { return std::cos(x); }
     static double acos(double x) { return std::acos(x); }
     static double atan2(double y, double x) { return std::atan2(y, x); }
+    static double exp(double x) { return std::exp(x); }
 };
 
 struct MathDual {
@@ -212,6 +243,10 @@ struct MathDual {
     static Dual atan2(Dual y, Dual x) {
         const double r2 = x.v * x.v + y.v * y.v;
         return {std::atan2(y.v, x.v), r2 > 0.0 ? (x.v * y.d - y.v * x.d) / r2 : 0.0};
+    }
+    static Dual exp(Dual x) {
+        const double e = std::exp(x.v);
+        return {e, e * x.d};
     }
 };
 

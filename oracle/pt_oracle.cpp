@@ -406,15 +406,16 @@ template <class R> inline bool better(R t, uint32_t id, const Hit<R>& h, uint32_
     return t == h.t && h.kind == kind && id < h.prim;
 }
 
-// testAnalyticPrim (PathTrace.slang:2066-2161) with shutter_t01 = 1 and
-// prev_pos == pos (no motion blur): the sphere centre is v0.xyz.
-template <class R> void test_prim(const orc_scene& S, uint32_t i, V3<R> ro, V3<R> rd, Hit<R>& h, Ctr& c) {
+// testAnalyticPrim (PathTrace.slang:2066-2161).  The sphere centre is
+// lerp(prev_pos, pos, shutter time) (:2093); without motion blur the time is 1
+// and the centre is v0.xyz.
+template <class R> void test_prim(const orc_scene& S, uint32_t i, V3<R> ro, V3<R> rd, Hit<R>& h, Ctr& c, R time) {
     const float* p = &S.prims[(size_t)i * 28];
     c.tests++;
     uint32_t type = (uint32_t)p[8];
     R t;
     if (type == PRIM_SPHERE) {
-        V3<R> ce = ld3<R>(p);
+        V3<R> ce = (time < R(1)) ? mix3(ld3<R>(p + 12), ld3<R>(p), time) : ld3<R>(p);
         R rad = R(p[3]);
         V3<R> oc = ro - ce;
         R b = dot(oc, rd);
@@ -513,12 +514,15 @@ template <class R> inline V3<R> safe_inv(V3<R> rd) {
     return V3<R>{R(1) / f(rd.x), R(1) / f(rd.y), R(1) / f(rd.z)};
 }
 
-template <class R> Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ctr& c, bool brute = false) {
+// traceScene (PathTrace.slang:2331-2693) at shutter time `time`; the mesh is
+// intersected from ro_mesh = ro - its translation at that time (:2483-2487).
+template <class R>
+Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ctr& c, bool brute, R time, V3<R> ro_mesh) {
     Hit<R> h;
-    for (uint32_t i : S.planes) test_prim(S, i, ro, rd, h, c);
+    for (uint32_t i : S.planes) test_prim(S, i, ro, rd, h, c, time);
     V3<R> inv = safe_inv(rd);
     if (!S.use_abvh || brute) {
-        for (uint32_t i : S.finite) test_prim(S, i, ro, rd, h, c);
+        for (uint32_t i : S.finite) test_prim(S, i, ro, rd, h, c, time);
     } else {
         uint32_t stack[64];
         int sp = 0;
@@ -528,7 +532,8 @@ template <class R> Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ct
             c.nodes++;
             if (!box_hit(ro, inv, nd.mn, nd.mx, h.t)) continue;
             if (nd.count > 0) {
-                for (uint32_t k = 0; k < nd.count; ++k) test_prim(S, S.abvh.perm[nd.left_first + k], ro, rd, h, c);
+                for (uint32_t k = 0; k < nd.count; ++k)
+                    test_prim(S, S.abvh.perm[nd.left_first + k], ro, rd, h, c, time);
             } else {
                 stack[sp++] = nd.left_first;
                 stack[sp++] = nd.left_first + 1;
@@ -542,7 +547,7 @@ template <class R> Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ct
         if (brute) {
             for (uint32_t tri = 0; tri < S.tri_count; ++tri) {
                 c.tests++;
-                TriT<R> r = tri_test(S, tri, ro, rd);
+                TriT<R> r = tri_test(S, tri, ro_mesh, rd);
                 if (r.ok && (r.t < t_mesh || (found && r.t == t_mesh && tri < best))) {
                     found = true; t_mesh = r.t; best = tri;
                 }
@@ -554,12 +559,12 @@ template <class R> Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ct
             while (sp > 0) {
                 const BNode& nd = S.tbvh.nodes[stack[--sp]];
                 c.nodes++;
-                if (!box_hit(ro, inv, nd.mn, nd.mx, t_mesh)) continue;
+                if (!box_hit(ro_mesh, inv, nd.mn, nd.mx, t_mesh)) continue;
                 if (nd.count > 0) {
                     for (uint32_t k = 0; k < nd.count; ++k) {
                         uint32_t tri = S.tbvh.perm[nd.left_first + k];
                         c.tests++;
-                        TriT<R> r = tri_test(S, tri, ro, rd);
+                        TriT<R> r = tri_test(S, tri, ro_mesh, rd);
                         if (r.ok && (r.t < t_mesh || (found && r.t == t_mesh && tri < best))) {
                             found = true; t_mesh = r.t; best = tri;
                         }
@@ -573,6 +578,9 @@ template <class R> Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ct
         if (found) mesh_fill(S, best, t_mesh, h);
     }
     return h;
+}
+template <class R> Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ctr& c, bool brute = false) {
+    return trace_scene(S, ro, rd, c, brute, R(1), ro);
 }
 
 template <class R> struct PT {
@@ -826,13 +834,22 @@ template <class R> struct PT {
         return s;
     }
 
+    // The closest hit at shutter time `time`, the mesh translated by
+    // lerp(mesh_shift_prev, mesh_shift_curr, time) when its translation is on
+    // (PathTrace.slang:2483-2487).
+    Hit<R> trace(V3<R> ro, V3<R> rd, R time) const {
+        V3<R> rm = ro;
+        if (F.mesh_shift_active) rm = ro - mix3(ld3<R>(F.mesh_shift_prev), ld3<R>(F.mesh_shift_curr), time);
+        return trace_scene(S, ro, rd, c, false, time, rm);
+    }
+
     // transmittance (PathTrace.slang:5089-5196), no water / clouds.
-    V3<R> transmittance(V3<R> o, V3<R> d, R t_max) const {
+    V3<R> transmittance(V3<R> o, V3<R> d, R t_max, R time) const {
         int kmax = (int)F.refract_bounces;
         V3<R> t{R(1), R(1), R(1)};
         for (int i = 0; i <= kmax; ++i) {
             c.shadow++;
-            Hit<R> h = trace_scene(S, o, d, c);
+            Hit<R> h = trace(o, d, time);
             if (!h.hit || h.t >= t_max) return t;
             if (h.mat != MAT_DIELECTRIC) return V3<R>{R(0), R(0), R(0)};
             if (i == kmax) return V3<R>{R(0), R(0), R(0)};
@@ -857,16 +874,135 @@ template <class R> struct PT {
         return V3<R>{R(0), R(0), R(0)};
     }
 
+    // cauchyIOR (PathTraceMath.slang:111-119): n = B + C / lambda^2 through
+    // (base, 587.6 nm), lambda in micrometres.
+    static R cauchy_ior(R base, R cc, R lambda_nm) {
+        R lum = lambda_nm * R(0.001);
+        R l2 = lum * lum;
+        R ld2 = R(0.5876) * R(0.5876);
+        R B = base - cc / ld2;
+        return B + cc / l2;
+    }
+    // wymanGauss, cieXYZFit, xyzToLinearSRGB, spectralWeightSRGB
+    // (PathTraceMath.slang:140-192): the Wyman-Sloan-Shirley fit of the CIE
+    // 1931 matching functions, to linear sRGB, over the band means.
+    static R wyman(R x, R mu, R s1, R s2) {
+        R t = (x - mu) * ((x < mu) ? s1 : s2);
+        return M::exp((R(-0.5) * t) * t);
+    }
+    static V3<R> spectral_weight(R l) {
+        R X = (R(1.056) * wyman(l, R(599.8), R(0.0264), R(0.0323)) + R(0.362) * wyman(l, R(442.0), R(0.0624), R(0.0374)))
+              - R(0.065) * wyman(l, R(501.1), R(0.0490), R(0.0382));
+        R Y = R(0.821) * wyman(l, R(568.8), R(0.0213), R(0.0247)) + R(0.286) * wyman(l, R(530.9), R(0.0613), R(0.0322));
+        R Z = R(1.217) * wyman(l, R(437.0), R(0.0845), R(0.0278)) + R(0.681) * wyman(l, R(459.0), R(0.0385), R(0.0725));
+        R r = (R(3.2404542) * X - R(1.5371385) * Y) - R(0.4985314) * Z;
+        R g = (R(-0.9692660) * X + R(1.8760108) * Y) + R(0.0415560) * Z;
+        R b = (R(0.0556434) * X - R(0.2040259) * Y) + R(1.0572252) * Z;
+        return V3<R>{r * R(2.730792), g * R(3.443936), b * R(3.606855)};
+    }
+
+    // ptConcentricDisk (PathTraceMath.slang:223-236).
+    static void concentric_disk(R u1, R u2, R& x, R& y) {
+        R a = R(2) * u1 - R(1);
+        R b = R(2) * u2 - R(1);
+        R r, phi;
+        const R qpi = R(3.14159265358979) / R(4), hpi = R(3.14159265358979) / R(2);
+        if (oabs(a) > oabs(b)) {
+            r = a;
+            phi = qpi * (b / std::max(oabs(a), R(1e-6)));
+        } else {
+            r = b;
+            phi = hpi - qpi * (a / std::max(oabs(b), R(1e-6)));
+        }
+        x = r * M::cos(phi);
+        y = r * M::sin(phi);
+    }
+    // ptPolygonSample (PathTraceMath.slang:268-286): a wedge of the regular
+    // N-gon picked by u1, then the sqrt-barycentric point of that triangle.
+    static void polygon_sample(R u1, R u2, int blades, R rot, R& x, R& y) {
+        R n = R(blades);
+        R wedge = R(6.28318530717958) / n;
+        R kf = ofloor(u1 * n);
+        R ku = u1 * n - kf;
+        R a0 = rot + kf * wedge;
+        R a1 = a0 + wedge;
+        R v1x = M::cos(a0), v1y = M::sin(a0);
+        R v2x = M::cos(a1), v2y = M::sin(a1);
+        R su = osqrt(ku);
+        R w1 = R(1) - u2;
+        x = su * (w1 * v1x + u2 * v2x);
+        y = su * (w1 * v1y + u2 * v2y);
+    }
+    // ptCatEye (PathTraceMath.slang:296-310): the radial component of the lens
+    // sample narrowed with the image eccentricity |uv|.
+    static void cat_eye(R& x, R& y, R ux, R uy, R strength) {
+        R len = osqrt(ux * ux + uy * uy);
+        R ecc = clampv(len, R(0), R(1));
+        R squeeze = strength * ecc;
+        if (squeeze < R(1e-5)) return;
+        R rx = R(1), ry = R(0);
+        if (ecc > R(1e-5)) { rx = ux / len; ry = uy / len; }
+        R tx = -ry, ty = rx;
+        R ar = x * rx + y * ry;
+        R at = x * tx + y * ty;
+        ar = ar * (R(1) - R(0.7) * squeeze);
+        x = rx * ar + tx * at;
+        y = ry * ar + ty * at;
+    }
+
+    // Camera ray of sample s of pixel (x, y) (PathTrace.slang:5729-5816):
+    // the jitter, then, each only when its feature is on, the hero wavelength
+    // (stratified over the samples), the shutter time and the lens point.
+    void camera(uint32_t x, uint32_t y, uint32_t s, uint32_t spp, uint32_t& seed, V3<R>& ro, V3<R>& rd, R& time,
+                R& lambda) const {
+        const R w = R(F.width), hgt = R(F.height);
+        R fovtan = R(F.pos_fovtan[3]), aspect = R(F.fwd_aspect[3]);
+        V3<R> fwd{R(F.fwd_aspect[0]), R(F.fwd_aspect[1]), R(F.fwd_aspect[2])};
+        V3<R> rt{R(F.right_xyz[0]), R(F.right_xyz[1]), R(F.right_xyz[2])};
+        V3<R> up{R(F.up_xyz[0]), R(F.up_xyz[1]), R(F.up_xyz[2])};
+        R jx = randf<R>(seed);
+        R jy = randf<R>(seed);
+        jx = jx - R(0.5);
+        jy = jy - R(0.5);
+        R ux = (((R(x) + R(0.5)) + jx) / w) * R(2) - R(1);
+        R uy = (((R(y) + R(0.5)) + jy) / hgt) * R(2) - R(1);
+        uy = -uy;
+        lambda = R(-1);
+        if (F.spectral) {
+            R u = (R(s) + randf<R>(seed)) / R(spp);
+            lambda = R(380) + u * (R(730) - R(380));
+        }
+        time = R(1);
+        if (F.motion_blur) time = randf<R>(seed);
+        ro = V3<R>{R(F.pos_fovtan[0]), R(F.pos_fovtan[1]), R(F.pos_fovtan[2])};
+        rd = normalize((fwd + rt * ((ux * aspect) * fovtan)) + up * (uy * fovtan));
+        if (F.dof[0] > 0.0f) {
+            const R aperture = R(F.dof[0]), focal = R(F.dof[1]);
+            const int blades = (int)(F.dof[2] + 0.5f);
+            R u1 = randf<R>(seed);
+            R u2 = randf<R>(seed);
+            R ax, ay;
+            if (blades >= 3) polygon_sample(u1, u2, blades, R(F.dof[3]), ax, ay);
+            else concentric_disk(u1, u2, ax, ay);
+            if (F.cat_eye > 0.0f) cat_eye(ax, ay, ux, uy, R(F.cat_eye));
+            V3<R> lens = rt * (ax * aperture) + up * (ay * aperture);
+            V3<R> fp = ro + rd * focal;
+            ro = ro + lens;
+            rd = normalize(fp - ro);
+        }
+    }
+
     // One path sample: PathTrace.slang:5803-6914 with the off-path features
-    // (water, clouds, DOF, spectral, PBR, clearcoat, SSS, ReSTIR) disabled.
-    V3<R> path(V3<R> ro, V3<R> rd, uint32_t& seed) const {
+    // (water, clouds, PBR, clearcoat, SSS, ReSTIR) disabled; `time` is the
+    // sample's shutter time, `lambda` its hero wavelength (< 0: RGB).
+    V3<R> path(V3<R> ro, V3<R> rd, uint32_t& seed, R time = R(1), R lambda = R(-1)) const {
         V3<R> T{R(1), R(1), R(1)}, L{R(0), R(0), R(0)};
         bool skip_sky = false, prev_lambert = false;
         R prev_pdf = R(0);
         const bool mis_on = F.mis_enabled != 0u;
         for (uint32_t b = 0; b < F.max_bounces; ++b) {
             if (b == 0) c.primary++; else c.extension++;
-            Hit<R> h = trace_scene(S, ro, rd, c);
+            Hit<R> h = trace(ro, rd, time);
             if (!h.hit) {
                 if (mis_on && prev_lambert) {
                     // transmittance(ro, rd, 1e30) re-traces this same missed
@@ -893,7 +1029,7 @@ template <class R> struct PT {
                     R ndl = dot(nf, ls.dir);
                     if (ls.pdf > R(0) && ndl > R(0)) {
                         R tm = std::max(ls.dist - R(1e-3), R(0));
-                        V3<R> tr = transmittance(hp + nf * R(1e-3), ls.dir, tm);
+                        V3<R> tr = transmittance(hp + nf * R(1e-3), ls.dir, tm, time);
                         if (any_pos(tr)) {
                             V3<R> brdf = h.albedo / PI;
                             V3<R> cc = ((((T * ls.Le) * tr) * brdf) * ndl) / ls.pdf;
@@ -907,7 +1043,7 @@ template <class R> struct PT {
                     EnvS e = sample_env(x0, x1);
                     R ndl = dot(nf, e.dir);
                     if (e.pdf > R(0) && ndl > R(0)) {
-                        V3<R> tr = transmittance(hp + nf * R(1e-3), e.dir, R(1e30));
+                        V3<R> tr = transmittance(hp + nf * R(1e-3), e.dir, R(1e30), time);
                         if (any_pos(tr)) {
                             V3<R> brdf = h.albedo / PI;
                             R pb = ndl * (R(1) / PI);
@@ -944,8 +1080,10 @@ template <class R> struct PT {
                 skip_sky = false;
                 prev_lambert = false;
             } else if (h.mat == MAT_DIELECTRIC) {
-                R ia = (ndi < R(0)) ? R(1) : h.ior;
-                R ib = (ndi < R(0)) ? h.ior : R(1);
+                // the hero wavelength's own index (PathTrace.slang:6721-6724)
+                R ior = (lambda >= R(0)) ? cauchy_ior(h.ior, R(F.cauchy_c), lambda) : h.ior;
+                R ia = (ndi < R(0)) ? R(1) : ior;
+                R ib = (ndi < R(0)) ? ior : R(1);
                 R eta = ia / ib;
                 R ci = oabs(dot(nf, -rd));
                 R r0 = (ia - ib) / (ia + ib);
@@ -979,23 +1117,14 @@ template <class R> struct PT {
     void pixel(uint32_t x, uint32_t y, float* accum) const {
         uint32_t seed = pcg_hash(x * 1973u + y * 9277u + F.frame_index * 26699u);
         V3<R> fr{R(0), R(0), R(0)};
-        const R w = R(F.width), hgt = R(F.height);
-        V3<R> pos{R(F.pos_fovtan[0]), R(F.pos_fovtan[1]), R(F.pos_fovtan[2])};
-        R fovtan = R(F.pos_fovtan[3]), aspect = R(F.fwd_aspect[3]);
-        V3<R> fwd{R(F.fwd_aspect[0]), R(F.fwd_aspect[1]), R(F.fwd_aspect[2])};
-        V3<R> rt{R(F.right_xyz[0]), R(F.right_xyz[1]), R(F.right_xyz[2])};
-        V3<R> up{R(F.up_xyz[0]), R(F.up_xyz[1]), R(F.up_xyz[2])};
         uint32_t spp = std::max(1u, F.spp);
         for (uint32_t s = 0; s < spp; ++s) {
-            R jx = randf<R>(seed);
-            R jy = randf<R>(seed);
-            jx = jx - R(0.5);
-            jy = jy - R(0.5);
-            R ux = (((R(x) + R(0.5)) + jx) / w) * R(2) - R(1);
-            R uy = (((R(y) + R(0.5)) + jy) / hgt) * R(2) - R(1);
-            uy = -uy;
-            V3<R> rd = normalize((fwd + rt * ((ux * aspect) * fovtan)) + up * (uy * fovtan));
-            fr = fr + path(pos, rd, seed);
+            V3<R> ro, rd;
+            R time, lambda;
+            camera(x, y, s, spp, seed, ro, rd, time, lambda);
+            V3<R> L = path(ro, rd, seed, time, lambda);
+            // the hero wavelength's sRGB response (PathTrace.slang:7693-7700)
+            fr = fr + ((lambda >= R(0)) ? L * spectral_weight(lambda) : L);
         }
         fr = fr / R(spp);
         float o[3] = {float(fr.x), float(fr.y), float(fr.z)};
@@ -1055,23 +1184,13 @@ void pixel_tangent(const PT<Dual>& pt, const orc_frame& F, uint32_t x, uint32_t 
     uint32_t seed = pcg_hash(x * 1973u + y * 9277u + F.frame_index * 26699u);
     using R = Dual;
     V3<R> fr{R(0), R(0), R(0)};
-    const R w = R(F.width), hgt = R(F.height);
-    V3<R> pos{R(F.pos_fovtan[0]), R(F.pos_fovtan[1]), R(F.pos_fovtan[2])};
-    R fovtan = R(F.pos_fovtan[3]), aspect = R(F.fwd_aspect[3]);
-    V3<R> fwd{R(F.fwd_aspect[0]), R(F.fwd_aspect[1]), R(F.fwd_aspect[2])};
-    V3<R> rt{R(F.right_xyz[0]), R(F.right_xyz[1]), R(F.right_xyz[2])};
-    V3<R> up{R(F.up_xyz[0]), R(F.up_xyz[1]), R(F.up_xyz[2])};
     const uint32_t spp = std::max(1u, F.spp);
     for (uint32_t s = 0; s < spp; ++s) {
-        R jx = randf<R>(seed);
-        R jy = randf<R>(seed);
-        jx = jx - R(0.5);
-        jy = jy - R(0.5);
-        R ux = (((R(x) + R(0.5)) + jx) / w) * R(2) - R(1);
-        R uy = (((R(y) + R(0.5)) + jy) / hgt) * R(2) - R(1);
-        uy = -uy;
-        V3<R> rd = normalize((fwd + rt * ((ux * aspect) * fovtan)) + up * (uy * fovtan));
-        fr = fr + pt.path(pos, rd, seed);
+        V3<R> ro, rd;
+        R time, lambda;
+        pt.camera(x, y, s, spp, seed, ro, rd, time, lambda);
+        V3<R> L = pt.path(ro, rd, seed, time, lambda);
+        fr = fr + ((lambda >= R(0)) ? L * PT<Dual>::spectral_weight(lambda) : L);
     }
     fr = fr / R(spp);
     const Dual c[3] = {fr.x, fr.y, fr.z};
@@ -1108,9 +1227,14 @@ int orc_scene_create(const orc_scene_desc* d, orc_scene** out) {
         S->finite.push_back(i);
         WorkItem w{};
         for (int a = 0; a < 3; ++a) {
+            // the sphere at its current and previous centre (motion blur moves
+            // it between them), widened by 2^-20 of the coordinates so that
+            // the rounding of the shutter-time lerp stays inside
+            const float lo = std::min(p[a], p[12 + a]), hi = std::max(p[a], p[12 + a]);
+            const float pad = (std::max(std::fabs(lo), std::fabs(hi)) + p[3]) * 9.5367431640625e-7f;
             w.c[a] = p[a];
-            w.mn[a] = p[a] - p[3];
-            w.mx[a] = p[a] + p[3];
+            w.mn[a] = (lo - p[3]) - pad;
+            w.mx[a] = (hi + p[3]) + pad;
         }
         w.id = i;
         fw.push_back(w);
@@ -1189,6 +1313,20 @@ int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_
     return trace_impl(s, rays, n, out, true);
 }
 
+int orc_trace_at(const orc_scene* s, const float* rays, uint32_t n, float time, const float* shift_prev,
+                 const float* shift_curr, int brute, orc_hit* out) {
+    if (!s || (!rays && n) || (!out && n)) return -1;
+    Ctr c;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        const V3<float> o = ld3<float>(r), d = ld3<float>(r + 4);
+        V3<float> om = o;
+        if (shift_prev && shift_curr) om = o - mix3(ld3<float>(shift_prev), ld3<float>(shift_curr), time);
+        fill_hit(out[i], trace_scene<float>(*s, o, d, c, brute != 0, time, om));
+    }
+    return 0;
+}
+
 // G-buffer pass (PathTrace.slang:5463-5648): the pixel-centre ray (5464-5469),
 // the hit's linear depth and surface fields or the sky sentinels (5495-5519),
 // pixel motion by reprojection through the two view-projections (5524-5528),
@@ -1219,7 +1357,11 @@ int orc_gbuffer(const orc_scene* s, const orc_frame* f, const float* vp_curr, co
             float uy = (((float)y + 0.5f) / H) * 2.0f - 1.0f;
             uy = -uy;
             const V d = normalize((fwd + rt * ((ux * aspect) * fovtan)) + up * (uy * fovtan));
-            const Hit<float> h = trace_scene<float>(*s, pos, d, c, false);
+            // shutter time 1: the mesh at its current translation (5470-5478)
+            const V pm = f->mesh_shift_active ? pos - mix3(ld3<float>(f->mesh_shift_prev),
+                                                            ld3<float>(f->mesh_shift_curr), 1.0f)
+                                              : pos;
+            const Hit<float> h = trace_scene<float>(*s, pos, d, c, false, 1.0f, pm);
             float* o = out + ((size_t)y * f->width + x) * 16u;
             for (int k = 0; k < 16; ++k) o[k] = 0.0f;
             V world;
@@ -1412,9 +1554,27 @@ void orc_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n
             case 1: out[i] = MathF::cos(a[i]); break;
             case 2: out[i] = MathF::acos(a[i]); break;
             case 3: out[i] = MathF::atan2(a[i], b[i]); break;
+            case 4: out[i] = MathF::exp(a[i]); break;
+            case 5: out[i] = PT<float>::spectral_weight(a[i]).x; break;
+            case 6: out[i] = PT<float>::spectral_weight(a[i]).y; break;
+            case 7: out[i] = PT<float>::spectral_weight(a[i]).z; break;
+            case 8: out[i] = PT<float>::cauchy_ior(b[i], 0.0042f, a[i]); break;
             default: out[i] = 0.0f;
         }
     }
+}
+
+int orc_lens_sample(const float* u, const float* uv, uint32_t n, int blades, float rot, float cat_eye, float* out) {
+    if (!u || !out || (cat_eye > 0.0f && !uv)) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        float x, y;
+        if (blades >= 3) PT<float>::polygon_sample(u[2 * i], u[2 * i + 1], blades, rot, x, y);
+        else PT<float>::concentric_disk(u[2 * i], u[2 * i + 1], x, y);
+        if (cat_eye > 0.0f) PT<float>::cat_eye(x, y, uv[2 * i], uv[2 * i + 1], cat_eye);
+        out[2 * i] = x;
+        out[2 * i + 1] = y;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -27,13 +27,15 @@ OFF = dict(pos_fovtan=0, fwd_aspect=16, right_xyz=32, up_xyz=48, frame_index=64,
            max_bounces=72, tlas_present=76, prim_count=80, spp=84, denoiser_enabled=88, env_map_present=92,
            halton_jitter=96, env_intensity=104, env_total_luminance=108, curr_view_proj=112, prev_view_proj=176,
            sun_and_mode=240, exposure_pad=256,
-           w2j_row0=272, w2j_row1=288, w2j_row2=304, dof_params=320, spectral_params=352, vol_params=368,
+           w2j_row0=272, w2j_row1=288, w2j_row2=304, dof_params=320, dof_bokeh_params=336, spectral_params=352,
+           vol_params=368,
            clouds_p1=384, clouds_p3=416, moon_dir_phase=448, hdri_lights_count=752, write_normal_gbuffer=756,
            write_albedo_gbuffer=760, write_hdr_aux=764, mis_enabled=768, accum_params=784, bvh_params=800,
            sdf_params=816, write_shadow_vis=856, light_count=880, light_tree_node_count=884,
            light_tree_enabled=888, write_specular_albedo_gbuffer=896, write_roughness_gbuffer=900,
            write_specular_hit_distance_gbuffer=904, restir_enabled=944, smoke_params=960,
-           motion_blur_params=1024, mesh_tex_indices=1152, hosek_params=1184, mat_override_count=1200,
+           motion_blur_params=1024, mesh_motion_prev=1088, mesh_motion_curr=1104, mesh_tex_indices=1152,
+           hosek_params=1184, mat_override_count=1200,
            tonemap_params=1600, sun_extra2=1616)
 
 
@@ -69,6 +71,12 @@ def pack_ptpush(frame: "dmt.FrameParams", *, prim_count: int, light_count: int, 
     put("light_tree_node_count", "I", max(0, 2 * light_count - 1) if frame.light_tree_enabled else 0)
     put("light_tree_enabled", "I", 1 if (frame.light_tree_enabled and light_count) else 0)
     put("mesh_tex_indices", "4I", *([NO_TEX_TILE] * 4))
+    put("dof_params", "4f", *frame.dof)
+    put("dof_bokeh_params", "4f", frame.cat_eye, 0.0, 0.0, 0.0)
+    put("spectral_params", "4f", 1.0 if frame.spectral else 0.0, frame.cauchy_c, 0.0, 0.0)
+    put("motion_blur_params", "4f", 1.0 if frame.motion_blur else 0.0, 1.0 / 60.0, 0.0, 0.0)
+    put("mesh_motion_prev", "4f", *frame.mesh_shift_prev, 1.0 if frame.mesh_shift_active else 0.0)
+    put("mesh_motion_curr", "4f", *frame.mesh_shift_curr, 0.0)
     put("hosek_params", "4f", 2.5, 0.3, 1.2, 0.0)                      # read by sky mode 3 only
     put("tonemap_params", "4I", tonemap_op, 0, 0, 0)
     put("sun_extra2", "4f", 1.0, 0.0, 0.0, 0.0)

@@ -191,7 +191,7 @@ def test_strict_refusal_and_lenient_rendering(device):
     label, sd = SCENES[0]
     f = sd.frame.copy()
     f.reset_accum = 1
-    dof = {"dof_params": ("4f", (0.05, 5.0, 0.0, 0.0))}
+    restir = {"restir_enabled": ("I", 1)}
     strict = rhi.HipDevice(strict=True)
     eng = EngineSim(strict, sd)
     eng.frame(f)
@@ -199,8 +199,8 @@ def test_strict_refusal_and_lenient_rendering(device):
     g = f.copy()
     g.frame_index += 1
     g.reset_accum = 0
-    eng.frame(g, overrides=dof)
-    assert "depth of field" in strict.last_error()
+    eng.frame(g, overrides=restir)
+    assert "ReSTIR" in strict.last_error()
     assert np.array_equal(eng.accum_image(), before)          # a refused dispatch is a no-op
     eng.frame(g, push=eng.push(g)[:100])                       # short push: no-op too
     assert "shorter than the 112 B" in strict.last_error()
@@ -209,8 +209,8 @@ def test_strict_refusal_and_lenient_rendering(device):
     strict.close()
     lenient = rhi.HipDevice(strict=False)
     eng = EngineSim(lenient, sd)
-    eng.frames(f, 2, overrides=dof)
-    assert "depth of field" in lenient.ignored_features()
+    eng.frames(f, 2, overrides=restir)
+    assert "ReSTIR" in lenient.ignored_features()
     assert np.array_equal(eng.accum_image(), _direct(device, sd, f, 2))
     lenient.close()
 

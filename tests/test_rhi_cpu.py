@@ -71,14 +71,11 @@ GATES = [
     ("sun_and_mode", "4f", (0.0, -0.5, -0.86, 2.0), "procedural sky"),
     ("sun_and_mode", "4f", (0.0, -0.5, -0.86, 3.0), "Hosek-Wilkie"),
     ("hdri_lights_count", "I", 2, "HDRI cluster lights"),
-    ("dof_params", "4f", (0.05, 4.0, 0.0, 0.0), "depth of field"),
-    ("spectral_params", "4f", (1.0, 0.0042, 0.0, 0.0), "spectral dispersion"),
     ("vol_params", "4f", (0.01, 0.0, 1.0, 8.0), "participating media"),
     ("clouds_p1", "4f", (100.0, 200.0, 0.5, 0.2), "participating media"),
     ("clouds_p3", "4f", (0.0, 0.0, 0.0, 1.0), "participating media"),
     ("sdf_params", "4I", (2, 128, 0, 0), "SDF clusters"),
     ("restir_enabled", "I", 1, "ReSTIR"),
-    ("motion_blur_params", "4f", (1.0, 0.016, 0.0, 0.0), "motion blur"),
     ("mat_override_count", "4I", (1, 0, 0, 0), "material overrides"),
     ("smoke_params", "4f", (1.0, 0.0, 0.0, 0.0), "smoke volumes"),
     ("write_shadow_vis", "I", 1, "SIGMA shadow visibility"),
@@ -96,6 +93,28 @@ def test_strict_refuses_and_lenient_lists_each_feature(field, fmt, value, what):
     d = rhi.decode_pathtrace_push(push, strict=False)
     assert what in d["ignored"]
     assert d["spp"] == 3   # the frame itself still decodes
+
+
+def test_decode_reads_the_camera_sample_features():
+    """dof_params 320, dof_bokeh_params 336, spectral_params 352, motion_blur_params
+    1024, mesh_motion_prev/curr 1088/1104 (Engine.cpp:8725-8739, 9020-9023, 9061-9068)
+    land in the frame; the gates are the shader's (> 0 aperture, > 0.5 flags)."""
+    f = _frame()
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides={
+        "dof_params": ("4f", (0.08, 6.5, 6.0, 0.25)), "dof_bokeh_params": ("4f", (0.4, 0.0, 0.0, 0.0)),
+        "spectral_params": ("4f", (1.0, 0.0042, 0.0, 0.0)), "motion_blur_params": ("4f", (1.0, 0.016, 0.0, 0.0)),
+        "mesh_motion_prev": ("4f", (0.5, -1.0, 2.0, 1.0)), "mesh_motion_curr": ("4f", (0.75, -1.0, 2.5, 0.0))})
+    d = rhi.decode_pathtrace_push(push, strict=True)
+    assert d["ignored"] == ""
+    assert np.allclose(d["dof"], [0.08, 6.5, 6.0, 0.25]) and np.isclose(d["cat_eye"], 0.4)
+    assert (d["spectral"], d["motion_blur"], d["mesh_shift_active"]) == (1, 1, 1)
+    assert np.isclose(d["cauchy_c"], 0.0042)
+    assert np.allclose(d["mesh_shift_prev"], [0.5, -1.0, 2.0]) and np.allclose(d["mesh_shift_curr"], [0.75, -1.0, 2.5])
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides={
+        "spectral_params": ("4f", (0.5, 0.0042, 0.0, 0.0)), "motion_blur_params": ("4f", (0.5, 0.016, 0.0, 0.0)),
+        "mesh_motion_prev": ("4f", (0.5, -1.0, 2.0, 0.5))})
+    d = rhi.decode_pathtrace_push(push)
+    assert (d["spectral"], d["motion_blur"], d["mesh_shift_active"]) == (0, 0, 0)
 
 
 def test_decode_reads_the_denoiser_fields():
