@@ -49,6 +49,14 @@ the spheres' parameters are then perturbed by N(0, 0.1) (albedo kept in
 [0.05, 1], emission >= 0.02).  A forward step is one dmt_render of the frame
 (to completion), a backward step one dmt_adjoint with dLoss/dI = 2 (I - target)
 (to completion).  value = backward time / forward time (lower is better).
+
+--workload features: the camera-sample features on the config-1 workload:
+thin lens (aperture 0.05, focus 12, six blades, cat's-eye 0.3), motion blur
+(the mesh translated from (-0.1, 0, 0) to its rest position over the
+shutter), hero-wavelength dispersion (C = 0.0042 um^2).  Such frames render
+through the per-pixel feature kernel (k_paths).  value = its Mrays/s over K
+frames issued back to back; vs_plain = that rate over the plain frame's rate
+(the same scene and camera without the features) measured the same way.
 """
 import argparse
 import json
@@ -82,7 +90,7 @@ def measured_traffic(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "c2", "batch", "adjoint"), default="c1")
+    ap.add_argument("--workload", choices=("c1", "c2", "batch", "adjoint", "features"), default="c1")
     ap.add_argument("--steps", type=int, default=None, help="timed frames (c1: 20, batch: 100)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed frames (c1: 3, batch: 2)")
     ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
@@ -117,6 +125,10 @@ def main():
         args.steps = 10 if args.steps is None else args.steps
         args.warmup = 2 if args.warmup is None else args.warmup
         return bench_adjoint(args, np, torch, dmt)
+    if args.workload == "features":
+        args.steps = 10 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        return bench_features(args, np, torch, dmt)
     args.steps = 20 if args.steps is None else args.steps
     args.warmup = 3 if args.warmup is None else args.warmup
     if args.calibration_copy_mb > 0:
@@ -487,6 +499,76 @@ def bench_adjoint(args, np, torch, dmt):
     }
     print(json.dumps(line), flush=True)
     sc.close()
+    dev.close()
+
+
+def bench_features(args, np, torch, dmt):
+    sd = dmt.generate(1, seed=args.seed)
+    dev = dmt.Device(0)
+    scene = dmt.Scene(dev, sd)
+    plain = sd.frame.copy()
+    feat = plain.copy()
+    feat.dof = (0.05, 12.0, 6.0, 0.2)
+    feat.cat_eye = 0.3
+    feat.motion_blur = 1
+    feat.spectral = 1
+    feat.cauchy_c = 0.0042
+    feat.mesh_shift_active = 1
+    feat.mesh_shift_prev = (-0.1, 0.0, 0.0)
+    feat.mesh_shift_curr = (0.0, 0.0, 0.0)
+
+    def sync():
+        dev.synchronize()
+        torch.cuda.synchronize()
+
+    def run(fr):
+        for k in range(args.warmup):
+            f = fr.copy()
+            f.frame_index, f.reset_accum = k, 1 if k == 0 else 0
+            scene.render(f)
+        sync()
+        scene.reset_stats()
+        sync()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            f = fr.copy()
+            f.frame_index, f.reset_accum = args.warmup + k, 0
+            scene.render(f)
+        sync()
+        dt = time.perf_counter() - t0
+        st = scene.stats()
+        rays = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+        return rays / dt / 1e6, dt * 1e3 / args.steps, rays / args.steps
+
+    mp, msp, rp = run(plain)
+    mf, msf, rf = run(feat)
+    log(f"[bench] plain frame {msp:.3f} ms, {mp:.1f} Mrays/s ({rp:.0f} rays); feature frame {msf:.3f} ms, "
+        f"{mf:.1f} Mrays/s ({rf:.0f} rays)")
+    line = {
+        "metric": "Mrays/s with camera-sample features",
+        "value": round(mf, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(msf, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, config 1)",
+        "config": {"workload": "config 1 (1.51 M triangles, 1920x1080, 1 spp, depth 8) with thin lens "
+                               "(aperture 0.05, focus 12, 6 blades, cat's eye 0.3), motion blur (mesh translated "
+                               "over the shutter) and hero-wavelength dispersion",
+                   "width": plain.width, "height": plain.height, "spp": plain.spp,
+                   "max_bounces": plain.max_bounces},
+        "vs_plain": round(mf / mp, 3),
+        "plain_mrays": round(mp, 2),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    scene.close()
     dev.close()
 
 
