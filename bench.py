@@ -54,7 +54,7 @@ the spheres' parameters are then perturbed by N(0, 0.1) (albedo kept in
 thin lens (aperture 0.05, focus 12, six blades, cat's-eye 0.3), motion blur
 (the mesh translated from (-0.1, 0, 0) to its rest position over the
 shutter), hero-wavelength dispersion (C = 0.0042 um^2).  Such frames render
-through the per-pixel feature kernel (k_paths).  value = its Mrays/s over K
+through the feature variant of k_tail.  value = their Mrays/s over K
 frames issued back to back; vs_plain = that rate over the plain frame's rate
 (the same scene and camera without the features) measured the same way.
 """

@@ -320,7 +320,7 @@ FrameArgs frame_args(const dmt_frame_params* f) {
 }
 
 // A frame that draws more than the jitter per sample (lens point, shutter
-// time, hero wavelength) or translates the mesh renders through k_paths.
+// time, hero wavelength) or translates the mesh renders through k_tail<COUNT, true>.
 bool uses_features(const dmt_frame_params* f) {
     return f->dof[0] > 0.0f || f->motion_blur || f->spectral || f->mesh_shift_active;
 }

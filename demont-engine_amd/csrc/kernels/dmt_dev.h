@@ -542,10 +542,13 @@ __device__ __forceinline__ bool walk_begin(const SceneView& S, f3 o, f3 d, Walk&
 }
 
 // One step of the query: a node or one leaf fetch.  Returns true when the
-// query is done; its hit is then final_hit(w.ab, w.tb).
-template <bool COUNT>
-__device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& w, Stack& st, TraceCounters& tc) {
+// query is done; its hit is then final_hit(w.ab, w.tb).  With MOTION the
+// query is at shutter time `time`, as in trace_closest.
+template <bool COUNT, bool MOTION = false>
+__device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& w, Stack& st, TraceCounters& tc,
+                                          float time = 1.0f) {
     const bool tri = w.phase == 1u;
+    if (MOTION && tri && S.mesh_shift) o = o - mix3(S.mesh_prev, S.mesh_curr, time);
     const bool at_leaf = (w.cur & kLeaf) != 0u;
     const uint32_t lf = w.cur & kLeafFirst;
     const float4* p = at_leaf ? (tri ? S.tris + (size_t)lf * 3u : S.spheres + lf)
@@ -558,7 +561,7 @@ __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& 
     if (at_leaf) {
         const uint32_t count = ((w.cur >> 27) & 15u) + 1u;
         const uint32_t used = tri ? tri_leaf<COUNT>(o, d, lf, count, r, w.tb, tc)
-                                  : sphere_leaf<COUNT>(S, o, d, lf, count, r, w.ab, tc);
+                                  : sphere_leaf<COUNT, MOTION>(S, o, d, lf, count, r, w.ab, tc, time);
         if (used < count) {
             w.cur = kLeaf | ((count - used - 1u) << 27) | (lf + used);
             return false;

@@ -115,9 +115,12 @@ __device__ __forceinline__ void tail_start_shadow(const ShadowQueue& q, uint32_t
 
 // After a vertex or one of its shadow queries: the next shadow query (light
 // before env, as k_shadow<0> runs before k_shadow<1>), the next vertex, the
-// next sample of the pixel, or the pixel's frame value.
-__device__ __forceinline__ void tail_advance(const FrameArgs& F, const ShadowQueue& sl, const ShadowQueue& se,
-                                             const PathState& ps, TailLane& l, TailCold& c) {
+// next sample of the pixel, or the pixel's frame value.  FEAT: the frame's
+// camera-sample features X are on; the samples draw them (camera_sample) and
+// add their radiance weighted by the hero wavelength's response.
+template <bool FEAT>
+__device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureArgs& X, const ShadowQueue& sl,
+                                             const ShadowQueue& se, const PathState& ps, TailLane& l, TailCold& c) {
     if (l.pending & 1u) {
         tail_start_shadow(sl, 0u, l, c);
         return;
@@ -140,14 +143,24 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const ShadowQue
         return;
     }
     const uint32_t s = c.getu(TailCold::kS);
-    const f3 sum = (s == 0u) ? l.P.L : c.get3(TailCold::kSum) + l.P.L;
+    f3 sum;
+    if (FEAT) {
+        const f3 v = (l.P.lambda >= 0.0f) ? l.P.L * spectral_weight(l.P.lambda) : l.P.L;
+        sum = ((s == 0u) ? f3{0.0f, 0.0f, 0.0f} : c.get3(TailCold::kSum)) + v;
+    } else {
+        sum = (s == 0u) ? l.P.L : c.get3(TailCold::kSum) + l.P.L;
+    }
     if (s + 1u < F.spp) {
         // the next sample continues the pixel's stream (k_finish + k_regen)
         c.set3(TailCold::kSum, sum);
         c.setu(TailCold::kS, s + 1u);
         const uint32_t x = l.pix % F.width, y = l.pix / F.width;
-        l.d = camera_ray(F, x, y, l.P.seed);
-        l.o = F.pos;
+        if (FEAT) {
+            camera_sample(F, X, x, y, s + 1u, l.P.seed, l.o, l.d, l.P.time, l.P.lambda);
+        } else {
+            l.d = camera_ray(F, x, y, l.P.seed);
+            l.o = F.pos;
+        }
         l.P.T_ = f3{1.0f, 1.0f, 1.0f};
         l.P.L = f3{0.0f, 0.0f, 0.0f};
         l.P.prev_pdf = 0.0f;

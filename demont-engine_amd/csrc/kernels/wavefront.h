@@ -132,7 +132,7 @@ struct LaunchOptions {
     uint32_t tail_blocks = 0;             // k_tail grid (0: grid_for(pixels))
     uint32_t extend_blocks = 0;           // k_extend grid cap (0: grid_for(pixels))
     // Camera-sample features (thin lens, shutter time, hero wavelength) or a
-    // translated mesh: the frame renders through the per-pixel kernel k_paths.
+    // translated mesh: the frame renders in one k_tail<COUNT, true> launch.
     const FeatureArgs* features = nullptr;
 };
 

@@ -216,14 +216,20 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
 //
 // Five waves per SIMD (the LDS of a workgroup, stack and cold fields, allows
 // five per CU): 96 VGPRs, a few spilled values outside the traversal loop.
-template <bool COUNT>
+//
+// FEAT (frames with camera-sample features: thin lens, shutter time, hero
+// wavelength, a translated mesh): there is no queue; entry i is slot i of the
+// frame's 8x8 tile order, whose pixel's first sample the lane starts itself
+// (camera_sample), and the shutter time and wavelength ride in the lane's
+// path state.
+template <bool COUNT, bool FEAT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerSimd, kTailWavesPerSimd)))
-k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, PathState ps, ShadowQueue sl,
-       ShadowQueue se, uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
+k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uint32_t* ctr, PathState ps,
+       ShadowQueue sl, ShadowQueue se, uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
     __shared__ float cold[tail::TailCold::kRows * kBlock];
     tail::TailCold c{cold + threadIdx.x};
-    const uint32_t n = ctr[kCtrRays + in_par];
+    const uint32_t n = FEAT ? camera_tiles(F) * 64u : ctr[kCtrRays + in_par];
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
     Stack stk = make_stack(lds, spill, spill_depth);
@@ -251,7 +257,27 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, Pat
             if (idle & (1ull << lane) ? (idle & below) == 0ull : false) base = atomicAdd(&ctr[kCtrTail], want);
             base = __shfl(base, __ffsll((long long)idle) - 1);
             const uint32_t i = base + (uint32_t)__popcll(idle & below);
-            if (state == tail::kLaneIdle && i < n) {
+            if (FEAT && state == tail::kLaneIdle && i < n) {
+                const uint32_t tiles_x = (F.width + 7u) / 8u;
+                const uint32_t tile = i >> 6, ln = i & 63u;
+                const uint32_t x = (tile % tiles_x) * 8u + (ln & 7u), y = (tile / tiles_x) * 8u + (ln >> 3);
+                if (x < F.width && y < F.height) {
+                    l.pix = y * F.width + x;
+                    l.slot = l.pix;   // the pixel's shadow-record rows
+                    l.P = PathVertexState{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false,
+                                          pixel_seed(F, x, y, F.frame_index)};
+                    camera_sample(F, X, x, y, 0u, l.P.seed, l.o, l.d, l.P.time, l.P.lambda);
+                    l.b = 0u;
+                    c.setu(tail::TailCold::kS, 0u);
+                    l.mode = 0u;
+                    l.active = true;
+                    state = tail::kLaneWalk;
+                    if (walk_begin(S, l.o, l.d, w)) {
+                        h = final_hit(w.ab, w.tb);
+                        state = tail::kLaneReady;
+                    }
+                }
+            } else if (!FEAT && state == tail::kLaneIdle && i < n) {
                 const float4 o4 = q.o[i], d4 = q.d[i];
                 l.pix = __float_as_uint(d4.w);
                 l.slot = i;
@@ -297,13 +323,13 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, Pat
                     c.set3(tail::TailCold::kNo, R.no);
                     c.set3(tail::TailCold::kNd, R.nd);
                 }
-                tail::tail_advance(F, sl, se, ps, l, c);
+                tail::tail_advance<FEAT>(F, X, sl, se, ps, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(tail::TailCold::kTmax);
                 f3 tr = c.get3(tail::TailCold::kTr);
                 const uint32_t step = c.getu(tail::TailCold::kStep);
-                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr)) {
+                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         // fields picked one by one: a selected reference to either
@@ -314,7 +340,7 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, Pat
                                                      k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    tail::tail_advance(F, sl, se, ps, l, c);
+                    tail::tail_advance<FEAT>(F, X, sl, se, ps, l, c);
                 } else {   // transmit_step ends every query by its step refract_bounces
                     c.set(tail::TailCold::kTmax, t_max);
                     c.set3(tail::TailCold::kTr, tr);
@@ -334,7 +360,7 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, Pat
         // traversal steps of every lane with a query in progress
 #pragma unroll
         for (uint32_t k = 0; k < kTailSteps; ++k) {
-            if (state == tail::kLaneWalk && walk_step<COUNT>(S, l.o, l.d, w, stk, tc)) {
+            if (state == tail::kLaneWalk && walk_step<COUNT, FEAT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
                 h = final_hit(w.ab, w.tb);
                 state = tail::kLaneReady;
             }
@@ -347,81 +373,6 @@ k_tail(SceneView S, FrameArgs F, uint32_t in_par, RayQueue q, uint32_t* ctr, Pat
         tail::add_stat(&st->tail_nodes, tc.nodes);
         tail::add_stat(&st->tail_tris, tc.tris);
         tail::add_stat(&st->tail_spheres, tc.spheres);
-    }
-}
-
-// Frames with camera-sample features (thin lens, motion blur, hero
-// wavelength) or a translated mesh: one thread per pixel runs the pixel's
-// samples in order, each path to its end, its shadow queries traced in place
-// after the vertex that formed them (light NEE, then env NEE or the ambient
-// term: the order in which the wavefront form adds them).  The sample's
-// shutter time and wavelength ride in its path state; the frame value is the
-// spectrally weighted sample mean (PathTrace.slang:7693-7707).
-struct InlineSink : NoGradientHooks {
-    ShadowRec rec[2];
-    bool emitted[2];
-    __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState&) {
-        rec[kind] = r;
-        emitted[kind] = true;
-    }
-};
-
-template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_paths(SceneView S, FrameArgs F, FeatureArgs X, PathState ps,
-                                                  uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
-    __shared__ uint32_t lds[kLdsDepth * kBlock];
-    Stack stk = make_stack(lds, spill, spill_depth);
-    TraceCounters tc{0u, 0u, 0u};
-    uint32_t n_primary = 0, n_ext = 0, n_shadow = 0;
-    const uint32_t npix = F.width * F.height;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += stride) {
-        const uint32_t x = p % F.width, y = p / F.width;
-        uint32_t seed = pixel_seed(F, x, y, F.frame_index);
-        f3 sum{0.0f, 0.0f, 0.0f};
-        for (uint32_t s = 0; s < F.spp; ++s) {
-            f3 o, d;
-            float time, lambda;
-            camera_sample(F, X, x, y, s, seed, o, d, time, lambda);
-            PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed, time, lambda};
-            for (uint32_t b = 0; b < F.max_bounces; ++b) {
-                if (b == 0u) n_primary++;
-                else n_ext++;
-                const HitRec h = trace_closest<COUNT, true>(S, o, d, stk, tc, time);
-                InlineSink sink{{}, {}, {false, false}};
-                VertexResult R;
-                shade_vertex(S, F, b, o, d, h, P, R, sink);
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    if (!sink.emitted[k]) continue;
-                    const ShadowRec& r = sink.rec[k];
-                    if (k == 1 && r.pdf < 0.0f) {   // the no-env ambient term: no query
-                        P.L = P.L + r.pre;
-                        continue;
-                    }
-                    uint32_t steps = 0;
-                    const f3 tr = transmittance<COUNT, true>(S, F, r.o, r.d, r.t_max, stk, tc, steps, time);
-                    n_shadow += steps;
-                    if (any_pos(tr))
-                        P.L = P.L + shadow_value(F, k, r.pre, r.ndl, r.brdf, r.pdf, k == 1 ? r.w : 1.0f, r.clamp, tr);
-                }
-                if (!(R.alive && (b + 1u) < F.max_bounces)) break;
-                o = R.no;
-                d = R.nd;
-            }
-            seed = P.seed;
-            sum = sum + ((lambda >= 0.0f) ? P.L * spectral_weight(lambda) : P.L);
-        }
-        const f3 v = frame_value(F, sum);
-        ps.F[p] = make_float4(v.x, v.y, v.z, 0.0f);
-    }
-    add_stat(&st->primary, n_primary);
-    add_stat(&st->extension, n_ext);
-    add_stat(&st->shadow, n_shadow);
-    if (COUNT) {
-        add_stat(&st->nodes, tc.nodes);
-        add_stat(&st->tris, tc.tris);
-        add_stat(&st->spheres, tc.spheres);
     }
 }
 
@@ -490,6 +441,10 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(FrameArgs F, PathState ps
 
 __global__ void k_snapshot(const uint32_t* ctr, uint32_t par, uint32_t* host_slot) {
     if (threadIdx.x == 0) *host_slot = ctr[kCtrRays + par];
+}
+
+__global__ void k_reset_counters(uint32_t* ctr) {
+    if (threadIdx.x < kCounterSlots) ctr[threadIdx.x] = 0u;
 }
 
 __global__ void __launch_bounds__(kBlock) k_trace_query(SceneView S, const float4* rays, uint32_t n, HitOut* out,
@@ -663,13 +618,25 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
     const uint32_t g = grid_for(npix);
     o.events_used = 0;
     o.iterations = 0;
+    const uint32_t tg = o.tail_blocks ? (o.tail_blocks < g ? o.tail_blocks : g) : g;
     if (o.features) {
+        // the whole frame in one k_tail launch that starts the camera paths itself
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, W.counters);
+        const bool ev = o.extend_events && o.events_used + 2u <= o.max_events;
+        if (ev) {
+            o.event_kinds[0] = 1u;
+            (void)hipEventRecord(o.extend_events[0], stream);
+        }
         if (o.counting)
-            hipLaunchKernelGGL(k_paths<true>, dim3(g), dim3(kBlock), 0, stream, S, F, *o.features, W.ps, W.spill,
-                               W.spill_depth, W.stats);
+            hipLaunchKernelGGL((k_tail<true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, *o.features, 0u,
+                               W.rq[0], W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
         else
-            hipLaunchKernelGGL(k_paths<false>, dim3(g), dim3(kBlock), 0, stream, S, F, *o.features, W.ps, W.spill,
-                               W.spill_depth, W.stats);
+            hipLaunchKernelGGL((k_tail<false, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, *o.features, 0u,
+                               W.rq[0], W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+        if (ev) {
+            (void)hipEventRecord(o.extend_events[1], stream);
+            o.events_used = 2u;
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         o.iterations = 1;
@@ -677,6 +644,7 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
         hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, accum);
         return hipGetLastError();
     }
+    const FeatureArgs none{};
     hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, stream, F, W.ps, W.rq[0], W.counters);
     // Every path ends after at most max_bounces vertices and a pixel's samples
     // run in order, so the frame needs at most spp * max_bounces iterations.
@@ -689,17 +657,16 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
         const uint32_t par = (uint32_t)(it & 1u);
         if (it == o.tail_after) {
             // the live queue (and the pixels' later samples) in one launch
-            const uint32_t tg = o.tail_blocks ? (o.tail_blocks < g ? o.tail_blocks : g) : g;
             const bool ev = o.extend_events && o.events_used + 2u <= o.max_events;
             if (ev) {
                 o.event_kinds[o.events_used / 2u] = 1u;
                 (void)hipEventRecord(o.extend_events[o.events_used], stream);
             }
             if (o.counting)
-                hipLaunchKernelGGL(k_tail<true>, dim3(tg), dim3(kBlock), 0, stream, S, F, par, W.rq[par],
+                hipLaunchKernelGGL((k_tail<true, false>), dim3(tg), dim3(kBlock), 0, stream, S, F, none, par, W.rq[par],
                                    W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
             else
-                hipLaunchKernelGGL(k_tail<false>, dim3(tg), dim3(kBlock), 0, stream, S, F, par, W.rq[par],
+                hipLaunchKernelGGL((k_tail<false, false>), dim3(tg), dim3(kBlock), 0, stream, S, F, none, par, W.rq[par],
                                    W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
             if (ev) {
                 (void)hipEventRecord(o.extend_events[o.events_used + 1u], stream);

@@ -97,7 +97,8 @@ typedef struct dmt_frame_params {
      * jitter, the reference draws the hero wavelength (spectral), then the
      * shutter time (motion_blur), then the lens point (dof[0] > 0)
      * (PathTrace.slang:5729-5816).  A frame that uses any of them renders
-     * through the per-pixel feature kernel. */
+     * in one launch of the frame-completion kernel, which starts the camera
+     * paths itself. */
     float    dof[4];            /* dof_params: aperture radius (0 = pinhole),
                                    focal distance, blade count (< 3 = round),
                                    blade rotation in radians               */
