@@ -1124,8 +1124,9 @@ std::vector<BatchScene> batch_table(const dmt_batch* b, uint32_t tile_w, const s
         const dmt_frame_params& f = b->frames[i];
         BatchScene& e = t[i];
         std::memset(&e, 0, sizeof(e));
-        e.S = b->scenes[i]->view;
+        e.S = frame_view(b->scenes[i], &f);
         e.F = frame_args(&f);
+        e.X = feature_args(&f);
         e.accum = accum ? (*accum)[i] : b->scenes[i]->accum;
         e.tiles_x = (f.width + tile_w - 1u) / tile_w;
         const uint32_t tiles_y = (f.height + tile_h - 1u) / tile_h;
@@ -1154,7 +1155,9 @@ BatchArgs batch_args(const dmt_batch* b, const dmt_batch_params& p, const BatchS
 }
 
 BatchLaunch batch_launch(const dmt_batch* b, const dmt_batch_params& p) {
-    return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds};
+    bool feat = false;
+    for (const dmt_frame_params& f : b->frames) feat = feat || uses_features(&f);
+    return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds, feat};
 }
 
 void drop_graph(dmt_batch* b) {
@@ -1236,9 +1239,6 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         if (!list[i]) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is null");
         if (list[i]->dev != dev) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is on another device");
         if ((st = check_frame(&frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
-        if (uses_features(&frames[i]))
-            return fail(DMT_ERR_UNSUPPORTED, "frame " + std::to_string(i) +
-                                                 ": batches render without lens, motion blur, spectral or mesh translation");
     }
     {
         std::vector<dmt_scene*> sorted = list;

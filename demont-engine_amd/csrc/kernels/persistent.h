@@ -34,6 +34,7 @@ constexpr uint32_t kRecRows = 4;          // float4 rows per shadow record
 struct BatchScene {
     SceneView S;
     FrameArgs F;          // frame_index and reset_accum of the batch's frame 0
+    FeatureArgs X;        // camera-sample features (read by the feature variant only)
     float4* accum;
     uint32_t tiles_x;     // tiles per image row
     uint32_t tiles;       // tiles of the frame (0: nothing to render)
@@ -59,6 +60,7 @@ struct BatchArgs {
 struct BatchLaunch {
     uint32_t blocks;                  // grid size in workgroups
     uint32_t stack_lds;               // LDS stack entries per thread: 16, 24 or 32
+    bool features;                    // some scene's frame uses camera-sample features
 };
 
 // Sets the frame counter (stream-ordered, not part of a captured frame).

@@ -84,9 +84,16 @@ struct Cold {
     __device__ __forceinline__ void set3(uint32_t f, f3 v) { set(f, v.x); set(f + 1, v.y); set(f + 2, v.z); }
 };
 
-__device__ __forceinline__ void start_sample(const FrameArgs& F, Lane& l, uint32_t x, uint32_t y) {
-    l.d = camera_ray(F, x, y, l.P.seed);
-    l.o = F.pos;
+// Sample s of pixel (x, y); FEAT: with the frame's camera-sample features.
+template <bool FEAT>
+__device__ __forceinline__ void start_sample(const FrameArgs& F, const FeatureArgs& X, Lane& l, uint32_t x,
+                                             uint32_t y, uint32_t s) {
+    if (FEAT) {
+        camera_sample(F, X, x, y, s, l.P.seed, l.o, l.d, l.P.time, l.P.lambda);
+    } else {
+        l.d = camera_ray(F, x, y, l.P.seed);
+        l.o = F.pos;
+    }
     l.P.T_ = f3{1.0f, 1.0f, 1.0f};
     l.P.L = f3{0.0f, 0.0f, 0.0f};
     l.P.prev_pdf = 0.0f;
@@ -111,8 +118,9 @@ __device__ __forceinline__ void start_shadow(const float4* rec, uint32_t threads
 // After the vertex's shading or one of its shadow records: start the next
 // record (light before env, as the wavefront's shadow kernels run), or close
 // the vertex.
-__device__ __forceinline__ void advance(const FrameArgs& F, const float4* rec, uint32_t threads, uint32_t tid,
-                                        float4* accum, bool reset, Lane& l, Cold& c) {
+template <bool FEAT>
+__device__ __forceinline__ void advance(const FrameArgs& F, const FeatureArgs& X, const float4* rec, uint32_t threads,
+                                        uint32_t tid, float4* accum, bool reset, Lane& l, Cold& c) {
     if (l.pending & 1u) {
         start_shadow(rec, threads, tid, 0u, l, c);
         return;
@@ -138,12 +146,18 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const float4* rec, u
     }
     // the sample is done
     const uint32_t s = c.getu(Cold::kS);
-    const f3 sum = (s == 0u) ? l.P.L : c.get3(Cold::kSum) + l.P.L;
+    f3 sum;
+    if (FEAT) {   // the hero wavelength's response (PathTrace.slang:7693-7700)
+        const f3 v = (l.P.lambda >= 0.0f) ? l.P.L * spectral_weight(l.P.lambda) : l.P.L;
+        sum = ((s == 0u) ? f3{0.0f, 0.0f, 0.0f} : c.get3(Cold::kSum)) + v;
+    } else {
+        sum = (s == 0u) ? l.P.L : c.get3(Cold::kSum) + l.P.L;
+    }
     const uint32_t xy = c.getu(Cold::kXY), x = xy & 0xFFFFu, y = xy >> 16;
     if (s + 1u < F.spp) {
         c.set3(Cold::kSum, sum);
         c.setu(Cold::kS, s + 1u);
-        start_sample(F, l, x, y);   // the next sample continues the pixel's stream
+        start_sample<FEAT>(F, X, l, x, y, s + 1u);   // the next sample continues the pixel's stream
         return;
     }
     const uint32_t p = y * F.width + x;
@@ -156,7 +170,10 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const float4* rec, u
 // the occupancy the register budget is set for.
 template <uint32_t DEPTH> constexpr uint32_t persistent_waves() { return (160u * 1024u) / ((DEPTH + Cold::kRows) * kBlock * 4u); }
 
-template <uint32_t DEPTH>
+// FEAT: some scene's frame uses camera-sample features (thin lens, shutter
+// time, hero wavelength, a translated mesh); each scene's FeatureArgs say
+// which, and the lane carries the sample's shutter time and wavelength.
+template <uint32_t DEPTH, bool FEAT>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<DEPTH>()))) k_persistent(BatchArgs A) {
     __shared__ uint32_t lds[DEPTH * kBlock];
@@ -204,6 +221,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
             if (pool_lo < pool_hi) {
                 const FrameArgs F = const_copy(&bs->F);
                 const SceneView S = const_copy(&bs->S);
+                const FeatureArgs X = FEAT ? const_copy(&bs->X) : FeatureArgs{};
                 const uint32_t ord = pool_lo + (uint32_t)__popcll(idle & below);
                 if (!l.active && ord < pool_hi) {
                     const uint32_t tile = ord >> 6, k = ord & 63u;
@@ -214,7 +232,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                         c.setu(Cold::kXY, x | (y << 16));
                         c.setu(Cold::kS, 0u);
                         l.P.seed = pixel_seed(F, x, y, frame_index);
-                        start_sample(F, l, x, y);
+                        start_sample<FEAT>(F, X, l, x, y, 0u);
                         state = kLaneWalk;
                         if (walk_begin(S, l.o, l.d, w)) {
                             h = final_hit(w.ab, w.tb);
@@ -273,6 +291,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
         // the wave's scene, read again each step (nothing held across steps)
         const SceneView S = const_copy(&bs->S);
         const FrameArgs F = const_copy(&bs->F);
+        const FeatureArgs X = FEAT ? const_copy(&bs->X) : FeatureArgs{};
         // ---- the finished queries of the wave, together ----
         if (ready != 0ull && ((uint32_t)__popcll(ready) >= kPersistShadeLanes || walk == 0ull) &&
             state == kLaneReady) {
@@ -286,13 +305,13 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                 l.pending = sink.pending | (cont ? 0u : kEnded);
                 c.set3(Cold::kNo, R.no);
                 c.set3(Cold::kNd, R.nd);
-                advance(F, A.rec, A.threads, tid, accum, reset, l, c);
+                advance<FEAT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(Cold::kTmax);
                 f3 tr = c.get3(Cold::kTr);
                 const uint32_t step = c.getu(Cold::kStep);
-                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr)) {
+                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         const float4* row = A.rec + ((size_t)k * kRecRows) * A.threads + tid;
@@ -302,7 +321,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                                                      (l.pending & (4u << k)) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    advance(F, A.rec, A.threads, tid, accum, reset, l, c);
+                    advance<FEAT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
                 } else {
                     c.set(Cold::kTmax, t_max);
                     c.set3(Cold::kTr, tr);
@@ -322,7 +341,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
         // ---- traversal steps of every lane with a query in progress ----
 #pragma unroll
         for (uint32_t k = 0; k < kTailSteps; ++k) {
-            if (state == kLaneWalk && walk_step<false>(S, l.o, l.d, w, stk, tc)) {
+            if (state == kLaneWalk && walk_step<false, FEAT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
                 h = final_hit(w.ab, w.tb);
                 state = kLaneReady;
             }
@@ -355,10 +374,13 @@ hipError_t batch_set_frame(uint32_t* frame, uint32_t value, hipStream_t stream) 
 hipError_t launch_batch_frame(const BatchArgs& A, const BatchLaunch& L, hipStream_t stream) {
     if (L.blocks == 0u || (size_t)L.blocks * kBlock != A.threads) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, stream, A.work, A.nscenes);
-    switch (L.stack_lds) {
-        case 16u: hipLaunchKernelGGL(k_persistent<16u>, dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 24u: hipLaunchKernelGGL(k_persistent<24u>, dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 32u: hipLaunchKernelGGL(k_persistent<32u>, dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+    switch (L.stack_lds * 2u + (L.features ? 1u : 0u)) {
+        case 32u: hipLaunchKernelGGL((k_persistent<16u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 33u: hipLaunchKernelGGL((k_persistent<16u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 48u: hipLaunchKernelGGL((k_persistent<24u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 49u: hipLaunchKernelGGL((k_persistent<24u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 64u: hipLaunchKernelGGL((k_persistent<32u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 65u: hipLaunchKernelGGL((k_persistent<32u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
         default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, stream, A.frame);

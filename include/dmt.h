@@ -372,8 +372,8 @@ typedef struct dmt_batch_params {
  * Each scene's accumulation buffer is (re)sized to its frame's resolution;
  * while the batch lives the scene cannot be rendered at another resolution,
  * and dmt_scene_destroy of a member is deferred to dmt_batch_destroy.
- * params may be null (defaults).  Frames that use the camera-sample features
- * (dof, motion_blur, spectral, mesh_shift_active) are DMT_ERR_UNSUPPORTED. */
+ * params may be null (defaults).  Frames may use the camera-sample features
+ * (each scene its own). */
 dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt_frame_params* frames,
                             uint32_t count, const dmt_batch_params* params, dmt_batch** out_batch);
 void       dmt_batch_destroy(dmt_batch* batch);

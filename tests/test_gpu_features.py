@@ -2,8 +2,8 @@
 thin lens (round, polygonal, cat's-eye), motion blur (moving spheres, a
 translated mesh), hero-wavelength dispersion and all of them together; the
 G-buffer of a translated mesh; the engine path (PtPush fields decoded by the
-RHI device) equal to the direct render; and the adjoint and batches refusing
-the features."""
+RHI device) equal to the direct render; a batch mixing them equal to the
+per-scene renders; and the adjoint refusing them."""
 import numpy as np
 import pytest
 import torch
@@ -194,7 +194,39 @@ def test_engine_push_renders_the_features(device):
     assert np.array_equal(got, want)
 
 
-def test_adjoint_and_batches_refuse_the_features(device):
+@pytest.mark.parametrize("plain", [0, 1], ids=["graph", "plain launches"])
+def test_batch_with_features_equals_per_scene_renders(device, plain):
+    """A batch whose scenes use different features (none; lens + motion blur;
+    dispersion + a translated mesh) equals each scene rendered on its own."""
+    a = _spheres()
+    b = _moving(_spheres(), seed=31)
+    c = _mesh()
+    fa = a.frame.copy()
+    fb = b.frame.copy()
+    fb.spp, fb.dof, fb.cat_eye, fb.motion_blur = 2, (0.3, 9.5, 6.0, 0.3), 0.5, 1
+    fc = c.frame.copy()
+    fc.spectral, fc.cauchy_c, fc.mesh_shift_active = 1, 0.02, 1
+    fc.mesh_shift_prev, fc.mesh_shift_curr = (0.0, 0.0, 0.0), (0.3, 0.1, 0.0)
+    scenes = [dmt.Scene(device, sd) for sd in (a, b, c)]
+    frames = [fa, fb, fc]
+    batch = dmt.Batch(device, scenes, frames, dmt.BatchParams(plain_launches=plain))
+    batch.render(0, 2)
+    device.synchronize()
+    for sc, sd, f in zip(scenes, (a, b, c), frames):
+        got = sc.readback(f.width, f.height)
+        one = dmt.Scene(device, sd)
+        one.render_frames(f, 2)
+        want = one.readback(f.width, f.height)
+        one.close()
+        assert np.array_equal(got, want)
+    _, ref = _render_both(device, b, fb)
+    assert np.array_equal(scenes[1].readback(fb.width, fb.height), ref)
+    batch.close()
+    for sc in scenes:
+        sc.close()
+
+
+def test_adjoint_refuses_the_features(device):
     sd = _spheres()
     f = sd.frame.copy()
     f.motion_blur = 1
@@ -203,11 +235,6 @@ def test_adjoint_and_batches_refuse_the_features(device):
     g = torch.zeros(max(1, sd.prims.shape[0]) * 7, dtype=torch.float32, device="cuda:0")
     with pytest.raises(dmt.DmtError) as e:
         sc.adjoint(f, d.data_ptr(), g.data_ptr())
-    assert e.value.status == dmt.DMT_ERR_UNSUPPORTED
-    h = sd.frame.copy()
-    h.dof = (0.1, 5.0, 0.0, 0.0)
-    with pytest.raises(dmt.DmtError) as e:
-        dmt.Batch(device, [sc], [h])
     assert e.value.status == dmt.DMT_ERR_UNSUPPORTED
     bad = sd.frame.copy()
     bad.dof = (float("nan"), 1.0, 0.0, 0.0)
