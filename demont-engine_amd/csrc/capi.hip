@@ -329,7 +329,10 @@ FeatureArgs feature_args(const dmt_frame_params* f) {
     FeatureArgs X{};
     X.aperture = f->dof[0];
     X.focal = f->dof[1];
-    X.blades = (int)(f->dof[2] + 0.5f);   // int(dof_params.z + 0.5), PathTrace.slang:5789
+    // int(dof_params.z + 0.5) (PathTrace.slang:5789); below 3 every count is
+    // the round aperture, and check_frame bounds it above
+    const float blades = f->dof[2] + 0.5f;
+    X.blades = (blades >= 3.0f && blades < 65.0f) ? (int)blades : 0;
     X.rot = f->dof[3];
     X.cat_eye = f->cat_eye;
     X.motion_blur = f->motion_blur;
@@ -1026,8 +1029,6 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
     dmt_status st = check_device(s->dev);
     if (st) return st;
     if ((st = check_frame(f))) return st;
-    if (uses_features(f))
-        return fail(DMT_ERR_UNSUPPORTED, "the adjoint renders without lens, motion blur, spectral or mesh translation");
     if (f->max_bounces > kAdjMaxVerts) return fail(DMT_ERR_LIMIT, "the adjoint supports max_bounces <= 8");
     if (f->refract_bounces > kAdjMaxDielectric) return fail(DMT_ERR_LIMIT, "the adjoint supports refract_bounces <= 8");
     const uint32_t prims = s->view.prim_count;
@@ -1043,8 +1044,10 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
         }
     }
     AdjointArgs A{};
-    A.S = s->view;
+    A.S = frame_view(s, f);
     A.F = frame_args(f);
+    A.X = feature_args(f);
+    A.features = uses_features(f) ? 1u : 0u;
     A.dimage = (const float4*)d_image_grad;
     A.grads = (float*)d_param_grad;
     A.prim_count = prims;

@@ -44,6 +44,8 @@ constexpr uint32_t kAdjWavesPerSimd = 3;
 struct AdjointArgs {
     SceneView S;
     FrameArgs F;
+    FeatureArgs X;          // camera-sample features of the frame
+    uint32_t features;      // any of them on (or a translated mesh): the feature variants
     const float4* dimage;   // dLoss/dI per pixel (rgb; w unused)
     float* grads;           // prim_count x kMaterialFloats
     uint32_t prim_count;

@@ -22,6 +22,7 @@ __device__ __forceinline__ void add_grad(float* g, uint32_t i, float v) {
 // Path-replay sink: forms the radiance terms exactly as the renderer does
 // (shadow queries traced at once, in emission order) and scatters each
 // term's derivative.
+template <bool FEAT>
 struct AdjointSink : NoGradientHooks {
     const SceneView& S;
     const FrameArgs& F;
@@ -102,8 +103,8 @@ struct AdjointSink : NoGradientHooks {
         float t_max = r.t_max;
         bool done = false;
         for (uint32_t i = 0; i <= F.refract_bounces && !done; ++i) {
-            const HitRec h = trace_closest<false>(S, o, d, st, tc);
-            done = transmit_step(S, F, h, i, o, d, t_max, tr);
+            const HitRec h = trace_closest<false, FEAT>(S, o, d, st, tc, P.time);
+            done = transmit_step(S, F, h, i, o, d, t_max, tr, P.time);
 #pragma unroll
             for (uint32_t j = 0; j < kAdjMaxDielectric; ++j)
                 if (!done && j == nd) diel[j] = h.code;
@@ -145,7 +146,7 @@ struct AdjointSink : NoGradientHooks {
     }
 };
 
-template <bool LDS>
+template <bool LDS, bool FEAT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
 k_adjoint(AdjointArgs A) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
@@ -167,12 +168,15 @@ k_adjoint(AdjointArgs A) {
         const f3 delta = xyz(A.dimage[p]) / (float)F.spp;
         uint32_t seed = pixel_seed(F, x, y, F.frame_index);
         for (uint32_t s = 0; s < F.spp; ++s) {
-            f3 d = camera_ray(F, x, y, seed);
-            f3 o = F.pos;
-            PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed};
-            AdjointSink sink{{}, A.S, F, stk, tc, G, delta, 0u, {}, {}};
+            f3 o, d;
+            float time, lambda;
+            camera_sample(F, A.X, x, y, s, seed, o, d, time, lambda);
+            PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed, time, lambda};
+            // a spectral sample enters the frame value weighted by its wavelength's response
+            const f3 ds = (lambda >= 0.0f) ? delta * spectral_weight(lambda) : delta;
+            AdjointSink<FEAT> sink{{}, A.S, F, stk, tc, G, ds, 0u, {}, {}};
             for (uint32_t b = 0; b < F.max_bounces; ++b) {
-                const HitRec h = trace_closest<false>(A.S, o, d, stk, tc);
+                const HitRec h = trace_closest<false, FEAT>(A.S, o, d, stk, tc, time);
                 VertexResult R;
                 shade_vertex(A.S, F, b, o, d, h, P, R, sink);
                 if (!(R.alive && (b + 1u) < F.max_bounces)) break;
@@ -227,6 +231,7 @@ struct SkipSink : NoGradientHooks {
 
 // Dual vertices: roughness derivative seeded at bounce seed_b; shadow queries
 // traced at once along the dual ray.
+template <bool FEAT>
 struct TangentSink : NoGradientHooks {
     const SceneView& S;
     const FrameArgs& F;
@@ -246,15 +251,15 @@ struct TangentSink : NoGradientHooks {
         df t_max = r.t_max;
         bool done = false;
         for (uint32_t i = 0; i <= F.refract_bounces && !done; ++i) {
-            const HitRec h = trace_closest<false>(S, value3(o), value3(d), st, tc);
-            done = transmit_step(S, F, h, i, o, d, t_max, tr);
+            const HitRec h = trace_closest<false, FEAT>(S, value3(o), value3(d), st, tc, P.time);
+            done = transmit_step(S, F, h, i, o, d, t_max, tr, P.time);
         }
         if (!done || !any_pos(tr)) return;
         P.L = P.L + shadow_value(F, kind, r.pre, r.ndl, r.brdf, r.pdf, r.w, r.clamp, tr);
     }
 };
 
-template <bool LDS>
+template <bool LDS, bool FEAT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
 k_adjoint_rough(AdjointArgs A) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
@@ -278,12 +283,13 @@ k_adjoint_rough(AdjointArgs A) {
             const uint32_t sample_seed = seed;
             // the sample's metal vertices
             MetalSink ms{{}, 0u, {}, {}};
+            float time = 1.0f, lambda = -1.0f;
             {
-                f3 d = camera_ray(F, x, y, seed);
-                f3 o = F.pos;
-                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed};
+                f3 o, d;
+                camera_sample(F, A.X, x, y, s, seed, o, d, time, lambda);
+                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed, time, lambda};
                 for (uint32_t b = 0; b < F.max_bounces; ++b) {
-                    const HitRec h = trace_closest<false>(A.S, o, d, stk, tc);
+                    const HitRec h = trace_closest<false, FEAT>(A.S, o, d, stk, tc, time);
                     VertexResult R;
                     shade_vertex(A.S, F, b, o, d, h, P, R, ms);
                     if (!(R.alive && (b + 1u) < F.max_bounces)) break;
@@ -301,13 +307,13 @@ k_adjoint_rough(AdjointArgs A) {
                         pj = ms.prim[k];
                     }
                 uint32_t rs = sample_seed;
-                f3 d = camera_ray(F, x, y, rs);
-                f3 o = F.pos;
-                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, rs};
+                f3 o, d;
+                camera_sample(F, A.X, x, y, s, rs, o, d, time, lambda);
+                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, rs, time, lambda};
                 SkipSink skip{};
                 uint32_t b = 0;
                 for (; b < bj; ++b) {
-                    const HitRec h = trace_closest<false>(A.S, o, d, stk, tc);
+                    const HitRec h = trace_closest<false, FEAT>(A.S, o, d, stk, tc, time);
                     VertexResult R;
                     shade_vertex(A.S, F, b, o, d, h, P, R, skip);
                     o = R.no;
@@ -315,17 +321,18 @@ k_adjoint_rough(AdjointArgs A) {
                 }
                 d3 od = to<df>(o), dd = to<df>(d);
                 PathVertexStateT<df> PD{to<df>(P.T_), to<df>(f3{0.0f, 0.0f, 0.0f}), df(P.prev_pdf), P.skip_sky,
-                                        P.prev_lambert, P.seed};
-                TangentSink ts{{}, A.S, F, stk, tc, bj};
+                                        P.prev_lambert, P.seed, time, lambda};
+                TangentSink<FEAT> ts{{}, A.S, F, stk, tc, bj};
                 for (; b < F.max_bounces; ++b) {
-                    const HitRec h = trace_closest<false>(A.S, value3(od), value3(dd), stk, tc);
+                    const HitRec h = trace_closest<false, FEAT>(A.S, value3(od), value3(dd), stk, tc, time);
                     VertexResultT<df> R;
                     shade_vertex(A.S, F, b, od, dd, h, PD, R, ts);
                     if (!(R.alive && (b + 1u) < F.max_bounces)) break;
                     od = R.no;
                     dd = R.nd;
                 }
-                const float g = dot(delta, deriv3(PD.L));
+                const f3 ds = (lambda >= 0.0f) ? delta * spectral_weight(lambda) : delta;
+                const float g = dot(ds, deriv3(PD.L));
                 if (LDS) add_grad(lds_grad, pj, g);
                 else add_grad(A.grads, pj * kMaterialFloats + 3u, g);
             }
@@ -353,15 +360,22 @@ hipError_t launch_adjoint(const AdjointArgs& A, hipStream_t stream) {
     const uint32_t npix = A.F.width * A.F.height;
     if (npix == 0 || A.F.max_bounces == 0 || nparam == 0) return hipGetLastError();
     const uint32_t blocks = min(kMaxGridBlocks, (npix + kBlock - 1u) / kBlock);
-    if (A.prim_count <= kAdjLdsPrims)
-        hipLaunchKernelGGL(k_adjoint<true>, dim3(blocks), dim3(kBlock), 0, stream, A);
-    else
-        hipLaunchKernelGGL(k_adjoint<false>, dim3(blocks), dim3(kBlock), 0, stream, A);
+    const bool lds = A.prim_count <= kAdjLdsPrims;
+    if (A.features) {
+        if (lds) hipLaunchKernelGGL((k_adjoint<true, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
+        else hipLaunchKernelGGL((k_adjoint<false, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
+    } else {
+        if (lds) hipLaunchKernelGGL((k_adjoint<true, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
+        else hipLaunchKernelGGL((k_adjoint<false, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
+    }
     if (A.metal_prims) {
-        if (A.prim_count <= kAdjLdsPrims)
-            hipLaunchKernelGGL(k_adjoint_rough<true>, dim3(blocks), dim3(kBlock), 0, stream, A);
-        else
-            hipLaunchKernelGGL(k_adjoint_rough<false>, dim3(blocks), dim3(kBlock), 0, stream, A);
+        if (A.features) {
+            if (lds) hipLaunchKernelGGL((k_adjoint_rough<true, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
+            else hipLaunchKernelGGL((k_adjoint_rough<false, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
+        } else {
+            if (lds) hipLaunchKernelGGL((k_adjoint_rough<true, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
+            else hipLaunchKernelGGL((k_adjoint_rough<false, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
+        }
     }
     return hipGetLastError();
 }

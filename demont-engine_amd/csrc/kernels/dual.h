@@ -62,8 +62,8 @@ __device__ __forceinline__ f3 deriv3(d3 a) { return f3{a.x.d, a.y.d, a.z.d}; }
 // Surface of a hit along a dual ray: the hit distance keeps the traversal's
 // value h.t and takes its derivative from the primitive's implicit equation
 // g(o + t d) = 0, dt = -(dg . do + t dg . dd) / (dg . d), with dg the sphere's
-// radius vector or the plane / triangle normal at the hit.  The adjoint
-// renders without motion blur: `time` is always 1.
+// radius vector or the plane / triangle normal at the hit.  A sphere is taken
+// about its centre at shutter time `time` (PathTrace.slang:2093).
 __device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 d, HitRec h, float time = 1.0f) {
     SurfaceT<df> s;
     surface_material(S, h, s);
@@ -81,6 +81,7 @@ __device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 
         if (v2.x == 0.0f) {
             sphere = true;
             centre = xyz(v0);
+            if (time < 1.0f) centre = mix3(xyz(p[3]), centre, time);
             g = (ov + dv * h.t) - centre;
         } else {
             g = quat_rotate(p[5], xyz(v0));

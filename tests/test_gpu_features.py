@@ -3,7 +3,8 @@ thin lens (round, polygonal, cat's-eye), motion blur (moving spheres, a
 translated mesh), hero-wavelength dispersion and all of them together; the
 G-buffer of a translated mesh; the engine path (PtPush fields decoded by the
 RHI device) equal to the direct render; a batch mixing them equal to the
-per-scene renders; and the adjoint refusing them."""
+per-scene renders; and invalid feature values refused.  (The adjoint with
+the features: tests/test_gpu_adjoint.py.)"""
 import numpy as np
 import pytest
 import torch
@@ -226,20 +227,19 @@ def test_batch_with_features_equals_per_scene_renders(device, plain):
         sc.close()
 
 
-def test_adjoint_refuses_the_features(device):
+def test_invalid_feature_values_are_refused(device):
     sd = _spheres()
-    f = sd.frame.copy()
-    f.motion_blur = 1
     sc = dmt.Scene(device, sd)
-    d = torch.zeros(f.width * f.height * 4, dtype=torch.float32, device="cuda:0")
-    g = torch.zeros(max(1, sd.prims.shape[0]) * 7, dtype=torch.float32, device="cuda:0")
-    with pytest.raises(dmt.DmtError) as e:
-        sc.adjoint(f, d.data_ptr(), g.data_ptr())
-    assert e.value.status == dmt.DMT_ERR_UNSUPPORTED
-    bad = sd.frame.copy()
-    bad.dof = (float("nan"), 1.0, 0.0, 0.0)
-    with pytest.raises(dmt.DmtError) as e:
-        sc.render(bad)
-    assert e.value.status == dmt.DMT_ERR_INVALID_ARG
+    cases = [("dof", (float("nan"), 1.0, 0.0, 0.0), dmt.DMT_ERR_INVALID_ARG),
+             ("dof", (0.1, 5.0, 100.0, 0.0), dmt.DMT_ERR_LIMIT),
+             ("motion_blur", 2, dmt.DMT_ERR_INVALID_ARG),
+             ("mesh_shift_curr", (0.0, float("inf"), 0.0), dmt.DMT_ERR_INVALID_ARG),
+             ("reserved", 1, dmt.DMT_ERR_INVALID_ARG)]
+    for field, value, status in cases:
+        bad = sd.frame.copy()
+        setattr(bad, field, value)
+        with pytest.raises(dmt.DmtError) as e:
+            sc.render(bad)
+        assert e.value.status == status, field
     sc.close()
     assert not device.is_lost()
