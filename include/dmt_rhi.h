@@ -26,7 +26,9 @@
  *     over the same triangles -- moved instances -- through
  *     dmt_scene_update_mesh, a refit of the mesh's tree on the device);
  *   - the frame renders into the accum texture's device memory (slot 1, bound
- *     with dmt_scene_bind_accum, so the running mean survives a rebuild);
+ *     with dmt_scene_bind_accum, so the running mean survives a rebuild),
+ *     with the push's depth of field, motion blur (v3 of the primitives as
+ *     each sphere's previous centre), spectral dispersion and mesh translation;
  *   - the output texture (slot 0) receives sRGB(tonemap(mean * exposure))
  *     through dmt_tonemap, exposure = exposure_state[0] of buffer slot 6;
  *   - with denoiser_enabled, the bound G-buffer textures (depth 3, motion 4,
