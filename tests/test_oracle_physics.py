@@ -10,9 +10,16 @@ mathematics, and checks the oracle against it:
     reflects exactly a (env NEE + BSDF sampling under MIS, PathTrace.slang:6457-6492,
     6056-6085; the furnace rationale is stated at PathTrace.slang:6536-6541);
   * glass furnace: a clear dielectric leaves throughput unchanged
-    (PathTrace.slang:6709-6767), so every escaping path carries L=1 exactly.
-Wrong pdfs (area->solid angle), a dropped 1/pi, a wrong MIS weight or a wrong
-Fresnel branch each fail one of them.
+    (PathTrace.slang:6709-6767), so every escaping path carries L=1 exactly;
+  * mirror: a smooth metal plane returns albedo * Le of an emitter behind the
+    camera, exactly;
+  * closed emitter: inside an emissive Lambert sphere every vertex adds T*Le,
+    so L is the bounce series Le (1 - a^N) / (1 - a) without Russian roulette
+    and its expectation with it;
+  * accumulation: the running mean of K frames and the EMA recursion.
+Wrong pdfs (area->solid angle), a dropped 1/pi, a wrong MIS weight, a wrong
+Fresnel branch, a wrong reflection, a dropped 1/p of Russian roulette or a
+wrong accumulation weight each fail one of them.
 """
 import math
 
@@ -137,3 +144,89 @@ def test_glass_furnace_preserves_radiance():
     # by the 32-bounce limit (internal reflections) come back darker
     assert np.mean(rgb == 1.0) > 0.97
     assert np.all(rgb <= 1.0)
+
+
+def test_mirror_reflects_an_emitter_exactly():
+    """A smooth metal plane facing the camera sends the centre rays straight
+    back to an emissive sphere behind the camera: radiance = albedo * Le
+    (PathTrace.slang:6683-6708: T *= albedo, r = reflect(rd, n) at roughness
+    0; the emitter adds T * Le at bounce 1, :6193-6199)."""
+    albedo, Le = np.float32(0.5), np.float32(1.5)
+    mirror = scenes.plane((0, 0, 1), 0.0, albedo=(albedo, albedo, albedo), mat=1, rough=0.0)
+    emitter = scenes.sphere((0, 0, 9), 1.0, albedo=(0, 0, 0), emission=(Le, Le, Le))
+    sd = scenes.scene([mirror, emitter], [], env=np.zeros((8, 16, 3), np.float32))
+    f = scenes.frame((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), 10.0, 16, 16, bounces=3)
+    img, _ = orc.OracleScene.from_scene_data(sd).render(f, 32, 0)
+    rgb = img[..., :3]
+    hit = np.all(rgb == albedo * Le, axis=-1)
+    assert np.all(hit | np.all(rgb == 0.0, axis=-1))   # each pixel sees the emitter or nothing
+    assert hit[6:10, 6:10].all() and 40 < hit.sum() < 256
+
+
+def _closed_emitter(albedo, Le, rr, bounces, frames, spp=4):
+    """Camera inside an emissive Lambert sphere, black env, no lights: every
+    vertex adds T * Le and multiplies T by the albedo."""
+    inside = scenes.sphere((0, 0, 0), 20.0, albedo=(albedo,) * 3, emission=(Le,) * 3)
+    sd = scenes.scene([inside], [], env=np.zeros((8, 16, 3), np.float32))
+    f = scenes.frame((0.0, 0.0, 0.0), (0.0, 0.0, -1.0), 60.0, 24, 16, spp=spp, bounces=bounces)
+    f.flags = 0 if rr else 1
+    sc = orc.OracleScene.from_scene_data(sd)
+    acc = None
+    for k in range(frames):
+        f.frame_index = k
+        f.reset_accum = 1 if k == 0 else 0
+        acc, _ = sc.render(f, 64, 0, accum=acc)
+    return acc[..., :3]
+
+
+def test_closed_emitter_sums_the_bounce_series():
+    """Without Russian roulette each path hits the enclosure at every vertex:
+    L = Le (1 + a + ... + a^(N-1)) on every pixel (the max_bounces limit and
+    the throughput chain, PathTrace.slang:6193-6199, 6668)."""
+    a, Le, n = 0.7, 0.5, 8
+    img = _closed_emitter(a, Le, rr=False, bounces=n, frames=1)
+    assert np.allclose(img, Le * (1.0 - a ** n) / (1.0 - a), rtol=1e-6)   # fp64 render, fp32 accumulator
+
+
+def test_russian_roulette_is_unbiased():
+    """From bounce 2 a path survives with p = clamp(max(T), 0.05, 1) and
+    carries T / p (PathTrace.slang:6908-6913): the mean stays the bounce
+    series (without the 1/p it would fall short of it)."""
+    a, Le, n = 0.7, 0.5, 8
+    want = Le * (1.0 - a ** n) / (1.0 - a)
+    img = _closed_emitter(a, Le, rr=True, bounces=n, frames=24)
+    assert abs(img.mean() - want) < 0.01 * want
+    assert img.std() > 0.01   # the estimator is random, pixel by pixel
+
+
+def test_accumulation_is_the_running_mean_and_the_ema():
+    """accum after K frames = the mean of the K frame values; with
+    r_accum_ema_alpha = e it follows accum = e accum + (1 - e) v from the
+    second frame on (PathTrace.slang:7736-7781)."""
+    sd = scenes.scene([scenes.sphere((0, 0, 0), 1.0, albedo=(0.6, 0.5, 0.4)), _floor()],
+                      [scenes.point_light((1, 4, 2), (30, 30, 30))])
+    f = scenes.frame((0.0, 1.0, 5.0), (0.0, 0.0, 0.0), 40.0, 12, 8, spp=2, bounces=4)
+    sc = orc.OracleScene.from_scene_data(sd)
+    singles = []
+    for k in range(5):
+        g = f.copy()
+        g.frame_index, g.reset_accum = k, 1
+        img, _ = sc.render(g, 32, 0)
+        singles.append(img[..., :3].astype(np.float64))
+    acc = None
+    for k in range(5):
+        g = f.copy()
+        g.frame_index, g.reset_accum = k, 1 if k == 0 else 0
+        acc, _ = sc.render(g, 32, 0, accum=acc)
+    assert np.all(acc[..., 3] == 5.0)
+    assert np.allclose(acc[..., :3], np.mean(singles, axis=0), rtol=1e-5, atol=1e-7)
+    e = 0.5
+    acc = None
+    for k in range(5):
+        g = f.copy()
+        g.frame_index, g.reset_accum, g.ema_alpha = k, 1 if k == 0 else 0, e
+        acc, _ = sc.render(g, 32, 0, accum=acc)
+    want = singles[0]
+    for k in range(1, 5):
+        want = e * want + (1.0 - e) * singles[k]
+    assert np.allclose(acc[..., :3], want, rtol=1e-5, atol=1e-7)
