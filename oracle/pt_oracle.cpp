@@ -3,7 +3,11 @@
 // Restates, on the CPU, the subset of the reference path tracer that the HIP
 // path implements:
 //   * RNG                 shaders/PathTraceMath.slang:16-24, seed PathTrace.slang:5457
-//   * camera ray          shaders/PathTrace.slang:5729-5778 (jitter 5733)
+//   * camera ray          shaders/PathTrace.slang:5729-5816 (jitter 5733; hero wavelength
+//                         5737-5759, shutter time 5761-5773, thin lens 5779-5816;
+//                         ptConcentricDisk / ptPolygonSample / ptCatEye, heroWavelength,
+//                         cauchyIOR, spectralWeightSRGB: PathTraceMath.slang:96-310)
+//   * motion blur         sphere centre lerp PathTrace.slang:2093, mesh translation 2461-2487
 //   * sphere / plane      shaders/PathTraceMath.slang:194-214, testAnalyticPrim PathTrace.slang:2066-2161
 //   * slab test           shaders/PathTrace.slang:2321-2329 (made conservative, see box_hit)
 //   * traceScene          shaders/PathTrace.slang:2331-2693 (planes, analytic BVH, tri BVH, MT)
@@ -25,7 +29,9 @@
 //   * the slab test is widened by 2*gamma(3) (PBRT's robust bound) so the BVH
 //     result equals the exhaustive scan; orc_trace_bruteforce pins that;
 //   * multi-draw constructors float2(randf, randf) draw left to right;
-//   * lerp(a, b, t) = a + t*(b - a).
+//   * lerp(a, b, t) = a + t*(b - a), except the shutter-time lerp, which is
+//     a*(1 - t) + b*t (exact at t = 1, as the reference's comments require);
+//   * exp is the Cephes expf restatement (orc_math.h).
 #include "oracle.h"
 #include "orc_math.h"
 
