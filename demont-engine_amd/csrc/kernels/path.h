@@ -407,13 +407,16 @@ __device__ __forceinline__ V3<T> shadow_value(const FrameArgs& F, int kind, V3<T
     return clamp ? clamp_indirect(F, c) : c;
 }
 
-// Final value of a frame from the sample sum: spp average, NaN/Inf/negative
-// sanitising (PathTrace.slang:7722-7734).
-__device__ __forceinline__ f3 frame_value(const FrameArgs& F, f3 sum) {
+// Final value of a frame from the sample sum: spp average, NaN/Inf zeroed,
+// negative channels zeroed unless the frame is spectral
+// (PathTrace.slang:7722-7734): the hero wavelength's sRGB weights are
+// legitimately negative, and those signed values must reach the running mean
+// for it to stay unbiased (the display clamps them).
+__device__ __forceinline__ f3 frame_value(const FrameArgs& F, f3 sum, bool spectral = false) {
     f3 fr = sum / (float)F.spp;
     float o[3] = {fr.x, fr.y, fr.z};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) o[k] = (isnan(o[k]) || isinf(o[k])) ? 0.0f : fmaxf(o[k], 0.0f);
+    for (int k = 0; k < 3; ++k) o[k] = (isnan(o[k]) || isinf(o[k])) ? 0.0f : (spectral ? o[k] : fmaxf(o[k], 0.0f));
     return f3{o[0], o[1], o[2]};
 }
 

@@ -161,7 +161,7 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const FeatureArgs& X
         return;
     }
     const uint32_t p = y * F.width + x;
-    const f3 v = frame_value(F, sum);
+    const f3 v = frame_value(F, sum, FEAT && X.spectral != 0u);
     accum[p] = accumulate(F, reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : accum[p], reset, v);
     l.active = false;
 }

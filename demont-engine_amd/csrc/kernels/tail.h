@@ -170,7 +170,7 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureAr
         l.mode = 0u;
         return;
     }
-    const f3 v = frame_value(F, sum);
+    const f3 v = frame_value(F, sum, FEAT && X.spectral != 0u);
     ps.F[l.pix] = make_float4(v.x, v.y, v.z, 0.0f);
     l.active = false;
 }

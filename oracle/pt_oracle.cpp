@@ -1133,10 +1133,13 @@ template <class R> struct PT {
             fr = fr + ((lambda >= R(0)) ? L * spectral_weight(lambda) : L);
         }
         fr = fr / R(spp);
+        // NaN/Inf -> 0; negative channels -> 0 unless the frame is spectral
+        // (PathTrace.slang:7722-7734: signed hero-wavelength values reach
+        // the running mean, which stays unbiased; display clamps them)
         float o[3] = {float(fr.x), float(fr.y), float(fr.z)};
         for (int k = 0; k < 3; ++k) {
             if (std::isnan(o[k]) || std::isinf(o[k])) o[k] = 0.0f;
-            else o[k] = std::max(o[k], 0.0f);
+            else if (!F.spectral) o[k] = std::max(o[k], 0.0f);
         }
         float* a = accum + ((size_t)y * F.width + x) * 4;
         float prev[4] = {0, 0, 0, 0};
@@ -1185,7 +1188,8 @@ void render_rows(const orc_scene& S, const orc_frame& F, int threads, uint32_t r
 }
 
 // d(frame value)/d(parameter) of pixel (x, y) by a PT<Dual> run; the
-// sanitising step (NaN/Inf -> 0, max(0, .)) is taken with derivative 1.
+// sanitising step (NaN/Inf -> 0, max(0, .) outside spectral frames) is taken
+// with derivative 1.
 void pixel_tangent(const PT<Dual>& pt, const orc_frame& F, uint32_t x, uint32_t y, double dI[3], double I[3]) {
     uint32_t seed = pcg_hash(x * 1973u + y * 9277u + F.frame_index * 26699u);
     using R = Dual;
@@ -1203,7 +1207,7 @@ void pixel_tangent(const PT<Dual>& pt, const orc_frame& F, uint32_t x, uint32_t 
     for (int k = 0; k < 3; ++k) {
         const bool ok = std::isfinite(c[k].v) && std::isfinite(c[k].d);
         dI[k] = ok ? c[k].d : 0.0;
-        I[k] = ok ? std::max(c[k].v, 0.0) : 0.0;
+        I[k] = ok ? (F.spectral ? c[k].v : std::max(c[k].v, 0.0)) : 0.0;
     }
 }
 

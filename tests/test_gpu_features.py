@@ -125,6 +125,30 @@ def test_spectral_dispersion_equals_the_oracle(device, cauchy):
     _check(f"spectral C={cauchy}", got, ref)
 
 
+def test_spectral_one_spp_keeps_signed_frame_values(device):
+    """At 1 spp a spectral frame value carries the hero wavelength's signed
+    sRGB weight; negative channels reach the running mean unclamped
+    (PathTrace.slang:7722-7734), in the frame-completion kernel and in the
+    batched kernel alike."""
+    sd = _spheres()
+    f = sd.frame.copy()
+    f.spp = 1
+    f.spectral = 1
+    f.cauchy_c = 0.0042
+    got1, ref1 = _render_both(device, sd, f, frames=1)
+    _check("spectral 1 spp, one frame", got1, ref1)
+    assert (got1[..., :3] < 0.0).any()
+    got, ref = _render_both(device, sd, f, frames=4)
+    _check("spectral 1 spp, four frames", got, ref)
+    sc = dmt.Scene(device, sd)
+    batch = dmt.Batch(device, [sc], [f])
+    batch.render(0, 4)
+    device.synchronize()
+    assert np.array_equal(sc.readback(f.width, f.height), ref)
+    batch.close()
+    sc.close()
+
+
 def test_all_features_together(device):
     sd = _moving(_mesh(), seed=23)
     f = sd.frame.copy()

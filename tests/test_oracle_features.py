@@ -224,3 +224,29 @@ def test_spectral_off_and_equal_indices_match_rgb():
         h.frame_index, h.reset_accum = k, 1 if k == 0 else 0
         acc, _ = osc.render(h, 32, 0, accum=acc)
     assert abs(acc[..., :3].mean() - rgb[..., :3].mean()) < 0.03 * rgb[..., :3].mean()
+
+
+def test_spectral_accumulation_keeps_signed_values_at_one_spp():
+    """At 1 spp a spectral frame value is L * weight(lambda), whose channels
+    are negative for out-of-gamut wavelengths; they must reach the running
+    mean (PathTrace.slang:7722-7734), so that the mean over frames converges
+    to L (the weight's band mean is 1).  Zeroing negatives per frame raises
+    the red channel by tens of percent, which fails here."""
+    sd = scene(env=np.full((8, 16, 3), 0.5, np.float32))   # camera sees only a uniform grey sky
+    f = frame((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), 40.0, 32, 16, spp=1)
+    f.spectral, f.cauchy_c = 1, 0.0
+    osc = orc.OracleScene.from_scene_data(sd)
+    acc = None
+    first_negative = False
+    nframes = 160
+    for k in range(nframes):
+        g = f.copy()
+        g.frame_index, g.reset_accum = k, 1 if k == 0 else 0
+        acc, _ = osc.render(g, 32, 0, accum=acc)
+        if k == 0:
+            first_negative = bool((acc[..., :3] < 0.0).any())
+    assert first_negative, "a single spectral frame at 1 spp must keep negative channels"
+    assert np.all(acc[..., 3] == nframes)
+    mean = acc[..., :3].reshape(-1, 3).mean(0)
+    # 81,920 wavelength samples: the mean of each channel is within ~1 % of 0.5
+    assert np.allclose(mean, 0.5, rtol=0.03), mean
