@@ -1,64 +1,78 @@
-"""Benchmark: Mrays/s of the path tracer on the 1.51 M-triangle 1080p workload.
+"""Benchmark of the path tracer: the three figures of its headline metric in
+one run -- Mrays/s on the 1.51 M-triangle 1080p workload, the batched form's
+speedup over plain per-scene launches, the backward/forward time ratio of the
+differentiable form -- plus Mrays/s on the many-light workload.
 
-Workload (config 1 of the synthetic generators): icosphere subdivided 8x
-(1,310,720 triangles, radius 3, value-noise displacement 0.1) + 200,000 random
-small triangles in [-5,5]^3, one quad light, 512x256 lognormal env map,
-1920x1080, 1 spp, 8 bounces, binned-SAH BVH4 built on the host (not timed).
+`python bench.py` (the default, --workload all) prints ONE JSON line: the
+1080p workload's figures at top level (value = its Mrays/s), and the objects
+"c2", "batched" and "adjoint" for the other workloads.  --workload c1 / c2 /
+batch / adjoint / features runs one of them alone (its own line).
 
-A step is one frame (1920x1080, 1 spp, up to 8 bounces, accumulated into the
-running mean), issued as one dmt_render call: camera rays into the path
-queue (k_generate), the frame-completion kernel (k_tail) that runs every path
-to its end, the accumulation pass (k_accumulate).  The library keeps four
-frames in flight on four HIP streams (the accumulation passes stay ordered
-frame after frame, so the image equals the sequential result).  value =
-(primary + extension + shadow rays) / wall time of the K frames, inputs
-resident in HBM.
+c1 (top level): icosphere subdivided 8x (1,310,720 triangles, radius 3, value-
+noise displacement 0.1) + 200,000 random small triangles in [-5,5]^3, one quad
+light, 512x256 lognormal env map, 1920x1080, 1 spp, 8 bounces, binned-SAH BVH4
+built on the host (not timed).  A step is one frame (1920x1080, 1 spp, up to 8
+bounces, accumulated into the running mean), issued as one dmt_render call:
+camera rays into the path queue (k_generate), the frame-completion kernel
+(k_tail) that runs every path to its end, the accumulation pass
+(k_accumulate).  The library keeps four frames in flight on four HIP streams
+(the accumulation passes stay ordered frame after frame, so the image equals
+the sequential result).  value = (primary + extension + shadow rays) / wall
+time of the K frames, inputs resident in HBM.
 
-roofline: the kernel with the most device time per frame (k_tail on the
-default schedule; k_extend, the closest-hit kernel, when the camera bounce
-runs through the per-bounce kernels), HBM-bound by design (no dense
-contraction).  Algorithmic bytes per launch = node visits x 128 B (BVH4 node)
-+ triangle tests x 48 B (TriRec) + sphere tests x 20 B (+ rays x 40 B for
-k_extend: 32 B ray record in, 8 B hit out), counted by an instrumented
-re-render of the same frame; divided by the HIP-event duration of that
-kernel's launches in frames rendered one at a time after the timed region (in
-the timed region four frames share the GPU, and a launch's duration there
-includes the kernels of the other frames).  traffic: the measured bytes per
-launch beyond L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE passes of this workload,
-scaled by a 512 MiB copy), read from profiles/<kernel>_c1_1080p_pmc.txt.
+c2: 4,096 spheres (a third each Lambert, metal, glass) + a 500k-triangle mesh
+lit by 4,096 emissive triangles through the light tree, 1280x720, 4 spp,
+depth 6: the workload that exercises next-event estimation, the light tree
+and shadow transmittance.  Same step and value definitions.
 
-cpu_baseline: the CPU oracle (reference semantics, fp32) rendering whole frames
-of the same workload for about 10 s, on this host's cores (hw - 4 threads as
-the reference's software tracer uses, and all cores).
+roofline (c1 and c2): the frame-completion kernel k_tail, which has no dense
+contraction (no MFMA), against the 8 TB/s HBM peak.  achieved = HBM-side bytes
+per launch, measured (rocprofv3 FETCH_SIZE + WRITE_SIZE passes of the same
+workload, scaled by a 512 MiB copy; recorded in profiles/k_tail_<c>.txt) /
+the kernel's launch duration, measured here with HIP events on its stream in
+frames rendered one at a time after the timed region.  frac = achieved /
+peak.  The profile records a hash of the kernel sources it was taken with;
+"profile_matches_sources" says whether the current sources hash the same.
+"algorithmic" gives the bytes the traversal reads at the node level: 128 B
+per BVH4 node visit + 48 B per triangle test + 20 B per sphere test, counted
+by an instrumented re-render of the same frame, and their rate; most of them
+are served by the L2 caches and the 256 MB MALL (l2_hit), which is why that
+rate can exceed the HBM peak.  l2_hit, valu_busy and valu_lane_utilisation
+come from the same profile; "limiter" names what they show.
 
---workload batch: the batched form.  64 scenes of the batched workload
-generator (resolution uniform in {128, 256, 512, 1024} squared, triangles
-log-uniform in 10k..2M, half with 64 spheres; 1 spp, depth 6).  The plain
-form renders each frame as 64 sequential per-scene dmt_render calls, each to
-completion; the batched form renders each frame of all 64 scenes with one
-persistent-kernel launch replayed from a captured hipGraph, with parameters
-picked by dmt_batch_autotune.  All run the same frames from frame 0; the
-images must be identical.  value = plain time / batched time over the timed
-frames.  vs_pipelined_plain compares with the same dmt_render calls issued
-back to back, the library keeping up to four frames in flight.
+batched: 64 scenes of the batched workload generator (resolution uniform in
+{128, 256, 512, 1024} squared, triangles log-uniform in 10k..2M, half with 64
+spheres; 1 spp, depth 6).  The plain form renders each frame as 64 per-scene
+dmt_render calls; the batched form renders each frame of all 64 scenes with
+one persistent-kernel launch replayed from a captured hipGraph, with
+parameters picked by dmt_batch_autotune.  All run the same frames from frame
+0; the images must be identical.  value = plain time / batched time over the
+timed frames, the plain renders issued back to back (the library keeps up to
+four frames in flight); vs_sequential_plain compares with the plain renders
+each run to completion (host sync after every scene's frame).
 
---workload adjoint: the differentiable form.  Config 4: 256 spheres with 256
-distinct materials over the ground plane, 512x512, 16 spp, depth 4, Russian
-roulette off.  The target image is rendered with the generator's materials;
-the spheres' parameters are then perturbed by N(0, 0.1) (albedo kept in
-[0.05, 1], emission >= 0.02).  A forward step is one dmt_render of the frame
-(to completion), a backward step one dmt_adjoint with dLoss/dI = 2 (I - target)
-(to completion).  value = backward time / forward time (lower is better).
+adjoint: config 4: 256 spheres with 256 distinct materials over the ground
+plane, 512x512, 16 spp, depth 4, Russian roulette off.  The target image is
+rendered with the generator's materials; the spheres' parameters are then
+perturbed by N(0, 0.1) (albedo kept in [0.05, 1], emission >= 0.02).  A
+forward step is one dmt_render of the frame (to completion), a backward step
+one dmt_adjoint with dLoss/dI = 2 (I - target) (to completion).  value =
+backward time / forward time (lower is better).
 
 --workload features: the camera-sample features on the config-1 workload:
 thin lens (aperture 0.05, focus 12, six blades, cat's-eye 0.3), motion blur
 (the mesh translated from (-0.1, 0, 0) to its rest position over the
-shutter), hero-wavelength dispersion (C = 0.0042 um^2).  Such frames render
-through the feature variant of k_tail.  value = their Mrays/s over K
-frames issued back to back; vs_plain = that rate over the plain frame's rate
-(the same scene and camera without the features) measured the same way.
+shutter), hero-wavelength dispersion (C = 0.0042 um^2).  value = their Mrays/s
+over K frames issued back to back; vs_plain = that rate over the plain
+frame's rate (the same scene and camera without the features).
+
+cpu_baseline (c1): the CPU oracle (reference semantics, fp32) rendering whole
+frames of the same workload for about 10 s, on this host's cores (hw - 4
+threads as the reference's software tracer uses, and all cores).
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
 import sys
@@ -68,31 +82,77 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "demont-engine_amd", "python"))
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = {"c1": "k_tail_c1_1080p.txt", "c2": "k_tail_c2_720p.txt"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the 1080p workload, from the
-    recorded rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel
-    of known size; a profiler cannot run inside this process)."""
+def kernel_sources_sha256():
+    """Hash of the device sources (kernel headers and sources, the C ABI's
+    launch code): a profile taken with other sources is flagged."""
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "demont-engine_amd", "csrc")
+    files = sorted(glob.glob(os.path.join(csrc, "kernels", "*.h")) + glob.glob(os.path.join(csrc, "kernels", "*.hip")))
+    files.append(os.path.join(csrc, "capi.hip"))
+    for p in files:
+        h.update(os.path.relpath(p, ROOT).encode())
+        with open(p, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def read_profile(workload):
+    """The figures section of profiles/k_tail_<workload>.txt (key: value)."""
+    name = PROFILES.get(workload)
+    if not name:
+        return None
+    path = os.path.join(ROOT, "profiles", name)
+    out = {"path": os.path.relpath(path, ROOT)}
     try:
-        for line in open(os.path.join(ROOT, "profiles", f"{kernel}_c1_1080p_pmc.txt")):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+        with open(path) as fh:
+            figures = False
+            for line in fh:
+                if line.startswith("## figures"):
+                    figures = True
+                    continue
+                if figures and ":" in line:
+                    k, v = line.split(":", 1)
+                    out[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    return out if "traffic_bytes_per_launch" in out else None
+
+
+def roofline(workload, kernel, avg_ms, algo_bytes, launches, frames):
+    r = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+         "kernel": kernel, "avg_launch_ms": round(avg_ms, 5),
+         "timing": f"{launches} launches of {frames} frames rendered one at a time, HIP events on the launch stream"}
+    prof = read_profile(workload)
+    if prof:
+        traffic = int(prof["traffic_bytes_per_launch"])
+        achieved = traffic / (avg_ms * 1e-3) / 1e9
+        busy, util = float(prof["valu_busy"]), float(prof["valu_lane_utilisation"])
+        r.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                 profile=prof["path"], profile_matches_sources=prof.get("kernel_sources_sha256") == kernel_sources_sha256(),
+                 l2_hit=float(prof["l2_hit"]), valu_busy=busy, valu_lane_utilisation=util,
+                 limiter=(f"VALU issue: {busy:.0%} VALU busy at {util:.0%} lane utilisation (divergence), "
+                          f"{achieved / HBM_PEAK_GBS:.0%} of HBM bandwidth" if busy >= 0.5 else
+                          f"dependent fetch latency: {busy:.0%} VALU busy, {achieved / HBM_PEAK_GBS:.0%} of HBM bandwidth"))
+    r["algorithmic"] = {"bytes_per_launch": int(algo_bytes),
+                        "per_unit": "128 B per BVH4 node visit + 48 B per triangle test + 20 B per sphere test",
+                        "GBps": round(algo_bytes / (avg_ms * 1e-3) / 1e9, 1),
+                        "served_by": "L2 and MALL for the most part (l2_hit), so this rate is not HBM bandwidth"}
+    return r
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "c2", "batch", "adjoint", "features"), default="c1")
-    ap.add_argument("--steps", type=int, default=None, help="timed frames (c1: 20, batch: 100)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed frames (c1: 3, batch: 2)")
+    ap.add_argument("--workload", choices=("all", "c1", "c2", "batch", "adjoint", "features"), default="all")
+    ap.add_argument("--steps", type=int, default=None, help="timed frames (c1/c2: 20, batch: 100, adjoint: 10)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed frames (c1/c2: 3, batch/adjoint: 2)")
     ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -101,7 +161,7 @@ def main():
                     help="per-bounce iterations before the frame-completion launch (-1: per-bounce kernels only; "
                          "default: the library's)")
     ap.add_argument("--roofline-frames", type=int, default=10,
-                    help="frames rendered one at a time to time the closest-hit kernel alone")
+                    help="frames rendered one at a time to time the frame-completion kernel alone")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -117,20 +177,6 @@ def main():
     if not torch.cuda.is_available():
         log("no GPU visible")
         sys.exit(1)
-    if args.workload == "batch":
-        args.steps = 100 if args.steps is None else args.steps
-        args.warmup = 2 if args.warmup is None else args.warmup
-        return bench_batch(args, np, torch, dmt)
-    if args.workload == "adjoint":
-        args.steps = 10 if args.steps is None else args.steps
-        args.warmup = 2 if args.warmup is None else args.warmup
-        return bench_adjoint(args, np, torch, dmt)
-    if args.workload == "features":
-        args.steps = 10 if args.steps is None else args.steps
-        args.warmup = 2 if args.warmup is None else args.warmup
-        return bench_features(args, np, torch, dmt)
-    args.steps = 20 if args.steps is None else args.steps
-    args.warmup = 3 if args.warmup is None else args.warmup
     if args.calibration_copy_mb > 0:
         # one plain device-to-device copy of known size, so that the HBM
         # counters of a profiler pass can be scaled to bytes
@@ -142,11 +188,43 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calibration_copy_mb} MiB done")
         del src, dst
-    cfg = 2 if args.workload == "c2" else 1
+
+    def steps(default):
+        return default if args.steps is None else args.steps
+
+    def warm(default):
+        return default if args.warmup is None else args.warmup
+
+    if args.workload == "batch":
+        line = bench_batch(np, torch, dmt, args.scenes, args.seed, steps(100), warm(2))
+    elif args.workload == "adjoint":
+        line = bench_adjoint(np, torch, dmt, args.seed, steps(10), warm(2))
+    elif args.workload == "features":
+        line = bench_features(np, torch, dmt, args.seed, steps(10), warm(2))
+    elif args.workload in ("c1", "c2"):
+        line = bench_frames(np, torch, dmt, args, 1 if args.workload == "c1" else 2, steps(20), warm(3),
+                            cpu=not args.no_cpu_baseline and args.workload == "c1")
+    else:
+        line = bench_frames(np, torch, dmt, args, 1, steps(20), warm(3), cpu=not args.no_cpu_baseline)
+        c2 = bench_frames(np, torch, dmt, args, 2, 20, 3, cpu=False)
+        line["c2"] = {k: c2[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "warmup", "config",
+                                         "roofline")}
+        b = bench_batch(np, torch, dmt, args.scenes, args.seed, 100, 2)
+        line["batched"] = {k: b[k] for k in b if k not in ("n_gpus", "higher_is_better", "scaling", "vs_baseline",
+                                                          "dtype", "data", "roofline", "cpu_baseline")}
+        a = bench_adjoint(np, torch, dmt, args.seed, 10, 2)
+        line["adjoint"] = {k: a[k] for k in a if k not in ("n_gpus", "scaling", "vs_baseline", "dtype", "data",
+                                                          "roofline", "cpu_baseline")}
+        line["metric"] = "Mrays/s @1080p 1M tris 1spp; batched vs plain speedup; bwd/fwd time ratio"
+    print(json.dumps(line), flush=True)
+
+
+def bench_frames(np, torch, dmt, args, cfg, nsteps, nwarm, cpu):
     t0 = time.time()
     sd = dmt.generate(cfg, seed=args.seed)
-    log(f"[bench] scene: {sd.indices.shape[0]} triangles, {sd.frame.width}x{sd.frame.height}, "
-        f"{sd.frame.max_bounces} bounces ({time.time() - t0:.1f}s)")
+    log(f"[bench] c{cfg} scene: {sd.indices.shape[0]} triangles, {sd.prims.shape[0]} prims, {sd.lights.shape[0]} lights,"
+        f" {sd.frame.width}x{sd.frame.height}, {sd.frame.spp} spp, {sd.frame.max_bounces} bounces "
+        f"({time.time() - t0:.1f}s)")
     dev = dmt.Device(0)
     t0 = time.time()
     scene = dmt.Scene(dev, sd)
@@ -180,36 +258,37 @@ def main():
 
     # K frames, one dmt_render call each (the library keeps four in flight)
     def per_frame():
-        for k in range(args.steps):
+        for k in range(nsteps):
             f = frame.copy()
-            f.frame_index = args.warmup + k
+            f.frame_index = nwarm + k
             f.reset_accum = 0
             scene.render(f)
 
-    run_frames(0, args.warmup)
+    run_frames(0, nwarm)
     dt, st = timed(per_frame)
     rays = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
     mrays = rays / dt / 1e6
-    log(f"[bench] {args.steps} frames in {dt * 1e3:.1f} ms: {mrays:.1f} Mrays/s "
+    log(f"[bench] {nsteps} frames in {dt * 1e3:.1f} ms: {mrays:.1f} Mrays/s "
         f"(primary {st['primary_rays']}, extension {st['extension_rays']}, shadow {st['shadow_rays']})")
 
-    # Algorithmic bytes of the closest-hit kernel (k_extend) and of the
+    # Node-level bytes of the closest-hit kernel (k_extend) and of the
     # frame-completion kernel (k_tail), from an instrumented re-render.
     f = frame.copy()
-    f.frame_index = args.warmup
+    f.frame_index = nwarm
     tr = scene.measure_traffic(f)
     ext_bytes = (tr["ext_node_visits"] * 128 + tr["ext_tri_tests"] * 48 + tr["ext_sphere_tests"] * 20
                  + tr["ext_rays"] * 40)
     tail_bytes = tr["tail_node_visits"] * 128 + tr["tail_tri_tests"] * 48 + tr["tail_sphere_tests"] * 20
+
     # In the timed region four frames share the GPU, so a launch's event
     # duration there includes other frames' kernels running beside it.  A
     # kernel's own launch duration comes from frames rendered one at a time
     # (each waits for the previous one): the same frame as the instrumented
-    # re-render, so its algorithmic bytes are exact.
+    # re-render, so its node-level bytes are exact.
     def isolated():
         for _ in range(args.roofline_frames):
             fi = frame.copy()
-            fi.frame_index = args.warmup
+            fi.frame_index = nwarm
             fi.reset_accum = 0
             scene.render(fi)
             dev.synchronize()
@@ -225,19 +304,18 @@ def main():
         avg_ms = st_iso[f"{key}_ms"] / launches
         shared = st[f"{key}_ms"] / max(st[f"{key}_launches"], 1)
         kern[name] = dict(bytes_per_launch=per_launch, avg_ms=avg_ms, ms_per_frame=st_iso[f"{key}_ms"] / nf,
-                          launches=launches, achieved=per_launch / (avg_ms * 1e-3) / 1e9)
-        log(f"[bench] {name}: {launches / nf:.0f} launch(es)/frame, {per_launch / 1e6:.1f} MB/launch, "
-            f"avg launch {avg_ms:.3f} ms alone -> {kern[name]['achieved']:.0f} GB/s "
-            f"({shared:.3f} ms with four frames in flight)")
-    log(f"[bench] k_extend: {tr['ext_rays']} rays, {tr['ext_node_visits'] / max(tr['ext_rays'], 1):.1f} nodes/ray, "
-        f"{tr['ext_tri_tests'] / max(tr['ext_rays'], 1):.2f} tri tests/ray; k_tail: {tr['tail_node_visits']} node "
-        f"visits, {tr['tail_tri_tests']} tri tests, {tr['tail_sphere_tests']} sphere tests")
+                          launches=launches)
+        log(f"[bench] {name}: {launches / nf:.0f} launch(es)/frame, {per_launch / 1e6:.1f} MB node-level/launch, "
+            f"avg launch {avg_ms:.3f} ms alone ({shared:.3f} ms with four frames in flight)")
+    log(f"[bench] k_tail: {tr['tail_node_visits']} node visits, {tr['tail_tri_tests']} tri tests, "
+        f"{tr['tail_sphere_tests']} sphere tests per frame")
     dom = max(kern, key=lambda k: kern[k]["ms_per_frame"])
     K = kern[dom]
-    bytes_per_launch, avg_launch_ms, achieved = K["bytes_per_launch"], K["avg_ms"], K["achieved"]
+    roof = roofline(f"c{cfg}", dom, K["avg_ms"], K["bytes_per_launch"], int(K["launches"]), nf)
+    log(f"[bench] roofline {roof}")
 
-    cpu = None
-    if not args.no_cpu_baseline:
+    cpu_line = None
+    if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_ctypes as orc
         ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
@@ -260,15 +338,15 @@ def main():
                     return crays, el, k
 
         crays, cdt, nfr = cpu_run(threads)
-        cpu = {"value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"{nfr} whole {frame.width}x{frame.height} frames ({crays} rays, {cdt:.1f} s) of the fp32 CPU oracle "
-                         f"(reference semantics, same scene, median BVH), {threads} threads of {ncpu} "
-                         f"(hw - 4, SoftwareTracer.cpp:867-868)"}
+        cpu_line = {"value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                    "sample": f"{nfr} whole {frame.width}x{frame.height} frames ({crays} rays, {cdt:.1f} s) of the fp32 "
+                              f"CPU oracle (reference semantics, same scene, median BVH), {threads} threads of {ncpu} "
+                              f"(hw - 4, SoftwareTracer.cpp:867-868)"}
         if threads != ncpu:
             crays2, cdt2, nfr2 = cpu_run(ncpu)
-            cpu["all_cores"] = {"value": round(crays2 / cdt2 / 1e6, 3), "cores": ncpu,
-                                "sample": f"{nfr2} frames, {cdt2:.1f} s"}
-        log(f"[bench] cpu baseline {cpu}")
+            cpu_line["all_cores"] = {"value": round(crays2 / cdt2 / 1e6, 3), "cores": ncpu,
+                                     "sample": f"{nfr2} frames, {cdt2:.1f} s"}
+        log(f"[bench] cpu baseline {cpu_line}")
 
     if cfg == 1:
         metric, workload = "Mrays/s @1080p 1M tris 1spp", (
@@ -276,16 +354,16 @@ def main():
             "1920x1080, 1 spp, max depth 8, binned-SAH BVH4")
     else:
         metric, workload = "Mrays/s (many-light mixed scene)", (
-            "4,096 spheres + 500k-tri mesh lit by 4,096 emissive triangles through the light tree, all-zero env, "
-            "1280x720, 4 spp, max depth 6, binned-SAH BVH4")
+            "4,096 spheres (Lambert, metal, glass) + 500k-tri mesh lit by 4,096 emissive triangles through the light "
+            "tree, all-zero env, 1280x720, 4 spp, max depth 6, binned-SAH BVH4")
     line = {
         "metric": metric,
         "value": round(mrays, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "steps": nsteps,
+        "warmup": nwarm,
+        "ms_per_step": round(dt * 1e3 / nsteps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -293,33 +371,25 @@ def main():
         "data": f"synthetic (seeded generator, config {cfg})",
         "config": {"workload": workload,
                    "triangles": int(sd.indices.shape[0]), "width": frame.width, "height": frame.height,
-                   "spp": frame.spp, "max_bounces": frame.max_bounces, "rays_per_frame": rays // args.steps,
+                   "spp": frame.spp, "max_bounces": frame.max_bounces, "rays_per_frame": rays // nsteps,
+                   "shadow_rays_per_frame": st["shadow_rays"] // nsteps,
                    "schedule": ("per-bounce kernels" if schedule == dmt.SCHEDULE_WAVEFRONT else
                                 f"{schedule} per-bounce iteration(s), then one frame-completion launch")},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom) if cfg == 1 else None,
-                     "kernel": dom, "bytes_per_launch": int(bytes_per_launch),
-                     "avg_launch_ms": round(avg_launch_ms, 5),
-                     "timing": f"{int(K['launches'])} launches of {args.roofline_frames} frames rendered "
-                               "one at a time, HIP events on the launch stream",
-                     "other_kernels": {k: {"achieved": round(v["achieved"], 1), "avg_launch_ms": round(v["avg_ms"], 5),
-                                           "bytes_per_launch": int(v["bytes_per_launch"])}
-                                       for k, v in kern.items() if k != dom}},
-        "cpu_baseline": cpu,
+        "roofline": roof,
+        "cpu_baseline": cpu_line,
     }
-    print(json.dumps(line), flush=True)
     scene.close()
     dev.close()
+    return line
 
 
-def bench_batch(args, np, torch, dmt):
+def bench_batch(np, torch, dmt, nscenes, seed, nsteps, nwarm):
     t0 = time.time()
     dev = dmt.Device(0)
     scenes, frames = [], []
     tris = pixels = 0
-    for i in range(args.scenes):
-        sd = dmt.generate_batch_scene(args.seed, i)
+    for i in range(nscenes):
+        sd = dmt.generate_batch_scene(seed, i)
         scenes.append(dmt.Scene(dev, sd))
         frames.append(sd.frame.copy())
         tris += int(sd.indices.shape[0])
@@ -332,10 +402,10 @@ def bench_batch(args, np, torch, dmt):
         dev.synchronize()
         torch.cuda.synchronize()
 
-    # plain: each frame is one dmt_render per scene.  Sequential: every
-    # render runs to completion before the next starts (host sync after each
-    # scene's frame).  Pipelined: the library's own frames in flight (up to
-    # four renders overlap on the device).
+    # plain: each frame is one dmt_render per scene.  Pipelined: the
+    # library's own frames in flight (up to four renders overlap on the
+    # device).  Sequential: every render runs to completion before the next
+    # starts (host sync after each scene's frame).
     def plain(k0, n, each=None):
         for k in range(k0, k0 + n):
             for sc, f in zip(scenes, frames):
@@ -347,13 +417,13 @@ def bench_batch(args, np, torch, dmt):
                     each()
 
     def timed_plain(each):
-        plain(0, args.warmup, each)
+        plain(0, nwarm, each)
         sync()
         for sc in scenes:
             sc.reset_stats()
         sync()
         t0 = time.perf_counter()
-        plain(args.warmup, args.steps, each)
+        plain(nwarm, nsteps, each)
         sync()
         dt = time.perf_counter() - t0
         rays = 0
@@ -363,39 +433,38 @@ def bench_batch(args, np, torch, dmt):
         return dt, rays, [sc.readback(f.width, f.height) for sc, f in zip(scenes, frames)]
 
     dt_pipe, rays_plain, images = timed_plain(None)
-    log(f"[bench] plain, pipelined: {args.steps} frames in {dt_pipe * 1e3:.1f} ms "
+    log(f"[bench] plain, pipelined: {nsteps} frames in {dt_pipe * 1e3:.1f} ms "
         f"({rays_plain / dt_pipe / 1e6:.1f} Mrays/s)")
-    dt_plain, rays_seq, images_seq = timed_plain(dev.synchronize)
+    dt_seq, rays_seq, images_seq = timed_plain(dev.synchronize)
     same_seq = all(np.array_equal(a, b) for a, b in zip(images, images_seq)) and rays_seq == rays_plain
-    log(f"[bench] plain, sequential: {args.steps} frames in {dt_plain * 1e3:.1f} ms "
-        f"({rays_plain / dt_plain / 1e6:.1f} Mrays/s), images equal to the pipelined renders {same_seq}")
+    log(f"[bench] plain, sequential: {nsteps} frames in {dt_seq * 1e3:.1f} ms "
+        f"({rays_plain / dt_seq / 1e6:.1f} Mrays/s), images equal to the pipelined renders {same_seq}")
 
     batch = dmt.Batch(dev, scenes, frames)
     t0 = time.time()
     tuned, tune_ms = batch.autotune(frames_per_trial=2)
     log(f"[bench] autotune {tuned.as_dict()} {tune_ms:.2f} ms/frame ({time.time() - t0:.1f}s)")
-    batch.render(0, args.warmup)
+    batch.render(0, nwarm)
     sync()
     batch.reset_stats()
     sync()
     t0 = time.perf_counter()
-    batch.render(args.warmup, args.steps)
+    batch.render(nwarm, nsteps)
     sync()
     dt_batch = time.perf_counter() - t0
     st = batch.stats()
     rays_batch = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
     same = [bool(np.array_equal(sc.readback(f.width, f.height), img)) for sc, f, img in zip(scenes, frames, images)]
-    log(f"[bench] batch: {args.steps} frames in {dt_batch * 1e3:.1f} ms ({rays_batch / dt_batch / 1e6:.1f} Mrays/s), "
+    log(f"[bench] batch: {nsteps} frames in {dt_batch * 1e3:.1f} ms ({rays_batch / dt_batch / 1e6:.1f} Mrays/s), "
         f"identical images {sum(same)}/{len(same)}, rays equal {rays_batch == rays_plain}")
-    speedup = dt_plain / dt_batch
     line = {
         "metric": "batched vs plain speedup",
-        "value": round(speedup, 3),
+        "value": round(dt_pipe / dt_batch, 3),
         "unit": "x",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt_batch * 1e3 / args.steps, 4),
+        "steps": nsteps,
+        "warmup": nwarm,
+        "ms_per_step": round(dt_batch * 1e3 / nsteps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -404,27 +473,27 @@ def bench_batch(args, np, torch, dmt):
         "config": {"workload": f"{len(scenes)} scenes: resolution uniform in {{128,256,512,1024}}^2, triangles "
                                "log-uniform 10k..2M (icosphere + random tris), odd scenes + 64 spheres; "
                                "1 spp, max depth 6; one persistent launch per frame from a captured hipGraph "
-                               "vs 64 sequential per-scene renders (each to completion)",
-                   "scenes": len(scenes), "pixels": pixels, "triangles": tris, "seed": args.seed,
+                               "vs 64 per-scene renders issued back to back",
+                   "scenes": len(scenes), "pixels": pixels, "triangles": tris, "seed": seed,
                    "params": tuned.as_dict()},
-        "plain_ms_per_step": round(dt_plain * 1e3 / args.steps, 4),
+        "plain_ms_per_step": round(dt_pipe * 1e3 / nsteps, 4),
         "batch_mrays": round(rays_batch / dt_batch / 1e6, 2),
-        "plain_mrays": round(rays_plain / dt_plain / 1e6, 2),
-        "vs_pipelined_plain": round(dt_pipe / dt_batch, 3),
-        "pipelined_plain_ms_per_step": round(dt_pipe * 1e3 / args.steps, 4),
+        "plain_mrays": round(rays_plain / dt_pipe / 1e6, 2),
+        "vs_sequential_plain": round(dt_seq / dt_batch, 3),
+        "sequential_plain_ms_per_step": round(dt_seq * 1e3 / nsteps, 4),
         "identical_images": all(same) and rays_batch == rays_plain and same_seq,
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
     batch.close()
     for sc in scenes:
         sc.close()
     dev.close()
+    return line
 
 
-def bench_adjoint(args, np, torch, dmt):
-    sd = dmt.generate(4, seed=args.seed)
+def bench_adjoint(np, torch, dmt, seed, nsteps, nwarm):
+    sd = dmt.generate(4, seed=seed)
     dev = dmt.Device(0)
     sc = dmt.Scene(dev, sd)
     f = sd.frame.copy()
@@ -434,7 +503,7 @@ def bench_adjoint(args, np, torch, dmt):
     gt = sc.materials()
     sc.render(f)
     target = sc.readback(W, H)[..., :3].astype(np.float64)
-    rng = np.random.default_rng(args.seed + 100)
+    rng = np.random.default_rng(seed + 100)
     init = gt.copy()
     init[1:] = gt[1:] + rng.normal(0.0, 0.1, size=gt[1:].shape).astype(np.float32)
     init[1:, 0:3] = np.clip(init[1:, 0:3], 0.05, 1.0)
@@ -445,11 +514,11 @@ def bench_adjoint(args, np, torch, dmt):
         dev.synchronize()
         torch.cuda.synchronize()
 
-    for _ in range(args.warmup):
+    for _ in range(nwarm):
         sc.render(f)
     sync()
     fwd = []
-    for _ in range(args.steps):
+    for _ in range(nsteps):
         t0 = time.perf_counter()
         sc.render(f)
         sync()
@@ -460,11 +529,11 @@ def bench_adjoint(args, np, torch, dmt):
     dimg[..., :3] = torch.from_numpy((2.0 * (img - target)).astype(np.float32)).cuda()
     grads = torch.empty((sd.prims.shape[0], dmt.MATERIAL_FLOATS), device="cuda", dtype=torch.float32)
     sync()
-    for _ in range(args.warmup):
+    for _ in range(nwarm):
         sc.adjoint(f, dimg.data_ptr(), grads.data_ptr())
     sync()
     bwd = []
-    for _ in range(args.steps):
+    for _ in range(nsteps):
         t0 = time.perf_counter()
         sc.adjoint(f, dimg.data_ptr(), grads.data_ptr())
         sync()
@@ -479,8 +548,8 @@ def bench_adjoint(args, np, torch, dmt):
         "value": round(bwd_ms / fwd_ms, 3),
         "unit": "x",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": nsteps,
+        "warmup": nwarm,
         "ms_per_step": round(bwd_ms, 4),
         "higher_is_better": False,
         "scaling": "weak",
@@ -494,16 +563,17 @@ def bench_adjoint(args, np, torch, dmt):
                    "params": int(sd.prims.shape[0] - 1) * 7},
         "forward_ms": round(fwd_ms, 4),
         "backward_ms": round(bwd_ms, 4),
+        "gradients_finite": bool(np.isfinite(g).all()),
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
     sc.close()
     dev.close()
+    return line
 
 
-def bench_features(args, np, torch, dmt):
-    sd = dmt.generate(1, seed=args.seed)
+def bench_features(np, torch, dmt, seed, nsteps, nwarm):
+    sd = dmt.generate(1, seed=seed)
     dev = dmt.Device(0)
     scene = dmt.Scene(dev, sd)
     plain = sd.frame.copy()
@@ -522,7 +592,7 @@ def bench_features(args, np, torch, dmt):
         torch.cuda.synchronize()
 
     def run(fr):
-        for k in range(args.warmup):
+        for k in range(nwarm):
             f = fr.copy()
             f.frame_index, f.reset_accum = k, 1 if k == 0 else 0
             scene.render(f)
@@ -530,15 +600,15 @@ def bench_features(args, np, torch, dmt):
         scene.reset_stats()
         sync()
         t0 = time.perf_counter()
-        for k in range(args.steps):
+        for k in range(nsteps):
             f = fr.copy()
-            f.frame_index, f.reset_accum = args.warmup + k, 0
+            f.frame_index, f.reset_accum = nwarm + k, 0
             scene.render(f)
         sync()
         dt = time.perf_counter() - t0
         st = scene.stats()
         rays = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
-        return rays / dt / 1e6, dt * 1e3 / args.steps, rays / args.steps
+        return rays / dt / 1e6, dt * 1e3 / nsteps, rays / nsteps
 
     mp, msp, rp = run(plain)
     mf, msf, rf = run(feat)
@@ -549,8 +619,8 @@ def bench_features(args, np, torch, dmt):
         "value": round(mf, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": nsteps,
+        "warmup": nwarm,
         "ms_per_step": round(msf, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -567,9 +637,9 @@ def bench_features(args, np, torch, dmt):
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
     scene.close()
     dev.close()
+    return line
 
 
 if __name__ == "__main__":
