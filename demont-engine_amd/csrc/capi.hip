@@ -62,8 +62,19 @@ hipError_t dalloc(T** p, size_t count) {
 
 }  // namespace
 
+// Scratch of dmt_trace_closest_noalloc: the traversal-stack spill of its
+// fixed grid (DMT_NOALLOC_BLOCKS workgroups of kBlock threads, each thread
+// DMT_NOALLOC_SPILL_DEPTH entries past its LDS stack), 24 MiB in every GPU's
+// copy of this code object, held until the process ends.  In host code the
+// name is a stand-in, not a device address: the device's own address comes
+// from hipGetSymbolAddress (dmt_device::noalloc_scratch).
+static_assert(DMT_NOALLOC_BLOCKS * 256u * DMT_NOALLOC_SPILL_DEPTH * 4u == 24u << 20, "scratch size");
+static __device__ __align__(256) uint32_t scratch[(size_t)DMT_NOALLOC_BLOCKS * kBlock * DMT_NOALLOC_SPILL_DEPTH];
+
 struct dmt_device {
     int id = 0;
+    uint32_t* noalloc_scratch = nullptr;   // this GPU's `scratch` (null: the lookup failed, noalloc_error says why)
+    std::string noalloc_error;
     hipStream_t stream = nullptr;   // exported stream; frame set 0, uploads, queries
     hipStream_t aux[kFrameSets - 1] = {};   // frame sets 1.. (frames in flight)
     bool lost = false;
@@ -399,6 +410,15 @@ dmt_status dmt_device_create(int32_t index, dmt_device** out) {
     if (e != hipSuccess) {
         device_free(d);
         return hip_fail(e, "hipStreamCreate");
+    }
+    // this GPU's copy of the allocation-free query's scratch (device `index`
+    // is current); only that entry point needs it, so a failed lookup is
+    // reported there
+    e = hipGetSymbolAddress((void**)&d->noalloc_scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        d->noalloc_scratch = nullptr;
+        d->noalloc_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        (void)hipGetLastError();
     }
     *out = d;
     return DMT_OK;
@@ -864,6 +884,32 @@ dmt_status dmt_trace_closest(dmt_scene* s, const float* rays, uint32_t n, dmt_hi
     (void)hipFree(d_rays);
     (void)hipFree(d_out);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "trace query");
+    return DMT_OK;
+}
+
+dmt_status dmt_trace_closest_noalloc(dmt_scene* s, const float* d_rays, uint32_t n, dmt_hit* d_out) {
+    static_assert(sizeof(dmt_hit) == sizeof(HitOut), "dmt_hit layout");
+    if (!s || (n && (!d_rays || !d_out))) return fail(DMT_ERR_INVALID_ARG, "null pointer");
+    if (((uintptr_t)d_rays & 15u) != 0u || ((uintptr_t)d_out & 3u) != 0u)
+        return fail(DMT_ERR_INVALID_ARG, "d_rays must be 16-byte and d_out 4-byte aligned");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if (!s->dev->noalloc_scratch) return fail(DMT_ERR_DEVICE, s->dev->noalloc_error);
+    // the deepest traversal stack of the scene's trees: 3 entries per level
+    // (trace_closest pushes at most three children per node), the first
+    // kLdsDepth in LDS
+    const uint32_t need = 3u * s->bvh_depth + 4u;
+    const uint32_t spill = need > kLdsDepth ? need - kLdsDepth : 0u;
+    if (spill > DMT_NOALLOC_SPILL_DEPTH)
+        return fail(DMT_ERR_LIMIT, "scene BVH depth " + std::to_string(s->bvh_depth) + " needs " +
+                                       std::to_string(spill) + " spilled stack entries per thread; the static scratch "
+                                       "holds " + std::to_string(DMT_NOALLOC_SPILL_DEPTH) + " (BVH depth <= " +
+                                       std::to_string((DMT_NOALLOC_SPILL_DEPTH + kLdsDepth - 4u) / 3u) + ")");
+    if (n == 0) return DMT_OK;
+    const hipError_t e = launch_trace_query_spill(s->view, (const float4*)d_rays, n, (HitOut*)d_out,
+                                                  s->dev->noalloc_scratch, DMT_NOALLOC_SPILL_DEPTH, DMT_NOALLOC_BLOCKS,
+                                                  s->dev->stream);
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "trace query (no allocation)");
     return DMT_OK;
 }
 

@@ -150,6 +150,11 @@ struct HitOut {
 };
 hipError_t launch_trace_query(const SceneView& S, const float4* rays, uint32_t n, HitOut* out, Wavefront& W,
                               hipStream_t stream);
+// The same with the traversal-stack spill in caller-given device memory: a
+// grid of at most max_blocks workgroups (grid-stride over the rays), so spill
+// must hold max_blocks * kBlock * spill_depth entries.
+hipError_t launch_trace_query_spill(const SceneView& S, const float4* rays, uint32_t n, HitOut* out, uint32_t* spill,
+                                    uint32_t spill_depth, uint32_t max_blocks, hipStream_t stream);
 
 // The denoiser inputs of a frame (PathTrace.slang:5463-5648): one closest-hit
 // query per pixel along the pixel-centre camera ray (no jitter), and per

@@ -709,9 +709,15 @@ hipError_t launch_gbuffer(const SceneView& S, const GbufferArgs& A, Wavefront& W
 
 hipError_t launch_trace_query(const SceneView& S, const float4* rays, uint32_t n, HitOut* out, Wavefront& W,
                               hipStream_t stream) {
+    return launch_trace_query_spill(S, rays, n, out, W.spill, W.spill_depth, kMaxGridBlocks, stream);
+}
+
+hipError_t launch_trace_query_spill(const SceneView& S, const float4* rays, uint32_t n, HitOut* out, uint32_t* spill,
+                                    uint32_t spill_depth, uint32_t max_blocks, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trace_query, dim3(grid_for(n)), dim3(kBlock), 0, stream, S, rays, n, out, W.spill,
-                       W.spill_depth);
+    uint32_t g = grid_for(n);
+    if (g > max_blocks) g = max_blocks;
+    hipLaunchKernelGGL(k_trace_query, dim3(g), dim3(kBlock), 0, stream, S, rays, n, out, spill, spill_depth);
     return hipGetLastError();
 }
 

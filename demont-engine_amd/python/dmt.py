@@ -28,6 +28,8 @@ ABI_VERSION = 4
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
 SCHEDULE_WAVEFRONT = 0xFFFFFFFF
 MATERIAL_FLOATS = 7
+DMT_NOALLOC_BLOCKS = 512        # dmt_trace_closest_noalloc's grid
+DMT_NOALLOC_SPILL_DEPTH = 48    # and its static per-thread stack spill
 
 PRIM_FLOATS = 28
 LIGHT_FLOATS = 20
@@ -191,6 +193,7 @@ def lib() -> C.CDLL:
         "dmt_render_gbuffer": (st, [vp, C.POINTER(FrameParams), C.POINTER(GbufferParams)]),
         "dmt_scene_update_mesh": (st, [vp, fp, u32]),
         "dmt_trace_closest": (st, [vp, fp, u32, vp]),
+        "dmt_trace_closest_noalloc": (st, [vp, vp, u32, vp]),
         "dmt_scene_stats": (st, [vp, C.POINTER(Stats)]),
         "dmt_scene_reset_stats": (st, [vp]),
         "dmt_scene_measure_traffic": (st, [vp, C.POINTER(FrameParams), C.POINTER(Stats)]),
@@ -225,6 +228,10 @@ def lib() -> C.CDLL:
         fn.argtypes = args
     _lib = L
     return L
+
+
+def last_error() -> str:
+    return lib().dmt_last_error().decode()
 
 
 def _check(status: int):
@@ -428,6 +435,11 @@ class Scene:
         out = np.zeros(n, HIT_DTYPE)
         _check(lib().dmt_trace_closest(self._h, _fptr(rays), n, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def trace_device_noalloc(self, d_rays: int, n: int, d_out: int) -> int:
+        """dmt_trace_closest_noalloc on device addresses (n rays of 8 floats
+        in, n dmt_hit out); stream-ordered, returns the status code."""
+        return lib().dmt_trace_closest_noalloc(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_out))
 
     def stats(self) -> dict:
         s = Stats()

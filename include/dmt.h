@@ -283,6 +283,24 @@ dmt_status dmt_render_gbuffer(dmt_scene* scene, const dmt_frame_params* frame, c
  * 8 floats per ray (ox, oy, oz, -, dx, dy, dz, -); host in, host out. */
 dmt_status dmt_trace_closest(dmt_scene* scene, const float* rays, uint32_t n, dmt_hit* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dmt_trace_closest on caller-owned device memory, with no
+ * hipMalloc, hipFree or other allocating call.  d_rays holds n rays of 8
+ * floats (ox, oy, oz, -, dx, dy, dz, -), 16-byte aligned; d_out receives n
+ * dmt_hit records, as dmt_trace_closest returns them.  Stream-ordered on the
+ * device stream (dmt_device_stream), no host synchronisation: d_rays must be
+ * complete when the call is made and d_out is valid after that stream's
+ * later work.  Its traversal-stack scratch is a static device array of each
+ * GPU (24 MiB, held until the process ends): the scene's trees must be at
+ * most (DMT_NOALLOC_SPILL_DEPTH + 12) / 3 = 20 levels deep, deeper ones are
+ * refused with DMT_ERR_LIMIT; any number of rays.  Calls on one GPU share
+ * that array, so calls that overlap on it (from several threads, or from two
+ * dmt_device handles of the same GPU, whose streams differ) race. */
+#define DMT_NOALLOC_BLOCKS       512u   /* workgroups of the query's grid (grid-stride over the rays) */
+#define DMT_NOALLOC_SPILL_DEPTH  48u    /* spilled stack entries per thread */
+dmt_status dmt_trace_closest_noalloc(dmt_scene* scene, const float* d_rays, uint32_t n, dmt_hit* d_out);
+
 /* Counters of the frames rendered since the last reset. */
 dmt_status dmt_scene_stats(dmt_scene* scene, dmt_stats* out);
 dmt_status dmt_scene_reset_stats(dmt_scene* scene);
