@@ -151,6 +151,7 @@ struct dmt_scene {
     std::vector<uint32_t> refit_level_start;   // offsets into refit_nodes, one past the end last
     float* positions_dev = nullptr;
     uint32_t* adj_spill = nullptr;    // traversal spill area of the adjoint kernel
+    float4* moon = nullptr;           // moon texture (dmt_scene_set_moon_map)
 };
 
 namespace {
@@ -247,6 +248,29 @@ dmt_status ensure_accum(dmt_scene* s, uint32_t w, uint32_t h, bool* resized) {
 }
 
 // The accumulation buffer and the wavefront sets at w x h.
+// The shadow queues of the sky-light records (sun or HDRI cluster, moon) in
+// every frame set, allocated when a frame first has sky lights.
+dmt_status ensure_sky_queues(dmt_scene* s) {
+    if (s->wf[0].sq[2].o) return DMT_OK;
+    const size_t n = s->wf[0].capacity;
+    for (int set = 0; set < (int)kFrameSets; ++set) {
+        for (int k = 2; k < 4; ++k) {
+            ShadowQueue& q = s->wf[set].sq[k];
+            void** parts[] = {(void**)&q.o, (void**)&q.d, (void**)&q.a, (void**)&q.b, (void**)&q.w};
+            for (int i = 0; i < 5; ++i) {
+                hipError_t e = hipMalloc(parts[i], i < 4 ? n * 16 : n * 4);
+                if (e != hipSuccess) {
+                    *parts[i] = nullptr;
+                    free_sets(s);
+                    return hip_fail(e, "hipMalloc(sky-light shadow queues)");
+                }
+                s->wf_buffers.push_back(*parts[i]);
+            }
+        }
+    }
+    return DMT_OK;
+}
+
 dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized) {
     dmt_status st = ensure_accum(s, w, h, resized);
     if (st) return st;
@@ -307,7 +331,21 @@ dmt_status check_frame(const dmt_frame_params* f) {
     if (f->spp < 1 || f->spp > DMT_MAX_SPP) return fail(DMT_ERR_INVALID_ARG, "spp must be in 1..32");
     if (f->max_bounces > DMT_MAX_BOUNCES) return fail(DMT_ERR_LIMIT, "max_bounces must be <= 64");
     if (f->refract_bounces > 64) return fail(DMT_ERR_LIMIT, "refract_bounces must be <= 64");
-    if (f->flags & ~DMT_FRAME_NO_RUSSIAN_ROULETTE) return fail(DMT_ERR_INVALID_ARG, "unknown frame flags");
+    if (f->flags & ~(DMT_FRAME_NO_RUSSIAN_ROULETTE | DMT_FRAME_FIXED_JITTER))
+        return fail(DMT_ERR_INVALID_ARG, "unknown frame flags");
+    if ((f->flags & DMT_FRAME_FIXED_JITTER) &&
+        !(std::fabs(f->jitter[0]) <= 0.5f && std::fabs(f->jitter[1]) <= 0.5f))
+        return fail(DMT_ERR_INVALID_ARG, "fixed jitter must be in [-0.5, 0.5]");
+    const float* sky_f[] = {f->sun, f->sun_extra, f->moon, f->moon_extra};
+    for (const float* v : sky_f)
+        for (int a = 0; a < 4; ++a)
+            if (!std::isfinite(v[a])) return fail(DMT_ERR_INVALID_ARG, "sun and moon parameters must be finite");
+    if (!std::isfinite(f->sun_disc_brightness)) return fail(DMT_ERR_INVALID_ARG, "sun_disc_brightness must be finite");
+    if (f->hdri_light_count > kMaxHdriLights) return fail(DMT_ERR_LIMIT, "at most 8 HDRI lights");
+    for (uint32_t i = 0; i < f->hdri_light_count; ++i)
+        for (int a = 0; a < 4; ++a)
+            if (!std::isfinite(f->hdri_dir[i][a]) || !std::isfinite(f->hdri_col[i][a]))
+                return fail(DMT_ERR_INVALID_ARG, "HDRI light values must be finite");
     const float lens[6] = {f->dof[0], f->dof[1], f->dof[2], f->dof[3], f->cat_eye, f->cauchy_c};
     for (float v : lens)
         if (!std::isfinite(v)) return fail(DMT_ERR_INVALID_ARG, "lens and spectral parameters must be finite");
@@ -330,14 +368,47 @@ FrameArgs frame_args(const dmt_frame_params* f) {
     return F;
 }
 
-// A frame that draws more than the jitter per sample (lens point, shutter
-// time, hero wavelength) or translates the mesh renders through k_tail<COUNT, true>.
-bool uses_features(const dmt_frame_params* f) {
-    return f->dof[0] > 0.0f || f->motion_blur || f->spectral || f->mesh_shift_active;
+// Sky lights that change the frame: with an env map its HDRI cluster lights;
+// without one a sun or moon whose disc can show (direction set, y >= -0.1:
+// sunDisc / moonDisc are 0 below, and a zero direction meets no ray).
+bool sky_lights(const dmt_scene* s, const dmt_frame_params* f) {
+    if (s->view.env_present) return f->hdri_light_count > 0u;
+    auto up = [](const float* d) { return (d[0] != 0.0f || d[1] != 0.0f || d[2] != 0.0f) && !(d[1] < -0.10f); };
+    return up(f->sun) || up(f->moon);
 }
 
-FeatureArgs feature_args(const dmt_frame_params* f) {
+// A frame that draws other numbers than the jitter per sample (lens point,
+// shutter time, hero wavelength), takes a fixed jitter, translates the mesh
+// or has sky lights renders through k_tail<COUNT, true>.
+bool uses_features(const dmt_scene* s, const dmt_frame_params* f) {
+    return f->dof[0] > 0.0f || f->motion_blur || f->spectral || f->mesh_shift_active ||
+           (f->flags & DMT_FRAME_FIXED_JITTER) || sky_lights(s, f);
+}
+
+FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
     FeatureArgs X{};
+    X.fixed_jitter = (f->flags & DMT_FRAME_FIXED_JITTER) ? 1u : 0u;
+    X.jit_x = f->jitter[0];
+    X.jit_y = f->jitter[1];
+    if (sky_lights(s, f)) {
+        X.sky = 1u;
+        SkyArgs& K = X.sky_args;
+        K.sun = f3{f->sun[0], f->sun[1], f->sun[2]};
+        K.sun_size = f->sun_extra[0];
+        K.sun_dist = f->sun_extra[1];
+        K.sun_vflat = f->sun_extra[2];
+        K.sun_bright = f->sun_disc_brightness;
+        K.moon = f3{f->moon[0], f->moon[1], f->moon[2]};
+        K.moon_phase = f->moon[3];
+        K.moon_size = f->moon_extra[0];
+        K.moon_dist = f->moon_extra[1];
+        K.moon_bright = f->moon_extra[2];
+        K.hdri_count = s->view.env_present ? f->hdri_light_count : 0u;
+        for (uint32_t i = 0; i < kMaxHdriLights; ++i) {
+            K.hdri_dir[i] = make_float4(f->hdri_dir[i][0], f->hdri_dir[i][1], f->hdri_dir[i][2], f->hdri_dir[i][3]);
+            K.hdri_col[i] = make_float4(f->hdri_col[i][0], f->hdri_col[i][1], f->hdri_col[i][2], f->hdri_col[i][3]);
+        }
+    }
     X.aperture = f->dof[0];
     X.focal = f->dof[1];
     // int(dof_params.z + 0.5) (PathTrace.slang:5789); below 3 every count is
@@ -461,10 +532,14 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
     PreparedScene P;
     std::string msg;
     try {
+        EnvCdfInput cdf;
+        cdf.marginal = desc->env_marginal;
+        cdf.conditional = desc->env_conditional;
+        cdf.total = desc->env_total_luminance;
         msg = prepare_scene(desc->prims, desc->prim_count, desc->lights, desc->light_count, desc->positions,
                             desc->vertex_count, desc->indices, desc->index_count, desc->env_rgb, desc->env_width,
                             desc->env_height, desc->env_intensity, bp ? bp->sah_bins : 0u,
-                            bp ? bp->max_leaf : 0u, P, !device_build);
+                            bp ? bp->max_leaf : 0u, P, !device_build, &cdf);
     } catch (const std::bad_alloc&) {
         return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation during scene build");
     }
@@ -636,6 +711,7 @@ void scene_free(dmt_scene* s) {
             if (e) (void)hipEventDestroy(e);
     if (s->host_count) (void)hipHostFree(s->host_count);
     if (s->adj_spill) (void)hipFree(s->adj_spill);
+    if (s->moon) (void)hipFree(s->moon);
     for (void* p : s->buffers) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     dmt_device* d = s->dev;
@@ -669,6 +745,7 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
     if (frame_count == 0) return fail(DMT_ERR_INVALID_ARG, "frame_count must be >= 1");
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
+    if (sky_lights(s, f) && (st = ensure_sky_queues(s))) return st;
     if (f->max_bounces == 0) return DMT_OK;
     for (uint32_t k = 0; k < frame_count; ++k) {
         FrameArgs F = frame_args(f);
@@ -695,8 +772,8 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
         o.tail_after = s->tail_after;
         o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
         o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
-        const FeatureArgs X = feature_args(f);
-        if (uses_features(f)) o.features = &X;
+        const FeatureArgs X = feature_args(s, f);
+        if (uses_features(s, f)) o.features = &X;
         hipError_t e = launch_frame(frame_view(s, f), F, s->wf[set], s->accum, stream, o);
         s->ev_used += o.events_used;
         if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
@@ -887,6 +964,38 @@ dmt_status dmt_trace_closest(dmt_scene* s, const float* rays, uint32_t n, dmt_hi
     return DMT_OK;
 }
 
+dmt_status dmt_scene_set_moon_map(dmt_scene* s, const float* rgba, uint32_t w, uint32_t h) {
+    if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if (s->batches > 0) return fail(DMT_ERR_INVALID_ARG, "the scene belongs to a batch");
+    if (rgba) {
+        if (w == 0 || h == 0) return fail(DMT_ERR_INVALID_ARG, "moon texture width and height must be >= 1");
+        if ((uint64_t)w * h > (1ull << 24)) return fail(DMT_ERR_LIMIT, "moon texture larger than 2^24 texels");
+        for (size_t i = 0; i < (size_t)w * h * 4u; ++i)
+            if (!std::isfinite(rgba[i])) return fail(DMT_ERR_INVALID_ARG, "moon texture has a non-finite value");
+    }
+    hipError_t e = sync_all(s->dev);   // frames in flight read the old texture
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize");
+    if (s->moon) (void)hipFree(s->moon);
+    s->moon = nullptr;
+    s->view.moon = nullptr;
+    s->view.moon_w = s->view.moon_h = 0u;
+    if (!rgba) return DMT_OK;
+    const size_t bytes = (size_t)w * h * 16u;
+    e = hipMalloc((void**)&s->moon, bytes);
+    if (e == hipSuccess) e = hipMemcpy(s->moon, rgba, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (s->moon) (void)hipFree(s->moon);
+        s->moon = nullptr;
+        return hip_fail(e, "moon texture upload");
+    }
+    s->view.moon = s->moon;
+    s->view.moon_w = w;
+    s->view.moon_h = h;
+    return DMT_OK;
+}
+
 dmt_status dmt_trace_closest_noalloc(dmt_scene* s, const float* d_rays, uint32_t n, dmt_hit* d_out) {
     static_assert(sizeof(dmt_hit) == sizeof(HitOut), "dmt_hit layout");
     if (!s || (n && (!d_rays || !d_out))) return fail(DMT_ERR_INVALID_ARG, "null pointer");
@@ -982,6 +1091,7 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     if ((st = check_frame(f))) return st;
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
+    if (sky_lights(s, f) && (st = ensure_sky_queues(s))) return st;
     // Render into a scratch accumulation buffer so the scene's image and
     // counters are left as they were.
     float4* scratch = nullptr;
@@ -999,8 +1109,8 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     o.tail_after = s->tail_after;
     o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
     o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
-    const FeatureArgs X = feature_args(f);
-    if (uses_features(f)) o.features = &X;
+    const FeatureArgs X = feature_args(s, f);
+    if (uses_features(s, f)) o.features = &X;
     if (e == hipSuccess) e = launch_frame(frame_view(s, f), F, W, scratch, s->dev->stream, o);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);
@@ -1077,6 +1187,8 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
     if ((st = check_frame(f))) return st;
     if (f->max_bounces > kAdjMaxVerts) return fail(DMT_ERR_LIMIT, "the adjoint supports max_bounces <= 8");
     if (f->refract_bounces > kAdjMaxDielectric) return fail(DMT_ERR_LIMIT, "the adjoint supports refract_bounces <= 8");
+    if (sky_lights(s, f))
+        return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate sky lights (sun, moon, HDRI clusters)");
     const uint32_t prims = s->view.prim_count;
     if (prims && (!d_image_grad || !d_param_grad)) return fail(DMT_ERR_INVALID_ARG, "null device pointer");
     if (prims == 0) return DMT_OK;
@@ -1092,8 +1204,8 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
     AdjointArgs A{};
     A.S = frame_view(s, f);
     A.F = frame_args(f);
-    A.X = feature_args(f);
-    A.features = uses_features(f) ? 1u : 0u;
+    A.X = feature_args(s, f);
+    A.features = uses_features(s, f) ? 1u : 0u;
     A.dimage = (const float4*)d_image_grad;
     A.grads = (float*)d_param_grad;
     A.prim_count = prims;
@@ -1175,7 +1287,7 @@ std::vector<BatchScene> batch_table(const dmt_batch* b, uint32_t tile_w, const s
         std::memset(&e, 0, sizeof(e));
         e.S = frame_view(b->scenes[i], &f);
         e.F = frame_args(&f);
-        e.X = feature_args(&f);
+        e.X = feature_args(b->scenes[i], &f);
         e.accum = accum ? (*accum)[i] : b->scenes[i]->accum;
         e.tiles_x = (f.width + tile_w - 1u) / tile_w;
         const uint32_t tiles_y = (f.height + tile_h - 1u) / tile_h;
@@ -1204,9 +1316,12 @@ BatchArgs batch_args(const dmt_batch* b, const dmt_batch_params& p, const BatchS
 }
 
 BatchLaunch batch_launch(const dmt_batch* b, const dmt_batch_params& p) {
-    bool feat = false;
-    for (const dmt_frame_params& f : b->frames) feat = feat || uses_features(&f);
-    return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds, feat};
+    bool feat = false, sky = false;
+    for (size_t i = 0; i < b->frames.size(); ++i) {
+        feat = feat || uses_features(b->scenes[i], &b->frames[i]);
+        sky = sky || sky_lights(b->scenes[i], &b->frames[i]);
+    }
+    return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds, feat, sky};
 }
 
 void drop_graph(dmt_batch* b) {
@@ -1335,12 +1450,15 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         b->order.insert(b->order.end(), queue[q].begin(), queue[q].end());
     }
     b->queue_begin[kXcds] = (uint32_t)b->order.size();
+    // records of the sky lights (sun or HDRI cluster, moon) when a frame has them
+    bool sky = false;
+    for (uint32_t i = 0; i < count; ++i) sky = sky || sky_lights(list[i], &frames[i]);
     const std::pair<void**, size_t> req[] = {
         {(void**)&b->d_scenes, count * sizeof(BatchScene)},
         {(void**)&b->d_order, count * sizeof(uint32_t)},
         {(void**)&b->d_work, count * sizeof(uint32_t)},
         {(void**)&b->d_frame, sizeof(uint32_t)},
-        {(void**)&b->d_rec, (size_t)2u * kRecRows * b->max_threads * sizeof(float4)},
+        {(void**)&b->d_rec, (size_t)(sky ? 4u : 2u) * kRecRows * b->max_threads * sizeof(float4)},
         {(void**)&b->d_spill, (size_t)b->max_threads * b->spill_depth * sizeof(uint32_t)},
         {(void**)&b->d_stats, sizeof(DeviceStats)},
     };

@@ -11,6 +11,9 @@
 //   same leaf summaries, cone union and median split, so the kernels draw
 //   the same pick probabilities.
 // * build_env_cdf: the reference env-map CDF (src/renderer/HdrImage.cpp:169-219).
+// * extract_env_lights: the engine's HDRI light extraction at env-map load
+//   (src/engine/Engine.cpp:4369-4532): bright-pixel threshold, connected
+//   components, the brightest clusters as directional lights.
 #include "prep.h"
 
 #include <algorithm>
@@ -463,7 +466,7 @@ std::vector<float> build_light_tree(const float* lights, uint32_t n) {
 }
 
 double build_env_cdf(const float* rgb, uint32_t W, uint32_t H, std::vector<float>& marginal,
-                     std::vector<float>& conditional) {
+                     std::vector<float>& conditional, const uint8_t* mask) {
     marginal.assign(H, 0.0f);
     conditional.assign((size_t)W * H, 0.0f);
     double total = 0.0;
@@ -473,7 +476,8 @@ double build_env_cdf(const float* rgb, uint32_t W, uint32_t H, std::vector<float
         float* row = &conditional[(size_t)v * W];
         const float* px = rgb + (size_t)v * W * 3;
         for (uint32_t u = 0; u < W; ++u) {
-            const float lum = 0.2126f * px[3 * u] + 0.7152f * px[3 * u + 1] + 0.0722f * px[3 * u + 2];
+            float lum = 0.2126f * px[3 * u] + 0.7152f * px[3 * u + 1] + 0.0722f * px[3 * u + 2];
+            if (mask && mask[(size_t)v * W + u]) lum = 0.0f;   // an extracted cluster's pixel
             run += double(lum) * s;
             row[u] = float(run);
         }
@@ -490,6 +494,121 @@ double build_env_cdf(const float* rgb, uint32_t W, uint32_t H, std::vector<float
     }
     if (H > 0 && total > 0.0) marginal[H - 1] = 1.0f;
     return total;
+}
+
+uint32_t extract_env_lights(const float* rgb, uint32_t W, uint32_t H, double top_frac, float dir[][4],
+                            float col[][4], uint8_t* mask) {
+    const size_t N = (size_t)W * H;
+    std::fill(mask, mask + N, uint8_t(0));
+    if (N == 0) return 0u;
+    // bright pixels: luminance at or above the top-fraction quantile, floored
+    // at 5 % of the peak and at 0.5; nothing on an image whose peak is <= 0.5
+    std::vector<float> lum(N);
+    float peak = 0.0f;
+    for (size_t p = 0; p < N; ++p) {
+        lum[p] = 0.2126f * rgb[3 * p] + 0.7152f * rgb[3 * p + 1] + 0.0722f * rgb[3 * p + 2];
+        peak = std::max(peak, lum[p]);
+    }
+    if (!(peak > 0.5f)) return 0u;
+    top_frac = std::min(std::max(top_frac, 1e-6), 0.5);
+    const size_t k = N - std::max<size_t>(1, size_t(double(N) * top_frac));
+    std::vector<float> order_stat = lum;
+    std::nth_element(order_stat.begin(), order_stat.begin() + (std::ptrdiff_t)k, order_stat.end());
+    float thresh = order_stat[k];
+    if (thresh < 0.05f * peak) thresh = 0.05f * peak;
+    if (thresh < 0.5f) thresh = 0.5f;
+    for (size_t p = 0; p < N; ++p) mask[p] = lum[p] >= thresh ? 1u : 0u;
+
+    // 4-connected components of the bright pixels (columns wrap around, rows
+    // do not), each walked depth-first from its first pixel in row-major
+    // order; per component the luminance-weighted direction sum, the flux and
+    // the luminance integral over the solid angle
+    struct Cluster {
+        double dir[3] = {0.0, 0.0, 0.0};
+        double wsum = 0.0;
+        double flux[3] = {0.0, 0.0, 0.0};
+        double lum = 0.0;
+        uint32_t pixels = 0;
+    };
+    const double kPi = 3.14159265358979323846;
+    const double d_omega = (2.0 * kPi * kPi) / double(N);
+    std::vector<Cluster> cl;
+    std::vector<uint32_t> id(N, 0u);   // 1-based component, 0 = none yet
+    std::vector<std::pair<uint32_t, uint32_t>> todo;
+    for (uint32_t v0 = 0; v0 < H; ++v0) {
+        for (uint32_t u0 = 0; u0 < W; ++u0) {
+            const size_t p0 = (size_t)v0 * W + u0;
+            if (!mask[p0] || id[p0] != 0u) continue;
+            cl.emplace_back();
+            const uint32_t cid = (uint32_t)cl.size();
+            Cluster& c = cl.back();
+            id[p0] = cid;
+            todo.assign(1, {u0, v0});
+            while (!todo.empty()) {
+                const auto [u, v] = todo.back();
+                todo.pop_back();
+                const size_t p = (size_t)v * W + u;
+                const double theta = kPi * (double(v) + 0.5) / double(H);
+                const double st = std::sin(theta), ct = std::cos(theta);
+                const double dw = d_omega * st;
+                const double phi = 2.0 * kPi * (double(u) + 0.5) / double(W) - kPi;
+                const float r = rgb[3 * p], g = rgb[3 * p + 1], b = rgb[3 * p + 2];
+                const double l = 0.2126 * r + 0.7152 * g + 0.0722 * b;
+                const double wl = l * dw;
+                c.dir[0] += (st * std::cos(phi)) * wl;
+                c.dir[1] += ct * wl;
+                c.dir[2] += (st * std::sin(phi)) * wl;
+                c.wsum += wl;
+                c.flux[0] += double(r) * dw;
+                c.flux[1] += double(g) * dw;
+                c.flux[2] += double(b) * dw;
+                c.lum += l * dw;
+                c.pixels += 1u;
+                const uint32_t nb[4][2] = {{u == 0u ? W - 1u : u - 1u, v}, {u + 1u >= W ? 0u : u + 1u, v},
+                                           {u, v - 1u}, {u, v + 1u}};
+                for (int k2 = 0; k2 < 4; ++k2) {
+                    if ((k2 == 2 && v == 0u) || (k2 == 3 && v + 1u >= H)) continue;
+                    const size_t q = (size_t)nb[k2][1] * W + nb[k2][0];
+                    if (mask[q] && id[q] == 0u) {
+                        id[q] = cid;
+                        todo.push_back({nb[k2][0], nb[k2][1]});
+                    }
+                }
+            }
+        }
+    }
+    // the brightest (luminance integral) kMaxHdriLights components of at
+    // least 3 pixels become lights, picked in proportion to that integral;
+    // the other components go back into the importance CDF
+    std::vector<uint32_t> rank(cl.size());
+    std::iota(rank.begin(), rank.end(), 0u);
+    std::sort(rank.begin(), rank.end(), [&](uint32_t a, uint32_t b) { return cl[a].lum > cl[b].lum; });
+    std::vector<uint8_t> keep(cl.size() + 1u, 0u);
+    uint32_t n = 0;
+    float lum_f[kMaxEnvLights] = {};
+    double total = 0.0;
+    for (uint32_t i : rank) {
+        if (n >= kMaxEnvLights) break;
+        const Cluster& c = cl[i];
+        if (c.pixels < 3u || !(c.wsum > 0.0)) continue;
+        keep[i + 1u] = 1u;
+        const double inv = 1.0 / std::sqrt((c.dir[0] * c.dir[0] + c.dir[1] * c.dir[1]) + c.dir[2] * c.dir[2]);
+        for (int a = 0; a < 3; ++a) {
+            dir[n][a] = float(c.dir[a] * inv);
+            col[n][a] = float(c.flux[a]);
+        }
+        col[n][3] = 0.0f;
+        lum_f[n] = float(c.lum);
+        total += c.lum;
+        n++;
+    }
+    if (total > 0.0) {
+        const float inv_total = float(1.0 / total);
+        for (uint32_t i = 0; i < n; ++i) dir[i][3] = lum_f[i] * inv_total;
+    }
+    for (size_t p = 0; p < N; ++p)
+        if (id[p] != 0u && !keep[id[p]]) mask[p] = 0u;
+    return n;
 }
 
 }  // namespace dmt

@@ -49,4 +49,22 @@ double dmt_build_env_cdf(const float* rgb, uint32_t W, uint32_t H, float* om, fl
     return total;
 }
 
+double dmt_build_env_cdf_masked(const float* rgb, uint32_t W, uint32_t H, const uint8_t* mask, float* om,
+                                float* oc) {
+    if (!rgb || !om || !oc) return 0.0;
+    std::vector<float> m, c;
+    double total = dmt::build_env_cdf(rgb, W, H, m, c, mask);
+    std::memcpy(om, m.data(), m.size() * sizeof(float));
+    std::memcpy(oc, c.data(), c.size() * sizeof(float));
+    return total;
+}
+
+uint32_t dmt_env_extract_lights(const float* rgb, uint32_t W, uint32_t H, double top_frac, float dir[8][4],
+                                float col[8][4], uint8_t* out_mask) {
+    if (!rgb || !dir || !col || !out_mask) return 0u;
+    for (int i = 0; i < 8; ++i)
+        for (int a = 0; a < 4; ++a) dir[i][a] = col[i][a] = 0.0f;
+    return dmt::extract_env_lights(rgb, W, H, top_frac, dir, col, out_mask);
+}
+
 }  // extern "C"

@@ -40,7 +40,7 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
                           const float* positions, uint32_t vertex_count, const uint32_t* indices,
                           uint32_t index_count, const float* env_rgb, uint32_t env_w, uint32_t env_h,
                           float env_intensity, uint32_t bins, uint32_t max_leaf, PreparedScene& S,
-                          bool host_tri_bvh) {
+                          bool host_tri_bvh, const EnvCdfInput* env_cdf) {
     if (prim_count && !prims) return "prims is null";
     if (light_count && !lights) return "lights is null";
     if (index_count % 3u) return "index_count must be a multiple of 3";
@@ -178,7 +178,19 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
             S.env_rgba[4 * p + 2] = env_rgb[3 * p + 2];
             S.env_rgba[4 * p + 3] = 1.0f;
         }
-        S.env_total = float(build_env_cdf(env_rgb, env_w, env_h, S.env_marginal, S.env_conditional));
+        if (env_cdf && (env_cdf->marginal || env_cdf->conditional)) {
+            if (!env_cdf->marginal || !env_cdf->conditional) return "env CDF: both marginal and conditional are needed";
+            if (!finite_all(env_cdf->marginal, env_h) || !finite_all(env_cdf->conditional, (size_t)env_w * env_h))
+                return "env CDF has a non-finite value";
+            if (!(std::isfinite(env_cdf->total) && env_cdf->total >= 0.0)) return "env CDF total must be finite and >= 0";
+            S.env_marginal.assign(env_cdf->marginal, env_cdf->marginal + env_h);
+            S.env_conditional.assign(env_cdf->conditional, env_cdf->conditional + (size_t)env_w * env_h);
+            S.env_total = float(env_cdf->total);
+        } else {
+            S.env_total = float(build_env_cdf(env_rgb, env_w, env_h, S.env_marginal, S.env_conditional));
+        }
+    } else if (env_cdf && (env_cdf->marginal || env_cdf->conditional)) {
+        return "env CDF given without an env map";
     }
     return "";
 }

@@ -55,8 +55,25 @@ Bvh4 build_bvh4(const std::vector<BoundsItem>& items, uint32_t bins, uint32_t ma
 std::vector<float> build_light_tree(const float* lights, uint32_t light_count);
 
 // HdrImage.cpp BuildEnvCdf.  Returns the sin-weighted luminance total.
+// mask (W*H, optional): pixels whose luminance the CDF leaves out.
 double build_env_cdf(const float* rgb, uint32_t W, uint32_t H, std::vector<float>& marginal,
-                     std::vector<float>& conditional);
+                     std::vector<float>& conditional, const uint8_t* mask = nullptr);
+
+constexpr uint32_t kMaxEnvLights = 8;   // Engine::kMaxHdriLights
+// The engine's HDRI light extraction (Engine.cpp:4369-4532): returns the
+// number of lights (<= kMaxEnvLights) with their direction and pick
+// probability (dir[i][3]) and rgb flux (col), and marks their pixels in mask
+// (W*H) for build_env_cdf.  top_frac: r_hdri_extract_percentile (0.005).
+uint32_t extract_env_lights(const float* rgb, uint32_t W, uint32_t H, double top_frac, float dir[][4],
+                            float col[][4], uint8_t* mask);
+
+// An importance CDF given by the caller instead of the one built from the
+// env map (the engine's uploads).
+struct EnvCdfInput {
+    const float* marginal = nullptr;      // H
+    const float* conditional = nullptr;   // W*H
+    double total = 0.0;
+};
 
 // Everything the kernels read, host copies in device layout.
 struct PreparedScene {
@@ -93,6 +110,6 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
                           const float* positions, uint32_t vertex_count, const uint32_t* indices,
                           uint32_t index_count, const float* env_rgb, uint32_t env_w, uint32_t env_h,
                           float env_intensity, uint32_t bins, uint32_t max_leaf, PreparedScene& out,
-                          bool host_tri_bvh = true);
+                          bool host_tri_bvh = true, const EnvCdfInput* env_cdf = nullptr);
 
 }  // namespace dmt

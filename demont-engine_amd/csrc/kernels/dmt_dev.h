@@ -181,6 +181,50 @@ __device__ __forceinline__ float c_exp(float xin) {
     return p * __uint_as_float((uint32_t)(n + 127) << 23);
 }
 
+// Cephes logf: x = m 2^e with m in [sqrt(1/2), sqrt(2)), the degree-9
+// minimax polynomial of log(1 + f), ln 2 in two parts.  x <= 0 gives -inf.
+__device__ __forceinline__ float c_log(float xin) {
+    if (!(xin > 0.0f)) return -__int_as_float(0x7f800000);
+    if (isinf(xin)) return xin;
+    uint32_t bits = __float_as_uint(xin);
+    int e = 0;
+    if (bits < 0x00800000u) {   // denormal: scale into the normal range
+        xin = xin * 8388608.0f;
+        bits = __float_as_uint(xin);
+        e = -23;
+    }
+    e += (int)((bits >> 23) & 0xFFu) - 126;
+    float x = __uint_as_float((bits & 0x007FFFFFu) | 0x3F000000u);   // [0.5, 1)
+    if (x < 0.707106781186547524f) {
+        e -= 1;
+        x = (x + x) - 1.0f;
+    } else {
+        x = x - 1.0f;
+    }
+    const float z = x * x;
+    float y = ((((((((7.0376836292e-2f * x - 1.1514610310e-1f) * x + 1.1676998740e-1f) * x - 1.2420140846e-1f) * x +
+                   1.4249322787e-1f) * x - 1.6668057665e-1f) * x + 2.0000714765e-1f) * x - 2.4999993993e-1f) * x +
+               3.3333331174e-1f) * x * z;
+    const float fe = (float)e;
+    y = y + -2.12194440e-4f * fe;
+    y = y + -0.5f * z;
+    float r = x + y;
+    r = r + 0.693359375f * fe;
+    return r;
+}
+// pow(x, y) for x >= 0 as exp(y log x) with the Cephes functions (0 at x = 0
+// for y > 0).
+__device__ __forceinline__ float c_pow(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : __int_as_float(0x7f800000));
+    return c_exp(y * c_log(x));
+}
+__device__ __forceinline__ float saturate(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+// smoothstep(e0, e1, x) = t^2 (3 - 2t), t = saturate((x - e0) / (e1 - e0)).
+__device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
+    const float t = saturate((x - e0) / (e1 - e0));
+    return (t * t) * (3.0f - 2.0f * t);
+}
+
 // ---------------------------------------------------------------------------
 // RNG (PathTraceMath.slang:16-24)
 
@@ -227,6 +271,8 @@ struct SceneView {
     uint32_t mesh_shift;          // the mesh is translated (set per frame)
     const float4* sph_prev;       // (previous c.xyz, r), leaf order
     f3 mesh_prev, mesh_curr;      // its translation at shutter times 0 and 1
+    const float4* moon;           // moon texture (RGBA, row-major), null: none
+    uint32_t moon_w, moon_h;
 };
 
 // Traversal stack: `lds_depth` entries in LDS (thread-interleaved), the rest

@@ -116,12 +116,20 @@ __device__ __forceinline__ void cat_eye(float& x, float& y, float ux, float uy, 
 // (PathTrace.slang:5729-5816): the jitter, then, each only when its feature
 // is on, the hero wavelength (stratified over the spp samples), the shutter
 // time and the lens point.  Without them: camera_ray, time 1, lambda -1.
+// With fixed_jitter (the denoiser's Halton offset, :5730-5731) sample 0
+// takes (jit_x, jit_y) and draws nothing for it.
 __device__ __forceinline__ void camera_sample(const FrameArgs& F, const FeatureArgs& X, uint32_t x, uint32_t y,
                                               uint32_t s, uint32_t& seed, f3& o, f3& d, float& time, float& lambda) {
-    float jx = randf(seed);
-    float jy = randf(seed);
-    jx = jx - 0.5f;
-    jy = jy - 0.5f;
+    float jx, jy;
+    if (X.fixed_jitter && s == 0u) {
+        jx = X.jit_x;
+        jy = X.jit_y;
+    } else {
+        jx = randf(seed);
+        jy = randf(seed);
+        jx = jx - 0.5f;
+        jy = jy - 0.5f;
+    }
     const float ux = ((((float)x + 0.5f) + jx) / (float)F.width) * 2.0f - 1.0f;
     float uy = ((((float)y + 0.5f) + jy) / (float)F.height) * 2.0f - 1.0f;
     uy = -uy;
@@ -148,8 +156,11 @@ __device__ __forceinline__ void camera_sample(const FrameArgs& F, const FeatureA
     }
 }
 
-// A shadow query and the factors of its contribution.  kind 0 = light NEE,
-// 1 = env NEE; pdf < 0 marks the no-env ambient term (added without tracing).
+// A shadow query and the factors of its contribution.  kind 0 = light NEE;
+// 1 = env NEE, or without an env map the ambient term (pdf < 0, added
+// without tracing); 2 = the HDRI cluster NEE (env map) or the sun NEE (no env
+// map); 3 = the moon NEE.  A vertex's terms are summed in the order
+// 0, 1, 2 with an env map and 0, 2, 3, 1 without one (shade_kind_order).
 template <class T> struct ShadowRecT {
     V3<T> o, d;
     T t_max;
@@ -195,28 +206,53 @@ struct NoGradientHooks {
     template <class T> __device__ __forceinline__ void on_surface(uint32_t b, SurfaceT<T>& sf) {}
 };
 
+// The k-th kind whose shadow record a vertex's terms are summed in
+// (k = 0 .. 3; kinds a vertex did not emit are skipped).
+__device__ __forceinline__ uint32_t shade_kind_order(bool env, uint32_t k) {
+    return env ? k : ((0x1320u >> (4u * k)) & 15u);   // no env map: 0, 2, 3, 1
+}
+
+// The sky of a missed ray: skyColor, with the sun and moon discs when SKY.
+template <bool SKY, class T>
+__device__ __forceinline__ V3<T> miss_sky(const SceneView& S, const SkyArgs* K, V3<T> d) {
+    if constexpr (SKY) {
+        if (!S.env_present) return gradient_sky(S, *K, d);
+    }
+    return sky_color(S, d);
+}
+
 // Shades the vertex at bounce b for the ray (o, d) with closest hit h.
-// Shadow queries go to sink.emit(kind, record) as soon as they are formed
-// (light NEE before env NEE, the order their contributions are summed).
-template <class T, class Sink>
+// Shadow queries go to sink.emit(kind, record) as soon as they are formed,
+// in the order their contributions are summed.  SKY (K non-null): the
+// frame's sky lights -- the sun and moon NEE of the no-env Lambert branch
+// with their discs in the sky (PathTrace.slang:6546-6667), the HDRI cluster
+// NEE of the env branch and the cluster test of the MIS miss (:6493-6518,
+// :6074); SKY frames are float only.
+template <bool SKY = false, class T, class Sink>
 __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs& F, uint32_t b, V3<T> o, V3<T> d,
-                                             HitRec h, PathVertexStateT<T>& P, VertexResultT<T>& R, Sink& sink) {
+                                             HitRec h, PathVertexStateT<T>& P, VertexResultT<T>& R, Sink& sink,
+                                             const SkyArgs* K = nullptr) {
     const bool mis_on = F.mis_enabled != 0u;
     R.alive = false;
     R.no = to<T>(f3{0.0f, 0.0f, 0.0f});
     R.nd = to<T>(f3{0.0f, 1.0f, 0.0f});
     if (h.code == kMiss) {
         if (mis_on && P.prev_lambert) {
-            // transmittance(ro, rd, 1e30) would re-trace this missed ray and
-            // return exactly 1 (PathTrace.slang:6081-6084).
-            T pe = pdf_env_of(S, d);
-            T w = mis_balance(P.prev_pdf, pe);
-            const V3<T> v = (P.T_ * sky_color(S, d)) * w;
-            const V3<T> c = clamp_indirect(F, v);
-            P.L = P.L + c;
-            sink.on_radiance(v, c, true);
+            // cluster directions are left to the clusters' own NEE (:6074)
+            bool cluster = false;
+            if constexpr (SKY) cluster = is_cluster_dir(S, *K, d);
+            if (!cluster) {
+                // transmittance(ro, rd, 1e30) would re-trace this missed ray and
+                // return exactly 1 (PathTrace.slang:6081-6084).
+                T pe = pdf_env_of(S, d);
+                T w = mis_balance(P.prev_pdf, pe);
+                const V3<T> v = (P.T_ * sky_color(S, d)) * w;
+                const V3<T> c = clamp_indirect(F, v);
+                P.L = P.L + c;
+                sink.on_radiance(v, c, true);
+            }
         } else if (!P.skip_sky) {
-            V3<T> v = P.T_ * sky_color(S, d);
+            V3<T> v = P.T_ * miss_sky<SKY>(S, K, d);
             const V3<T> c = (b == 0u) ? v : clamp_indirect(F, v);
             P.L = P.L + c;
             sink.on_radiance(v, c, b != 0u);
@@ -274,8 +310,73 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
                 r.clamp = true;
                 sink.emit(1, r, P);
             }
+            if constexpr (SKY) {
+                if (K->hdri_count > 0u) {
+                    // one cluster per sample, contribution / its probability;
+                    // unclamped at the primary hit (:6503-6518)
+                    const HdriSample hl = sample_hdri(S, *K, P.seed);
+                    const float ndl = dot(nf, hl.dir);
+                    if (ndl > 0.0f && hl.pdf > 0.0f) {
+                        ShadowRecT<T> r;
+                        r.o = hp + nf * 1e-3f;
+                        r.d = hl.dir;
+                        r.t_max = 1e30f;
+                        r.pre = P.T_ * hl.color;
+                        r.ndl = ndl;
+                        r.brdf = sf.albedo / kPi;
+                        r.pdf = hl.pdf;
+                        r.w = 1.0f;
+                        r.clamp = b > 0u;
+                        sink.emit(2, r, P);
+                    }
+                }
+            }
             P.skip_sky = !mis_on;
         } else {
+            if constexpr (SKY) {
+                // the sun's NEE: a shadow ray into its disc, radiance by the
+                // air mass, unclamped (:6552-6620)
+                const f3 sd = K->sun;
+                if (sd.y > 0.0f) {
+                    const float ndl = fmaxf(0.0f, dot(nf, sd));
+                    const f3 jit = sample_sun_disc(sd, P.seed);
+                    if (ndl > 0.0f) {
+                        ShadowRecT<T> r;
+                        r.o = hp + nf * 1e-3f;
+                        r.d = jit;
+                        r.t_max = 1e30f;
+                        r.pre = P.T_ * sun_radiance(sd.y);
+                        r.ndl = ndl;
+                        r.brdf = sf.albedo / kPi;
+                        r.pdf = 1.0f;
+                        r.w = 1.0f;
+                        r.clamp = false;
+                        sink.emit(2, r, P);
+                    }
+                }
+                // the moon's NEE when it is up and more than 1 % lit (:6624-6660)
+                const f3 md = K->moon;
+                if (md.y > 0.0f) {
+                    const float phase_b = 0.5f * (1.0f - c_cos(K->moon_phase));
+                    if (phase_b > 0.01f) {
+                        const f3 jit = sample_sun_disc(md, P.seed);
+                        const float ndl = fmaxf(0.0f, dot(nf, jit));
+                        if (ndl > 0.0f) {
+                            ShadowRecT<T> r;
+                            r.o = hp + nf * 1e-3f;
+                            r.d = jit;
+                            r.t_max = 1e30f;
+                            r.pre = P.T_ * moon_radiance(md.y, phase_b);
+                            r.ndl = ndl;
+                            r.brdf = sf.albedo / kPi;
+                            r.pdf = 1.0f;
+                            r.w = 1.0f;
+                            r.clamp = false;
+                            sink.emit(3, r, P);
+                        }
+                    }
+                }
+            }
             // No env map: ambient albedo * skyColorBase(nf) (PathTrace.slang:6667),
             // added after the light NEE term.
             T tt = clampf(0.5f * (nf.y + 1.0f), 0.0f, 1.0f);

@@ -49,7 +49,7 @@ struct BatchArgs {
     uint32_t* spill;
     uint32_t spill_depth;
     uint32_t threads;                 // grid size in threads (scratch row stride)
-    float4* rec;                      // 2 records x kRecRows rows x threads
+    float4* rec;                      // kinds x kRecRows rows x threads (2 kinds, 4 with sky lights)
     DeviceStats* stats;
     uint32_t tile_w;                  // 8, 16, 32 or 64
     uint32_t tile_shift;              // log2(tile_w)
@@ -61,6 +61,7 @@ struct BatchLaunch {
     uint32_t blocks;                  // grid size in workgroups
     uint32_t stack_lds;               // LDS stack entries per thread: 16, 24 or 32
     bool features;                    // some scene's frame uses camera-sample features
+    bool sky;                         // some scene's frame has sky lights (4 record kinds in rec)
 };
 
 // Sets the frame counter (stream-ordered, not part of a captured frame).

@@ -6,7 +6,7 @@ namespace dmt {
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr uint32_t kEnded = 16u;          // pending bit: the path ends after this vertex
+constexpr uint32_t kEnded = 256u;         // pending bit: the path ends after this vertex
 
 // lane states
 constexpr uint32_t kLaneIdle = 0u;    // no path
@@ -16,7 +16,7 @@ constexpr uint32_t kPersistShadeLanes = 32u;    // shade once this many lanes ar
 constexpr uint32_t kPersistRefillLanes = 16u;   // take pixels once this many lanes are idle
 
 // Shadow records of one vertex go to this thread's scratch rows; bit k of
-// `pending` marks record k, bit k + 2 its clampIndirect flag.
+// `pending` marks record k, bit k + 4 its clampIndirect flag.
 struct ScratchSink : NoGradientHooks {
     float4* rec;
     uint32_t threads, tid;
@@ -27,7 +27,7 @@ struct ScratchSink : NoGradientHooks {
         row[(size_t)threads] = make_float4(r.d.x, r.d.y, r.d.z, r.w);
         row[2 * (size_t)threads] = make_float4(r.pre.x, r.pre.y, r.pre.z, r.ndl);
         row[3 * (size_t)threads] = make_float4(r.brdf.x, r.brdf.y, r.brdf.z, r.pdf);
-        pending |= (1u << kind) | (r.clamp ? (4u << kind) : 0u);
+        pending |= (1u << kind) | (r.clamp ? (16u << kind) : 0u);
     }
 };
 
@@ -116,25 +116,52 @@ __device__ __forceinline__ void start_shadow(const float4* rec, uint32_t threads
 }
 
 // After the vertex's shading or one of its shadow records: start the next
-// record (light before env, as the wavefront's shadow kernels run), or close
-// the vertex.
-template <bool FEAT>
+// record in the order the vertex's terms are summed -- light, then env NEE
+// and the HDRI cluster NEE with an env map; light, sun, moon, then the
+// ambient term without one -- or close the vertex.  Records 2 and 3 (sky
+// lights) exist only in SKY frames.
+template <bool FEAT, bool SKY = false>
 __device__ __forceinline__ void advance(const FrameArgs& F, const FeatureArgs& X, const float4* rec, uint32_t threads,
                                         uint32_t tid, float4* accum, bool reset, Lane& l, Cold& c) {
     if (l.pending & 1u) {
         start_shadow(rec, threads, tid, 0u, l, c);
         return;
     }
-    if (l.pending & 2u) {
-        const float4* row = rec + ((size_t)1u * kRecRows) * threads + tid;
-        const float4 b4 = row[3 * (size_t)threads];
-        if (b4.w < 0.0f) {
-            // ambient term of the no-env Lambert branch: added without a query
+    if (!SKY) {
+        if (l.pending & 2u) {
+            const float4* row = rec + ((size_t)1u * kRecRows) * threads + tid;
+            const float4 b4 = row[3 * (size_t)threads];
+            if (b4.w < 0.0f) {
+                // ambient term of the no-env Lambert branch: added without a query
+                l.P.L = l.P.L + xyz(row[2 * (size_t)threads]);
+                l.pending &= ~2u;
+            } else {
+                start_shadow(rec, threads, tid, 1u, l, c);
+                return;
+            }
+        }
+    } else {
+        bool ambient = false;
+        if (l.pending & 2u) {
+            const float4* row = rec + ((size_t)1u * kRecRows) * threads + tid;
+            ambient = row[3 * (size_t)threads].w < 0.0f;   // the no-env ambient term: after the sun and moon
+            if (!ambient) {
+                start_shadow(rec, threads, tid, 1u, l, c);
+                return;
+            }
+        }
+        if (l.pending & 4u) {
+            start_shadow(rec, threads, tid, 2u, l, c);
+            return;
+        }
+        if (l.pending & 8u) {
+            start_shadow(rec, threads, tid, 3u, l, c);
+            return;
+        }
+        if (ambient) {
+            const float4* row = rec + ((size_t)1u * kRecRows) * threads + tid;
             l.P.L = l.P.L + xyz(row[2 * (size_t)threads]);
             l.pending &= ~2u;
-        } else {
-            start_shadow(rec, threads, tid, 1u, l, c);
-            return;
         }
     }
     if (!(l.pending & kEnded)) {
@@ -173,7 +200,7 @@ template <uint32_t DEPTH> constexpr uint32_t persistent_waves() { return (160u *
 // FEAT: some scene's frame uses camera-sample features (thin lens, shutter
 // time, hero wavelength, a translated mesh); each scene's FeatureArgs say
 // which, and the lane carries the sample's shutter time and wavelength.
-template <uint32_t DEPTH, bool FEAT>
+template <uint32_t DEPTH, bool FEAT, bool SKY = false>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<DEPTH>()))) k_persistent(BatchArgs A) {
     __shared__ uint32_t lds[DEPTH * kBlock];
@@ -300,12 +327,12 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                 else n_ext++;
                 ScratchSink sink{{}, A.rec, A.threads, tid, 0u};
                 VertexResult R;
-                shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
+                shade_vertex<SKY>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.pending | (cont ? 0u : kEnded);
                 c.set3(Cold::kNo, R.no);
                 c.set3(Cold::kNd, R.nd);
-                advance<FEAT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                advance<FEAT, SKY>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(Cold::kTmax);
@@ -318,10 +345,10 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                         const float wt = row[(size_t)A.threads].w;
                         const float4 a4 = row[2 * (size_t)A.threads], b4 = row[3 * (size_t)A.threads];
                         l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w, k == 1u ? wt : 1.0f,
-                                                     (l.pending & (4u << k)) != 0u, tr);
+                                                     (l.pending & (16u << k)) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    advance<FEAT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                    advance<FEAT, SKY>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
                 } else {
                     c.set(Cold::kTmax, t_max);
                     c.set3(Cold::kTr, tr);
@@ -374,13 +401,17 @@ hipError_t batch_set_frame(uint32_t* frame, uint32_t value, hipStream_t stream) 
 hipError_t launch_batch_frame(const BatchArgs& A, const BatchLaunch& L, hipStream_t stream) {
     if (L.blocks == 0u || (size_t)L.blocks * kBlock != A.threads) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, stream, A.work, A.nscenes);
-    switch (L.stack_lds * 2u + (L.features ? 1u : 0u)) {
-        case 32u: hipLaunchKernelGGL((k_persistent<16u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 33u: hipLaunchKernelGGL((k_persistent<16u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 48u: hipLaunchKernelGGL((k_persistent<24u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 49u: hipLaunchKernelGGL((k_persistent<24u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 64u: hipLaunchKernelGGL((k_persistent<32u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 65u: hipLaunchKernelGGL((k_persistent<32u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+    const uint32_t v = L.sky ? 2u : (L.features ? 1u : 0u);   // sky lights imply features
+    switch (L.stack_lds * 4u + v) {
+        case 64u: hipLaunchKernelGGL((k_persistent<16u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 65u: hipLaunchKernelGGL((k_persistent<16u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 66u: hipLaunchKernelGGL((k_persistent<16u, true, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 96u: hipLaunchKernelGGL((k_persistent<24u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 97u: hipLaunchKernelGGL((k_persistent<24u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 98u: hipLaunchKernelGGL((k_persistent<24u, true, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 128u: hipLaunchKernelGGL((k_persistent<32u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 129u: hipLaunchKernelGGL((k_persistent<32u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+        case 130u: hipLaunchKernelGGL((k_persistent<32u, true, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
         default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, stream, A.frame);

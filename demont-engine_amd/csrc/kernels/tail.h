@@ -42,20 +42,22 @@ __device__ __forceinline__ void camera_path(const FrameArgs& F, uint32_t frame_i
     start_path(ps, q, slot, p, F.pos, rd, seed, meta_pack(0u, 0u, 0u));
 }
 
-// Writes a path's shadow queries at its own slot of the iteration.
+// Writes a path's shadow queries at its own slot of the iteration: light
+// NEE, env NEE (or the no-env ambient term), and in frames with sky lights
+// the sun or HDRI cluster NEE and the moon NEE.
 struct SlotSink : NoGradientHooks {
-    ShadowQueue q0, q1;   // light NEE, env NEE
+    ShadowQueue q0, q1, q2, q3;
     uint32_t slot, pix;
-    bool emitted[2];
+    uint32_t emitted;     // bit k: a record of kind k
     __device__ __forceinline__ void emit(int kind, const ShadowRec& r, PathVertexState&) {
-        ShadowQueue& s = kind == 0 ? q0 : q1;
+        ShadowQueue& s = kind == 0 ? q0 : (kind == 1 ? q1 : (kind == 2 ? q2 : q3));
         const uint32_t tag = pix | kValidBit | (r.clamp ? kClampBit : 0u);
         s.o[slot] = make_float4(r.o.x, r.o.y, r.o.z, r.t_max);
         s.d[slot] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(tag));
         s.a[slot] = make_float4(r.pre.x, r.pre.y, r.pre.z, r.ndl);
         s.b[slot] = make_float4(r.brdf.x, r.brdf.y, r.brdf.z, r.pdf);
         if (kind == 1) s.w[slot] = r.w;
-        emitted[kind] = true;
+        emitted |= 1u << kind;
     }
 };
 
@@ -63,7 +65,7 @@ struct SlotSink : NoGradientHooks {
 namespace tail {
 
 
-constexpr uint32_t kEnded = 16u;   // pending bit: the path ends after this vertex
+constexpr uint32_t kEnded = 16u;   // pending bit: the path ends after this vertex (bits 0-3: records)
 
 // k_tail lane states
 constexpr uint32_t kLaneIdle = 0u;    // no path
@@ -113,26 +115,53 @@ __device__ __forceinline__ void tail_start_shadow(const ShadowQueue& q, uint32_t
     l.mode = k + 1u;
 }
 
-// After a vertex or one of its shadow queries: the next shadow query (light
-// before env, as k_shadow<0> runs before k_shadow<1>), the next vertex, the
-// next sample of the pixel, or the pixel's frame value.  FEAT: the frame's
-// camera-sample features X are on; the samples draw them (camera_sample) and
-// add their radiance weighted by the hero wavelength's response.
-template <bool FEAT>
+// After a vertex or one of its shadow queries: the next shadow query in the
+// order the vertex's terms are summed (light, then env NEE and the HDRI
+// cluster NEE with an env map; light, sun, moon, then the ambient term
+// without one), the next vertex, the next sample of the pixel, or the
+// pixel's frame value.  FEAT: the frame's camera-sample features and sky
+// lights X are on; the samples draw them (camera_sample) and add their
+// radiance weighted by the hero wavelength's response.  SKY: the frame has
+// sky lights, whose records (2 and 3) exist only then.
+template <bool FEAT, bool SKY = false>
 __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureArgs& X, const ShadowQueue& sl,
-                                             const ShadowQueue& se, const PathState& ps, TailLane& l, TailCold& c) {
+                                             const ShadowQueue& se, const ShadowQueue& s2, const ShadowQueue& s3,
+                                             const PathState& ps, TailLane& l, TailCold& c) {
     if (l.pending & 1u) {
         tail_start_shadow(sl, 0u, l, c);
         return;
     }
-    if (l.pending & 2u) {
-        if (se.b[l.slot].w < 0.0f) {
-            // ambient term of the no-env Lambert branch: added without a query
+    if (!SKY) {
+        if (l.pending & 2u) {
+            if (se.b[l.slot].w < 0.0f) {
+                // ambient term of the no-env Lambert branch: added without a query
+                l.P.L = l.P.L + xyz(se.a[l.slot]);
+                l.pending &= ~2u;
+            } else {
+                tail_start_shadow(se, 1u, l, c);
+                return;
+            }
+        }
+    } else {
+        bool ambient = false;
+        if (l.pending & 2u) {
+            ambient = se.b[l.slot].w < 0.0f;   // the no-env ambient term: after the sun and moon
+            if (!ambient) {
+                tail_start_shadow(se, 1u, l, c);
+                return;
+            }
+        }
+        if (l.pending & 4u) {
+            tail_start_shadow(s2, 2u, l, c);
+            return;
+        }
+        if (l.pending & 8u) {
+            tail_start_shadow(s3, 3u, l, c);
+            return;
+        }
+        if (ambient) {
             l.P.L = l.P.L + xyz(se.a[l.slot]);
             l.pending &= ~2u;
-        } else {
-            tail_start_shadow(se, 1u, l, c);
-            return;
         }
     }
     if (!(l.pending & kEnded)) {

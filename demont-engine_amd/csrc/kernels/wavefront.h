@@ -85,7 +85,7 @@ struct Wavefront {
     PathState ps{};
     RayQueue rq[2]{};
     uint2* hits = nullptr;    // (t bits, code) per slot
-    ShadowQueue sq[2]{};      // 0: light NEE, 1: env NEE
+    ShadowQueue sq[4]{};      // 0: light NEE, 1: env NEE; 2, 3: sky lights (allocated for such frames only)
     uint32_t* finished = nullptr;     // per slot: pixel of a path that ended, or kMiss
     uint8_t* regen = nullptr;         // per pixel: next sample pending
     uint32_t* counters = nullptr;

@@ -135,10 +135,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView S, FrameArgs F, uint
             const uint32_t b = meta & 0xFFu;
             const uint32_t flags = __float_as_uint(Lv.w);
             PathVertexState P{xyz(Tv), xyz(Lv), Tv.w, (flags & 1u) != 0u, (flags & 2u) != 0u, ps.rng[pix]};
-            SlotSink sink{{}, sl, se, i, pix, {false, false}};
+            SlotSink sink{{}, sl, se, ShadowQueue{}, ShadowQueue{}, i, pix, 0u};
             shade_vertex(S, F, b, xyz(o4), xyz(d4), HitRec{__uint_as_float(hr.x), hr.y}, P, R, sink);
-            if (!sink.emitted[0]) sl.d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-            if (!sink.emitted[1]) se.d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+            if (!(sink.emitted & 1u)) sl.d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+            if (!(sink.emitted & 2u)) se.d[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
             if (b == 0u) n_primary++;
             else n_ext++;
             const uint32_t nflags = (P.skip_sky ? 1u : 0u) | (P.prev_lambert ? 2u : 0u);
@@ -222,10 +222,11 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
 // frame's 8x8 tile order, whose pixel's first sample the lane starts itself
 // (camera_sample), and the shutter time and wavelength ride in the lane's
 // path state.
-template <bool COUNT, bool FEAT>
+template <bool COUNT, bool FEAT, bool SKY = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerSimd, kTailWavesPerSimd)))
 k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uint32_t* ctr, PathState ps,
-       ShadowQueue sl, ShadowQueue se, uint32_t* spill, uint32_t spill_depth, DeviceStats* st) {
+       ShadowQueue sl, ShadowQueue se, ShadowQueue s2, ShadowQueue s3, uint32_t* spill, uint32_t spill_depth,
+       DeviceStats* st) {
     __shared__ uint32_t lds[kLdsDepth * kBlock];
     __shared__ float cold[tail::TailCold::kRows * kBlock];
     tail::TailCold c{cold + threadIdx.x};
@@ -314,16 +315,16 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
             if (l.mode == 0u) {
                 if (l.b == 0u) n_primary++;
                 else n_ext++;
-                SlotSink sink{{}, sl, se, l.slot, l.pix, {false, false}};
+                SlotSink sink{{}, sl, se, s2, s3, l.slot, l.pix, 0u};
                 VertexResult R;
-                shade_vertex(S, F, l.b, l.o, l.d, h, l.P, R, sink);
+                shade_vertex<SKY>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
-                l.pending = (sink.emitted[0] ? 1u : 0u) | (sink.emitted[1] ? 2u : 0u) | (cont ? 0u : tail::kEnded);
+                l.pending = sink.emitted | (cont ? 0u : tail::kEnded);
                 if (cont) {
                     c.set3(tail::TailCold::kNo, R.no);
                     c.set3(tail::TailCold::kNd, R.nd);
                 }
-                tail::tail_advance<FEAT>(F, X, sl, se, ps, l, c);
+                tail::tail_advance<FEAT, SKY>(F, X, sl, se, s2, s3, ps, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(tail::TailCold::kTmax);
@@ -332,15 +333,23 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
-                        // fields picked one by one: a selected reference to either
-                        // queue argument would put both in private memory
-                        const float4 a4 = (k == 0u ? sl.a : se.a)[l.slot], b4 = (k == 0u ? sl.b : se.b)[l.slot];
-                        const uint32_t tag = __float_as_uint((k == 0u ? sl.d : se.d)[l.slot].w);
+                        // fields picked one by one: a selected reference to one of
+                        // the queue arguments would put them in private memory
+                        const float4* qa = k == 0u ? sl.a : se.a;
+                        const float4* qb = k == 0u ? sl.b : se.b;
+                        const float4* qd = k == 0u ? sl.d : se.d;
+                        if (SKY && k >= 2u) {
+                            qa = k == 2u ? s2.a : s3.a;
+                            qb = k == 2u ? s2.b : s3.b;
+                            qd = k == 2u ? s2.d : s3.d;
+                        }
+                        const float4 a4 = qa[l.slot], b4 = qb[l.slot];
+                        const uint32_t tag = __float_as_uint(qd[l.slot].w);
                         l.P.L = l.P.L + shadow_value(F, (int)k, xyz(a4), a4.w, xyz(b4), b4.w,
                                                      k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    tail::tail_advance<FEAT>(F, X, sl, se, ps, l, c);
+                    tail::tail_advance<FEAT, SKY>(F, X, sl, se, s2, s3, ps, l, c);
                 } else {   // transmit_step ends every query by its step refract_bounces
                     c.set(tail::TailCold::kTmax, t_max);
                     c.set3(tail::TailCold::kTr, tr);
@@ -627,12 +636,19 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
             o.event_kinds[0] = 1u;
             (void)hipEventRecord(o.extend_events[0], stream);
         }
-        if (o.counting)
-            hipLaunchKernelGGL((k_tail<true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, *o.features, 0u,
-                               W.rq[0], W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+        const FeatureArgs& X = *o.features;
+        if (o.counting && X.sky)
+            hipLaunchKernelGGL((k_tail<true, true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
+                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
+        else if (o.counting)
+            hipLaunchKernelGGL((k_tail<true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
+                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
+        else if (X.sky)
+            hipLaunchKernelGGL((k_tail<false, true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
+                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
         else
-            hipLaunchKernelGGL((k_tail<false, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, *o.features, 0u,
-                               W.rq[0], W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+            hipLaunchKernelGGL((k_tail<false, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
+                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
         if (ev) {
             (void)hipEventRecord(o.extend_events[1], stream);
             o.events_used = 2u;
@@ -664,10 +680,12 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
             }
             if (o.counting)
                 hipLaunchKernelGGL((k_tail<true, false>), dim3(tg), dim3(kBlock), 0, stream, S, F, none, par, W.rq[par],
-                                   W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+                                   W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth,
+                                   W.stats);
             else
                 hipLaunchKernelGGL((k_tail<false, false>), dim3(tg), dim3(kBlock), 0, stream, S, F, none, par, W.rq[par],
-                                   W.counters, W.ps, W.sq[0], W.sq[1], W.spill, W.spill_depth, W.stats);
+                                   W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth,
+                                   W.stats);
             if (ev) {
                 (void)hipEventRecord(o.extend_events[o.events_used + 1u], stream);
                 o.events_used += 2u;

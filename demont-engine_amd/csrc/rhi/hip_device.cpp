@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <new>
 #include <unordered_map>
@@ -90,6 +91,7 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     r.light_count = u32(p, kLightCount);
     r.env_map_present = u32(p, kEnvMapPresent) != 0u;
     r.env_intensity = f32(p, kEnvIntensity);
+    r.env_total_luminance = f32(p, kEnvTotalLuminance);
     r.tonemap_op = u32(p, kTonemapParams);
     // camera-sample features: thin lens (PathTrace.slang:5786-5816), shutter
     // time (:5771), hero wavelength (:5756), mesh translation (:2483)
@@ -105,6 +107,28 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     }
     F.mesh_shift_active = f32(p, kMeshMotionPrev + 12) > 0.5f;
     r.denoiser = u32(p, kDenoiserEnabled) != 0u;
+    // with a denoiser every pixel's first sample takes the frame's Halton
+    // offset (PathTrace.slang:5730-5731)
+    if (r.denoiser) {
+        F.flags |= DMT_FRAME_FIXED_JITTER;
+        F.jitter[0] = f32(p, kHaltonJitter);
+        F.jitter[1] = f32(p, kHaltonJitter + 4);
+    }
+    // sky lights: the gradient sky's sun and moon, the env map's cluster lights
+    for (int i = 0; i < 4; ++i) {
+        F.sun[i] = i < 3 ? f32(p, kSunAndMode + 4 * i) : 0.0f;
+        F.sun_extra[i] = i < 3 ? f32(p, kSunExtra + 4 * i) : 0.0f;
+        F.moon[i] = f32(p, kMoonDirPhase + 4 * i);
+        F.moon_extra[i] = i < 3 ? f32(p, kMoonExtra + 4 * i) : 0.0f;
+    }
+    F.sun_disc_brightness = f32(p, kSunExtra2);
+    const std::uint32_t hdri = u32(p, kHdriLightsCount);
+    F.hdri_light_count = r.env_map_present ? std::min<std::uint32_t>(hdri, 8u) : 0u;
+    for (std::uint32_t i = 0; i < 8u; ++i)
+        for (int c = 0; c < 4; ++c) {
+            F.hdri_dir[i][c] = i < F.hdri_light_count ? f32(p, kHdriLightsDir + 16 * i + 4 * c) : 0.0f;
+            F.hdri_col[i][c] = i < F.hdri_light_count ? f32(p, kHdriLightsCol + 16 * i + 4 * c) : 0.0f;
+        }
     r.write_normal = u32(p, kWriteNormalGbuffer) != 0u;
     r.write_albedo = u32(p, kWriteAlbedoGbuffer) != 0u;
     r.write_specular_albedo = u32(p, kWriteSpecularAlbedo) != 0u;
@@ -129,14 +153,7 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     const std::uint32_t mode = (mode_f >= 1.0f && mode_f < 4.0f) ? (std::uint32_t)mode_f : 0u;
     need(mode == 2u, "procedural sky (sun_and_mode.w = 2)");
     need(mode == 3u, "Hosek-Wilkie sky (sun_and_mode.w = 3)");
-    if (!r.env_map_present && mode < 2u) {
-        // the gradient sky adds the sun and moon discs (zero below y = -0.1,
-        // PathTrace.slang:3113, 3213) and the Lambert vertex their NEE (y > 0,
-        // :6555, :6626)
-        need(!(f32(p, kSunAndMode + 4) < -0.10f), "sun above the horizon without an env map");
-        need(!(f32(p, kMoonDirPhase + 4) < -0.10f), "moon above the horizon without an env map");
-    }
-    need(r.env_map_present && u32(p, kHdriLightsCount) != 0u, "HDRI cluster lights");       // :3680, :3752
+    need(r.env_map_present && hdri > 8u, "more than 8 HDRI cluster lights");
     need(f32(p, kVolParams) > 0.0f || f32(p, kCloudsP1 + 12) > 0.0f || f32(p, kCloudsP3 + 12) > 0.0f,
          "participating media (fog, clouds, Rayleigh)");                                     // :7011
     need(u32(p, kSdfParams) != 0u, "SDF clusters");                                          // :2224
@@ -196,6 +213,10 @@ struct SceneKey {
     std::uint64_t tlas = 0;
     std::uint64_t env = 0, env_gen = 0;
     std::uint32_t env_intensity = 0;
+    // the engine's importance CDF (buffer slots 4, 5), read when the env map
+    // has extracted cluster lights (their pixels are out of it)
+    std::uint64_t cdf_m = 0, cdf_m_gen = 0, cdf_c = 0, cdf_c_gen = 0;
+    std::uint32_t cdf_total = 0;
     bool operator==(const SceneKey&) const = default;
 };
 
@@ -256,6 +277,9 @@ struct HipDevice::Impl {
     float* scene_accum = nullptr;        // texture memory the scene renders into
     std::uint64_t builds = 0, material_updates = 0, mesh_refits = 0;
     std::vector<std::uint32_t> scene_indices;   // the mesh triangles the scene was built with
+    // the moon texture last handed to the scene (dmt_scene_set_moon_map)
+    const dmt_scene* moon_scene = nullptr;
+    std::uint64_t moon_id = 0, moon_gen = 0;
 
     void report(const std::string& m) {
         error = m;
@@ -522,6 +546,24 @@ bool HipDevice::WriteTexture(TextureHandle h, const void* src, std::size_t size)
         if (size != channels) return false;
         const auto* b = static_cast<const std::uint8_t*>(src);
         for (std::size_t i = 0; i < channels; ++i) rgba[i] = (float)b[i] * (1.0f / 255.0f);
+    } else if (t->format == TextureFormat::RGBA16F) {
+        // IEEE binary16 per channel, as the path tracer's shader reads an
+        // RGBA16F texture (the engine's moon map, Engine.cpp:4626-4645); the
+        // software device, which never samples one, copies the raw bytes
+        if (size != channels * 2u) return false;
+        const auto* hb = static_cast<const std::uint8_t*>(src);
+        for (std::size_t i = 0; i < channels; ++i) {
+            const std::uint32_t h16 = (std::uint32_t)hb[2 * i] | ((std::uint32_t)hb[2 * i + 1] << 8);
+            const std::uint32_t sign = (h16 & 0x8000u) << 16, e = (h16 >> 10) & 0x1Fu, m = h16 & 0x3FFu;
+            float v;
+            if (e == 0u) v = std::ldexp((float)m, -24);                                 // zero, subnormal
+            else if (e == 31u) v = m ? std::numeric_limits<float>::quiet_NaN() : INFINITY;
+            else v = std::ldexp((float)(m | 0x400u), (int)e - 25);
+            std::uint32_t vb;
+            std::memcpy(&vb, &v, 4);
+            vb |= sign;
+            std::memcpy(&rgba[i], &vb, 4);
+        }
     } else {
         // raw bytes into the float backing, as many as fit (SoftwareDevice.cpp:524-537)
         std::memcpy(rgba.data(), src, std::min(size, channels * 4u));
@@ -643,6 +685,8 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
     const Buffer* lights = nullptr;
     const Accel* tlas = nullptr;
     Texture* env = nullptr;
+    const Buffer* cdf_m = nullptr;
+    const Buffer* cdf_c = nullptr;
     if (P.prim_count) {
         k.prims = cb.buffers_[slot::kPrimBuffer];
         k.prims_off = cb.buffer_offsets_[slot::kPrimBuffer];
@@ -686,6 +730,22 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
         }
         k.env_gen = env->gen;
         k.env_intensity = bits(P.env_intensity);
+        if (P.frame.hdri_light_count) {
+            k.cdf_m = cb.buffers_[slot::kEnvMarginalBuffer];
+            k.cdf_c = cb.buffers_[slot::kEnvConditionalBuffer];
+            cdf_m = buffer(k.cdf_m);
+            cdf_c = buffer(k.cdf_c);
+            if (!cdf_m || !cdf_c || cdf_m->data.size() < (std::size_t)env->h * 4u ||
+                cdf_c->data.size() < (std::size_t)env->w * env->h * 4u) {
+                report("pathtrace: HDRI cluster lights need the env map's CDF at buffer slots 4 (" +
+                       std::to_string(env->h) + " floats) and 5 (" + std::to_string((std::size_t)env->w * env->h) +
+                       " floats)");
+                return false;
+            }
+            k.cdf_m_gen = cdf_m->gen;
+            k.cdf_c_gen = cdf_c->gen;
+            k.cdf_total = bits(P.env_total_luminance);
+        }
     }
     if (scene && k == key) return true;
 
@@ -757,11 +817,20 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
         d.indices = tlas->indices.data();
         d.index_count = (std::uint32_t)tlas->indices.size();
     }
+    std::vector<float> cdf_copy;
     if (env) {
         d.env_rgb = env_rgb.data();
         d.env_width = env->w;
         d.env_height = env->h;
         d.env_intensity = P.env_intensity;
+        if (cdf_m && cdf_c) {   // float-aligned copies of the engine's CDF
+            cdf_copy.resize((std::size_t)env->h + (std::size_t)env->w * env->h);
+            std::memcpy(cdf_copy.data(), cdf_m->data.data(), (std::size_t)env->h * 4u);
+            std::memcpy(cdf_copy.data() + env->h, cdf_c->data.data(), (std::size_t)env->w * env->h * 4u);
+            d.env_marginal = cdf_copy.data();
+            d.env_conditional = cdf_copy.data() + env->h;
+            d.env_total_luminance = P.env_total_luminance;
+        }
     }
     // the light records are copied out of the byte buffer so they are float-aligned
     std::vector<float> light_copy;
@@ -872,6 +941,25 @@ void HipDevice::dispatch_impl(const HipCommandBuffer& cb) {
         if (!I.check(dmt_render_gbuffer(I.scene, &P.frame, &gp), "dmt_render_gbuffer")) return;
         for (int k = 0; k < 8; ++k)
             if (*dst[k]) written(aux[k].t);
+    }
+    // the moon disc of the gradient sky samples the moon texture at storage
+    // slot 7 (PathTrace.slang:3306-3318)
+    if (!P.env_map_present) {
+        const std::uint64_t mid = cb.textures_[slot::kMoonTexture];
+        Texture* mt = I.texture(mid);
+        const std::uint64_t mgen = mt ? mt->gen : 0u;
+        if (I.moon_scene != I.scene || I.moon_id != mid || I.moon_gen != mgen) {
+            if (mt && mt->w && mt->h) {
+                if (!I.host_copy(*mt)) return;
+                if (!I.check(dmt_scene_set_moon_map(I.scene, mt->host.data(), mt->w, mt->h), "dmt_scene_set_moon_map"))
+                    return;
+            } else if (!I.check(dmt_scene_set_moon_map(I.scene, nullptr, 0, 0), "dmt_scene_set_moon_map")) {
+                return;
+            }
+            I.moon_scene = I.scene;
+            I.moon_id = mid;
+            I.moon_gen = mgen;
+        }
     }
     if (!I.check(dmt_render(I.scene, &P.frame), "dmt_render")) return;
     written(acc);

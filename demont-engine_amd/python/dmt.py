@@ -24,8 +24,9 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
+DMT_FRAME_FIXED_JITTER = 2
 SCHEDULE_WAVEFRONT = 0xFFFFFFFF
 MATERIAL_FLOATS = 7
 DMT_NOALLOC_BLOCKS = 512        # dmt_trace_closest_noalloc's grid
@@ -62,6 +63,15 @@ class FrameParams(C.Structure):
         ("mesh_shift_active", C.c_uint32),
         ("mesh_shift_curr", C.c_float * 3),
         ("reserved", C.c_uint32),
+        ("jitter", C.c_float * 2),
+        ("sun", C.c_float * 4),
+        ("sun_extra", C.c_float * 4),
+        ("sun_disc_brightness", C.c_float),
+        ("moon", C.c_float * 4),
+        ("moon_extra", C.c_float * 4),
+        ("hdri_light_count", C.c_uint32),
+        ("hdri_dir", (C.c_float * 4) * 8),
+        ("hdri_col", (C.c_float * 4) * 8),
     ]
 
     def copy(self) -> "FrameParams":
@@ -79,6 +89,9 @@ class SceneDesc(C.Structure):
         ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
         ("env_intensity", C.c_float),
         ("reserved", C.c_uint32),
+        ("env_marginal", C.POINTER(C.c_float)),
+        ("env_conditional", C.POINTER(C.c_float)),
+        ("env_total_luminance", C.c_double),
     ]
 
 
@@ -169,6 +182,14 @@ def lib() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+    # PyTorch-ROCm ships its own HIP runtime.  Two runtimes in one process
+    # cannot both open the GPU: the one initialised second reports no device.
+    # Loading torch first makes this library bind to torch's runtime, so
+    # device pointers pass between the two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     v = C.c_uint32()
     if L.dmt_get_abi_version(C.byref(v)) != 0 or v.value != ABI_VERSION:
@@ -207,6 +228,9 @@ def lib() -> C.CDLL:
         "dmt_build_bvh4": (st, [fp, u32, C.POINTER(u32), u32, u32, u32, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
         "dmt_build_light_tree": (u32, [fp, u32, fp]),
         "dmt_build_env_cdf": (C.c_double, [fp, u32, u32, fp, fp]),
+        "dmt_build_env_cdf_masked": (C.c_double, [fp, u32, u32, C.POINTER(C.c_uint8), fp, fp]),
+        "dmt_env_extract_lights": (u32, [fp, u32, u32, C.c_double, fp, fp, C.POINTER(C.c_uint8)]),
+        "dmt_scene_set_moon_map": (st, [vp, fp, u32, u32]),
         "dmt_scene_get_materials": (st, [vp, fp, C.c_size_t]),
         "dmt_scene_set_materials": (st, [vp, fp, C.c_size_t]),
         "dmt_adjoint": (st, [vp, C.POINTER(FrameParams), vp, vp]),
@@ -263,6 +287,8 @@ class SceneData:
     env: np.ndarray | None = None          # (H, W, 3) float32
     env_intensity: float = 1.0
     frame: FrameParams = field(default_factory=FrameParams)
+    env_cdf: tuple | None = None           # (marginal (H,), conditional (H, W), total) given instead of built
+    moon: np.ndarray | None = None         # (H, W, 4) moon texture
 
     def desc(self) -> SceneDesc:
         for name in ("prims", "lights", "positions"):
@@ -277,6 +303,11 @@ class SceneData:
             self.env = np.ascontiguousarray(self.env, np.float32)
             d.env_rgb, d.env_height, d.env_width = _fptr(self.env), self.env.shape[0], self.env.shape[1]
         d.env_intensity = self.env_intensity
+        if self.env_cdf is not None:
+            m, c, t = self.env_cdf
+            self._cdf = (np.ascontiguousarray(m, np.float32), np.ascontiguousarray(c, np.float32))
+            d.env_marginal, d.env_conditional = _fptr(self._cdf[0]), _fptr(self._cdf[1])
+            d.env_total_luminance = float(t)
         return d
 
 
@@ -383,6 +414,15 @@ class Scene:
         d = data.desc()
         bp = BuildParams(sah_bins, max_leaf, BUILD_DEVICE if device_build else 0)
         _check(lib().dmt_scene_create(device.handle, C.byref(d), C.byref(bp), C.byref(self._h)))
+        if data.moon is not None:
+            self.set_moon_map(data.moon)
+
+    def set_moon_map(self, rgba: np.ndarray | None):
+        if rgba is None:
+            _check(lib().dmt_scene_set_moon_map(self._h, None, 0, 0))
+            return
+        m = np.ascontiguousarray(rgba, np.float32)
+        _check(lib().dmt_scene_set_moon_map(self._h, _fptr(m), m.shape[1], m.shape[0]))
 
     def render(self, frame: FrameParams):
         _check(lib().dmt_render(self._h, C.byref(frame)))
@@ -566,10 +606,35 @@ def build_light_tree(lights: np.ndarray) -> np.ndarray:
     return out[:cnt]
 
 
-def build_env_cdf(env: np.ndarray):
+def build_env_cdf(env: np.ndarray, mask: np.ndarray | None = None):
     e = np.ascontiguousarray(env, np.float32)
     H, W = e.shape[0], e.shape[1]
     marg = np.zeros(H, np.float32)
     cond = np.zeros(H * W, np.float32)
-    total = lib().dmt_build_env_cdf(_fptr(e), W, H, _fptr(marg), _fptr(cond))
+    if mask is None:
+        total = lib().dmt_build_env_cdf(_fptr(e), W, H, _fptr(marg), _fptr(cond))
+    else:
+        mk = np.ascontiguousarray(mask, np.uint8)
+        total = lib().dmt_build_env_cdf_masked(_fptr(e), W, H, mk.ctypes.data_as(C.POINTER(C.c_uint8)), _fptr(marg),
+                                               _fptr(cond))
     return marg, cond.reshape(H, W), total
+
+
+def extract_env_lights(env: np.ndarray, top_frac: float = 0.005):
+    """The engine's HDRI light extraction: (count, dir (8, 4), col (8, 4), mask (H, W))."""
+    e = np.ascontiguousarray(env, np.float32)
+    H, W = e.shape[0], e.shape[1]
+    d = np.zeros((8, 4), np.float32)
+    c = np.zeros((8, 4), np.float32)
+    mask = np.zeros((H, W), np.uint8)
+    n = lib().dmt_env_extract_lights(_fptr(e), W, H, top_frac, _fptr(d), _fptr(c),
+                                     mask.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return int(n), d, c, mask
+
+
+def set_hdri_lights(frame: FrameParams, count: int, dirs: np.ndarray, cols: np.ndarray):
+    frame.hdri_light_count = count
+    for i in range(8):
+        for a in range(4):
+            frame.hdri_dir[i][a] = float(dirs[i][a]) if i < len(dirs) else 0.0
+            frame.hdri_col[i][a] = float(cols[i][a]) if i < len(cols) else 0.0

@@ -61,6 +61,22 @@ py::dict decoded(const PathtracePush& P) {
     d["write_hdr_aux"] = P.write_hdr_aux;
     d["hdr_pipeline"] = P.hdr_pipeline;
     d["clouds_mode"] = P.clouds_mode;
+    d["jitter"] = std::vector<float>(F.jitter, F.jitter + 2);
+    d["sun"] = std::vector<float>(F.sun, F.sun + 4);
+    d["sun_extra"] = std::vector<float>(F.sun_extra, F.sun_extra + 4);
+    d["sun_disc_brightness"] = F.sun_disc_brightness;
+    d["moon"] = std::vector<float>(F.moon, F.moon + 4);
+    d["moon_extra"] = std::vector<float>(F.moon_extra, F.moon_extra + 4);
+    d["hdri_light_count"] = F.hdri_light_count;
+    std::vector<float> hd, hc;
+    for (int i = 0; i < 8; ++i)
+        for (int c = 0; c < 4; ++c) {
+            hd.push_back(F.hdri_dir[i][c]);
+            hc.push_back(F.hdri_col[i][c]);
+        }
+    d["hdri_dir"] = hd;
+    d["hdri_col"] = hc;
+    d["env_total_luminance"] = P.env_total_luminance;
     d["curr_view_proj"] = std::vector<float>(P.curr_view_proj, P.curr_view_proj + 16);
     d["prev_view_proj"] = std::vector<float>(P.prev_view_proj, P.prev_view_proj + 16);
     d["ignored"] = P.ignored;

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define DMT_ABI_VERSION 4u
+#define DMT_ABI_VERSION 5u
 
 typedef int32_t dmt_status;
 #define DMT_OK                  0
@@ -73,6 +73,16 @@ typedef struct dmt_scene_desc {
     const float*    env_rgb;    uint32_t env_width;  uint32_t env_height;
     float           env_intensity;
     uint32_t        reserved;
+    /* Optional importance CDF of the env map, as the engine uploads it
+     * (env_marginal_cdf H floats, env_conditional_cdf W*H floats, and
+     * env_total_luminance; Engine.cpp:4571-4605).  NULL: the scene builds it
+     * from env_rgb (BuildEnvCdf, HdrImage.cpp:169-219).  An engine that
+     * extracts HDRI cluster lights builds it with those clusters' pixels
+     * left out (dmt_build_env_cdf_masked), which the path tracer reads as
+     * the clusters' directions (isExtractedClusterDir). */
+    const float*    env_marginal;
+    const float*    env_conditional;
+    double          env_total_luminance;
 } dmt_scene_desc;
 
 /* Per-frame parameters: the PtPush fields the path reads
@@ -114,6 +124,28 @@ typedef struct dmt_frame_params {
                                    without motion blur (PathTrace.slang:2483) */
     float    mesh_shift_curr[3];/* mesh_motion_curr.xyz                      */
     uint32_t reserved;          /* 0                                         */
+    /* Sample 0's sub-pixel offset when flags has DMT_FRAME_FIXED_JITTER: the
+     * engine's per-frame Halton(2,3) offset while a denoiser runs
+     * (halton_jitter, PathTrace.slang:5730-5731, Engine.cpp:8245-8254),
+     * each in [-0.5, 0.5]; that sample draws no jitter. */
+    float    jitter[2];
+    /* Sky lights, all zero for none.  Without an env map the gradient sky
+     * carries the sun and moon: their discs in the sky (sunDisc, moonDisc,
+     * PathTrace.slang:3112, 3212) and, when above the horizon, their NEE at
+     * Lambert vertices (:6546-6667).  With an env map, hdri_light_count
+     * directional lights extracted from its bright clusters get their own NEE
+     * (:6493-6518, :3749). */
+    float    sun[4];            /* sun_and_mode.xyz: unit direction; w: 0    */
+    float    sun_extra[4];      /* r_sun_size, distance ratio, vertical
+                                   flattening (1 = round), 0               */
+    float    sun_disc_brightness; /* sun_extra2.x                            */
+    float    moon[4];           /* moon_dir_phase: unit direction, phase (rad) */
+    float    moon_extra[4];     /* r_moon_size, distance ratio, disc
+                                   brightness, 0                            */
+    uint32_t hdri_light_count;  /* hdri_lights_count, <= 8                   */
+    float    hdri_dir[8][4];    /* hdri_lights_dir: unit direction, pick
+                                   probability (summing to 1)              */
+    float    hdri_col[8][4];    /* hdri_lights_col: rgb flux, 0              */
 } dmt_frame_params;
 
 /* Russian roulette off (the reference applies it from bounce 2 on,
@@ -122,6 +154,9 @@ typedef struct dmt_frame_params {
  * perturbations; differentiable renders checked against finite differences
  * turn it off. */
 #define DMT_FRAME_NO_RUSSIAN_ROULETTE 1u
+/* Sample 0 takes jitter[] instead of two random draws (the engine sets it
+ * while a denoiser runs, PathTrace.slang:5730). */
+#define DMT_FRAME_FIXED_JITTER 2u
 
 typedef struct dmt_build_params {
     uint32_t sah_bins;          /* binned-SAH bin count (0 = default 16)     */
@@ -223,6 +258,12 @@ dmt_status dmt_accum_device_ptr(dmt_scene* scene, void** out_ptr);
  * own (allocated and zeroed at the next render).  Waits for the frames in
  * flight on the device.  Not allowed while the scene belongs to a batch. */
 dmt_status dmt_scene_bind_accum(dmt_scene* scene, void* d_rgba, uint32_t width, uint32_t height);
+
+/* The moon texture the moon disc samples (moon_map, Engine.cpp:4612-4653,
+ * bound at storage-texture slot 7): width*height RGBA floats, row 0 the
+ * disc's north; copied.  NULL (or never set): the disc's surface reads 0.
+ * Waits for the scene's frames in flight. */
+dmt_status dmt_scene_set_moon_map(dmt_scene* scene, const float* rgba, uint32_t width, uint32_t height);
 
 /* Display output (r_tonemap_op, exposure_state[0]). */
 #define DMT_TONEMAP_DISPLAY  0u   /* sRGB OETF of tonemap(mean * exposure)     */

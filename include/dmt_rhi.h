@@ -126,6 +126,7 @@ constexpr std::uint32_t kDenoiseColorTexture = 2;   /* denoiser colour input (wr
 constexpr std::uint32_t kDepthTexture = 3;     /* denoiser inputs (denoiser_enabled) ...  */
 constexpr std::uint32_t kMotionTexture = 4;
 constexpr std::uint32_t kEnvTexture = 5;       /* BindStorageTexture(5, env map)         */
+constexpr std::uint32_t kMoonTexture = 7;      /* BindStorageTexture(7, moon_map)        */
 constexpr std::uint32_t kNormalTexture = 8;    /* write_normal_gbuffer                   */
 constexpr std::uint32_t kAlbedoTexture = 9;    /* write_albedo_gbuffer                   */
 constexpr std::uint32_t kCloudTransTexture = 10;
@@ -134,6 +135,8 @@ constexpr std::uint32_t kRoughnessTexture = 12;
 constexpr std::uint32_t kSpecularDistanceTexture = 13;
 constexpr std::uint32_t kTlas = 2;             /* BindAccelStruct(2, scene TLAS)         */
 constexpr std::uint32_t kPrimBuffer = 3;       /* BindBuffer(3, primitives)              */
+constexpr std::uint32_t kEnvMarginalBuffer = 4;      /* BindBuffer(4, env_marginal_cdf)    */
+constexpr std::uint32_t kEnvConditionalBuffer = 5;   /* BindBuffer(5, env_conditional_cdf) */
 constexpr std::uint32_t kExposureBuffer = 6;   /* BindBuffer(6, exposure_state)          */
 constexpr std::uint32_t kLightBuffer = 12;     /* BindBuffer(12, lights)                 */
 constexpr std::uint32_t kBufferSlots = 24;     /* BindState sizes (SoftwareDevice.h:112-124) */
@@ -150,18 +153,20 @@ namespace ptpush {
 constexpr std::size_t kPosFovTan = 0, kFwdAspect = 16, kRightXyz = 32, kUpXyz = 48;
 constexpr std::size_t kFrameIndex = 64, kResetAccum = 68, kMaxBounces = 72, kTlasPresent = 76;
 constexpr std::size_t kPrimCount = 80, kSpp = 84, kDenoiserEnabled = 88, kEnvMapPresent = 92;
-constexpr std::size_t kEnvIntensity = 104, kCurrViewProj = 112, kPrevViewProj = 176;
+constexpr std::size_t kHaltonJitter = 96, kEnvIntensity = 104, kEnvTotalLuminance = 108;
+constexpr std::size_t kCurrViewProj = 112, kPrevViewProj = 176;
 constexpr std::size_t kHeadBytes = 112;        /* PtPushHead (SoftwareTracer.cpp:38-56) */
 constexpr std::size_t kSunAndMode = 240, kW2jRow0 = 272, kW2jRow1 = 288, kW2jRow2 = 304;
 constexpr std::size_t kDofParams = 320, kDofBokehParams = 336, kSpectralParams = 352, kVolParams = 368;
-constexpr std::size_t kCloudsP1 = 384, kCloudsP3 = 416, kMoonDirPhase = 448;
+constexpr std::size_t kCloudsP1 = 384, kCloudsP3 = 416, kMoonDirPhase = 448, kMoonExtra = 464, kSunExtra = 480;
+constexpr std::size_t kHdriLightsDir = 496, kHdriLightsCol = 624;
 constexpr std::size_t kHdriLightsCount = 752, kWriteNormalGbuffer = 756, kWriteAlbedoGbuffer = 760;
 constexpr std::size_t kWriteHdrAux = 764, kMisEnabled = 768, kAccumParams = 784, kSdfParams = 816;
 constexpr std::size_t kWriteShadowVis = 856, kLightCount = 880, kLightTreeEnabled = 888;
 constexpr std::size_t kWriteSpecularAlbedo = 896, kWriteRoughness = 900, kWriteSpecularHitDistance = 904;
 constexpr std::size_t kRestirEnabled = 944, kSmokeParams = 960, kMotionBlurParams = 1024, kCloudsRuntime = 1072;
 constexpr std::size_t kMeshMotionPrev = 1088, kMeshMotionCurr = 1104;
-constexpr std::size_t kMeshTexIndices = 1152, kMatOverrideCount = 1200, kTonemapParams = 1600;
+constexpr std::size_t kMeshTexIndices = 1152, kMatOverrideCount = 1200, kTonemapParams = 1600, kSunExtra2 = 1616;
 constexpr std::size_t kSize = 1792;            /* sizeof(PtPush) with kMaxHdriLights = kMaxMatOverrides = 8 */
 }  // namespace ptpush
 
@@ -173,6 +178,7 @@ struct PathtracePush {
     std::uint32_t light_count = 0;
     std::uint32_t env_map_present = 0;
     float env_intensity = 1.0f;
+    float env_total_luminance = 0.0f;   /* the engine's CDF total (used with its CDF buffers) */
     std::uint32_t tonemap_op = 0;   /* r_tonemap_op; values above 4 select ACES */
     std::uint32_t motion_blur = 0;  /* motion_blur_params.x > 0.5 (also frame.motion_blur) */
     /* denoiser inputs (PathTrace.slang:5463-5648, 7797-7811) */
@@ -191,13 +197,14 @@ struct PathtracePush {
  * reason in *error, for a push shorter than the 112-byte head (the software
  * tracer's no-op, SoftwareTracer.cpp:686-690) and, when strict, for a push that
  * turns on a feature this backend does not render: a procedural / Hosek sky
- * (sun_and_mode.w 2 or 3), the sun or moon above the horizon without an env
- * map, HDRI cluster lights, participating
- * media, SDF clusters, ReSTIR, material overrides, smoke, the
- * SIGMA shadow-visibility output, a per-frame denoiser colour input without
- * reset_accum, or PBR mesh textures.  Lenient decoding renders
- * without them and lists them in out->ignored.  Fields past the pushed size
- * read as zero. */
+ * (sun_and_mode.w 2 or 3), participating media, SDF clusters, ReSTIR,
+ * material overrides, smoke, the SIGMA shadow-visibility output, a
+ * per-frame denoiser colour input without reset_accum, or PBR mesh textures.
+ * Lenient decoding renders without them and lists them in out->ignored.  The
+ * gradient sky's sun and moon (sun_and_mode, sun_extra, sun_extra2,
+ * moon_dir_phase, moon_extra), the HDRI cluster lights (hdri_lights_*) and,
+ * while a denoiser runs, the Halton jitter are decoded into the frame.
+ * Fields past the pushed size read as zero. */
 bool DecodePathtracePush(const void* data, std::size_t size, bool strict, PathtracePush* out, std::string* error);
 
 struct HipDeviceOptions {

@@ -89,6 +89,21 @@ uint32_t dmt_build_light_tree(const float* lights, uint32_t light_count, float* 
  * returns the sin-weighted luminance total. */
 double dmt_build_env_cdf(const float* rgb, uint32_t width, uint32_t height, float* out_marginal,
                          float* out_conditional);
+/* The same with the pixels mask[] != 0 left out of the luminance (the
+ * engine's light_mask of extracted HDRI clusters, Engine.cpp:4559-4563). */
+double dmt_build_env_cdf_masked(const float* rgb, uint32_t width, uint32_t height, const uint8_t* mask,
+                                float* out_marginal, float* out_conditional);
+/* The engine's HDRI light extraction at env-map load (ReloadEnvMap,
+ * Engine.cpp:4369-4532): pixels at or above the top_frac luminance quantile
+ * (r_hdri_extract_percentile, 0.005 by default; floored at 5 % of the peak
+ * and at 0.5, none when the peak is <= 0.5), 4-connected with wrap-around
+ * columns, the 8 brightest components of >= 3 pixels as directional lights:
+ * dir[i] = (unit direction, pick probability), col[i] = (rgb flux, 0), as
+ * dmt_frame_params.hdri_dir / hdri_col take them; out_mask (width*height)
+ * marks their pixels for dmt_build_env_cdf_masked.  Returns the number of
+ * lights. */
+uint32_t dmt_env_extract_lights(const float* rgb, uint32_t width, uint32_t height, double top_frac,
+                                float dir[8][4], float col[8][4], uint8_t* out_mask);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,13 @@ typedef struct orc_scene_desc {
     const float*    env_rgb;    uint32_t env_width;  uint32_t env_height;
     float           env_intensity;
     uint32_t        reserved;
+    /* optional importance CDF of the env map (as the engine uploads it;
+     * NULL: built from env_rgb) */
+    const float*    env_marginal;
+    const float*    env_conditional;
+    double          env_total_luminance;
+    /* optional moon texture (RGBA floats) of the moon disc */
+    const float*    moon_rgba;  uint32_t moon_width; uint32_t moon_height;
 } orc_scene_desc;
 
 typedef struct orc_frame {
@@ -58,6 +65,19 @@ typedef struct orc_frame {
     uint32_t mesh_shift_active;
     float    mesh_shift_curr[3];
     uint32_t reserved;
+    /* Fixed jitter of sample 0 (flags bit 1) and the sky lights (same fields
+     * as dmt_frame_params): halton_jitter (PathTrace.slang:5730), the sun and
+     * moon of the gradient sky (:3112, :3212, :6546-6667), the HDRI cluster
+     * lights of an env map (:3679-3768, :6493-6518). */
+    float    jitter[2];
+    float    sun[4];
+    float    sun_extra[4];
+    float    sun_disc_brightness;
+    float    moon[4];
+    float    moon_extra[4];
+    uint32_t hdri_light_count;
+    float    hdri_dir[8][4];
+    float    hdri_col[8][4];
 } orc_frame;
 
 typedef struct orc_hit {
@@ -104,6 +124,14 @@ int orc_tangent(orc_scene* s, const orc_frame* f, uint32_t prim, uint32_t param,
                 int threads, double* out_dloss, float* out_dimage);
 /* The fp64 frame value (spp mean, sanitised) of every pixel, rgb doubles. */
 int orc_render_f64_image(orc_scene* s, const orc_frame* f, int threads, double* out_rgb);
+
+/* skyColor of a missed ray for n unit directions (3 floats each) under the
+ * frame's sky lights: the env map, or the gradient sky with the sun and moon
+ * discs; out: 3 floats each. */
+int orc_sky(const orc_scene* s, const orc_frame* f, const float* dirs, uint32_t n, float* out_rgb);
+/* sampleSunDisc (PathTraceMath.slang:55-67) around dir with n successive
+ * seeds seed0, seed0 + 1, ...: out 3 floats each. */
+void orc_sample_sun_disc(const float* dir, uint32_t seed0, uint32_t n, float* out);
 
 /* Closest-hit query (traceScene) for n rays: rays are 8 floats each
  * (ox, oy, oz, unused, dx, dy, dz, unused). */

@@ -23,6 +23,9 @@ class OrcSceneDesc(C.Structure):
         ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
         ("env_intensity", C.c_float),
         ("flags", C.c_uint32),
+        ("env_marginal", C.POINTER(C.c_float)), ("env_conditional", C.POINTER(C.c_float)),
+        ("env_total_luminance", C.c_double),
+        ("moon_rgba", C.POINTER(C.c_float)), ("moon_width", C.c_uint32), ("moon_height", C.c_uint32),
     ]
 
 
@@ -81,6 +84,10 @@ def lib():
         L.orc_trace_at.restype = C.c_int
         L.orc_lens_sample.argtypes = [fp, fp, u32, C.c_int, C.c_float, C.c_float, fp]
         L.orc_lens_sample.restype = C.c_int
+        L.orc_sky.argtypes = [vp, vp, fp, u32, fp]
+        L.orc_sky.restype = C.c_int
+        L.orc_sample_sun_disc.argtypes = [fp, u32, u32, fp]
+        L.orc_sample_sun_disc.restype = None
         _lib = L
     return _lib
 
@@ -94,7 +101,7 @@ def _fp(a):
 class OracleScene:
     """Oracle copy of a scene given in the reference wire formats."""
 
-    def __init__(self, prims, lights, positions, indices, env=None, env_intensity=1.0):
+    def __init__(self, prims, lights, positions, indices, env=None, env_intensity=1.0, env_cdf=None, moon=None):
         self._keep = [np.ascontiguousarray(prims, np.float32), np.ascontiguousarray(lights, np.float32),
                       np.ascontiguousarray(positions, np.float32), np.ascontiguousarray(indices, np.uint32)]
         d = OrcSceneDesc()
@@ -109,6 +116,15 @@ class OracleScene:
             self._keep.append(e)
             d.env_rgb, d.env_height, d.env_width = _fp(e), e.shape[0], e.shape[1]
         d.env_intensity = env_intensity
+        if env_cdf is not None:
+            m = np.ascontiguousarray(env_cdf[0], np.float32)
+            c = np.ascontiguousarray(env_cdf[1], np.float32)
+            self._keep += [m, c]
+            d.env_marginal, d.env_conditional, d.env_total_luminance = _fp(m), _fp(c), float(env_cdf[2])
+        if moon is not None:
+            mo = np.ascontiguousarray(moon, np.float32)
+            self._keep.append(mo)
+            d.moon_rgba, d.moon_height, d.moon_width = _fp(mo), mo.shape[0], mo.shape[1]
         self._h = C.c_void_p()
         rc = lib().orc_scene_create(C.byref(d), C.byref(self._h))
         if rc != 0:
@@ -116,7 +132,8 @@ class OracleScene:
 
     @classmethod
     def from_scene_data(cls, sd):
-        return cls(sd.prims, sd.lights, sd.positions, sd.indices, sd.env, sd.env_intensity)
+        return cls(sd.prims, sd.lights, sd.positions, sd.indices, sd.env, sd.env_intensity,
+                   getattr(sd, "env_cdf", None), getattr(sd, "moon", None))
 
     def render(self, frame, precision=32, threads=0, rows=None, col_step=1, accum=None):
         """Render one frame; `frame` is any ctypes struct with the dmt_frame_params layout.
@@ -198,6 +215,14 @@ class OracleScene:
             raise ValueError("oracle gbuffer failed")
         return out
 
+    def sky(self, frame, dirs):
+        """skyColor of missed rays along unit dirs (n, 3) under the frame's sky lights."""
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros_like(d)
+        if lib().orc_sky(self._h, C.cast(C.byref(frame), C.c_void_p), _fp(d), d.shape[0], _fp(out)) != 0:
+            raise ValueError("oracle sky failed")
+        return out
+
     def env_sample(self, xi):
         xi = np.ascontiguousarray(xi, np.float32)
         n = xi.shape[0]
@@ -260,7 +285,7 @@ def math_f32(fn, a, b=None):
     b = np.ascontiguousarray(b if b is not None else np.zeros_like(a), np.float32)
     out = np.zeros_like(a)
     code = {"sin": 0, "cos": 1, "acos": 2, "atan2": 3, "exp": 4, "spectral_r": 5, "spectral_g": 6,
-            "spectral_b": 7, "cauchy": 8}[fn]
+            "spectral_b": 7, "cauchy": 8, "log": 9, "pow": 10}[fn]
     lib().orc_math_f32(code, _fp(a), _fp(b), _fp(out), a.size)
     return out
 
@@ -276,4 +301,12 @@ def lens_sample(u, blades=0, rot=0.0, cat_eye=0.0, uv=None):
                                float(cat_eye), _fp(out))
     if rc != 0:
         raise ValueError("lens sample rejected")
+    return out
+
+
+def sample_sun_disc(direction, seed0, n):
+    """sampleSunDisc around the unit direction with seeds seed0 .. seed0 + n - 1."""
+    d = np.ascontiguousarray(direction, np.float32)
+    out = np.zeros((n, 3), np.float32)
+    lib().orc_sample_sun_disc(_fp(d), seed0, n, _fp(out))
     return out

@@ -122,6 +122,42 @@ template <class R> inline V3<R> quat_rotate(const R q[4], V3<R> v) {
 
 // ---- transcendental functions -------------------------------------------
 
+// Cephes logf (logf.c): x = m 2^e, m in [sqrt(1/2), sqrt(2)), the degree-9
+// minimax polynomial of log(1 + f), ln 2 in two parts; and pow(x, y) for
+// x >= 0 as exp(y log x) from it.
+inline float cephes_logf(float xin) {
+    if (!(xin > 0.0f)) return -INFINITY;
+    if (std::isinf(xin)) return xin;
+    uint32_t bits;
+    std::memcpy(&bits, &xin, 4);
+    int e = 0;
+    if (bits < 0x00800000u) {
+        xin = xin * 8388608.0f;
+        std::memcpy(&bits, &xin, 4);
+        e = -23;
+    }
+    e += (int)((bits >> 23) & 0xFFu) - 126;
+    const uint32_t mb = (bits & 0x007FFFFFu) | 0x3F000000u;
+    float x;
+    std::memcpy(&x, &mb, 4);
+    if (x < 0.707106781186547524f) {
+        e -= 1;
+        x = (x + x) - 1.0f;
+    } else {
+        x = x - 1.0f;
+    }
+    const float z = x * x;
+    float y = ((((((((7.0376836292e-2f * x - 1.1514610310e-1f) * x + 1.1676998740e-1f) * x - 1.2420140846e-1f) * x +
+                   1.4249322787e-1f) * x - 1.6668057665e-1f) * x + 2.0000714765e-1f) * x - 2.4999993993e-1f) * x +
+               3.3333331174e-1f) * x * z;
+    const float fe = (float)e;
+    y = y + -2.12194440e-4f * fe;
+    y = y + -0.5f * z;
+    float r = x + y;
+    r = r + 0.693359375f * fe;
+    return r;
+}
+
 struct MathF {
     // Cephes sinf/cosf: reduce |x| by multiples of pi/4 (DP1+DP2+DP3), select
     // octant, evaluate the sine or cosine minimax polynomial.
@@ -223,6 +259,11 @@ struct MathF {
         std::memcpy(&scale, &bits, 4);
         return p * scale;
     }
+    static float log(float x) { return cephes_logf(x); }
+    static float pow(float x, float y) {
+        if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : INFINITY);
+        return exp(y * log(x));
+    }
 };
 
 struct MathD {
@@ -231,6 +272,9 @@ struct MathD {
     static double acos(double x) { return std::acos(x); }
     static double atan2(double y, double x) { return std::atan2(y, x); }
     static double exp(double x) { return std::exp(x); }
+    static double asin(double x) { return std::asin(x); }
+    static double log(double x) { return std::log(x); }
+    static double pow(double x, double y) { return std::pow(x, y); }
 };
 
 struct MathDual {
@@ -247,6 +291,17 @@ struct MathDual {
     static Dual exp(Dual x) {
         const double e = std::exp(x.v);
         return {e, e * x.d};
+    }
+    static Dual asin(Dual x) {
+        const double s = std::sqrt(std::max(0.0, 1.0 - x.v * x.v));
+        return {std::asin(x.v), s > 0.0 ? x.d / s : 0.0};
+    }
+    static Dual log(Dual x) { return {std::log(x.v), x.v > 0.0 ? x.d / x.v : 0.0}; }
+    static Dual pow(Dual x, Dual y) {
+        const double p = std::pow(x.v, y.v);
+        const double dx = x.v > 0.0 ? y.v * std::pow(x.v, y.v - 1.0) * x.d : 0.0;
+        const double dy = x.v > 0.0 ? p * std::log(x.v) * y.d : 0.0;
+        return {p, dx + dy};
     }
 };
 

@@ -366,6 +366,8 @@ struct orc_scene {
     float env_intensity = 1.0f;
     std::vector<float> marg, cond;
     float env_total = 0.0f;
+    std::vector<float> moon;         // RGBA moon texture (empty: none)
+    uint32_t moon_w = 0, moon_h = 0;
     // tangent direction of a PT<Dual> run: material parameter k (albedo rgb,
     // roughness, emission rgb) of analytic prim tangent_prim
     uint32_t tangent_prim = 0xFFFFFFFFu;
@@ -623,13 +625,172 @@ template <class R> struct PT {
         return V3<R>{R(p[0]), R(p[1]), R(p[2])};
     }
 
+    // ---- sky lights: the frame's sun and moon (gradient sky) ----------------
+    static R sat(R x) { return clampv(x, R(0), R(1)); }
+    V3<R> sun_dir() const { return V3<R>{R(F.sun[0]), R(F.sun[1]), R(F.sun[2])}; }
+    V3<R> moon_dir() const { return V3<R>{R(F.moon[0]), R(F.moon[1]), R(F.moon[2])}; }
+
+    // sampleSunDisc (PathTraceMath.slang:55-67).
+    V3<R> sample_sun_disc(V3<R> dir, uint32_t& seed) const {
+        R u1 = randf<R>(seed);
+        R u2 = randf<R>(seed);
+        R r = osqrt(u1) * R(0.00465);
+        R phi = R(6.28318530718) * u2;
+        R dx = M::cos(phi) * r, dy = M::sin(phi) * r;
+        V3<R> ax = oabs(dir.y) < R(0.99) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
+        V3<R> t = normalize(cross(ax, dir));
+        V3<R> b = cross(dir, t);
+        return normalize((dir + t * dx) + b * dy);
+    }
+    // sun_rad of the sun NEE (PathTrace.slang:6577-6600).
+    static V3<R> sun_radiance(R elev) {
+        R am = R(1) / std::max(elev, R(0.04));
+        R trans_atm = M::exp(R(-0.30) * am);
+        R warmth = sat((am - R(1)) * R(0.18));
+        V3<R> col = lerp3(V3<R>{R(1.00), R(0.97), R(0.92)}, V3<R>{R(1.55), R(0.55), R(0.20)}, warmth);
+        return col * (R(80) * trans_atm);
+    }
+    // moon_rad of the moon NEE (PathTrace.slang:6636-6654).
+    static V3<R> moon_radiance(R elev, R phase_b) {
+        R m_am = R(1) / std::max(elev, R(0.04));
+        R m_atm = M::exp(R(-0.30) * m_am);
+        R m_warmth = sat((m_am - R(1)) * R(0.18));
+        V3<R> m_col = lerp3(V3<R>{R(0.92), R(0.94), R(1.00)}, V3<R>{R(1.20), R(0.65), R(0.40)}, m_warmth);
+        return m_col * ((R(0.3) * phase_b) * m_atm);
+    }
+    // sunDisc (PathTrace.slang:3112-3192).
+    V3<R> sun_disc(V3<R> rd) const {
+        const V3<R> zero{R(0), R(0), R(0)};
+        V3<R> sd = sun_dir();
+        if (sd.y < R(-0.10)) return zero;
+        R ang = dot(rd, sd);
+        R sun_mult = std::max(R(F.sun_extra[0]) * R(F.sun_extra[1]), R(0.1));
+        R r_deg = R(0.55) * sun_mult;
+        R cos_r = M::cos(r_deg * R(0.0174533));
+        R hz = sat(R(1) - sd.y);
+        R halo_deg = R(2) + hz * (R(8) - R(2));
+        R cos_halo = M::cos(halo_deg * R(0.0174533));
+        if (ang <= cos_halo) return zero;
+        R vflat = R(F.sun_extra[2]);
+        bool inside = false;
+        if (ang > cos_r) {
+            if (!(vflat < R(0.999))) {
+                inside = true;
+            } else {
+                V3<R> lr = normalize(cross(V3<R>{R(0), R(1), R(0)}, sd));
+                V3<R> lu = cross(sd, lr);
+                V3<R> delta = rd - sd * ang;
+                R h = dot(delta, lr), v = dot(delta, lu);
+                R rh = r_deg * R(0.0174533);
+                R rv = rh * vflat;
+                inside = (h * h) / (rh * rh) + (v * v) / (rv * rv) <= R(1);
+            }
+        }
+        R am = R(1) / std::max(sd.y, R(0.04));
+        R atm = M::exp(R(-0.30) * am);
+        R warmth = sat((am - R(1)) * R(0.18));
+        V3<R> col = lerp3(V3<R>{R(1.00), R(0.98), R(0.94)}, V3<R>{R(1.55), R(0.55), R(0.20)}, warmth);
+        R t_h = (ang - cos_halo) / (R(1) - cos_halo);
+        R falloff = R(8) + hz * (R(3) - R(8));
+        R halo = M::pow(t_h, falloff);
+        R bright = std::max(R(F.sun_disc_brightness), R(0));
+        V3<R> halo_col = ((col * halo) * atm) * R(4);
+        if (inside) return (col * (R(250) * atm) + halo_col) * bright;
+        return halo_col * bright;
+    }
+    V3<R> moon_texel(int x, int y) const {
+        if (S.moon.empty()) return V3<R>{R(0), R(0), R(0)};
+        const float* p = &S.moon[((size_t)y * S.moon_w + x) * 4];
+        return V3<R>{R(p[0]), R(p[1]), R(p[2])};
+    }
+    // moonDisc (PathTrace.slang:3212-3353).
+    V3<R> moon_disc(V3<R> rd) const {
+        const V3<R> zero{R(0), R(0), R(0)};
+        V3<R> md = moon_dir();
+        if (md.y < R(-0.10)) return zero;
+        R ang = dot(rd, md);
+        R size_mult = std::max(R(F.moon_extra[0]) * R(F.moon_extra[1]), R(0.1));
+        R r_deg = R(0.55) * size_mult;
+        R halo_deg = R(1.5) * size_mult;
+        R cos_r = M::cos(r_deg * R(0.0174533));
+        R cos_halo = M::cos(halo_deg * R(0.0174533));
+        if (ang <= cos_halo) return zero;
+        V3<R> tint{R(0.92), R(0.94), R(1.00)};
+        R m_am = R(1) / std::max(md.y, R(0.04));
+        R m_atm = M::exp(R(-0.30) * m_am);
+        R warmth = sat((m_am - R(1)) * R(0.18));
+        V3<R> col_atm = lerp3(V3<R>{R(1), R(1), R(1)}, V3<R>{R(1.20), R(0.65), R(0.40)}, warmth);
+        R t_h = (ang - cos_halo) / (R(1) - cos_halo);
+        R halo = M::pow(t_h, R(4)) * R(0.6);
+        V3<R> tc = tint * col_atm;
+        if (ang > cos_r) {
+            R t_d = (ang - cos_r) / (R(1) - cos_r);
+            V3<R> ax = oabs(md.y) < R(0.99) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
+            V3<R> ta = normalize(cross(ax, md));
+            V3<R> tb = cross(md, ta);
+            R disc_scale = R(1) / std::max(M::sin(r_deg * R(0.0174533)), R(1e-5));
+            R du = dot(rd, ta) * disc_scale;
+            R dv = dot(rd, tb) * disc_scale;
+            R r2 = du * du + dv * dv;
+            R zz = osqrt(std::max(R(1) - r2, R(0)));
+            V3<R> n_world = (ta * du + tb * dv) - md * zz;
+            R lit = sat(dot(n_world, sun_dir()) * R(10) + R(0.05));
+            if (lit <= R(0)) return ((tc * halo) * m_atm) * R(0.05);
+            R anti_sun_ang = M::acos(sat(-dot(md, sun_dir())));
+            R et = sat((anti_sun_ang - R(0.0122)) / (R(0.0218) - R(0.0122)));
+            R eclipse = R(1) - (et * et) * (R(3) - R(2) * et);
+            R lon = M::atan2(du, zz);
+            R lat = M::asin(clampv(dv, R(-1), R(1)));
+            R u_tex = sat(R(0.5) + lon / R(6.28318530718));
+            R v_tex = sat(R(0.5) - lat / R(3.14159265358979));
+            V3<R> surface{R(0), R(0), R(0)};
+            if (!S.moon.empty()) {
+                const int W = (int)S.moon_w, H = (int)S.moon_h;
+                R fx = u_tex * R(W) - R(0.5), fy = v_tex * R(H) - R(0.5);
+                int ix = (int)ofloor(fx), iy = (int)ofloor(fy);
+                R ax2 = fx - R(ix), ay2 = fy - R(iy);
+                int x0 = std::min(std::max(ix, 0), W - 1), x1 = std::min(std::max(ix + 1, 0), W - 1);
+                int y0 = std::min(std::max(iy, 0), H - 1), y1 = std::min(std::max(iy + 1, 0), H - 1);
+                surface = lerp3(lerp3(moon_texel(x0, y0), moon_texel(x1, y0), ax2),
+                                lerp3(moon_texel(x0, y1), moon_texel(x1, y1), ax2), ay2);
+            }
+            R limb = R(0.55) + R(0.45) * osqrt(sat(t_d));
+            R bright_mult = std::max(R(F.moon_extra[2]), R(0));
+            R disc_b = R(3.5) * bright_mult;
+            V3<R> lit_col = (tc * surface) * (((disc_b * lit) * limb) * m_atm) +
+                            (((tc * halo) * lit) * m_atm) * (R(0.6) * bright_mult);
+            V3<R> eclipse_col = (V3<R>{R(0.55), R(0.15), R(0.04)} * surface) * ((limb * m_atm) * R(1.2));
+            return lerp3(lit_col, eclipse_col, eclipse);
+        }
+        R phase_b = R(0.5) * (R(1) - M::cos(R(F.moon[3])));
+        return (((tc * halo) * phase_b) * m_atm) * R(0.3);
+    }
+
+    // isExtractedClusterDir (PathTrace.slang:3679-3696).
+    bool is_cluster_dir(V3<R> d) const {
+        if (!S.env_present || F.hdri_light_count == 0u) return false;
+        R th = M::acos(clampv(d.y, R(-1), R(1)));
+        R st = M::sin(th);
+        if (st < R(1e-6)) return false;
+        R ph = M::atan2(d.z, d.x);
+        uint32_t W = S.W, H = S.H;
+        R u = (ph + PI) * (R(1) / R(6.28318530717958));
+        R v = th * (R(1) / PI);
+        uint32_t ui = std::min((uint32_t)(u * R(W)), W - 1u);
+        uint32_t vi = std::min((uint32_t)(v * R(H)), H - 1u);
+        const float* row = S.cond.data() + (size_t)vi * W;
+        float c_prev = ui > 0u ? row[ui - 1u] : 0.0f;
+        return row[ui] <= c_prev;
+    }
+
     // skyColor (PathTrace.slang:3373-3431).  Without an env map: the base
-    // gradient, with the sun and moon below the horizon (y < -0.1), where
-    // sunDisc/moonDisc return 0 (PathTrace.slang:3113, 3213).
+    // gradient plus the moon and sun discs (zero below y = -0.1,
+    // PathTrace.slang:3113, 3213, and for no sun or moon).
     V3<R> sky(V3<R> rd) const {
         if (!S.env_present) {
             R t = clampv(R(0.5) * (rd.y + R(1)), R(0), R(1));
-            return lerp3(V3<R>{R(0.5), R(0.55), R(0.62)}, V3<R>{R(1), R(1), R(1)}, t);
+            V3<R> base = lerp3(V3<R>{R(0.5), R(0.55), R(0.62)}, V3<R>{R(1), R(1), R(1)}, t);
+            return (base + moon_disc(rd)) + sun_disc(rd);
         }
         int W = (int)S.W, H = (int)S.H;
         R u = M::atan2(rd.z, rd.x) * (R(1) / R(6.28318530718)) + R(0.5);
@@ -966,10 +1127,16 @@ template <class R> struct PT {
         V3<R> fwd{R(F.fwd_aspect[0]), R(F.fwd_aspect[1]), R(F.fwd_aspect[2])};
         V3<R> rt{R(F.right_xyz[0]), R(F.right_xyz[1]), R(F.right_xyz[2])};
         V3<R> up{R(F.up_xyz[0]), R(F.up_xyz[1]), R(F.up_xyz[2])};
-        R jx = randf<R>(seed);
-        R jy = randf<R>(seed);
-        jx = jx - R(0.5);
-        jy = jy - R(0.5);
+        R jx, jy;
+        if ((F.flags & 2u) && s == 0u) {   // the denoiser's fixed offset (PathTrace.slang:5730-5731)
+            jx = R(F.jitter[0]);
+            jy = R(F.jitter[1]);
+        } else {
+            jx = randf<R>(seed);
+            jy = randf<R>(seed);
+            jx = jx - R(0.5);
+            jy = jy - R(0.5);
+        }
         R ux = (((R(x) + R(0.5)) + jx) / w) * R(2) - R(1);
         R uy = (((R(y) + R(0.5)) + jy) / hgt) * R(2) - R(1);
         uy = -uy;
@@ -1011,11 +1178,14 @@ template <class R> struct PT {
             Hit<R> h = trace(ro, rd, time);
             if (!h.hit) {
                 if (mis_on && prev_lambert) {
+                    // cluster directions are the clusters' own NEE's (:6074);
                     // transmittance(ro, rd, 1e30) re-traces this same missed
                     // ray and returns exactly 1: not re-traced, not counted.
-                    R pe = pdf_env(rd);
-                    R w = mis(prev_pdf, pe);
-                    L = L + clamp_indirect((T * sky(rd)) * w);
+                    if (!is_cluster_dir(rd)) {
+                        R pe = pdf_env(rd);
+                        R w = mis(prev_pdf, pe);
+                        L = L + clamp_indirect((T * sky(rd)) * w);
+                    }
                 } else if (!skip_sky) {
                     if (b == 0) L = L + T * sky(rd);
                     else L = L + clamp_indirect(T * sky(rd));
@@ -1059,9 +1229,61 @@ template <class R> struct PT {
                             L = L + clamp_indirect(cc);
                         }
                     }
+                    if (F.hdri_light_count > 0u) {
+                        // sampleHdriLight (:3749-3768) and its NEE (:6503-6518)
+                        R u = randf<R>(seed);
+                        R acc = R(0);
+                        uint32_t sel = 0u;
+                        for (uint32_t i = 0; i < 8u; ++i) {
+                            if (i >= F.hdri_light_count) break;
+                            acc = acc + R(F.hdri_dir[i][3]);
+                            sel = i;
+                            if (u <= acc) break;
+                        }
+                        V3<R> hd = ld3<R>(F.hdri_dir[sel]);
+                        V3<R> hc = ld3<R>(F.hdri_col[sel]) * R(S.env_intensity);
+                        R hpdf = std::max(R(F.hdri_dir[sel][3]), R(1e-6));
+                        R ndl = dot(nf, hd);
+                        if (ndl > R(0) && hpdf > R(0)) {
+                            V3<R> tr = transmittance(hp + nf * R(1e-3), hd, R(1e30), time);
+                            if (any_pos(tr)) {
+                                V3<R> brdf = h.albedo / PI;
+                                V3<R> cc = ((((T * hc) * tr) * brdf) * ndl) / hpdf;
+                                L = L + ((b == 0) ? cc : clamp_indirect(cc));
+                            }
+                        }
+                    }
                     skip_sky = !mis_on;
                 } else {
-                    // No env map: sun and moon below the horizon, base sky gradient.
+                    // No env map: the sun's and the moon's NEE when they are up
+                    // (:6552-6660), unclamped, then the base sky as ambient.
+                    V3<R> sd = sun_dir();
+                    if (sd.y > R(0)) {
+                        R ndl = std::max(R(0), dot(nf, sd));
+                        V3<R> jit = sample_sun_disc(sd, seed);
+                        if (ndl > R(0)) {
+                            V3<R> tr = transmittance(hp + nf * R(1e-3), jit, R(1e30), time);
+                            if (any_pos(tr)) {
+                                V3<R> brdf = h.albedo / PI;
+                                L = L + (((T * sun_radiance(sd.y)) * tr) * brdf) * ndl;
+                            }
+                        }
+                    }
+                    V3<R> md = moon_dir();
+                    if (md.y > R(0)) {
+                        R phase_b = R(0.5) * (R(1) - M::cos(R(F.moon[3])));
+                        if (phase_b > R(0.01)) {
+                            V3<R> jit = sample_sun_disc(md, seed);
+                            R ndl = std::max(R(0), dot(nf, jit));
+                            if (ndl > R(0)) {
+                                V3<R> tr = transmittance(hp + nf * R(1e-3), jit, R(1e30), time);
+                                if (any_pos(tr)) {
+                                    V3<R> brdf = h.albedo / PI;
+                                    L = L + (((T * moon_radiance(md.y, phase_b)) * tr) * brdf) * ndl;
+                                }
+                            }
+                        }
+                    }
                     R t = clampv(R(0.5) * (nf.y + R(1)), R(0), R(1));
                     V3<R> a{R(0.5), R(0.55), R(0.62)}, one{R(1), R(1), R(1)};
                     L = L + (T * h.albedo) * lerp3(a, one, t);
@@ -1283,13 +1505,48 @@ int orc_scene_create(const orc_scene_desc* d, orc_scene** out) {
         S->H = d->env_height;
         S->env.assign(d->env_rgb, d->env_rgb + (size_t)S->W * S->H * 3);
         S->env_intensity = d->env_intensity;
-        S->env_total = float(build_env_cdf(S->env.data(), S->W, S->H, S->marg, S->cond));
+        if (d->env_marginal && d->env_conditional) {   // the engine's upload
+            S->marg.assign(d->env_marginal, d->env_marginal + S->H);
+            S->cond.assign(d->env_conditional, d->env_conditional + (size_t)S->W * S->H);
+            S->env_total = float(d->env_total_luminance);
+        } else {
+            S->env_total = float(build_env_cdf(S->env.data(), S->W, S->H, S->marg, S->cond));
+        }
+    }
+    if (d->moon_rgba && d->moon_width && d->moon_height) {
+        S->moon_w = d->moon_width;
+        S->moon_h = d->moon_height;
+        S->moon.assign(d->moon_rgba, d->moon_rgba + (size_t)S->moon_w * S->moon_h * 4);
     }
     *out = S;
     return 0;
 }
 
 void orc_scene_destroy(orc_scene* s) { delete s; }
+
+int orc_sky(const orc_scene* s, const orc_frame* f, const float* dirs, uint32_t n, float* out) {
+    if (!s || !f || (n && (!dirs || !out))) return -1;
+    Ctr c;
+    PT<float> pt{*s, *f, c};
+    for (uint32_t i = 0; i < n; ++i) {
+        V3<float> v = pt.sky(V3<float>{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]});
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+    return 0;
+}
+
+void orc_sample_sun_disc(const float* dir, uint32_t seed0, uint32_t n, float* out) {
+    orc_frame f{};
+    orc_scene sc;
+    Ctr c;
+    PT<float> pt{sc, f, c};
+    const V3<float> d{dir[0], dir[1], dir[2]};
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t seed = seed0 + i;
+        V3<float> v = pt.sample_sun_disc(d, seed);
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+}
 
 int orc_render(const orc_scene* s, const orc_frame* f, int precision, int threads, uint32_t row_begin,
                uint32_t row_end, uint32_t row_step, uint32_t col_step, float* accum, orc_counts* counts) {
@@ -1569,6 +1826,8 @@ void orc_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n
             case 6: out[i] = PT<float>::spectral_weight(a[i]).y; break;
             case 7: out[i] = PT<float>::spectral_weight(a[i]).z; break;
             case 8: out[i] = PT<float>::cauchy_ior(b[i], 0.0042f, a[i]); break;
+            case 9: out[i] = MathF::log(a[i]); break;
+            case 10: out[i] = MathF::pow(a[i], b[i]); break;
             default: out[i] = 0.0f;
         }
     }

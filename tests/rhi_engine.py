@@ -36,7 +36,18 @@ OFF = dict(pos_fovtan=0, fwd_aspect=16, right_xyz=32, up_xyz=48, frame_index=64,
            write_specular_hit_distance_gbuffer=904, restir_enabled=944, smoke_params=960,
            motion_blur_params=1024, mesh_motion_prev=1088, mesh_motion_curr=1104, mesh_tex_indices=1152,
            hosek_params=1184, mat_override_count=1200,
-           tonemap_params=1600, sun_extra2=1616)
+           tonemap_params=1600, sun_extra2=1616, moon_extra=464, sun_extra=480, hdri_lights_dir=496,
+           hdri_lights_col=624)
+
+
+def _offset(name: str) -> int:
+    """OFF[name], or hdri_dir<i> / hdri_col<i>: entry i of hdri_lights_dir / _col."""
+    if name in OFF:
+        return OFF[name]
+    for short, field in (("hdri_dir", "hdri_lights_dir"), ("hdri_col", "hdri_lights_col")):
+        if name.startswith(short) and name[len(short):].isdigit():
+            return OFF[field] + 16 * int(name[len(short):])
+    raise KeyError(name)
 
 
 def pack_ptpush(frame: "dmt.FrameParams", *, prim_count: int, light_count: int, env_present: bool,
@@ -46,7 +57,7 @@ def pack_ptpush(frame: "dmt.FrameParams", *, prim_count: int, light_count: int, 
     below the horizon (the gradient sky without discs) when there is no env map.
     overrides: {field: (struct format, value or tuple)} applied last."""
     b = bytearray(PTPUSH_SIZE)
-    put = lambda name, fmt, *v: struct.pack_into("<" + fmt, b, OFF[name], *v)
+    put = lambda name, fmt, *v: struct.pack_into("<" + fmt, b, _offset(name), *v)
     put("pos_fovtan", "4f", *frame.pos_fovtan)
     put("fwd_aspect", "4f", *frame.fwd_aspect)
     put("right_xyz", "4f", *frame.right_xyz)
@@ -129,8 +140,10 @@ class EngineSim:
                            env_present=self.env_present, env_intensity=self.sd.env_intensity,
                            tlas_present=bool(self.tlas), **kw)
 
-    def frame(self, frame: "dmt.FrameParams", push: bytes | None = None, textures: dict | None = None, **kw):
-        """One engine frame; `textures` binds further storage textures {slot: handle}."""
+    def frame(self, frame: "dmt.FrameParams", push: bytes | None = None, textures: dict | None = None,
+              buffers: dict | None = None, **kw):
+        """One engine frame; `textures` / `buffers` bind further storage
+        textures and buffers {slot: handle}."""
         d = self.dev
         out, w, h, _ = d.begin_frame()
         d.bind_pipeline(self.pipeline)
@@ -146,6 +159,8 @@ class EngineSim:
         d.bind_buffer(13, self.tree)
         for slot, tex in (textures or {}).items():
             d.bind_texture(slot, tex)
+        for slot, buf in (buffers or {}).items():
+            d.bind_buffer(slot, buf)
         d.push_constants(push if push is not None else self.push(frame, **kw))
         d.dispatch((w + 7) // 8, (h + 7) // 8, 1)
         d.submit()

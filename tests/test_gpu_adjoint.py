@@ -82,11 +82,12 @@ def test_adjoint_matches_fp64_tangent_oracle(device):
     assert all(ok)
 
 
-def test_adjoint_with_camera_sample_features(device):
+@pytest.mark.parametrize("fixed_jitter", [False, True], ids=["drawn jitter", "fixed jitter"])
+def test_adjoint_with_camera_sample_features(device, fixed_jitter):
     """Thin lens, motion blur of moving spheres and hero-wavelength dispersion
-    replay as they render: the gradients match the fp64 tangent oracle run on
-    the same frame (a spectral sample's gradient carries its wavelength's
-    weight)."""
+    (and the denoiser's fixed first-sample jitter) replay as they render: the
+    gradients match the fp64 tangent oracle run on the same frame (a spectral
+    sample's gradient carries its wavelength's weight)."""
     sd = dmt.generate(4, seed=6, spheres=40, width=80, height=56)
     prims = sd.prims.copy()
     rng = np.random.default_rng(12)
@@ -96,6 +97,9 @@ def test_adjoint_with_camera_sample_features(device):
     f = sd.frame.copy()
     f.spp = 4
     f.flags = dmt.DMT_FRAME_NO_RUSSIAN_ROULETTE
+    if fixed_jitter:
+        f.flags |= dmt.DMT_FRAME_FIXED_JITTER
+        f.jitter[0], f.jitter[1] = 0.1875, -0.3125
     f.reset_accum = 1
     f.dof = (0.15, 11.0, 6.0, 0.2)
     f.cat_eye = 0.3
@@ -275,7 +279,7 @@ def test_adjoint_limits_and_errors(device):
         sc.adjoint(f, d.data_ptr(), g.data_ptr())
     assert e.value.status == dmt.DMT_ERR_LIMIT
     f = sd.frame.copy()
-    f.flags = 2
+    f.flags = 4   # no such flag
     with pytest.raises(dmt.DmtError) as e:
         sc.adjoint(f, d.data_ptr(), g.data_ptr())
     assert e.value.status == dmt.DMT_ERR_INVALID_ARG

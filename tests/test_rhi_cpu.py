@@ -70,7 +70,7 @@ def test_decode_refract_budget_and_light_tree_gate():
 GATES = [
     ("sun_and_mode", "4f", (0.0, -0.5, -0.86, 2.0), "procedural sky"),
     ("sun_and_mode", "4f", (0.0, -0.5, -0.86, 3.0), "Hosek-Wilkie"),
-    ("hdri_lights_count", "I", 2, "HDRI cluster lights"),
+    ("hdri_lights_count", "I", 9, "more than 8 HDRI cluster lights"),
     ("vol_params", "4f", (0.01, 0.0, 1.0, 8.0), "participating media"),
     ("clouds_p1", "4f", (100.0, 200.0, 0.5, 0.2), "participating media"),
     ("clouds_p3", "4f", (0.0, 0.0, 0.0, 1.0), "participating media"),
@@ -143,29 +143,44 @@ def test_decode_reads_the_denoiser_fields():
     assert rhi.decode_pathtrace_push(push)["write_hdr_aux"] == 1
 
 
-def test_sky_without_env_map_needs_sun_and_moon_below_the_horizon():
+def test_decode_reads_the_sky_lights_and_the_denoiser_jitter():
+    """sun_and_mode.xyz 240, sun_extra 480, sun_extra2.x 1616, moon_dir_phase
+    448, moon_extra 464 (the gradient sky's sun and moon, PathTrace.slang:3112,
+    3212, 6546-6667); hdri_lights_dir 496, hdri_lights_col 624, count 752 (the
+    env map's cluster lights, :6503); halton_jitter 96 while the denoiser runs
+    (:5730).  None of them is refused."""
     f = _frame()
-    below = pack_ptpush(f, prim_count=1, light_count=1, env_present=False)
-    assert rhi.decode_pathtrace_push(below)["env_map_present"] == 0
-    # sunDisc / moonDisc vanish below y = -0.1 and sun / moon NEE below y = 0
-    # (PathTrace.slang:3113, 3213, 6555, 6626): -0.1 itself is refused
-    for field, y, what in (("sun_and_mode", -0.1, "sun above"), ("sun_and_mode", 0.3, "sun above"),
-                           ("moon_dir_phase", -0.1, "moon above"), ("moon_dir_phase", 0.9, "moon above")):
-        push = pack_ptpush(f, prim_count=1, light_count=1, env_present=False,
-                           overrides={field: ("4f", (0.0, y, 0.5, 0.0 if field == "sun_and_mode" else 1.0))})
-        with pytest.raises(ValueError, match=what):
-            rhi.decode_pathtrace_push(push)
-    # with the env map the HDRI is the sky: the sun's position does not matter
-    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True,
-                       overrides={"sun_and_mode": ("4f", (0.0, 0.9, 0.4, 1.0))})
-    assert rhi.decode_pathtrace_push(push)["env_map_present"] == 1
-    # HDRI cluster lights only count with an env map (PathTrace.slang:3680)
-    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=False, overrides={"hdri_lights_count": ("I", 4)})
-    assert rhi.decode_pathtrace_push(push)["ignored"] == ""
-    # PBR mesh tiles only count with a mesh
-    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, tlas_present=False,
-                       overrides={"mesh_tex_indices": ("4I", (0, 0, 0, 0))})
-    assert rhi.decode_pathtrace_push(push)["ignored"] == ""
+    sun, moon = (0.3, 0.8, -0.52, 0.0), (-0.4, 0.6, 0.69, 2.1)
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=False,
+                       overrides={"sun_and_mode": ("4f", sun[:3] + (1.0,)), "sun_extra": ("4f", (1.5, 1.01, 0.9, 6.4e6)),
+                                  "sun_extra2": ("4f", (0.7, 0.0, 0.0, 0.0)), "moon_dir_phase": ("4f", moon),
+                                  "moon_extra": ("4f", (2.0, 0.97, 0.8, 1.2))})
+    d = rhi.decode_pathtrace_push(push)
+    assert d["ignored"] == "" and d["env_map_present"] == 0
+    assert np.allclose(d["sun"], sun) and np.allclose(d["moon"], moon)
+    assert np.allclose(d["sun_extra"], (1.5, 1.01, 0.9, 0.0)) and np.allclose(d["moon_extra"], (2.0, 0.97, 0.8, 0.0))
+    assert abs(d["sun_disc_brightness"] - 0.7) < 1e-7
+    # cluster lights count with the env map only (PathTrace.slang:3680)
+    dirs = [(0.0, 1.0, 0.0, 0.25), (0.6, 0.8, 0.0, 0.75)]
+    cols = [(10.0, 9.0, 8.0, 0.0), (1.0, 2.0, 3.0, 0.0)]
+    ov = {"hdri_lights_count": ("I", 2), "env_total_luminance": ("f", 12.5)}
+    for i in range(2):
+        ov[f"hdri_dir{i}"] = ("4f", dirs[i])
+        ov[f"hdri_col{i}"] = ("4f", cols[i])
+    d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides=ov))
+    assert d["hdri_light_count"] == 2 and d["ignored"] == "" and d["env_total_luminance"] == 12.5
+    assert np.allclose(np.reshape(d["hdri_dir"], (8, 4))[:2], dirs)
+    assert np.allclose(np.reshape(d["hdri_col"], (8, 4))[:2], cols)
+    assert not np.any(np.reshape(d["hdri_dir"], (8, 4))[2:])
+    d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=False, overrides=ov))
+    assert d["hdri_light_count"] == 0
+    # the denoiser's Halton offset replaces sample 0's jitter
+    ov = {"denoiser_enabled": ("I", 1), "halton_jitter": ("2f", (0.25, -0.125))}
+    d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides=ov))
+    assert d["flags"] & 2 and d["jitter"] == [0.25, -0.125]
+    d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=True,
+                                              overrides={"halton_jitter": ("2f", (0.25, -0.125))}))
+    assert not d["flags"] & 2
 
 
 def test_short_push_is_refused_and_missing_fields_read_as_zero():
@@ -178,10 +193,10 @@ def test_short_push_is_refused_and_missing_fields_read_as_zero():
     assert (d["prim_count"], d["spp"], d["env_map_present"]) == (9, 3, 1)
     assert (d["light_count"], d["mis_enabled"], d["light_tree_enabled"], d["refract_bounces"]) == (0, 0, 0, 0)
     assert (d["firefly_clamp"], d["ema_alpha"], d["tonemap_op"]) == (0.0, 0.0, 0)
-    # a head-only push without the env map leaves the sun at y = 0: refused
+    # a head-only push without the env map has no sun or moon (zero directions)
     noenv = pack_ptpush(f, prim_count=9, light_count=1, env_present=False)
-    with pytest.raises(ValueError, match="sun above"):
-        rhi.decode_pathtrace_push(noenv[:112])
+    d = rhi.decode_pathtrace_push(noenv[:112])
+    assert d["sun"] == [0.0] * 4 and d["moon"] == [0.0] * 4 and d["ignored"] == ""
     # bytes past the 2048-byte push budget are dropped, not read
     assert rhi.decode_pathtrace_push(full + bytes(4096))["prim_count"] == 9
 
