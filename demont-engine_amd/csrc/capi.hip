@@ -359,6 +359,20 @@ dmt_status check_frame(const dmt_frame_params* f) {
     return DMT_OK;
 }
 
+// check_frame, and the frame's fields that the scene makes read: the SDF
+// sphere-trace parameters of a scene with clusters.
+dmt_status check_frame_for(const dmt_scene* s, const dmt_frame_params* f) {
+    dmt_status st = check_frame(f);
+    if (st) return st;
+    if (s->view.sdf_count) {
+        if (f->sdf_max_iters < 1u || f->sdf_max_iters > 256u)
+            return fail(DMT_ERR_INVALID_ARG, "sdf_max_iters must be in 1..256 for a scene with SDF clusters");
+        if (!(std::isfinite(f->sdf_epsilon) && f->sdf_epsilon > 0.0f))
+            return fail(DMT_ERR_INVALID_ARG, "sdf_epsilon must be finite and > 0 for a scene with SDF clusters");
+    }
+    return DMT_OK;
+}
+
 FrameArgs frame_args(const dmt_frame_params* f) {
     FrameArgs F = make_frame_args(f->pos_fovtan, f->fwd_aspect, f->right_xyz, f->up_xyz, f->width, f->height,
                                   f->frame_index, f->spp, f->max_bounces, f->reset_accum, f->mis_enabled,
@@ -377,12 +391,17 @@ bool sky_lights(const dmt_scene* s, const dmt_frame_params* f) {
     return up(f->sun) || up(f->moon);
 }
 
+// Sky lights, or SDF clusters in the scene: the frame runs the kernel
+// variant with their code (and the four shadow-record kinds).
+bool ext_path(const dmt_scene* s, const dmt_frame_params* f) { return sky_lights(s, f) || s->view.sdf_count > 0u; }
+
 // A frame that draws other numbers than the jitter per sample (lens point,
 // shutter time, hero wavelength), takes a fixed jitter, translates the mesh
-// or has sky lights renders through k_tail<COUNT, true>.
+// or has sky lights, and any frame of a scene with SDF clusters, renders
+// through k_tail<COUNT, true>: its traversal is the one that sees them.
 bool uses_features(const dmt_scene* s, const dmt_frame_params* f) {
     return f->dof[0] > 0.0f || f->motion_blur || f->spectral || f->mesh_shift_active ||
-           (f->flags & DMT_FRAME_FIXED_JITTER) || sky_lights(s, f);
+           (f->flags & DMT_FRAME_FIXED_JITTER) || sky_lights(s, f) || s->view.sdf_count > 0u;
 }
 
 FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
@@ -390,8 +409,8 @@ FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
     X.fixed_jitter = (f->flags & DMT_FRAME_FIXED_JITTER) ? 1u : 0u;
     X.jit_x = f->jitter[0];
     X.jit_y = f->jitter[1];
+    X.ext = ext_path(s, f) ? 1u : 0u;
     if (sky_lights(s, f)) {
-        X.sky = 1u;
         SkyArgs& K = X.sky_args;
         K.sun = f3{f->sun[0], f->sun[1], f->sun[2]};
         K.sun_size = f->sun_extra[0];
@@ -422,9 +441,12 @@ FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
     return X;
 }
 
-// The scene as a frame sees it: the mesh translation is per frame.
+// The scene as a frame sees it: the mesh translation and the SDF sphere-trace
+// parameters are per frame.
 SceneView frame_view(const dmt_scene* s, const dmt_frame_params* f) {
     SceneView V = s->view;
+    V.sdf_iters = f->sdf_max_iters;
+    V.sdf_eps = f->sdf_epsilon;
     V.mesh_shift = f->mesh_shift_active;
     V.mesh_prev = f3{f->mesh_shift_prev[0], f->mesh_shift_prev[1], f->mesh_shift_prev[2]};
     V.mesh_curr = f3{f->mesh_shift_curr[0], f->mesh_shift_curr[1], f->mesh_shift_curr[2]};
@@ -540,6 +562,11 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
                             desc->vertex_count, desc->indices, desc->index_count, desc->env_rgb, desc->env_width,
                             desc->env_height, desc->env_intensity, bp ? bp->sah_bins : 0u,
                             bp ? bp->max_leaf : 0u, P, !device_build, &cdf);
+        if (msg.empty()) {
+            bool unsupported = false;
+            msg = prepare_sdf(desc->sdf_clusters, desc->sdf_cluster_count, P, &unsupported);
+            if (unsupported) return fail(DMT_ERR_UNSUPPORTED, msg);
+        }
     } catch (const std::bad_alloc&) {
         return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation during scene build");
     }
@@ -625,6 +652,17 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         V.env_intensity = P.env_intensity;
         V.env_present = 1u;
     }
+    if (P.sdf_count) {
+        if ((st = upload(s, (const float4*)P.sdf.data(), (size_t)P.sdf_count * 28u, &V.sdf))) return bail(st);
+        if ((st = upload(s, (const float4*)P.sdf_bvh.nodes.data(), P.sdf_bvh.nodes.size() * 8u, &f4))) return bail(st);
+        V.sdf_nodes = f4;
+        if ((st = upload(s, (const float4*)P.sdf_leaf.data(), P.sdf_count, &V.sdf_leaf, kLeafPad))) return bail(st);
+        V.sdf_count = P.sdf_count;
+    }
+    // queries without a frame (dmt_trace_closest) trace the clusters with the
+    // engine's defaults, r_sdf_max_iters 128 and r_sdf_epsilon 1e-4
+    V.sdf_iters = 128u;
+    V.sdf_eps = 1e-4f;
     if (P.tri_count && device_build) {
         // the device build allocates its nodes level by level: the refit's
         // lists are those index ranges, deepest first
@@ -673,8 +711,8 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         s->mesh_vertices = desc->vertex_count;
         s->mesh_tris = P.tri_count;
     }
-    s->bvh_depth = std::max(tri_depth, P.sphere_bvh.depth);
-    s->bvh_nodes = (uint32_t)(tri_nodes + P.sphere_bvh.nodes.size());
+    s->bvh_depth = std::max(std::max(tri_depth, P.sphere_bvh.depth), P.sdf_bvh.depth);
+    s->bvh_nodes = (uint32_t)(tri_nodes + P.sphere_bvh.nodes.size() + P.sdf_bvh.nodes.size());
     hipError_t e = hipMalloc((void**)&s->count_stats, sizeof(DeviceStats));
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(stats)"));
     s->buffers.push_back(s->count_stats);
@@ -741,11 +779,11 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
     if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
     dmt_status st = check_device(s->dev);
     if (st) return st;
-    if ((st = check_frame(f))) return st;
+    if ((st = check_frame_for(s, f))) return st;
     if (frame_count == 0) return fail(DMT_ERR_INVALID_ARG, "frame_count must be >= 1");
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
-    if (sky_lights(s, f) && (st = ensure_sky_queues(s))) return st;
+    if (ext_path(s, f) && (st = ensure_sky_queues(s))) return st;
     if (f->max_bounces == 0) return DMT_OK;
     for (uint32_t k = 0; k < frame_count; ++k) {
         FrameArgs F = frame_args(f);
@@ -909,7 +947,7 @@ dmt_status dmt_render_gbuffer(dmt_scene* s, const dmt_frame_params* f, const dmt
     if (!s || !p) return fail(DMT_ERR_INVALID_ARG, "null pointer");
     dmt_status st = check_device(s->dev);
     if (st) return st;
-    if ((st = check_frame(f))) return st;
+    if ((st = check_frame_for(s, f))) return st;
     GbufferArgs A{};
     A.F = frame_args(f);
     std::memcpy(A.vp_curr, p->curr_view_proj, sizeof(A.vp_curr));
@@ -1088,10 +1126,10 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     if (!s || !out) return fail(DMT_ERR_INVALID_ARG, "null pointer");
     dmt_status st = check_device(s->dev);
     if (st) return st;
-    if ((st = check_frame(f))) return st;
+    if ((st = check_frame_for(s, f))) return st;
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
-    if (sky_lights(s, f) && (st = ensure_sky_queues(s))) return st;
+    if (ext_path(s, f) && (st = ensure_sky_queues(s))) return st;
     // Render into a scratch accumulation buffer so the scene's image and
     // counters are left as they were.
     float4* scratch = nullptr;
@@ -1184,11 +1222,13 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
     if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
     dmt_status st = check_device(s->dev);
     if (st) return st;
-    if ((st = check_frame(f))) return st;
+    if ((st = check_frame_for(s, f))) return st;
     if (f->max_bounces > kAdjMaxVerts) return fail(DMT_ERR_LIMIT, "the adjoint supports max_bounces <= 8");
     if (f->refract_bounces > kAdjMaxDielectric) return fail(DMT_ERR_LIMIT, "the adjoint supports refract_bounces <= 8");
     if (sky_lights(s, f))
         return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate sky lights (sun, moon, HDRI clusters)");
+    if (s->view.sdf_count)
+        return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate scenes with SDF clusters");
     const uint32_t prims = s->view.prim_count;
     if (prims && (!d_image_grad || !d_param_grad)) return fail(DMT_ERR_INVALID_ARG, "null device pointer");
     if (prims == 0) return DMT_OK;
@@ -1319,7 +1359,7 @@ BatchLaunch batch_launch(const dmt_batch* b, const dmt_batch_params& p) {
     bool feat = false, sky = false;
     for (size_t i = 0; i < b->frames.size(); ++i) {
         feat = feat || uses_features(b->scenes[i], &b->frames[i]);
-        sky = sky || sky_lights(b->scenes[i], &b->frames[i]);
+        sky = sky || ext_path(b->scenes[i], &b->frames[i]);
     }
     return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds, feat, sky};
 }
@@ -1402,7 +1442,7 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
     for (uint32_t i = 0; i < count; ++i) {
         if (!list[i]) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is null");
         if (list[i]->dev != dev) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is on another device");
-        if ((st = check_frame(&frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
+        if ((st = check_frame_for(list[i], &frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
     }
     {
         std::vector<dmt_scene*> sorted = list;
@@ -1450,9 +1490,10 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         b->order.insert(b->order.end(), queue[q].begin(), queue[q].end());
     }
     b->queue_begin[kXcds] = (uint32_t)b->order.size();
-    // records of the sky lights (sun or HDRI cluster, moon) when a frame has them
+    // records of the sky lights (sun or HDRI cluster, moon) in the kernel
+    // variant that has them (a frame with sky lights or a scene with SDF clusters)
     bool sky = false;
-    for (uint32_t i = 0; i < count; ++i) sky = sky || sky_lights(list[i], &frames[i]);
+    for (uint32_t i = 0; i < count; ++i) sky = sky || ext_path(list[i], &frames[i]);
     const std::pair<void**, size_t> req[] = {
         {(void**)&b->d_scenes, count * sizeof(BatchScene)},
         {(void**)&b->d_order, count * sizeof(uint32_t)},

@@ -195,4 +195,59 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
     return "";
 }
 
+std::string prepare_sdf(const float* clusters, uint32_t count, PreparedScene& S, bool* unsupported) {
+    *unsupported = false;
+    if (count && !clusters) return "sdf_clusters is null";
+    if (count >= (1u << 27)) return "too many SDF clusters (max 2^27)";
+    auto bits = [](float v) {
+        uint32_t u;
+        std::memcpy(&u, &v, 4);
+        return u;
+    };
+    std::vector<BoundsItem> items(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const float* c = clusters + (size_t)i * kSdfClusterFloats;
+        const std::string tag = "SDF cluster " + std::to_string(i);
+        const uint32_t nodes = bits(c[3]), mat = bits(c[7]);
+        if (nodes > kSdfNodeLimit) return tag + ": node_count must be <= 8";
+        if (mat > 2u) return tag + ": material must be 0, 1 or 2";
+        const float hdr[] = {c[0], c[1], c[2], c[4], c[5], c[6], c[8], c[9], c[10], c[11], c[12]};
+        if (!finite_all(hdr, 11)) return tag + " has a non-finite box or material value";
+        BoundsItem& b = items[i];
+        for (int a = 0; a < 3; ++a) {
+            if (!(c[a] <= c[4 + a])) return tag + ": aabb_min must be <= aabb_max";
+            b.lo[a] = c[a];
+            b.hi[a] = c[4 + a];
+            b.c[a] = (c[a] + c[4 + a]) * 0.5f;
+        }
+        for (uint32_t k = 0; k < nodes; ++k) {
+            const float* n = c + 16u + 12u * k;
+            const uint32_t op = bits(n[0]), ca = bits(n[2]), cb = bits(n[3]);
+            if (!finite_all(n + 4, 7)) return tag + ": node " + std::to_string(k) + " has a non-finite parameter";
+            // children come before their op (post-order); ops 5.. are the
+            // procedural ones, which evaluate to 1e30 and read no child
+            if ((op >= 1u && op <= 4u && ca >= k) || (op >= 1u && op <= 3u && cb >= k))
+                return tag + ": node " + std::to_string(k) + " names a child at or after itself";
+        }
+        if (nodes) {
+            const float* root = c + 16u + 12u * (nodes - 1u);
+            if (bits(root[0]) == 0u && bits(root[1]) >= 6u) {
+                *unsupported = true;
+                return tag + ": fractal SDF clusters are not supported";
+            }
+        }
+    }
+    S.sdf_count = count;
+    S.sdf.assign(clusters, clusters + (size_t)count * kSdfClusterFloats);
+    if (count) {
+        S.sdf_bvh = build_bvh4(items, 0u, 1u);
+        S.sdf_leaf.assign((size_t)count * 4u, 0.0f);
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t ci = S.sdf_bvh.order[k];
+            std::memcpy(&S.sdf_leaf[4u * k], &ci, 4);
+        }
+    }
+    return "";
+}
+
 }  // namespace dmt

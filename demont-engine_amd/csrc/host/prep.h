@@ -103,7 +103,23 @@ struct PreparedScene {
     std::vector<float> env_marginal, env_conditional;
     float env_total = 0.0f;
     float env_intensity = 1.0f;
+    // SDF clusters
+    std::vector<float> sdf;               // kSdfClusterFloats per cluster, input order
+    uint32_t sdf_count = 0;
+    Bvh4 sdf_bvh;                         // over the clusters' boxes, one cluster per leaf
+    std::vector<float> sdf_leaf;          // float4 per leaf slot: (cluster index bits, 0, 0, 0)
 };
+
+// The engine's SDF cluster record (Engine.cpp:5076-5190): a 4-float4 header
+// and kSdfNodeLimit nodes of 3 float4.
+constexpr uint32_t kSdfClusterFloats = 112u;
+constexpr uint32_t kSdfNodeLimit = 8u;   // SdfPrim::kMaxNodes
+
+// Validates the clusters and builds their BVH into S.  Returns "" on success,
+// else the error message; *unsupported is set for clusters that are valid
+// in the reference but outside this build (a fractal root,
+// SdfFractals.slang:55-62).
+std::string prepare_sdf(const float* clusters, uint32_t count, PreparedScene& S, bool* unsupported);
 
 // Validates and prepares.  Returns "" on success, else the error message.
 std::string prepare_scene(const float* prims, uint32_t prim_count, const float* lights, uint32_t light_count,

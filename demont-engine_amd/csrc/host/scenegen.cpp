@@ -79,6 +79,117 @@ void gen_spheres(Rng& rng, int n, std::vector<float>& prims, float extent = 4.0f
     }
 }
 
+// An SDF cluster record (dmt.h, DMT_SDF_CLUSTER_FLOATS): nodes in post-order.
+struct SdfNode {
+    uint32_t op, shape, a, b;
+    float q[4];
+    float c[3];
+};
+
+void put_sdf(std::vector<float>& out, const std::vector<SdfNode>& nodes, const float lo[3], const float hi[3],
+             uint32_t mat, const float alb[3], float rough, float ior) {
+    float r[112] = {};
+    auto u2f = [](uint32_t u) {
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    };
+    for (int a = 0; a < 3; ++a) {
+        r[a] = lo[a];
+        r[4 + a] = hi[a];
+        r[8 + a] = alb[a];
+    }
+    r[3] = u2f((uint32_t)nodes.size());
+    r[7] = u2f(mat);
+    r[11] = rough;
+    r[12] = ior;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        float* n = r + 16 + 12 * i;
+        n[0] = u2f(nodes[i].op); n[1] = u2f(nodes[i].shape); n[2] = u2f(nodes[i].a); n[3] = u2f(nodes[i].b);
+        for (int k = 0; k < 4; ++k) n[4 + k] = nodes[i].q[k];
+        for (int k = 0; k < 3; ++k) n[8 + k] = nodes[i].c[k];
+    }
+    out.insert(out.end(), r, r + 112);
+}
+
+// SDF clusters of six kinds (i mod 6) with config 0 materials (seeded shuffle
+// of i mod 3), centres uniform in [-extent, extent]^3; each box encloses the
+// surface with a 1e-3 margin (smooth unions reach k / 4 past their parts).
+void gen_sdf(Rng& rng, int n, std::vector<float>& out, float extent) {
+    std::vector<uint32_t> mats((size_t)n);
+    for (int i = 0; i < n; ++i) mats[(size_t)i] = (uint32_t)(i % 3);
+    for (int i = n - 1; i > 0; --i) {
+        int j = (int)(rng.next() % (uint64_t)(i + 1));
+        std::swap(mats[(size_t)i], mats[(size_t)j]);
+    }
+    for (int i = 0; i < n; ++i) {
+        const float c[3] = {rng.uf(-extent, extent), rng.uf(-extent, extent), rng.uf(-extent, extent)};
+        float alb[3] = {rng.uf(0.1f, 0.9f), rng.uf(0.1f, 0.9f), rng.uf(0.1f, 0.9f)};
+        float rough = rng.uf(0.0f, 0.5f);
+        const uint32_t m = mats[(size_t)i];
+        if (m == 0u) rough = 0.0f;
+        if (m == 2u) { alb[0] = alb[1] = alb[2] = 1.0f; rough = 0.0f; }
+        std::vector<SdfNode> nodes;
+        float ext[3];   // half size of the box around c
+        switch (i % 6) {
+            case 0: {   // box
+                const float h[3] = {rng.uf(0.08f, 0.25f), rng.uf(0.08f, 0.25f), rng.uf(0.08f, 0.25f)};
+                nodes.push_back({0u, 1u, 0u, 0u, {h[0], h[1], h[2], 0.0f}, {c[0], c[1], c[2]}});
+                for (int a = 0; a < 3; ++a) ext[a] = h[a];
+                break;
+            }
+            case 1: {   // rounded box
+                const float h[3] = {rng.uf(0.06f, 0.2f), rng.uf(0.06f, 0.2f), rng.uf(0.06f, 0.2f)}, rr = 0.04f;
+                nodes.push_back({0u, 2u, 0u, 0u, {h[0], h[1], h[2], rr}, {c[0], c[1], c[2]}});
+                for (int a = 0; a < 3; ++a) ext[a] = h[a] + rr;
+                break;
+            }
+            case 2: {   // box united with a sphere (hard union)
+                const float h[3] = {rng.uf(0.06f, 0.18f), rng.uf(0.06f, 0.18f), rng.uf(0.06f, 0.18f)};
+                const float sr = rng.uf(0.08f, 0.2f);
+                const float o[3] = {rng.uf(-0.15f, 0.15f), rng.uf(-0.15f, 0.15f), rng.uf(-0.15f, 0.15f)};
+                nodes.push_back({0u, 1u, 0u, 0u, {h[0], h[1], h[2], 0.0f}, {c[0], c[1], c[2]}});
+                nodes.push_back({0u, 0u, 0u, 0u, {sr, 0.0f, 0.0f, 0.0f}, {c[0] + o[0], c[1] + o[1], c[2] + o[2]}});
+                nodes.push_back({1u, 0u, 0u, 1u, {0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}});
+                for (int a = 0; a < 3; ++a) ext[a] = std::fmax(h[a], std::fabs(o[a]) + sr);
+                break;
+            }
+            case 3: {   // sphere smoothly united with a capsule
+                const float sr = rng.uf(0.08f, 0.18f), ch = rng.uf(0.05f, 0.15f), cr = rng.uf(0.04f, 0.1f), k = 0.08f;
+                nodes.push_back({0u, 0u, 0u, 0u, {sr, 0.0f, 0.0f, 0.0f}, {c[0] - 0.12f, c[1], c[2]}});
+                nodes.push_back({0u, 4u, 0u, 0u, {ch, cr, 0.0f, 0.0f}, {c[0] + 0.12f, c[1], c[2]}});
+                nodes.push_back({1u, 0u, 0u, 1u, {k, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}});
+                ext[0] = 0.12f + std::fmax(sr, cr) + 0.25f * k;
+                ext[1] = std::fmax(sr, ch + cr) + 0.25f * k;
+                ext[2] = std::fmax(sr, cr) + 0.25f * k;
+                break;
+            }
+            case 4: {   // box with a sphere carved out of one corner (hard subtraction)
+                const float h[3] = {rng.uf(0.1f, 0.22f), rng.uf(0.1f, 0.22f), rng.uf(0.1f, 0.22f)};
+                const float sr = rng.uf(0.1f, 0.2f);
+                nodes.push_back({0u, 1u, 0u, 0u, {h[0], h[1], h[2], 0.0f}, {c[0], c[1], c[2]}});
+                nodes.push_back({0u, 0u, 0u, 0u, {sr, 0.0f, 0.0f, 0.0f}, {c[0] + h[0], c[1] + h[1], c[2] + h[2]}});
+                nodes.push_back({2u, 0u, 0u, 1u, {0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}});
+                for (int a = 0; a < 3; ++a) ext[a] = h[a];
+                break;
+            }
+            default: {   // torus
+                const float R = rng.uf(0.12f, 0.25f), r = rng.uf(0.03f, 0.07f);
+                nodes.push_back({0u, 3u, 0u, 0u, {R, r, 0.0f, 0.0f}, {c[0], c[1], c[2]}});
+                ext[0] = ext[2] = R + r;
+                ext[1] = r;
+                break;
+            }
+        }
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = c[a] - (ext[a] + 1e-3f);
+            hi[a] = c[a] + (ext[a] + 1e-3f);
+        }
+        put_sdf(out, nodes, lo, hi, m, alb, rough, m == 2u ? 1.5f : 1.0f);
+    }
+}
+
 // 3-D value noise: trilinear blend (smoothstep weights) of hashed lattice values.
 struct ValueNoise {
     uint64_t seed;
@@ -178,10 +289,10 @@ dmt_status dmt_camera_look_at(const float eye[3], const float target[3], float f
 dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_options* opt, dmt_generated_scene** out) {
     if (!out) return DMT_ERR_INVALID_ARG;
     if (config > 4u || config == 3u) return DMT_ERR_UNSUPPORTED;
-    dmt_gen_options o{-1, -1, -1, 0, 0, 0, 0, -1};
+    dmt_gen_options o{-1, -1, -1, 0, 0, 0, 0, -1, -1};
     if (opt) o = *opt;
     Rng rng(seed + 0x1000u * config);
-    std::vector<float> prims, lights, pos, env;
+    std::vector<float> prims, lights, pos, env, sdf;
     std::vector<uint32_t> idx;
     uint32_t W = 256, H = 256, spp = 1, bounces = 4, ew = 8, eh = 4;
     const float zero3[3] = {0, 0, 0};
@@ -190,9 +301,10 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
         gen_spheres(rng, n, prims);
         if (config == 4u) { W = H = 512; spp = 16; }
     } else if (config == 2u) {
-        // 4,096 spheres + a 500k-triangle mesh lit by 4,096 emissive triangles
+        // 3,584 spheres and 512 SDF clusters + a 500k-triangle mesh lit by
+        // 4,096 emissive triangles
         W = 1280; H = 720; spp = 4; bounces = 6;
-        int n = o.spheres >= 0 ? o.spheres : 4096;
+        int n = o.spheres >= 0 ? o.spheres : 3584;
         gen_spheres(rng, n, prims, 4.5f, 0.05f, 0.3f);
         int sub = o.subdivisions >= 0 ? o.subdivisions : 7;
         int nrand = o.random_tris >= 0 ? o.random_tris : 500000 - 20 * (1 << (2 * 7));
@@ -281,6 +393,8 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
         const float I[3] = {10.0f, 10.0f, 10.0f};
         put_quad_light(lights, lp, ln, lu, 1.5f, I);
     }
+    // config 2: the SDF clusters, drawn last
+    if (config == 2u) gen_sdf(rng, o.sdf_clusters >= 0 ? o.sdf_clusters : 512, sdf, 4.5f);
     (void)zero3;
     auto* g = new (std::nothrow) dmt_generated_scene();
     if (!g) return DMT_ERR_OUT_OF_MEMORY;
@@ -297,6 +411,8 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
     g->index_count = (uint32_t)idx.size();
     g->env_rgb = dup_f(env); g->env_width = ew; g->env_height = eh;
     g->env_intensity = 1.0f;
+    g->sdf_clusters = dup_f(sdf);
+    g->sdf_cluster_count = (uint32_t)(sdf.size() / 112);
     std::memset(&g->frame, 0, sizeof(g->frame));
     const float eye[3] = {0.0f, 2.0f, 12.0f}, tgt[3] = {0.0f, 0.0f, 0.0f};
     dmt_camera_look_at(eye, tgt, 50.0f, W, H, &g->frame);
@@ -309,6 +425,8 @@ dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_opti
     g->frame.refract_bounces = 4;
     g->frame.firefly_clamp = 10.0f;
     g->frame.ema_alpha = 0.0f;
+    g->frame.sdf_max_iters = 128u;   // r_sdf_max_iters, r_sdf_epsilon defaults
+    g->frame.sdf_epsilon = 1e-4f;
     *out = g;
     return DMT_OK;
 }
@@ -326,6 +444,7 @@ dmt_status dmt_batch_workload_scene(uint64_t seed, uint32_t index, dmt_gen_optio
     const uint32_t ico = 20u * (1u << (2 * sub));
     std::memset(opt, 0, sizeof(*opt));
     opt->lights = -1;
+    opt->sdf_clusters = -1;
     opt->subdivisions = sub;
     opt->random_tris = (int32_t)(tris - ico);
     opt->spheres = (index & 1u) ? 64 : 0;
@@ -339,6 +458,7 @@ dmt_status dmt_batch_workload_scene(uint64_t seed, uint32_t index, dmt_gen_optio
 void dmt_scene_generated_free(dmt_generated_scene* g) {
     if (!g) return;
     std::free(g->prims); std::free(g->lights); std::free(g->positions); std::free(g->indices); std::free(g->env_rgb);
+    std::free(g->sdf_clusters);
     delete g;
 }
 

@@ -238,6 +238,12 @@ __device__ __forceinline__ float randf(uint32_t& seed) {
     return (float)(seed & 0x00FFFFFFu) / 16777216.0f;
 }
 
+}  // namespace dmt
+
+#include "sdf.h"
+
+namespace dmt {
+
 // ---------------------------------------------------------------------------
 // scene view
 
@@ -273,6 +279,15 @@ struct SceneView {
     f3 mesh_prev, mesh_curr;      // its translation at shutter times 0 and 1
     const float4* moon;           // moon texture (RGBA, row-major), null: none
     uint32_t moon_w, moon_h;
+    // SDF clusters (sdf.h): records in input order, a BVH4 over their boxes
+    // whose leaf slots hold (cluster index bits, 0, 0, 0), and the frame's
+    // sphere-trace step cap and surface distance (sdf_params.y, sdf_params_f.x)
+    const float4* sdf;
+    uint32_t sdf_count;
+    uint32_t sdf_iters;
+    const float4* sdf_nodes;
+    const float4* sdf_leaf;
+    float sdf_eps;
 };
 
 // Traversal stack: `lds_depth` entries in LDS (thread-interleaved), the rest
@@ -530,17 +545,45 @@ __device__ __forceinline__ uint32_t tri_leaf(f3 o, f3 d, uint32_t first, uint32_
     return n;
 }
 
+// The SDF clusters of a leaf held in the fetch (slot k: the cluster index
+// bits in .x), each sphere-traced up to the closest distance so far
+// (traceSdfClusters, PathTrace.slang:2212-2306, clusters after the analytic
+// primitives): a cluster replaces an analytic hit only when strictly closer,
+// and of two clusters at one distance the lower index wins, as in the
+// reference's in-order scan.  For a later-visited lower index to reach a tie
+// its trace runs to just past the current distance.  Returns how many it tested.
+template <bool COUNT>
+__device__ __forceinline__ uint32_t sdf_leaf(const SceneView& S, f3 o, f3 d, f3 inv, uint32_t count,
+                                             const float4 (&r)[kFetch], AnalyticBest& b, TraceCounters& tc) {
+    const uint32_t n = count < 4u ? count : 4u;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t ci = __float_as_uint(r[k].x);
+        if (COUNT) tc.spheres++;
+        const bool tie_ok = b.code != kMiss && (b.code & kSdfBit) != 0u && (b.code & ~kSdfBit) > ci;
+        // b.t > 1e-3 here: the next float up is the next bit pattern
+        const float t_max = tie_ok ? __uint_as_float(__float_as_uint(b.t) + 1u) : b.t;
+        float t;
+        if (sdf_march(S.sdf + (size_t)ci * kSdfStride, o, d, inv, 1e-3f, t_max, S.sdf_iters, S.sdf_eps, t)) {
+            b.t = t;
+            b.id = 0xFFFFFFFFu;
+            b.code = kSdfBit | ci;
+        }
+    }
+    return n;
+}
+
 __device__ __forceinline__ HitRec final_hit(const AnalyticBest& a, const TriBest& t) {
     return (t.pos != kMiss) ? HitRec{t.t, kMeshBit | t.pos} : HitRec{a.t, a.code};
 }
 
 // traceScene (PathTrace.slang:2331-2693): planes, then the sphere BVH, then
-// the triangle BVH limited to the analytic hit distance.  Ties resolve to
-// the lower primitive index; analytic hits win ties against triangles.
-// With MOTION the query is at shutter time `time`: moving spheres at their
-// lerped centres, and a translated mesh intersected from the origin shifted
-// by -lerp(mesh_prev, mesh_curr, time) (:2483-2487).
-template <bool COUNT, bool MOTION = false>
+// (with SDF) the SDF clusters, then the triangle BVH limited to the analytic
+// hit distance.  Ties resolve to the lower primitive index; analytic hits win
+// ties against clusters and triangles.  With MOTION the query is at shutter
+// time `time`: moving spheres at their lerped centres, and a translated mesh
+// intersected from the origin shifted by -lerp(mesh_prev, mesh_curr, time)
+// (:2483-2487).
+template <bool COUNT, bool MOTION = false, bool SDF = false>
 __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, Stack& st, TraceCounters& tc,
                                                 float time = 1.0f) {
     AnalyticBest ab = planes_closest(S, o, d);
@@ -549,6 +592,12 @@ __device__ __forceinline__ HitRec trace_closest(const SceneView& S, f3 o, f3 d, 
         bvh4_walk<COUNT, 1u>(S.sph_nodes, S.spheres, o, inv, ab.t, st, [&](uint32_t first, uint32_t count,
                                                                           const float4 (&r)[kFetch]) {
             return sphere_leaf<COUNT, MOTION>(S, o, d, first, count, r, ab, tc, time);
+        }, tc);
+    }
+    if (SDF && S.sdf_nodes) {
+        bvh4_walk<COUNT, 1u>(S.sdf_nodes, S.sdf_leaf, o, inv, ab.t, st, [&](uint32_t, uint32_t count,
+                                                                          const float4 (&r)[kFetch]) {
+            return sdf_leaf<COUNT>(S, o, d, inv, count, r, ab, tc);
         }, tc);
     }
     TriBest tb{ab.t, 0u, kMiss};
@@ -571,34 +620,46 @@ struct Walk {
     f3 inv;
     AnalyticBest ab;
     TriBest tb;
-    uint32_t phase;   // 0 sphere BVH, 1 triangle BVH
+    uint32_t phase;   // 0 sphere BVH, 1 SDF cluster BVH, 2 triangle BVH
     uint32_t cur, sp;
 };
 
+// The hierarchy after `phase` that the scene has (3: none).  SDF: the
+// query sees the SDF clusters (the feature path).
+template <bool SDF>
+__device__ __forceinline__ uint32_t walk_next_phase(const SceneView& S, uint32_t phase) {
+    if (phase < 1u && SDF && S.sdf_nodes) return 1u;
+    if (phase < 2u && S.tri_nodes) return 2u;
+    return 3u;
+}
+
 // Starts the query (o, d); returns true when it is already done (no
 // hierarchies: the planes decide).
+template <bool SDF = false>
 __device__ __forceinline__ bool walk_begin(const SceneView& S, f3 o, f3 d, Walk& w) {
     w.inv = safe_inverse(d);
     w.ab = planes_closest(S, o, d);
     w.tb = TriBest{w.ab.t, 0u, kMiss};
-    w.phase = S.sph_nodes ? 0u : 1u;
+    w.phase = S.sph_nodes ? 0u : walk_next_phase<SDF>(S, 0u);
     w.cur = 0u;
     w.sp = 0u;
-    return !S.sph_nodes && !S.tri_nodes;
+    return w.phase == 3u;
 }
 
-// One step of the query: a node or one leaf fetch.  Returns true when the
-// query is done; its hit is then final_hit(w.ab, w.tb).  With MOTION the
-// query is at shutter time `time`, as in trace_closest.
-template <bool COUNT, bool MOTION = false>
+// One step of the query: a node or one leaf fetch (an SDF leaf: its
+// clusters' sphere traces).  Returns true when the query is done; its hit is
+// then final_hit(w.ab, w.tb).  MOTION and SDF as in trace_closest; a query
+// with SDF must have been started by walk_begin<true>.
+template <bool COUNT, bool MOTION = false, bool SDF = false>
 __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& w, Stack& st, TraceCounters& tc,
                                           float time = 1.0f) {
-    const bool tri = w.phase == 1u;
+    const bool tri = w.phase == 2u;
+    const bool sdf = SDF && w.phase == 1u;
     if (MOTION && tri && S.mesh_shift) o = o - mix3(S.mesh_prev, S.mesh_curr, time);
     const bool at_leaf = (w.cur & kLeaf) != 0u;
     const uint32_t lf = w.cur & kLeafFirst;
-    const float4* p = at_leaf ? (tri ? S.tris + (size_t)lf * 3u : S.spheres + lf)
-                              : (tri ? S.tri_nodes : S.sph_nodes) + (size_t)w.cur * 8u;
+    const float4* p = at_leaf ? (tri ? S.tris + (size_t)lf * 3u : (sdf ? S.sdf_leaf : S.spheres) + lf)
+                              : (tri ? S.tri_nodes : (sdf ? S.sdf_nodes : S.sph_nodes)) + (size_t)w.cur * 8u;
     float4 r[kFetch];
 #pragma unroll
     for (uint32_t k = 0; k < kFetch; ++k) r[k] = p[k];
@@ -606,8 +667,10 @@ __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& 
     for (uint32_t k = 0; k < kFetch; ++k) asm volatile("" ::"v"(r[k].x), "v"(r[k].y), "v"(r[k].z), "v"(r[k].w));
     if (at_leaf) {
         const uint32_t count = ((w.cur >> 27) & 15u) + 1u;
-        const uint32_t used = tri ? tri_leaf<COUNT>(o, d, lf, count, r, w.tb, tc)
-                                  : sphere_leaf<COUNT, MOTION>(S, o, d, lf, count, r, w.ab, tc, time);
+        uint32_t used;
+        if (tri) used = tri_leaf<COUNT>(o, d, lf, count, r, w.tb, tc);
+        else if (sdf) used = sdf_leaf<COUNT>(S, o, d, w.inv, count, r, w.ab, tc);
+        else used = sphere_leaf<COUNT, MOTION>(S, o, d, lf, count, r, w.ab, tc, time);
         if (used < count) {
             w.cur = kLeaf | ((count - used - 1u) << 27) | (lf + used);
             return false;
@@ -635,13 +698,12 @@ __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& 
         w.cur = st.get(--w.sp);
         return false;
     }
-    if (w.phase == 0u && S.tri_nodes) {   // the sphere walk is done
-        w.tb = TriBest{w.ab.t, 0u, kMiss};
-        w.phase = 1u;
-        w.cur = 0u;
-        return false;
-    }
-    return true;
+    const uint32_t next = walk_next_phase<SDF>(S, w.phase);   // this hierarchy is done
+    if (next == 3u) return true;
+    if (next == 2u) w.tb = TriBest{w.ab.t, 0u, kMiss};
+    w.phase = next;
+    w.cur = 0u;
+    return false;
 }
 
 // Closest hits of the rays 0 .. n-1 of a queue, every lane on its own ray,
@@ -705,15 +767,16 @@ template <class T> struct SurfaceT {
     float ior;
     uint32_t mat;
     uint32_t user_id;
-    uint32_t prim;    // analytic input index or triangle id
-    uint32_t kind;    // 1 analytic, 2 mesh
+    uint32_t prim;    // analytic input index, triangle id or SDF cluster index
+    uint32_t kind;    // 1 analytic, 2 mesh, 3 SDF cluster
     T t;              // hit distance
 };
 using Surface = SurfaceT<float>;
 
 // The material fields of a hit (no normal): shared by the float and the dual
 // surface evaluation.
-template <class T> __device__ __forceinline__ void surface_material(const SceneView& S, HitRec h, SurfaceT<T>& s) {
+template <bool SDF = false, class T>
+__device__ __forceinline__ void surface_material(const SceneView& S, HitRec h, SurfaceT<T>& s) {
     if (h.code & kMeshBit) {
         const float4 a = S.tris[(size_t)(h.code & ~kMeshBit) * 3u];
         s.albedo = f3{1.0f, 0.85f, 0.45f};   // MESH_ALBEDO (PathTrace.slang:1428)
@@ -724,6 +787,20 @@ template <class T> __device__ __forceinline__ void surface_material(const SceneV
         s.user_id = 0u;
         s.prim = __float_as_uint(a.w);
         s.kind = 2u;
+    } else if (SDF && (h.code & kSdfBit)) {
+        // an SDF cluster hit: the cluster's material, no emission, prim id 0
+        // (PathTrace.slang:2289-2303)
+        const uint32_t ci = h.code & ~kSdfBit;
+        const float4* c = S.sdf + (size_t)ci * kSdfStride;
+        const float4 h1 = c[1], h2 = c[2], h3 = c[3];
+        s.albedo = xyz(h2);
+        s.rough = T(h2.w);
+        s.mat = __float_as_uint(h1.w);
+        s.ior = h3.x;
+        s.emission = f3{0.0f, 0.0f, 0.0f};
+        s.user_id = 0u;
+        s.prim = ci;
+        s.kind = 3u;
     } else {
         const float4* p = S.prims + (size_t)h.code * 7u;
         const float4 v1 = p[1], v2 = p[2], v4 = p[4];
@@ -740,13 +817,20 @@ template <class T> __device__ __forceinline__ void surface_material(const SceneV
 
 // The normal of a sphere hit at shutter time `time` is taken about the
 // centre the ray saw (PathTrace.slang:2093-2097).
+// SDF: the hit may be an SDF cluster's (the feature path's traversal).
+template <bool SDF = false>
 __device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, HitRec h, float time = 1.0f) {
     Surface s;
-    surface_material(S, h, s);
+    surface_material<SDF>(S, h, s);
     s.t = h.t;
     if (h.code & kMeshBit) {
         const float4* tr = S.tris + (size_t)(h.code & ~kMeshBit) * 3u;
         s.n = normalize(cross(xyz(tr[1]), xyz(tr[2])));
+    } else if (SDF && (h.code & kSdfBit)) {
+        // the analytic gradient at the point the sphere trace stopped
+        // (SdfPrimitives.slang:1430-1433)
+        const float4* c = S.sdf + (size_t)(h.code & ~kSdfBit) * kSdfStride;
+        s.n = normalize(sdf_grad(c, __float_as_uint(c[0].w), o + d * h.t));
     } else {
         const float4* p = S.prims + (size_t)h.code * 7u;
         const float4 v0 = p[0], v2 = p[2];
@@ -760,8 +844,9 @@ __device__ __forceinline__ Surface surface_at(const SceneView& S, f3 o, f3 d, Hi
     }
     return s;
 }
+template <bool SDF = false>
 __device__ __forceinline__ Surface surface_of(const SceneView& S, f3 o, f3 d, HitRec h, float time = 1.0f) {
-    return surface_at(S, o, d, h, time);
+    return surface_at<SDF>(S, o, d, h, time);
 }
 
 }  // namespace dmt

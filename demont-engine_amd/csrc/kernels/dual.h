@@ -64,6 +64,7 @@ __device__ __forceinline__ f3 deriv3(d3 a) { return f3{a.x.d, a.y.d, a.z.d}; }
 // g(o + t d) = 0, dt = -(dg . do + t dg . dd) / (dg . d), with dg the sphere's
 // radius vector or the plane / triangle normal at the hit.  A sphere is taken
 // about its centre at shutter time `time` (PathTrace.slang:2093).
+template <bool SDF = false>
 __device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 d, HitRec h, float time = 1.0f) {
     SurfaceT<df> s;
     surface_material(S, h, s);
@@ -75,7 +76,7 @@ __device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 
         const float4* tr = S.tris + (size_t)(h.code & ~kMeshBit) * 3u;
         g = cross(xyz(tr[1]), xyz(tr[2]));
         s.n = to<df>(normalize(g));
-    } else {
+    } else {   // analytic (the adjoint refuses scenes with SDF clusters)
         const float4* p = S.prims + (size_t)h.code * 7u;
         const float4 v0 = p[0], v2 = p[2];
         if (v2.x == 0.0f) {

@@ -227,8 +227,9 @@ __device__ __forceinline__ V3<T> miss_sky(const SceneView& S, const SkyArgs* K, 
 // frame's sky lights -- the sun and moon NEE of the no-env Lambert branch
 // with their discs in the sky (PathTrace.slang:6546-6667), the HDRI cluster
 // NEE of the env branch and the cluster test of the MIS miss (:6493-6518,
-// :6074); SKY frames are float only.
-template <bool SKY = false, class T, class Sink>
+// :6074); SKY frames are float only.  SDF: h may be an SDF cluster's hit
+// (the feature path's traversal).
+template <bool SKY = false, bool SDF = false, class T, class Sink>
 __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs& F, uint32_t b, V3<T> o, V3<T> d,
                                              HitRec h, PathVertexStateT<T>& P, VertexResultT<T>& R, Sink& sink,
                                              const SkyArgs* K = nullptr) {
@@ -259,7 +260,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
         }
         return;
     }
-    SurfaceT<T> sf = surface_of(S, o, d, h, P.time);
+    SurfaceT<T> sf = surface_of<SDF>(S, o, d, h, P.time);
     sink.on_surface(b, sf);
     const V3<T> hp = o + d * sf.t;
     const T ndi = dot(sf.n, d);
@@ -451,12 +452,12 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
 // hit h of the ray (o, d): returns true when t is final (a miss or a hit
 // beyond t_max keeps it, an opaque hit, the step limit or total internal
 // reflection zero it); otherwise tints t by (1 - F) * albedo and moves the
-// ray through the dielectric surface.
-template <class T>
+// ray through the dielectric surface.  SDF as in shade_vertex.
+template <bool SDF = false, class T>
 __device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArgs& F, HitRec h, uint32_t i, V3<T>& o,
                                               V3<T>& d, T& t_max, V3<T>& t, float time = 1.0f) {
     if (h.code == kMiss || h.t >= primal(t_max)) return true;
-    const SurfaceT<T> sf = surface_of(S, o, d, h, time);
+    const SurfaceT<T> sf = surface_of<SDF>(S, o, d, h, time);
     if (sf.mat != 2u || i == F.refract_bounces) {
         t = to<T>(f3{0.0f, 0.0f, 0.0f});
         return true;

@@ -61,7 +61,8 @@ struct BatchLaunch {
     uint32_t blocks;                  // grid size in workgroups
     uint32_t stack_lds;               // LDS stack entries per thread: 16, 24 or 32
     bool features;                    // some scene's frame uses camera-sample features
-    bool sky;                         // some scene's frame has sky lights (4 record kinds in rec)
+    bool ext;                         // some scene's frame has sky lights (4 record kinds in rec)
+                                      // or some scene SDF clusters
 };
 
 // Sets the frame counter (stream-ordered, not part of a captured frame).

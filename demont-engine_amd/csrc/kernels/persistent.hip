@@ -200,7 +200,8 @@ template <uint32_t DEPTH> constexpr uint32_t persistent_waves() { return (160u *
 // FEAT: some scene's frame uses camera-sample features (thin lens, shutter
 // time, hero wavelength, a translated mesh); each scene's FeatureArgs say
 // which, and the lane carries the sample's shutter time and wavelength.
-template <uint32_t DEPTH, bool FEAT, bool SKY = false>
+// EXT: some scene's frame has sky lights or some scene SDF clusters.
+template <uint32_t DEPTH, bool FEAT, bool EXT = false>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<DEPTH>()))) k_persistent(BatchArgs A) {
     __shared__ uint32_t lds[DEPTH * kBlock];
@@ -261,7 +262,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                         l.P.seed = pixel_seed(F, x, y, frame_index);
                         start_sample<FEAT>(F, X, l, x, y, 0u);
                         state = kLaneWalk;
-                        if (walk_begin(S, l.o, l.d, w)) {
+                        if (walk_begin<EXT>(S, l.o, l.d, w)) {
                             h = final_hit(w.ab, w.tb);
                             state = kLaneReady;
                         }
@@ -327,18 +328,18 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                 else n_ext++;
                 ScratchSink sink{{}, A.rec, A.threads, tid, 0u};
                 VertexResult R;
-                shade_vertex<SKY>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
+                shade_vertex<EXT, EXT>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.pending | (cont ? 0u : kEnded);
                 c.set3(Cold::kNo, R.no);
                 c.set3(Cold::kNd, R.nd);
-                advance<FEAT, SKY>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                advance<FEAT, EXT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(Cold::kTmax);
                 f3 tr = c.get3(Cold::kTr);
                 const uint32_t step = c.getu(Cold::kStep);
-                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
+                if (transmit_step<EXT>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         const float4* row = A.rec + ((size_t)k * kRecRows) * A.threads + tid;
@@ -348,7 +349,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                                                      (l.pending & (16u << k)) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    advance<FEAT, SKY>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                    advance<FEAT, EXT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
                 } else {
                     c.set(Cold::kTmax, t_max);
                     c.set3(Cold::kTr, tr);
@@ -357,7 +358,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
             }
             if (l.active) {
                 state = kLaneWalk;
-                if (walk_begin(S, l.o, l.d, w)) {
+                if (walk_begin<EXT>(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
                     state = kLaneReady;
                 }
@@ -368,7 +369,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
         // ---- traversal steps of every lane with a query in progress ----
 #pragma unroll
         for (uint32_t k = 0; k < kTailSteps; ++k) {
-            if (state == kLaneWalk && walk_step<false, FEAT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
+            if (state == kLaneWalk && walk_step<false, FEAT, EXT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
                 h = final_hit(w.ab, w.tb);
                 state = kLaneReady;
             }
@@ -401,7 +402,7 @@ hipError_t batch_set_frame(uint32_t* frame, uint32_t value, hipStream_t stream) 
 hipError_t launch_batch_frame(const BatchArgs& A, const BatchLaunch& L, hipStream_t stream) {
     if (L.blocks == 0u || (size_t)L.blocks * kBlock != A.threads) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, stream, A.work, A.nscenes);
-    const uint32_t v = L.sky ? 2u : (L.features ? 1u : 0u);   // sky lights imply features
+    const uint32_t v = L.ext ? 2u : (L.features ? 1u : 0u);   // sky lights and SDF clusters imply features
     switch (L.stack_lds * 4u + v) {
         case 64u: hipLaunchKernelGGL((k_persistent<16u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
         case 65u: hipLaunchKernelGGL((k_persistent<16u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;

@@ -52,7 +52,8 @@ struct FeatureArgs {
     uint32_t spectral;                     // draw a hero wavelength per sample
     uint32_t fixed_jitter;                 // sample 0 takes jit_x, jit_y (halton_jitter) instead of two draws
     float jit_x, jit_y;
-    uint32_t sky;                          // sky lights on (SkyArgs below)
+    uint32_t ext;                          // sky lights (SkyArgs below) or SDF clusters: the
+                                           // kernel variant that has their code
     uint32_t pad;
     SkyArgs sky_args;
 };

@@ -218,11 +218,13 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
 // five per CU): 96 VGPRs, a few spilled values outside the traversal loop.
 //
 // FEAT (frames with camera-sample features: thin lens, shutter time, hero
-// wavelength, a translated mesh): there is no queue; entry i is slot i of the
-// frame's 8x8 tile order, whose pixel's first sample the lane starts itself
-// (camera_sample), and the shutter time and wavelength ride in the lane's
-// path state.
-template <bool COUNT, bool FEAT, bool SKY = false>
+// wavelength, a fixed jitter, a translated mesh): there is no queue; entry i
+// is slot i of the frame's 8x8 tile order, whose pixel's first sample the
+// lane starts itself (camera_sample), and the shutter time and wavelength
+// ride in the lane's path state.  EXT (implies FEAT): the frame has sky
+// lights or the scene SDF clusters; the sky-light records and the clusters'
+// traversal exist only in this variant.
+template <bool COUNT, bool FEAT, bool EXT = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerSimd, kTailWavesPerSimd)))
 k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uint32_t* ctr, PathState ps,
        ShadowQueue sl, ShadowQueue se, ShadowQueue s2, ShadowQueue s3, uint32_t* spill, uint32_t spill_depth,
@@ -273,7 +275,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                     l.mode = 0u;
                     l.active = true;
                     state = tail::kLaneWalk;
-                    if (walk_begin(S, l.o, l.d, w)) {
+                    if (walk_begin<EXT>(S, l.o, l.d, w)) {
                         h = final_hit(w.ab, w.tb);
                         state = tail::kLaneReady;
                     }
@@ -295,7 +297,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 l.mode = 0u;
                 l.active = true;
                 state = tail::kLaneWalk;
-                if (walk_begin(S, l.o, l.d, w)) {
+                if (walk_begin<EXT>(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
                     state = tail::kLaneReady;
                 }
@@ -317,20 +319,20 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 else n_ext++;
                 SlotSink sink{{}, sl, se, s2, s3, l.slot, l.pix, 0u};
                 VertexResult R;
-                shade_vertex<SKY>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
+                shade_vertex<EXT, EXT>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.emitted | (cont ? 0u : tail::kEnded);
                 if (cont) {
                     c.set3(tail::TailCold::kNo, R.no);
                     c.set3(tail::TailCold::kNd, R.nd);
                 }
-                tail::tail_advance<FEAT, SKY>(F, X, sl, se, s2, s3, ps, l, c);
+                tail::tail_advance<FEAT, EXT>(F, X, sl, se, s2, s3, ps, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(tail::TailCold::kTmax);
                 f3 tr = c.get3(tail::TailCold::kTr);
                 const uint32_t step = c.getu(tail::TailCold::kStep);
-                if (transmit_step(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
+                if (transmit_step<EXT>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         // fields picked one by one: a selected reference to one of
@@ -338,7 +340,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                         const float4* qa = k == 0u ? sl.a : se.a;
                         const float4* qb = k == 0u ? sl.b : se.b;
                         const float4* qd = k == 0u ? sl.d : se.d;
-                        if (SKY && k >= 2u) {
+                        if (EXT && k >= 2u) {
                             qa = k == 2u ? s2.a : s3.a;
                             qb = k == 2u ? s2.b : s3.b;
                             qd = k == 2u ? s2.d : s3.d;
@@ -349,7 +351,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                                                      k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    tail::tail_advance<FEAT, SKY>(F, X, sl, se, s2, s3, ps, l, c);
+                    tail::tail_advance<FEAT, EXT>(F, X, sl, se, s2, s3, ps, l, c);
                 } else {   // transmit_step ends every query by its step refract_bounces
                     c.set(tail::TailCold::kTmax, t_max);
                     c.set3(tail::TailCold::kTr, tr);
@@ -358,7 +360,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
             }
             if (l.active) {
                 state = tail::kLaneWalk;
-                if (walk_begin(S, l.o, l.d, w)) {
+                if (walk_begin<EXT>(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
                     state = tail::kLaneReady;
                 }
@@ -369,7 +371,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
         // traversal steps of every lane with a query in progress
 #pragma unroll
         for (uint32_t k = 0; k < kTailSteps; ++k) {
-            if (state == tail::kLaneWalk && walk_step<COUNT, FEAT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
+            if (state == tail::kLaneWalk && walk_step<COUNT, FEAT, EXT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
                 h = final_hit(w.ab, w.tb);
                 state = tail::kLaneReady;
             }
@@ -465,10 +467,10 @@ __global__ void __launch_bounds__(kBlock) k_trace_query(SceneView S, const float
     TraceCounters tc{0u, 0u, 0u};
     for (uint32_t i = gid; i < n; i += stride) {
         const f3 o = xyz(rays[2 * i]), d = xyz(rays[2 * i + 1]);
-        HitRec h = trace_closest<false>(S, o, d, stk, tc);
+        HitRec h = trace_closest<false, false, true>(S, o, d, stk, tc);
         HitOut r{};
         if (h.code != kMiss) {
-            Surface sf = surface_at(S, o, d, h);
+            Surface sf = surface_at<true>(S, o, d, h);
             r.t = h.t;
             r.n[0] = sf.n.x; r.n[1] = sf.n.y; r.n[2] = sf.n.z;
             r.hit = sf.kind;
@@ -515,11 +517,11 @@ __global__ void __launch_bounds__(kBlock) k_gbuffer(SceneView S, GbufferArgs A, 
         const f3 d = normalize((F.fwd + F.right * ((ux * F.aspect) * F.fovtan)) + F.up * (uy * F.fovtan));
         // shutter time 1: a translated mesh at its current translation
         // (PathTrace.slang:5470-5478)
-        const HitRec h = trace_closest<false, true>(S, F.pos, d, stk, tc, 1.0f);
+        const HitRec h = trace_closest<false, true, true>(S, F.pos, d, stk, tc, 1.0f);
         f3 world, nrm{0.0f, 0.0f, 0.0f}, alb{0.0f, 0.0f, 0.0f}, f0{0.0f, 0.0f, 0.0f};
         float depth, rough = 1.0f, dist = 0.0f;
         if (h.code != kMiss) {
-            const Surface sf = surface_at(S, F.pos, d, h);
+            const Surface sf = surface_at<true>(S, F.pos, d, h);
             world = F.pos + d * h.t;
             nrm = sf.n;
             alb = sf.albedo;
@@ -637,13 +639,13 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
             (void)hipEventRecord(o.extend_events[0], stream);
         }
         const FeatureArgs& X = *o.features;
-        if (o.counting && X.sky)
+        if (o.counting && X.ext)
             hipLaunchKernelGGL((k_tail<true, true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
                                W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
         else if (o.counting)
             hipLaunchKernelGGL((k_tail<true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
                                W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
-        else if (X.sky)
+        else if (X.ext)
             hipLaunchKernelGGL((k_tail<false, true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
                                W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
         else

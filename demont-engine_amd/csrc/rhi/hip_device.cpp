@@ -156,7 +156,11 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     need(r.env_map_present && hdri > 8u, "more than 8 HDRI cluster lights");
     need(f32(p, kVolParams) > 0.0f || f32(p, kCloudsP1 + 12) > 0.0f || f32(p, kCloudsP3 + 12) > 0.0f,
          "participating media (fog, clouds, Rayleigh)");                                     // :7011
-    need(u32(p, kSdfParams) != 0u, "SDF clusters");                                          // :2224
+    // SDF clusters (PathTrace.slang:2224-2238): count, step cap, surface distance
+    r.sdf_count = u32(p, kSdfParams);
+    F.sdf_max_iters = u32(p, kSdfParams + 4);
+    F.sdf_epsilon = f32(p, kSdfParamsF);
+    need(r.sdf_count != 0u && u32(p, kSdfParams + 8) != 0u, "SDF iteration heat map (sdf_params.z)");
     need(u32(p, kRestirEnabled) != 0u, "ReSTIR");                                            // :6315
     need(u32(p, kMatOverrideCount) != 0u, "material overrides");                             // :1164
     need(f32(p, kSmokeParams) != 0.0f, "smoke volumes");                                     // :981
@@ -217,6 +221,9 @@ struct SceneKey {
     // has extracted cluster lights (their pixels are out of it)
     std::uint64_t cdf_m = 0, cdf_m_gen = 0, cdf_c = 0, cdf_c_gen = 0;
     std::uint32_t cdf_total = 0;
+    // the SDF clusters (buffer slot 10)
+    std::uint64_t sdf = 0, sdf_gen = 0, sdf_off = 0;
+    std::uint32_t sdf_count = 0;
     bool operator==(const SceneKey&) const = default;
 };
 
@@ -687,6 +694,7 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
     Texture* env = nullptr;
     const Buffer* cdf_m = nullptr;
     const Buffer* cdf_c = nullptr;
+    const Buffer* sdf = nullptr;
     if (P.prim_count) {
         k.prims = cb.buffers_[slot::kPrimBuffer];
         k.prims_off = cb.buffer_offsets_[slot::kPrimBuffer];
@@ -720,6 +728,19 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
             report("pathtrace: tlas_present is set but accel slot 2 holds no TLAS");
             return false;
         }
+    }
+    if (P.sdf_count) {
+        k.sdf = cb.buffers_[slot::kSdfBuffer];
+        k.sdf_off = cb.buffer_offsets_[slot::kSdfBuffer];
+        sdf = buffer(k.sdf);
+        if (!sdf || k.sdf_off > sdf->data.size() ||
+            (sdf->data.size() - k.sdf_off) / (DMT_SDF_CLUSTER_FLOATS * 4u) < P.sdf_count) {
+            report("pathtrace: sdf_params.x " + std::to_string(P.sdf_count) + " needs " +
+                   std::to_string((std::size_t)P.sdf_count * DMT_SDF_CLUSTER_FLOATS * 4u) + " B at buffer slot 10");
+            return false;
+        }
+        k.sdf_gen = sdf->gen;
+        k.sdf_count = P.sdf_count;
     }
     if (P.env_map_present) {
         k.env = cb.textures_[slot::kEnvTexture];
@@ -832,7 +853,14 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
             d.env_total_luminance = P.env_total_luminance;
         }
     }
-    // the light records are copied out of the byte buffer so they are float-aligned
+    // the cluster and light records are copied out of the byte buffers so they are float-aligned
+    std::vector<float> sdf_copy;
+    if (sdf) {
+        sdf_copy.resize((std::size_t)P.sdf_count * DMT_SDF_CLUSTER_FLOATS);
+        std::memcpy(sdf_copy.data(), sdf->data.data() + k.sdf_off, sdf_copy.size() * 4u);
+        d.sdf_clusters = sdf_copy.data();
+        d.sdf_cluster_count = P.sdf_count;
+    }
     std::vector<float> light_copy;
     if (lights) {
         light_copy.resize((std::size_t)P.light_count * kLightFloats);

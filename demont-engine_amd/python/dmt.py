@@ -24,7 +24,7 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
 DMT_FRAME_FIXED_JITTER = 2
 SCHEDULE_WAVEFRONT = 0xFFFFFFFF
@@ -34,6 +34,7 @@ DMT_NOALLOC_SPILL_DEPTH = 48    # and its static per-thread stack spill
 
 PRIM_FLOATS = 28
 LIGHT_FLOATS = 20
+SDF_CLUSTER_FLOATS = 112
 
 
 class FrameParams(C.Structure):
@@ -72,6 +73,8 @@ class FrameParams(C.Structure):
         ("hdri_light_count", C.c_uint32),
         ("hdri_dir", (C.c_float * 4) * 8),
         ("hdri_col", (C.c_float * 4) * 8),
+        ("sdf_max_iters", C.c_uint32),
+        ("sdf_epsilon", C.c_float),
     ]
 
     def copy(self) -> "FrameParams":
@@ -92,6 +95,8 @@ class SceneDesc(C.Structure):
         ("env_marginal", C.POINTER(C.c_float)),
         ("env_conditional", C.POINTER(C.c_float)),
         ("env_total_luminance", C.c_double),
+        ("sdf_clusters", C.POINTER(C.c_float)), ("sdf_cluster_count", C.c_uint32),
+        ("reserved2", C.c_uint32),
     ]
 
 
@@ -129,7 +134,7 @@ class Stats(C.Structure):
 class GenOptions(C.Structure):
     _fields_ = [("subdivisions", C.c_int32), ("random_tris", C.c_int32), ("spheres", C.c_int32),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("env_width", C.c_uint32),
-                ("env_height", C.c_uint32), ("lights", C.c_int32)]
+                ("env_height", C.c_uint32), ("lights", C.c_int32), ("sdf_clusters", C.c_int32)]
 
 
 class GeneratedScene(C.Structure):
@@ -141,6 +146,7 @@ class GeneratedScene(C.Structure):
         ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
         ("env_intensity", C.c_float),
         ("frame", FrameParams),
+        ("sdf_clusters", C.POINTER(C.c_float)), ("sdf_cluster_count", C.c_uint32),
     ]
 
 
@@ -289,6 +295,7 @@ class SceneData:
     frame: FrameParams = field(default_factory=FrameParams)
     env_cdf: tuple | None = None           # (marginal (H,), conditional (H, W), total) given instead of built
     moon: np.ndarray | None = None         # (H, W, 4) moon texture
+    sdf: np.ndarray | None = None          # (N, 112) SDF cluster records
 
     def desc(self) -> SceneDesc:
         for name in ("prims", "lights", "positions"):
@@ -308,13 +315,17 @@ class SceneData:
             self._cdf = (np.ascontiguousarray(m, np.float32), np.ascontiguousarray(c, np.float32))
             d.env_marginal, d.env_conditional = _fptr(self._cdf[0]), _fptr(self._cdf[1])
             d.env_total_luminance = float(t)
+        if self.sdf is not None and np.asarray(self.sdf).size:
+            self.sdf = np.ascontiguousarray(self.sdf, np.float32).reshape(-1, SDF_CLUSTER_FLOATS)
+            d.sdf_clusters, d.sdf_cluster_count = _fptr(self.sdf), self.sdf.shape[0]
         return d
 
 
 def generate(config: int, seed: int = 1, subdivisions: int = -1, random_tris: int = -1, spheres: int = -1,
-             width: int = 0, height: int = 0, env_width: int = 0, env_height: int = 0, lights: int = -1) -> SceneData:
+             width: int = 0, height: int = 0, env_width: int = 0, env_height: int = 0, lights: int = -1,
+             sdf_clusters: int = -1) -> SceneData:
     """Seeded synthetic workload (dmt_scene_generate), copied to numpy."""
-    opt = GenOptions(subdivisions, random_tris, spheres, width, height, env_width, env_height, lights)
+    opt = GenOptions(subdivisions, random_tris, spheres, width, height, env_width, env_height, lights, sdf_clusters)
     out = C.POINTER(GeneratedScene)()
     _check(lib().dmt_scene_generate(config, seed, C.byref(opt), C.byref(out)))
     g = out.contents
@@ -331,6 +342,8 @@ def generate(config: int, seed: int = 1, subdivisions: int = -1, random_tris: in
             env=arr(g.env_rgb, g.env_width * g.env_height * 3, np.float32, (g.env_height, g.env_width, 3)),
             env_intensity=g.env_intensity,
             frame=g.frame.copy() if hasattr(g.frame, "copy") else g.frame,
+            sdf=arr(g.sdf_clusters, g.sdf_cluster_count * SDF_CLUSTER_FLOATS, np.float32,
+                    (g.sdf_cluster_count, SDF_CLUSTER_FLOATS)) if g.sdf_cluster_count else None,
         )
         f = FrameParams()
         C.memmove(C.byref(f), C.byref(g.frame), C.sizeof(FrameParams))
@@ -366,7 +379,13 @@ def generate_batch_scene(seed: int, index: int, scale: float = 1.0) -> SceneData
 
 
 def look_at(eye, target, fov_deg: float, width: int, height: int, base: FrameParams | None = None) -> FrameParams:
-    f = base.copy() if base is not None else FrameParams()
+    """A frame with the camera at eye looking at target; the other fields
+    from base, or zero with the engine's SDF trace defaults (128 steps, 1e-4)."""
+    if base is not None:
+        f = base.copy()
+    else:
+        f = FrameParams()
+        f.sdf_max_iters, f.sdf_epsilon = 128, 1e-4
     e = (C.c_float * 3)(*eye)
     t = (C.c_float * 3)(*target)
     _check(lib().dmt_camera_look_at(e, t, fov_deg, width, height, C.byref(f)))

@@ -77,6 +77,9 @@ py::dict decoded(const PathtracePush& P) {
     d["hdri_dir"] = hd;
     d["hdri_col"] = hc;
     d["env_total_luminance"] = P.env_total_luminance;
+    d["sdf_count"] = P.sdf_count;
+    d["sdf_max_iters"] = F.sdf_max_iters;
+    d["sdf_epsilon"] = F.sdf_epsilon;
     d["curr_view_proj"] = std::vector<float>(P.curr_view_proj, P.curr_view_proj + 16);
     d["prev_view_proj"] = std::vector<float>(P.prev_view_proj, P.prev_view_proj + 16);
     d["ignored"] = P.ignored;

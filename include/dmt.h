@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define DMT_ABI_VERSION 5u
+#define DMT_ABI_VERSION 6u
 
 typedef int32_t dmt_status;
 #define DMT_OK                  0
@@ -83,7 +83,19 @@ typedef struct dmt_scene_desc {
     const float*    env_marginal;
     const float*    env_conditional;
     double          env_total_luminance;
+    /* Optional SDF clusters (the engine's sdf_clusters buffer, slot 10;
+     * Engine.cpp:5076-5190): DMT_SDF_CLUSTER_FLOATS floats each --
+     * hdr0 (aabb_min.xyz, node_count bits), hdr1 (aabb_max.xyz, material
+     * bits), hdr2 (albedo, roughness), hdr3 (ior, 0, 0, 0), then 8 nodes of
+     * (op, shape, child_a, child_b bits), params, (center.xyz, 0) in
+     * post-order.  Sphere-traced as PathTrace.slang:2163-2306 does in the
+     * build without procedural ops; fractal roots are DMT_ERR_UNSUPPORTED. */
+    const float*    sdf_clusters;
+    uint32_t        sdf_cluster_count;
+    uint32_t        reserved2;
 } dmt_scene_desc;
+
+#define DMT_SDF_CLUSTER_FLOATS 112u
 
 /* Per-frame parameters: the PtPush fields the path reads
  * (Engine.cpp:7243, 7821-8676; camera basis Engine.cpp:7821-7828). */
@@ -146,6 +158,12 @@ typedef struct dmt_frame_params {
     float    hdri_dir[8][4];    /* hdri_lights_dir: unit direction, pick
                                    probability (summing to 1)              */
     float    hdri_col[8][4];    /* hdri_lights_col: rgb flux, 0              */
+    /* SDF sphere trace (sdf_params.y, sdf_params_f.x), read when the scene
+     * has SDF clusters: the step cap, 1..256 (r_sdf_max_iters, default 128),
+     * and the surface distance, > 0 (r_sdf_epsilon, default 1e-4; the
+     * engine clamps it to >= 1e-7). */
+    uint32_t sdf_max_iters;
+    float    sdf_epsilon;
 } dmt_frame_params;
 
 /* Russian roulette off (the reference applies it from bounce 2 on,
@@ -176,8 +194,9 @@ typedef struct dmt_build_params {
 typedef struct dmt_hit {
     float    t;
     float    normal[3];
-    uint32_t hit;               /* 0 miss, 1 analytic, 2 mesh                */
-    uint32_t prim;              /* analytic: input index; mesh: triangle id  */
+    uint32_t hit;               /* 0 miss, 1 analytic, 2 mesh, 3 SDF cluster */
+    uint32_t prim;              /* analytic: input index; mesh: triangle id;
+                                   SDF: cluster index                        */
     uint32_t mat;
     uint32_t user_id;           /* analytic v2.w bits; 0 for mesh            */
 } dmt_hit;
@@ -321,7 +340,8 @@ typedef struct dmt_gbuffer_params {
 dmt_status dmt_render_gbuffer(dmt_scene* scene, const dmt_frame_params* frame, const dmt_gbuffer_params* params);
 
 /* traceScene() closest-hit query (PathTrace.slang:2331-2693) for n rays,
- * 8 floats per ray (ox, oy, oz, -, dx, dy, dz, -); host in, host out. */
+ * 8 floats per ray (ox, oy, oz, -, dx, dy, dz, -); host in, host out.
+ * SDF clusters are traced with the engine's default 128 steps and 1e-4. */
 dmt_status dmt_trace_closest(dmt_scene* scene, const float* rays, uint32_t n, dmt_hit* out);
 
 /* No counterpart in the reference.  For callers that cannot allocate device

@@ -138,6 +138,7 @@ constexpr std::uint32_t kPrimBuffer = 3;       /* BindBuffer(3, primitives)     
 constexpr std::uint32_t kEnvMarginalBuffer = 4;      /* BindBuffer(4, env_marginal_cdf)    */
 constexpr std::uint32_t kEnvConditionalBuffer = 5;   /* BindBuffer(5, env_conditional_cdf) */
 constexpr std::uint32_t kExposureBuffer = 6;   /* BindBuffer(6, exposure_state)          */
+constexpr std::uint32_t kSdfBuffer = 10;       /* BindBuffer(10, sdf_clusters)           */
 constexpr std::uint32_t kLightBuffer = 12;     /* BindBuffer(12, lights)                 */
 constexpr std::uint32_t kBufferSlots = 24;     /* BindState sizes (SoftwareDevice.h:112-124) */
 constexpr std::uint32_t kTextureSlots = 20;
@@ -161,7 +162,7 @@ constexpr std::size_t kDofParams = 320, kDofBokehParams = 336, kSpectralParams =
 constexpr std::size_t kCloudsP1 = 384, kCloudsP3 = 416, kMoonDirPhase = 448, kMoonExtra = 464, kSunExtra = 480;
 constexpr std::size_t kHdriLightsDir = 496, kHdriLightsCol = 624;
 constexpr std::size_t kHdriLightsCount = 752, kWriteNormalGbuffer = 756, kWriteAlbedoGbuffer = 760;
-constexpr std::size_t kWriteHdrAux = 764, kMisEnabled = 768, kAccumParams = 784, kSdfParams = 816;
+constexpr std::size_t kWriteHdrAux = 764, kMisEnabled = 768, kAccumParams = 784, kSdfParams = 816, kSdfParamsF = 832;
 constexpr std::size_t kWriteShadowVis = 856, kLightCount = 880, kLightTreeEnabled = 888;
 constexpr std::size_t kWriteSpecularAlbedo = 896, kWriteRoughness = 900, kWriteSpecularHitDistance = 904;
 constexpr std::size_t kRestirEnabled = 944, kSmokeParams = 960, kMotionBlurParams = 1024, kCloudsRuntime = 1072;
@@ -179,6 +180,7 @@ struct PathtracePush {
     std::uint32_t env_map_present = 0;
     float env_intensity = 1.0f;
     float env_total_luminance = 0.0f;   /* the engine's CDF total (used with its CDF buffers) */
+    std::uint32_t sdf_count = 0;    /* sdf_params.x: SDF clusters at buffer slot 10 */
     std::uint32_t tonemap_op = 0;   /* r_tonemap_op; values above 4 select ACES */
     std::uint32_t motion_blur = 0;  /* motion_blur_params.x > 0.5 (also frame.motion_blur) */
     /* denoiser inputs (PathTrace.slang:5463-5648, 7797-7811) */
@@ -197,13 +199,15 @@ struct PathtracePush {
  * reason in *error, for a push shorter than the 112-byte head (the software
  * tracer's no-op, SoftwareTracer.cpp:686-690) and, when strict, for a push that
  * turns on a feature this backend does not render: a procedural / Hosek sky
- * (sun_and_mode.w 2 or 3), participating media, SDF clusters, ReSTIR,
+ * (sun_and_mode.w 2 or 3), participating media, the SDF iteration heat map, ReSTIR,
  * material overrides, smoke, the SIGMA shadow-visibility output, a
  * per-frame denoiser colour input without reset_accum, or PBR mesh textures.
  * Lenient decoding renders without them and lists them in out->ignored.  The
  * gradient sky's sun and moon (sun_and_mode, sun_extra, sun_extra2,
  * moon_dir_phase, moon_extra), the HDRI cluster lights (hdri_lights_*) and,
- * while a denoiser runs, the Halton jitter are decoded into the frame.
+ * while a denoiser runs, the Halton jitter are decoded into the frame, and
+ * the SDF clusters' count and sphere-trace parameters (sdf_params.xy,
+ * sdf_params_f.x) into sdf_count and the frame.
  * Fields past the pushed size read as zero. */
 bool DecodePathtracePush(const void* data, std::size_t size, bool strict, PathtracePush* out, std::string* error);
 

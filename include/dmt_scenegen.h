@@ -14,8 +14,12 @@
  *             seeded value noise (amplitude 0.1), plus 200k random triangles
  *             (edges <= 0.05) uniform in [-5,5]^3; one quad light; 512x256
  *             env with lognormal texels exp(N(0,1)); 1920x1080, 1 spp, 8 bounces.
- *   config 2  4,096 spheres (centres U[-4.5,4.5]^3, radii U[0.05,0.3], config 0
- *             materials) over the plane y = -5, a 500k-triangle mesh
+ *   config 2  4,096 analytic objects over the plane y = -5: 3,584 spheres
+ *             (centres U[-4.5,4.5]^3, radii U[0.05,0.3], config 0 materials)
+ *             and 512 SDF clusters (boxes, rounded boxes, a box united with a
+ *             sphere, a sphere smoothly united with a capsule, a box with a
+ *             sphere carved out, tori; sizes ~0.1-0.3, centres U[-4.5,4.5]^3,
+ *             config 0 materials), a 500k-triangle mesh
  *             (icosphere-7 of radius 2.5 + 172,320 random small triangles in
  *             [-5,5]^3), lit only by 4,096 emissive triangles (light type 4,
  *             edges ~0.3, facing down, centres in [-6,6] x [5,7] x [-6,6],
@@ -42,6 +46,7 @@ typedef struct dmt_gen_options {
     uint32_t width, height;  /* 0: config default                              */
     uint32_t env_width, env_height; /* 0: config default                       */
     int32_t  lights;         /* emissive triangle lights (config 2; < 0: default) */
+    int32_t  sdf_clusters;   /* SDF clusters (config 2; < 0: default 512)      */
 } dmt_gen_options;
 
 typedef struct dmt_generated_scene {
@@ -52,6 +57,7 @@ typedef struct dmt_generated_scene {
     float*           env_rgb;    uint32_t env_width;  uint32_t env_height;
     float            env_intensity;
     dmt_frame_params frame;      /* the config's camera and render settings  */
+    float*           sdf_clusters; uint32_t sdf_cluster_count;   /* DMT_SDF_CLUSTER_FLOATS each */
 } dmt_generated_scene;
 
 dmt_status dmt_scene_generate(uint32_t config, uint64_t seed, const dmt_gen_options* options,

@@ -35,6 +35,9 @@ typedef struct orc_scene_desc {
     double          env_total_luminance;
     /* optional moon texture (RGBA floats) of the moon disc */
     const float*    moon_rgba;  uint32_t moon_width; uint32_t moon_height;
+    /* optional SDF clusters, 112 floats each (the engine's sdf_clusters
+     * records, Engine.cpp:5076-5190), traced after the analytic prims */
+    const float*    sdf_clusters; uint32_t sdf_cluster_count;
 } orc_scene_desc;
 
 typedef struct orc_frame {
@@ -78,6 +81,10 @@ typedef struct orc_frame {
     uint32_t hdri_light_count;
     float    hdri_dir[8][4];
     float    hdri_col[8][4];
+    /* SDF sphere trace: step cap and surface distance (sdf_params.y,
+     * sdf_params_f.x), read when the scene has SDF clusters */
+    uint32_t sdf_max_iters;
+    float    sdf_epsilon;
 } orc_frame;
 
 typedef struct orc_hit {
@@ -136,6 +143,15 @@ void orc_sample_sun_disc(const float* dir, uint32_t seed0, uint32_t n, float* ou
 /* Closest-hit query (traceScene) for n rays: rays are 8 floats each
  * (ox, oy, oz, unused, dx, dy, dz, unused). */
 int orc_trace(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
+/* Signed distance and unit normal (sdfClusterDist, normalize(sdfClusterNormal),
+ * SdfPrimitives.slang:377-491) of one 112-float SDF cluster at n points
+ * (3 floats each): out 4 floats each (d, nx, ny, nz). */
+void orc_sdf_eval(const float* cluster, const float* points, uint32_t n, float* out);
+/* sdfClusterTrace (SdfPrimitives.slang:1330-1444) of one cluster for n rays
+ * (8 floats each) with t_min 1e-3 and t_max 1e30: out 4 floats each
+ * (t or -1 for none, normal xyz). */
+void orc_sdf_trace(const float* cluster, const float* rays, uint32_t n, uint32_t max_iters, float epsilon,
+                   float* out);
 /* Same through an exhaustive linear scan (no BVH). */
 int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out);
 /* traceScene at shutter time `time` (spheres at lerp(v3, v0, time)); with

@@ -26,6 +26,7 @@ class OrcSceneDesc(C.Structure):
         ("env_marginal", C.POINTER(C.c_float)), ("env_conditional", C.POINTER(C.c_float)),
         ("env_total_luminance", C.c_double),
         ("moon_rgba", C.POINTER(C.c_float)), ("moon_width", C.c_uint32), ("moon_height", C.c_uint32),
+        ("sdf_clusters", C.POINTER(C.c_float)), ("sdf_cluster_count", C.c_uint32),
     ]
 
 
@@ -88,6 +89,10 @@ def lib():
         L.orc_sky.restype = C.c_int
         L.orc_sample_sun_disc.argtypes = [fp, u32, u32, fp]
         L.orc_sample_sun_disc.restype = None
+        L.orc_sdf_eval.argtypes = [fp, fp, u32, fp]
+        L.orc_sdf_eval.restype = None
+        L.orc_sdf_trace.argtypes = [fp, fp, u32, u32, C.c_float, fp]
+        L.orc_sdf_trace.restype = None
         _lib = L
     return _lib
 
@@ -101,7 +106,8 @@ def _fp(a):
 class OracleScene:
     """Oracle copy of a scene given in the reference wire formats."""
 
-    def __init__(self, prims, lights, positions, indices, env=None, env_intensity=1.0, env_cdf=None, moon=None):
+    def __init__(self, prims, lights, positions, indices, env=None, env_intensity=1.0, env_cdf=None, moon=None,
+                 sdf=None):
         self._keep = [np.ascontiguousarray(prims, np.float32), np.ascontiguousarray(lights, np.float32),
                       np.ascontiguousarray(positions, np.float32), np.ascontiguousarray(indices, np.uint32)]
         d = OrcSceneDesc()
@@ -125,6 +131,10 @@ class OracleScene:
             mo = np.ascontiguousarray(moon, np.float32)
             self._keep.append(mo)
             d.moon_rgba, d.moon_height, d.moon_width = _fp(mo), mo.shape[0], mo.shape[1]
+        if sdf is not None and np.asarray(sdf).size:
+            sc = np.ascontiguousarray(sdf, np.float32).reshape(-1, 112)
+            self._keep.append(sc)
+            d.sdf_clusters, d.sdf_cluster_count = _fp(sc), sc.shape[0]
         self._h = C.c_void_p()
         rc = lib().orc_scene_create(C.byref(d), C.byref(self._h))
         if rc != 0:
@@ -133,7 +143,7 @@ class OracleScene:
     @classmethod
     def from_scene_data(cls, sd):
         return cls(sd.prims, sd.lights, sd.positions, sd.indices, sd.env, sd.env_intensity,
-                   getattr(sd, "env_cdf", None), getattr(sd, "moon", None))
+                   getattr(sd, "env_cdf", None), getattr(sd, "moon", None), getattr(sd, "sdf", None))
 
     def render(self, frame, precision=32, threads=0, rows=None, col_step=1, accum=None):
         """Render one frame; `frame` is any ctypes struct with the dmt_frame_params layout.
@@ -309,4 +319,26 @@ def sample_sun_disc(direction, seed0, n):
     d = np.ascontiguousarray(direction, np.float32)
     out = np.zeros((n, 3), np.float32)
     lib().orc_sample_sun_disc(_fp(d), seed0, n, _fp(out))
+    return out
+
+
+def sdf_eval(cluster, points):
+    """(distance, unit normal) of one 112-float SDF cluster at points (n, 3): (n, 4)."""
+    c = np.ascontiguousarray(cluster, np.float32).reshape(112)
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.zeros((p.shape[0], 4), np.float32)
+    lib().orc_sdf_eval(_fp(c), _fp(p), p.shape[0], _fp(out))
+    return out
+
+
+def sdf_trace(cluster, origins, dirs, max_iters=128, epsilon=1e-4):
+    """sdfClusterTrace of one cluster (t_min 1e-3, t_max 1e30): (n, 4) of
+    (t or -1, normal xyz)."""
+    c = np.ascontiguousarray(cluster, np.float32).reshape(112)
+    n = np.asarray(origins).reshape(-1, 3).shape[0]
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = np.asarray(origins, np.float32).reshape(-1, 3)
+    rays[:, 4:7] = np.asarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros((n, 4), np.float32)
+    lib().orc_sdf_trace(_fp(c), _fp(rays), n, max_iters, epsilon, _fp(out))
     return out

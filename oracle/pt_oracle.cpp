@@ -368,6 +368,8 @@ struct orc_scene {
     float env_total = 0.0f;
     std::vector<float> moon;         // RGBA moon texture (empty: none)
     uint32_t moon_w = 0, moon_h = 0;
+    std::vector<float> sdf;          // SDF clusters, 112 floats each
+    uint32_t sdf_count = 0;
     // tangent direction of a PT<Dual> run: material parameter k (albedo rgb,
     // roughness, emission rgb) of analytic prim tangent_prim
     uint32_t tangent_prim = 0xFFFFFFFFu;
@@ -385,7 +387,7 @@ template <class R> struct Hit {
     R rough = R(0), ior = R(1);
     V3<R> emission{R(0), R(0), R(0)};
     uint32_t user_id = 0;
-    uint32_t kind = 0;   // 0 miss 1 analytic 2 mesh
+    uint32_t kind = 0;   // 0 miss 1 analytic 2 mesh 3 SDF cluster
     uint32_t prim = 0;
 };
 
@@ -522,10 +524,224 @@ template <class R> inline V3<R> safe_inv(V3<R> rd) {
     return V3<R>{R(1) / f(rd.x), R(1) / f(rd.y), R(1) / f(rd.z)};
 }
 
-// traceScene (PathTrace.slang:2331-2693) at shutter time `time`; the mesh is
-// intersected from ro_mesh = ro - its translation at that time (:2483-2487).
+// SDF clusters (shaders/SdfPrimitives.slang, traceSdfClusters
+// PathTrace.slang:2212-2306) as the reference builds them without procedural
+// ops (PT_SDF_PROCEDURAL_OPS 0, SdfPrimitives.slang:56-58): a node with an op
+// beyond displace evaluates to 1e30, and the normal is the analytic one.
+// min/max are IEEE minNum/maxNum (a NaN operand yields the other), as the
+// GPU's.  Cluster record: hdr0 (aabb_min, node_count bits), hdr1 (aabb_max,
+// material bits), hdr2 (albedo, roughness), hdr3 (ior), then node i at float
+// 16 + 12 i: (op, shape, child_a, child_b bits), params, (center, 0).
+template <class S> struct Sdf {
+    using V = V3<S>;
+    static S mn(S a, S b) { return std::fmin(a, b); }
+    static S mx(S a, S b) { return std::fmax(a, b); }
+    static S clamp01(S x) { return mn(mx(x, S(0)), S(1)); }
+    static S sgn(S x) { return x > S(0) ? S(1) : (x < S(0) ? S(-1) : S(0)); }
+    static S len2(S x, S y) { return osqrt(x * x + y * y); }
+    static V vabs(V p) { return V{oabs(p.x), oabs(p.y), oabs(p.z)}; }
+
+    // sdSphere .. sdPlane (SdfPrimitives.slang:163-195), evalLeafDist (:308-322)
+    static S shape_dist(V p, uint32_t shape, const float* q) {
+        switch (shape) {
+            case 0: return length(p) - S(q[0]);
+            case 1:
+            case 2: {
+                V a = vabs(p) - V{S(q[0]), S(q[1]), S(q[2])};
+                S d = length(V{mx(a.x, S(0)), mx(a.y, S(0)), mx(a.z, S(0))}) + mn(mx(a.x, mx(a.y, a.z)), S(0));
+                return shape == 1 ? d : d - S(q[3]);
+            }
+            case 3: return len2(len2(p.x, p.z) - S(q[0]), p.y) - S(q[1]);
+            case 4: {
+                S y = mn(mx(p.y, -S(q[0])), S(q[0]));
+                return length(V{p.x, p.y - y, p.z}) - S(q[1]);
+            }
+            case 5: return dot(p, V{S(q[0]), S(q[1]), S(q[2])}) + S(q[3]);
+            default: return S(1e30f);
+        }
+    }
+    // sdSphereNormal .. sdPlaneNormal (:203-261), evalLeafNormal (:324-340)
+    static V shape_normal(V p, uint32_t shape, const float* q) {
+        const V up{S(0), S(1), S(0)};
+        switch (shape) {
+            case 0: {
+                S l = length(p);
+                return l > S(1e-12f) ? p / l : up;
+            }
+            case 1:
+            case 2: {
+                V a = vabs(p) - V{S(q[0]), S(q[1]), S(q[2])};
+                V sg{sgn(p.x), sgn(p.y), sgn(p.z)};
+                if (mx(a.x, mx(a.y, a.z)) > S(0)) {
+                    V m{mx(a.x, S(0)), mx(a.y, S(0)), mx(a.z, S(0))};
+                    S lm = length(m);
+                    if (lm < S(1e-12f)) return sg * up;
+                    return (m / lm) * sg;
+                }
+                V n{S(0), S(0), S(0)};
+                if (a.x > a.y && a.x > a.z) n.x = sg.x;
+                else if (a.y > a.z) n.y = sg.y;
+                else n.z = sg.z;
+                return n;
+            }
+            case 3: {
+                S lxz = len2(p.x, p.z);
+                if (lxz < S(1e-12f)) return V{S(0), sgn(p.y), S(0)};
+                S k = S(q[0]) / lxz;
+                V d = p - V{p.x * k, S(0), p.z * k};
+                S ld = length(d);
+                return ld > S(1e-12f) ? d / ld : up;
+            }
+            case 4: {
+                S y = mn(mx(p.y, -S(q[0])), S(q[0]));
+                V d = p - V{S(0), y, S(0)};
+                S l = length(d);
+                return l > S(1e-12f) ? d / l : up;
+            }
+            case 5: return V{S(q[0]), S(q[1]), S(q[2])};
+            default: return up;
+        }
+    }
+    // the blend factor h of opSmoothUnion / Subtract / Intersect (:276-292)
+    static S blend(uint32_t op, S da, S db, S k) {
+        if (op == 1) return clamp01(S(0.5f) + (S(0.5f) * (db - da)) / k);
+        if (op == 2) return clamp01(S(0.5f) - (S(0.5f) * (db + da)) / k);
+        return clamp01(S(0.5f) - (S(0.5f) * (db - da)) / k);
+    }
+    // lerp(a, b, h) = a + (b - a) h
+    static S combine(uint32_t op, S da, S db, S k) {
+        if (op == 1) {
+            if (k <= S(0)) return mn(da, db);
+            S h = blend(op, da, db, k);
+            return (db + (da - db) * h) - (k * h) * (S(1) - h);
+        }
+        if (op == 2) {
+            if (k <= S(0)) return mx(da, -db);
+            S h = blend(op, da, db, k);
+            return (da + (-db - da) * h) + (k * h) * (S(1) - h);
+        }
+        if (k <= S(0)) return mx(da, db);
+        S h = blend(op, da, db, k);
+        return (db + (da - db) * h) + (k * h) * (S(1) - h);
+    }
+    static uint32_t count(const float* c) { return std::min(f2u(c[3]), 8u); }
+    // sdfClusterDist (:377-400)
+    static S dist(const float* c, V p) {
+        S dv[8];
+        const uint32_t n = count(c);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* nd = c + 16 + 12 * i;
+            const uint32_t op = f2u(nd[0]);
+            if (op == 0) dv[i] = shape_dist(p - V{S(nd[8]), S(nd[9]), S(nd[10])}, f2u(nd[1]), nd + 4);
+            else if (op >= 1 && op <= 3) dv[i] = combine(op, dv[f2u(nd[2])], dv[f2u(nd[3])], S(nd[4]));
+            else if (op == 4) dv[i] = dv[f2u(nd[2])] + S(nd[4]);
+            else dv[i] = S(1e30f);
+        }
+        return n ? dv[n - 1] : S(1e30f);
+    }
+    // sdfClusterNormal (:407-491)
+    static V grad(const float* c, V p) {
+        S dv[8];
+        V gv[8];
+        const V up{S(0), S(1), S(0)};
+        const uint32_t n = count(c);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* nd = c + 16 + 12 * i;
+            const uint32_t op = f2u(nd[0]);
+            const S k = S(nd[4]);
+            if (op == 0) {
+                V pl = p - V{S(nd[8]), S(nd[9]), S(nd[10])};
+                dv[i] = shape_dist(pl, f2u(nd[1]), nd + 4);
+                gv[i] = shape_normal(pl, f2u(nd[1]), nd + 4);
+            } else if (op >= 1 && op <= 3) {
+                const S da = dv[f2u(nd[2])], db = dv[f2u(nd[3])];
+                const V ga = gv[f2u(nd[2])], gb = gv[f2u(nd[3])];
+                dv[i] = combine(op, da, db, k);
+                if (k <= S(0)) {
+                    if (op == 1) gv[i] = (da < db) ? ga : gb;
+                    else if (op == 2) gv[i] = (da > -db) ? ga : -gb;
+                    else gv[i] = (da > db) ? ga : gb;
+                } else {
+                    const S h = blend(op, da, db, k);
+                    gv[i] = op == 2 ? normalize(lerp3(ga, -gb, h)) : normalize(lerp3(gb, ga, h));
+                }
+            } else if (op == 4) {
+                dv[i] = dv[f2u(nd[2])] + k;
+                gv[i] = gv[f2u(nd[2])];
+            } else {
+                dv[i] = S(1e30f);
+                gv[i] = up;
+            }
+        }
+        return n ? gv[n - 1] : up;
+    }
+    // sdfClusterTrace (:1330-1444): the slab interval of the cluster box
+    // (sdfClusterAabbT, :1306-1316) clipped to [t_min, t_max], then at most
+    // min(iters, 256) steps of max(|d|, eps) until |d| < eps.
+    static bool march(const float* c, V ro, V rd, V inv, S t_min, S t_max, uint32_t iters, S eps, S& t_hit) {
+        const S t0x = (S(c[0]) - ro.x) * inv.x, t1x = (S(c[4]) - ro.x) * inv.x;
+        const S t0y = (S(c[1]) - ro.y) * inv.y, t1y = (S(c[5]) - ro.y) * inv.y;
+        const S t0z = (S(c[2]) - ro.z) * inv.z, t1z = (S(c[6]) - ro.z) * inv.z;
+        const S t_enter = mx(mx(mn(t0x, t1x), mn(t0y, t1y)), mn(t0z, t1z));
+        const S t_exit = mn(mn(mx(t0x, t1x), mx(t0y, t1y)), mx(t0z, t1z));
+        S t = mx(t_enter, t_min);
+        const S t_end = mn(t_exit, t_max);
+        if (t >= t_end) return false;
+        const uint32_t n = std::min(iters, 256u);
+        for (uint32_t i = 0; i < n; ++i) {
+            const V p = ro + rd * t;
+            const S ad = oabs(dist(c, p));
+            if (ad < eps) {
+                t_hit = t;
+                return true;
+            }
+            t += mx(ad, eps);
+            if (t >= t_end) return false;
+        }
+        return false;
+    }
+};
+
+template <class R> inline double vd(R x) { return value_of(x); }
+
+// traceSdfClusters (PathTrace.slang:2212-2306): every cluster in order, each
+// traced up to the closest distance so far, replacing the hit only when
+// strictly closer.  Float runs trace in float; double and dual runs in double
+// (the clusters carry no differentiated parameter).
 template <class R>
-Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ctr& c, bool brute, R time, V3<R> ro_mesh) {
+void trace_sdf(const orc_scene& S, V3<R> ro, V3<R> rd, Hit<R>& h, Ctr& c, uint32_t iters, float eps) {
+    using SR = std::conditional_t<std::is_same_v<R, float>, float, double>;
+    using SV = V3<SR>;
+    const SV o{SR(vd(ro.x)), SR(vd(ro.y)), SR(vd(ro.z))}, d{SR(vd(rd.x)), SR(vd(rd.y)), SR(vd(rd.z))};
+    auto f = [](SR x) { return oabs(x) > SR(1e-30f) ? x : SR(1e-30f); };
+    const SV inv{SR(1) / f(d.x), SR(1) / f(d.y), SR(1) / f(d.z)};
+    for (uint32_t i = 0; i < S.sdf_count; ++i) {
+        const float* cl = &S.sdf[(size_t)i * 112];
+        c.tests++;
+        SR t;
+        if (!Sdf<SR>::march(cl, o, d, inv, SR(1e-3f), SR(vd(h.t)), iters, SR(eps), t)) continue;
+        if (!(R(t) < h.t)) continue;
+        const SV n = normalize(Sdf<SR>::grad(cl, o + d * t));
+        h.hit = true;
+        h.t = R(t);
+        h.n = V3<R>{R(n.x), R(n.y), R(n.z)};
+        h.mat = f2u(cl[7]);
+        h.albedo = ld3<R>(cl + 8);
+        h.rough = R(cl[11]);
+        h.ior = R(cl[12]);
+        h.emission = V3<R>{R(0), R(0), R(0)};
+        h.user_id = 0u;
+        h.kind = 3u;
+        h.prim = i;
+    }
+}
+
+// traceScene (PathTrace.slang:2331-2693) at shutter time `time`; the mesh is
+// intersected from ro_mesh = ro - its translation at that time (:2483-2487);
+// SDF clusters with sdf_iters steps and surface distance sdf_eps.
+template <class R>
+Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ctr& c, bool brute, R time, V3<R> ro_mesh,
+                   uint32_t sdf_iters = 128u, float sdf_eps = 1e-4f) {
     Hit<R> h;
     for (uint32_t i : S.planes) test_prim(S, i, ro, rd, h, c, time);
     V3<R> inv = safe_inv(rd);
@@ -548,6 +764,7 @@ Hit<R> trace_scene(const orc_scene& S, V3<R> ro, V3<R> rd, Ctr& c, bool brute, R
             }
         }
     }
+    if (S.sdf_count > 0) trace_sdf(S, ro, rd, h, c, sdf_iters, sdf_eps);
     if (S.tri_count > 0) {
         R t_mesh = h.t;
         uint32_t best = 0;
@@ -1007,7 +1224,7 @@ template <class R> struct PT {
     Hit<R> trace(V3<R> ro, V3<R> rd, R time) const {
         V3<R> rm = ro;
         if (F.mesh_shift_active) rm = ro - mix3(ld3<R>(F.mesh_shift_prev), ld3<R>(F.mesh_shift_curr), time);
-        return trace_scene(S, ro, rd, c, false, time, rm);
+        return trace_scene(S, ro, rd, c, false, time, rm, F.sdf_max_iters, F.sdf_epsilon);
     }
 
     // transmittance (PathTrace.slang:5089-5196), no water / clouds.
@@ -1513,6 +1730,10 @@ int orc_scene_create(const orc_scene_desc* d, orc_scene** out) {
             S->env_total = float(build_env_cdf(S->env.data(), S->W, S->H, S->marg, S->cond));
         }
     }
+    if (d->sdf_clusters && d->sdf_cluster_count) {
+        S->sdf_count = d->sdf_cluster_count;
+        S->sdf.assign(d->sdf_clusters, d->sdf_clusters + (size_t)S->sdf_count * 112);
+    }
     if (d->moon_rgba && d->moon_width && d->moon_height) {
         S->moon_w = d->moon_width;
         S->moon_h = d->moon_height;
@@ -1578,6 +1799,33 @@ int orc_trace(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out) {
 }
 int orc_trace_bruteforce(const orc_scene* s, const float* rays, uint32_t n, orc_hit* out) {
     return trace_impl(s, rays, n, out, true);
+}
+
+void orc_sdf_eval(const float* cluster, const float* points, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3<float> p = ld3<float>(points + 3 * (size_t)i);
+        const V3<float> g = normalize(Sdf<float>::grad(cluster, p));
+        float* o = out + 4 * (size_t)i;
+        o[0] = Sdf<float>::dist(cluster, p);
+        o[1] = g.x; o[2] = g.y; o[3] = g.z;
+    }
+}
+
+void orc_sdf_trace(const float* cluster, const float* rays, uint32_t n, uint32_t max_iters, float epsilon,
+                   float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        const V3<float> o = ld3<float>(r), d = ld3<float>(r + 4);
+        const V3<float> inv = safe_inv(d);
+        float t = 0.0f;
+        float* q = out + 4 * (size_t)i;
+        if (Sdf<float>::march(cluster, o, d, inv, 1e-3f, 1e30f, max_iters, epsilon, t)) {
+            const V3<float> g = normalize(Sdf<float>::grad(cluster, o + d * t));
+            q[0] = t; q[1] = g.x; q[2] = g.y; q[3] = g.z;
+        } else {
+            q[0] = -1.0f; q[1] = q[2] = q[3] = 0.0f;
+        }
+    }
 }
 
 int orc_trace_at(const orc_scene* s, const float* rays, uint32_t n, float time, const float* shift_prev,

@@ -31,7 +31,7 @@ OFF = dict(pos_fovtan=0, fwd_aspect=16, right_xyz=32, up_xyz=48, frame_index=64,
            vol_params=368,
            clouds_p1=384, clouds_p3=416, moon_dir_phase=448, hdri_lights_count=752, write_normal_gbuffer=756,
            write_albedo_gbuffer=760, write_hdr_aux=764, mis_enabled=768, accum_params=784, bvh_params=800,
-           sdf_params=816, write_shadow_vis=856, light_count=880, light_tree_node_count=884,
+           sdf_params=816, sdf_params_f=832, write_shadow_vis=856, light_count=880, light_tree_node_count=884,
            light_tree_enabled=888, write_specular_albedo_gbuffer=896, write_roughness_gbuffer=900,
            write_specular_hit_distance_gbuffer=904, restir_enabled=944, smoke_params=960,
            motion_blur_params=1024, mesh_motion_prev=1088, mesh_motion_curr=1104, mesh_tex_indices=1152,
@@ -51,7 +51,7 @@ def _offset(name: str) -> int:
 
 
 def pack_ptpush(frame: "dmt.FrameParams", *, prim_count: int, light_count: int, env_present: bool,
-                env_intensity: float = 1.0, tlas_present: bool = False, tonemap_op: int = 0,
+                env_intensity: float = 1.0, tlas_present: bool = False, tonemap_op: int = 0, sdf_count: int = 0,
                 overrides: dict | None = None) -> bytes:
     """PtPush bytes as the engine fills them for `frame`, with the sun and moon
     below the horizon (the gradient sky without discs) when there is no env map.
@@ -89,6 +89,10 @@ def pack_ptpush(frame: "dmt.FrameParams", *, prim_count: int, light_count: int, 
     put("mesh_motion_prev", "4f", *frame.mesh_shift_prev, 1.0 if frame.mesh_shift_active else 0.0)
     put("mesh_motion_curr", "4f", *frame.mesh_shift_curr, 0.0)
     put("hosek_params", "4f", 2.5, 0.3, 1.2, 0.0)                      # read by sky mode 3 only
+    # SDF clusters at buffer slot 10, the frame's step cap and surface distance,
+    # the fractal knobs at the engine's defaults (Engine.cpp:7947-7981)
+    put("sdf_params", "4I", sdf_count, frame.sdf_max_iters, 0, 12)
+    put("sdf_params_f", "4f", frame.sdf_epsilon, 8.0, 1.0, 0.0)
     put("tonemap_params", "4I", tonemap_op, 0, 0, 0)
     put("sun_extra2", "4f", 1.0, 0.0, 0.0, 0.0)
     for name, (fmt, v) in (overrides or {}).items():
@@ -132,13 +136,18 @@ class EngineSim:
             assert dev.write_texture(self.env, rgba_env(sd.env))
         self.accum = dev.create_texture(self.width, self.height, RGBA32F, STORAGE)
         assert self.accum, dev.last_error()
+        self.sdf = 0
+        self.sdf_count = 0 if sd.sdf is None else int(np.asarray(sd.sdf).reshape(-1, 112).shape[0])
+        if self.sdf_count:
+            self.sdf = dev.create_buffer(self.sdf_count * 448)
+            dev.write_buffer(self.sdf, np.ascontiguousarray(sd.sdf, np.float32))
         self.pipeline = dev.create_pipeline("pathtrace")
         dev.resize(self.width, self.height)
 
     def push(self, frame: "dmt.FrameParams", **kw) -> bytes:
         return pack_ptpush(frame, prim_count=self.sd.prims.shape[0], light_count=self.sd.lights.shape[0],
                            env_present=self.env_present, env_intensity=self.sd.env_intensity,
-                           tlas_present=bool(self.tlas), **kw)
+                           tlas_present=bool(self.tlas), sdf_count=self.sdf_count, **kw)
 
     def frame(self, frame: "dmt.FrameParams", push: bytes | None = None, textures: dict | None = None,
               buffers: dict | None = None, **kw):
@@ -155,6 +164,8 @@ class EngineSim:
         if self.env:
             d.bind_texture(5, self.env)
         d.bind_buffer(6, self.exposure)
+        if self.sdf:
+            d.bind_buffer(10, self.sdf)
         d.bind_buffer(12, self.lights)
         d.bind_buffer(13, self.tree)
         for slot, tex in (textures or {}).items():

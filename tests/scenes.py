@@ -89,13 +89,14 @@ def frame(eye, target, fov, w, h, spp=1, bounces=4, frame_index=0):
     return f
 
 
-def scene(prims=(), lights=(), positions=None, indices=None, env=None, env_intensity=1.0):
+def scene(prims=(), lights=(), positions=None, indices=None, env=None, env_intensity=1.0, sdf=None):
     return dmt.SceneData(
         prims=np.array(prims, np.float32).reshape(-1, 28),
         lights=np.array(lights, np.float32).reshape(-1, 20),
         positions=np.zeros((0, 3), np.float32) if positions is None else np.asarray(positions, np.float32),
         indices=np.zeros((0, 3), np.uint32) if indices is None else np.asarray(indices, np.uint32),
-        env=env, env_intensity=env_intensity)
+        env=env, env_intensity=env_intensity,
+        sdf=None if sdf is None or len(sdf) == 0 else np.asarray(sdf, np.float32).reshape(-1, 112))
 
 
 def black_env(w=8, h=4):
@@ -113,3 +114,37 @@ def polygon_irradiance(x, n, verts):
         c = np.cross(a, b)
         e += math.acos(max(-1.0, min(1.0, float(np.dot(a, b))))) * float(np.dot(n, c / np.linalg.norm(c)))
     return abs(0.5 * e)
+
+
+SDF_SPHERE, SDF_BOX, SDF_ROUND_BOX, SDF_TORUS, SDF_CAPSULE, SDF_PLANE = range(6)
+SDF_LEAF, SDF_UNION, SDF_SUBTRACT, SDF_INTERSECT, SDF_DISPLACE = range(5)
+
+
+def sdf_node(op=SDF_LEAF, shape=0, a=0, b=0, params=(0, 0, 0, 0), center=(0, 0, 0)):
+    return (op, shape, a, b, tuple(params), tuple(center))
+
+
+def sdf_cluster(nodes, lo, hi, mat=MAT_LAMBERT, albedo=(0.5, 0.5, 0.5), rough=0.0, ior=1.0):
+    """One SDF cluster record (112 floats, the engine's sdf_clusters layout):
+    nodes in post-order as sdf_node tuples, box [lo, hi]."""
+    r = np.zeros(112, np.float32)
+    u = r.view(np.uint32)
+    r[0:3], r[4:7] = lo, hi
+    u[3], u[7] = len(nodes), mat
+    r[8:11], r[11], r[12] = albedo, rough, ior
+    for i, (op, shape, a, b, q, c) in enumerate(nodes):
+        o = 16 + 12 * i
+        u[o:o + 4] = (op, shape, a, b)
+        r[o + 4:o + 8] = q
+        r[o + 8:o + 11] = c
+    return r
+
+
+def sdf_box(center, half, pad=1e-3, **kw):
+    c, h = np.asarray(center, np.float32), np.asarray(half, np.float32)
+    return sdf_cluster([sdf_node(SDF_LEAF, SDF_BOX, params=(*h, 0.0), center=c)], c - h - pad, c + h + pad, **kw)
+
+
+def sdf_ball(center, r, pad=1e-3, **kw):
+    c = np.asarray(center, np.float32)
+    return sdf_cluster([sdf_node(SDF_LEAF, SDF_SPHERE, params=(r, 0, 0, 0), center=c)], c - r - pad, c + r + pad, **kw)

@@ -35,7 +35,7 @@ DECODER_FIELDS = {
     "kMoonDirPhase": "moon_dir_phase", "kHdriLightsCount": "hdri_lights_count",
     "kWriteNormalGbuffer": "write_normal_gbuffer", "kWriteAlbedoGbuffer": "write_albedo_gbuffer",
     "kWriteHdrAux": "write_hdr_aux", "kMisEnabled": "mis_enabled", "kAccumParams": "accum_params",
-    "kSdfParams": "sdf_params", "kWriteShadowVis": "write_shadow_vis", "kLightCount": "light_count",
+    "kSdfParams": "sdf_params", "kSdfParamsF": "sdf_params_f", "kWriteShadowVis": "write_shadow_vis", "kLightCount": "light_count",
     "kLightTreeEnabled": "light_tree_enabled", "kWriteSpecularAlbedo": "write_specular_albedo_gbuffer",
     "kWriteRoughness": "write_roughness_gbuffer", "kWriteSpecularHitDistance": "write_specular_hit_distance_gbuffer",
     "kRestirEnabled": "restir_enabled", "kSmokeParams": "smoke_params", "kMotionBlurParams": "motion_blur_params",

@@ -74,7 +74,7 @@ GATES = [
     ("vol_params", "4f", (0.01, 0.0, 1.0, 8.0), "participating media"),
     ("clouds_p1", "4f", (100.0, 200.0, 0.5, 0.2), "participating media"),
     ("clouds_p3", "4f", (0.0, 0.0, 0.0, 1.0), "participating media"),
-    ("sdf_params", "4I", (2, 128, 0, 0), "SDF clusters"),
+    ("sdf_params", "4I", (2, 128, 1, 0), "SDF iteration heat map"),
     ("restir_enabled", "I", 1, "ReSTIR"),
     ("mat_override_count", "4I", (1, 0, 0, 0), "material overrides"),
     ("smoke_params", "4f", (1.0, 0.0, 0.0, 0.0), "smoke volumes"),
@@ -115,6 +115,18 @@ def test_decode_reads_the_camera_sample_features():
         "mesh_motion_prev": ("4f", (0.5, -1.0, 2.0, 0.5))})
     d = rhi.decode_pathtrace_push(push)
     assert (d["spectral"], d["motion_blur"], d["mesh_shift_active"]) == (0, 0, 0)
+
+
+def test_decode_reads_the_sdf_fields():
+    """sdf_params 816 (count, r_sdf_max_iters) and sdf_params_f 832 (r_sdf_epsilon)
+    (Engine.cpp:7947-7981) reach the frame; the fractal knobs (.w, f.yzw) are
+    read by fractal clusters only, which the scene build refuses."""
+    f = _frame()
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides={
+        "sdf_params": ("4I", (3, 96, 0, 12)), "sdf_params_f": ("4f", (2.5e-4, 8.0, 1.0, 0.0))})
+    d = rhi.decode_pathtrace_push(push, strict=True)
+    assert d["ignored"] == ""
+    assert (d["sdf_count"], d["sdf_max_iters"]) == (3, 96) and np.isclose(d["sdf_epsilon"], 2.5e-4)
 
 
 def test_decode_reads_the_denoiser_fields():
