@@ -20,10 +20,12 @@ camera rays into the path queue (k_generate), the frame-completion kernel
 the sequential result).  value = (primary + extension + shadow rays) / wall
 time of the K frames, inputs resident in HBM.
 
-c2: 4,096 spheres (a third each Lambert, metal, glass) + a 500k-triangle mesh
-lit by 4,096 emissive triangles through the light tree, 1280x720, 4 spp,
-depth 6: the workload that exercises next-event estimation, the light tree
-and shadow transmittance.  Same step and value definitions.
+c2: 4,096 analytic objects -- 3,584 spheres and 512 sphere-traced SDF clusters
+(boxes, rounded boxes, tori, hard and smooth CSG unions, a subtraction), a
+third each Lambert, metal, glass -- + a 500k-triangle mesh lit by 4,096
+emissive triangles through the light tree, 1280x720, 4 spp, depth 6: the
+workload that exercises next-event estimation, the light tree, shadow
+transmittance and the SDF clusters.  Same step and value definitions.
 
 roofline (c1 and c2): the frame-completion kernel k_tail, which has no dense
 contraction (no MFMA), against the 8 TB/s HBM peak.  achieved = HBM-side bytes
@@ -34,7 +36,8 @@ frames rendered one at a time after the timed region.  frac = achieved /
 peak.  The profile records a hash of the kernel sources it was taken with;
 "profile_matches_sources" says whether the current sources hash the same.
 "algorithmic" gives the bytes the traversal reads at the node level: 128 B
-per BVH4 node visit + 48 B per triangle test + 20 B per sphere test, counted
+per BVH4 node visit + 48 B per triangle test + 20 B per sphere or SDF-cluster
+test (a cluster's trace steps read its record from the L1/L2), counted
 by an instrumented re-render of the same frame, and their rate; most of them
 are served by the L2 caches and the 256 MB MALL (l2_hit), which is why that
 rate can exceed the HBM peak.  l2_hit, valu_busy and valu_lane_utilisation
@@ -141,7 +144,8 @@ def roofline(workload, kernel, avg_ms, algo_bytes, launches, frames):
                           f"{achieved / HBM_PEAK_GBS:.0%} of HBM bandwidth" if busy >= 0.5 else
                           f"dependent fetch latency: {busy:.0%} VALU busy, {achieved / HBM_PEAK_GBS:.0%} of HBM bandwidth"))
     r["algorithmic"] = {"bytes_per_launch": int(algo_bytes),
-                        "per_unit": "128 B per BVH4 node visit + 48 B per triangle test + 20 B per sphere test",
+                        "per_unit": "128 B per BVH4 node visit + 48 B per triangle test + 20 B per sphere or "
+                                    "SDF-cluster test",
                         "GBps": round(algo_bytes / (avg_ms * 1e-3) / 1e9, 1),
                         "served_by": "L2 and MALL for the most part (l2_hit), so this rate is not HBM bandwidth"}
     return r
@@ -308,7 +312,7 @@ def bench_frames(np, torch, dmt, args, cfg, nsteps, nwarm, cpu):
         log(f"[bench] {name}: {launches / nf:.0f} launch(es)/frame, {per_launch / 1e6:.1f} MB node-level/launch, "
             f"avg launch {avg_ms:.3f} ms alone ({shared:.3f} ms with four frames in flight)")
     log(f"[bench] k_tail: {tr['tail_node_visits']} node visits, {tr['tail_tri_tests']} tri tests, "
-        f"{tr['tail_sphere_tests']} sphere tests per frame")
+        f"{tr['tail_sphere_tests']} sphere and SDF-cluster tests per frame")
     dom = max(kern, key=lambda k: kern[k]["ms_per_frame"])
     K = kern[dom]
     roof = roofline(f"c{cfg}", dom, K["avg_ms"], K["bytes_per_launch"], int(K["launches"]), nf)
@@ -354,8 +358,9 @@ def bench_frames(np, torch, dmt, args, cfg, nsteps, nwarm, cpu):
             "1920x1080, 1 spp, max depth 8, binned-SAH BVH4")
     else:
         metric, workload = "Mrays/s (many-light mixed scene)", (
-            "4,096 spheres (Lambert, metal, glass) + 500k-tri mesh lit by 4,096 emissive triangles through the light "
-            "tree, all-zero env, 1280x720, 4 spp, max depth 6, binned-SAH BVH4")
+            "3,584 spheres + 512 SDF clusters (boxes, rounded boxes, tori, CSG unions; Lambert, metal, glass) + "
+            "500k-tri mesh lit by 4,096 emissive triangles through the light tree, all-zero env, 1280x720, 4 spp, "
+            "max depth 6, binned-SAH BVH4")
     line = {
         "metric": metric,
         "value": round(mrays, 2),

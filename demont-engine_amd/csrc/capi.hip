@@ -391,9 +391,11 @@ bool sky_lights(const dmt_scene* s, const dmt_frame_params* f) {
     return up(f->sun) || up(f->moon);
 }
 
-// Sky lights, or SDF clusters in the scene: the frame runs the kernel
-// variant with their code (and the four shadow-record kinds).
-bool ext_path(const dmt_scene* s, const dmt_frame_params* f) { return sky_lights(s, f) || s->view.sdf_count > 0u; }
+// The kernel variant a frame runs: kExtSky with its sky lights (and the four
+// shadow-record kinds), kExtSdf with the scene's SDF clusters.
+uint32_t ext_bits(const dmt_scene* s, const dmt_frame_params* f) {
+    return (sky_lights(s, f) ? kExtSky : 0u) | (s->view.sdf_count > 0u ? kExtSdf : 0u);
+}
 
 // A frame that draws other numbers than the jitter per sample (lens point,
 // shutter time, hero wavelength), takes a fixed jitter, translates the mesh
@@ -409,7 +411,7 @@ FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
     X.fixed_jitter = (f->flags & DMT_FRAME_FIXED_JITTER) ? 1u : 0u;
     X.jit_x = f->jitter[0];
     X.jit_y = f->jitter[1];
-    X.ext = ext_path(s, f) ? 1u : 0u;
+    X.ext = ext_bits(s, f);
     if (sky_lights(s, f)) {
         SkyArgs& K = X.sky_args;
         K.sun = f3{f->sun[0], f->sun[1], f->sun[2]};
@@ -783,7 +785,7 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
     if (frame_count == 0) return fail(DMT_ERR_INVALID_ARG, "frame_count must be >= 1");
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
-    if (ext_path(s, f) && (st = ensure_sky_queues(s))) return st;
+    if (sky_lights(s, f) && (st = ensure_sky_queues(s))) return st;
     if (f->max_bounces == 0) return DMT_OK;
     for (uint32_t k = 0; k < frame_count; ++k) {
         FrameArgs F = frame_args(f);
@@ -1129,7 +1131,7 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     if ((st = check_frame_for(s, f))) return st;
     bool resized = false;
     if ((st = ensure_wavefront(s, f->width, f->height, &resized))) return st;
-    if (ext_path(s, f) && (st = ensure_sky_queues(s))) return st;
+    if (sky_lights(s, f) && (st = ensure_sky_queues(s))) return st;
     // Render into a scratch accumulation buffer so the scene's image and
     // counters are left as they were.
     float4* scratch = nullptr;
@@ -1356,12 +1358,13 @@ BatchArgs batch_args(const dmt_batch* b, const dmt_batch_params& p, const BatchS
 }
 
 BatchLaunch batch_launch(const dmt_batch* b, const dmt_batch_params& p) {
-    bool feat = false, sky = false;
+    bool feat = false;
+    uint32_t ext = 0u;
     for (size_t i = 0; i < b->frames.size(); ++i) {
         feat = feat || uses_features(b->scenes[i], &b->frames[i]);
-        sky = sky || ext_path(b->scenes[i], &b->frames[i]);
+        ext |= ext_bits(b->scenes[i], &b->frames[i]);
     }
-    return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds, feat, sky};
+    return BatchLaunch{b->cus * p.blocks_per_cu, p.stack_lds, feat, ext};
 }
 
 void drop_graph(dmt_batch* b) {
@@ -1490,10 +1493,9 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         b->order.insert(b->order.end(), queue[q].begin(), queue[q].end());
     }
     b->queue_begin[kXcds] = (uint32_t)b->order.size();
-    // records of the sky lights (sun or HDRI cluster, moon) in the kernel
-    // variant that has them (a frame with sky lights or a scene with SDF clusters)
+    // records of the sky lights (sun or HDRI cluster, moon) when a frame has them
     bool sky = false;
-    for (uint32_t i = 0; i < count; ++i) sky = sky || ext_path(list[i], &frames[i]);
+    for (uint32_t i = 0; i < count; ++i) sky = sky || sky_lights(list[i], &frames[i]);
     const std::pair<void**, size_t> req[] = {
         {(void**)&b->d_scenes, count * sizeof(BatchScene)},
         {(void**)&b->d_order, count * sizeof(uint32_t)},

@@ -545,6 +545,15 @@ __device__ __forceinline__ uint32_t tri_leaf(f3 o, f3 d, uint32_t first, uint32_
     return n;
 }
 
+// How far cluster ci is traced given the closest hit b so far: to b.t, or
+// just past it when b is a cluster of a higher index, which ci replaces at
+// an equal distance (b.t > 1e-3 there: the next float up is the next bit
+// pattern).
+__device__ __forceinline__ float sdf_t_max(const AnalyticBest& b, uint32_t ci) {
+    const bool tie_ok = b.code != kMiss && (b.code & kSdfBit) != 0u && (b.code & ~kSdfBit) > ci;
+    return tie_ok ? __uint_as_float(__float_as_uint(b.t) + 1u) : b.t;
+}
+
 // The SDF clusters of a leaf held in the fetch (slot k: the cluster index
 // bits in .x), each sphere-traced up to the closest distance so far
 // (traceSdfClusters, PathTrace.slang:2212-2306, clusters after the analytic
@@ -559,11 +568,9 @@ __device__ __forceinline__ uint32_t sdf_leaf(const SceneView& S, f3 o, f3 d, f3 
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t ci = __float_as_uint(r[k].x);
         if (COUNT) tc.spheres++;
-        const bool tie_ok = b.code != kMiss && (b.code & kSdfBit) != 0u && (b.code & ~kSdfBit) > ci;
-        // b.t > 1e-3 here: the next float up is the next bit pattern
-        const float t_max = tie_ok ? __uint_as_float(__float_as_uint(b.t) + 1u) : b.t;
         float t;
-        if (sdf_march(S.sdf + (size_t)ci * kSdfStride, o, d, inv, 1e-3f, t_max, S.sdf_iters, S.sdf_eps, t)) {
+        if (sdf_march(S.sdf + (size_t)ci * kSdfStride, o, d, inv, 1e-3f, sdf_t_max(b, ci), S.sdf_iters, S.sdf_eps,
+                      t)) {
             b.t = t;
             b.id = 0xFFFFFFFFu;
             b.code = kSdfBit | ci;
@@ -622,7 +629,16 @@ struct Walk {
     TriBest tb;
     uint32_t phase;   // 0 sphere BVH, 1 SDF cluster BVH, 2 triangle BVH
     uint32_t cur, sp;
+    // a cluster's sphere trace in progress, one step per walk step (kMiss:
+    // none): its index, the distance to test next, the end of its interval,
+    // the steps left
+    uint32_t sdf_ci;
+    float sdf_t, sdf_end;
+    uint32_t sdf_left;
 };
+
+// w.cur after an SDF leaf's last cluster: the walk continues from the stack.
+constexpr uint32_t kWalkPop = 0x7FFFFFFFu;
 
 // The hierarchy after `phase` that the scene has (3: none).  SDF: the
 // query sees the SDF clusters (the feature path).
@@ -643,16 +659,50 @@ __device__ __forceinline__ bool walk_begin(const SceneView& S, f3 o, f3 d, Walk&
     w.phase = S.sph_nodes ? 0u : walk_next_phase<SDF>(S, 0u);
     w.cur = 0u;
     w.sp = 0u;
+    w.sdf_ci = kMiss;
     return w.phase == 3u;
 }
 
-// One step of the query: a node or one leaf fetch (an SDF leaf: its
-// clusters' sphere traces).  Returns true when the query is done; its hit is
+// After a leaf: the next stack entry, else the next hierarchy; true when the
+// query is done.
+template <bool SDF>
+__device__ __forceinline__ bool walk_pop(const SceneView& S, Walk& w, Stack& st) {
+    if (w.sp > 0u) {
+        w.cur = st.get(--w.sp);
+        return false;
+    }
+    const uint32_t next = walk_next_phase<SDF>(S, w.phase);   // this hierarchy is done
+    if (next == 3u) return true;
+    if (next == 2u) w.tb = TriBest{w.ab.t, 0u, kMiss};
+    w.phase = next;
+    w.cur = 0u;
+    return false;
+}
+
+// One step of the query: a node, one leaf fetch, or one step of an SDF
+// cluster's sphere trace (a lane tracing a cluster then keeps pace with the
+// lanes that walk nodes).  Returns true when the query is done; its hit is
 // then final_hit(w.ab, w.tb).  MOTION and SDF as in trace_closest; a query
 // with SDF must have been started by walk_begin<true>.
 template <bool COUNT, bool MOTION = false, bool SDF = false>
 __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& w, Stack& st, TraceCounters& tc,
                                           float time = 1.0f) {
+    if (SDF && w.sdf_ci != kMiss) {
+        const float t = w.sdf_t;
+        bool done;
+        if (sdf_step(S.sdf + (size_t)w.sdf_ci * kSdfStride, o, d, S.sdf_eps, w.sdf_t)) {
+            w.ab.t = t;
+            w.ab.id = 0xFFFFFFFFu;
+            w.ab.code = kSdfBit | w.sdf_ci;
+            done = true;
+        } else {
+            done = w.sdf_t >= w.sdf_end || --w.sdf_left == 0u;
+        }
+        if (!done) return false;
+        w.sdf_ci = kMiss;
+        if (w.cur != kWalkPop) return false;   // the leaf's next cluster
+        return walk_pop<SDF>(S, w, st);
+    }
     const bool tri = w.phase == 2u;
     const bool sdf = SDF && w.phase == 1u;
     if (MOTION && tri && S.mesh_shift) o = o - mix3(S.mesh_prev, S.mesh_curr, time);
@@ -667,9 +717,26 @@ __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& 
     for (uint32_t k = 0; k < kFetch; ++k) asm volatile("" ::"v"(r[k].x), "v"(r[k].y), "v"(r[k].z), "v"(r[k].w));
     if (at_leaf) {
         const uint32_t count = ((w.cur >> 27) & 15u) + 1u;
+        if (sdf) {
+            // the leaf's first cluster: its interval, then its trace steps
+            const uint32_t ci = __float_as_uint(r[0].x);
+            if (COUNT) tc.spheres++;
+            w.cur = count > 1u ? (kLeaf | ((count - 2u) << 27) | (lf + 1u)) : kWalkPop;
+            float t, t_end;
+            sdf_interval(S.sdf + (size_t)ci * kSdfStride, o, w.inv, 1e-3f, sdf_t_max(w.ab, ci), t, t_end);
+            const uint32_t n = S.sdf_iters < 256u ? S.sdf_iters : 256u;
+            if (t < t_end && n > 0u) {
+                w.sdf_ci = ci;
+                w.sdf_t = t;
+                w.sdf_end = t_end;
+                w.sdf_left = n;
+                return false;
+            }
+            if (w.cur != kWalkPop) return false;
+            return walk_pop<SDF>(S, w, st);
+        }
         uint32_t used;
         if (tri) used = tri_leaf<COUNT>(o, d, lf, count, r, w.tb, tc);
-        else if (sdf) used = sdf_leaf<COUNT>(S, o, d, w.inv, count, r, w.ab, tc);
         else used = sphere_leaf<COUNT, MOTION>(S, o, d, lf, count, r, w.ab, tc, time);
         if (used < count) {
             w.cur = kLeaf | ((count - used - 1u) << 27) | (lf + used);
@@ -694,16 +761,7 @@ __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& 
             return false;
         }
     }
-    if (w.sp > 0u) {
-        w.cur = st.get(--w.sp);
-        return false;
-    }
-    const uint32_t next = walk_next_phase<SDF>(S, w.phase);   // this hierarchy is done
-    if (next == 3u) return true;
-    if (next == 2u) w.tb = TriBest{w.ab.t, 0u, kMiss};
-    w.phase = next;
-    w.cur = 0u;
-    return false;
+    return walk_pop<SDF>(S, w, st);
 }
 
 // Closest hits of the rays 0 .. n-1 of a queue, every lane on its own ray,

@@ -61,8 +61,8 @@ struct BatchLaunch {
     uint32_t blocks;                  // grid size in workgroups
     uint32_t stack_lds;               // LDS stack entries per thread: 16, 24 or 32
     bool features;                    // some scene's frame uses camera-sample features
-    bool ext;                         // some scene's frame has sky lights (4 record kinds in rec)
-                                      // or some scene SDF clusters
+    uint32_t ext;                     // kExtSky: some scene's frame has sky lights (4 record kinds
+                                      // in rec); kExtSdf: some scene has SDF clusters
 };
 
 // Sets the frame counter (stream-ordered, not part of a captured frame).

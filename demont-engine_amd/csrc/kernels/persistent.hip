@@ -200,8 +200,9 @@ template <uint32_t DEPTH> constexpr uint32_t persistent_waves() { return (160u *
 // FEAT: some scene's frame uses camera-sample features (thin lens, shutter
 // time, hero wavelength, a translated mesh); each scene's FeatureArgs say
 // which, and the lane carries the sample's shutter time and wavelength.
-// EXT: some scene's frame has sky lights or some scene SDF clusters.
-template <uint32_t DEPTH, bool FEAT, bool EXT = false>
+// EXT (implies FEAT): kExtSky, some scene's frame has sky lights (four record
+// kinds); kExtSdf, some scene has SDF clusters.
+template <uint32_t DEPTH, bool FEAT, uint32_t EXT = 0u>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<DEPTH>()))) k_persistent(BatchArgs A) {
     __shared__ uint32_t lds[DEPTH * kBlock];
@@ -262,7 +263,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                         l.P.seed = pixel_seed(F, x, y, frame_index);
                         start_sample<FEAT>(F, X, l, x, y, 0u);
                         state = kLaneWalk;
-                        if (walk_begin<EXT>(S, l.o, l.d, w)) {
+                        if (walk_begin<(EXT & kExtSdf) != 0u>(S, l.o, l.d, w)) {
                             h = final_hit(w.ab, w.tb);
                             state = kLaneReady;
                         }
@@ -328,18 +329,19 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                 else n_ext++;
                 ScratchSink sink{{}, A.rec, A.threads, tid, 0u};
                 VertexResult R;
-                shade_vertex<EXT, EXT>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
+                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u>(S, F, l.b, l.o, l.d, h, l.P, R, sink,
+                                                                              &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.pending | (cont ? 0u : kEnded);
                 c.set3(Cold::kNo, R.no);
                 c.set3(Cold::kNd, R.nd);
-                advance<FEAT, EXT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                advance<FEAT, (EXT & kExtSky) != 0u>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(Cold::kTmax);
                 f3 tr = c.get3(Cold::kTr);
                 const uint32_t step = c.getu(Cold::kStep);
-                if (transmit_step<EXT>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
+                if (transmit_step<(EXT & kExtSdf) != 0u>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         const float4* row = A.rec + ((size_t)k * kRecRows) * A.threads + tid;
@@ -349,7 +351,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                                                      (l.pending & (16u << k)) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    advance<FEAT, EXT>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                    advance<FEAT, (EXT & kExtSky) != 0u>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
                 } else {
                     c.set(Cold::kTmax, t_max);
                     c.set3(Cold::kTr, tr);
@@ -358,7 +360,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
             }
             if (l.active) {
                 state = kLaneWalk;
-                if (walk_begin<EXT>(S, l.o, l.d, w)) {
+                if (walk_begin<(EXT & kExtSdf) != 0u>(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
                     state = kLaneReady;
                 }
@@ -369,7 +371,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
         // ---- traversal steps of every lane with a query in progress ----
 #pragma unroll
         for (uint32_t k = 0; k < kTailSteps; ++k) {
-            if (state == kLaneWalk && walk_step<false, FEAT, EXT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
+            if (state == kLaneWalk && walk_step<false, FEAT, (EXT & kExtSdf) != 0u>(S, l.o, l.d, w, stk, tc, l.P.time)) {
                 h = final_hit(w.ab, w.tb);
                 state = kLaneReady;
             }
@@ -399,22 +401,34 @@ hipError_t batch_set_frame(uint32_t* frame, uint32_t value, hipStream_t stream) 
     return hipGetLastError();
 }
 
+template <uint32_t DEPTH>
+hipError_t launch_persistent(uint32_t variant, uint32_t blocks, const BatchArgs& A, hipStream_t stream) {
+    switch (variant) {
+        case 0u: hipLaunchKernelGGL((k_persistent<DEPTH, false>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
+        case 1u: hipLaunchKernelGGL((k_persistent<DEPTH, true>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
+        case 2u: hipLaunchKernelGGL((k_persistent<DEPTH, true, kExtSky>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
+        case 3u: hipLaunchKernelGGL((k_persistent<DEPTH, true, kExtSdf>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
+        case 4u:
+            hipLaunchKernelGGL((k_persistent<DEPTH, true, kExtSky | kExtSdf>), dim3(blocks), dim3(kBlock), 0, stream, A);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_batch_frame(const BatchArgs& A, const BatchLaunch& L, hipStream_t stream) {
     if (L.blocks == 0u || (size_t)L.blocks * kBlock != A.threads) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, stream, A.work, A.nscenes);
-    const uint32_t v = L.ext ? 2u : (L.features ? 1u : 0u);   // sky lights and SDF clusters imply features
-    switch (L.stack_lds * 4u + v) {
-        case 64u: hipLaunchKernelGGL((k_persistent<16u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 65u: hipLaunchKernelGGL((k_persistent<16u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 66u: hipLaunchKernelGGL((k_persistent<16u, true, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 96u: hipLaunchKernelGGL((k_persistent<24u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 97u: hipLaunchKernelGGL((k_persistent<24u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 98u: hipLaunchKernelGGL((k_persistent<24u, true, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 128u: hipLaunchKernelGGL((k_persistent<32u, false>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 129u: hipLaunchKernelGGL((k_persistent<32u, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
-        case 130u: hipLaunchKernelGGL((k_persistent<32u, true, true>), dim3(L.blocks), dim3(kBlock), 0, stream, A); break;
+    // variant: 0 plain, 1 features, 2 + ext (sky lights and SDF clusters imply features)
+    const uint32_t v = L.features || L.ext ? 1u + (L.ext & 3u) : 0u;
+    hipError_t e;
+    switch (L.stack_lds) {
+        case 16u: e = launch_persistent<16u>(v, L.blocks, A, stream); break;
+        case 24u: e = launch_persistent<24u>(v, L.blocks, A, stream); break;
+        case 32u: e = launch_persistent<32u>(v, L.blocks, A, stream); break;
         default: return hipErrorInvalidValue;
     }
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, stream, A.frame);
     return hipGetLastError();
 }

@@ -179,32 +179,47 @@ __device__ __forceinline__ f3 sdf_grad(const float4* c, uint32_t count, f3 p) {
     return count > 0u ? sdf_pick(gv, count - 1u) : up;
 }
 
-// sdfClusterTrace (SdfPrimitives.slang:1330-1444) without the normal, which
-// the shading step derives at o + d t (the point the trace stopped at):
-// the slab interval of the cluster box clipped to [t_min, t_max], then at
-// most min(iters, 256) steps of |d|, at least eps each, until |d| < eps.
-// inv is the ray's safe inverse direction (1 / d, |d| floored at 1e-30).
-__device__ __forceinline__ bool sdf_march(const float4* c, f3 o, f3 d, f3 inv, float t_min, float t_max,
-                                          uint32_t iters, float eps, float& t_hit) {
+// The interval sdfClusterTrace (SdfPrimitives.slang:1330-1444) marches: the
+// slab interval of the cluster box (sdfClusterAabbT, :1306-1316) clipped to
+// [t_min, t_max]; empty when t >= t_end.  inv is the ray's safe inverse
+// direction (1 / d, |d| floored at 1e-30).
+__device__ __forceinline__ void sdf_interval(const float4* c, f3 o, f3 inv, float t_min, float t_max, float& t,
+                                             float& t_end) {
     const float4 h0 = c[0], h1 = c[1];
     const float t0x = (h0.x - o.x) * inv.x, t1x = (h1.x - o.x) * inv.x;
     const float t0y = (h0.y - o.y) * inv.y, t1y = (h1.y - o.y) * inv.y;
     const float t0z = (h0.z - o.z) * inv.z, t1z = (h1.z - o.z) * inv.z;
     const float t_enter = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
     const float t_exit = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    float t = fmaxf(t_enter, t_min);
-    const float t_end = fminf(t_exit, t_max);
+    t = fmaxf(t_enter, t_min);
+    t_end = fminf(t_exit, t_max);
+}
+
+// One step of the trace at distance t: true for a hit there (|d| < eps),
+// else t advances by max(|d|, eps).
+__device__ __forceinline__ bool sdf_step(const float4* c, f3 o, f3 d, float eps, float& t) {
+    const f3 p = o + d * t;
+    const float ad = fabsf(sdf_dist(c, __float_as_uint(c[0].w), p));
+    if (ad < eps) return true;
+    t += fmaxf(ad, eps);
+    return false;
+}
+
+// sdfClusterTrace without the normal, which the shading step derives at
+// o + d t (the point the trace stopped at): at most min(iters, 256) steps
+// over the interval.
+__device__ __forceinline__ bool sdf_march(const float4* c, f3 o, f3 d, f3 inv, float t_min, float t_max,
+                                          uint32_t iters, float eps, float& t_hit) {
+    float t, t_end;
+    sdf_interval(c, o, inv, t_min, t_max, t, t_end);
     if (t >= t_end) return false;
-    const uint32_t count = __float_as_uint(h0.w);
     const uint32_t n = iters < 256u ? iters : 256u;
     for (uint32_t i = 0; i < n; ++i) {
-        const f3 p = o + d * t;
-        const float ad = fabsf(sdf_dist(c, count, p));
-        if (ad < eps) {
-            t_hit = t;
+        const float ti = t;
+        if (sdf_step(c, o, d, eps, t)) {
+            t_hit = ti;
             return true;
         }
-        t += fmaxf(ad, eps);
         if (t >= t_end) return false;
     }
     return false;

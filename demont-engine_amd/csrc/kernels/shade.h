@@ -43,6 +43,10 @@ struct SkyArgs {
     float4 hdri_col[kMaxHdriLights];       // rgb flux
 };
 
+// Extensions a frame-completion kernel variant carries: the frame's sky
+// lights (four shadow-record kinds) and the scene's SDF clusters.
+constexpr uint32_t kExtSky = 1u, kExtSdf = 2u;
+
 // Camera-sample features and sky lights of a frame (all off: the plain
 // camera ray and no sky lights).
 struct FeatureArgs {
@@ -52,8 +56,8 @@ struct FeatureArgs {
     uint32_t spectral;                     // draw a hero wavelength per sample
     uint32_t fixed_jitter;                 // sample 0 takes jit_x, jit_y (halton_jitter) instead of two draws
     float jit_x, jit_y;
-    uint32_t ext;                          // sky lights (SkyArgs below) or SDF clusters: the
-                                           // kernel variant that has their code
+    uint32_t ext;                          // kExtSky: sky lights (SkyArgs below); kExtSdf: the
+                                           // scene's SDF clusters -- the kernel variant with their code
     uint32_t pad;
     SkyArgs sky_args;
 };

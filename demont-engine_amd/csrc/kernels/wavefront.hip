@@ -221,10 +221,10 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
 // wavelength, a fixed jitter, a translated mesh): there is no queue; entry i
 // is slot i of the frame's 8x8 tile order, whose pixel's first sample the
 // lane starts itself (camera_sample), and the shutter time and wavelength
-// ride in the lane's path state.  EXT (implies FEAT): the frame has sky
-// lights or the scene SDF clusters; the sky-light records and the clusters'
-// traversal exist only in this variant.
-template <bool COUNT, bool FEAT, bool EXT = false>
+// ride in the lane's path state.  EXT (implies FEAT), kExtSky: the frame has
+// sky lights (their records and NEE exist only in such a variant); kExtSdf:
+// the scene has SDF clusters (likewise their traversal).
+template <bool COUNT, bool FEAT, uint32_t EXT = 0u>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerSimd, kTailWavesPerSimd)))
 k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uint32_t* ctr, PathState ps,
        ShadowQueue sl, ShadowQueue se, ShadowQueue s2, ShadowQueue s3, uint32_t* spill, uint32_t spill_depth,
@@ -275,7 +275,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                     l.mode = 0u;
                     l.active = true;
                     state = tail::kLaneWalk;
-                    if (walk_begin<EXT>(S, l.o, l.d, w)) {
+                    if (walk_begin<(EXT & kExtSdf) != 0u>(S, l.o, l.d, w)) {
                         h = final_hit(w.ab, w.tb);
                         state = tail::kLaneReady;
                     }
@@ -297,7 +297,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 l.mode = 0u;
                 l.active = true;
                 state = tail::kLaneWalk;
-                if (walk_begin<EXT>(S, l.o, l.d, w)) {
+                if (walk_begin<(EXT & kExtSdf) != 0u>(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
                     state = tail::kLaneReady;
                 }
@@ -319,20 +319,21 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 else n_ext++;
                 SlotSink sink{{}, sl, se, s2, s3, l.slot, l.pix, 0u};
                 VertexResult R;
-                shade_vertex<EXT, EXT>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
+                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u>(S, F, l.b, l.o, l.d, h, l.P, R, sink,
+                                                                              &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.emitted | (cont ? 0u : tail::kEnded);
                 if (cont) {
                     c.set3(tail::TailCold::kNo, R.no);
                     c.set3(tail::TailCold::kNd, R.nd);
                 }
-                tail::tail_advance<FEAT, EXT>(F, X, sl, se, s2, s3, ps, l, c);
+                tail::tail_advance<FEAT, (EXT & kExtSky) != 0u>(F, X, sl, se, s2, s3, ps, l, c);
             } else {
                 n_shadow++;
                 float t_max = c.get(tail::TailCold::kTmax);
                 f3 tr = c.get3(tail::TailCold::kTr);
                 const uint32_t step = c.getu(tail::TailCold::kStep);
-                if (transmit_step<EXT>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
+                if (transmit_step<(EXT & kExtSdf) != 0u>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         // fields picked one by one: a selected reference to one of
@@ -340,7 +341,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                         const float4* qa = k == 0u ? sl.a : se.a;
                         const float4* qb = k == 0u ? sl.b : se.b;
                         const float4* qd = k == 0u ? sl.d : se.d;
-                        if (EXT && k >= 2u) {
+                        if ((EXT & kExtSky) && k >= 2u) {
                             qa = k == 2u ? s2.a : s3.a;
                             qb = k == 2u ? s2.b : s3.b;
                             qd = k == 2u ? s2.d : s3.d;
@@ -351,7 +352,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                                                      k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    tail::tail_advance<FEAT, EXT>(F, X, sl, se, s2, s3, ps, l, c);
+                    tail::tail_advance<FEAT, (EXT & kExtSky) != 0u>(F, X, sl, se, s2, s3, ps, l, c);
                 } else {   // transmit_step ends every query by its step refract_bounces
                     c.set(tail::TailCold::kTmax, t_max);
                     c.set3(tail::TailCold::kTr, tr);
@@ -360,7 +361,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
             }
             if (l.active) {
                 state = tail::kLaneWalk;
-                if (walk_begin<EXT>(S, l.o, l.d, w)) {
+                if (walk_begin<(EXT & kExtSdf) != 0u>(S, l.o, l.d, w)) {
                     h = final_hit(w.ab, w.tb);
                     state = tail::kLaneReady;
                 }
@@ -371,7 +372,8 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
         // traversal steps of every lane with a query in progress
 #pragma unroll
         for (uint32_t k = 0; k < kTailSteps; ++k) {
-            if (state == tail::kLaneWalk && walk_step<COUNT, FEAT, EXT>(S, l.o, l.d, w, stk, tc, l.P.time)) {
+            if (state == tail::kLaneWalk &&
+                walk_step<COUNT, FEAT, (EXT & kExtSdf) != 0u>(S, l.o, l.d, w, stk, tc, l.P.time)) {
                 h = final_hit(w.ab, w.tb);
                 state = tail::kLaneReady;
             }
@@ -639,18 +641,21 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
             (void)hipEventRecord(o.extend_events[0], stream);
         }
         const FeatureArgs& X = *o.features;
-        if (o.counting && X.ext)
-            hipLaunchKernelGGL((k_tail<true, true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
-                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
-        else if (o.counting)
-            hipLaunchKernelGGL((k_tail<true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
-                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
-        else if (X.ext)
-            hipLaunchKernelGGL((k_tail<false, true, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
-                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
-        else
-            hipLaunchKernelGGL((k_tail<false, true>), dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0],
-                               W.counters, W.ps, W.sq[0], W.sq[1], W.sq[2], W.sq[3], W.spill, W.spill_depth, W.stats);
+        const ShadowQueue* sq = W.sq;
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0], W.counters, W.ps,
+                               sq[0], sq[1], sq[2], sq[3], W.spill, W.spill_depth, W.stats);
+        };
+        switch ((o.counting ? 4u : 0u) + (X.ext & 3u)) {
+            case 0u: go(k_tail<false, true, 0u>); break;
+            case 1u: go(k_tail<false, true, kExtSky>); break;
+            case 2u: go(k_tail<false, true, kExtSdf>); break;
+            case 3u: go(k_tail<false, true, kExtSky | kExtSdf>); break;
+            case 4u: go(k_tail<true, true, 0u>); break;
+            case 5u: go(k_tail<true, true, kExtSky>); break;
+            case 6u: go(k_tail<true, true, kExtSdf>); break;
+            default: go(k_tail<true, true, kExtSky | kExtSdf>); break;
+        }
         if (ev) {
             (void)hipEventRecord(o.extend_events[1], stream);
             o.events_used = 2u;
