@@ -810,10 +810,10 @@ dmt_status render_impl(dmt_scene* s, const dmt_frame_params* f, uint32_t frame_c
             o.max_events = (uint32_t)bound;
         }
         o.tail_after = s->tail_after;
-        o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
-        o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
         const FeatureArgs X = feature_args(s, f);
         if (uses_features(s, f)) o.features = &X;
+        o.tail_blocks = s->dev->cus * (o.features ? tail_waves(X.ext) : kTailBlocksPerCu);
+        o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
         hipError_t e = launch_frame(frame_view(s, f), F, s->wf[set], s->accum, stream, o);
         s->ev_used += o.events_used;
         if (e == hipSuccess) e = hipEventRecord(s->acc_done[set], stream);
@@ -1147,10 +1147,10 @@ dmt_status dmt_scene_measure_traffic(dmt_scene* s, const dmt_frame_params* f, dm
     LaunchOptions o;
     o.counting = true;
     o.tail_after = s->tail_after;
-    o.tail_blocks = s->dev->cus * kTailBlocksPerCu;
-    o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
     const FeatureArgs X = feature_args(s, f);
     if (uses_features(s, f)) o.features = &X;
+    o.tail_blocks = s->dev->cus * (o.features ? tail_waves(X.ext) : kTailBlocksPerCu);
+    o.extend_blocks = s->dev->cus * kExtendBlocksPerCu;
     if (e == hipSuccess) e = launch_frame(frame_view(s, f), F, W, scratch, s->dev->stream, o);
     DeviceStats h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, s->count_stats, sizeof(h), hipMemcpyDeviceToHost, s->dev->stream);

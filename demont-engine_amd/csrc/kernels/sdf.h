@@ -146,37 +146,50 @@ __device__ __forceinline__ f3 sdf_grad(const float4* c, uint32_t count, f3 p) {
     const f3 up{0.0f, 1.0f, 0.0f};
 #pragma unroll
     for (uint32_t i = 0; i < kSdfMaxNodes; ++i) {
-        dv[i] = 1e30f;
-        gv[i] = up;
-        if (i >= count) continue;
-        const float4 n0 = c[4u + 3u * i], n1 = c[5u + 3u * i];
-        const uint32_t op = __float_as_uint(n0.x);
-        if (op == kSdfLeaf) {
-            const f3 pl = p - xyz(c[6u + 3u * i]);
-            const uint32_t shape = __float_as_uint(n0.y);
-            dv[i] = sdf_shape_dist(pl, shape, n1);
-            gv[i] = sdf_shape_normal(pl, shape, n1);
-            continue;
-        }
-        const uint32_t ia = __float_as_uint(n0.z), ib = __float_as_uint(n0.w);
-        const float da = sdf_pick(dv, ia), db = sdf_pick(dv, ib), k = n1.x;
-        const f3 ga = sdf_pick(gv, ia), gb = sdf_pick(gv, ib);
-        if (op == kSdfUnion || op == kSdfSubtract || op == kSdfIntersect) {
-            dv[i] = sdf_combine(op, da, db, k);
-            if (k <= 0.0f) {
-                if (op == kSdfUnion) gv[i] = (da < db) ? ga : gb;
-                else if (op == kSdfSubtract) gv[i] = (da > -db) ? ga : -gb;
-                else gv[i] = (da > db) ? ga : gb;
-            } else {
-                const float h = sdf_h(op, da, db, k);
-                gv[i] = op == kSdfSubtract ? normalize(lerp3(ga, -gb, h)) : normalize(lerp3(gb, ga, h));
+        float d = 1e30f;
+        f3 g = up;
+        if (i < count) {
+            const float4 n0 = c[4u + 3u * i], n1 = c[5u + 3u * i];
+            const uint32_t op = __float_as_uint(n0.x);
+            if (op == kSdfLeaf) {
+                const f3 pl = p - xyz(c[6u + 3u * i]);
+                const uint32_t shape = __float_as_uint(n0.y);
+                d = sdf_shape_dist(pl, shape, n1);
+                g = sdf_shape_normal(pl, shape, n1);
+            } else if (i > 0u) {
+                const uint32_t ia = __float_as_uint(n0.z), ib = __float_as_uint(n0.w);
+                float da = dv[0], db = dv[0];
+                f3 ga = gv[0], gb = gv[0];
+#pragma unroll
+                for (uint32_t j = 1; j < i; ++j) {   // the children come before node i
+                    if (ia == j) { da = dv[j]; ga = gv[j]; }
+                    if (ib == j) { db = dv[j]; gb = gv[j]; }
+                }
+                const float k = n1.x;
+                if (op == kSdfUnion || op == kSdfSubtract || op == kSdfIntersect) {
+                    d = sdf_combine(op, da, db, k);
+                    if (k <= 0.0f) {
+                        if (op == kSdfUnion) g = (da < db) ? ga : gb;
+                        else if (op == kSdfSubtract) g = (da > -db) ? ga : -gb;
+                        else g = (da > db) ? ga : gb;
+                    } else {
+                        const float h = sdf_h(op, da, db, k);
+                        g = op == kSdfSubtract ? normalize(lerp3(ga, -gb, h)) : normalize(lerp3(gb, ga, h));
+                    }
+                } else if (op == kSdfDisplace) {
+                    d = da + k;
+                    g = ga;
+                }
             }
-        } else if (op == kSdfDisplace) {
-            dv[i] = da + k;
-            gv[i] = ga;
         }
+        dv[i] = d;
+        gv[i] = g;
     }
-    return count > 0u ? sdf_pick(gv, count - 1u) : up;
+    f3 r = up;
+#pragma unroll
+    for (uint32_t i = 0; i < kSdfMaxNodes; ++i)
+        if (count == i + 1u) r = gv[i];
+    return r;
 }
 
 // The interval sdfClusterTrace (SdfPrimitives.slang:1330-1444) marches: the

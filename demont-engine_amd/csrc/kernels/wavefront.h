@@ -108,6 +108,13 @@ constexpr uint32_t kNoTail = 0xFFFFFFFFu;   // every iteration by the per-bounce
 #define DMT_TAIL_WAVES 5
 #endif
 constexpr uint32_t kTailWavesPerSimd = DMT_TAIL_WAVES;   // k_tail occupancy target (and workgroups per CU)
+// The extension variants (sky lights, SDF clusters) carry more live state:
+// four waves per SIMD give them 128 VGPRs instead of 96, and fewer spills.
+#ifndef DMT_TAIL_WAVES_EXT
+#define DMT_TAIL_WAVES_EXT 4
+#endif
+constexpr uint32_t kTailWavesExt = DMT_TAIL_WAVES_EXT;
+constexpr uint32_t tail_waves(uint32_t ext) { return ext ? kTailWavesExt : kTailWavesPerSimd; }
 // Traversal steps per pass of the step-granular loops (k_tail, k_persistent)
 // between the wave-level bookkeeping (ballots, refill, batched shading): three
 // measured 1.5 % (1.51 M-triangle 1080p) to 4.6 % (many-light scene) faster

@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView S, FrameArgs F, uin
 // sky lights (their records and NEE exist only in such a variant); kExtSdf:
 // the scene has SDF clusters (likewise their traversal).
 template <bool COUNT, bool FEAT, uint32_t EXT = 0u>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerSimd, kTailWavesPerSimd)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(tail_waves(EXT), tail_waves(EXT))))
 k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uint32_t* ctr, PathState ps,
        ShadowQueue sl, ShadowQueue se, ShadowQueue s2, ShadowQueue s3, uint32_t* spill, uint32_t spill_depth,
        DeviceStats* st) {
