@@ -151,6 +151,9 @@ struct dmt_scene {
     std::vector<uint32_t> refit_level_start;   // offsets into refit_nodes, one past the end last
     float* positions_dev = nullptr;
     uint32_t* adj_spill = nullptr;    // traversal spill area of the adjoint kernel
+    float4* adj_rec = nullptr;        // the adjoint's metal-vertex records and their count
+    size_t adj_rec_bytes = 0;
+    uint32_t* adj_rec_count = nullptr;
     float4* moon = nullptr;           // moon texture (dmt_scene_set_moon_map)
 };
 
@@ -751,6 +754,8 @@ void scene_free(dmt_scene* s) {
             if (e) (void)hipEventDestroy(e);
     if (s->host_count) (void)hipHostFree(s->host_count);
     if (s->adj_spill) (void)hipFree(s->adj_spill);
+    if (s->adj_rec) (void)hipFree(s->adj_rec);
+    if (s->adj_rec_count) (void)hipFree(s->adj_rec_count);
     if (s->moon) (void)hipFree(s->moon);
     for (void* p : s->buffers) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -1255,6 +1260,36 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
         if (s->prims_host[(size_t)i * 28u + 9u] == 1.0f && s->prims_host[(size_t)i * 28u + 8u] == 0.0f) A.metal_prims++;
     A.spill = s->adj_spill;
     A.spill_depth = spill_depth;
+    if (A.metal_prims) {
+        // at most one record per vertex of every sample; pixel chunks keep the
+        // list within kAdjRecMaxBytes
+        const uint64_t per_px = (uint64_t)f->spp * f->max_bounces;
+        const uint64_t npix = (uint64_t)f->width * f->height;
+        const uint64_t fit = kAdjRecMaxBytes / (kAdjRecFloat4 * sizeof(float4)) / per_px;
+        A.chunk_px = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(npix, fit));
+        A.rec_cap = (uint32_t)(A.chunk_px * per_px);
+        const size_t bytes = (size_t)A.rec_cap * kAdjRecFloat4 * sizeof(float4);
+        if (bytes > s->adj_rec_bytes) {
+            if (s->adj_rec) (void)hipFree(s->adj_rec);
+            s->adj_rec = nullptr;
+            s->adj_rec_bytes = 0;
+            hipError_t e = hipMalloc((void**)&s->adj_rec, bytes);
+            if (e != hipSuccess) {
+                s->adj_rec = nullptr;
+                return hip_fail(e, "hipMalloc(adjoint records)");
+            }
+            s->adj_rec_bytes = bytes;
+        }
+        if (!s->adj_rec_count) {
+            hipError_t e = hipMalloc((void**)&s->adj_rec_count, sizeof(uint32_t));
+            if (e != hipSuccess) {
+                s->adj_rec_count = nullptr;
+                return hip_fail(e, "hipMalloc(adjoint record count)");
+            }
+        }
+        A.rec = s->adj_rec;
+        A.rec_count = s->adj_rec_count;
+    }
     hipError_t e = launch_adjoint(A, s->dev->stream);
     if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "adjoint");
     return DMT_OK;

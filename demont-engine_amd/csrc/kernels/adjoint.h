@@ -17,9 +17,12 @@
 //   gradient (flushed when the path ends).
 // * emission: a term T * Le at the hit surface, with the clampIndirect
 //   Jacobian.
-// * roughness: forward mode.  For each metal vertex the sample is replayed
-//   again, in dual numbers from that vertex on, carrying d/d(roughness)
-//   through the reflected ray and everything downstream of it.
+// * roughness: forward mode.  The replay above records, for each metal
+//   vertex, the ray that reached it, its hit and the path state it arrived
+//   with; a second kernel then continues each recorded vertex in dual
+//   numbers from that state on, carrying d/d(roughness) through the reflected
+//   ray and everything downstream of it.  The frame's pixels are processed
+//   in chunks whose records fit the record list.
 //
 // Sampling decisions are held fixed (the derivative of the estimator with
 // its random numbers fixed, away from decision boundaries); with Russian
@@ -37,6 +40,11 @@ constexpr uint32_t kAdjMaxVerts = 8;       // max_bounces limit of the adjoint
 constexpr uint32_t kAdjMaxDielectric = 8;  // refract_bounces limit of the adjoint
 constexpr uint32_t kAdjLdsPrims = 1024;    // per-workgroup LDS accumulation up to this many prims
 constexpr uint32_t kMaterialFloats = 7;    // albedo rgb, roughness, emission rgb
+// A recorded metal vertex: (o, h.t), (d, h.code), (T, prev_pdf),
+// (seed, prim, bounce | skip_sky << 8 | prev_lambert << 9, lambda),
+// (dLoss/dI of the sample, time).
+constexpr uint32_t kAdjRecFloat4 = 5;
+constexpr size_t kAdjRecMaxBytes = size_t(1) << 31;   // record list limit (pixel chunks beyond)
 // Register budget of the adjoint kernels: three waves per SIMD (168 VGPRs,
 // a few values spilled outside the traversal loop) instead of two.
 constexpr uint32_t kAdjWavesPerSimd = 3;
@@ -52,6 +60,10 @@ struct AdjointArgs {
     uint32_t metal_prims;   // analytic metal prims (0: no roughness pass)
     uint32_t* spill;
     uint32_t spill_depth;
+    float4* rec;            // metal-vertex records (rec_cap x kAdjRecFloat4 float4)
+    uint32_t* rec_count;    // records of the current chunk
+    uint32_t rec_cap;       // records one chunk may produce at most
+    uint32_t chunk_px;      // pixels per chunk (rec_cap / (spp * max_bounces))
 };
 
 hipError_t launch_adjoint(const AdjointArgs& A, hipStream_t stream);

@@ -21,8 +21,9 @@ __device__ __forceinline__ void add_grad(float* g, uint32_t i, float v) {
 
 // Path-replay sink: forms the radiance terms exactly as the renderer does
 // (shadow queries traced at once, in emission order) and scatters each
-// term's derivative.
-template <bool FEAT>
+// term's derivative.  REC: records the sample's metal vertices on analytic
+// primitives (at most kAdjMaxVerts) for the roughness pass.
+template <bool FEAT, bool REC = false>
 struct AdjointSink : NoGradientHooks {
     const SceneView& S;
     const FrameArgs& F;
@@ -33,6 +34,32 @@ struct AdjointSink : NoGradientHooks {
     uint32_t n;                        // albedo vertices so far
     uint32_t prim[kAdjMaxVerts];
     f3 acc[kAdjMaxVerts];
+    float4* rec;                       // REC: the record list and its length
+    uint32_t* rec_count;
+    uint32_t metal;                    // REC: metal vertices recorded for this sample
+
+    // A metal vertex of an analytic primitive: its record, appended with one
+    // counter atomic per wave (the lanes at such a vertex take consecutive
+    // records).  Samples whose dLoss/dI is zero have no roughness gradient.
+    __device__ __forceinline__ void on_vertex(uint32_t b, const Surface& sf, f3 o, f3 d, HitRec h,
+                                              const PathVertexState& P) {
+        if (!REC || sf.mat != 1u || sf.kind != 1u || metal >= kAdjMaxVerts) return;
+        if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) return;
+        metal++;
+        const unsigned long long m = __ballot(1);
+        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+        uint32_t base = 0u;
+        if (__lane_id() == leader) base = atomicAdd(rec_count, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+        const uint32_t i = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        float4* r = rec + (size_t)i * kAdjRecFloat4;
+        const uint32_t bits = b | (P.skip_sky ? 256u : 0u) | (P.prev_lambert ? 512u : 0u);
+        r[0] = make_float4(o.x, o.y, o.z, h.t);
+        r[1] = make_float4(d.x, d.y, d.z, __uint_as_float(h.code));
+        r[2] = make_float4(P.T_.x, P.T_.y, P.T_.z, P.prev_pdf);
+        r[3] = make_float4(__uint_as_float(P.seed), __uint_as_float(sf.prim), __uint_as_float(bits), P.lambda);
+        r[4] = make_float4(delta.x, delta.y, delta.z, P.time);
+    }
 
     // delta . dc / dlog(albedo_ch), per channel ch.
     __device__ __forceinline__ f3 log_derivative(f3 v, f3 c, bool may_clamp) const {
@@ -146,9 +173,9 @@ struct AdjointSink : NoGradientHooks {
     }
 };
 
-template <bool LDS, bool FEAT>
+template <bool LDS, bool FEAT, bool REC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
-k_adjoint(AdjointArgs A) {
+k_adjoint(AdjointArgs A, uint32_t p0, uint32_t p1) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
     __shared__ float lds_grad[LDS ? kAdjLdsPrims * kMaterialFloats : 1u];
     const uint32_t nparam = A.prim_count * kMaterialFloats;
@@ -162,8 +189,7 @@ k_adjoint(AdjointArgs A) {
     Stack stk = stack_at(lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     const FrameArgs& F = A.F;
-    const uint32_t npix = F.width * F.height;
-    for (uint32_t p = gid; p < npix; p += stride) {
+    for (uint32_t p = p0 + gid; p < p1; p += stride) {
         const uint32_t x = p % F.width, y = p / F.width;
         const f3 delta = xyz(A.dimage[p]) / (float)F.spp;
         uint32_t seed = pixel_seed(F, x, y, F.frame_index);
@@ -174,7 +200,7 @@ k_adjoint(AdjointArgs A) {
             PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed, time, lambda};
             // a spectral sample enters the frame value weighted by its wavelength's response
             const f3 ds = (lambda >= 0.0f) ? delta * spectral_weight(lambda) : delta;
-            AdjointSink<FEAT> sink{{}, A.S, F, stk, tc, G, ds, 0u, {}, {}};
+            AdjointSink<FEAT, REC> sink{{}, A.S, F, stk, tc, G, ds, 0u, {}, {}, A.rec, A.rec_count, 0u};
             for (uint32_t b = 0; b < F.max_bounces; ++b) {
                 const HitRec h = trace_closest<false, FEAT>(A.S, o, d, stk, tc, time);
                 VertexResult R;
@@ -199,35 +225,15 @@ k_adjoint(AdjointArgs A) {
 // ---------------------------------------------------------------------------
 // Roughness: forward-mode tangents.  A metal vertex's roughness bends its
 // reflected ray, which moves every later hit, normal, light sample and shadow
-// query.  For every metal vertex j of a sample, the path is replayed from its
-// seed: float up to vertex j (no shadow queries: terms before j do not depend
-// on its roughness), then in dual numbers with d(roughness_j) = 1, and
-// delta . dL/d(roughness_j) goes to the vertex's primitive.  The hit set, the
-// material of every hit and every sampling decision are held fixed, so this
-// is the derivative away from visibility boundaries.
-
-// Records the metal vertices of a path (no shadow queries).
-struct MetalSink : NoGradientHooks {
-    uint32_t n;
-    uint32_t bounce[kAdjMaxVerts];
-    uint32_t prim[kAdjMaxVerts];
-    __device__ __forceinline__ void on_surface(uint32_t b, Surface& sf) {
-        if (sf.mat != 1u || sf.kind != 1u || n >= kAdjMaxVerts) return;
-#pragma unroll
-        for (uint32_t i = 0; i < kAdjMaxVerts; ++i) {
-            if (i == n) {
-                bounce[i] = b;
-                prim[i] = sf.prim;
-            }
-        }
-        n++;
-    }
-    __device__ __forceinline__ void emit(int, const ShadowRec&, PathVertexState&) {}
-};
-
-struct SkipSink : NoGradientHooks {
-    __device__ __forceinline__ void emit(int, const ShadowRec&, PathVertexState&) {}
-};
+// query; terms before the vertex do not depend on it.  k_adjoint recorded
+// each metal vertex with the ray that reached it, its hit and the path state
+// it arrived with (the state the renderer has there: the sink does not touch
+// the random stream).  Each record is continued here in dual numbers with
+// d(roughness) = 1 at that vertex, and delta . dL/d(roughness) goes to the
+// vertex's primitive.  The hit set, the material of every hit and every
+// sampling decision are held fixed, so this is the derivative away from
+// visibility boundaries.  The records are a flat list, so the grid's lanes
+// take one vertex each whatever pixel or sample it belongs to.
 
 // Dual vertices: roughness derivative seeded at bounce seed_b; shadow queries
 // traced at once along the dual ray.
@@ -261,7 +267,7 @@ struct TangentSink : NoGradientHooks {
 
 template <bool LDS, bool FEAT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
-k_adjoint_rough(AdjointArgs A) {
+k_adjoint_dual(AdjointArgs A) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
     __shared__ float lds_grad[LDS ? kAdjLdsPrims : 1u];
     if (LDS) {
@@ -273,70 +279,29 @@ k_adjoint_rough(AdjointArgs A) {
     Stack stk = stack_at(lds_stack + threadIdx.x, kBlock, kLdsDepth, A.spill + (size_t)gid * A.spill_depth, A.spill_depth);
     TraceCounters tc{0u, 0u, 0u};
     const FrameArgs& F = A.F;
-    const uint32_t npix = F.width * F.height;
-    for (uint32_t p = gid; p < npix; p += stride) {
-        const f3 delta = xyz(A.dimage[p]) / (float)F.spp;
-        if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) continue;
-        const uint32_t x = p % F.width, y = p / F.width;
-        uint32_t seed = pixel_seed(F, x, y, F.frame_index);
-        for (uint32_t s = 0; s < F.spp; ++s) {
-            const uint32_t sample_seed = seed;
-            // the sample's metal vertices
-            MetalSink ms{{}, 0u, {}, {}};
-            float time = 1.0f, lambda = -1.0f;
-            {
-                f3 o, d;
-                camera_sample(F, A.X, x, y, s, seed, o, d, time, lambda);
-                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed, time, lambda};
-                for (uint32_t b = 0; b < F.max_bounces; ++b) {
-                    const HitRec h = trace_closest<false, FEAT>(A.S, o, d, stk, tc, time);
-                    VertexResult R;
-                    shade_vertex(A.S, F, b, o, d, h, P, R, ms);
-                    if (!(R.alive && (b + 1u) < F.max_bounces)) break;
-                    o = R.no;
-                    d = R.nd;
-                }
-                seed = P.seed;
-            }
-            for (uint32_t j = 0; j < ms.n; ++j) {
-                uint32_t bj = 0u, pj = 0u;
-#pragma unroll
-                for (uint32_t k = 0; k < kAdjMaxVerts; ++k)
-                    if (k == j) {
-                        bj = ms.bounce[k];
-                        pj = ms.prim[k];
-                    }
-                uint32_t rs = sample_seed;
-                f3 o, d;
-                camera_sample(F, A.X, x, y, s, rs, o, d, time, lambda);
-                PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, rs, time, lambda};
-                SkipSink skip{};
-                uint32_t b = 0;
-                for (; b < bj; ++b) {
-                    const HitRec h = trace_closest<false, FEAT>(A.S, o, d, stk, tc, time);
-                    VertexResult R;
-                    shade_vertex(A.S, F, b, o, d, h, P, R, skip);
-                    o = R.no;
-                    d = R.nd;
-                }
-                d3 od = to<df>(o), dd = to<df>(d);
-                PathVertexStateT<df> PD{to<df>(P.T_), to<df>(f3{0.0f, 0.0f, 0.0f}), df(P.prev_pdf), P.skip_sky,
-                                        P.prev_lambert, P.seed, time, lambda};
-                TangentSink<FEAT> ts{{}, A.S, F, stk, tc, bj};
-                for (; b < F.max_bounces; ++b) {
-                    const HitRec h = trace_closest<false, FEAT>(A.S, value3(od), value3(dd), stk, tc, time);
-                    VertexResultT<df> R;
-                    shade_vertex(A.S, F, b, od, dd, h, PD, R, ts);
-                    if (!(R.alive && (b + 1u) < F.max_bounces)) break;
-                    od = R.no;
-                    dd = R.nd;
-                }
-                const f3 ds = (lambda >= 0.0f) ? delta * spectral_weight(lambda) : delta;
-                const float g = dot(ds, deriv3(PD.L));
-                if (LDS) add_grad(lds_grad, pj, g);
-                else add_grad(A.grads, pj * kMaterialFloats + 3u, g);
-            }
+    const uint32_t n = min(*A.rec_count, A.rec_cap);
+    for (uint32_t e = gid; e < n; e += stride) {
+        const float4* r = A.rec + (size_t)e * kAdjRecFloat4;
+        const float4 e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3], e4 = r[4];
+        const uint32_t bits = __float_as_uint(e3.z);
+        const uint32_t bj = bits & 0xFFu;
+        const uint32_t pj = __float_as_uint(e3.y);
+        d3 od = to<df>(xyz(e0)), dd = to<df>(xyz(e1));
+        HitRec h{e0.w, __float_as_uint(e1.w)};
+        PathVertexStateT<df> PD{to<df>(xyz(e2)), to<df>(f3{0.0f, 0.0f, 0.0f}), df(e2.w), (bits & 256u) != 0u,
+                                (bits & 512u) != 0u, __float_as_uint(e3.x), e4.w, e3.w};
+        TangentSink<FEAT> ts{{}, A.S, F, stk, tc, bj};
+        for (uint32_t b = bj; b < F.max_bounces; ++b) {
+            if (b != bj) h = trace_closest<false, FEAT>(A.S, value3(od), value3(dd), stk, tc, PD.time);
+            VertexResultT<df> R;
+            shade_vertex(A.S, F, b, od, dd, h, PD, R, ts);
+            if (!(R.alive && (b + 1u) < F.max_bounces)) break;
+            od = R.no;
+            dd = R.nd;
         }
+        const float g = dot(xyz(e4), deriv3(PD.L));
+        if (LDS) add_grad(lds_grad, pj, g);
+        else add_grad(A.grads, pj * kMaterialFloats + 3u, g);
     }
     if (LDS) {
         __syncthreads();
@@ -359,23 +324,35 @@ hipError_t launch_adjoint(const AdjointArgs& A, hipStream_t stream) {
     if (nparam) hipLaunchKernelGGL(k_fill_zero, dim3(min(1024u, (nparam + 255u) / 256u)), dim3(256), 0, stream, A.grads, nparam);
     const uint32_t npix = A.F.width * A.F.height;
     if (npix == 0 || A.F.max_bounces == 0 || nparam == 0) return hipGetLastError();
-    const uint32_t blocks = min(kMaxGridBlocks, (npix + kBlock - 1u) / kBlock);
     const bool lds = A.prim_count <= kAdjLdsPrims;
-    if (A.features) {
-        if (lds) hipLaunchKernelGGL((k_adjoint<true, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
-        else hipLaunchKernelGGL((k_adjoint<false, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
-    } else {
-        if (lds) hipLaunchKernelGGL((k_adjoint<true, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
-        else hipLaunchKernelGGL((k_adjoint<false, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
-    }
-    if (A.metal_prims) {
+    const bool rough = A.metal_prims != 0u;
+    const uint32_t chunk = rough ? A.chunk_px : npix;
+    if (chunk == 0u) return hipErrorInvalidValue;
+    // the records' pass: a resident grid (one workgroup per SIMD and wave
+    // slot), every lane on the next record until the list is done
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const uint32_t dual_blocks = min(kMaxGridBlocks, (uint32_t)(cus > 0 ? cus : 1) * kAdjWavesPerSimd);
+    for (uint32_t p0 = 0; p0 < npix; p0 += chunk) {
+        const uint32_t p1 = npix - p0 > chunk ? p0 + chunk : npix;
+        const uint32_t blocks = min(kMaxGridBlocks, (p1 - p0 + kBlock - 1u) / kBlock);
+        if (rough) hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, stream, (float*)A.rec_count, 1u);
+#define DMT_ADJ(L, FT, R) hipLaunchKernelGGL((k_adjoint<L, FT, R>), dim3(blocks), dim3(kBlock), 0, stream, A, p0, p1)
         if (A.features) {
-            if (lds) hipLaunchKernelGGL((k_adjoint_rough<true, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
-            else hipLaunchKernelGGL((k_adjoint_rough<false, true>), dim3(blocks), dim3(kBlock), 0, stream, A);
+            if (rough) { if (lds) DMT_ADJ(true, true, true); else DMT_ADJ(false, true, true); }
+            else { if (lds) DMT_ADJ(true, true, false); else DMT_ADJ(false, true, false); }
         } else {
-            if (lds) hipLaunchKernelGGL((k_adjoint_rough<true, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
-            else hipLaunchKernelGGL((k_adjoint_rough<false, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
+            if (rough) { if (lds) DMT_ADJ(true, false, true); else DMT_ADJ(false, false, true); }
+            else { if (lds) DMT_ADJ(true, false, false); else DMT_ADJ(false, false, false); }
         }
+#undef DMT_ADJ
+        if (!rough) continue;
+#define DMT_DUAL(L, FT) hipLaunchKernelGGL((k_adjoint_dual<L, FT>), dim3(dual_blocks), dim3(kBlock), 0, stream, A)
+        if (A.features) { if (lds) DMT_DUAL(true, true); else DMT_DUAL(false, true); }
+        else { if (lds) DMT_DUAL(true, false); else DMT_DUAL(false, false); }
+#undef DMT_DUAL
     }
     return hipGetLastError();
 }

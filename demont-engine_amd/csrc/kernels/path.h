@@ -204,6 +204,12 @@ struct NoGradientHooks {
     // The surface of bounce b before it is shaded (the tangent pass seeds a
     // roughness derivative here).
     template <class T> __device__ __forceinline__ void on_surface(uint32_t b, SurfaceT<T>& sf) {}
+    // The same vertex with the ray (o, d) that reached it, its closest hit h
+    // and the path state P it arrived with (the adjoint records its metal
+    // vertices here, to replay them in dual numbers from this state).
+    template <class T>
+    __device__ __forceinline__ void on_vertex(uint32_t b, const SurfaceT<T>& sf, V3<T> o, V3<T> d, HitRec h,
+                                              const PathVertexStateT<T>& P) {}
 };
 
 // The k-th kind whose shadow record a vertex's terms are summed in
@@ -262,6 +268,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
     }
     SurfaceT<T> sf = surface_of<SDF>(S, o, d, h, P.time);
     sink.on_surface(b, sf);
+    sink.on_vertex(b, sf, o, d, h, P);
     const V3<T> hp = o + d * sf.t;
     const T ndi = dot(sf.n, d);
     const V3<T> nf = (ndi < 0.0f) ? sf.n : -sf.n;
