@@ -637,6 +637,13 @@ struct Walk {
     uint32_t sdf_left;
 };
 
+// Sphere-trace steps a lane takes per walk step while it traces a cluster
+// (the steps and their results do not depend on how they are grouped).
+#ifndef DMT_SDF_STEPS
+#define DMT_SDF_STEPS 1
+#endif
+constexpr uint32_t kSdfStepsPerWalk = DMT_SDF_STEPS;
+
 // w.cur after an SDF leaf's last cluster: the walk continues from the stack.
 constexpr uint32_t kWalkPop = 0x7FFFFFFFu;
 
@@ -688,15 +695,18 @@ template <bool COUNT, bool MOTION = false, bool SDF = false>
 __device__ __forceinline__ bool walk_step(const SceneView& S, f3 o, f3 d, Walk& w, Stack& st, TraceCounters& tc,
                                           float time = 1.0f) {
     if (SDF && w.sdf_ci != kMiss) {
-        const float t = w.sdf_t;
-        bool done;
-        if (sdf_step(S.sdf + (size_t)w.sdf_ci * kSdfStride, o, d, S.sdf_eps, w.sdf_t)) {
-            w.ab.t = t;
-            w.ab.id = 0xFFFFFFFFu;
-            w.ab.code = kSdfBit | w.sdf_ci;
-            done = true;
-        } else {
-            done = w.sdf_t >= w.sdf_end || --w.sdf_left == 0u;
+        bool done = false;
+        const float4* c = S.sdf + (size_t)w.sdf_ci * kSdfStride;
+        for (uint32_t k = 0; k < kSdfStepsPerWalk && !done; ++k) {
+            const float t = w.sdf_t;
+            if (sdf_step(c, o, d, S.sdf_eps, w.sdf_t)) {
+                w.ab.t = t;
+                w.ab.id = 0xFFFFFFFFu;
+                w.ab.code = kSdfBit | w.sdf_ci;
+                done = true;
+            } else {
+                done = w.sdf_t >= w.sdf_end || --w.sdf_left == 0u;
+            }
         }
         if (!done) return false;
         w.sdf_ci = kMiss;

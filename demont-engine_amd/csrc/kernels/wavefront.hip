@@ -369,13 +369,31 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 state = tail::kLaneIdle;
             }
         }
-        // traversal steps of every lane with a query in progress
+        // traversal steps of every lane with a query in progress.  With SDF
+        // clusters a pass is either a node pass (lanes at BVH nodes and
+        // leaves) or a trace pass (lanes inside a cluster's sphere trace):
+        // the wave runs one of the two codes per pass, the trace pass once
+        // kSdfGather lanes trace or nothing else walks.
+        if constexpr ((EXT & kExtSdf) != 0u && kSdfGather != 0u) {
+            const unsigned long long tracing = __ballot(state == tail::kLaneWalk && w.sdf_ci != kMiss);
+            const unsigned long long nodes = __ballot(state == tail::kLaneWalk && w.sdf_ci == kMiss);
+            const bool trace_pass = tracing != 0ull && ((uint32_t)__popcll(tracing) >= kSdfGather || nodes == 0ull);
+            const uint32_t pass_steps = trace_pass ? kSdfBurst : kTailSteps;
+            for (uint32_t k = 0; k < pass_steps; ++k) {
+                if (state == tail::kLaneWalk && (w.sdf_ci != kMiss) == trace_pass &&
+                    walk_step<COUNT, FEAT, true>(S, l.o, l.d, w, stk, tc, l.P.time)) {
+                    h = final_hit(w.ab, w.tb);
+                    state = tail::kLaneReady;
+                }
+            }
+        } else {
 #pragma unroll
-        for (uint32_t k = 0; k < kTailSteps; ++k) {
-            if (state == tail::kLaneWalk &&
-                walk_step<COUNT, FEAT, (EXT & kExtSdf) != 0u>(S, l.o, l.d, w, stk, tc, l.P.time)) {
-                h = final_hit(w.ab, w.tb);
-                state = tail::kLaneReady;
+            for (uint32_t k = 0; k < kTailSteps; ++k) {
+                if (state == tail::kLaneWalk &&
+                    walk_step<COUNT, FEAT, (EXT & kExtSdf) != 0u>(S, l.o, l.d, w, stk, tc, l.P.time)) {
+                    h = final_hit(w.ab, w.tb);
+                    state = tail::kLaneReady;
+                }
             }
         }
     }
