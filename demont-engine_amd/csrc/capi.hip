@@ -155,6 +155,8 @@ struct dmt_scene {
     size_t adj_rec_bytes = 0;
     uint32_t* adj_rec_count = nullptr;
     float4* moon = nullptr;           // moon texture (dmt_scene_set_moon_map)
+    uchar4* atlas = nullptr;          // PBR atlas (dmt_scene_set_pbr_atlas)
+    bool prims_textured = false;      // an analytic primitive has a tile the atlas holds
 };
 
 namespace {
@@ -334,7 +336,7 @@ dmt_status check_frame(const dmt_frame_params* f) {
     if (f->spp < 1 || f->spp > DMT_MAX_SPP) return fail(DMT_ERR_INVALID_ARG, "spp must be in 1..32");
     if (f->max_bounces > DMT_MAX_BOUNCES) return fail(DMT_ERR_LIMIT, "max_bounces must be <= 64");
     if (f->refract_bounces > 64) return fail(DMT_ERR_LIMIT, "refract_bounces must be <= 64");
-    if (f->flags & ~(DMT_FRAME_NO_RUSSIAN_ROULETTE | DMT_FRAME_FIXED_JITTER))
+    if (f->flags & ~(DMT_FRAME_NO_RUSSIAN_ROULETTE | DMT_FRAME_FIXED_JITTER | DMT_FRAME_MESH_TEXTURES))
         return fail(DMT_ERR_INVALID_ARG, "unknown frame flags");
     if ((f->flags & DMT_FRAME_FIXED_JITTER) &&
         !(std::fabs(f->jitter[0]) <= 0.5f && std::fabs(f->jitter[1]) <= 0.5f))
@@ -394,19 +396,40 @@ bool sky_lights(const dmt_scene* s, const dmt_frame_params* f) {
     return up(f->sun) || up(f->moon);
 }
 
+// The mesh's map tiles a frame asks for (none without DMT_FRAME_MESH_TEXTURES).
+void mesh_tex_of(const dmt_frame_params* f, uint32_t out[4]) {
+    for (int k = 0; k < 4; ++k) out[k] = (f->flags & DMT_FRAME_MESH_TEXTURES) ? f->mesh_tex[k] : kPbrNoTile;
+}
+
+// A frame that samples the PBR atlas: an analytic primitive has a tile the
+// atlas holds, or the frame gives the UV-mapped mesh one.
+bool pbr_active(const dmt_scene* s, const dmt_frame_params* f) {
+    if (!s->view.atlas) return false;
+    if (s->prims_textured) return true;
+    if (!s->view.mesh_uv) return false;
+    uint32_t t[4];
+    mesh_tex_of(f, t);
+    for (uint32_t k : t)
+        if (k < s->view.atlas_tiles) return true;
+    return false;
+}
+
 // The kernel variant a frame runs: kExtSky with its sky lights (and the four
-// shadow-record kinds), kExtSdf with the scene's SDF clusters.
+// shadow-record kinds), kExtSdf with the scene's SDF clusters, kExtPbr with
+// its material maps.
 uint32_t ext_bits(const dmt_scene* s, const dmt_frame_params* f) {
-    return (sky_lights(s, f) ? kExtSky : 0u) | (s->view.sdf_count > 0u ? kExtSdf : 0u);
+    return (sky_lights(s, f) ? kExtSky : 0u) | (s->view.sdf_count > 0u ? kExtSdf : 0u) |
+           (pbr_active(s, f) ? kExtPbr : 0u);
 }
 
 // A frame that draws other numbers than the jitter per sample (lens point,
-// shutter time, hero wavelength), takes a fixed jitter, translates the mesh
-// or has sky lights, and any frame of a scene with SDF clusters, renders
-// through k_tail<COUNT, true>: its traversal is the one that sees them.
+// shutter time, hero wavelength), takes a fixed jitter, translates the mesh,
+// has sky lights or samples the PBR atlas, and any frame of a scene with SDF
+// clusters, renders through k_tail<COUNT, true>: its traversal and shading
+// are the ones that see them.
 bool uses_features(const dmt_scene* s, const dmt_frame_params* f) {
     return f->dof[0] > 0.0f || f->motion_blur || f->spectral || f->mesh_shift_active ||
-           (f->flags & DMT_FRAME_FIXED_JITTER) || sky_lights(s, f) || s->view.sdf_count > 0u;
+           (f->flags & DMT_FRAME_FIXED_JITTER) || sky_lights(s, f) || s->view.sdf_count > 0u || pbr_active(s, f);
 }
 
 FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
@@ -455,6 +478,10 @@ SceneView frame_view(const dmt_scene* s, const dmt_frame_params* f) {
     V.mesh_shift = f->mesh_shift_active;
     V.mesh_prev = f3{f->mesh_shift_prev[0], f->mesh_shift_prev[1], f->mesh_shift_prev[2]};
     V.mesh_curr = f3{f->mesh_shift_curr[0], f->mesh_shift_curr[1], f->mesh_shift_curr[2]};
+    mesh_tex_of(f, V.mesh_tex);
+    // the ray cone's spread: one pixel's angle, 2 tan(fov_y / 2) / height
+    // (PathTrace.slang:5918)
+    V.cone_spread = (2.0f * f->pos_fovtan[3]) / (float)std::max(f->height, 1u);
     return V;
 }
 
@@ -576,6 +603,10 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation during scene build");
     }
     if (!msg.empty()) return fail(DMT_ERR_INVALID_ARG, msg);
+    if (desc->mesh_uvs && P.tri_count)
+        for (size_t i = 0; i < (size_t)desc->vertex_count * 2u; ++i)
+            if (!std::isfinite(desc->mesh_uvs[i]))
+                return fail(DMT_ERR_INVALID_ARG, "mesh UV of vertex " + std::to_string(i / 2) + " is not finite");
     auto* s = new (std::nothrow) dmt_scene();
     if (!s) return fail(DMT_ERR_OUT_OF_MEMORY, "host allocation");
     s->dev = dev;
@@ -716,6 +747,11 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         s->mesh_vertices = desc->vertex_count;
         s->mesh_tris = P.tri_count;
     }
+    if (desc->mesh_uvs && P.tri_count) {
+        // per-vertex UVs, read through the vertex indices of each leaf slot
+        if ((st = upload(s, (const float2*)desc->mesh_uvs, desc->vertex_count, &V.mesh_uv))) return bail(st);
+        V.mesh_tri_vtx = s->leaf_idx;
+    }
     s->bvh_depth = std::max(std::max(tri_depth, P.sphere_bvh.depth), P.sdf_bvh.depth);
     s->bvh_nodes = (uint32_t)(tri_nodes + P.sphere_bvh.nodes.size() + P.sdf_bvh.nodes.size());
     hipError_t e = hipMalloc((void**)&s->count_stats, sizeof(DeviceStats));
@@ -757,6 +793,7 @@ void scene_free(dmt_scene* s) {
     if (s->adj_rec) (void)hipFree(s->adj_rec);
     if (s->adj_rec_count) (void)hipFree(s->adj_rec_count);
     if (s->moon) (void)hipFree(s->moon);
+    if (s->atlas) (void)hipFree(s->atlas);
     for (void* p : s->buffers) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     dmt_device* d = s->dev;
@@ -1041,6 +1078,44 @@ dmt_status dmt_scene_set_moon_map(dmt_scene* s, const float* rgba, uint32_t w, u
     return DMT_OK;
 }
 
+dmt_status dmt_scene_set_pbr_atlas(dmt_scene* s, const uint8_t* rgba8, uint32_t w, uint32_t h) {
+    if (!s) return fail(DMT_ERR_INVALID_ARG, "scene is null");
+    dmt_status st = check_device(s->dev);
+    if (st) return st;
+    if (s->batches > 0) return fail(DMT_ERR_INVALID_ARG, "the scene belongs to a batch");
+    if (rgba8) {
+        if (w != kPbrAtlasW) return fail(DMT_ERR_INVALID_ARG, "PBR atlas width must be 384");
+        if (h == 0 || h % kPbrTile != 0u) return fail(DMT_ERR_INVALID_ARG, "PBR atlas height must be a multiple of 256");
+        if (h / kPbrTile > 4096u) return fail(DMT_ERR_LIMIT, "PBR atlas larger than 4096 tiles");
+    }
+    hipError_t e = sync_all(s->dev);   // frames in flight read the old atlas
+    if (e != hipSuccess) return mark_lost_if_fatal(s->dev, e, "synchronize");
+    if (s->atlas) (void)hipFree(s->atlas);
+    s->atlas = nullptr;
+    s->view.atlas = nullptr;
+    s->view.atlas_tiles = 0u;
+    s->prims_textured = false;
+    if (!rgba8) return DMT_OK;
+    const size_t bytes = (size_t)w * h * 4u;
+    e = hipMalloc((void**)&s->atlas, bytes);
+    if (e == hipSuccess) e = hipMemcpy(s->atlas, rgba8, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (s->atlas) (void)hipFree(s->atlas);
+        s->atlas = nullptr;
+        return hip_fail(e, "PBR atlas upload");
+    }
+    s->view.atlas = s->atlas;
+    s->view.atlas_tiles = h / kPbrTile;
+    const size_t n = s->prims_host.size() / 28u;
+    for (size_t i = 0; i < n && !s->prims_textured; ++i)
+        for (int c = 24; c < 28; ++c) {
+            uint32_t t;
+            std::memcpy(&t, &s->prims_host[i * 28u + c], 4);
+            if (t < s->view.atlas_tiles) s->prims_textured = true;
+        }
+    return DMT_OK;
+}
+
 dmt_status dmt_trace_closest_noalloc(dmt_scene* s, const float* d_rays, uint32_t n, dmt_hit* d_out) {
     static_assert(sizeof(dmt_hit) == sizeof(HitOut), "dmt_hit layout");
     if (!s || (n && (!d_rays || !d_out))) return fail(DMT_ERR_INVALID_ARG, "null pointer");
@@ -1236,6 +1311,8 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
         return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate sky lights (sun, moon, HDRI clusters)");
     if (s->view.sdf_count)
         return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate scenes with SDF clusters");
+    if (pbr_active(s, f))
+        return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate PBR texture maps");
     const uint32_t prims = s->view.prim_count;
     if (prims && (!d_image_grad || !d_param_grad)) return fail(DMT_ERR_INVALID_ARG, "null device pointer");
     if (prims == 0) return DMT_OK;
@@ -1481,6 +1558,8 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         if (!list[i]) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is null");
         if (list[i]->dev != dev) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is on another device");
         if ((st = check_frame_for(list[i], &frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
+        if (pbr_active(list[i], &frames[i]))
+            return fail(DMT_ERR_UNSUPPORTED, "frame " + std::to_string(i) + ": the batched form does not sample PBR texture maps");
     }
     {
         std::vector<dmt_scene*> sorted = list;

@@ -288,6 +288,17 @@ struct SceneView {
     const float4* sdf_nodes;
     const float4* sdf_leaf;
     float sdf_eps;
+    // PBR material atlas (pbr.h): RGBA8 texels, kPbrAtlasW wide, atlas_tiles
+    // tiles of kPbrTile rows (null: none); per-vertex mesh UVs (null: none)
+    // and the vertex indices of each leaf slot's triangle; per frame the
+    // mesh's map tiles (albedo, normal, roughness, metallic) and the ray
+    // cone's spread per unit of distance (2 tan(fov_y / 2) / height)
+    const uchar4* atlas;
+    uint32_t atlas_tiles;
+    const float2* mesh_uv;
+    const uint32_t* mesh_tri_vtx;
+    uint32_t mesh_tex[4];
+    float cone_spread;
 };
 
 // Traversal stack: `lds_depth` entries in LDS (thread-interleaved), the rest

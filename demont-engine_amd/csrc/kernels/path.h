@@ -11,6 +11,7 @@
 //                   contribution once the transmittance is known
 #pragma once
 #include "shade.h"
+#include "pbr.h"
 
 namespace dmt {
 
@@ -180,6 +181,7 @@ template <class T> struct PathVertexStateT {
     uint32_t seed;
     float time = 1.0f;      // shutter time of the sample (1: no motion blur)
     float lambda = -1.0f;   // hero wavelength in nm (< 0: RGB)
+    float cone = 0.0f;      // ray-cone width at the current origin (PBR frames)
 };
 
 using PathVertexState = PathVertexStateT<float>;
@@ -235,7 +237,7 @@ __device__ __forceinline__ V3<T> miss_sky(const SceneView& S, const SkyArgs* K, 
 // NEE of the env branch and the cluster test of the MIS miss (:6493-6518,
 // :6074); SKY frames are float only.  SDF: h may be an SDF cluster's hit
 // (the feature path's traversal).
-template <bool SKY = false, bool SDF = false, class T, class Sink>
+template <bool SKY = false, bool SDF = false, bool PBR = false, class T, class Sink>
 __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs& F, uint32_t b, V3<T> o, V3<T> d,
                                              HitRec h, PathVertexStateT<T>& P, VertexResultT<T>& R, Sink& sink,
                                              const SkyArgs* K = nullptr) {
@@ -267,6 +269,14 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
         return;
     }
     SurfaceT<T> sf = surface_of<SDF>(S, o, d, h, P.time);
+    if constexpr (PBR) {
+        // the ray cone grows by this segment, then the hit's material maps
+        // apply before the shading normal and the BSDF are formed
+        // (PathTrace.slang:6162-6171); PBR frames are float only
+        static_assert(sizeof(T) == sizeof(float), "PBR frames are float only");
+        P.cone = P.cone + S.cone_spread * sf.t;
+        apply_pbr(S, o, d, o + d * sf.t, h, sf, P.time, P.cone);
+    }
     sink.on_surface(b, sf);
     sink.on_vertex(b, sf, o, d, h, P);
     const V3<T> hp = o + d * sf.t;

@@ -44,8 +44,9 @@ struct SkyArgs {
 };
 
 // Extensions a frame-completion kernel variant carries: the frame's sky
-// lights (four shadow-record kinds) and the scene's SDF clusters.
-constexpr uint32_t kExtSky = 1u, kExtSdf = 2u;
+// lights (four shadow-record kinds), the scene's SDF clusters and its PBR
+// material maps.
+constexpr uint32_t kExtSky = 1u, kExtSdf = 2u, kExtPbr = 4u;
 
 // Camera-sample features and sky lights of a frame (all off: the plain
 // camera ray and no sky lights).

@@ -195,6 +195,7 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureAr
         l.P.prev_pdf = 0.0f;
         l.P.skip_sky = false;
         l.P.prev_lambert = false;
+        l.P.cone = 0.0f;
         l.b = 0u;
         l.mode = 0u;
         return;

@@ -319,8 +319,8 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 else n_ext++;
                 SlotSink sink{{}, sl, se, s2, s3, l.slot, l.pix, 0u};
                 VertexResult R;
-                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u>(S, F, l.b, l.o, l.d, h, l.P, R, sink,
-                                                                              &X.sky_args);
+                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u, (EXT & kExtPbr) != 0u>(
+                    S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.emitted | (cont ? 0u : tail::kEnded);
                 if (cont) {
@@ -664,15 +664,23 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
             hipLaunchKernelGGL(kernel, dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0], W.counters, W.ps,
                                sq[0], sq[1], sq[2], sq[3], W.spill, W.spill_depth, W.stats);
         };
-        switch ((o.counting ? 4u : 0u) + (X.ext & 3u)) {
+        switch ((o.counting ? 8u : 0u) + (X.ext & 7u)) {
             case 0u: go(k_tail<false, true, 0u>); break;
             case 1u: go(k_tail<false, true, kExtSky>); break;
             case 2u: go(k_tail<false, true, kExtSdf>); break;
             case 3u: go(k_tail<false, true, kExtSky | kExtSdf>); break;
-            case 4u: go(k_tail<true, true, 0u>); break;
-            case 5u: go(k_tail<true, true, kExtSky>); break;
-            case 6u: go(k_tail<true, true, kExtSdf>); break;
-            default: go(k_tail<true, true, kExtSky | kExtSdf>); break;
+            case 4u: go(k_tail<false, true, kExtPbr>); break;
+            case 5u: go(k_tail<false, true, kExtPbr | kExtSky>); break;
+            case 6u: go(k_tail<false, true, kExtPbr | kExtSdf>); break;
+            case 7u: go(k_tail<false, true, kExtPbr | kExtSky | kExtSdf>); break;
+            case 8u: go(k_tail<true, true, 0u>); break;
+            case 9u: go(k_tail<true, true, kExtSky>); break;
+            case 10u: go(k_tail<true, true, kExtSdf>); break;
+            case 11u: go(k_tail<true, true, kExtSky | kExtSdf>); break;
+            case 12u: go(k_tail<true, true, kExtPbr>); break;
+            case 13u: go(k_tail<true, true, kExtPbr | kExtSky>); break;
+            case 14u: go(k_tail<true, true, kExtPbr | kExtSdf>); break;
+            default: go(k_tail<true, true, kExtPbr | kExtSky | kExtSdf>); break;
         }
         if (ev) {
             (void)hipEventRecord(o.extend_events[1], stream);

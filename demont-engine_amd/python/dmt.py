@@ -24,9 +24,10 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
 DMT_FRAME_FIXED_JITTER = 2
+FRAME_MESH_TEXTURES = 4          # DMT_FRAME_MESH_TEXTURES
 SCHEDULE_WAVEFRONT = 0xFFFFFFFF
 MATERIAL_FLOATS = 7
 DMT_NOALLOC_BLOCKS = 512        # dmt_trace_closest_noalloc's grid
@@ -75,6 +76,7 @@ class FrameParams(C.Structure):
         ("hdri_col", (C.c_float * 4) * 8),
         ("sdf_max_iters", C.c_uint32),
         ("sdf_epsilon", C.c_float),
+        ("mesh_tex", C.c_uint32 * 4),
     ]
 
     def copy(self) -> "FrameParams":
@@ -97,6 +99,7 @@ class SceneDesc(C.Structure):
         ("env_total_luminance", C.c_double),
         ("sdf_clusters", C.POINTER(C.c_float)), ("sdf_cluster_count", C.c_uint32),
         ("reserved2", C.c_uint32),
+        ("mesh_uvs", C.POINTER(C.c_float)),
     ]
 
 
@@ -237,6 +240,7 @@ def lib() -> C.CDLL:
         "dmt_build_env_cdf_masked": (C.c_double, [fp, u32, u32, C.POINTER(C.c_uint8), fp, fp]),
         "dmt_env_extract_lights": (u32, [fp, u32, u32, C.c_double, fp, fp, C.POINTER(C.c_uint8)]),
         "dmt_scene_set_moon_map": (st, [vp, fp, u32, u32]),
+        "dmt_scene_set_pbr_atlas": (st, [vp, C.POINTER(C.c_uint8), u32, u32]),
         "dmt_scene_get_materials": (st, [vp, fp, C.c_size_t]),
         "dmt_scene_set_materials": (st, [vp, fp, C.c_size_t]),
         "dmt_adjoint": (st, [vp, C.POINTER(FrameParams), vp, vp]),
@@ -296,6 +300,8 @@ class SceneData:
     env_cdf: tuple | None = None           # (marginal (H,), conditional (H, W), total) given instead of built
     moon: np.ndarray | None = None         # (H, W, 4) moon texture
     sdf: np.ndarray | None = None          # (N, 112) SDF cluster records
+    atlas: np.ndarray | None = None        # (256 * tiles, 384, 4) uint8 PBR atlas
+    uvs: np.ndarray | None = None          # (V, 2) float32 mesh UVs
 
     def desc(self) -> SceneDesc:
         for name in ("prims", "lights", "positions"):
@@ -318,6 +324,9 @@ class SceneData:
         if self.sdf is not None and np.asarray(self.sdf).size:
             self.sdf = np.ascontiguousarray(self.sdf, np.float32).reshape(-1, SDF_CLUSTER_FLOATS)
             d.sdf_clusters, d.sdf_cluster_count = _fptr(self.sdf), self.sdf.shape[0]
+        if self.uvs is not None:
+            self.uvs = np.ascontiguousarray(self.uvs, np.float32).reshape(-1, 2)
+            d.mesh_uvs = _fptr(self.uvs)
         return d
 
 
@@ -435,6 +444,16 @@ class Scene:
         _check(lib().dmt_scene_create(device.handle, C.byref(d), C.byref(bp), C.byref(self._h)))
         if data.moon is not None:
             self.set_moon_map(data.moon)
+        if data.atlas is not None:
+            self.set_pbr_atlas(data.atlas)
+
+    def set_pbr_atlas(self, rgba8: np.ndarray | None):
+        """The PBR atlas: (256 * tiles, 384, 4) uint8, or None."""
+        if rgba8 is None:
+            _check(lib().dmt_scene_set_pbr_atlas(self._h, None, 0, 0))
+            return
+        a = np.ascontiguousarray(rgba8, np.uint8)
+        _check(lib().dmt_scene_set_pbr_atlas(self._h, a.ctypes.data_as(C.POINTER(C.c_uint8)), a.shape[1], a.shape[0]))
 
     def set_moon_map(self, rgba: np.ndarray | None):
         if rgba is None:

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define DMT_ABI_VERSION 6u
+#define DMT_ABI_VERSION 7u
 
 typedef int32_t dmt_status;
 #define DMT_OK                  0
@@ -93,6 +93,11 @@ typedef struct dmt_scene_desc {
     const float*    sdf_clusters;
     uint32_t        sdf_cluster_count;
     uint32_t        reserved2;
+    /* Optional per-vertex mesh UVs, 2 floats per vertex (the engine's
+     * mesh_uvs buffer, slot 17, parallel to the positions): the mesh's PBR
+     * maps (dmt_frame_params.mesh_tex) are sampled at the hit's barycentric
+     * UV.  NULL: the mesh is untextured. */
+    const float*    mesh_uvs;
 } dmt_scene_desc;
 
 #define DMT_SDF_CLUSTER_FLOATS 112u
@@ -164,6 +169,10 @@ typedef struct dmt_frame_params {
      * engine clamps it to >= 1e-7). */
     uint32_t sdf_max_iters;
     float    sdf_epsilon;
+    /* The mesh's PBR map tiles (mesh_tex_indices: albedo, normal, roughness,
+     * metallic; 0xFFFFFFFF = none), read when flags has
+     * DMT_FRAME_MESH_TEXTURES and the scene has mesh UVs and an atlas. */
+    uint32_t mesh_tex[4];
 } dmt_frame_params;
 
 /* Russian roulette off (the reference applies it from bounce 2 on,
@@ -175,6 +184,9 @@ typedef struct dmt_frame_params {
 /* Sample 0 takes jitter[] instead of two random draws (the engine sets it
  * while a denoiser runs, PathTrace.slang:5730). */
 #define DMT_FRAME_FIXED_JITTER 2u
+/* mesh_tex holds the mesh's map tiles (the engine sets mesh_tex_indices every
+ * frame; a zero-initialised frame keeps the mesh untextured). */
+#define DMT_FRAME_MESH_TEXTURES 4u
 
 typedef struct dmt_build_params {
     uint32_t sah_bins;          /* binned-SAH bin count (0 = default 16)     */
@@ -283,6 +295,19 @@ dmt_status dmt_scene_bind_accum(dmt_scene* scene, void* d_rgba, uint32_t width, 
  * disc's north; copied.  NULL (or never set): the disc's surface reads 0.
  * Waits for the scene's frames in flight. */
 dmt_status dmt_scene_set_moon_map(dmt_scene* scene, const float* rgba, uint32_t width, uint32_t height);
+
+/* The PBR material atlas (pbr_atlas, Engine.cpp:3984-4148, bound at
+ * storage-texture slot 16): RGBA8, width 384 (a 256-square base image per tile
+ * and its mips 1..8 in the right column, Engine::BuildPbrTileMips), height a
+ * multiple of 256 (one 256-row band per tile); copied.  A texel reads as
+ * byte * (1/255).  An analytic primitive's v6 holds its albedo, normal,
+ * roughness and metallic tiles (bit patterns; 0xFFFFFFFF = none), the mesh's
+ * come with the frame; a tile applies when it is below height / 256
+ * (applyPbrTextures, PathTrace.slang:1435-1605).  Frames of a scene with
+ * textured surfaces render in the frame-completion kernel's PBR variant; the
+ * batched form and the adjoint refuse them.  NULL: no atlas.  Waits for the
+ * scene's frames in flight. */
+dmt_status dmt_scene_set_pbr_atlas(dmt_scene* scene, const uint8_t* rgba8, uint32_t width, uint32_t height);
 
 /* Display output (r_tonemap_op, exposure_state[0]). */
 #define DMT_TONEMAP_DISPLAY  0u   /* sRGB OETF of tonemap(mean * exposure)     */

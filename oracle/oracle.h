@@ -38,6 +38,12 @@ typedef struct orc_scene_desc {
     /* optional SDF clusters, 112 floats each (the engine's sdf_clusters
      * records, Engine.cpp:5076-5190), traced after the analytic prims */
     const float*    sdf_clusters; uint32_t sdf_cluster_count;
+    /* optional PBR material atlas as the engine builds it (RGBA8, 384 wide:
+     * a 256-square base image per tile band and its mips in the right
+     * column, 256 rows per tile; PathTrace.slang:1435-1605), sampled as
+     * byte * (1/255); and per-vertex mesh UVs (2 floats each) */
+    const uint8_t*  pbr_atlas;  uint32_t pbr_atlas_width; uint32_t pbr_atlas_height;
+    const float*    mesh_uvs;
 } orc_scene_desc;
 
 typedef struct orc_frame {
@@ -85,6 +91,9 @@ typedef struct orc_frame {
      * sdf_params_f.x), read when the scene has SDF clusters */
     uint32_t sdf_max_iters;
     float    sdf_epsilon;
+    /* atlas tiles of the mesh's albedo, normal, roughness and metallic maps
+     * (mesh_tex_indices; 0xFFFFFFFF = none) */
+    uint32_t mesh_tex[4];
 } orc_frame;
 
 typedef struct orc_hit {
@@ -165,6 +174,18 @@ int orc_trace_at(const orc_scene* s, const float* rays, uint32_t n, float time, 
  * normal.xyz, albedo.xyz, specular albedo (F0).xyz, roughness, specular hit
  * distance, 0, 0.  vp_curr / vp_prev are column-major (clip = M * (p, 1)). */
 int orc_gbuffer(const orc_scene* s, const orc_frame* f, const float* vp_curr, const float* vp_prev, float* out);
+
+/* PBR atlas probes (fp32): samplePbrTileLinear of `tile` at (u, v, lod)
+ * triples -> 4 floats each; the hit's PBR surface for n rays (8 floats each)
+ * with the given ray-cone width -> 16 floats each: u, v, uv_scale, lod,
+ * textured albedo rgb, roughness, material, shading normal xyz, t, kind
+ * (0 miss), 0, 0; applyNormalMap of n normals by n tangent-space vectors;
+ * srgbToLinear of n values. */
+int  orc_pbr_sample(const orc_scene* s, uint32_t tile, const float* uvl, uint32_t n, float* out);
+int  orc_pbr_surface(const orc_scene* s, const orc_frame* f, const float* rays, uint32_t n, float cone_width,
+                     float* out);
+void orc_pbr_normal_map(const float* normals, const float* ts, uint32_t n, float* out);
+void orc_pbr_srgb(const float* c, uint32_t n, float* out);
 
 /* Host builders, exposed for tests. */
 double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H,

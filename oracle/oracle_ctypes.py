@@ -27,6 +27,8 @@ class OrcSceneDesc(C.Structure):
         ("env_total_luminance", C.c_double),
         ("moon_rgba", C.POINTER(C.c_float)), ("moon_width", C.c_uint32), ("moon_height", C.c_uint32),
         ("sdf_clusters", C.POINTER(C.c_float)), ("sdf_cluster_count", C.c_uint32),
+        ("pbr_atlas", C.POINTER(C.c_uint8)), ("pbr_atlas_width", C.c_uint32), ("pbr_atlas_height", C.c_uint32),
+        ("mesh_uvs", C.POINTER(C.c_float)),
     ]
 
 
@@ -93,6 +95,14 @@ def lib():
         L.orc_sdf_eval.restype = None
         L.orc_sdf_trace.argtypes = [fp, fp, u32, u32, C.c_float, fp]
         L.orc_sdf_trace.restype = None
+        L.orc_pbr_sample.argtypes = [vp, u32, fp, u32, fp]
+        L.orc_pbr_sample.restype = C.c_int
+        L.orc_pbr_surface.argtypes = [vp, vp, fp, u32, C.c_float, fp]
+        L.orc_pbr_surface.restype = C.c_int
+        L.orc_pbr_normal_map.argtypes = [fp, fp, u32, fp]
+        L.orc_pbr_normal_map.restype = None
+        L.orc_pbr_srgb.argtypes = [fp, u32, fp]
+        L.orc_pbr_srgb.restype = None
         _lib = L
     return _lib
 
@@ -107,7 +117,7 @@ class OracleScene:
     """Oracle copy of a scene given in the reference wire formats."""
 
     def __init__(self, prims, lights, positions, indices, env=None, env_intensity=1.0, env_cdf=None, moon=None,
-                 sdf=None):
+                 sdf=None, atlas=None, uvs=None):
         self._keep = [np.ascontiguousarray(prims, np.float32), np.ascontiguousarray(lights, np.float32),
                       np.ascontiguousarray(positions, np.float32), np.ascontiguousarray(indices, np.uint32)]
         d = OrcSceneDesc()
@@ -135,6 +145,15 @@ class OracleScene:
             sc = np.ascontiguousarray(sdf, np.float32).reshape(-1, 112)
             self._keep.append(sc)
             d.sdf_clusters, d.sdf_cluster_count = _fp(sc), sc.shape[0]
+        if atlas is not None:
+            at = np.ascontiguousarray(atlas, np.uint8)
+            self._keep.append(at)
+            d.pbr_atlas = at.ctypes.data_as(C.POINTER(C.c_uint8))
+            d.pbr_atlas_height, d.pbr_atlas_width = at.shape[0], at.shape[1]
+        if uvs is not None:
+            uv = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+            self._keep.append(uv)
+            d.mesh_uvs = _fp(uv)
         self._h = C.c_void_p()
         rc = lib().orc_scene_create(C.byref(d), C.byref(self._h))
         if rc != 0:
@@ -143,7 +162,8 @@ class OracleScene:
     @classmethod
     def from_scene_data(cls, sd):
         return cls(sd.prims, sd.lights, sd.positions, sd.indices, sd.env, sd.env_intensity,
-                   getattr(sd, "env_cdf", None), getattr(sd, "moon", None), getattr(sd, "sdf", None))
+                   getattr(sd, "env_cdf", None), getattr(sd, "moon", None), getattr(sd, "sdf", None),
+                   getattr(sd, "atlas", None), getattr(sd, "uvs", None))
 
     def render(self, frame, precision=32, threads=0, rows=None, col_step=1, accum=None):
         """Render one frame; `frame` is any ctypes struct with the dmt_frame_params layout.
@@ -254,6 +274,27 @@ class OracleScene:
             raise ValueError("no light tree")
         return out
 
+    def pbr_sample(self, tile, uvl):
+        """samplePbrTileLinear of `tile` at (u, v, lod) rows (n, 3): (n, 4)."""
+        q = np.ascontiguousarray(uvl, np.float32).reshape(-1, 3)
+        out = np.zeros((q.shape[0], 4), np.float32)
+        if lib().orc_pbr_sample(self._h, tile, _fp(q), q.shape[0], _fp(out)) != 0:
+            raise ValueError("no atlas or tile out of range")
+        return out
+
+    def pbr_surface(self, frame, origins, dirs, cone_width=0.0):
+        """The hit's PBR surface (n, 16): u, v, uv_scale, lod, albedo rgb,
+        roughness, material, normal xyz, t, kind, 0, 0."""
+        n = np.asarray(origins).reshape(-1, 3).shape[0]
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = np.asarray(origins, np.float32).reshape(-1, 3)
+        rays[:, 4:7] = np.asarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((n, 16), np.float32)
+        if lib().orc_pbr_surface(self._h, C.cast(C.byref(frame), C.c_void_p), _fp(rays), n, float(cone_width),
+                                 _fp(out)) != 0:
+            raise ValueError("oracle pbr surface failed")
+        return out
+
     def __del__(self):
         try:
             if self._h:
@@ -341,4 +382,21 @@ def sdf_trace(cluster, origins, dirs, max_iters=128, epsilon=1e-4):
     rays[:, 4:7] = np.asarray(dirs, np.float32).reshape(-1, 3)
     out = np.zeros((n, 4), np.float32)
     lib().orc_sdf_trace(_fp(c), _fp(rays), n, max_iters, epsilon, _fp(out))
+    return out
+
+
+def pbr_normal_map(normals, ts):
+    """applyNormalMap (fp32) of unit normals (n, 3) by tangent-space vectors (n, 3)."""
+    n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(ts, np.float32).reshape(-1, 3)
+    out = np.zeros_like(n)
+    lib().orc_pbr_normal_map(_fp(n), _fp(t), n.shape[0], _fp(out))
+    return out
+
+
+def pbr_srgb(c):
+    """srgbToLinear (fp32) of values c."""
+    c = np.ascontiguousarray(c, np.float32).reshape(-1)
+    out = np.zeros_like(c)
+    lib().orc_pbr_srgb(_fp(c), c.size, _fp(out))
     return out

@@ -260,6 +260,8 @@ struct MathF {
         return p * scale;
     }
     static float log(float x) { return cephes_logf(x); }
+    // log2 as log(x) * log2(e) (the ray-cone LOD's log2)
+    static float log2(float x) { return cephes_logf(x) * 1.44269504088896341f; }
     static float pow(float x, float y) {
         if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : INFINITY);
         return exp(y * log(x));
@@ -274,6 +276,7 @@ struct MathD {
     static double exp(double x) { return std::exp(x); }
     static double asin(double x) { return std::asin(x); }
     static double log(double x) { return std::log(x); }
+    static double log2(double x) { return std::log2(x); }
     static double pow(double x, double y) { return std::pow(x, y); }
 };
 
@@ -297,6 +300,7 @@ struct MathDual {
         return {std::asin(x.v), s > 0.0 ? x.d / s : 0.0};
     }
     static Dual log(Dual x) { return {std::log(x.v), x.v > 0.0 ? x.d / x.v : 0.0}; }
+    static Dual log2(Dual x) { return {std::log2(x.v), x.v > 0.0 ? x.d / (x.v * 0.6931471805599453) : 0.0}; }
     static Dual pow(Dual x, Dual y) {
         const double p = std::pow(x.v, y.v);
         const double dx = x.v > 0.0 ? y.v * std::pow(x.v, y.v - 1.0) * x.d : 0.0;

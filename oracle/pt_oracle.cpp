@@ -370,6 +370,9 @@ struct orc_scene {
     uint32_t moon_w = 0, moon_h = 0;
     std::vector<float> sdf;          // SDF clusters, 112 floats each
     uint32_t sdf_count = 0;
+    std::vector<float> atlas;        // PBR atlas, RGBA floats (byte * (1/255)), 384 wide
+    uint32_t atlas_w = 0, atlas_h = 0, atlas_tiles = 0;
+    std::vector<float> uvs;          // mesh UVs, 2 per vertex
     // tangent direction of a PT<Dual> run: material parameter k (albedo rgb,
     // roughness, emission rgb) of analytic prim tangent_prim
     uint32_t tangent_prim = 0xFFFFFFFFu;
@@ -499,6 +502,21 @@ template <class R> inline TriT<R> tri_test(const orc_scene& S, uint32_t tri, V3<
     if (v < R(0) || u + v > R(1)) return {false, R(0)};
     R t = dot(e2, qv) * inv_det;
     return {t > R(1e-3), t};
+}
+
+// The barycentrics (u, v) Moller-Trumbore forms for triangle tri and the ray
+// (the weights of v1 and v2 in the mesh UV interpolation).
+template <class R> inline void tri_bary(const orc_scene& S, uint32_t tri, V3<R> ro, V3<R> rd, R& u, R& v) {
+    const uint32_t* ix = &S.idx[(size_t)tri * 3];
+    V3<R> v0 = ld3<R>(&S.pos[(size_t)ix[0] * 3]);
+    V3<R> v1 = ld3<R>(&S.pos[(size_t)ix[1] * 3]);
+    V3<R> v2 = ld3<R>(&S.pos[(size_t)ix[2] * 3]);
+    V3<R> e1 = v1 - v0, e2 = v2 - v0;
+    V3<R> pv = cross(rd, e2);
+    R inv_det = R(1) / dot(e1, pv);
+    V3<R> tv = ro - v0;
+    u = dot(tv, pv) * inv_det;
+    v = dot(rd, cross(tv, e1)) * inv_det;
 }
 
 template <class R> void mesh_fill(const orc_scene& S, uint32_t tri, R t, Hit<R>& h) {
@@ -1382,6 +1400,139 @@ template <class R> struct PT {
         }
     }
 
+    // ---- PBR material atlas (PathTrace.slang:1435-1605, applyPbrTextures;
+    // the software device's CPU mirror SoftwareTracer.cpp:356-525) ----
+    static constexpr uint32_t kPbrTile = 256u, kPbrMaxLod = 8u;
+    struct V4 {
+        R x, y, z, w;
+    };
+    static V4 mix4(V4 a, V4 b, R t) { return {a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z), a.w + t * (b.w - a.w)}; }
+    V4 texel(uint32_t x, uint32_t y) const {
+        const float* q = &S.atlas[((size_t)y * S.atlas_w + x) * 4];
+        return {R(q[0]), R(q[1]), R(q[2]), R(q[3])};
+    }
+    // pbrMipRegion + samplePbrTileLevel: bilinear within mip `level` of the
+    // tile, wrapped at the level's resolution.
+    V4 tile_level(uint32_t tile, R u, R v, uint32_t level) const {
+        const uint32_t row_base = tile * kPbrTile;
+        uint32_t x0, y0, dim;
+        if (level == 0u) {
+            x0 = 0u; y0 = row_base; dim = kPbrTile;
+        } else {
+            dim = kPbrTile >> level;
+            x0 = kPbrTile;
+            y0 = row_base + (kPbrTile - 2u * dim);
+        }
+        const R fu = u - ofloor(u), fv = v - ofloor(v);
+        const R fx = fu * R(dim) - R(0.5), fy = fv * R(dim) - R(0.5);
+        const R fx0 = ofloor(fx), fy0 = ofloor(fy);
+        const R tx = fx - fx0, ty = fy - fy0;
+        const int D = (int)dim;
+        int ix0 = (int)value_of(fx0), iy0 = (int)value_of(fy0);
+        const int ix1 = ((ix0 + 1) % D + D) % D, iy1 = ((iy0 + 1) % D + D) % D;
+        ix0 = (ix0 % D + D) % D;
+        iy0 = (iy0 % D + D) % D;
+        const V4 c00 = texel(x0 + (uint32_t)ix0, y0 + (uint32_t)iy0), c10 = texel(x0 + (uint32_t)ix1, y0 + (uint32_t)iy0);
+        const V4 c01 = texel(x0 + (uint32_t)ix0, y0 + (uint32_t)iy1), c11 = texel(x0 + (uint32_t)ix1, y0 + (uint32_t)iy1);
+        return mix4(mix4(c00, c10, tx), mix4(c01, c11, tx), ty);
+    }
+    // samplePbrTileLinear: trilinear between the two bracketing levels.
+    V4 tile_linear(uint32_t tile, R u, R v, R lod) const {
+        const R l = clampv(lod, R(0), R(kPbrMaxLod));
+        const uint32_t l0 = (uint32_t)value_of(ofloor(l));
+        const uint32_t l1 = std::min(l0 + 1u, kPbrMaxLod);
+        const R f = l - R(l0);
+        const V4 a = tile_level(tile, u, v, l0);
+        if (f <= R(0)) return a;
+        return mix4(a, tile_level(tile, u, v, l1), f);
+    }
+    static R srgb_to_linear(R c) {
+        const R lo = c / R(12.92);
+        const R hi = M::pow(std::max((c + R(0.055)) / R(1.055), R(0)), R(2.4));
+        return (c <= R(0.04045)) ? lo : hi;
+    }
+    // pbrConeTexLod: the ray cone's footprint projected onto the surface.
+    static R cone_lod(R cone_width, V3<R> rd, V3<R> n, R uv_scale) {
+        if (uv_scale <= R(0)) return R(0);
+        const R cos_i = std::max(oabs(dot(rd, n)), R(1e-3));
+        const R texels = ((cone_width / cos_i) * uv_scale) * R(kPbrTile);
+        return clampv(M::log2(std::max(texels, R(1))), R(0), R(kPbrMaxLod));
+    }
+    // applyNormalMap: Duff et al.'s branchless basis around n.
+    static V3<R> normal_map(V3<R> n, V3<R> ts) {
+        const R s = (n.z >= R(0)) ? R(1) : R(-1);
+        const R a = R(-1) / (s + n.z);
+        const R b = (n.x * n.y) * a;
+        const V3<R> t{R(1) + ((s * n.x) * n.x) * a, s * b, -(s * n.x)};
+        const V3<R> bt{b, s + (n.y * n.y) * a, -n.y};
+        return normalize((t * ts.x + bt * ts.y) + n * ts.z);
+    }
+    // The hit's UV, UV density and tiles (testAnalyticPrim's sphere and plane
+    // maps, PathTrace.slang:2104-2158; the mesh's barycentric UVs, :2536-2543),
+    // its ray-cone LOD, then the maps applied in place: albedo (sRGB decoded,
+    // times the flat tint), roughness, metallic (Lambert -> metal above 0.5),
+    // the normal map last.  A tile applies when it is below the atlas's tile
+    // count (the CPU mirror's gate; kPbrNoTexTile never is).
+    void pbr_uv(const Hit<R>& h, V3<R> ro, V3<R> rd, R time, uint32_t tiles[4], R& u, R& v, R& uv_scale) const {
+        for (int k = 0; k < 4; ++k) tiles[k] = 0xFFFFFFFFu;
+        u = R(0); v = R(0); uv_scale = R(0);
+        if (h.kind == 1u) {
+            const float* p = &S.prims[(size_t)h.prim * 28];
+            for (int k = 0; k < 4; ++k) tiles[k] = f2u(p[24 + k]);
+            if ((uint32_t)p[8] == PRIM_SPHERE) {
+                const V3<R> sn = h.n;
+                u = R(0.5) + M::atan2(sn.z, sn.x) / (R(2) * PI);
+                v = R(0.5) - M::asin(clampv(sn.y, R(-1), R(1))) / PI;
+                const R sr = std::max(R(p[3]), R(1e-4));
+                const R cos_lat = std::max(osqrt(sn.x * sn.x + sn.z * sn.z), R(0.05));
+                uv_scale = std::max(R(1) / (PI * sr), R(1) / (((R(2) * PI) * sr) * cos_lat));
+            } else {
+                const V3<R> hp = ro + rd * h.t;
+                const V3<R> an{oabs(h.n.x), oabs(h.n.y), oabs(h.n.z)};
+                if (an.y >= an.x && an.y >= an.z) { u = hp.x; v = hp.z; }
+                else if (an.x >= an.z) { u = hp.z; v = hp.y; }
+                else { u = hp.x; v = hp.y; }
+                uv_scale = R(1);
+            }
+        } else if (h.kind == 2u && !S.uvs.empty()) {
+            for (int k = 0; k < 4; ++k) tiles[k] = (F.flags & 4u) ? F.mesh_tex[k] : 0xFFFFFFFFu;   // DMT_FRAME_MESH_TEXTURES
+            V3<R> rm = ro;
+            if (F.mesh_shift_active) rm = ro - mix3(ld3<R>(F.mesh_shift_prev), ld3<R>(F.mesh_shift_curr), time);
+            R bu, bv;
+            tri_bary(S, h.prim, rm, rd, bu, bv);
+            const uint32_t* ix = &S.idx[(size_t)h.prim * 3];
+            const float* q0 = &S.uvs[(size_t)ix[0] * 2];
+            const float* q1 = &S.uvs[(size_t)ix[1] * 2];
+            const float* q2 = &S.uvs[(size_t)ix[2] * 2];
+            const R w0 = (R(1) - bu) - bv;
+            u = (R(q0[0]) * w0 + R(q1[0]) * bu) + R(q2[0]) * bv;
+            v = (R(q0[1]) * w0 + R(q1[1]) * bu) + R(q2[1]) * bv;
+        }
+    }
+    void apply_pbr(Hit<R>& h, V3<R> ro, V3<R> rd, R time, R cone_width, R* lod_out = nullptr) const {
+        uint32_t tiles[4];
+        R u, v, uv_scale;
+        pbr_uv(h, ro, rd, time, tiles, u, v, uv_scale);
+        const uint32_t n = S.atlas_tiles;
+        if (lod_out) *lod_out = cone_lod(cone_width, rd, h.n, uv_scale);
+        if (!(tiles[0] < n || tiles[1] < n || tiles[2] < n || tiles[3] < n)) return;
+        const R lod = cone_lod(cone_width, rd, h.n, uv_scale);
+        if (tiles[0] < n) {
+            const V4 a = tile_linear(tiles[0], u, v, lod);
+            h.albedo = V3<R>{h.albedo.x * srgb_to_linear(a.x), h.albedo.y * srgb_to_linear(a.y),
+                             h.albedo.z * srgb_to_linear(a.z)};
+        }
+        if (tiles[2] < n) h.rough = sat(h.rough * tile_linear(tiles[2], u, v, lod).x);
+        if (tiles[3] < n) {
+            const R m = tile_linear(tiles[3], u, v, lod).x;
+            if (m > R(0.5) && h.mat == MAT_LAMBERT) h.mat = MAT_METAL;
+        }
+        if (tiles[1] < n) {
+            const V4 t = tile_linear(tiles[1], u, v, lod);
+            h.n = normal_map(h.n, V3<R>{t.x * R(2) - R(1), t.y * R(2) - R(1), t.z * R(2) - R(1)});
+        }
+    }
+
     // One path sample: PathTrace.slang:5803-6914 with the off-path features
     // (water, clouds, PBR, clearcoat, SSS, ReSTIR) disabled; `time` is the
     // sample's shutter time, `lambda` its hero wavelength (< 0: RGB).
@@ -1390,6 +1541,9 @@ template <class R> struct PT {
         bool skip_sky = false, prev_lambert = false;
         R prev_pdf = R(0);
         const bool mis_on = F.mis_enabled != 0u;
+        // ray-cone LOD state (PathTrace.slang:5913-5919)
+        const R cone_spread = (R(2) * R(F.pos_fovtan[3])) / R(std::max(F.height, 1u));
+        R cone_width = R(0);
         for (uint32_t b = 0; b < F.max_bounces; ++b) {
             if (b == 0) c.primary++; else c.extension++;
             Hit<R> h = trace(ro, rd, time);
@@ -1410,6 +1564,10 @@ template <class R> struct PT {
                 break;
             }
             V3<R> hp = ro + rd * h.t;
+            if (!S.atlas.empty()) {
+                cone_width = cone_width + cone_spread * h.t;   // :6162-6171
+                apply_pbr(h, ro, rd, time, cone_width);
+            }
             R ndi = dot(h.n, rd);
             V3<R> nf = (ndi < R(0)) ? h.n : -h.n;
             if (any_pos(h.emission)) {
@@ -1734,6 +1892,15 @@ int orc_scene_create(const orc_scene_desc* d, orc_scene** out) {
         S->sdf_count = d->sdf_cluster_count;
         S->sdf.assign(d->sdf_clusters, d->sdf_clusters + (size_t)S->sdf_count * 112);
     }
+    if (d->pbr_atlas && d->pbr_atlas_width == 384u && d->pbr_atlas_height >= 256u) {
+        S->atlas_w = d->pbr_atlas_width;
+        S->atlas_h = d->pbr_atlas_height;
+        S->atlas_tiles = d->pbr_atlas_height / 256u;
+        const size_t n = (size_t)S->atlas_w * S->atlas_h * 4;
+        S->atlas.resize(n);
+        for (size_t i = 0; i < n; ++i) S->atlas[i] = (float)d->pbr_atlas[i] * (1.0f / 255.0f);
+    }
+    if (d->mesh_uvs && d->vertex_count) S->uvs.assign(d->mesh_uvs, d->mesh_uvs + (size_t)d->vertex_count * 2);
     if (d->moon_rgba && d->moon_width && d->moon_height) {
         S->moon_w = d->moon_width;
         S->moon_h = d->moon_height;
@@ -1973,6 +2140,52 @@ int orc_render_f64_image(orc_scene* s, const orc_frame* f, int threads, double* 
     work();
     for (auto& t : ts) t.join();
     return 0;
+}
+
+int orc_pbr_sample(const orc_scene* s, uint32_t tile, const float* uvl, uint32_t n, float* out) {
+    if (!s || s->atlas.empty() || tile >= s->atlas_tiles) return 1;
+    orc_frame f{};
+    Ctr c;
+    PT<float> pt{*s, f, c};
+    for (uint32_t i = 0; i < n; ++i) {
+        const auto r = pt.tile_linear(tile, uvl[3 * i], uvl[3 * i + 1], uvl[3 * i + 2]);
+        out[4 * i] = r.x; out[4 * i + 1] = r.y; out[4 * i + 2] = r.z; out[4 * i + 3] = r.w;
+    }
+    return 0;
+}
+
+int orc_pbr_surface(const orc_scene* s, const orc_frame* f, const float* rays, uint32_t n, float cone_width, float* out) {
+    if (!s || !f) return 1;
+    Ctr c;
+    PT<float> pt{*s, *f, c};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        const V3<float> ro{r[0], r[1], r[2]}, rd{r[4], r[5], r[6]};
+        Hit<float> h = pt.trace(ro, rd, 1.0f);
+        float* o = out + 16 * i;
+        for (int k = 0; k < 16; ++k) o[k] = 0.0f;
+        if (!h.hit) continue;
+        uint32_t tiles[4];
+        float u, v, scale, lod = 0.0f;
+        pt.pbr_uv(h, ro, rd, 1.0f, tiles, u, v, scale);
+        if (!s->atlas.empty()) pt.apply_pbr(h, ro, rd, 1.0f, cone_width, &lod);
+        const float vals[16] = {u, v, scale, lod, h.albedo.x, h.albedo.y, h.albedo.z, h.rough, (float)h.mat,
+                                h.n.x, h.n.y, h.n.z, h.t, (float)h.kind, 0.0f, 0.0f};
+        for (int k = 0; k < 16; ++k) o[k] = vals[k];
+    }
+    return 0;
+}
+
+void orc_pbr_normal_map(const float* n, const float* ts, uint32_t count, float* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const auto r = PT<float>::normal_map(V3<float>{n[3 * i], n[3 * i + 1], n[3 * i + 2]},
+                                             V3<float>{ts[3 * i], ts[3 * i + 1], ts[3 * i + 2]});
+        out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+    }
+}
+
+void orc_pbr_srgb(const float* c, uint32_t count, float* out) {
+    for (uint32_t i = 0; i < count; ++i) out[i] = PT<float>::srgb_to_linear(c[i]);
 }
 
 double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H, float* om, float* oc) {

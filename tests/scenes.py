@@ -148,3 +148,43 @@ def sdf_box(center, half, pad=1e-3, **kw):
 def sdf_ball(center, r, pad=1e-3, **kw):
     c = np.asarray(center, np.float32)
     return sdf_cluster([sdf_node(SDF_LEAF, SDF_SPHERE, params=(r, 0, 0, 0), center=c)], c - r - pad, c + r + pad, **kw)
+
+
+NO_TILE = 0xFFFFFFFF
+PBR_TILE, PBR_ATLAS_W = 256, 384
+
+
+def set_tiles(p, albedo=NO_TILE, normal=NO_TILE, rough=NO_TILE, metal=NO_TILE):
+    """A primitive's v6: its albedo, normal, roughness and metallic atlas tiles."""
+    p[24:28] = np.array([albedo, normal, rough, metal], np.uint32).view(np.float32)
+    return p
+
+
+def atlas_band(base):
+    """One 256-row tile band of the engine's atlas (384 x 256 RGBA8): the base
+    image on the left, mips 1..8 (2x2 means rounded in byte space) stacked in
+    the right column at rows 256 - 2 * dim (Engine::BuildPbrTileMips)."""
+    base = np.asarray(base, np.uint8).reshape(PBR_TILE, PBR_TILE, 4)
+    band = np.zeros((PBR_TILE, PBR_ATLAS_W, 4), np.uint8)
+    band[:, :PBR_TILE] = base
+    level = base.astype(np.uint32)
+    dim = PBR_TILE
+    while dim > 1:
+        nd = dim // 2
+        s = level[0::2, 0::2] + level[0::2, 1::2] + level[1::2, 0::2] + level[1::2, 1::2]
+        level = (s + 2) >> 2
+        y0 = PBR_TILE - 2 * nd
+        band[y0:y0 + nd, PBR_TILE:PBR_TILE + nd] = level.astype(np.uint8)
+        dim = nd
+    return band
+
+
+def atlas(bases):
+    """The atlas of tile bases (each 256 x 256 x 4 uint8, or a constant RGBA
+    tuple); tile 0 is the engine's reserved white band."""
+    bands = [np.full((PBR_TILE, PBR_ATLAS_W, 4), 255, np.uint8)]
+    for b in bases:
+        if np.asarray(b).size == 4:
+            b = np.broadcast_to(np.asarray(b, np.uint8), (PBR_TILE, PBR_TILE, 4))
+        bands.append(atlas_band(b))
+    return np.concatenate(bands, axis=0)

@@ -279,7 +279,7 @@ def test_adjoint_limits_and_errors(device):
         sc.adjoint(f, d.data_ptr(), g.data_ptr())
     assert e.value.status == dmt.DMT_ERR_LIMIT
     f = sd.frame.copy()
-    f.flags = 4   # no such flag
+    f.flags = 8   # no such flag
     with pytest.raises(dmt.DmtError) as e:
         sc.adjoint(f, d.data_ptr(), g.data_ptr())
     assert e.value.status == dmt.DMT_ERR_INVALID_ARG
