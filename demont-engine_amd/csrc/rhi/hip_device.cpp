@@ -170,11 +170,14 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     // resets the accumulation (the engine resets every frame while a denoiser
     // runs, Engine.cpp:7834)
     need(r.denoiser && r.write_hdr_aux && !F.reset_accum, "denoiser colour input without reset_accum");
-    if (r.tlas_present) {
-        bool tex = false;
-        for (int k = 0; k < 4; ++k) tex |= u32(p, kMeshTexIndices + 4 * k) != kPbrNoTexTile;
-        need(tex, "PBR mesh textures");                                                      // :586-592
+    // the mesh's PBR map tiles (the engine sets them only with a UV buffer at
+    // slot 17, Engine.cpp:9131-9134)
+    bool mesh_tex = false;
+    for (int k = 0; k < 4; ++k) {
+        F.mesh_tex[k] = u32(p, kMeshTexIndices + 4 * k);
+        mesh_tex |= F.mesh_tex[k] != kPbrNoTexTile;
     }
+    if (mesh_tex) F.flags |= DMT_FRAME_MESH_TEXTURES;
     if (!feats.empty()) {
         if (strict)
             return fail("pathtrace push asks for what this backend does not render: " + feats +
@@ -198,6 +201,7 @@ struct Texture {
     float* dev = nullptr;        // w * h float4
     std::vector<float> host;     // last host upload, while host_valid
     bool host_valid = false;
+    std::vector<std::uint8_t> rgba8;   // RGBA8_UNORM: the bytes of the last upload (the PBR atlas)
     std::uint64_t gen = 0;
     std::size_t bytes() const { return (std::size_t)w * h * 16u; }
 };
@@ -206,6 +210,7 @@ struct Accel {
     bool tlas = false;
     std::vector<float> positions;        // BLAS: its mesh; TLAS: its instances flattened
     std::vector<std::uint32_t> indices;
+    std::vector<std::uint32_t> instance_vertices;   // TLAS: the vertex count of each flattened instance
 };
 
 // What the current scene was built from.
@@ -224,6 +229,8 @@ struct SceneKey {
     // the SDF clusters (buffer slot 10)
     std::uint64_t sdf = 0, sdf_gen = 0, sdf_off = 0;
     std::uint32_t sdf_count = 0;
+    // the mesh UVs (buffer slot 17), read while the frame gives the mesh map tiles
+    std::uint64_t uvs = 0, uvs_gen = 0, uvs_off = 0;
     bool operator==(const SceneKey&) const = default;
 };
 
@@ -258,6 +265,7 @@ void append_instance(Accel& t, const Accel& b, const float* m) {
     }
     t.indices.reserve(t.indices.size() + b.indices.size());
     for (std::uint32_t ix : b.indices) t.indices.push_back((std::uint32_t)base + ix);
+    t.instance_vertices.push_back((std::uint32_t)(b.positions.size() / 3u));
 }
 
 }  // namespace
@@ -287,6 +295,9 @@ struct HipDevice::Impl {
     // the moon texture last handed to the scene (dmt_scene_set_moon_map)
     const dmt_scene* moon_scene = nullptr;
     std::uint64_t moon_id = 0, moon_gen = 0;
+    // the PBR atlas last handed to the scene (dmt_scene_set_pbr_atlas)
+    const dmt_scene* atlas_scene = nullptr;
+    std::uint64_t atlas_id = 0, atlas_gen = 0;
 
     void report(const std::string& m) {
         error = m;
@@ -553,6 +564,12 @@ bool HipDevice::WriteTexture(TextureHandle h, const void* src, std::size_t size)
         if (size != channels) return false;
         const auto* b = static_cast<const std::uint8_t*>(src);
         for (std::size_t i = 0; i < channels; ++i) rgba[i] = (float)b[i] * (1.0f / 255.0f);
+        try {
+            t->rgba8.assign(b, b + channels);
+        } catch (const std::bad_alloc&) {
+            I.report("WriteTexture: out of host memory");
+            return false;
+        }
     } else if (t->format == TextureFormat::RGBA16F) {
         // IEEE binary16 per channel, as the path tracer's shader reads an
         // RGBA16F texture (the engine's moon map, Engine.cpp:4626-4645); the
@@ -721,12 +738,27 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
         k.lights_gen = lights->gen;
         k.light_count = P.light_count;
     }
+    const Buffer* uvs = nullptr;
     if (P.tlas_present) {
         k.tlas = cb.accels_[slot::kTlas];
         tlas = accel(k.tlas);
         if (!tlas || !tlas->tlas) {
             report("pathtrace: tlas_present is set but accel slot 2 holds no TLAS");
             return false;
+        }
+        // the mesh UVs, 2 floats per vertex of the mesh each instance draws
+        if ((P.frame.flags & DMT_FRAME_MESH_TEXTURES) && !tlas->indices.empty()) {
+            k.uvs = cb.buffers_[slot::kMeshUvBuffer];
+            k.uvs_off = cb.buffer_offsets_[slot::kMeshUvBuffer];
+            uvs = buffer(k.uvs);
+            std::uint32_t need = 0;
+            for (std::uint32_t nv : tlas->instance_vertices) need = std::max(need, nv);
+            if (!uvs || k.uvs_off > uvs->data.size() || (uvs->data.size() - k.uvs_off) / 8u < need) {
+                report("pathtrace: mesh_tex_indices needs the mesh UVs (" + std::to_string(need) +
+                       " vertices) at buffer slot 17");
+                return false;
+            }
+            k.uvs_gen = uvs->gen;
         }
     }
     if (P.sdf_count) {
@@ -784,24 +816,10 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
     }
 
     // The primitives as the scene takes them (v3, the previous centre, is the
-    // start of the motion-blur shutter); PBR texture tiles in v6 are outside
-    // this backend (strict: refused).
+    // start of the motion-blur shutter; v6 holds the PBR map tiles the atlas
+    // at storage-texture slot 16 serves).
     std::vector<float> pr((std::size_t)P.prim_count * kPrimFloats);
     if (P.prim_count) std::memcpy(pr.data(), prims->data.data() + k.prims_off, pr.size() * 4u);
-    for (std::uint32_t i = 0; i < P.prim_count; ++i) {
-        float* q = &pr[(std::size_t)i * kPrimFloats];
-        bool tex = false;
-        for (int c = 24; c < 28; ++c) tex |= bits(q[c]) != kPbrNoTexTile;
-        if (tex) {
-            if (opt.strict) {
-                report("pathtrace: primitive " + std::to_string(i) +
-                       " has PBR texture tiles (v6), which this backend does not render");
-                return false;
-            }
-            if (ignored.find("PBR primitive textures") == std::string::npos)
-                ignored += ignored.empty() ? "PBR primitive textures" : ", PBR primitive textures";
-        }
-    }
 
     SceneKey only_prims = key;
     only_prims.prims_gen = k.prims_gen;
@@ -832,11 +850,24 @@ bool HipDevice::Impl::ensure_scene(const HipCommandBuffer& cb, const PathtracePu
     d.prim_count = P.prim_count;
     d.lights = lights ? reinterpret_cast<const float*>(lights->data.data() + k.lights_off) : nullptr;
     d.light_count = P.light_count;
+    std::vector<float> uv_flat;
     if (tlas && !tlas->indices.empty()) {
         d.positions = tlas->positions.data();
         d.vertex_count = (std::uint32_t)(tlas->positions.size() / 3u);
         d.indices = tlas->indices.data();
         d.index_count = (std::uint32_t)tlas->indices.size();
+        if (uvs) {
+            // each flattened instance reads the UVs of its mesh's vertices
+            // (the shader indexes mesh_uvs by the mesh's own vertex index)
+            uv_flat.reserve((std::size_t)d.vertex_count * 2u);
+            const auto* src = reinterpret_cast<const std::uint8_t*>(uvs->data.data() + k.uvs_off);
+            for (std::uint32_t nv : tlas->instance_vertices) {
+                const std::size_t at = uv_flat.size();
+                uv_flat.resize(at + (std::size_t)nv * 2u);
+                std::memcpy(uv_flat.data() + at, src, (std::size_t)nv * 8u);
+            }
+            d.mesh_uvs = uv_flat.data();
+        }
     }
     std::vector<float> cdf_copy;
     if (env) {
@@ -987,6 +1018,25 @@ void HipDevice::dispatch_impl(const HipCommandBuffer& cb) {
             I.moon_scene = I.scene;
             I.moon_id = mid;
             I.moon_gen = mgen;
+        }
+    }
+    // the PBR atlas at storage slot 16 (RGBA8, the engine's 384-wide layout);
+    // none bound or none uploaded: untextured
+    {
+        const std::uint64_t aid = cb.textures_[slot::kPbrAtlasTexture];
+        Texture* at = I.texture(aid);
+        const std::uint64_t agen = at ? at->gen : 0u;
+        if (I.atlas_scene != I.scene || I.atlas_id != aid || I.atlas_gen != agen) {
+            if (at && !at->rgba8.empty()) {
+                if (!I.check(dmt_scene_set_pbr_atlas(I.scene, at->rgba8.data(), at->w, at->h),
+                             "dmt_scene_set_pbr_atlas"))
+                    return;
+            } else if (!I.check(dmt_scene_set_pbr_atlas(I.scene, nullptr, 0, 0), "dmt_scene_set_pbr_atlas")) {
+                return;
+            }
+            I.atlas_scene = I.scene;
+            I.atlas_id = aid;
+            I.atlas_gen = agen;
         }
     }
     if (!I.check(dmt_render(I.scene, &P.frame), "dmt_render")) return;

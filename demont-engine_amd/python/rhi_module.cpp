@@ -80,6 +80,7 @@ py::dict decoded(const PathtracePush& P) {
     d["sdf_count"] = P.sdf_count;
     d["sdf_max_iters"] = F.sdf_max_iters;
     d["sdf_epsilon"] = F.sdf_epsilon;
+    d["mesh_tex"] = std::vector<uint32_t>(F.mesh_tex, F.mesh_tex + 4);
     d["curr_view_proj"] = std::vector<float>(P.curr_view_proj, P.curr_view_proj + 16);
     d["prev_view_proj"] = std::vector<float>(P.prev_view_proj, P.prev_view_proj + 16);
     d["ignored"] = P.ignored;

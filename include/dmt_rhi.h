@@ -133,6 +133,7 @@ constexpr std::uint32_t kCloudTransTexture = 10;
 constexpr std::uint32_t kSpecularAlbedoTexture = 11;
 constexpr std::uint32_t kRoughnessTexture = 12;
 constexpr std::uint32_t kSpecularDistanceTexture = 13;
+constexpr std::uint32_t kPbrAtlasTexture = 16; /* BindStorageTexture(16, pbr_atlas), RGBA8 */
 constexpr std::uint32_t kTlas = 2;             /* BindAccelStruct(2, scene TLAS)         */
 constexpr std::uint32_t kPrimBuffer = 3;       /* BindBuffer(3, primitives)              */
 constexpr std::uint32_t kEnvMarginalBuffer = 4;      /* BindBuffer(4, env_marginal_cdf)    */
@@ -140,6 +141,7 @@ constexpr std::uint32_t kEnvConditionalBuffer = 5;   /* BindBuffer(5, env_condit
 constexpr std::uint32_t kExposureBuffer = 6;   /* BindBuffer(6, exposure_state)          */
 constexpr std::uint32_t kSdfBuffer = 10;       /* BindBuffer(10, sdf_clusters)           */
 constexpr std::uint32_t kLightBuffer = 12;     /* BindBuffer(12, lights)                 */
+constexpr std::uint32_t kMeshUvBuffer = 17;    /* BindBuffer(17, mesh_uvs)               */
 constexpr std::uint32_t kBufferSlots = 24;     /* BindState sizes (SoftwareDevice.h:112-124) */
 constexpr std::uint32_t kTextureSlots = 20;
 constexpr std::uint32_t kAccelSlots = 4;
@@ -200,14 +202,16 @@ struct PathtracePush {
  * tracer's no-op, SoftwareTracer.cpp:686-690) and, when strict, for a push that
  * turns on a feature this backend does not render: a procedural / Hosek sky
  * (sun_and_mode.w 2 or 3), participating media, the SDF iteration heat map, ReSTIR,
- * material overrides, smoke, the SIGMA shadow-visibility output, a
- * per-frame denoiser colour input without reset_accum, or PBR mesh textures.
+ * material overrides, smoke, the SIGMA shadow-visibility output, or a
+ * per-frame denoiser colour input without reset_accum.
  * Lenient decoding renders without them and lists them in out->ignored.  The
  * gradient sky's sun and moon (sun_and_mode, sun_extra, sun_extra2,
  * moon_dir_phase, moon_extra), the HDRI cluster lights (hdri_lights_*) and,
  * while a denoiser runs, the Halton jitter are decoded into the frame, and
  * the SDF clusters' count and sphere-trace parameters (sdf_params.xy,
- * sdf_params_f.x) into sdf_count and the frame.
+ * sdf_params_f.x) into sdf_count and the frame, and the mesh's PBR map tiles
+ * (mesh_tex_indices) into frame.mesh_tex with DMT_FRAME_MESH_TEXTURES when
+ * any is set.
  * Fields past the pushed size read as zero. */
 bool DecodePathtracePush(const void* data, std::size_t size, bool strict, PathtracePush* out, std::string* error);
 

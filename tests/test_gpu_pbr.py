@@ -179,3 +179,58 @@ def test_textured_frames_are_refused_by_the_batch_and_the_adjoint(device):
     b = dmt.Batch(device, [sc], [f])
     b.close()
     sc.close()
+
+
+def test_engine_sequence_with_the_atlas_and_mesh_uvs(device):
+    """The engine's binds: the atlas as an RGBA8 texture at storage slot 16
+    (Engine.cpp:3984-4020, :7216), the mesh UVs at buffer slot 17 (:7137-7150),
+    the prims' tiles in v6 and mesh_tex_indices in the push (:9131-9134); the
+    accumulation equals the C ABI's render of the same scene and frames."""
+    import _dmt_rhi as rhi
+    from rhi_engine import NO_TEX_TILE, RGBA8_UNORM, SAMPLED, EngineSim
+
+    base, f = _textured_spheres(seed=16, w=40, h=30)
+    mesh = dmt.generate(1, seed=6, subdivisions=2, random_tris=120)
+    pos = (mesh.positions * 0.6 + np.array([0.0, -1.0, 1.5], np.float32)).astype(np.float32)
+    uvs = RNG.uniform(-1.0, 2.0, size=(pos.shape[0], 2)).astype(np.float32)
+    sd = dmt.SceneData(prims=base.prims, lights=base.lights, positions=pos, indices=mesh.indices, env=base.env,
+                       env_intensity=base.env_intensity, frame=f.copy(), atlas=base.atlas, uvs=uvs)
+    f.flags |= dmt.FRAME_MESH_TEXTURES
+    f.mesh_tex[:] = (2, 4, 3, NO_TEX_TILE)
+    sc = dmt.Scene(device, sd)
+    sc.render_frames(f, 2)
+    want = sc.readback(f.width, f.height)
+    sc.close()
+
+    dev = rhi.HipDevice(strict=True)
+    eng = EngineSim(dev, sd)
+    at = dev.create_texture(base.atlas.shape[1], base.atlas.shape[0], RGBA8_UNORM, SAMPLED)
+    assert dev.write_texture(at, np.ascontiguousarray(base.atlas))
+    ub = dev.create_buffer(uvs.nbytes)
+    dev.write_buffer(ub, uvs)
+    ov = {"mesh_tex_indices": ("4I", tuple(int(t) for t in f.mesh_tex))}
+    eng.frames(f, 2, textures={16: at}, buffers={17: ub}, overrides=ov)
+    assert dev.last_error() == "", dev.last_error()
+    assert np.array_equal(eng.accum_image(), want)
+    # mesh tiles with a UV buffer short of the mesh's vertices are refused;
+    # without the atlas the frame renders untextured, as the C ABI renders the
+    # scene without one
+    short = dev.create_buffer(16)
+    dev.write_buffer(short, np.zeros(4, np.float32))
+    eng.frames(f, 1, textures={16: at}, buffers={17: short}, overrides=ov)
+    assert "buffer slot 17" in dev.last_error()
+    plain = dmt.Scene(device, dmt.SceneData(prims=base.prims, lights=base.lights, positions=pos,
+                                            indices=mesh.indices, env=base.env, env_intensity=base.env_intensity,
+                                            frame=f.copy()))
+    g = f.copy()
+    g.flags &= ~dmt.FRAME_MESH_TEXTURES
+    plain.render_frames(g, 2)
+    want_plain = plain.readback(f.width, f.height)
+    plain.close()
+    dev2 = rhi.HipDevice(strict=True)
+    eng2 = EngineSim(dev2, sd)
+    eng2.frames(f, 2)
+    assert dev2.last_error() == "", dev2.last_error()
+    assert np.array_equal(eng2.accum_image(), want_plain)
+    dev.close()
+    dev2.close()

@@ -79,7 +79,6 @@ GATES = [
     ("mat_override_count", "4I", (1, 0, 0, 0), "material overrides"),
     ("smoke_params", "4f", (1.0, 0.0, 0.0, 0.0), "smoke volumes"),
     ("write_shadow_vis", "I", 1, "SIGMA shadow visibility"),
-    ("mesh_tex_indices", "4I", (3, NO_TEX_TILE, NO_TEX_TILE, NO_TEX_TILE), "PBR mesh textures"),
 ]
 
 
@@ -283,3 +282,18 @@ def test_reference_side_binding_compiles_against_the_reference_headers(tmp_path)
     subprocess.run(["g++", "-shared", "-o", so, obj, "-L", libdir, "-ldmt_rhi", "-Wl,--no-undefined",
                     "-Wl,-rpath," + libdir], check=True)
     assert "pt::rhi::CreateHipDevice(" in _nm_demangled(so)
+
+
+def test_decode_reads_the_mesh_map_tiles():
+    """mesh_tex_indices (1152, Engine.cpp:9131-9134) into frame.mesh_tex, with
+    DMT_FRAME_MESH_TEXTURES when any channel names a tile."""
+    f = _frame()
+    push = pack_ptpush(f, prim_count=1, light_count=1, env_present=True, tlas_present=True,
+                       overrides={"mesh_tex_indices": ("4I", (3, NO_TEX_TILE, 5, 0))})
+    d = rhi.decode_pathtrace_push(push, strict=True)
+    assert d["mesh_tex"] == [3, NO_TEX_TILE, 5, 0]
+    assert d["flags"] & dmt.FRAME_MESH_TEXTURES
+    assert d["ignored"] == ""
+    d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=True, tlas_present=True),
+                                  strict=True)
+    assert d["mesh_tex"] == [NO_TEX_TILE] * 4 and not (d["flags"] & dmt.FRAME_MESH_TEXTURES)
