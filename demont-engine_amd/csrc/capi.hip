@@ -157,6 +157,7 @@ struct dmt_scene {
     float4* moon = nullptr;           // moon texture (dmt_scene_set_moon_map)
     uchar4* atlas = nullptr;          // PBR atlas (dmt_scene_set_pbr_atlas)
     bool prims_textured = false;      // an analytic primitive has a tile the atlas holds
+    bool has_water = false;           // a primitive or cluster of material 3
 };
 
 namespace {
@@ -361,6 +362,14 @@ dmt_status check_frame(const dmt_frame_params* f) {
     if (f->motion_blur > 1u || f->spectral > 1u || f->mesh_shift_active > 1u)
         return fail(DMT_ERR_INVALID_ARG, "motion_blur, spectral and mesh_shift_active are 0 or 1");
     if (f->reserved) return fail(DMT_ERR_INVALID_ARG, "reserved must be 0");
+    const float water[11] = {f->water_absorb[0], f->water_absorb[1], f->water_absorb[2], f->water_ior, f->wave[0],
+                             f->wave[1], f->wave[2], f->wave[3], f->deep_color[0], f->deep_color[1], f->deep_color[2]};
+    for (float v : water)
+        if (!std::isfinite(v)) return fail(DMT_ERR_INVALID_ARG, "water parameters must be finite");
+    for (int a = 0; a < 3; ++a)
+        if (f->water_absorb[a] < 0.0f || f->deep_color[a] < 0.0f)
+            return fail(DMT_ERR_INVALID_ARG, "water absorption and deep colour must be >= 0");
+    if (f->camera_in_water > 1u) return fail(DMT_ERR_INVALID_ARG, "camera_in_water is 0 or 1");
     return DMT_OK;
 }
 
@@ -396,6 +405,10 @@ bool sky_lights(const dmt_scene* s, const dmt_frame_params* f) {
     return up(f->sun) || up(f->moon);
 }
 
+// A frame with water: the scene has water surfaces, or the camera starts in
+// the water.
+bool water_active(const dmt_scene* s, const dmt_frame_params* f) { return s->has_water || f->camera_in_water; }
+
 // The mesh's map tiles a frame asks for (none without DMT_FRAME_MESH_TEXTURES).
 void mesh_tex_of(const dmt_frame_params* f, uint32_t out[4]) {
     for (int k = 0; k < 4; ++k) out[k] = (f->flags & DMT_FRAME_MESH_TEXTURES) ? f->mesh_tex[k] : kPbrNoTile;
@@ -416,20 +429,21 @@ bool pbr_active(const dmt_scene* s, const dmt_frame_params* f) {
 
 // The kernel variant a frame runs: kExtSky with its sky lights (and the four
 // shadow-record kinds), kExtSdf with the scene's SDF clusters, kExtPbr with
-// its material maps.
+// its material maps, kExtWater with water.
 uint32_t ext_bits(const dmt_scene* s, const dmt_frame_params* f) {
     return (sky_lights(s, f) ? kExtSky : 0u) | (s->view.sdf_count > 0u ? kExtSdf : 0u) |
-           (pbr_active(s, f) ? kExtPbr : 0u);
+           (pbr_active(s, f) ? kExtPbr : 0u) | (water_active(s, f) ? kExtWater : 0u);
 }
 
 // A frame that draws other numbers than the jitter per sample (lens point,
 // shutter time, hero wavelength), takes a fixed jitter, translates the mesh,
-// has sky lights or samples the PBR atlas, and any frame of a scene with SDF
-// clusters, renders through k_tail<COUNT, true>: its traversal and shading
-// are the ones that see them.
+// has sky lights, samples the PBR atlas or has water, and any frame of a scene
+// with SDF clusters, renders through k_tail<COUNT, true>: its traversal and
+// shading are the ones that see them.
 bool uses_features(const dmt_scene* s, const dmt_frame_params* f) {
     return f->dof[0] > 0.0f || f->motion_blur || f->spectral || f->mesh_shift_active ||
-           (f->flags & DMT_FRAME_FIXED_JITTER) || sky_lights(s, f) || s->view.sdf_count > 0u || pbr_active(s, f);
+           (f->flags & DMT_FRAME_FIXED_JITTER) || sky_lights(s, f) || s->view.sdf_count > 0u || pbr_active(s, f) ||
+           water_active(s, f);
 }
 
 FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
@@ -438,6 +452,7 @@ FeatureArgs feature_args(const dmt_scene* s, const dmt_frame_params* f) {
     X.jit_x = f->jitter[0];
     X.jit_y = f->jitter[1];
     X.ext = ext_bits(s, f);
+    X.cam_in_water = f->camera_in_water ? 1u : 0u;
     if (sky_lights(s, f)) {
         SkyArgs& K = X.sky_args;
         K.sun = f3{f->sun[0], f->sun[1], f->sun[2]};
@@ -479,6 +494,10 @@ SceneView frame_view(const dmt_scene* s, const dmt_frame_params* f) {
     V.mesh_prev = f3{f->mesh_shift_prev[0], f->mesh_shift_prev[1], f->mesh_shift_prev[2]};
     V.mesh_curr = f3{f->mesh_shift_curr[0], f->mesh_shift_curr[1], f->mesh_shift_curr[2]};
     mesh_tex_of(f, V.mesh_tex);
+    V.water_absorb = f3{f->water_absorb[0], f->water_absorb[1], f->water_absorb[2]};
+    V.water_ior = f->water_ior;
+    V.wave = make_float4(f->wave[0], f->wave[1], f->wave[2], f->wave[3]);
+    V.deep_color = f3{f->deep_color[0], f->deep_color[1], f->deep_color[2]};
     // the ray cone's spread: one pixel's angle, 2 tan(fov_y / 2) / height
     // (PathTrace.slang:5918)
     V.cone_spread = (2.0f * f->pos_fovtan[3]) / (float)std::max(f->height, 1u);
@@ -746,6 +765,12 @@ dmt_status dmt_scene_create(dmt_device* dev, const dmt_scene_desc* desc, const d
         if (e != hipSuccess) return bail(hip_fail(e, "refit tables"));
         s->mesh_vertices = desc->vertex_count;
         s->mesh_tris = P.tri_count;
+    }
+    for (uint32_t i = 0; i < P.prim_count && !s->has_water; ++i) s->has_water = P.prims[(size_t)i * 28u + 9u] == 3.0f;
+    for (uint32_t i = 0; i < P.sdf_count && !s->has_water; ++i) {
+        uint32_t m;
+        std::memcpy(&m, &P.sdf[(size_t)i * kSdfClusterFloats + 7u], 4);
+        s->has_water = m == 3u;
     }
     if (desc->mesh_uvs && P.tri_count) {
         // per-vertex UVs, read through the vertex indices of each leaf slot
@@ -1313,6 +1338,7 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
         return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate scenes with SDF clusters");
     if (pbr_active(s, f))
         return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate PBR texture maps");
+    if (water_active(s, f)) return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate water");
     const uint32_t prims = s->view.prim_count;
     if (prims && (!d_image_grad || !d_param_grad)) return fail(DMT_ERR_INVALID_ARG, "null device pointer");
     if (prims == 0) return DMT_OK;
@@ -1560,6 +1586,8 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         if ((st = check_frame_for(list[i], &frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
         if (pbr_active(list[i], &frames[i]))
             return fail(DMT_ERR_UNSUPPORTED, "frame " + std::to_string(i) + ": the batched form does not sample PBR texture maps");
+        if (water_active(list[i], &frames[i]))
+            return fail(DMT_ERR_UNSUPPORTED, "frame " + std::to_string(i) + ": the batched form does not render water");
     }
     {
         std::vector<dmt_scene*> sorted = list;

@@ -61,8 +61,8 @@ std::string prepare_scene(const float* prims, uint32_t prim_count, const float* 
         if (!finite_all(p, 24)) return "prim " + std::to_string(i) + " has a non-finite value";
         const float type = p[8], mat = p[9];
         if (type != 0.0f && type != 1.0f) return "prim " + std::to_string(i) + ": type must be 0 (sphere) or 1 (plane)";
-        if (mat != 0.0f && mat != 1.0f && mat != 2.0f)
-            return "prim " + std::to_string(i) + ": material must be 0, 1 or 2";
+        if (mat != 0.0f && mat != 1.0f && mat != 2.0f && mat != 3.0f)
+            return "prim " + std::to_string(i) + ": material must be 0, 1, 2 or 3 (water)";
         if (type == 1.0f) {
             float n[3];
             quat_rotate(p + 20, p, n);
@@ -210,7 +210,7 @@ std::string prepare_sdf(const float* clusters, uint32_t count, PreparedScene& S,
         const std::string tag = "SDF cluster " + std::to_string(i);
         const uint32_t nodes = bits(c[3]), mat = bits(c[7]);
         if (nodes > kSdfNodeLimit) return tag + ": node_count must be <= 8";
-        if (mat > 2u) return tag + ": material must be 0, 1 or 2";
+        if (mat > 3u) return tag + ": material must be 0, 1, 2 or 3 (water)";
         const float hdr[] = {c[0], c[1], c[2], c[4], c[5], c[6], c[8], c[9], c[10], c[11], c[12]};
         if (!finite_all(hdr, 11)) return tag + " has a non-finite box or material value";
         BoundsItem& b = items[i];

@@ -299,6 +299,13 @@ struct SceneView {
     const uint32_t* mesh_tri_vtx;
     uint32_t mesh_tex[4];
     float cone_spread;
+    // water, per frame (water.h): Beer's-law absorption rgb (1/m), the
+    // surfaces' default index, the wave normal's scale, amplitude, speed and
+    // time, the deep-water tint
+    f3 water_absorb;
+    float water_ior;
+    float4 wave;
+    f3 deep_color;
 };
 
 // Traversal stack: `lds_depth` entries in LDS (thread-interleaved), the rest

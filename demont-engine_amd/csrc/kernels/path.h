@@ -12,6 +12,7 @@
 #pragma once
 #include "shade.h"
 #include "pbr.h"
+#include "water.h"
 
 namespace dmt {
 
@@ -182,6 +183,7 @@ template <class T> struct PathVertexStateT {
     float time = 1.0f;      // shutter time of the sample (1: no motion blur)
     float lambda = -1.0f;   // hero wavelength in nm (< 0: RGB)
     float cone = 0.0f;      // ray-cone width at the current origin (PBR frames)
+    bool in_water = false;  // the ray travels in the water medium (water frames)
 };
 
 using PathVertexState = PathVertexStateT<float>;
@@ -237,7 +239,7 @@ __device__ __forceinline__ V3<T> miss_sky(const SceneView& S, const SkyArgs* K, 
 // NEE of the env branch and the cluster test of the MIS miss (:6493-6518,
 // :6074); SKY frames are float only.  SDF: h may be an SDF cluster's hit
 // (the feature path's traversal).
-template <bool SKY = false, bool SDF = false, bool PBR = false, class T, class Sink>
+template <bool SKY = false, bool SDF = false, bool PBR = false, bool WATER = false, class T, class Sink>
 __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs& F, uint32_t b, V3<T> o, V3<T> d,
                                              HitRec h, PathVertexStateT<T>& P, VertexResultT<T>& R, Sink& sink,
                                              const SkyArgs* K = nullptr) {
@@ -245,6 +247,20 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
     R.alive = false;
     R.no = to<T>(f3{0.0f, 0.0f, 0.0f});
     R.nd = to<T>(f3{0.0f, 1.0f, 0.0f});
+    if constexpr (WATER) {
+        static_assert(sizeof(T) == sizeof(float), "water frames are float only");
+        if (P.in_water) {
+            if (h.code != kMiss) {
+                // Beer's law over the segment just traced (PathTrace.slang:6028-6031)
+                P.T_ = P.T_ * water_absorb(S, h.t);
+            } else {
+                // escaped under water: the deep-water tint (:6046-6052)
+                const f3 v = P.T_ * S.deep_color;
+                P.L = P.L + ((b == 0u) ? v : clamp_indirect(F, v));
+                return;
+            }
+        }
+    }
     if (h.code == kMiss) {
         if (mis_on && P.prev_lambert) {
             // cluster directions are left to the clusters' own NEE (:6074)
@@ -421,6 +437,39 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
             P.prev_lambert = false;
         }
         R.alive = true;
+    } else if (WATER && sf.mat == kMatWater) {
+        if constexpr (WATER) {
+            // the water surface (:6768-6906 without the FFT ocean and its
+            // foam): the wave normal, Schlick with F0 0.02; reflection (or
+            // total internal reflection) keeps the medium, refraction
+            // crosses it; no NEE, the throughput unchanged
+            const float wior = water_ior_of(S, sf.ior);
+            const f3 sn = water_normal(S, hp, sf.n);
+            const float ndi_s = dot(sn, d);
+            const f3 nf_s = (ndi_s < 0.0f) ? sn : -sn;
+            const float ia = (ndi_s < 0.0f) ? 1.0f : wior;
+            const float ib = (ndi_s < 0.0f) ? wior : 1.0f;
+            const float eta = ia / ib;
+            const float ci = fabsf(dot(nf_s, -d));
+            const float r0 = 0.02f;
+            const float m = 1.0f - ci;
+            const float fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
+            const f3 refl = reflect(d, nf_s);
+            const f3 refr = refract(d, nf_s, eta);
+            bool take_refl = length(refr) < 1e-3f;
+            if (!take_refl) take_refl = randf(P.seed) < fr;
+            if (take_refl) {
+                R.nd = refl;
+                R.no = hp + nf_s * 1e-3f;
+                P.skip_sky = false;
+                P.prev_lambert = false;
+            } else {
+                P.in_water = ndi_s < 0.0f;
+                R.nd = normalize(refr);
+                R.no = hp - nf_s * 1e-3f;
+            }
+            R.alive = true;
+        }
     } else if (sf.mat == 1u) {
         sink.on_albedo_vertex(sf);
         V3<T> r = reflect(d, nf);
@@ -469,24 +518,36 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const FrameArgs
 // hit h of the ray (o, d): returns true when t is final (a miss or a hit
 // beyond t_max keeps it, an opaque hit, the step limit or total internal
 // reflection zero it); otherwise tints t by (1 - F) * albedo and moves the
-// ray through the dielectric surface.  SDF as in shade_vertex.
-template <bool SDF = false, class T>
+// ray through the dielectric surface.  SDF as in shade_vertex.  WATER (water
+// frames): in_water is the query's medium -- the segment in the water is
+// absorbed first, and a water surface transmits (1 - F) with F0 0.02 and
+// flips the medium.
+template <bool SDF = false, bool WATER = false, class T>
 __device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArgs& F, HitRec h, uint32_t i, V3<T>& o,
-                                              V3<T>& d, T& t_max, V3<T>& t, float time = 1.0f) {
+                                              V3<T>& d, T& t_max, V3<T>& t, float time, bool& in_water) {
+    if constexpr (WATER) {
+        // Beer's law along the part of this segment in the water (:5121-5124)
+        if (in_water) {
+            const float seg = (h.code != kMiss) ? fminf(h.t, primal(t_max)) : primal(t_max);
+            t = t * to<T>(water_absorb(S, fmaxf(seg, 0.0f)));
+        }
+    }
     if (h.code == kMiss || h.t >= primal(t_max)) return true;
     const SurfaceT<T> sf = surface_of<SDF>(S, o, d, h, time);
-    if (sf.mat != 2u || i == F.refract_bounces) {
+    const bool water = WATER && sf.mat == kMatWater;
+    if ((sf.mat != 2u && !water) || i == F.refract_bounces) {
         t = to<T>(f3{0.0f, 0.0f, 0.0f});
         return true;
     }
+    const float ior = water ? water_ior_of(S, sf.ior) : sf.ior;
     T ndi = dot(sf.n, d);
     V3<T> nf = (ndi < 0.0f) ? sf.n : -sf.n;
-    float ia = (ndi < 0.0f) ? 1.0f : sf.ior;
-    float ib = (ndi < 0.0f) ? sf.ior : 1.0f;
+    float ia = (ndi < 0.0f) ? 1.0f : ior;
+    float ib = (ndi < 0.0f) ? ior : 1.0f;
     float eta = ia / ib;
     T ci = fabs_(dot(nf, -d));
     float q = (ia - ib) / (ia + ib);
-    float r0 = q * q;
+    float r0 = water ? 0.02f : q * q;
     T m = 1.0f - ci;
     T fr = r0 + ((((1.0f - r0) * m) * m * m) * m) * m;
     V3<T> rf = refract(d, nf, eta);
@@ -494,12 +555,25 @@ __device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArg
         t = to<T>(f3{0.0f, 0.0f, 0.0f});
         return true;
     }
-    t = t * (to<T>(sf.albedo) * (1.0f - fr));
+    if (water) {
+        // water: the Fresnel-transmitted fraction, and the medium flips
+        t = t * (1.0f - fr);
+        in_water = !in_water;
+    } else {
+        t = t * (to<T>(sf.albedo) * (1.0f - fr));
+    }
     T adv = sf.t + 1e-3f;
     o = o + d * adv;
     d = normalize(rf);
     t_max = fmax_(t_max - adv, T(0.0f));
     return false;
+}
+
+template <bool SDF = false, class T>
+__device__ __forceinline__ bool transmit_step(const SceneView& S, const FrameArgs& F, HitRec h, uint32_t i, V3<T>& o,
+                                              V3<T>& d, T& t_max, V3<T>& t, float time = 1.0f) {
+    bool dry = false;
+    return transmit_step<SDF, false>(S, F, h, i, o, d, t_max, t, time, dry);
 }
 
 // transmittance: closest hits along the ray, refracting through dielectrics,

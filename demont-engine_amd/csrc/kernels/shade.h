@@ -44,9 +44,10 @@ struct SkyArgs {
 };
 
 // Extensions a frame-completion kernel variant carries: the frame's sky
-// lights (four shadow-record kinds), the scene's SDF clusters and its PBR
-// material maps.
-constexpr uint32_t kExtSky = 1u, kExtSdf = 2u, kExtPbr = 4u;
+// lights (four shadow-record kinds), the scene's SDF clusters, its PBR
+// material maps and its water.
+constexpr uint32_t kExtSky = 1u, kExtSdf = 2u, kExtPbr = 4u, kExtWater = 8u;
+constexpr uint32_t kExtVariants = 16u;
 
 // Camera-sample features and sky lights of a frame (all off: the plain
 // camera ray and no sky lights).
@@ -59,7 +60,7 @@ struct FeatureArgs {
     float jit_x, jit_y;
     uint32_t ext;                          // kExtSky: sky lights (SkyArgs below); kExtSdf: the
                                            // scene's SDF clusters -- the kernel variant with their code
-    uint32_t pad;
+    uint32_t cam_in_water;                 // camera paths start in the water medium (water_state.x)
     SkyArgs sky_args;
 };
 

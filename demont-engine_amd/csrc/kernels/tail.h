@@ -105,13 +105,16 @@ struct TailCold {
     __device__ __forceinline__ void set3(uint32_t f, f3 v) { set(f, v.x); set(f + 1, v.y); set(f + 2, v.z); }
 };
 
+template <bool WATER = false>
 __device__ __forceinline__ void tail_start_shadow(const ShadowQueue& q, uint32_t k, TailLane& l, TailCold& c) {
     const float4 o4 = q.o[l.slot], d4 = q.d[l.slot];
     l.o = xyz(o4);
     l.d = xyz(d4);
     c.set(TailCold::kTmax, o4.w);
     c.set3(TailCold::kTr, f3{1.0f, 1.0f, 1.0f});
-    c.setu(TailCold::kStep, 0u);
+    // the query starts in the path's medium (a Lambert vertex, whose NEE
+    // records these are, does not change it): bit 31 of the step field
+    c.setu(TailCold::kStep, (WATER && l.P.in_water) ? 0x80000000u : 0u);
     l.mode = k + 1u;
 }
 
@@ -122,13 +125,14 @@ __device__ __forceinline__ void tail_start_shadow(const ShadowQueue& q, uint32_t
 // pixel's frame value.  FEAT: the frame's camera-sample features and sky
 // lights X are on; the samples draw them (camera_sample) and add their
 // radiance weighted by the hero wavelength's response.  SKY: the frame has
-// sky lights, whose records (2 and 3) exist only then.
-template <bool FEAT, bool SKY = false>
+// sky lights, whose records (2 and 3) exist only then.  WATER: the frame has
+// water (the paths' medium).
+template <bool FEAT, bool SKY = false, bool WATER = false>
 __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureArgs& X, const ShadowQueue& sl,
                                              const ShadowQueue& se, const ShadowQueue& s2, const ShadowQueue& s3,
                                              const PathState& ps, TailLane& l, TailCold& c) {
     if (l.pending & 1u) {
-        tail_start_shadow(sl, 0u, l, c);
+        tail_start_shadow<WATER>(sl, 0u, l, c);
         return;
     }
     if (!SKY) {
@@ -138,7 +142,7 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureAr
                 l.P.L = l.P.L + xyz(se.a[l.slot]);
                 l.pending &= ~2u;
             } else {
-                tail_start_shadow(se, 1u, l, c);
+                tail_start_shadow<WATER>(se, 1u, l, c);
                 return;
             }
         }
@@ -147,16 +151,16 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureAr
         if (l.pending & 2u) {
             ambient = se.b[l.slot].w < 0.0f;   // the no-env ambient term: after the sun and moon
             if (!ambient) {
-                tail_start_shadow(se, 1u, l, c);
+                tail_start_shadow<WATER>(se, 1u, l, c);
                 return;
             }
         }
         if (l.pending & 4u) {
-            tail_start_shadow(s2, 2u, l, c);
+            tail_start_shadow<WATER>(s2, 2u, l, c);
             return;
         }
         if (l.pending & 8u) {
-            tail_start_shadow(s3, 3u, l, c);
+            tail_start_shadow<WATER>(s3, 3u, l, c);
             return;
         }
         if (ambient) {
@@ -196,6 +200,7 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureAr
         l.P.skip_sky = false;
         l.P.prev_lambert = false;
         l.P.cone = 0.0f;
+        if (WATER) l.P.in_water = X.cam_in_water != 0u;
         l.b = 0u;
         l.mode = 0u;
         return;

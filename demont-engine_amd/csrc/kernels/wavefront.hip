@@ -269,6 +269,7 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                     l.slot = l.pix;   // the pixel's shadow-record rows
                     l.P = PathVertexState{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false,
                                           pixel_seed(F, x, y, F.frame_index)};
+                    if (EXT & kExtWater) l.P.in_water = X.cam_in_water != 0u;
                     camera_sample(F, X, x, y, 0u, l.P.seed, l.o, l.d, l.P.time, l.P.lambda);
                     l.b = 0u;
                     c.setu(tail::TailCold::kS, 0u);
@@ -319,21 +320,25 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                 else n_ext++;
                 SlotSink sink{{}, sl, se, s2, s3, l.slot, l.pix, 0u};
                 VertexResult R;
-                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u, (EXT & kExtPbr) != 0u>(
-                    S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
+                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u, (EXT & kExtPbr) != 0u,
+                             (EXT & kExtWater) != 0u>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.emitted | (cont ? 0u : tail::kEnded);
                 if (cont) {
                     c.set3(tail::TailCold::kNo, R.no);
                     c.set3(tail::TailCold::kNd, R.nd);
                 }
-                tail::tail_advance<FEAT, (EXT & kExtSky) != 0u>(F, X, sl, se, s2, s3, ps, l, c);
+                tail::tail_advance<FEAT, (EXT & kExtSky) != 0u, (EXT & kExtWater) != 0u>(F, X, sl, se, s2, s3, ps, l,
+                                                                                                    c);
             } else {
                 n_shadow++;
                 float t_max = c.get(tail::TailCold::kTmax);
                 f3 tr = c.get3(tail::TailCold::kTr);
-                const uint32_t step = c.getu(tail::TailCold::kStep);
-                if (transmit_step<(EXT & kExtSdf) != 0u>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
+                const uint32_t step_w = c.getu(tail::TailCold::kStep);   // step | medium << 31
+                const uint32_t step = step_w & 0x7FFFFFFFu;
+                bool wet = (step_w >> 31) != 0u;
+                if (transmit_step<(EXT & kExtSdf) != 0u, (EXT & kExtWater) != 0u>(S, F, h, step, l.o, l.d, t_max, tr,
+                                                                                   l.P.time, wet)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         // fields picked one by one: a selected reference to one of
@@ -352,11 +357,12 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
                                                      k == 1u ? se.w[l.slot] : 1.0f, (tag & kClampBit) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    tail::tail_advance<FEAT, (EXT & kExtSky) != 0u>(F, X, sl, se, s2, s3, ps, l, c);
+                    tail::tail_advance<FEAT, (EXT & kExtSky) != 0u, (EXT & kExtWater) != 0u>(F, X, sl, se, s2, s3, ps, l,
+                                                                                                    c);
                 } else {   // transmit_step ends every query by its step refract_bounces
                     c.set(tail::TailCold::kTmax, t_max);
                     c.set3(tail::TailCold::kTr, tr);
-                    c.setu(tail::TailCold::kStep, step + 1u);
+                    c.setu(tail::TailCold::kStep, (step + 1u) | (wet ? 0x80000000u : 0u));
                 }
             }
             if (l.active) {
@@ -643,6 +649,16 @@ static hipError_t iteration(const SceneView& S, const FrameArgs& F, Wavefront& W
     return hipGetLastError();
 }
 
+// The feature variant of k_tail for the frame's extension bits (one
+// instantiation per combination).
+template <bool COUNT, uint32_t E = 0u, class Go>
+void launch_tail_variant(uint32_t ext, Go&& go) {
+    if constexpr (E < kExtVariants) {
+        if (ext == E) go(k_tail<COUNT, true, E>);
+        else launch_tail_variant<COUNT, E + 1u>(ext, go);
+    }
+}
+
 hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, float4* accum, hipStream_t stream,
                         LaunchOptions& o) {
     const uint32_t npix = F.width * F.height;
@@ -664,24 +680,8 @@ hipError_t launch_frame(const SceneView& S, const FrameArgs& F, Wavefront& W, fl
             hipLaunchKernelGGL(kernel, dim3(tg), dim3(kBlock), 0, stream, S, F, X, 0u, W.rq[0], W.counters, W.ps,
                                sq[0], sq[1], sq[2], sq[3], W.spill, W.spill_depth, W.stats);
         };
-        switch ((o.counting ? 8u : 0u) + (X.ext & 7u)) {
-            case 0u: go(k_tail<false, true, 0u>); break;
-            case 1u: go(k_tail<false, true, kExtSky>); break;
-            case 2u: go(k_tail<false, true, kExtSdf>); break;
-            case 3u: go(k_tail<false, true, kExtSky | kExtSdf>); break;
-            case 4u: go(k_tail<false, true, kExtPbr>); break;
-            case 5u: go(k_tail<false, true, kExtPbr | kExtSky>); break;
-            case 6u: go(k_tail<false, true, kExtPbr | kExtSdf>); break;
-            case 7u: go(k_tail<false, true, kExtPbr | kExtSky | kExtSdf>); break;
-            case 8u: go(k_tail<true, true, 0u>); break;
-            case 9u: go(k_tail<true, true, kExtSky>); break;
-            case 10u: go(k_tail<true, true, kExtSdf>); break;
-            case 11u: go(k_tail<true, true, kExtSky | kExtSdf>); break;
-            case 12u: go(k_tail<true, true, kExtPbr>); break;
-            case 13u: go(k_tail<true, true, kExtPbr | kExtSky>); break;
-            case 14u: go(k_tail<true, true, kExtPbr | kExtSdf>); break;
-            default: go(k_tail<true, true, kExtPbr | kExtSky | kExtSdf>); break;
-        }
+        if (o.counting) launch_tail_variant<true>(X.ext & (kExtVariants - 1u), go);
+        else launch_tail_variant<false>(X.ext & (kExtVariants - 1u), go);
         if (ev) {
             (void)hipEventRecord(o.extend_events[1], stream);
             o.events_used = 2u;

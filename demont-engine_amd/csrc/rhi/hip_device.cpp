@@ -162,6 +162,15 @@ bool DecodePathtracePush(const void* data, std::size_t size, bool strict, Pathtr
     F.sdf_epsilon = f32(p, kSdfParamsF);
     need(r.sdf_count != 0u && u32(p, kSdfParams + 8) != 0u, "SDF iteration heat map (sdf_params.z)");
     need(u32(p, kRestirEnabled) != 0u, "ReSTIR");                                            // :6315
+    need(f32(p, kOceanParams0) != 0.0f, "FFT ocean");                                         // :6791, :5484
+    // water (Engine.cpp:8883-8890, 9263-9266): absorption rgb and index,
+    // wave scale / amplitude / speed / time, the camera's medium and the
+    // deep-water tint
+    for (int k = 0; k < 3; ++k) F.water_absorb[k] = f32(p, kWaterParams0 + 4 * k);
+    F.water_ior = f32(p, kWaterParams0 + 12);
+    for (int k = 0; k < 4; ++k) F.wave[k] = f32(p, kWaterParams1 + 4 * k);
+    F.camera_in_water = f32(p, kWaterState) > 0.5f ? 1u : 0u;
+    for (int k = 0; k < 3; ++k) F.deep_color[k] = f32(p, kWaterState + 4 + 4 * k);
     need(u32(p, kMatOverrideCount) != 0u, "material overrides");                             // :1164
     need(f32(p, kSmokeParams) != 0.0f, "smoke volumes");                                     // :981
     need(u32(p, kWriteShadowVis) != 0u, "SIGMA shadow visibility (write_shadow_vis)");       // :5693-5702

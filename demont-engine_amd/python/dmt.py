@@ -24,10 +24,11 @@ DMT_ERR_UNSUPPORTED = -5
 DMT_ERR_DEVICE_LOST = -6
 DMT_ERR_LIMIT = -7
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 DMT_FRAME_NO_RUSSIAN_ROULETTE = 1
 DMT_FRAME_FIXED_JITTER = 2
 FRAME_MESH_TEXTURES = 4          # DMT_FRAME_MESH_TEXTURES
+MAT_WATER = 3                    # DMT_MAT_WATER
 SCHEDULE_WAVEFRONT = 0xFFFFFFFF
 MATERIAL_FLOATS = 7
 DMT_NOALLOC_BLOCKS = 512        # dmt_trace_closest_noalloc's grid
@@ -77,6 +78,11 @@ class FrameParams(C.Structure):
         ("sdf_max_iters", C.c_uint32),
         ("sdf_epsilon", C.c_float),
         ("mesh_tex", C.c_uint32 * 4),
+        ("water_absorb", C.c_float * 3),
+        ("water_ior", C.c_float),
+        ("wave", C.c_float * 4),
+        ("camera_in_water", C.c_uint32),
+        ("deep_color", C.c_float * 3),
     ]
 
     def copy(self) -> "FrameParams":

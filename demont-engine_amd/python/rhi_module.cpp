@@ -81,6 +81,11 @@ py::dict decoded(const PathtracePush& P) {
     d["sdf_max_iters"] = F.sdf_max_iters;
     d["sdf_epsilon"] = F.sdf_epsilon;
     d["mesh_tex"] = std::vector<uint32_t>(F.mesh_tex, F.mesh_tex + 4);
+    d["water_absorb"] = std::vector<float>(F.water_absorb, F.water_absorb + 3);
+    d["water_ior"] = F.water_ior;
+    d["wave"] = std::vector<float>(F.wave, F.wave + 4);
+    d["camera_in_water"] = F.camera_in_water;
+    d["deep_color"] = std::vector<float>(F.deep_color, F.deep_color + 3);
     d["curr_view_proj"] = std::vector<float>(P.curr_view_proj, P.curr_view_proj + 16);
     d["prev_view_proj"] = std::vector<float>(P.prev_view_proj, P.prev_view_proj + 16);
     d["ignored"] = P.ignored;

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define DMT_ABI_VERSION 7u
+#define DMT_ABI_VERSION 8u
 
 typedef int32_t dmt_status;
 #define DMT_OK                  0
@@ -173,6 +173,19 @@ typedef struct dmt_frame_params {
      * metallic; 0xFFFFFFFF = none), read when flags has
      * DMT_FRAME_MESH_TEXTURES and the scene has mesh UVs and an atlas. */
     uint32_t mesh_tex[4];
+    /* Water (water_params0, water_params1, water_state; PathTrace.slang:
+     * 5089-5296, 5847-5872, 6014-6050, 6768-6906): surfaces of material 3
+     * (DMT_MAT_WATER) bend their normal by drifting value-noise waves,
+     * reflect with Schlick's F0 0.02 or refract into and out of the water
+     * medium, in which a ray loses exp(-water_absorb * distance) per segment
+     * and, escaping, returns deep_color.  Frames with water render in the
+     * frame-completion kernel's feature form; the batched form and the
+     * adjoint refuse them. */
+    float    water_absorb[3];   /* water_params0.xyz, 1/m, >= 0            */
+    float    water_ior;         /* water_params0.w: index of surfaces whose own is <= 0 */
+    float    wave[4];           /* water_params1: scale (cycles/m), amplitude, speed, time (s) */
+    uint32_t camera_in_water;   /* water_state.x > 0.5: camera paths start in the water */
+    float    deep_color[3];     /* water_state.yzw, >= 0                   */
 } dmt_frame_params;
 
 /* Russian roulette off (the reference applies it from bounce 2 on,
@@ -187,6 +200,8 @@ typedef struct dmt_frame_params {
 /* mesh_tex holds the mesh's map tiles (the engine sets mesh_tex_indices every
  * frame; a zero-initialised frame keeps the mesh untextured). */
 #define DMT_FRAME_MESH_TEXTURES 4u
+/* Material id of water surfaces (MAT_WATER, PathTrace.slang:1403). */
+#define DMT_MAT_WATER 3u
 
 typedef struct dmt_build_params {
     uint32_t sah_bins;          /* binned-SAH bin count (0 = default 16)     */

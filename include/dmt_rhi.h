@@ -167,6 +167,7 @@ constexpr std::size_t kHdriLightsCount = 752, kWriteNormalGbuffer = 756, kWriteA
 constexpr std::size_t kWriteHdrAux = 764, kMisEnabled = 768, kAccumParams = 784, kSdfParams = 816, kSdfParamsF = 832;
 constexpr std::size_t kWriteShadowVis = 856, kLightCount = 880, kLightTreeEnabled = 888;
 constexpr std::size_t kWriteSpecularAlbedo = 896, kWriteRoughness = 900, kWriteSpecularHitDistance = 904;
+constexpr std::size_t kWaterParams0 = 912, kWaterParams1 = 928, kWaterState = 1040, kOceanParams0 = 1120;
 constexpr std::size_t kRestirEnabled = 944, kSmokeParams = 960, kMotionBlurParams = 1024, kCloudsRuntime = 1072;
 constexpr std::size_t kMeshMotionPrev = 1088, kMeshMotionCurr = 1104;
 constexpr std::size_t kMeshTexIndices = 1152, kMatOverrideCount = 1200, kTonemapParams = 1600, kSunExtra2 = 1616;
@@ -202,8 +203,9 @@ struct PathtracePush {
  * tracer's no-op, SoftwareTracer.cpp:686-690) and, when strict, for a push that
  * turns on a feature this backend does not render: a procedural / Hosek sky
  * (sun_and_mode.w 2 or 3), participating media, the SDF iteration heat map, ReSTIR,
- * material overrides, smoke, the SIGMA shadow-visibility output, or a
- * per-frame denoiser colour input without reset_accum.
+ * material overrides, smoke, the SIGMA shadow-visibility output, the FFT
+ * ocean (ocean_params0.x), or a per-frame denoiser colour input without
+ * reset_accum.
  * Lenient decoding renders without them and lists them in out->ignored.  The
  * gradient sky's sun and moon (sun_and_mode, sun_extra, sun_extra2,
  * moon_dir_phase, moon_extra), the HDRI cluster lights (hdri_lights_*) and,
@@ -211,7 +213,8 @@ struct PathtracePush {
  * the SDF clusters' count and sphere-trace parameters (sdf_params.xy,
  * sdf_params_f.x) into sdf_count and the frame, and the mesh's PBR map tiles
  * (mesh_tex_indices) into frame.mesh_tex with DMT_FRAME_MESH_TEXTURES when
- * any is set.
+ * any is set, and the water (water_params0, water_params1, water_state) into
+ * the frame's water fields.
  * Fields past the pushed size read as zero. */
 bool DecodePathtracePush(const void* data, std::size_t size, bool strict, PathtracePush* out, std::string* error);
 

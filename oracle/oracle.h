@@ -94,6 +94,15 @@ typedef struct orc_frame {
     /* atlas tiles of the mesh's albedo, normal, roughness and metallic maps
      * (mesh_tex_indices; 0xFFFFFFFF = none) */
     uint32_t mesh_tex[4];
+    /* Water (water_params0, water_params1, water_state): Beer's-law absorption
+     * rgb (1/m), the surface's index, the wave normal's scale (cycles/m),
+     * amplitude, speed and time, the camera's in-water flag and the deep-water
+     * tint (PathTrace.slang:5089-5296, 5847-5872, 6014-6050, 6768-6906) */
+    float    water_absorb[3];
+    float    water_ior;
+    float    wave[4];
+    uint32_t camera_in_water;
+    float    deep_color[3];
 } orc_frame;
 
 typedef struct orc_hit {
@@ -186,6 +195,12 @@ int  orc_pbr_surface(const orc_scene* s, const orc_frame* f, const float* rays, 
                      float* out);
 void orc_pbr_normal_map(const float* normals, const float* ts, uint32_t n, float* out);
 void orc_pbr_srgb(const float* c, uint32_t n, float* out);
+
+/* cloud_value_noise (PathTraceCloud.slang:31-69) at n points (3 floats
+ * each), and waterShadingNormal (PathTrace.slang:5231-5270) of the frame's
+ * wave parameters at n points with macro normals (3 floats each). */
+void orc_value_noise(const float* p, uint32_t n, float* out);
+int  orc_water_normal(const orc_frame* f, const float* p, const float* n_macro, uint32_t n, float* out);
 
 /* Host builders, exposed for tests. */
 double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H,

@@ -103,6 +103,10 @@ def lib():
         L.orc_pbr_normal_map.restype = None
         L.orc_pbr_srgb.argtypes = [fp, u32, fp]
         L.orc_pbr_srgb.restype = None
+        L.orc_value_noise.argtypes = [fp, u32, fp]
+        L.orc_value_noise.restype = None
+        L.orc_water_normal.argtypes = [vp, fp, fp, u32, fp]
+        L.orc_water_normal.restype = C.c_int
         _lib = L
     return _lib
 
@@ -399,4 +403,22 @@ def pbr_srgb(c):
     c = np.ascontiguousarray(c, np.float32).reshape(-1)
     out = np.zeros_like(c)
     lib().orc_pbr_srgb(_fp(c), c.size, _fp(out))
+    return out
+
+
+def value_noise(points):
+    """cloud_value_noise (fp32) at points (n, 3)."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.zeros(p.shape[0], np.float32)
+    lib().orc_value_noise(_fp(p), p.shape[0], _fp(out))
+    return out
+
+
+def water_normal(frame, points, normals):
+    """waterShadingNormal (fp32) of the frame's wave parameters at points (n, 3) with macro normals (n, 3)."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    out = np.zeros_like(p)
+    if lib().orc_water_normal(C.cast(C.byref(frame), C.c_void_p), _fp(p), _fp(n), p.shape[0], _fp(out)) != 0:
+        raise ValueError("oracle water normal failed")
     return out

@@ -48,7 +48,7 @@ using namespace orc;
 namespace {
 
 constexpr uint32_t PRIM_SPHERE = 0u, PRIM_PLANE = 1u;
-constexpr uint32_t MAT_LAMBERT = 0u, MAT_METAL = 1u, MAT_DIELECTRIC = 2u;
+constexpr uint32_t MAT_LAMBERT = 0u, MAT_METAL = 1u, MAT_DIELECTRIC = 2u, MAT_WATER = 3u;
 constexpr uint32_t LIGHT_POINT = 0u, LIGHT_SPOT = 1u, LIGHT_SPHERE = 2u, LIGHT_QUAD = 3u;
 constexpr uint32_t kAnalyticBvhThreshold = 16u;   // r_analytic_bvh_threshold (Engine.cpp:690)
 constexpr uint32_t kMaxLeaf = 4u;                 // TriangleBvh/AnalyticBvh kMaxLeafSize
@@ -1245,29 +1245,39 @@ template <class R> struct PT {
         return trace_scene(S, ro, rd, c, false, time, rm, F.sdf_max_iters, F.sdf_epsilon);
     }
 
-    // transmittance (PathTrace.slang:5089-5196), no water / clouds.
-    V3<R> transmittance(V3<R> o, V3<R> d, R t_max, R time) const {
+    // transmittance (PathTrace.slang:5089-5196), no clouds; water absorbs
+    // along the segments the shadow ray runs in it (start_in_water: the
+    // query starts under water) and refracts it like glass, F0 0.02.
+    V3<R> transmittance(V3<R> o, V3<R> d, R t_max, R time, bool start_in_water = false) const {
         int kmax = (int)F.refract_bounces;
         V3<R> t{R(1), R(1), R(1)};
+        bool sr_in_water = start_in_water;
         for (int i = 0; i <= kmax; ++i) {
             c.shadow++;
             Hit<R> h = trace(o, d, time);
+            if (sr_in_water) t = t * absorb_over(std::max(h.hit ? std::min(h.t, t_max) : t_max, R(0)));
             if (!h.hit || h.t >= t_max) return t;
-            if (h.mat != MAT_DIELECTRIC) return V3<R>{R(0), R(0), R(0)};
+            if (h.mat != MAT_DIELECTRIC && h.mat != MAT_WATER) return V3<R>{R(0), R(0), R(0)};
             if (i == kmax) return V3<R>{R(0), R(0), R(0)};
             R ndi = dot(h.n, d);
             V3<R> nf = (ndi < R(0)) ? h.n : -h.n;
-            R ia = (ndi < R(0)) ? R(1) : h.ior;
-            R ib = (ndi < R(0)) ? h.ior : R(1);
+            const R ior = (h.mat == MAT_WATER) ? water_ior_of(h.ior) : h.ior;
+            R ia = (ndi < R(0)) ? R(1) : ior;
+            R ib = (ndi < R(0)) ? ior : R(1);
             R eta = ia / ib;
             R ci = oabs(dot(nf, -d));
             R q = (ia - ib) / (ia + ib);
-            R r0 = q * q;
+            R r0 = (h.mat == MAT_WATER) ? R(0.02) : q * q;
             R m = R(1) - ci;
             R fr = r0 + ((((R(1) - r0) * m) * m * m) * m) * m;
             V3<R> rf = refract(d, nf, eta);
             if (length(rf) < R(1e-3)) return V3<R>{R(0), R(0), R(0)};
-            t = t * (h.albedo * (R(1) - fr));
+            if (h.mat == MAT_WATER) {
+                t = t * (R(1) - fr);
+                sr_in_water = !sr_in_water;
+            } else {
+                t = t * (h.albedo * (R(1) - fr));
+            }
             R adv = h.t + R(1e-3);
             o = o + d * adv;
             d = normalize(rf);
@@ -1399,6 +1409,65 @@ template <class R> struct PT {
             rd = normalize(fp - ro);
         }
     }
+
+    // ---- Water (PathTrace.slang:5207-5270, PathTraceCloud.slang:31-69) ----
+    // cloud_hash: lowbias32 of the integer lattice point, 24-bit uniform.
+    static uint32_t lowbias32(uint32_t x) {
+        x ^= x >> 16; x *= 0x7feb352du;
+        x ^= x >> 15; x *= 0x846ca68bu;
+        x ^= x >> 16;
+        return x;
+    }
+    static R lattice(R x, R y, R z) {
+        uint32_t h = lowbias32((uint32_t)(int32_t)value_of(x));
+        h = lowbias32(h ^ (uint32_t)(int32_t)value_of(y));
+        h = lowbias32(h ^ (uint32_t)(int32_t)value_of(z));
+        return R(float(h & 0x00FFFFFFu) * (1.0f / 16777216.0f));
+    }
+    static R lerpf(R a, R b, R t) { return a + t * (b - a); }
+    // cloud_value_noise: quintic-smoothed trilinear value noise.
+    static R value_noise(V3<R> p) {
+        const V3<R> i{ofloor(p.x), ofloor(p.y), ofloor(p.z)};
+        V3<R> f = p - i;
+        auto q = [](R t) { return ((t * t) * t) * (t * (t * R(6) - R(15)) + R(10)); };
+        f = V3<R>{q(f.x), q(f.y), q(f.z)};
+        const R a = lattice(i.x, i.y, i.z), b = lattice(i.x + R(1), i.y, i.z);
+        const R c = lattice(i.x, i.y + R(1), i.z), d = lattice(i.x + R(1), i.y + R(1), i.z);
+        const R e = lattice(i.x, i.y, i.z + R(1)), fx = lattice(i.x + R(1), i.y, i.z + R(1));
+        const R gx = lattice(i.x, i.y + R(1), i.z + R(1)), hx = lattice(i.x + R(1), i.y + R(1), i.z + R(1));
+        return lerpf(lerpf(lerpf(a, b, f.x), lerpf(c, d, f.x), f.y), lerpf(lerpf(e, fx, f.x), lerpf(gx, hx, f.x), f.y),
+                     f.z);
+    }
+    // waterShadingNormal: the macro normal bent by the central-difference
+    // gradient of two drifting octaves of value noise over the xz plane.
+    V3<R> water_normal(V3<R> wp, V3<R> n) const {
+        const R scale = R(F.wave[0]), amp = R(F.wave[1]), speed = R(F.wave[2]);
+        const R t = R(F.wave[3]) * speed;
+        if (amp <= R(0) || scale <= R(0)) return n;
+        const R eps = R(0.15);
+        const R px = wp.x * scale, pz = wp.z * scale;
+        const R dax = R(0.13) * t, daz = R(0.07) * t;
+        const R dbx = R(-0.09) * t, dbz = R(0.11) * t;
+        const R k = R(2.02);
+        auto h = [&](R x, R z, R x2, R z2) {
+            return value_noise(V3<R>{x + dax, R(0), z + daz}) + R(0.5) * value_noise(V3<R>{x2 + dbx, R(0), z2 + dbz});
+        };
+        const R n_px = h(px + eps, pz, px * k + eps, pz * k);
+        const R n_nx = h(px - eps, pz, px * k - eps, pz * k);
+        const R n_pz = h(px, pz + eps, px * k, pz * k + eps);
+        const R n_nz = h(px, pz - eps, px * k, pz * k - eps);
+        const R dhx = (n_px - n_nx) / (R(2) * eps);
+        const R dhz = (n_pz - n_nz) / (R(2) * eps);
+        const V3<R> t1 = (oabs(n.y) < R(0.99)) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
+        const V3<R> tx = normalize(cross(t1, n));
+        const V3<R> tz = cross(n, tx);
+        return normalize(n - (tx * dhx + tz * dhz) * amp);
+    }
+    V3<R> absorb_over(R seg) const {
+        return V3<R>{M::exp(-R(F.water_absorb[0]) * seg), M::exp(-R(F.water_absorb[1]) * seg),
+                     M::exp(-R(F.water_absorb[2]) * seg)};
+    }
+    R water_ior_of(R h_ior) const { return clampv((h_ior > R(0)) ? h_ior : R(F.water_ior), R(1), R(2.4)); }
 
     // ---- PBR material atlas (PathTrace.slang:1435-1605, applyPbrTextures;
     // the software device's CPU mirror SoftwareTracer.cpp:356-525) ----
@@ -1544,9 +1613,20 @@ template <class R> struct PT {
         // ray-cone LOD state (PathTrace.slang:5913-5919)
         const R cone_spread = (R(2) * R(F.pos_fovtan[3])) / R(std::max(F.height, 1u));
         R cone_width = R(0);
+        // the ray's medium (:5872): the camera's in-water flag, flipped by
+        // each refraction through a water surface
+        bool in_water = F.camera_in_water != 0u;
         for (uint32_t b = 0; b < F.max_bounces; ++b) {
             if (b == 0) c.primary++; else c.extension++;
             Hit<R> h = trace(ro, rd, time);
+            // Beer's law over the segment just traced (:6028-6031)
+            if (in_water && h.hit) T = T * absorb_over(h.t);
+            if (!h.hit && in_water) {
+                // escaped under water: the deep-water tint (:6046-6052)
+                const V3<R> v = T * ld3<R>(F.deep_color);
+                L = L + ((b == 0) ? v : clamp_indirect(v));
+                break;
+            }
             if (!h.hit) {
                 if (mis_on && prev_lambert) {
                     // cluster directions are the clusters' own NEE's (:6074);
@@ -1580,7 +1660,7 @@ template <class R> struct PT {
                     R ndl = dot(nf, ls.dir);
                     if (ls.pdf > R(0) && ndl > R(0)) {
                         R tm = std::max(ls.dist - R(1e-3), R(0));
-                        V3<R> tr = transmittance(hp + nf * R(1e-3), ls.dir, tm, time);
+                        V3<R> tr = transmittance(hp + nf * R(1e-3), ls.dir, tm, time, in_water);
                         if (any_pos(tr)) {
                             V3<R> brdf = h.albedo / PI;
                             V3<R> cc = ((((T * ls.Le) * tr) * brdf) * ndl) / ls.pdf;
@@ -1594,7 +1674,7 @@ template <class R> struct PT {
                     EnvS e = sample_env(x0, x1);
                     R ndl = dot(nf, e.dir);
                     if (e.pdf > R(0) && ndl > R(0)) {
-                        V3<R> tr = transmittance(hp + nf * R(1e-3), e.dir, R(1e30), time);
+                        V3<R> tr = transmittance(hp + nf * R(1e-3), e.dir, R(1e30), time, in_water);
                         if (any_pos(tr)) {
                             V3<R> brdf = h.albedo / PI;
                             R pb = ndl * (R(1) / PI);
@@ -1620,7 +1700,7 @@ template <class R> struct PT {
                         R hpdf = std::max(R(F.hdri_dir[sel][3]), R(1e-6));
                         R ndl = dot(nf, hd);
                         if (ndl > R(0) && hpdf > R(0)) {
-                            V3<R> tr = transmittance(hp + nf * R(1e-3), hd, R(1e30), time);
+                            V3<R> tr = transmittance(hp + nf * R(1e-3), hd, R(1e30), time, in_water);
                             if (any_pos(tr)) {
                                 V3<R> brdf = h.albedo / PI;
                                 V3<R> cc = ((((T * hc) * tr) * brdf) * ndl) / hpdf;
@@ -1637,7 +1717,7 @@ template <class R> struct PT {
                         R ndl = std::max(R(0), dot(nf, sd));
                         V3<R> jit = sample_sun_disc(sd, seed);
                         if (ndl > R(0)) {
-                            V3<R> tr = transmittance(hp + nf * R(1e-3), jit, R(1e30), time);
+                            V3<R> tr = transmittance(hp + nf * R(1e-3), jit, R(1e30), time, in_water);
                             if (any_pos(tr)) {
                                 V3<R> brdf = h.albedo / PI;
                                 L = L + (((T * sun_radiance(sd.y)) * tr) * brdf) * ndl;
@@ -1651,7 +1731,7 @@ template <class R> struct PT {
                             V3<R> jit = sample_sun_disc(md, seed);
                             R ndl = std::max(R(0), dot(nf, jit));
                             if (ndl > R(0)) {
-                                V3<R> tr = transmittance(hp + nf * R(1e-3), jit, R(1e30), time);
+                                V3<R> tr = transmittance(hp + nf * R(1e-3), jit, R(1e30), time, in_water);
                                 if (any_pos(tr)) {
                                     V3<R> brdf = h.albedo / PI;
                                     L = L + (((T * moon_radiance(md.y, phase_b)) * tr) * brdf) * ndl;
@@ -1706,6 +1786,36 @@ template <class R> struct PT {
                 } else {
                     rd = normalize(refr);
                     ro = hp - nf * R(1e-3);
+                }
+            } else if (h.mat == MAT_WATER) {
+                // the water surface (:6768-6906 without the FFT ocean and its
+                // foam): the wave normal, Schlick with F0 0.02, reflection
+                // (or total internal reflection) keeps the medium, refraction
+                // crosses it; no NEE, the throughput unchanged
+                const R wior = water_ior_of(h.ior);
+                const V3<R> sn = water_normal(hp, h.n);
+                const R ndi_s = dot(sn, rd);
+                const V3<R> nf_s = (ndi_s < R(0)) ? sn : -sn;
+                const R ia = (ndi_s < R(0)) ? R(1) : wior;
+                const R ib = (ndi_s < R(0)) ? wior : R(1);
+                const R eta = ia / ib;
+                const R ci = oabs(dot(nf_s, -rd));
+                const R r0 = R(0.02);
+                const R m = R(1) - ci;
+                const R fr = r0 + ((((R(1) - r0) * m) * m * m) * m) * m;
+                const V3<R> refl = reflect(rd, nf_s);
+                const V3<R> refr = refract(rd, nf_s, eta);
+                bool take_refl = length(refr) < R(1e-3);
+                if (!take_refl) take_refl = randf<R>(seed) < fr;
+                if (take_refl) {
+                    rd = refl;
+                    ro = hp + nf_s * R(1e-3);
+                    skip_sky = false;
+                    prev_lambert = false;
+                } else {
+                    in_water = ndi_s < R(0);
+                    rd = normalize(refr);
+                    ro = hp - nf_s * R(1e-3);
                 }
             }
             if (b >= 2u && !(F.flags & 1u)) {
@@ -2186,6 +2296,23 @@ void orc_pbr_normal_map(const float* n, const float* ts, uint32_t count, float* 
 
 void orc_pbr_srgb(const float* c, uint32_t count, float* out) {
     for (uint32_t i = 0; i < count; ++i) out[i] = PT<float>::srgb_to_linear(c[i]);
+}
+
+void orc_value_noise(const float* p, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = PT<float>::value_noise(V3<float>{p[3 * i], p[3 * i + 1], p[3 * i + 2]});
+}
+
+int orc_water_normal(const orc_frame* f, const float* p, const float* nm, uint32_t n, float* out) {
+    if (!f) return 1;
+    orc_scene s;
+    Ctr c;
+    PT<float> pt{s, *f, c};
+    for (uint32_t i = 0; i < n; ++i) {
+        const auto r = pt.water_normal(V3<float>{p[3 * i], p[3 * i + 1], p[3 * i + 2]},
+                                       V3<float>{nm[3 * i], nm[3 * i + 1], nm[3 * i + 2]});
+        out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+    }
+    return 0;
 }
 
 double orc_build_env_cdf(const float* rgb, uint32_t W, uint32_t H, float* om, float* oc) {

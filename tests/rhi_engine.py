@@ -35,7 +35,8 @@ OFF = dict(pos_fovtan=0, fwd_aspect=16, right_xyz=32, up_xyz=48, frame_index=64,
            light_tree_enabled=888, write_specular_albedo_gbuffer=896, write_roughness_gbuffer=900,
            write_specular_hit_distance_gbuffer=904, restir_enabled=944, smoke_params=960,
            motion_blur_params=1024, mesh_motion_prev=1088, mesh_motion_curr=1104, mesh_tex_indices=1152,
-           hosek_params=1184, mat_override_count=1200,
+           hosek_params=1184, mat_override_count=1200, water_params0=912, water_params1=928, water_state=1040,
+           ocean_params0=1120,
            tonemap_params=1600, sun_extra2=1616, moon_extra=464, sun_extra=480, hdri_lights_dir=496,
            hdri_lights_col=624)
 

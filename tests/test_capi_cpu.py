@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported():
 def test_abi_version():
     v = C.c_uint32()
     assert dmt.lib().dmt_get_abi_version(C.byref(v)) == 0
-    assert v.value == dmt.ABI_VERSION == 7
+    assert v.value == dmt.ABI_VERSION == 8
     assert dmt.lib().dmt_get_abi_version(None) == dmt.DMT_ERR_INVALID_ARG
 
 

@@ -219,7 +219,7 @@ def test_invalid_clusters_and_parameters_are_refused(device):
     bad.view(np.uint32)[3] = 9                       # node_count
     cases.append((bad, dmt.DMT_ERR_INVALID_ARG))
     bad = good.copy()
-    bad.view(np.uint32)[7] = 3                       # material
+    bad.view(np.uint32)[7] = 4                       # material (3 is water)
     cases.append((bad, dmt.DMT_ERR_INVALID_ARG))
     bad = good.copy()
     bad[0] = 1.0                                     # aabb_min.x > aabb_max.x

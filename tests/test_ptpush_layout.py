@@ -41,7 +41,8 @@ DECODER_FIELDS = {
     "kRestirEnabled": "restir_enabled", "kSmokeParams": "smoke_params", "kMotionBlurParams": "motion_blur_params",
     "kCloudsRuntime": "clouds_runtime", "kMeshMotionPrev": "mesh_motion_prev", "kMeshMotionCurr": "mesh_motion_curr",
     "kMeshTexIndices": "mesh_tex_indices", "kMatOverrideCount": "mat_override_count",
-    "kTonemapParams": "tonemap_params",
+    "kTonemapParams": "tonemap_params", "kWaterParams0": "water_params0", "kWaterParams1": "water_params1",
+    "kWaterState": "water_state", "kOceanParams0": "ocean_params0",
 }
 
 

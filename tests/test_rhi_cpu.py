@@ -79,6 +79,7 @@ GATES = [
     ("mat_override_count", "4I", (1, 0, 0, 0), "material overrides"),
     ("smoke_params", "4f", (1.0, 0.0, 0.0, 0.0), "smoke volumes"),
     ("write_shadow_vis", "I", 1, "SIGMA shadow visibility"),
+    ("ocean_params0", "4f", (1.0, 256.0, 0.0, 0.0), "FFT ocean"),
 ]
 
 
@@ -297,3 +298,21 @@ def test_decode_reads_the_mesh_map_tiles():
     d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=True, tlas_present=True),
                                   strict=True)
     assert d["mesh_tex"] == [NO_TEX_TILE] * 4 and not (d["flags"] & dmt.FRAME_MESH_TEXTURES)
+
+
+def test_decode_reads_the_water_fields():
+    """water_params0 912 (absorption rgb, index), water_params1 928 (wave scale,
+    amplitude, speed, time), water_state 1040 (camera in water, deep tint),
+    as Engine.cpp:8883-8890 and :9263-9266 fill them."""
+    f = _frame()
+    ov = {"water_params0": ("4f", (0.1, 0.2, 0.4, 1.33)), "water_params1": ("4f", (1.5, 0.3, 0.7, 12.5)),
+          "water_state": ("4f", (1.0, 0.01, 0.02, 0.05))}
+    d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides=ov),
+                                  strict=True)
+    assert d["water_absorb"] == pytest.approx([0.1, 0.2, 0.4]) and d["water_ior"] == pytest.approx(1.33)
+    assert d["wave"] == pytest.approx([1.5, 0.3, 0.7, 12.5])
+    assert d["camera_in_water"] == 1 and d["deep_color"] == pytest.approx([0.01, 0.02, 0.05])
+    ov["water_state"] = ("4f", (0.4, 0.0, 0.0, 0.0))   # the engine writes 0.0 or 1.0; > 0.5 is in the water
+    d = rhi.decode_pathtrace_push(pack_ptpush(f, prim_count=1, light_count=1, env_present=True, overrides=ov),
+                                  strict=True)
+    assert d["camera_in_water"] == 0
