@@ -313,8 +313,8 @@ k_tail(SceneView S, FrameArgs F, FeatureArgs X, uint32_t in_par, RayQueue q, uin
         }
         // the finished queries of the wave, together: the vertex, or one step
         // of a shadow query; then the lane's next query
-        if (ready != 0ull && ((uint32_t)__popcll(ready) >= tail::kTailShadeLanes || walk == 0ull) &&
-            state == tail::kLaneReady) {
+        const uint32_t shade_at = drained ? tail::kTailShadeLanesDrained : tail::kTailShadeLanes;
+        if (ready != 0ull && ((uint32_t)__popcll(ready) >= shade_at || walk == 0ull) && state == tail::kLaneReady) {
             if (l.mode == 0u) {
                 if (l.b == 0u) n_primary++;
                 else n_ext++;
