@@ -296,8 +296,10 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
             {(void**)&W.ps.T, n * 16}, {(void**)&W.ps.L, n * 16}, {(void**)&W.ps.F, n * 16},
             {(void**)&W.ps.rng, n * 4}, {(void**)&W.rq[0].o, n * 16}, {(void**)&W.rq[0].d, n * 16},
             {(void**)&W.rq[1].o, n * 16}, {(void**)&W.rq[1].d, n * 16}, {(void**)&W.hits, n * 8},
-            {(void**)&W.counters, kCounterSlots * 4}, {(void**)&W.ps.meta, n * 4},
-            {(void**)&W.finished, n * 4}, {(void**)&W.regen, n},
+            {(void**)&W.counters, kCounterSlotsAll * 4}, {(void**)&W.ps.meta, n * 4},
+            {(void**)&W.finished, n * 4}, {(void**)&W.regen, n}, {(void**)&W.ps.cost, n * 4},
+            {(void**)&W.tile_bucket, (size_t)tile_capacity((uint32_t)n) * 4},
+            {(void**)&W.tile_order, (size_t)tile_capacity((uint32_t)n) * 4},
         };
         for (const auto& r : per_set) req.push_back(r);
         for (int k = 0; k < 2; ++k) {
@@ -319,8 +321,10 @@ dmt_status ensure_wavefront(dmt_scene* s, uint32_t w, uint32_t h, bool* resized)
         s->wf_buffers.push_back(*r.first);
     }
     hipError_t e = hipSuccess;
-    for (int set = 0; set < (int)kFrameSets && e == hipSuccess; ++set)
+    for (int set = 0; set < (int)kFrameSets && e == hipSuccess; ++set) {
         e = hipMemsetAsync(s->wf[set].regen, 0, n, s->dev->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(s->wf[set].ps.cost, 0, n * 4, s->dev->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s->dev->stream);
     if (e != hipSuccess) {
         free_sets(s);

@@ -23,19 +23,22 @@ __device__ __forceinline__ uint32_t camera_tiles(const FrameArgs& F) {
     return ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
 }
 
-// The first path of pixel (lane of 8x8 tile `tile`), queued at slot = pixels
-// of the full tile rows above + full tiles to the left in this tile row +
-// position inside the (possibly clipped) tile: a wave64 traces one tile of
-// neighbouring camera rays; the path state stays indexed by pixel.
+// The first path of queue entry qs: lane qs % 64 of 8x8 tile qs / 64 (of
+// tile ps.tile_order[qs / 64] when the frame is ordered, which only frames of
+// whole tiles are), queued at slot = pixels of the full tile rows above + full
+// tiles to the left in this tile row + position inside the (possibly clipped)
+// tile, or at qs when ordered: a wave64 traces one tile of neighbouring camera
+// rays; the path state stays indexed by pixel.
 __device__ __forceinline__ void camera_path(const FrameArgs& F, uint32_t frame_index, PathState& ps, RayQueue& q,
-                                            uint32_t tile, uint32_t lane) {
+                                            uint32_t qs) {
+    const uint32_t tile = ps.tile_order ? ps.tile_order[qs >> 6] : qs >> 6, lane = qs & 63u;
     const uint32_t tiles_x = (F.width + 7u) / 8u;
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const uint32_t wx = min(8u, F.width - tx * 8u), hy = min(8u, F.height - ty * 8u);
     const uint32_t lx = lane & 7u, ly = lane >> 3;
     if (lx >= wx || ly >= hy) return;
     const uint32_t x = tx * 8u + lx, y = ty * 8u + ly;
-    const uint32_t slot = 8u * ty * F.width + tx * 8u * hy + ly * wx + lx;
+    const uint32_t slot = ps.tile_order ? qs : 8u * ty * F.width + tx * 8u * hy + ly * wx + lx;
     const uint32_t p = y * F.width + x;
     uint32_t seed = pixel_seed(F, x, y, frame_index);
     f3 rd = camera_ray(F, x, y, seed);
@@ -219,6 +222,7 @@ __device__ __forceinline__ void tail_advance(const FrameArgs& F, const FeatureAr
     }
     const f3 v = frame_value(F, sum, FEAT && X.spectral != 0u);
     ps.F[l.pix] = make_float4(v.x, v.y, v.z, 0.0f);
+    if (ps.cost) ps.cost[l.pix] = (uint32_t)__builtin_amdgcn_s_memrealtime() - ps.cost[l.pix];
     l.active = false;
 }
 

@@ -32,6 +32,17 @@ constexpr uint32_t kCtrFinished = 2;
 constexpr uint32_t kCtrTail = 3;          // k_tail: queue entries handed out
 constexpr uint32_t kCtrExtend = 4;        // + parity: k_extend rays handed out beyond the first grid pass
 constexpr uint32_t kCounterSlots = 8;
+// Tile order of a frame (heaviest 8x8 tiles first): per log2 bucket of a
+// tile's cost on the frame set's previous frame, its tile count, then the
+// next free position.  Zeroed with the counters at the start of a frame.
+constexpr uint32_t kTileBuckets = 34;     // cost 0, then 2^0 .. 2^32
+constexpr uint32_t kTileBinCopies = 32;   // bin copies (workgroup % copies): fewer same-address atomics
+constexpr uint32_t kCtrTileBins = kCounterSlots;
+constexpr uint32_t kCtrTileCursor = kCtrTileBins + kTileBinCopies * kTileBuckets;
+constexpr uint32_t kCounterSlotsAll = kCtrTileCursor + kTileBuckets;
+constexpr uint32_t kTileOrderBlocks = 128;   // workgroups of k_tile_order
+// Tiles of a frame of n pixels in any shape: ceil(w / 8) * ceil(h / 8) <= n / 8 + 1.
+__host__ __device__ inline uint32_t tile_capacity(uint32_t n) { return n / 8u + 2u; }
 
 // Shadow records live at the slot of the path that formed them (no
 // compaction); the pixel word carries these flags.
@@ -45,6 +56,12 @@ struct PathState {
     float4* F;        // sample sum of the frame, then the frame value for k_accumulate
     uint32_t* rng;
     uint32_t* meta;   // bounce | sample << 8
+    // Per pixel: k_tail's start time (s_memrealtime, low 32 bits) while the
+    // pixel runs, its elapsed time once it has ended; null: not recorded.
+    uint32_t* cost = nullptr;
+    // Queue position / 64 -> 8x8 tile, heaviest tiles of the set's previous
+    // frame first; null: tiles in row order.
+    const uint32_t* tile_order = nullptr;
 };
 
 __host__ __device__ inline uint32_t meta_pack(uint32_t b, uint32_t s, uint32_t k) { return b | (s << 8) | (k << 16); }
@@ -88,7 +105,9 @@ struct Wavefront {
     ShadowQueue sq[4]{};      // 0: light NEE, 1: env NEE; 2, 3: sky lights (allocated for such frames only)
     uint32_t* finished = nullptr;     // per slot: pixel of a path that ended, or kMiss
     uint8_t* regen = nullptr;         // per pixel: next sample pending
-    uint32_t* counters = nullptr;
+    uint32_t* counters = nullptr;     // kCounterSlotsAll
+    uint32_t* tile_bucket = nullptr;  // per tile: log2 bucket of its cost (tile_capacity entries)
+    uint32_t* tile_order = nullptr;   // the order ps.tile_order points to (tile_capacity entries)
     uint32_t* spill = nullptr;
     uint32_t spill_depth = 0;
     uint32_t spill_threads = 0;
