@@ -1588,10 +1588,6 @@ dmt_status dmt_batch_create(dmt_device* dev, dmt_scene* const* scenes, const dmt
         if (!list[i]) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is null");
         if (list[i]->dev != dev) return fail(DMT_ERR_INVALID_ARG, "scene " + std::to_string(i) + " is on another device");
         if ((st = check_frame_for(list[i], &frames[i]))) return fail(st, "frame " + std::to_string(i) + ": " + g_err);
-        if (pbr_active(list[i], &frames[i]))
-            return fail(DMT_ERR_UNSUPPORTED, "frame " + std::to_string(i) + ": the batched form does not sample PBR texture maps");
-        if (water_active(list[i], &frames[i]))
-            return fail(DMT_ERR_UNSUPPORTED, "frame " + std::to_string(i) + ": the batched form does not render water");
     }
     {
         std::vector<dmt_scene*> sorted = list;

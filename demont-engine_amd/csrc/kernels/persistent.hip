@@ -88,7 +88,9 @@ struct Cold {
 };
 
 // Sample s of pixel (x, y); FEAT: with the frame's camera-sample features.
-template <bool FEAT>
+// A sample starts with a zero ray cone and, in WATER frames, in the camera's
+// medium.
+template <bool FEAT, bool WATER = false>
 __device__ __forceinline__ void start_sample(const FrameArgs& F, const FeatureArgs& X, Lane& l, uint32_t x,
                                              uint32_t y, uint32_t s) {
     if (FEAT) {
@@ -102,10 +104,15 @@ __device__ __forceinline__ void start_sample(const FrameArgs& F, const FeatureAr
     l.P.prev_pdf = 0.0f;
     l.P.skip_sky = false;
     l.P.prev_lambert = false;
+    l.P.cone = 0.0f;
+    if (WATER) l.P.in_water = X.cam_in_water != 0u;
     l.b = 0u;
     l.mode = 0u;
 }
 
+// A shadow record's query, in the path's medium (bit 31 of the step field):
+// a Lambert vertex, whose NEE records these are, does not change it.
+template <bool WATER = false>
 __device__ __forceinline__ void start_shadow(const float4* rec, uint32_t threads, uint32_t tid, uint32_t k, Lane& l,
                                              Cold& c) {
     const float4* row = rec + ((size_t)k * kRecRows) * threads + tid;
@@ -114,7 +121,7 @@ __device__ __forceinline__ void start_shadow(const float4* rec, uint32_t threads
     l.d = xyz(d4);
     c.set(Cold::kTmax, o4.w);
     c.set3(Cold::kTr, f3{1.0f, 1.0f, 1.0f});
-    c.setu(Cold::kStep, 0u);
+    c.setu(Cold::kStep, (WATER && l.P.in_water) ? 0x80000000u : 0u);
     l.mode = k + 1u;
 }
 
@@ -123,11 +130,11 @@ __device__ __forceinline__ void start_shadow(const float4* rec, uint32_t threads
 // and the HDRI cluster NEE with an env map; light, sun, moon, then the
 // ambient term without one -- or close the vertex.  Records 2 and 3 (sky
 // lights) exist only in SKY frames.
-template <bool FEAT, bool SKY = false>
+template <bool FEAT, bool SKY = false, bool WATER = false>
 __device__ __forceinline__ void advance(const FrameArgs& F, const FeatureArgs& X, const float4* rec, uint32_t threads,
                                         uint32_t tid, float4* accum, bool reset, Lane& l, Cold& c) {
     if (l.pending & 1u) {
-        start_shadow(rec, threads, tid, 0u, l, c);
+        start_shadow<WATER>(rec, threads, tid, 0u, l, c);
         return;
     }
     if (!SKY) {
@@ -139,7 +146,7 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const FeatureArgs& X
                 l.P.L = l.P.L + xyz(row[2 * (size_t)threads]);
                 l.pending &= ~2u;
             } else {
-                start_shadow(rec, threads, tid, 1u, l, c);
+                start_shadow<WATER>(rec, threads, tid, 1u, l, c);
                 return;
             }
         }
@@ -149,16 +156,16 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const FeatureArgs& X
             const float4* row = rec + ((size_t)1u * kRecRows) * threads + tid;
             ambient = row[3 * (size_t)threads].w < 0.0f;   // the no-env ambient term: after the sun and moon
             if (!ambient) {
-                start_shadow(rec, threads, tid, 1u, l, c);
+                start_shadow<WATER>(rec, threads, tid, 1u, l, c);
                 return;
             }
         }
         if (l.pending & 4u) {
-            start_shadow(rec, threads, tid, 2u, l, c);
+            start_shadow<WATER>(rec, threads, tid, 2u, l, c);
             return;
         }
         if (l.pending & 8u) {
-            start_shadow(rec, threads, tid, 3u, l, c);
+            start_shadow<WATER>(rec, threads, tid, 3u, l, c);
             return;
         }
         if (ambient) {
@@ -187,7 +194,7 @@ __device__ __forceinline__ void advance(const FrameArgs& F, const FeatureArgs& X
     if (s + 1u < F.spp) {
         c.set3(Cold::kSum, sum);
         c.setu(Cold::kS, s + 1u);
-        start_sample<FEAT>(F, X, l, x, y, s + 1u);   // the next sample continues the pixel's stream
+        start_sample<FEAT, WATER>(F, X, l, x, y, s + 1u);   // the next sample continues the pixel's stream
         return;
     }
     const uint32_t p = y * F.width + x;
@@ -204,7 +211,8 @@ template <uint32_t DEPTH> constexpr uint32_t persistent_waves() { return (160u *
 // time, hero wavelength, a translated mesh); each scene's FeatureArgs say
 // which, and the lane carries the sample's shutter time and wavelength.
 // EXT (implies FEAT): kExtSky, some scene's frame has sky lights (four record
-// kinds); kExtSdf, some scene has SDF clusters.
+// kinds); kExtSdf, some scene has SDF clusters; kExtPbr, some scene's frame
+// samples the PBR atlas; kExtWater, some scene has water or a camera in it.
 template <uint32_t DEPTH, bool FEAT, uint32_t EXT = 0u>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<DEPTH>()))) k_persistent(BatchArgs A) {
@@ -264,7 +272,7 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                         c.setu(Cold::kXY, x | (y << 16));
                         c.setu(Cold::kS, 0u);
                         l.P.seed = pixel_seed(F, x, y, frame_index);
-                        start_sample<FEAT>(F, X, l, x, y, 0u);
+                        start_sample<FEAT, (EXT & kExtWater) != 0u>(F, X, l, x, y, 0u);
                         state = kLaneWalk;
                         if (walk_begin<(EXT & kExtSdf) != 0u>(S, l.o, l.d, w)) {
                             h = final_hit(w.ab, w.tb);
@@ -332,19 +340,23 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                 else n_ext++;
                 ScratchSink sink{{}, A.rec, A.threads, tid, 0u};
                 VertexResult R;
-                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u>(S, F, l.b, l.o, l.d, h, l.P, R, sink,
-                                                                              &X.sky_args);
+                shade_vertex<(EXT & kExtSky) != 0u, (EXT & kExtSdf) != 0u, (EXT & kExtPbr) != 0u,
+                             (EXT & kExtWater) != 0u>(S, F, l.b, l.o, l.d, h, l.P, R, sink, &X.sky_args);
                 const bool cont = R.alive && (l.b + 1u) < F.max_bounces;
                 l.pending = sink.pending | (cont ? 0u : kEnded);
                 c.set3(Cold::kNo, R.no);
                 c.set3(Cold::kNd, R.nd);
-                advance<FEAT, (EXT & kExtSky) != 0u>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                advance<FEAT, (EXT & kExtSky) != 0u, (EXT & kExtWater) != 0u>(F, X, A.rec, A.threads, tid, accum, reset, l,
+                                                                                       c);
             } else {
                 n_shadow++;
                 float t_max = c.get(Cold::kTmax);
                 f3 tr = c.get3(Cold::kTr);
-                const uint32_t step = c.getu(Cold::kStep);
-                if (transmit_step<(EXT & kExtSdf) != 0u>(S, F, h, step, l.o, l.d, t_max, tr, l.P.time)) {
+                const uint32_t step_w = c.getu(Cold::kStep);   // step | medium << 31
+                const uint32_t step = step_w & 0x7FFFFFFFu;
+                bool wet = (step_w >> 31) != 0u;
+                if (transmit_step<(EXT & kExtSdf) != 0u, (EXT & kExtWater) != 0u>(S, F, h, step, l.o, l.d, t_max, tr,
+                                                                                   l.P.time, wet)) {
                     const uint32_t k = l.mode - 1u;
                     if (any_pos(tr)) {
                         const float4* row = A.rec + ((size_t)k * kRecRows) * A.threads + tid;
@@ -354,11 +366,12 @@ __attribute__((amdgpu_waves_per_eu(persistent_waves<DEPTH>(), persistent_waves<D
                                                      (l.pending & (16u << k)) != 0u, tr);
                     }
                     l.pending &= ~(1u << k);
-                    advance<FEAT, (EXT & kExtSky) != 0u>(F, X, A.rec, A.threads, tid, accum, reset, l, c);
+                    advance<FEAT, (EXT & kExtSky) != 0u, (EXT & kExtWater) != 0u>(F, X, A.rec, A.threads, tid, accum, reset, l,
+                                                                                       c);
                 } else {
                     c.set(Cold::kTmax, t_max);
                     c.set3(Cold::kTr, tr);
-                    c.setu(Cold::kStep, step + 1u);
+                    c.setu(Cold::kStep, (step + 1u) | (wet ? 0x80000000u : 0u));
                 }
             }
             if (l.active) {
@@ -404,26 +417,33 @@ hipError_t batch_set_frame(uint32_t* frame, uint32_t value, hipStream_t stream) 
     return hipGetLastError();
 }
 
+// Variant v: 0 plain; 1 + ext, features with the extensions of ext (sky
+// lights, SDF clusters, PBR maps, water: any combination).
+template <uint32_t DEPTH, uint32_t E = 0u>
+hipError_t launch_persistent_ext(uint32_t ext, uint32_t blocks, const BatchArgs& A, hipStream_t stream) {
+    if constexpr (E < kExtVariants) {
+        if (ext != E) return launch_persistent_ext<DEPTH, E + 1u>(ext, blocks, A, stream);
+        hipLaunchKernelGGL((k_persistent<DEPTH, true, E>), dim3(blocks), dim3(kBlock), 0, stream, A);
+        return hipSuccess;
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
 template <uint32_t DEPTH>
 hipError_t launch_persistent(uint32_t variant, uint32_t blocks, const BatchArgs& A, hipStream_t stream) {
-    switch (variant) {
-        case 0u: hipLaunchKernelGGL((k_persistent<DEPTH, false>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
-        case 1u: hipLaunchKernelGGL((k_persistent<DEPTH, true>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
-        case 2u: hipLaunchKernelGGL((k_persistent<DEPTH, true, kExtSky>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
-        case 3u: hipLaunchKernelGGL((k_persistent<DEPTH, true, kExtSdf>), dim3(blocks), dim3(kBlock), 0, stream, A); break;
-        case 4u:
-            hipLaunchKernelGGL((k_persistent<DEPTH, true, kExtSky | kExtSdf>), dim3(blocks), dim3(kBlock), 0, stream, A);
-            break;
-        default: return hipErrorInvalidValue;
+    if (variant == 0u) {
+        hipLaunchKernelGGL((k_persistent<DEPTH, false>), dim3(blocks), dim3(kBlock), 0, stream, A);
+        return hipSuccess;
     }
-    return hipSuccess;
+    return launch_persistent_ext<DEPTH>(variant - 1u, blocks, A, stream);
 }
 
 hipError_t launch_batch_frame(const BatchArgs& A, const BatchLaunch& L, hipStream_t stream) {
     if (L.blocks == 0u || (size_t)L.blocks * kBlock != A.threads) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, stream, A.work, A.nscenes);
-    // variant: 0 plain, 1 features, 2 + ext (sky lights and SDF clusters imply features)
-    const uint32_t v = L.features || L.ext ? 1u + (L.ext & 3u) : 0u;
+    // variant: 0 plain, 1 + ext features with extensions (an extension implies features)
+    const uint32_t v = L.features || L.ext ? 1u + (L.ext & (kExtVariants - 1u)) : 0u;
     hipError_t e;
     switch (L.stack_lds) {
         case 16u: e = launch_persistent<16u>(v, L.blocks, A, stream); break;

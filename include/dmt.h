@@ -179,8 +179,8 @@ typedef struct dmt_frame_params {
      * reflect with Schlick's F0 0.02 or refract into and out of the water
      * medium, in which a ray loses exp(-water_absorb * distance) per segment
      * and, escaping, returns deep_color.  Frames with water render in the
-     * frame-completion kernel's feature form; the batched form and the
-     * adjoint refuse them. */
+     * water variants of the frame-completion and batched kernels; the
+     * adjoint refuses them. */
     float    water_absorb[3];   /* water_params0.xyz, 1/m, >= 0            */
     float    water_ior;         /* water_params0.w: index of surfaces whose own is <= 0 */
     float    wave[4];           /* water_params1: scale (cycles/m), amplitude, speed, time (s) */
@@ -319,8 +319,8 @@ dmt_status dmt_scene_set_moon_map(dmt_scene* scene, const float* rgba, uint32_t 
  * roughness and metallic tiles (bit patterns; 0xFFFFFFFF = none), the mesh's
  * come with the frame; a tile applies when it is below height / 256
  * (applyPbrTextures, PathTrace.slang:1435-1605).  Frames of a scene with
- * textured surfaces render in the frame-completion kernel's PBR variant; the
- * batched form and the adjoint refuse them.  NULL: no atlas.  Waits for the
+ * textured surfaces render in the PBR variants of the frame-completion and
+ * batched kernels; the adjoint refuses them.  NULL: no atlas.  Waits for the
  * scene's frames in flight. */
 dmt_status dmt_scene_set_pbr_atlas(dmt_scene* scene, const uint8_t* rgba8, uint32_t width, uint32_t height);
 

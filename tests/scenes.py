@@ -188,3 +188,28 @@ def atlas(bases):
             b = np.broadcast_to(np.asarray(b, np.uint8), (PBR_TILE, PBR_TILE, 4))
         bands.append(atlas_band(b))
     return np.concatenate(bands, axis=0)
+
+
+def batch_against_scenes(device, cases, frames=2):
+    """Each (scene data, frame) of cases rendered alone (frames consecutive
+    frames) and all of them as one batch: per case the two accumulations, and
+    the ray counts (primary, extension, shadow) summed over the cases, alone
+    and batched."""
+    alone, counts = [], [0, 0, 0]
+    for sd, f in cases:
+        sc = dmt.Scene(device, sd)
+        sc.render_frames(f, frames)
+        alone.append(sc.readback(f.width, f.height))
+        st = sc.stats()
+        for i, k in enumerate(("primary_rays", "extension_rays", "shadow_rays")):
+            counts[i] += st[k]
+        sc.close()
+    scenes = [dmt.Scene(device, sd) for sd, _ in cases]
+    b = dmt.Batch(device, scenes, [f for _, f in cases])
+    b.render(0, frames)
+    st = b.stats()
+    batched = [sc.readback(f.width, f.height) for sc, (_, f) in zip(scenes, cases)]
+    b.close()
+    for sc in scenes:
+        sc.close()
+    return alone, batched, tuple(counts), (st["primary_rays"], st["extension_rays"], st["shadow_rays"])

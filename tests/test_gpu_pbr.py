@@ -4,9 +4,9 @@ tiles, so every level and both bilinear taps matter, at grazing angles where
 the ray cone selects coarse levels), several samples per pixel (the cone
 restarts per sample), the UV-mapped mesh with the frame's mesh tiles, maps
 together with SDF clusters and the sun, and with depth of field, motion blur
-and dispersion; removing the atlas restores the untextured image; the batched
-form and the adjoint refuse textured frames, and malformed atlases are
-refused."""
+and dispersion; removing the atlas restores the untextured image; a batch of
+textured and untextured scenes equals the scenes rendered alone; the adjoint
+refuses textured frames, and malformed atlases are refused."""
 import numpy as np
 import pytest
 import torch
@@ -14,7 +14,7 @@ import torch
 import dmt
 import oracle_ctypes as orc
 from scenes import (MAT_DIELECTRIC, MAT_LAMBERT, MAT_METAL, NO_TILE, SDF_BOX, SDF_LEAF, SDF_SPHERE, PBR_TILE,
-                    atlas, sdf_cluster, sdf_node, set_tiles)
+                    atlas, batch_against_scenes, sdf_cluster, sdf_node, set_tiles)
 
 pytestmark = pytest.mark.gpu
 
@@ -158,12 +158,24 @@ def test_maps_with_lens_motion_blur_and_dispersion(device):
     _check("maps + DOF + motion blur + spectral", got, ref, counts, oc)
 
 
-def test_textured_frames_are_refused_by_the_batch_and_the_adjoint(device):
+def test_textured_frames_in_a_batch_equal_the_scenes_rendered_alone(device):
+    """A batch of two textured scenes (different atlases, sizes and sample
+    counts) and an untextured one runs the batched kernel's PBR variant; each
+    scene's accumulation equals its render alone bit for bit, as do the ray
+    counts."""
+    sd1, f1 = _textured_spheres(seed=15, w=32, h=24)
+    sd2, f2 = _textured_spheres(seed=16, w=40, h=24, spp=3)
+    sd3 = dmt.generate(0, seed=17, width=24, height=16)
+    cases = [(sd1, f1), (sd2, f2), (sd3, sd3.frame)]
+    alone, batched, c_alone, c_batch = batch_against_scenes(device, cases)
+    for k, (a, b) in enumerate(zip(alone, batched)):
+        assert np.array_equal(a, b), k
+    assert c_alone == c_batch
+
+
+def test_textured_frames_are_refused_by_the_adjoint(device):
     sd, f = _textured_spheres(seed=15, w=32, h=24)
     sc = dmt.Scene(device, sd)
-    with pytest.raises(dmt.DmtError) as e:
-        dmt.Batch(device, [sc], [f])
-    assert e.value.status == dmt.DMT_ERR_UNSUPPORTED
     grads = torch.zeros(sd.prims.shape[0] * dmt.MATERIAL_FLOATS, dtype=torch.float32, device="cuda")
     dimg = torch.ones(f.width * f.height * 4, dtype=torch.float32, device="cuda")
     with pytest.raises(dmt.DmtError) as e:
@@ -174,10 +186,6 @@ def test_textured_frames_are_refused_by_the_batch_and_the_adjoint(device):
         with pytest.raises(dmt.DmtError) as e:
             sc.set_pbr_atlas(np.zeros(shape, np.uint8))
         assert e.value.status == dmt.DMT_ERR_INVALID_ARG
-    # without the atlas the same scene is an ordinary batch member
-    sc.set_pbr_atlas(None)
-    b = dmt.Batch(device, [sc], [f])
-    b.close()
     sc.close()
 
 

@@ -3,16 +3,17 @@ surface over the sphere workload seen from above (submerged spheres and
 floor, shadow rays through the surface) and from below (the camera in the
 water, the deep-water tint), water with SDF clusters under the sun, water
 with PBR maps, depth of field, motion blur and dispersion; the engine's push
-(water_params0/1, water_state) equals the C ABI's render; the batched form
-and the adjoint refuse water, invalid water parameters are refused."""
+(water_params0/1, water_state) equals the C ABI's render; a batch with water
+scenes equals the scenes rendered alone; the adjoint refuses water, invalid
+water parameters are refused."""
 import numpy as np
 import pytest
 import torch
 
 import dmt
 import oracle_ctypes as orc
-from scenes import (MAT_LAMBERT, MAT_METAL, NO_TILE, PBR_TILE, SDF_BOX, SDF_LEAF, SDF_SPHERE, atlas, plane,
-                    sdf_cluster, sdf_node, set_tiles)
+from scenes import (MAT_LAMBERT, MAT_METAL, NO_TILE, PBR_TILE, SDF_BOX, SDF_LEAF, SDF_SPHERE, atlas,
+                    batch_against_scenes, plane, sdf_cluster, sdf_node, set_tiles)
 
 pytestmark = pytest.mark.gpu
 
@@ -157,12 +158,38 @@ def test_engine_push_renders_water(device):
     dev.close()
 
 
-def test_water_is_refused_by_the_batch_and_the_adjoint_and_checked(device):
+def test_water_in_a_batch_equals_the_scenes_rendered_alone(device):
+    """A batch of a lake seen from above, a camera in the water, an SDF
+    cluster of water with PBR maps on the spheres, and a dry scene runs the
+    batched kernel's water variants; each accumulation equals the scene
+    rendered alone bit for bit, as do the ray counts."""
+    sd1, f1 = _lake(seed=28, w=32, h=24)
+    f1 = dmt.look_at((0.0, 3.0, 11.0), (0.0, -2.0, 0.0), 55.0, 32, 24, base=f1)
+    sd2, f2 = _lake(seed=29, w=24, h=24, spp=3)
+    f2 = dmt.look_at((0.5, -2.5, 7.0), (0.0, -1.0, 0.0), 60.0, 24, 24, base=f2)
+    f2.camera_in_water = 1
+    sd3 = dmt.generate(0, seed=30, width=32, height=16)
+    sd3.sdf = np.stack([
+        sdf_cluster([sdf_node(SDF_LEAF, SDF_BOX, params=(2.5, 0.6, 2.0, 0.1), center=(0.0, -0.6, 1.0))],
+                    (-2.65, -1.35, -1.15), (2.65, 0.15, 3.15), mat=3, albedo=(1, 1, 1), ior=1.33),
+    ])
+    sd3.atlas = atlas([RNG.integers(0, 256, size=(PBR_TILE, PBR_TILE, 4), dtype=np.uint8)])
+    for i in range(sd3.prims.shape[0]):
+        set_tiles(sd3.prims[i], albedo=1)
+    f3 = _water(sd3.frame.copy(), amp=0.25)
+    f3.spp = 2
+    f3.sdf_max_iters, f3.sdf_epsilon = 128, 1e-4
+    sd4 = dmt.generate(0, seed=31, width=16, height=16)
+    cases = [(sd1, f1), (sd2, f2), (sd3, f3), (sd4, sd4.frame)]
+    alone, batched, c_alone, c_batch = batch_against_scenes(device, cases)
+    for k, (a, b) in enumerate(zip(alone, batched)):
+        assert np.array_equal(a, b), k
+    assert c_alone == c_batch
+
+
+def test_water_is_refused_by_the_adjoint_and_checked(device):
     sd, f = _lake(seed=26, w=32, h=24)
     sc = dmt.Scene(device, sd)
-    with pytest.raises(dmt.DmtError) as e:
-        dmt.Batch(device, [sc], [f])
-    assert e.value.status == dmt.DMT_ERR_UNSUPPORTED
     grads = torch.zeros(sd.prims.shape[0] * dmt.MATERIAL_FLOATS, dtype=torch.float32, device="cuda")
     dimg = torch.ones(f.width * f.height * 4, dtype=torch.float32, device="cuda")
     with pytest.raises(dmt.DmtError) as e:
