@@ -170,7 +170,8 @@ def main():
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+        print(f"bench.py measures one GPU: the library renders a frame on one GPU and has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
 
