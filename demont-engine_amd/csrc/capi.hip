@@ -1338,8 +1338,6 @@ dmt_status dmt_adjoint(dmt_scene* s, const dmt_frame_params* f, const void* d_im
     if (f->refract_bounces > kAdjMaxDielectric) return fail(DMT_ERR_LIMIT, "the adjoint supports refract_bounces <= 8");
     if (sky_lights(s, f))
         return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate sky lights (sun, moon, HDRI clusters)");
-    if (s->view.sdf_count)
-        return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate scenes with SDF clusters");
     if (pbr_active(s, f))
         return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate PBR texture maps");
     if (water_active(s, f)) return fail(DMT_ERR_UNSUPPORTED, "the adjoint does not differentiate water");

@@ -22,8 +22,9 @@ __device__ __forceinline__ void add_grad(float* g, uint32_t i, float v) {
 // Path-replay sink: forms the radiance terms exactly as the renderer does
 // (shadow queries traced at once, in emission order) and scatters each
 // term's derivative.  REC: records the sample's metal vertices on analytic
-// primitives (at most kAdjMaxVerts) for the roughness pass.
-template <bool FEAT, bool REC = false>
+// primitives (at most kAdjMaxVerts) for the roughness pass.  SDF: the
+// queries see the scene's SDF clusters.
+template <bool FEAT, bool REC = false, bool SDF = false>
 struct AdjointSink : NoGradientHooks {
     const SceneView& S;
     const FrameArgs& F;
@@ -130,8 +131,8 @@ struct AdjointSink : NoGradientHooks {
         float t_max = r.t_max;
         bool done = false;
         for (uint32_t i = 0; i <= F.refract_bounces && !done; ++i) {
-            const HitRec h = trace_closest<false, FEAT>(S, o, d, st, tc, P.time);
-            done = transmit_step(S, F, h, i, o, d, t_max, tr, P.time);
+            const HitRec h = trace_closest<false, FEAT, SDF>(S, o, d, st, tc, P.time);
+            done = transmit_step<SDF>(S, F, h, i, o, d, t_max, tr, P.time);
 #pragma unroll
             for (uint32_t j = 0; j < kAdjMaxDielectric; ++j)
                 if (!done && j == nd) diel[j] = h.code;
@@ -148,7 +149,7 @@ struct AdjointSink : NoGradientHooks {
         add_prefix(q);
 #pragma unroll
         for (uint32_t j = 0; j < kAdjMaxDielectric; ++j) {
-            if (j < nd) {
+            if (j < nd && !(diel[j] & (kMeshBit | kSdfBit))) {   // dielectric SDF clusters have no parameters
                 const uint32_t pid = diel[j];
                 const float4 a = S.prims[(size_t)pid * 7u + 1u];
                 const uint32_t base = pid * kMaterialFloats;
@@ -173,7 +174,9 @@ struct AdjointSink : NoGradientHooks {
     }
 };
 
-template <bool LDS, bool FEAT, bool REC>
+// SDF (implies FEAT): the scene has SDF clusters.  Their surfaces carry no
+// parameters; the replay traces and shades them as the renderer does.
+template <bool LDS, bool FEAT, bool REC, bool SDF = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
 k_adjoint(AdjointArgs A, uint32_t p0, uint32_t p1) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
@@ -200,11 +203,11 @@ k_adjoint(AdjointArgs A, uint32_t p0, uint32_t p1) {
             PathVertexState P{f3{1.0f, 1.0f, 1.0f}, f3{0.0f, 0.0f, 0.0f}, 0.0f, false, false, seed, time, lambda};
             // a spectral sample enters the frame value weighted by its wavelength's response
             const f3 ds = (lambda >= 0.0f) ? delta * spectral_weight(lambda) : delta;
-            AdjointSink<FEAT, REC> sink{{}, A.S, F, stk, tc, G, ds, 0u, {}, {}, A.rec, A.rec_count, 0u};
+            AdjointSink<FEAT, REC, SDF> sink{{}, A.S, F, stk, tc, G, ds, 0u, {}, {}, A.rec, A.rec_count, 0u};
             for (uint32_t b = 0; b < F.max_bounces; ++b) {
-                const HitRec h = trace_closest<false, FEAT>(A.S, o, d, stk, tc, time);
+                const HitRec h = trace_closest<false, FEAT, SDF>(A.S, o, d, stk, tc, time);
                 VertexResult R;
-                shade_vertex(A.S, F, b, o, d, h, P, R, sink);
+                shade_vertex<false, SDF>(A.S, F, b, o, d, h, P, R, sink);
                 if (!(R.alive && (b + 1u) < F.max_bounces)) break;
                 o = R.no;
                 d = R.nd;
@@ -237,7 +240,7 @@ k_adjoint(AdjointArgs A, uint32_t p0, uint32_t p1) {
 
 // Dual vertices: roughness derivative seeded at bounce seed_b; shadow queries
 // traced at once along the dual ray.
-template <bool FEAT>
+template <bool FEAT, bool SDF = false>
 struct TangentSink : NoGradientHooks {
     const SceneView& S;
     const FrameArgs& F;
@@ -257,15 +260,15 @@ struct TangentSink : NoGradientHooks {
         df t_max = r.t_max;
         bool done = false;
         for (uint32_t i = 0; i <= F.refract_bounces && !done; ++i) {
-            const HitRec h = trace_closest<false, FEAT>(S, value3(o), value3(d), st, tc, P.time);
-            done = transmit_step(S, F, h, i, o, d, t_max, tr, P.time);
+            const HitRec h = trace_closest<false, FEAT, SDF>(S, value3(o), value3(d), st, tc, P.time);
+            done = transmit_step<SDF>(S, F, h, i, o, d, t_max, tr, P.time);
         }
         if (!done || !any_pos(tr)) return;
         P.L = P.L + shadow_value(F, kind, r.pre, r.ndl, r.brdf, r.pdf, r.w, r.clamp, tr);
     }
 };
 
-template <bool LDS, bool FEAT>
+template <bool LDS, bool FEAT, bool SDF = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWavesPerSimd, kAdjWavesPerSimd)))
 k_adjoint_dual(AdjointArgs A) {
     __shared__ uint32_t lds_stack[kLdsDepth * kBlock];
@@ -290,11 +293,11 @@ k_adjoint_dual(AdjointArgs A) {
         HitRec h{e0.w, __float_as_uint(e1.w)};
         PathVertexStateT<df> PD{to<df>(xyz(e2)), to<df>(f3{0.0f, 0.0f, 0.0f}), df(e2.w), (bits & 256u) != 0u,
                                 (bits & 512u) != 0u, __float_as_uint(e3.x), e4.w, e3.w};
-        TangentSink<FEAT> ts{{}, A.S, F, stk, tc, bj};
+        TangentSink<FEAT, SDF> ts{{}, A.S, F, stk, tc, bj};
         for (uint32_t b = bj; b < F.max_bounces; ++b) {
-            if (b != bj) h = trace_closest<false, FEAT>(A.S, value3(od), value3(dd), stk, tc, PD.time);
+            if (b != bj) h = trace_closest<false, FEAT, SDF>(A.S, value3(od), value3(dd), stk, tc, PD.time);
             VertexResultT<df> R;
-            shade_vertex(A.S, F, b, od, dd, h, PD, R, ts);
+            shade_vertex<false, SDF>(A.S, F, b, od, dd, h, PD, R, ts);
             if (!(R.alive && (b + 1u) < F.max_bounces)) break;
             od = R.no;
             dd = R.nd;
@@ -315,6 +318,22 @@ k_adjoint_dual(AdjointArgs A) {
 __global__ void k_fill_zero(float* p, uint32_t n) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
+}
+
+// One chunk of pixels [p0, p1): the replay (recording metal vertices when
+// rough) and the roughness pass over the records, in variant <FT, SDF>.
+template <bool FT, bool SDF>
+void adjoint_pass(const AdjointArgs& A, bool lds, bool rough, uint32_t blocks, uint32_t dual_blocks, uint32_t p0,
+                  uint32_t p1, hipStream_t stream) {
+    if (rough) {
+        if (lds) hipLaunchKernelGGL((k_adjoint<true, FT, true, SDF>), dim3(blocks), dim3(kBlock), 0, stream, A, p0, p1);
+        else hipLaunchKernelGGL((k_adjoint<false, FT, true, SDF>), dim3(blocks), dim3(kBlock), 0, stream, A, p0, p1);
+        if (lds) hipLaunchKernelGGL((k_adjoint_dual<true, FT, SDF>), dim3(dual_blocks), dim3(kBlock), 0, stream, A);
+        else hipLaunchKernelGGL((k_adjoint_dual<false, FT, SDF>), dim3(dual_blocks), dim3(kBlock), 0, stream, A);
+    } else {
+        if (lds) hipLaunchKernelGGL((k_adjoint<true, FT, false, SDF>), dim3(blocks), dim3(kBlock), 0, stream, A, p0, p1);
+        else hipLaunchKernelGGL((k_adjoint<false, FT, false, SDF>), dim3(blocks), dim3(kBlock), 0, stream, A, p0, p1);
+    }
 }
 
 }  // namespace
@@ -339,20 +358,9 @@ hipError_t launch_adjoint(const AdjointArgs& A, hipStream_t stream) {
         const uint32_t p1 = npix - p0 > chunk ? p0 + chunk : npix;
         const uint32_t blocks = min(kMaxGridBlocks, (p1 - p0 + kBlock - 1u) / kBlock);
         if (rough) hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, stream, (float*)A.rec_count, 1u);
-#define DMT_ADJ(L, FT, R) hipLaunchKernelGGL((k_adjoint<L, FT, R>), dim3(blocks), dim3(kBlock), 0, stream, A, p0, p1)
-        if (A.features) {
-            if (rough) { if (lds) DMT_ADJ(true, true, true); else DMT_ADJ(false, true, true); }
-            else { if (lds) DMT_ADJ(true, true, false); else DMT_ADJ(false, true, false); }
-        } else {
-            if (rough) { if (lds) DMT_ADJ(true, false, true); else DMT_ADJ(false, false, true); }
-            else { if (lds) DMT_ADJ(true, false, false); else DMT_ADJ(false, false, false); }
-        }
-#undef DMT_ADJ
-        if (!rough) continue;
-#define DMT_DUAL(L, FT) hipLaunchKernelGGL((k_adjoint_dual<L, FT>), dim3(dual_blocks), dim3(kBlock), 0, stream, A)
-        if (A.features) { if (lds) DMT_DUAL(true, true); else DMT_DUAL(false, true); }
-        else { if (lds) DMT_DUAL(true, false); else DMT_DUAL(false, false); }
-#undef DMT_DUAL
+        if (!A.features) adjoint_pass<false, false>(A, lds, rough, blocks, dual_blocks, p0, p1, stream);
+        else if (A.X.ext & kExtSdf) adjoint_pass<true, true>(A, lds, rough, blocks, dual_blocks, p0, p1, stream);
+        else adjoint_pass<true, false>(A, lds, rough, blocks, dual_blocks, p0, p1, stream);
     }
     return hipGetLastError();
 }

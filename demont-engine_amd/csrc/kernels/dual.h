@@ -63,11 +63,13 @@ __device__ __forceinline__ f3 deriv3(d3 a) { return f3{a.x.d, a.y.d, a.z.d}; }
 // value h.t and takes its derivative from the primitive's implicit equation
 // g(o + t d) = 0, dt = -(dg . do + t dg . dd) / (dg . d), with dg the sphere's
 // radius vector or the plane / triangle normal at the hit.  A sphere is taken
-// about its centre at shutter time `time` (PathTrace.slang:2093).
+// about its centre at shutter time `time` (PathTrace.slang:2093).  A hit on an
+// SDF cluster (SDF) is held fixed: the clusters carry no differentiated
+// parameter and their hit distance and normal carry no tangent.
 template <bool SDF = false>
 __device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 d, HitRec h, float time = 1.0f) {
     SurfaceT<df> s;
-    surface_material(S, h, s);
+    surface_material<SDF>(S, h, s);
     const f3 ov = value3(o), dv = value3(d), od = deriv3(o), dd = deriv3(d);
     f3 g;
     bool sphere = false;
@@ -76,7 +78,12 @@ __device__ __forceinline__ SurfaceT<df> surface_of(const SceneView& S, d3 o, d3 
         const float4* tr = S.tris + (size_t)(h.code & ~kMeshBit) * 3u;
         g = cross(xyz(tr[1]), xyz(tr[2]));
         s.n = to<df>(normalize(g));
-    } else {   // analytic (the adjoint refuses scenes with SDF clusters)
+    } else if (SDF && (h.code & kSdfBit)) {
+        const float4* c = S.sdf + (size_t)(h.code & ~kSdfBit) * kSdfStride;
+        s.t = df(h.t, 0.0f);
+        s.n = to<df>(normalize(sdf_grad(c, __float_as_uint(c[0].w), ov + dv * h.t)));
+        return s;
+    } else {   // analytic
         const float4* p = S.prims + (size_t)h.code * 7u;
         const float4 v0 = p[0], v2 = p[2];
         if (v2.x == 0.0f) {

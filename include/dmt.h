@@ -457,7 +457,10 @@ dmt_status dmt_scene_set_materials(dmt_scene* scene, const float* materials, siz
  * constant (set DMT_FRAME_NO_RUSSIAN_ROULETTE for a gradient that finite
  * differences reproduce).  Limits: max_bounces <= 8, refract_bounces <= 8.
  * The camera-sample features replay as rendered (a spectral sample's
- * gradient carries its wavelength's weight).  Stream-ordered on the device stream; d_image_grad must be complete when
+ * gradient carries its wavelength's weight); SDF clusters are traced and
+ * shaded as rendered and carry no parameters (their hit distance and normal
+ * are held fixed along the roughness tangent).  Frames with sky lights, PBR
+ * maps or water are refused (DMT_ERR_UNSUPPORTED).  Stream-ordered on the device stream; d_image_grad must be complete when
  * the call is made. */
 dmt_status dmt_adjoint(dmt_scene* scene, const dmt_frame_params* frame, const void* d_image_grad,
                        void* d_param_grad);

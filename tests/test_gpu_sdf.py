@@ -3,7 +3,7 @@ and op in Lambert, metal and glass clusters, the many-light workload with its
 boxes and unions, closest-hit queries (ties to the lower cluster index), the
 frame's step cap and surface distance, the G-buffer, a batch, and the engine
 path (sdf_params, the clusters at buffer slot 10); invalid clusters, a
-fractal root, bad trace parameters and the adjoint are refused."""
+fractal root and bad trace parameters are refused."""
 import numpy as np
 import pytest
 import torch
@@ -242,9 +242,8 @@ def test_invalid_clusters_and_parameters_are_refused(device):
         with pytest.raises(dmt.DmtError) as e:
             sc.render(g)
         assert e.value.status == dmt.DMT_ERR_INVALID_ARG, field
-    with pytest.raises(dmt.DmtError) as e:
-        adjoint_numpy(sc, f, np.ones((8, 8, 4), np.float32), device)
-    assert e.value.status == dmt.DMT_ERR_UNSUPPORTED
+    g = adjoint_numpy(sc, f, np.ones((8, 8, 4), np.float32), device)   # the adjoint runs on cluster scenes
+    assert np.all(np.isfinite(g))
     sc.render(f)   # the scene still renders
     sc.close()
     assert not device.is_lost()
