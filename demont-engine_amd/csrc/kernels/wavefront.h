@@ -137,9 +137,12 @@ constexpr uint32_t tail_waves(uint32_t ext) { return ext ? kTailWavesExt : kTail
 // Traversal steps per pass of the step-granular loops (k_tail, k_persistent)
 // between the wave-level bookkeeping (ballots, refill, batched shading): three
 // measured 1.5 % (1.51 M-triangle 1080p) to 4.6 % (many-light scene) faster
-// than one; a lane that finishes in an earlier step waits out the pass.
+// than one; four (with shading at 40 ready lanes) runs the batched kernel
+// 2.3 % and the many-light scene 0.7 % faster than three, the 1080p frame
+// within noise (two: as three); a lane that finishes in an earlier step waits
+// out the pass.
 #ifndef DMT_TAIL_STEPS
-#define DMT_TAIL_STEPS 3
+#define DMT_TAIL_STEPS 4
 #endif
 constexpr uint32_t kTailSteps = DMT_TAIL_STEPS;
 // k_tail with SDF clusters: a pass of sphere-trace steps (kSdfBurst of them)
